@@ -1,0 +1,114 @@
+/*
+ * C ABI of libnlccsd.so: the MI355X (gfx950) kernels behind the CCSD amplitude iteration of
+ * NuLattice's CCM module.
+ *
+ * The reference is pure Python (src/NuLattice/CCM/coupled_cluster.py, ccDgrams.py) and has no
+ * FFI of its own; every entry point below replaces the array arithmetic of one or more of its
+ * contraction sites.  A binding layer (ctypes; see nulattice_amd/_lib.py) keeps the reference's
+ * Python signatures (ccsd_solver, t1Iter, t2Iter, ccsd_energy, ...) above this ABI.
+ *
+ * Conventions: all pointers are device pointers to caller-allocated buffers; sizes and strides
+ * are in elements; every call is asynchronous on the given hipStream_t (null = default stream)
+ * and returns 0 (NL_OK), -1 (invalid argument) or -2 (HIP launch error).  No call allocates,
+ * frees or synchronises; no call throws.
+ */
+#ifndef NULATTICE_CCSD_H
+#define NULATTICE_CCSD_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* C[b] = alpha * op(A[b]) * op(B[b]) + beta * C[b]  (FP64, MFMA 16x16x4).
+ * Replaces every dense opt_einsum pairwise contraction of the diagram library
+ * (ccDgrams.py:48-578, coupled_cluster.py:282-284,405-408,528-536) after a layout change.
+ * a_kcontig: A(m,k)=A[m*lda+k] (1) or A[k*lda+m] (0); b_ncontig: B(k,n)=B[k*ldb+n] (1) or
+ * B[n*ldb+k] (0).  splits>1 splits K and needs ws of batch*splits*M*N doubles. */
+int nl_gemm_f64(int a_kcontig, int b_ncontig, int64_t M, int64_t N, int64_t K, double alpha,
+                const double* A, int64_t lda, int64_t strideA, const double* B, int64_t ldb,
+                int64_t strideB, double beta, double* C, int64_t ldc, int64_t strideC,
+                int64_t batch, int64_t splits, double* ws, void* stream);
+
+/* C[b] (FP64) = alpha * diag(ia) (A[b] B[b]^T) diag(ib) + beta * C[b] with BF16 (dtype 0) or
+ * FP16 (dtype 1) operands, FP32 accumulation (MFMA 16x16x32).  The precision form of the ring
+ * and ladder contractions (ccDgrams.py:240-303); no reference counterpart for the precision.
+ * K, lda, ldb multiples of 8; A, B 16-byte aligned.  splits>1 splits K (FP64 partials in ws,
+ * batch*splits*M*N doubles, then a fixed-order reduction). */
+int nl_gemm_lp(int dtype, int64_t M, int64_t N, int64_t K, double alpha, const void* A,
+               int64_t lda, int64_t strideA, const float* ia, int64_t strideIa, const void* B,
+               int64_t ldb, int64_t strideB, const float* ib, int64_t strideIb, double beta,
+               double* C, int64_t ldc, int64_t strideC, int64_t batch, int64_t splits,
+               double* ws, void* stream);
+
+/* out (contiguous, out_shape) = alpha * in[sum_d idx_d * in_strides[d]] + beta * out.
+ * Replaces the transposes inside pAB/pIJ (ccDgrams.py:193-213) and opt_einsum's operand
+ * permutations. */
+int nl_permute(int rank, const int64_t* out_shape, const int64_t* in_strides, double alpha,
+               const double* in, double beta, double* out, void* stream);
+
+/* Row-wise FP64 -> BF16 (dtype 0) / FP16 (dtype 1) conversion of a 4-index view
+ * (rows r1*R2+r2, cols c1*C2+c2, element in[r1*s0+r2*s1+c1*s2+c2*s3]) with a per-row power-of-two
+ * scale bringing the row max to <= target; iscale[r] receives the inverse scale; columns
+ * [C1*C2, ldo) are zero-filled.  Precision-form operand preparation (no reference counterpart). */
+int nl_convert_rows(int dtype, int64_t R1, int64_t R2, int64_t C1, int64_t C2,
+                    const int64_t* strides4, const double* in, void* out, int64_t ldo,
+                    float* iscale, float target, void* stream);
+
+/* tau[a,b,i,j] = t2[a,b,i,j] + t1[a,i] t1[b,j] - t1[a,j] t1[b,i]: the t2 + pIJ(t1 t1) combination
+ * that the pp ladder applies to (ccDgrams.py:215-237 with coupled_cluster.py:524-526). */
+int nl_tau(int64_t p, int64_t h, const double* t1, const double* t2, double* tau, void* stream);
+
+/* Segmented COO accumulation: for each segment s,
+ *   out[dst_off[s] + x*dst_stride] += alpha * sum_{e=ptr[s]}^{ptr[s+1]-1} val[e] * src[src_off[e] + x*src_stride]
+ * for x < X.  Replaces the scalar loops of v_ppph_dgrams (ccDgrams.py:13-45) for the
+ * intermediates ret0..ret4 with precomputed segment plans (conflict-free, no atomics). */
+int nl_segsum(int64_t S, int64_t X, const int64_t* ptr, const int64_t* src_off, const double* val,
+              const int64_t* dst_off, int64_t src_stride, int64_t dst_stride, double alpha,
+              const double* src, double* out, void* stream);
+
+/* out[r][:] = src[idx[r]][:] (0 where idx<0) and out[idx[r]][:] += alpha*src[r][:]:
+ * pair-row gather/scatter around the block-dense pp ladder and v_ppph contractions
+ * (ccDgrams.py:13-45, 215-237 regrouped by connected pair blocks). */
+int nl_gather_rows(int64_t R, int64_t Cn, const int64_t* idx, const double* src, int64_t lds,
+                   double* out, void* stream);
+int nl_scatter_add_rows(int64_t R, int64_t Cn, const int64_t* idx, const double* src, double alpha,
+                        double* out, int64_t ldo, void* stream);
+
+/* t1n = m*t1 + (1-m)*(t1 + H1 / (-(Xpp[a,a] + Xhh[i,i])))
+ * (coupled_cluster.py:410-414 and the mixing of :614). */
+int nl_update_t1(int64_t p, int64_t h, const double* t1, const double* H1, const double* Xpp,
+                 const double* Xhh, double mixing, double* t1n, void* stream);
+
+/* Fused T2 assembly, Jacobi update, mixing and energy partials:
+ *   H2  = S0 + pAB(Sab) + pIJ(Sij) + pIJ(pAB(Spp))   (Spp stored as [(a,i)][(b,j)])
+ *   t2n = m*t2 + (1-m)*(t2 + H2 / (-(Xpp_aa + Xpp_bb + Xhh_ii + Xhh_jj)))
+ *   part[blk] = partial of 1/4 v.t2n + 1/2 v t1n t1n; part[nblk+blk] = partial of |H2|^2
+ * (coupled_cluster.py:536-545, :618, :265-286).  nblk = nl_partials(p*p*h*h). */
+int nl_assemble_t2(int64_t p, int64_t h, const double* S0, const double* Sab, const double* Sij,
+                   const double* Spp, const double* Xpp, const double* Xhh, const double* t2,
+                   const double* vpphh, const double* t1n, double mixing, double* t2n,
+                   double* H2out, double* part, void* stream);
+
+/* Number of partial slots used by nl_assemble_t2 / nl_energy_part / nl_dot_part for n elements. */
+int64_t nl_partials(int64_t n);
+
+/* Partials of 1/4 v.t2 + 1/2 v t1 t1 (ccsd_energy, coupled_cluster.py:265-286). */
+int nl_energy_part(int64_t p, int64_t h, const double* vpphh, const double* t2, const double* t1,
+                   double* part, void* stream);
+
+/* out[0] = sum(part[0:nparts]) + x.y (nx elements), fixed summation order. */
+int nl_finalize(int64_t nparts, const double* part, int64_t nx, const double* x, const double* y,
+                double* out, void* stream);
+
+/* Partials of x.y (DIIS B-matrix, soa/coupled_cluster.py:536-543). */
+int nl_dot_part(int64_t n, const double* x, const double* y, double* part, void* stream);
+
+/* y = a*x + b*y and c = x - y (mixing, DIIS extrapolation and error vectors). */
+int nl_axpby(int64_t n, double a, const double* x, double b, double* y, void* stream);
+int nl_sub(int64_t n, const double* x, const double* y, double* c, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

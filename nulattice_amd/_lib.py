@@ -1,0 +1,68 @@
+"""ctypes binding of libnlccsd.so (declarations in include/nulattice_ccsd.h).
+
+The library is loaded from the package directory; there is no CPU fallback: if it is missing
+or a GPU call is attempted without a HIP device, the call raises.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libnlccsd.so")
+
+_i64, _i32, _dbl, _flt = ctypes.c_int64, ctypes.c_int, ctypes.c_double, ctypes.c_float
+_vp = ctypes.c_void_p
+
+# name -> (restype, argtypes); mirrors include/nulattice_ccsd.h
+SIGNATURES = {
+    "nl_gemm_f64": (_i32, [_i32, _i32, _i64, _i64, _i64, _dbl, _vp, _i64, _i64, _vp, _i64, _i64,
+                           _dbl, _vp, _i64, _i64, _i64, _i64, _vp, _vp]),
+    "nl_gemm_lp": (_i32, [_i32, _i64, _i64, _i64, _dbl, _vp, _i64, _i64, _vp, _i64, _vp, _i64,
+                          _i64, _vp, _i64, _dbl, _vp, _i64, _i64, _i64, _i64, _vp, _vp]),
+    "nl_permute": (_i32, [_i32, _vp, _vp, _dbl, _vp, _dbl, _vp, _vp]),
+    "nl_convert_rows": (_i32, [_i32, _i64, _i64, _i64, _i64, _vp, _vp, _vp, _i64, _vp, _flt,
+                               _vp]),
+    "nl_tau": (_i32, [_i64, _i64, _vp, _vp, _vp, _vp]),
+    "nl_segsum": (_i32, [_i64, _i64, _vp, _vp, _vp, _vp, _i64, _i64, _dbl, _vp, _vp, _vp]),
+    "nl_gather_rows": (_i32, [_i64, _i64, _vp, _vp, _i64, _vp, _vp]),
+    "nl_scatter_add_rows": (_i32, [_i64, _i64, _vp, _vp, _dbl, _vp, _i64, _vp]),
+    "nl_update_t1": (_i32, [_i64, _i64, _vp, _vp, _vp, _vp, _dbl, _vp, _vp]),
+    "nl_assemble_t2": (_i32, [_i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _dbl,
+                              _vp, _vp, _vp, _vp]),
+    "nl_partials": (_i64, [_i64]),
+    "nl_energy_part": (_i32, [_i64, _i64, _vp, _vp, _vp, _vp, _vp]),
+    "nl_finalize": (_i32, [_i64, _vp, _i64, _vp, _vp, _vp, _vp]),
+    "nl_dot_part": (_i32, [_i64, _vp, _vp, _vp, _vp]),
+    "nl_axpby": (_i32, [_i64, _dbl, _vp, _dbl, _vp, _vp]),
+    "nl_sub": (_i32, [_i64, _vp, _vp, _vp, _vp]),
+}
+
+_LIB = None
+
+
+class NLError(RuntimeError):
+    pass
+
+
+def load():
+    """Load libnlccsd.so once and attach signatures; raises if it is not built."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise NLError(f"{LIB_PATH} is not built; run `python -m nulattice_amd.build` "
+                      "(there is no CPU fallback for the CCSD kernels)")
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _LIB = lib
+    return lib
+
+
+def call(name, *args):
+    """Call an entry point and raise on a nonzero status."""
+    rc = getattr(load(), name)(*args)
+    if rc != 0:
+        raise NLError(f"{name} returned {rc} ({'invalid argument' if rc == -1 else 'HIP error'})")
+    return rc

@@ -1,0 +1,275 @@
+"""Drop-in for NuLattice's CCSD amplitude iteration (``NuLattice.CCM.coupled_cluster``) on MI355X.
+
+Same entry points, argument order and meaning, return types and failure behaviour as the
+reference (src/NuLattice/CCM/coupled_cluster.py:265-687, ccDgrams.py), with the arithmetic in
+libnlccsd.so.  Inputs may be NumPy arrays, torch tensors, reference COO lists
+([[i0,i1,i2,i3,val], ...]) or (indices, values) pairs; amplitudes are returned as float64 HIP
+device tensors in the reference layout t1[a,i], t2[a,b,i,j].
+
+Extensions (no reference counterpart), all keyword-only with reference-compatible defaults:
+  diis       "legacy" (default: the current reference, whose DIIS block never runs),
+             "pulsed" (the pulsed DIIS that produced the reference notebooks' trajectories) or
+             "soa" (src/NuLattice/CCM/soa/coupled_cluster.py:520-567)
+  precision  "fp64" (default) | "bf16" | "fp16": ring, ladder and dressing contractions on
+             low-precision MFMA with FP32 accumulation; with ``refine=True`` the iteration then
+             continues in FP64 from the low-precision fixed point until ``eps`` is met.
+  device, history (list receiving every step's energy), info (dict receiving timings/steps).
+"""
+import ctypes
+import math
+import time
+
+import numpy as np
+import torch
+
+from . import _lib, engine, ops
+from .engine import Buffers, Hamiltonian
+
+F64 = torch.float64
+
+_PREC = {
+    "fp64": engine.DEFAULT_PREC,
+    "bf16": {"ring": "bf16", "hh": "bf16", "dress": "bf16"},
+    "fp16": {"ring": "fp16", "hh": "fp16", "dress": "fp16"},
+}
+
+
+def _device(device):
+    if device is None:
+        if not torch.cuda.is_available():
+            raise _lib.NLError("no HIP device visible (the CCSD kernels have no CPU fallback)")
+        device = "cuda"
+    return torch.device(device)
+
+
+def _t(x, device):
+    return engine._dev(x, device)
+
+
+def _dot(x, y, B):
+    """x.y over all elements, on the GPU, returned to the host."""
+    n = x.numel()
+    _lib.call("nl_dot_part", n, ops._p(x), ops._p(y), ops._p(B.part), ops._stream())
+    nparts = int(_lib.load().nl_partials(n))
+    _lib.call("nl_finalize", nparts, ops._p(B.part), 0, None, None, ops._p(B.E), ops._stream())
+    return float(B.E.item())
+
+
+class _Diis:
+    """Pulsed DIIS on device vectors; B-matrix solve in FP64 on the host (a 10x10 system)."""
+
+    def __init__(self, mode, max_diis, t1, t2, ccs):
+        self.mode, self.n, self.ccs = mode, max_diis, ccs
+        self.h1, self.h2, self.e1, self.e2 = [ops.copy(t1)], [ops.copy(t2)], [], []
+
+    def step(self, t1, t2, o1, o2, B):
+        """Record (t, t - old); return extrapolated (t1, t2) or None."""
+        soa = self.mode == "soa"
+        self.h1.append(ops.copy(t1)); self.h2.append(ops.copy(t2))
+        e1 = torch.empty_like(t1); e2 = torch.empty_like(t2)
+        _lib.call("nl_sub", t1.numel(), ops._p(t1), ops._p(o1), ops._p(e1), ops._stream())
+        _lib.call("nl_sub", t2.numel(), ops._p(t2), ops._p(o2), ops._p(e2), ops._stream())
+        self.e1.append(e1); self.e2.append(e2)
+        if soa and len(self.e1) > self.n:
+            for L in (self.h1, self.h2, self.e1, self.e2):
+                L.pop(0)
+        if len(self.e1) != self.n:
+            return None
+        if not soa:
+            for L in (self.h1, self.h2, self.e1, self.e2):
+                L.pop(0)
+        n = len(self.e1)
+        Bm = np.empty((n, n))
+        for a in range(n):
+            for b in range(a, n):
+                Bm[a, b] = Bm[b, a] = _dot(self.e1[a], self.e1[b], B) + _dot(self.e2[a],
+                                                                               self.e2[b], B)
+        Bm = Bm / (np.abs(Bm).max() + (1e-16 if soa else 0.0))
+        A = -np.ones((n + 1, n + 1)); A[:n, :n] = Bm; A[n, n] = 0.0
+        rhs = np.zeros(n + 1); rhs[n] = -1.0
+        out = None
+        try:
+            c = np.linalg.solve(A, rhs)[:n]
+            n1 = torch.empty_like(t1); n2 = torch.empty_like(t2)
+            ops.axpby(float(c[0]), self.h1[1], 0.0, n1)
+            for k in range(1, n):
+                ops.axpby(float(c[k]), self.h1[k + 1], 1.0, n1)
+            if not self.ccs:
+                ops.axpby(float(c[0]), self.h2[1], 0.0, n2)
+                for k in range(1, n):
+                    ops.axpby(float(c[k]), self.h2[k + 1], 1.0, n2)
+            else:
+                n2 = t2
+            out = (n1, n2)
+        except np.linalg.LinAlgError:
+            pass
+        r1, r2 = out if out is not None else (t1, t2)
+        self.h1, self.h2, self.e1, self.e2 = [ops.copy(r1)], [ops.copy(r2)], [], []
+        return out
+
+
+def ccsd_energy(f_ph, v_pphh, t2, t1, device=None):
+    """E = sum f_ai t_ai + 1/4 sum v_abij t_abij + 1/2 sum v_abij t_ai t_bj
+    (coupled_cluster.py:265-286), evaluated on the GPU; returns a Python float."""
+    dev = _device(device if device is not None else getattr(t2, "device", None))
+    f_ph, v_pphh, t1, t2 = (_t(x, dev) for x in (f_ph, v_pphh, t1, t2))
+    p, h = f_ph.shape
+    part = torch.empty(int(_lib.load().nl_partials(p * p * h * h)) + 1, dtype=F64, device=dev)
+    E = torch.empty(1, dtype=F64, device=dev)
+    _lib.call("nl_energy_part", p, h, ops._p(v_pphh), ops._p(t2), ops._p(t1), ops._p(part),
+              ops._stream())
+    _lib.call("nl_finalize", int(_lib.load().nl_partials(p * p * h * h)), ops._p(part), p * h,
+              ops._p(f_ph), ops._p(t1), ops._p(E), ops._stream())
+    return float(E.item())
+
+
+def t1Init(f_ph, f_pp, f_hh, delta, device=None):
+    """t1 = f_ph / (f_ii - f_aa + delta) (coupled_cluster.py:319-340)."""
+    dev = _device(device)
+    ham = _MiniHam(dev, f_pp=f_pp, f_ph=f_ph, f_hh=f_hh)
+    t1 = torch.empty_like(ham.f_ph)
+    Xhh = torch.empty_like(ham.f_hh)
+    ops.permute_into(ham.f_hh, Xhh, alpha=-1.0)
+    if delta:
+        engine._diag_add(Xhh, -delta)
+    zero = torch.zeros_like(t1)
+    _lib.call("nl_update_t1", ham.p, ham.h, ops._p(zero), ops._p(ham.f_ph), ops._p(ham.f_pp),
+              ops._p(Xhh), ctypes.c_double(0.0), ops._p(t1), ops._stream())
+    return t1
+
+
+def t2Init(f_pp, f_hh, v_pphh, delta, device=None):
+    """t2 = v_pphh / (f_ii + f_jj - f_aa - f_bb + delta) (coupled_cluster.py:416-439)."""
+    dev = _device(device)
+    ham = _MiniHam(dev, f_pp=f_pp, f_hh=f_hh, v_pphh=v_pphh)
+    p, h = ham.f_pp.shape[0], ham.f_hh.shape[0]
+    t2 = torch.empty((p, p, h, h), dtype=F64, device=dev)
+    Xhh = torch.empty_like(ham.f_hh)
+    ops.permute_into(ham.f_hh, Xhh, alpha=-1.0)
+    if delta:
+        engine._diag_add(Xhh, -0.5 * delta)
+    zero = torch.zeros_like(t2)
+    part = torch.empty(2 * int(_lib.load().nl_partials(p * p * h * h)) + 1, dtype=F64, device=dev)
+    _lib.call("nl_assemble_t2", p, h, ops._p(ham.v_pphh), None, None, None, ops._p(ham.f_pp),
+              ops._p(Xhh), ops._p(zero), None, None, ctypes.c_double(0.0), ops._p(t2), None,
+              ops._p(part), ops._stream())
+    return t2
+
+
+class _MiniHam:
+    def __init__(self, dev, **kw):
+        for k, v in kw.items():
+            setattr(self, k, _t(v, dev))
+        if hasattr(self, "f_ph"):
+            self.p, self.h = self.f_ph.shape
+
+
+def _solver_ham(fock_mats, two_body_int, device):
+    return Hamiltonian(fock_mats, two_body_int, device=device)
+
+
+def ccsd_solver(fock_mats, two_body_int, t1initial=None, eps=1e-8, maxSteps=1000, max_diis=10,
+                delta=0, mixing=0.5, verbose=False, sparse=True, ccs=False, *, device=None,
+                diis="legacy", precision="fp64", refine=True, eps_lowprec=None, history=None,
+                info=None, hamiltonian=None):
+    """Damped-Jacobi CCSD (coupled_cluster.py:547-687) on one MI355X.
+
+    Returns (E_corr, t1, t2); like the reference, returns None after a divergence or NaN.
+    ``sparse`` only says how v_pppp / v_ppph are given (COO lists or dense arrays); both are
+    packed into the same block plans.  ``hamiltonian`` may pass a prebuilt engine.Hamiltonian.
+    With a low ``precision`` the iteration first runs to |dE|/|E| < ``eps_lowprec`` (default
+    max(eps, 1e-7): the low-precision fixed point is only defined to about that level) and then,
+    if ``refine``, continues in FP64 to ``eps``.
+    """
+    if diis not in ("legacy", "pulsed", "soa"):
+        raise ValueError(f"diis must be 'legacy', 'pulsed' or 'soa', not {diis!r}")
+    if precision not in _PREC:
+        raise ValueError(f"precision must be one of {sorted(_PREC)}")
+    dev = _device(device)
+    ham = hamiltonian if hamiltonian is not None else _solver_ham(fock_mats, two_body_int, dev)
+    p, h = ham.p, ham.h
+    B = Buffers(p, h, dev)
+    t1 = torch.empty((p, h), dtype=F64, device=dev)
+    t2 = torch.empty((p, p, h, h), dtype=F64, device=dev)
+    t1n, t2n = torch.empty_like(t1), torch.empty_like(t2)
+    if t1initial is None:
+        engine.init_amplitudes(ham, B, t1, t2, delta)
+    else:
+        ops.permute_into(_t(t1initial, dev), t1)
+    if ccs or t1initial is not None:
+        t2.zero_()
+    pulsed = diis != "legacy" and max_diis > 0
+    dstate = _Diis(diis, max_diis, t1, t2, ccs) if pulsed else None
+    engine.energy(ham, t1, t2, B)
+    prev = float(B.E.item())
+    if history is not None:
+        history.append(prev)
+    if verbose:
+        print(f"Step {0}: {prev}")
+    prec = _PREC[precision]
+    lp_phase = precision != "fp64"
+    eps_lp = max(eps, 1e-7) if eps_lowprec is None else eps_lowprec
+    steps_lp = steps_fp64 = 0
+    t_start = time.perf_counter()
+    energy = prev
+    i = 0
+    while i < maxSteps:
+        if ccs:
+            engine.t1_part(ham, t1, t2, B, prec)
+            _lib.call("nl_update_t1", p, h, ops._p(t1), ops._p(B.H1), ops._p(B.X1pp),
+                      ops._p(B.X1hh), ctypes.c_double(mixing), ops._p(t1n), ops._stream())
+            ops.permute_into(t2, t2n)
+            engine.energy(ham, t1n, t2n, B)
+        else:
+            engine.update(ham, t1, t2, B, t1n, t2n, mixing, prec)
+        energy = float(B.E.item())
+        if lp_phase:
+            steps_lp += 1
+        else:
+            steps_fp64 += 1
+        if history is not None:
+            history.append(energy)
+        if verbose:
+            print(f"Step {i + 1}: {energy}", "difference =", abs(energy - prev) / abs(energy))
+        old1, old2 = t1, t2
+        t1, t1n = t1n, t1
+        t2, t2n = t2n, t2
+        rel = abs(energy - prev) / abs(energy) if energy != 0 else 0.0
+        converged = (energy == 0 and prev == 0) or rel < eps
+        if lp_phase and refine and ((energy == 0 and prev == 0) or rel < eps_lp):
+            # the low-precision fixed point is reached: continue in FP64 from here
+            lp_phase, prec = False, _PREC["fp64"]
+            if info is not None:
+                info["E_lowprec"] = energy
+            prev = energy
+            i += 1
+            continue
+        if converged:
+            if verbose:
+                print(f"Energy found in {i + 1} iterations")
+            _fill_info(info, steps_lp, steps_fp64, t_start, energy)
+            return energy, t1, t2
+        if dstate is not None:
+            ext = dstate.step(t1, t2, old1, old2, B)
+            if ext is not None:
+                t1, t2 = ext
+                t1n, t2n = torch.empty_like(t1), torch.empty_like(t2)
+        if abs(energy) > 1e10:
+            print("Iteration Diverged")
+            _fill_info(info, steps_lp, steps_fp64, t_start, energy)
+            return None
+        if math.isnan(energy):
+            print("nan error")
+            _fill_info(info, steps_lp, steps_fp64, t_start, energy)
+            return None
+        prev = energy
+        i += 1
+    print("Max Iterations Reached")
+    _fill_info(info, steps_lp, steps_fp64, t_start, energy)
+    return energy, t1, t2
+
+
+def _fill_info(info, steps_lp, steps_fp64, t_start, energy):
+    if info is not None:
+        info.update(steps_lowprec=steps_lp, steps_fp64=steps_fp64,
+                    seconds=time.perf_counter() - t_start, energy=energy)

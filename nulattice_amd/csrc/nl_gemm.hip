@@ -1,0 +1,413 @@
+// MFMA GEMMs for the CCSD contractions on gfx950.
+//
+//  * nl_gemm_f64: C = alpha * op(A) op(B) + beta * C in FP64 on v_mfma_f64_16x16x4f64.
+//    Batched and strided; A may be K- or M-contiguous, B may be N- or K-contiguous, so most
+//    tensor contractions map onto it after at most one permute.  Split-K with a deterministic
+//    workspace reduction serves the tall-K shapes (Fock dressings, hh-ladder intermediates).
+//  * nl_gemm_lp: C (FP64) = alpha * diag(ia) (A B^T) diag(ib) + beta * C with A, B in BF16 or
+//    FP16 (K-contiguous rows, per-row power-of-two scales folded out through ia/ib) on
+//    v_mfma_f32_16x16x32_{bf16,f16}, FP32 accumulation.
+//
+// Both use a 128x128 workgroup tile, 4 waves of 64x64, LDS staging with a register prefetch of
+// the next K-tile, and an XCD-aware tile order: consecutive workgroup ids land on different XCDs,
+// so the id is remapped such that each XCD walks a contiguous band of tiles (shared A rows / B
+// columns stay in that XCD's L2), then grouped by 8 tile-rows.
+#include "nl_common.h"
+
+namespace {
+
+constexpr int TM = 128, TN = 128;
+
+struct TileCoord { int tm, tn; };
+
+__device__ inline TileCoord tile_of(int wg, int nwg, int tiles_m, int tiles_n) {
+  // XCD-aware remap: hardware dispatches workgroup ids round-robin over the 8 XCDs.
+  if ((nwg & 7) == 0) wg = (wg & 7) * (nwg >> 3) + (wg >> 3);
+  const int G = 8;
+  int per_group = G * tiles_n;
+  int gid = wg / per_group;
+  int first_m = gid * G;
+  int gsize = min(tiles_m - first_m, G);
+  int in = wg % per_group;
+  TileCoord c;
+  c.tm = first_m + in % gsize;
+  c.tn = in / gsize;
+  return c;
+}
+
+// ---------------------------------------------------------------------------------- FP64
+constexpr int BK64 = 16;
+constexpr int PAD64 = 4;
+
+struct GemmF64Args {
+  int M, N, K;
+  double alpha, beta;
+  const double* A; int64_t lda, sA;
+  const double* B; int64_t ldb, sB;
+  double* C; int64_t ldc, sC;
+  int tiles_m, tiles_n, splits, kchunk;
+  double* ws;  // split-K partials [batch][splits][M][N] (only when splits > 1)
+};
+
+template <bool AK, bool BNC>
+__global__ __launch_bounds__(256) void gemm_f64_kernel(GemmF64Args g) {
+  __shared__ double As[BK64][TM + PAD64];
+  __shared__ double Bs[BK64][TN + PAD64];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int wm = w >> 1, wn = w & 1;
+  const int batch = blockIdx.z, split = blockIdx.y;
+  TileCoord tc = tile_of(blockIdx.x, gridDim.x, g.tiles_m, g.tiles_n);
+  const int m0 = tc.tm * TM, n0 = tc.tn * TN;
+  const double* A = g.A + (int64_t)batch * g.sA;
+  const double* B = g.B + (int64_t)batch * g.sB;
+  const int kbeg = split * g.kchunk;
+  const int kend = min(g.K, kbeg + g.kchunk);
+
+  v4d acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
+
+  double ra[8], rb[8];
+  auto load = [&](int k0) {
+    if (AK) {  // A[m*lda + k]
+      int r = tid >> 1, kk = (tid & 1) * 8, m = m0 + r;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        int k = k0 + kk + e;
+        ra[e] = (m < g.M && k < kend) ? A[(int64_t)m * g.lda + k] : 0.0;
+      }
+    } else {   // A[k*lda + m]
+      int kk = tid >> 4, r = (tid & 15) * 8, k = k0 + kk;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        int m = m0 + r + e;
+        ra[e] = (m < g.M && k < kend) ? A[(int64_t)k * g.lda + m] : 0.0;
+      }
+    }
+    if (BNC) {  // B[k*ldb + n]
+      int kk = tid >> 4, c = (tid & 15) * 8, k = k0 + kk;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        int n = n0 + c + e;
+        rb[e] = (n < g.N && k < kend) ? B[(int64_t)k * g.ldb + n] : 0.0;
+      }
+    } else {    // B[n*ldb + k]
+      int c = tid >> 1, kk = (tid & 1) * 8, n = n0 + c;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        int k = k0 + kk + e;
+        rb[e] = (n < g.N && k < kend) ? B[(int64_t)n * g.ldb + k] : 0.0;
+      }
+    }
+  };
+  auto store = [&]() {
+    if (AK) {
+      int r = tid >> 1, kk = (tid & 1) * 8;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) As[kk + e][r] = ra[e];
+    } else {
+      int kk = tid >> 4, r = (tid & 15) * 8;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) As[kk][r + e] = ra[e];
+    }
+    if (BNC) {
+      int kk = tid >> 4, c = (tid & 15) * 8;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) Bs[kk][c + e] = rb[e];
+    } else {
+      int c = tid >> 1, kk = (tid & 1) * 8;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) Bs[kk + e][c] = rb[e];
+    }
+  };
+
+  if (kbeg < kend) {
+    load(kbeg);
+    store();
+    __syncthreads();
+    for (int k0 = kbeg; k0 < kend; k0 += BK64) {
+      const bool more = k0 + BK64 < kend;
+      if (more) load(k0 + BK64);
+#pragma unroll
+      for (int kk = 0; kk < BK64 / 4; ++kk) {
+        const int kr = kk * 4 + (lane >> 4);
+        double a[4], b[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) a[i] = As[kr][wm * 64 + i * 16 + (lane & 15)];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) b[j] = Bs[kr][wn * 64 + j * 16 + (lane & 15)];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+          for (int j = 0; j < 4; ++j)
+            acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+      }
+      __syncthreads();
+      if (more) {
+        store();
+        __syncthreads();
+      }
+    }
+  }
+
+  if (g.splits > 1) {
+    double* W = g.ws + ((int64_t)batch * g.splits + split) * (int64_t)g.M * g.N;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          int row = m0 + wm * 64 + i * 16 + r * 4 + (lane >> 4);  // f64 16x16x4: D[4r + l/16][l%16]
+          int col = n0 + wn * 64 + j * 16 + (lane & 15);
+          if (row < g.M && col < g.N) W[(int64_t)row * g.N + col] = g.alpha * acc[i][j][r];
+        }
+    return;
+  }
+  double* C = g.C + (int64_t)batch * g.sC;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        int row = m0 + wm * 64 + i * 16 + r * 4 + (lane >> 4);
+        int col = n0 + wn * 64 + j * 16 + (lane & 15);
+        if (row < g.M && col < g.N) {
+          double* p = C + (int64_t)row * g.ldc + col;
+          double v = g.alpha * acc[i][j][r];
+          *p = (g.beta != 0.0) ? v + g.beta * *p : v;
+        }
+      }
+}
+
+__global__ void splitk_reduce_kernel(int M, int N, int splits, int batch, const double* ws,
+                                     double beta, double* C, int64_t ldc, int64_t sC) {
+  int64_t total = (int64_t)batch * M * N;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    int64_t b = t / ((int64_t)M * N);
+    int64_t mn = t - b * (int64_t)M * N;
+    int64_t m = mn / N, n = mn - m * N;
+    const double* W = ws + b * splits * (int64_t)M * N + mn;
+    double s = 0.0;
+    for (int q = 0; q < splits; ++q) s += W[(int64_t)q * M * N];
+    double* p = C + b * sC + m * ldc + n;
+    *p = (beta != 0.0) ? s + beta * *p : s;
+  }
+}
+
+// ---------------------------------------------------------------------------------- BF16/FP16
+constexpr int BKL = 64;
+constexpr int PADL = 8;
+
+struct GemmLPArgs {
+  int M, N, K;
+  double alpha, beta;
+  const uint16_t* A; int64_t lda, sA;   // [M][lda] K-contiguous, 16-bit elements
+  const uint16_t* B; int64_t ldb, sB;   // [N][ldb] K-contiguous
+  const float* ia; int64_t sia;         // per-row inverse scale of A (or null)
+  const float* ib; int64_t sib;         // per-row inverse scale of B (or null)
+  double* C; int64_t ldc, sC;
+  int tiles_m, tiles_n, splits, kchunk;
+  double* ws;                           // split-K partials [batch][splits][M][N]
+};
+
+template <bool IS_BF16>
+__global__ __launch_bounds__(256) void gemm_lp_kernel(GemmLPArgs g) {
+  __shared__ __attribute__((aligned(16))) uint16_t As[TM][BKL + PADL];
+  __shared__ __attribute__((aligned(16))) uint16_t Bs[TN][BKL + PADL];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int wm = w >> 1, wn = w & 1;
+  const int batch = blockIdx.z;
+  TileCoord tc = tile_of(blockIdx.x, gridDim.x, g.tiles_m, g.tiles_n);
+  const int m0 = tc.tm * TM, n0 = tc.tn * TN;
+  const uint16_t* A = g.A + (int64_t)batch * g.sA;
+  const uint16_t* B = g.B + (int64_t)batch * g.sB;
+  const int split = blockIdx.y;
+  const int kbeg = split * g.kchunk;
+  const int kend = min(g.K, kbeg + g.kchunk);
+
+  v4f acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = (v4f){0.f, 0.f, 0.f, 0.f};
+
+  // 128 rows x 64 k = 1024 chunks of 8 elements (16 B); 4 chunks per thread per operand.
+  uint4 ra[4], rb[4];
+  auto load = [&](int k0) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      int c = tid + 256 * q, r = c >> 3, kc = (c & 7) * 8, k = k0 + kc;
+      int m = m0 + r, n = n0 + r;
+      // K, kchunk, lda and ldb are multiples of 8 (checked / arranged by the host entry point),
+      // so a chunk is either fully inside or fully outside [kbeg, kend).
+      ra[q] = (m < g.M && k < kend) ? *(const uint4*)(A + (int64_t)m * g.lda + k)
+                                    : make_uint4(0, 0, 0, 0);
+      rb[q] = (n < g.N && k < kend) ? *(const uint4*)(B + (int64_t)n * g.ldb + k)
+                                    : make_uint4(0, 0, 0, 0);
+    }
+  };
+  auto store = [&]() {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      int c = tid + 256 * q, r = c >> 3, kc = (c & 7) * 8;
+      *(uint4*)&As[r][kc] = ra[q];
+      *(uint4*)&Bs[r][kc] = rb[q];
+    }
+  };
+
+  if (kbeg < kend) {
+  load(kbeg);
+  store();
+  __syncthreads();
+  for (int k0 = kbeg; k0 < kend; k0 += BKL) {
+    const bool more = k0 + BKL < kend;
+    if (more) load(k0 + BKL);
+#pragma unroll
+    for (int s = 0; s < BKL / 32; ++s) {
+      const int kc = s * 32 + (lane >> 4) * 8;
+      uint4 a[4], b[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) a[i] = *(const uint4*)&As[wm * 64 + i * 16 + (lane & 15)][kc];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) b[j] = *(const uint4*)&Bs[wn * 64 + j * 16 + (lane & 15)][kc];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          if (IS_BF16)
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                __builtin_bit_cast(v8bf, a[i]), __builtin_bit_cast(v8bf, b[j]), acc[i][j], 0, 0, 0);
+          else
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(
+                __builtin_bit_cast(v8h, a[i]), __builtin_bit_cast(v8h, b[j]), acc[i][j], 0, 0, 0);
+        }
+    }
+    __syncthreads();
+    if (more) {
+      store();
+      __syncthreads();
+    }
+  }
+  }
+
+  const bool partial = g.splits > 1;
+  double* C = partial ? g.ws + ((int64_t)batch * g.splits + split) * (int64_t)g.M * g.N
+                      : g.C + (int64_t)batch * g.sC;
+  const int64_t ldc = partial ? g.N : g.ldc;
+  const double beta = partial ? 0.0 : g.beta;
+  const float* ia = g.ia ? g.ia + (int64_t)batch * g.sia : nullptr;
+  const float* ib = g.ib ? g.ib + (int64_t)batch * g.sib : nullptr;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      int row = m0 + wm * 64 + i * 16 + (lane >> 4) * 4 + r;
+      if (row >= g.M) continue;
+      double sr = g.alpha * (ia ? (double)ia[row] : 1.0);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        int col = n0 + wn * 64 + j * 16 + (lane & 15);
+        if (col < g.N) {
+          double v = sr * (double)acc[i][j][r] * (ib ? (double)ib[col] : 1.0);
+          double* p = C + (int64_t)row * ldc + col;
+          *p = (beta != 0.0) ? v + beta * *p : v;
+        }
+      }
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+// C[b] = alpha * op(A[b]) * op(B[b]) + beta * C[b], row-major C (ldc), FP64.
+//   a_kcontig = 1: A(m,k) = A[m*lda + k];  0: A(m,k) = A[k*lda + m]
+//   b_ncontig = 1: B(k,n) = B[k*ldb + n];  0: B(k,n) = B[n*ldb + k]
+//   splits > 1 splits K; ws must hold batch*splits*M*N doubles.
+int nl_gemm_f64(int a_kcontig, int b_ncontig, int64_t M, int64_t N, int64_t K, double alpha,
+                const double* A, int64_t lda, int64_t strideA, const double* B, int64_t ldb,
+                int64_t strideB, double beta, double* C, int64_t ldc, int64_t strideC,
+                int64_t batch, int64_t splits, double* ws, void* stream) {
+  if (M < 0 || N < 0 || K < 0 || batch < 0 || splits < 1) return NL_EARG;
+  if (M == 0 || N == 0 || batch == 0) return NL_OK;
+  if (!A || !B || !C || (splits > 1 && !ws)) return NL_EARG;
+  if (M > INT32_MAX || N > INT32_MAX || K > INT32_MAX || batch > 65535 || splits > 65535)
+    return NL_EARG;
+  hipStream_t st = (hipStream_t)stream;
+  GemmF64Args g;
+  g.M = (int)M; g.N = (int)N; g.K = (int)K; g.alpha = alpha; g.beta = beta;
+  g.A = A; g.lda = lda; g.sA = strideA; g.B = B; g.ldb = ldb; g.sB = strideB;
+  g.C = C; g.ldc = ldc; g.sC = strideC;
+  g.tiles_m = (int)nl_cdiv(M, TM); g.tiles_n = (int)nl_cdiv(N, TN);
+  int64_t kchunk = nl_cdiv(nl_cdiv(K > 0 ? K : 1, splits), BK64) * BK64;
+  g.splits = (int)splits; g.kchunk = (int)kchunk; g.ws = ws;
+  if (K == 0) {  // C = beta * C
+    g.splits = 1;
+  }
+  dim3 grid((unsigned)(g.tiles_m * g.tiles_n), (unsigned)g.splits, (unsigned)batch);
+  if (a_kcontig && b_ncontig) hipLaunchKernelGGL((gemm_f64_kernel<true, true>), grid, dim3(256), 0, st, g);
+  else if (a_kcontig)         hipLaunchKernelGGL((gemm_f64_kernel<true, false>), grid, dim3(256), 0, st, g);
+  else if (b_ncontig)         hipLaunchKernelGGL((gemm_f64_kernel<false, true>), grid, dim3(256), 0, st, g);
+  else                        hipLaunchKernelGGL((gemm_f64_kernel<false, false>), grid, dim3(256), 0, st, g);
+  NL_LAUNCH_CHECK();
+  if (g.splits > 1) {
+    int64_t total = batch * M * N;
+    int blocks = (int)std::min<int64_t>(nl_cdiv(total, 256), 8192);
+    hipLaunchKernelGGL(splitk_reduce_kernel, dim3(blocks), dim3(256), 0, st, (int)M, (int)N,
+                       g.splits, (int)batch, (const double*)ws, beta, C, ldc, strideC);
+    NL_LAUNCH_CHECK();
+  }
+  return NL_OK;
+}
+
+// C[b] (FP64, row-major) = alpha * diag(ia) * (A[b] * B[b]^T) * diag(ib) + beta * C[b]
+// A: [M][lda], B: [N][ldb], 16-bit elements (dtype 0 = bf16, 1 = fp16), K-contiguous rows.
+// K, lda, ldb must be multiples of 8 and A, B 16-byte aligned.
+int nl_gemm_lp(int dtype, int64_t M, int64_t N, int64_t K, double alpha, const void* A,
+               int64_t lda, int64_t strideA, const float* ia, int64_t strideIa, const void* B,
+               int64_t ldb, int64_t strideB, const float* ib, int64_t strideIb, double beta,
+               double* C, int64_t ldc, int64_t strideC, int64_t batch, int64_t splits,
+               double* ws, void* stream) {
+  if (M < 0 || N < 0 || K < 0 || batch < 0 || (dtype != 0 && dtype != 1)) return NL_EARG;
+  if (M == 0 || N == 0 || batch == 0) return NL_OK;
+  if (!A || !B || !C) return NL_EARG;
+  if ((K % 8) || (lda % 8) || (ldb % 8) || lda < K || ldb < K) return NL_EARG;
+  if ((((uintptr_t)A) & 15) || (((uintptr_t)B) & 15) || (strideA % 8) || (strideB % 8))
+    return NL_EARG;
+  if (M > INT32_MAX || N > INT32_MAX || K > INT32_MAX || batch > 65535) return NL_EARG;
+  if (splits < 1 || splits > 65535 || (splits > 1 && !ws)) return NL_EARG;
+  GemmLPArgs g;
+  g.M = (int)M; g.N = (int)N; g.K = (int)K; g.alpha = alpha; g.beta = beta;
+  g.A = (const uint16_t*)A; g.lda = lda; g.sA = strideA;
+  g.B = (const uint16_t*)B; g.ldb = ldb; g.sB = strideB;
+  g.ia = ia; g.sia = strideIa; g.ib = ib; g.sib = strideIb;
+  g.C = C; g.ldc = ldc; g.sC = strideC;
+  g.tiles_m = (int)nl_cdiv(M, TM); g.tiles_n = (int)nl_cdiv(N, TN);
+  g.splits = (int)splits; g.ws = ws;
+  g.kchunk = (int)(nl_cdiv(nl_cdiv(K, splits), BKL) * BKL);   // multiple of 64 (hence of 8)
+  dim3 grid((unsigned)(g.tiles_m * g.tiles_n), (unsigned)splits, (unsigned)batch);
+  hipStream_t st = (hipStream_t)stream;
+  if (K == 0) {
+    // nothing to accumulate: C = beta * C via a degenerate FP64 call path is not needed by
+    // any caller; reject to keep the contract simple.
+    return NL_EARG;
+  }
+  if (dtype == 0) hipLaunchKernelGGL((gemm_lp_kernel<true>), grid, dim3(256), 0, st, g);
+  else            hipLaunchKernelGGL((gemm_lp_kernel<false>), grid, dim3(256), 0, st, g);
+  NL_LAUNCH_CHECK();
+  if (splits > 1) {
+    int64_t total = batch * M * N;
+    int blocks = (int)std::min<int64_t>(nl_cdiv(total, 256), 8192);
+    hipLaunchKernelGGL(splitk_reduce_kernel, dim3(blocks), dim3(256), 0, st, (int)M, (int)N,
+                       (int)splits, (int)batch, (const double*)ws, beta, C, ldc, strideC);
+    NL_LAUNCH_CHECK();
+  }
+  return NL_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,549 @@
+// Bandwidth-bound kernels of the CCSD iteration on gfx950: layout changes, precision conversion
+// with per-row scaling, COO (segment) accumulations, the fused T2 assembly/Jacobi update/energy,
+// and deterministic reductions.  Every kernel is a grid-stride loop over coalesced output
+// elements; reductions use wavefront shuffles and a fixed-order second pass (bitwise
+// reproducible run to run).
+#include "nl_common.h"
+
+namespace {
+
+constexpr int NT = 256;
+
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+
+__device__ inline float wave_max(float v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+// Block-wide sum; result valid in thread 0.
+__device__ inline double block_sum(double v) {
+  __shared__ double part[NT / 64];
+  v = wave_sum(v);
+  int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) part[w] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < NT / 64; ++i) s += part[i];
+  __syncthreads();
+  return s;
+}
+
+struct PermArgs {
+  int rank;
+  int64_t shape[6];    // output shape (row-major, contiguous output)
+  int64_t istr[6];     // input element stride of each output dimension
+  int64_t total;
+  double alpha, beta;
+  const double* in;
+  double* out;
+};
+
+__global__ void permute_kernel(PermArgs a) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < a.total;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    int64_t rem = t, off = 0;
+    for (int d = a.rank - 1; d >= 0; --d) {
+      int64_t q = rem / a.shape[d];
+      off += (rem - q * a.shape[d]) * a.istr[d];
+      rem = q;
+    }
+    double v = a.alpha * a.in[off];
+    a.out[t] = (a.beta != 0.0) ? v + a.beta * a.out[t] : v;
+  }
+}
+
+struct ConvArgs {
+  int64_t R2, C, C2, ldo;   // rows = R1*R2 (grid), cols = C1*C2 = C
+  int64_t s[4];             // strides of (r1, r2, c1, c2)
+  const double* in;
+  uint16_t* out;
+  float* iscale;            // per-row inverse scale (power of two)
+  float target;             // row max is scaled to at most this
+  int bf16;
+};
+
+__device__ inline uint16_t to_bf16_bits(float f) {
+  __bf16 b = (__bf16)f;  // round to nearest even
+  return __builtin_bit_cast(uint16_t, b);
+}
+__device__ inline uint16_t to_f16_bits(float f) {
+  _Float16 h = (_Float16)f;
+  return __builtin_bit_cast(uint16_t, h);
+}
+
+// Largest e with 2^e <= target/amax (clamped), the per-row power-of-two scale exponent.
+__device__ inline int scale_exp(float amax, float target) {
+  int e = 0;
+  if (amax > 0.f && isfinite(amax)) {
+    int ex;
+    frexpf(target / amax, &ex);   // target/amax = f * 2^ex, f in [0.5, 1)
+    e = max(-126, min(126, ex - 1));
+  }
+  return e;
+}
+
+// One workgroup per output row: amax pass, then scaled conversion (per-row power-of-two scale).
+// Used when the row's elements are unit-stride or when the rows themselves are strided.
+__global__ __launch_bounds__(NT) void convert_rows_kernel(ConvArgs a) {
+  __shared__ float red[NT / 64];
+  int64_t r = blockIdx.x;
+  int64_t r1 = r / a.R2, r2 = r - r1 * a.R2;
+  const double* base = a.in + r1 * a.s[0] + r2 * a.s[1];
+  float m = 0.f;
+  for (int64_t c = threadIdx.x; c < a.C; c += NT) {
+    int64_t c1 = c / a.C2, c2 = c - c1 * a.C2;
+    m = fmaxf(m, (float)fabs(base[c1 * a.s[2] + c2 * a.s[3]]));
+  }
+  m = wave_max(m);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  float amax = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+  int e = scale_exp(amax, a.target);
+  float sc = ldexpf(1.f, e);
+  if (threadIdx.x == 0) a.iscale[r] = ldexpf(1.f, -e);
+  uint16_t* o = a.out + r * a.ldo;
+  for (int64_t c = threadIdx.x; c < a.ldo; c += NT) {
+    float v = 0.f;
+    if (c < a.C) {
+      int64_t c1 = c / a.C2, c2 = c - c1 * a.C2;
+      v = (float)base[c1 * a.s[2] + c2 * a.s[3]] * sc;
+    }
+    o[c] = a.bf16 ? to_bf16_bits(v) : to_f16_bits(v);
+  }
+}
+
+// Transposing variant (the output rows are the unit-stride direction of the source): 64x64
+// tiles through LDS so that both the FP64 reads and the 16-bit writes are coalesced.
+//   pass 1: per-row amax (as float bits, atomicMax; |x| >= 0 orders like its bit pattern)
+//   pass 2: scaled conversion, transposed through LDS
+//   pass 3: amax bits -> inverse scales
+__global__ __launch_bounds__(NT) void zero_u32_kernel(int64_t n, unsigned* x) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n;
+       t += (int64_t)gridDim.x * blockDim.x)
+    x[t] = 0u;
+}
+
+__global__ __launch_bounds__(NT) void amax_tiles_kernel(ConvArgs a, int64_t R, unsigned* bits) {
+  __shared__ float red[4][64];
+  const int tr = threadIdx.x & 63, tg = threadIdx.x >> 6;
+  const int64_t r = blockIdx.x * 64 + tr, c0 = blockIdx.y * 64;
+  float m = 0.f;
+  if (r < R) {
+    int64_t r1 = r / a.R2, r2 = r - r1 * a.R2;
+    const double* base = a.in + r1 * a.s[0] + r2 * a.s[1];
+    for (int cc = tg; cc < 64; cc += 4) {
+      int64_t c = c0 + cc;
+      if (c < a.C) {
+        int64_t c1 = c / a.C2, c2 = c - c1 * a.C2;
+        m = fmaxf(m, (float)fabs(base[c1 * a.s[2] + c2 * a.s[3]]));
+      }
+    }
+  }
+  red[tg][tr] = m;
+  __syncthreads();
+  if (tg == 0 && r < R) {
+    m = fmaxf(fmaxf(red[0][tr], red[1][tr]), fmaxf(red[2][tr], red[3][tr]));
+    atomicMax(&bits[r], __float_as_uint(m));
+  }
+}
+
+__global__ __launch_bounds__(NT) void convert_tiles_kernel(ConvArgs a, int64_t R,
+                                                           const unsigned* bits) {
+  __shared__ uint16_t tile[64][66];
+  const int tr = threadIdx.x & 63, tg = threadIdx.x >> 6;
+  const int64_t r0 = blockIdx.x * 64, c0 = blockIdx.y * 64, r = r0 + tr;
+  if (r < R) {
+    float sc = ldexpf(1.f, scale_exp(__uint_as_float(bits[r]), a.target));
+    int64_t r1 = r / a.R2, r2 = r - r1 * a.R2;
+    const double* base = a.in + r1 * a.s[0] + r2 * a.s[1];
+    for (int cc = tg; cc < 64; cc += 4) {
+      int64_t c = c0 + cc;
+      float v = 0.f;
+      if (c < a.C) {
+        int64_t c1 = c / a.C2, c2 = c - c1 * a.C2;
+        v = (float)base[c1 * a.s[2] + c2 * a.s[3]] * sc;
+      }
+      tile[tr][cc] = a.bf16 ? to_bf16_bits(v) : to_f16_bits(v);
+    }
+  }
+  __syncthreads();
+  for (int rr = tg; rr < 64; rr += 4) {
+    int64_t row = r0 + rr, c = c0 + tr;
+    if (row < R && c < a.ldo) a.out[row * a.ldo + c] = tile[rr][tr];
+  }
+}
+
+__global__ __launch_bounds__(NT) void iscale_kernel(int64_t R, float target, float* iscale) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < R;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    float amax = __uint_as_float(__float_as_uint(iscale[t]));
+    iscale[t] = ldexpf(1.f, -scale_exp(amax, target));
+  }
+}
+
+// tau[a,b,i,j] = t2[a,b,i,j] + t1[a,i] t1[b,j] - t1[a,j] t1[b,i]
+__global__ void tau_kernel(int64_t p, int64_t h, const double* t1, const double* t2, double* tau) {
+  int64_t total = p * p * h * h;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    int64_t j = t % h, r = t / h, i = r % h;
+    r /= h;
+    int64_t b = r % p, a = r / p;
+    tau[t] = t2[t] + t1[a * h + i] * t1[b * h + j] - t1[a * h + j] * t1[b * h + i];
+  }
+}
+
+// out[dst[s] + x*ds] += alpha * sum_{e in [ptr[s], ptr[s+1])} val[e] * src[so[e] + x*ss]
+// one wavefront per segment.
+__global__ __launch_bounds__(NT) void segsum_kernel(int64_t S, int64_t X, const int64_t* ptr,
+                                                  const int64_t* so, const double* val,
+                                                  const int64_t* dst, int64_t ss, int64_t ds,
+                                                  double alpha, const double* src, double* out) {
+  int lane = threadIdx.x & 63;
+  for (int64_t s = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6; s < S;
+       s += ((int64_t)gridDim.x * blockDim.x) >> 6) {
+    int64_t e0 = ptr[s], e1 = ptr[s + 1];
+    for (int64_t x = lane; x < X; x += 64) {
+      double acc = 0.0;
+      for (int64_t e = e0; e < e1; ++e) acc += val[e] * src[so[e] + x * ss];
+      double* o = out + dst[s] + x * ds;
+      *o += alpha * acc;
+    }
+  }
+}
+
+// out[r][c] = idx[r] >= 0 ? src[idx[r]*lds + c] : 0   (r < R, c < Cn), out contiguous
+__global__ void gather_rows_kernel(int64_t R, int64_t Cn, const int64_t* idx, const double* src,
+                                   int64_t lds, double* out) {
+  int64_t total = R * Cn;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    int64_t r = t / Cn, c = t - r * Cn;
+    int64_t k = idx[r];
+    out[t] = (k >= 0) ? src[k * lds + c] : 0.0;
+  }
+}
+
+// out[idx[r]*ldo + c] += alpha * src[r*Cn + c]   (rows with idx < 0 skipped; idx unique)
+__global__ void scatter_add_rows_kernel(int64_t R, int64_t Cn, const int64_t* idx,
+                                        const double* src, double alpha, double* out,
+                                        int64_t ldo) {
+  int64_t total = R * Cn;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    int64_t r = t / Cn, c = t - r * Cn;
+    int64_t k = idx[r];
+    if (k >= 0) out[k * ldo + c] += alpha * src[t];
+  }
+}
+
+// t1n = m*t1 + (1-m)*(t1 + H1/(-(Xpp[a,a] + Xhh[i,i])))
+__global__ void update_t1_kernel(int64_t p, int64_t h, const double* t1, const double* H1,
+                                 const double* Xpp, const double* Xhh, double mixing,
+                                 double* t1n) {
+  int64_t total = p * h;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    int64_t a = t / h, i = t - a * h;
+    double denom = -(Xpp[a * p + a] + Xhh[i * h + i]);
+    double nt = t1[t] + H1[t] / denom;
+    t1n[t] = mixing * t1[t] + (1.0 - mixing) * nt;
+  }
+}
+
+struct AssembleArgs {
+  int64_t p, h;
+  const double *S0, *Sab, *Sij, *Spp;   // Spp in (a,i),(b,j) layout; any may be null
+  const double *Xpp, *Xhh;              // dressed diagonals (full p x p, h x h matrices)
+  const double *t2, *vpphh, *t1n;       // old t2, v_pphh, new t1 (for the fused energy)
+  double mixing;
+  double* t2n;                          // new t2 (may alias nothing)
+  double* part;                         // per-block partials: [3][gridDim]: E2, |H2|^2, spare
+  double* H2out;                        // optional: raw residual H2 (for adjoint / diagnostics)
+};
+
+__global__ __launch_bounds__(NT) void assemble_t2_kernel(AssembleArgs g) {
+  const int64_t p = g.p, h = g.h, hh = h * h, total = p * p * hh;
+  double e_acc = 0.0, r_acc = 0.0;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    int64_t j = t % h, r = t / h, i = r % h;
+    r /= h;
+    int64_t b = r % p, a = r / p;
+    double H = g.S0 ? g.S0[t] : 0.0;
+    if (g.Sab) H += g.Sab[t] - g.Sab[((b * p + a) * h + i) * h + j];
+    if (g.Sij) H += g.Sij[t] - g.Sij[((a * p + b) * h + j) * h + i];
+    if (g.Spp) {
+      const int64_t ph = p * h;
+      H += g.Spp[(a * h + i) * ph + b * h + j] - g.Spp[(b * h + i) * ph + a * h + j]
+         - g.Spp[(a * h + j) * ph + b * h + i] + g.Spp[(b * h + j) * ph + a * h + i];
+    }
+    double denom = -(g.Xpp[a * p + a] + g.Xpp[b * p + b] + g.Xhh[i * h + i] + g.Xhh[j * h + j]);
+    double told = g.t2[t];
+    double nt = told + H / denom;
+    double tn = g.mixing * told + (1.0 - g.mixing) * nt;
+    if (g.t2n) g.t2n[t] = tn;
+    if (g.H2out) g.H2out[t] = H;
+    if (g.vpphh) {
+      double v = g.vpphh[t];
+      e_acc += 0.25 * v * tn + 0.5 * v * g.t1n[a * h + i] * g.t1n[b * h + j];
+    }
+    r_acc += H * H;
+  }
+  double e = block_sum(e_acc);
+  double rr = block_sum(r_acc);
+  if (threadIdx.x == 0) {
+    g.part[blockIdx.x] = e;
+    g.part[gridDim.x + blockIdx.x] = rr;
+  }
+}
+
+// E2 partials for E = f.t1 + 1/4 v.t2 + 1/2 v t1 t1 (the f.t1 part is added by finalize).
+__global__ __launch_bounds__(NT) void energy_part_kernel(int64_t p, int64_t h, const double* v,
+                                                         const double* t2, const double* t1,
+                                                         double* part) {
+  const int64_t total = p * p * h * h;
+  double acc = 0.0;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    int64_t j = t % h, r = t / h, i = r % h;
+    r /= h;
+    int64_t b = r % p, a = r / p;
+    double x = v[t];
+    acc += 0.25 * x * t2[t] + 0.5 * x * t1[a * h + i] * t1[b * h + j];
+  }
+  double s = block_sum(acc);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// out[0] = sum(part[0:n]) + sum_k x[k]*y[k] over nx elements (fixed order, one workgroup)
+__global__ __launch_bounds__(NT) void finalize_kernel(int64_t n, const double* part, int64_t nx,
+                                                      const double* x, const double* y,
+                                                      double* out) {
+  double acc = 0.0;
+  for (int64_t k = threadIdx.x; k < nx; k += NT) acc += x[k] * y[k];
+  double a = block_sum(acc);
+  double b = 0.0;
+  for (int64_t k = threadIdx.x; k < n; k += NT) b += part[k];
+  b = block_sum(b);
+  if (threadIdx.x == 0) out[0] = a + b;
+}
+
+__global__ __launch_bounds__(NT) void dot_part_kernel(int64_t n, const double* x, const double* y,
+                                                      double* part) {
+  double acc = 0.0;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n;
+       t += (int64_t)gridDim.x * blockDim.x)
+    acc += x[t] * y[t];
+  double s = block_sum(acc);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// y = a*x + b*y
+__global__ void axpby_kernel(int64_t n, double a, const double* x, double b, double* y) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    double v = a * x[t];
+    y[t] = (b != 0.0) ? v + b * y[t] : v;
+  }
+}
+
+// c = x - y (difference vectors for DIIS errors)
+__global__ void sub_kernel(int64_t n, const double* x, const double* y, double* c) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n;
+       t += (int64_t)gridDim.x * blockDim.x)
+    c[t] = x[t] - y[t];
+}
+
+inline int grid_for(int64_t n, int cap = 16384) {
+  int64_t g = nl_cdiv(n, NT);
+  if (g < 1) g = 1;
+  return (int)(g < cap ? g : cap);
+}
+
+}  // namespace
+
+extern "C" {
+
+int nl_permute(int rank, const int64_t* out_shape, const int64_t* in_strides, double alpha,
+               const double* in, double beta, double* out, void* stream) {
+  if (rank < 1 || rank > 6 || !out_shape || !in_strides || !in || !out) return NL_EARG;
+  PermArgs a;
+  a.rank = rank;
+  a.total = 1;
+  for (int d = 0; d < rank; ++d) {
+    if (out_shape[d] < 0) return NL_EARG;
+    a.shape[d] = out_shape[d];
+    a.istr[d] = in_strides[d];
+    a.total *= out_shape[d];
+  }
+  if (a.total == 0) return NL_OK;
+  a.alpha = alpha; a.beta = beta; a.in = in; a.out = out;
+  hipLaunchKernelGGL(permute_kernel, dim3(grid_for(a.total)), dim3(NT), 0, (hipStream_t)stream, a);
+  NL_LAUNCH_CHECK();
+  return NL_OK;
+}
+
+int nl_convert_rows(int dtype, int64_t R1, int64_t R2, int64_t C1, int64_t C2,
+                    const int64_t* strides4, const double* in, void* out, int64_t ldo,
+                    float* iscale, float target, void* stream) {
+  if ((dtype != 0 && dtype != 1) || !strides4 || !in || !out || !iscale) return NL_EARG;
+  if (R1 < 0 || R2 < 1 || C1 < 0 || C2 < 1 || ldo < C1 * C2 || (ldo % 8)) return NL_EARG;
+  if (R1 * R2 == 0) return NL_OK;
+  if (R1 * R2 > INT32_MAX) return NL_EARG;
+  ConvArgs a;
+  a.R2 = R2; a.C = C1 * C2; a.C2 = C2; a.ldo = ldo;
+  for (int d = 0; d < 4; ++d) a.s[d] = strides4[d];
+  a.in = in; a.out = (uint16_t*)out; a.iscale = iscale; a.target = target; a.bf16 = dtype == 0;
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t R = R1 * R2;
+  if (a.s[1] == 1 && R2 >= 64 && a.s[3] != 1 && nl_cdiv(ldo, 64) <= 65535) {
+    // rows are the unit-stride direction of the source: transpose through LDS tiles
+    unsigned* bits = (unsigned*)iscale;   // holds amax bits until the last pass
+    hipLaunchKernelGGL(zero_u32_kernel, dim3(grid_for(R)), dim3(NT), 0, st, R, bits);
+    NL_LAUNCH_CHECK();
+    dim3 g1((unsigned)nl_cdiv(R, 64), (unsigned)nl_cdiv(a.C, 64));
+    hipLaunchKernelGGL(amax_tiles_kernel, g1, dim3(NT), 0, st, a, R, bits);
+    NL_LAUNCH_CHECK();
+    dim3 g2((unsigned)nl_cdiv(R, 64), (unsigned)nl_cdiv(ldo, 64));
+    hipLaunchKernelGGL(convert_tiles_kernel, g2, dim3(NT), 0, st, a, R, (const unsigned*)bits);
+    NL_LAUNCH_CHECK();
+    hipLaunchKernelGGL(iscale_kernel, dim3(grid_for(R)), dim3(NT), 0, st, R, target, iscale);
+    NL_LAUNCH_CHECK();
+    return NL_OK;
+  }
+  hipLaunchKernelGGL(convert_rows_kernel, dim3((unsigned)R), dim3(NT), 0, st, a);
+  NL_LAUNCH_CHECK();
+  return NL_OK;
+}
+
+int nl_tau(int64_t p, int64_t h, const double* t1, const double* t2, double* tau, void* stream) {
+  if (p < 0 || h < 0 || !t1 || !t2 || !tau) return NL_EARG;
+  int64_t n = p * p * h * h;
+  if (n == 0) return NL_OK;
+  hipLaunchKernelGGL(tau_kernel, dim3(grid_for(n)), dim3(NT), 0, (hipStream_t)stream, p, h, t1,
+                     t2, tau);
+  NL_LAUNCH_CHECK();
+  return NL_OK;
+}
+
+int nl_segsum(int64_t S, int64_t X, const int64_t* ptr, const int64_t* src_off, const double* val,
+              const int64_t* dst_off, int64_t src_stride, int64_t dst_stride, double alpha,
+              const double* src, double* out, void* stream) {
+  if (S < 0 || X < 0) return NL_EARG;
+  if (S == 0 || X == 0) return NL_OK;
+  if (!ptr || !src_off || !val || !dst_off || !src || !out) return NL_EARG;
+  int blocks = grid_for(S * 64);
+  hipLaunchKernelGGL(segsum_kernel, dim3(blocks), dim3(NT), 0, (hipStream_t)stream, S, X, ptr,
+                     src_off, val, dst_off, src_stride, dst_stride, alpha, src, out);
+  NL_LAUNCH_CHECK();
+  return NL_OK;
+}
+
+int nl_gather_rows(int64_t R, int64_t Cn, const int64_t* idx, const double* src, int64_t lds,
+                   double* out, void* stream) {
+  if (R < 0 || Cn < 0) return NL_EARG;
+  if (R * Cn == 0) return NL_OK;
+  if (!idx || !src || !out) return NL_EARG;
+  hipLaunchKernelGGL(gather_rows_kernel, dim3(grid_for(R * Cn)), dim3(NT), 0, (hipStream_t)stream,
+                     R, Cn, idx, src, lds, out);
+  NL_LAUNCH_CHECK();
+  return NL_OK;
+}
+
+int nl_scatter_add_rows(int64_t R, int64_t Cn, const int64_t* idx, const double* src, double alpha,
+                        double* out, int64_t ldo, void* stream) {
+  if (R < 0 || Cn < 0) return NL_EARG;
+  if (R * Cn == 0) return NL_OK;
+  if (!idx || !src || !out) return NL_EARG;
+  hipLaunchKernelGGL(scatter_add_rows_kernel, dim3(grid_for(R * Cn)), dim3(NT), 0,
+                     (hipStream_t)stream, R, Cn, idx, src, alpha, out, ldo);
+  NL_LAUNCH_CHECK();
+  return NL_OK;
+}
+
+int nl_update_t1(int64_t p, int64_t h, const double* t1, const double* H1, const double* Xpp,
+                 const double* Xhh, double mixing, double* t1n, void* stream) {
+  if (p < 0 || h < 0 || !t1 || !H1 || !Xpp || !Xhh || !t1n) return NL_EARG;
+  if (p * h == 0) return NL_OK;
+  hipLaunchKernelGGL(update_t1_kernel, dim3(grid_for(p * h)), dim3(NT), 0, (hipStream_t)stream, p,
+                     h, t1, H1, Xpp, Xhh, mixing, t1n);
+  NL_LAUNCH_CHECK();
+  return NL_OK;
+}
+
+// Number of partial-sum slots (per quantity) nl_assemble_t2 / nl_energy / nl_dot use.
+int64_t nl_partials(int64_t n) { return grid_for(n, 4096); }
+
+int nl_assemble_t2(int64_t p, int64_t h, const double* S0, const double* Sab, const double* Sij,
+                   const double* Spp, const double* Xpp, const double* Xhh, const double* t2,
+                   const double* vpphh, const double* t1n, double mixing, double* t2n,
+                   double* H2out, double* part, void* stream) {
+  if (p < 0 || h < 0 || !Xpp || !Xhh || !t2 || !part) return NL_EARG;
+  if (vpphh && !t1n) return NL_EARG;
+  int64_t n = p * p * h * h;
+  if (n == 0) return NL_OK;
+  AssembleArgs g;
+  g.p = p; g.h = h; g.S0 = S0; g.Sab = Sab; g.Sij = Sij; g.Spp = Spp; g.Xpp = Xpp; g.Xhh = Xhh;
+  g.t2 = t2; g.vpphh = vpphh; g.t1n = t1n; g.mixing = mixing; g.t2n = t2n; g.part = part;
+  g.H2out = H2out;
+  hipLaunchKernelGGL(assemble_t2_kernel, dim3(grid_for(n, 4096)), dim3(NT), 0, (hipStream_t)stream,
+                     g);
+  NL_LAUNCH_CHECK();
+  return NL_OK;
+}
+
+int nl_energy_part(int64_t p, int64_t h, const double* vpphh, const double* t2, const double* t1,
+                   double* part, void* stream) {
+  if (p < 0 || h < 0 || !vpphh || !t2 || !t1 || !part) return NL_EARG;
+  int64_t n = p * p * h * h;
+  if (n == 0) return NL_OK;
+  hipLaunchKernelGGL(energy_part_kernel, dim3(grid_for(n, 4096)), dim3(NT), 0, (hipStream_t)stream,
+                     p, h, vpphh, t2, t1, part);
+  NL_LAUNCH_CHECK();
+  return NL_OK;
+}
+
+int nl_finalize(int64_t nparts, const double* part, int64_t nx, const double* x, const double* y,
+                double* out, void* stream) {
+  if (nparts < 0 || nx < 0 || !out || (nparts && !part) || (nx && (!x || !y))) return NL_EARG;
+  hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(NT), 0, (hipStream_t)stream, nparts, part, nx,
+                     x, y, out);
+  NL_LAUNCH_CHECK();
+  return NL_OK;
+}
+
+int nl_dot_part(int64_t n, const double* x, const double* y, double* part, void* stream) {
+  if (n < 0 || !x || !y || !part) return NL_EARG;
+  if (n == 0) return NL_OK;
+  hipLaunchKernelGGL(dot_part_kernel, dim3(grid_for(n, 4096)), dim3(NT), 0, (hipStream_t)stream,
+                     n, x, y, part);
+  NL_LAUNCH_CHECK();
+  return NL_OK;
+}
+
+int nl_axpby(int64_t n, double a, const double* x, double b, double* y, void* stream) {
+  if (n < 0 || !x || !y) return NL_EARG;
+  if (n == 0) return NL_OK;
+  hipLaunchKernelGGL(axpby_kernel, dim3(grid_for(n)), dim3(NT), 0, (hipStream_t)stream, n, a, x, b,
+                     y);
+  NL_LAUNCH_CHECK();
+  return NL_OK;
+}
+
+int nl_sub(int64_t n, const double* x, const double* y, double* c, void* stream) {
+  if (n < 0 || !x || !y || !c) return NL_EARG;
+  if (n == 0) return NL_OK;
+  hipLaunchKernelGGL(sub_kernel, dim3(grid_for(n)), dim3(NT), 0, (hipStream_t)stream, n, x, y, c);
+  NL_LAUNCH_CHECK();
+  return NL_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,306 @@
+"""Device-resident CCSD residual of NuLattice's CCM path, evaluated with libnlccsd.so kernels.
+
+``Hamiltonian`` holds the normal-ordered blocks on the GPU in the reference layout
+(f_pp[a,b], f_ph[a,i], f_hh[i,j], v_pphh[a,b,i,j], v_phph[a,i,b,j], v_phhh[a,i,j,k],
+v_hhhh[i,j,k,l]; coupled_cluster.py:99-263) plus the derived layouts the kernels want, and the
+packed plans of the sparse blocks v_pppp / v_ppph.
+
+``t1_part``/``t2_part``/``update`` compute, for amplitudes (t1, t2), exactly the sums of the
+reference's t1Iter and t2Iter (coupled_cluster.py:342-545, ccDgrams.py) regrouped for the GPU:
+
+* T2 terms are accumulated into four buffers by antisymmetriser, S0 (none), Sab (P(ab)),
+  Sij (P(ij)) and Spp (P(ij)P(ab), stored as [(a,i)][(b,j)]); one fused kernel applies the
+  antisymmetrisers, the dressed denominator, the Jacobi step, the mixing and the energy.
+* Ring family (A22 v_phph ring, A24 quadratic ring, A37 ppph ring, A29):
+      Spp[(a,i),(b,j)] += sum_(c,k) t2[a,c,i,k] W[(c,k),(b,j)],
+      W = -v_phph[b,k,c,j] + 1/2 sum_(d,l) v_pphh[c,d,k,l] t2[d,b,l,j] + ret4[b,c,j,k]
+          - sum_l v_phhh[c,j,k,l] t1[b,l]
+  (P(ij)P(ab) is invariant under (a,i)<->(b,j), which moves every ring term onto t2[a,c,i,k]).
+* hh-ladder family (A21, A25, A26, A30-A33, A35):  S0 += 1/2 sum_kl tau[a,b,k,l] Whh[k,l,i,j],
+      Whh = v_hhhh + P(ij) sum_c v_phhh[c,j,k,l] t1[c,i] + 1/2 sum_cd v_pphh[c,d,k,l] tau[c,d,i,j],
+      tau = t2 + P(ij) t1 t1.
+* pp ladder (A9): S0 += 1/2 sum_cd v_pppp[a,b,c,d] tau[c,d,i,j], block-dense batched GEMM.
+* v_ppph consumers (A10, A36-A39): segment plans; A39's two terms become one contraction with
+  R56[b,k,i,j] = sum_cd v_ppph[c,d,b,k] tau[c,d,i,j].
+Every regrouping is an exact identity of the reference's sums; only floating-point summation
+order changes.
+
+Precision form: ``prec`` maps the families "ring", "hh", "dress" to "fp64", "bf16" or "fp16";
+low-precision contractions use BF16/FP16 MFMA with FP32 accumulation and per-row power-of-two
+scales, everything else (Fock, denominators, update, energy, DIIS) stays FP64.
+"""
+import ctypes
+import weakref
+
+import numpy as np
+import torch
+
+from . import _lib
+from . import ops
+from .ops import contract, permute_into
+from .plans import PairBlocks, PpphPlans, coo_arrays
+
+F64 = torch.float64
+
+
+def _dev(x, device):
+    if isinstance(x, torch.Tensor):
+        return x.detach().to(device=device, dtype=F64).contiguous()
+    return torch.tensor(np.asarray(x, dtype=np.float64), dtype=F64, device=device)
+
+
+class Hamiltonian:
+    """Normal-ordered Hamiltonian blocks on one GPU (reference layout + derived layouts)."""
+
+    def __init__(self, fock_mats, two_body_int, device="cuda", min_block_rows=128):
+        device = torch.device(device)
+        if device.type != "cuda":
+            raise _lib.NLError("Hamiltonian: a HIP device is required (no CPU fallback)")
+        _lib.load()
+        f_pp, f_ph, f_hh = fock_mats
+        v_pppp, v_ppph, v_pphh, v_phph, v_phhh, v_hhhh = two_body_int
+        self.device = device
+        self.f_pp, self.f_ph, self.f_hh = (_dev(x, device) for x in (f_pp, f_ph, f_hh))
+        self.p, self.h = self.f_ph.shape
+        p, h = self.p, self.h
+        self.v_pphh = _dev(v_pphh, device)
+        self.v_phph = _dev(v_phph, device)
+        self.v_phhh = _dev(v_phhh, device)
+        self.v_hhhh = _dev(v_hhhh, device)
+        for name, t, shp in (("f_pp", self.f_pp, (p, p)), ("f_hh", self.f_hh, (h, h)),
+                             ("v_pphh", self.v_pphh, (p, p, h, h)),
+                             ("v_phph", self.v_phph, (p, h, p, h)),
+                             ("v_phhh", self.v_phhh, (p, h, h, h)),
+                             ("v_hhhh", self.v_hhhh, (h, h, h, h))):
+            if tuple(t.shape) != shp:
+                raise ValueError(f"{name} has shape {tuple(t.shape)}, expected {shp}")
+        # v_pphh with (c,k),(d,l) adjacent: physical [c][k][d][l]
+        self.vP = ops.copy(self.v_pphh.permute(0, 2, 1, 3))
+        # constant part of the ring intermediate: -v_phph[b,k,c,j] at W[c][k][b][j]
+        self.Wc = torch.empty((p, h, p, h), dtype=F64, device=device)
+        permute_into(self.v_phph.permute(2, 1, 0, 3), self.Wc, alpha=-1.0)
+        # v_phph as [b][k][i][c] (A27) and v_pphh as [c][d][k][l] = v_pphh[d,c,k,l] (A19)
+        self.v_phph_bkic = ops.copy(self.v_phph.permute(0, 1, 3, 2))
+        self.v_pphh_T = ops.copy(self.v_pphh.permute(1, 0, 2, 3))
+        consts = (self.f_pp, self.f_ph, self.f_hh, self.v_pphh, self.v_phph, self.v_phhh,
+                  self.v_hhhh, self.vP, self.Wc, self.v_phph_bkic, self.v_pphh_T)
+        ops.mark_constant(*consts)
+        weakref.finalize(self, ops.forget_ptrs, [t.data_ptr() for t in consts])
+        idx, val = coo_arrays(v_pppp, p, h, "pppp")
+        self.pppp = PairBlocks(idx[:, 0] * p + idx[:, 1], idx[:, 2] * p + idx[:, 3], val, p * p,
+                               p * p, device, min_block_rows)
+        idx3, val3 = coo_arrays(v_ppph, p, h, "ppph")
+        self.ppph = PpphPlans(idx3, val3, p, h, device)
+
+    def vpphh_view(self):
+        """v_pphh[c,d,k,l] as a view of the [c][k][d][l] copy."""
+        return self.vP.permute(0, 2, 1, 3)
+
+
+class Buffers:
+    """Per-iteration device buffers (allocated once per problem size)."""
+
+    def __init__(self, p, h, device):
+        z = lambda *s: torch.zeros(s, dtype=F64, device=device)
+        self.H1, self.X1pp, self.X1hh = z(p, h), z(p, p), z(h, h)
+        self.X2pp, self.X2hh = z(p, p), z(h, h)
+        self.s_ph, self.u_ph, self.w_ph, self.z_ph = z(p, h), z(p, h), z(p, h), z(p, h)
+        self.q_hh = z(h, h)
+        self.ret1 = z(p, p)
+        self.tau = z(p, p, h, h)
+        self.S0, self.Sab, self.Sij = z(p, p, h, h), z(p, p, h, h), z(p, p, h, h)
+        self.Spp = z(p, h, p, h)
+        self.Tr = z(p, h, p, h)      # t2 as [a][i][c][k]
+        self.W = z(p, h, p, h)       # ring intermediate [c][k][b][j]
+        self.Whh, self.zz = z(h, h, h, h), z(h, h, h, h)
+        self.N56 = z(p, h, h, h)
+        self.Xidkl = z(h, p, h, h)
+        self.Yilaj = z(h, h, p, h)
+        self.Ubkij = z(p, h, h, h)
+        self.part = z(2 * int(_lib.load().nl_partials(p * p * h * h)) + 8)
+        self.E = z(1)
+
+
+def _segsum(plan, X, src, out, alpha, src_stride=1, dst_stride=1):
+    if plan is None:
+        return
+    _lib.call("nl_segsum", plan.S, X, ops._p(plan.ptr), ops._p(plan.src), ops._p(plan.val),
+              ops._p(plan.dst), src_stride, dst_stride, ctypes.c_double(alpha), ops._p(src),
+              ops._p(out), ops._stream())
+
+
+DEFAULT_PREC = {"ring": "fp64", "hh": "fp64", "dress": "fp64"}
+
+
+def t1_part(ham, t1, t2, B, prec=DEFAULT_PREC):
+    """H1, X1pp, X1hh of t1Iter (coupled_cluster.py:372-408) into B.H1/B.X1pp/B.X1hh."""
+    vpphh = ham.vpphh_view()
+    H1, Xpp, Xhh = B.H1, B.X1pp, B.X1hh
+    permute_into(ham.f_ph, H1)
+    contract("akci,ck->ai", ham.v_phph, t1, H1, alpha=-1.0)                  # A11
+    contract("ck,acik->ai", ham.f_ph, t2, H1)                                 # A12
+    contract("cikl,cakl->ai", ham.v_phhh, t2, H1, alpha=-0.5)                # A13
+    contract("cdkl,ck->dl", vpphh, t1, B.s_ph, beta=0.0)                     # A14
+    contract("dl,dali->ai", B.s_ph, t2, H1)
+    pl = ham.ppph
+    if pl.n:
+        _segsum(pl.ret0, ham.h, t2, H1, -0.5)                                # ret0
+        B.ret1.zero_()
+        _segsum(pl.ret1, 1, t1, B.ret1, -1.0)                                # ret1
+    permute_into(ham.f_hh, Xhh, alpha=-1.0)
+    contract("ck,ci->ki", ham.f_ph, t1, Xhh, alpha=-0.5)                     # A15
+    contract("bijk,bj->ki", ham.v_phhh, t1, Xhh, alpha=-1.0)                 # A17
+    contract("cdlk,cdli->ki", ham.v_pphh, t2, Xhh, alpha=-0.5, precision=prec["dress"])  # A18
+    contract("cdlk,cl->dk", vpphh, t1, B.u_ph, beta=0.0)                     # A20 (hh)
+    contract("dk,di->ki", B.u_ph, t1, Xhh, alpha=-0.5)
+    permute_into(ham.f_pp, Xpp)
+    contract("ck,ak->ac", ham.f_ph, t1, Xpp, alpha=-0.5)                     # A16
+    contract("dckl,dakl->ac", ham.v_pphh_T.permute(1, 0, 2, 3), t2, Xpp, alpha=-0.5,
+             precision=prec["dress"])                                        # A19
+    contract("cdkl,dk->cl", vpphh, t1, B.w_ph, beta=0.0)                     # A20 (pp)
+    contract("cl,al->ac", B.w_ph, t1, Xpp, alpha=0.5)
+    if pl.n:
+        ops.axpby(1.0, B.ret1, 1.0, Xpp)                                     # X_pp += ret1
+    contract("ac,ci->ai", Xpp, t1, H1)
+    contract("ki,ak->ai", Xhh, t1, H1)
+
+
+def t2_part(ham, t1, t2, B, prec=DEFAULT_PREC):
+    """S0/Sab/Sij/Spp and X2pp/X2hh of t2Iter (coupled_cluster.py:476-536).
+
+    Requires B.ret1 and B.u_ph from t1_part at the same amplitudes."""
+    p, h = ham.p, ham.h
+    vpphh = ham.vpphh_view()
+    pl = ham.ppph
+    _lib.call("nl_tau", p, h, ops._p(t1), ops._p(t2), ops._p(B.tau), ops._stream())
+    tau = B.tau
+    # ---- dressed Fock matrices
+    Xpp, Xhh = B.X2pp, B.X2hh
+    permute_into(ham.f_pp, Xpp)
+    contract("cdkl,bdkl->bc", ham.v_pphh, t2, Xpp, alpha=-0.5, precision=prec["dress"])  # A41
+    contract("ck,bk->bc", ham.f_ph, t1, Xpp, alpha=-1.0)                     # A43
+    contract("cdlk,dk->cl", ham.v_pphh, t1, B.z_ph, beta=0.0)               # A46
+    contract("cl,bl->bc", B.z_ph, t1, Xpp, alpha=-1.0)
+    permute_into(ham.f_hh, Xhh, alpha=-1.0)
+    contract("cdkl,cdjl->kj", ham.v_pphh, t2, Xhh, alpha=-0.5, precision=prec["dress"])  # A42
+    contract("ck,cj->kj", ham.f_ph, t1, Xhh, alpha=-1.0)                     # A44
+    contract("dk,dj->kj", B.u_ph, t1, Xhh, alpha=-1.0)                      # A45
+    # ---- S0: v_pphh + 1/2 tau.Whh + 1/2 pp ladder(tau)
+    S0 = B.S0
+    permute_into(ham.v_pphh, S0)
+    permute_into(ham.v_hhhh, B.Whh)
+    contract("ci,cjkl->klij", t1, ham.v_phhh, B.zz, beta=0.0)
+    permute_into(B.zz, B.Whh, 1.0, 1.0)
+    permute_into(B.zz.permute(0, 1, 3, 2), B.Whh, -1.0, 1.0)
+    contract("cdkl,cdij->klij", ham.v_pphh, tau, B.Whh, alpha=0.5, precision=prec["hh"])
+    contract("abkl,klij->abij", tau, B.Whh, S0, alpha=0.5, precision=prec["hh"])
+    _pp_ladder(ham, tau, S0, prec)
+    # ---- Sab: P(ab) group
+    Sab = B.Sab
+    permute_into(ham.v_phhh, B.N56)
+    if pl.n:
+        _segsum(pl.r56, h * h, tau, B.N56, 0.5)
+    contract("bkij,ak->abij", B.N56, t1, Sab, beta=0.0)                      # A23 + A39
+    if pl.n:
+        contract("ad,dbij->abij", B.ret1, t2, Sab)                           # A38
+    contract("bc,acij->abij", Xpp, t2, Sab, precision=prec["dress"])         # X_pp t2
+    # ---- Sij: P(ij) group
+    Sij = B.Sij
+    Sij.zero_()
+    if pl.n:
+        _segsum(pl.ret2, h, t1, Sij, 1.0, 1, h)                              # A36
+    contract("cikl,ck->li", ham.v_phhh, t1, B.q_hh, beta=0.0)                # A28
+    contract("li,ablj->abij", B.q_hh, t2, Sij, alpha=-1.0)
+    contract("kj,abik->abij", Xhh, t2, Sij, precision=prec["dress"])         # X_hh t2
+    # ---- Spp: P(ij)P(ab) group in [(a,i)][(b,j)] layout
+    Tr = B.Tr
+    permute_into(t2.permute(0, 2, 1, 3), Tr)                                 # t2 as [a][i][c][k]
+    t2r = Tr.permute(0, 2, 1, 3)                                             # view t2[a,c,i,k]
+    W = B.W
+    permute_into(ham.Wc, W)                                                  # A22
+    contract("cdkl,dblj->ckbj", vpphh, t2r, W, alpha=0.5, precision=prec["ring"],
+             tag="ring_W")                                                   # A24
+    if pl.n:
+        _segsum(pl.ret4, h, t1, W, 1.0)                                      # A37
+    contract("cjkl,bl->ckbj", ham.v_phhh, t1, W, alpha=-1.0)                 # A29
+    contract("acik,ckbj->aibj", t2r, W, B.Spp, beta=0.0, precision=prec["ring"], tag="ring")
+    contract("cdkl,ci->idkl", ham.v_pphh, t1, B.Xidkl, beta=0.0)             # A34
+    contract("idkl,adkj->ilaj", B.Xidkl, t2, B.Yilaj, beta=0.0)
+    contract("ilaj,bl->aibj", B.Yilaj, t1, B.Spp)
+    contract("bkci,cj->bkij", ham.v_phph_bkic.permute(0, 1, 3, 2), t1, B.Ubkij, beta=0.0)  # A27
+    contract("ak,bkij->aibj", t1, B.Ubkij, B.Spp, alpha=-1.0)
+
+
+def _pp_ladder(ham, tau, S0, prec):
+    """S0 += 1/2 sum_cd v_pppp[a,b,c,d] tau[c,d,i,j] over packed pair blocks."""
+    pb = ham.pppp
+    if pb.nb == 0:
+        return
+    hh = ham.h * ham.h
+    G = ops.WS.get("ladder_G", pb.nb * pb.kb * hh, device=tau.device)
+    R = ops.WS.get("ladder_R", pb.nb * pb.mb * hh, device=tau.device)
+    st = ops._stream()
+    _lib.call("nl_gather_rows", pb.nb * pb.kb, hh, ops._p(pb.cols), ops._p(tau), hh, ops._p(G), st)
+    _lib.call("nl_gemm_f64", 1, 1, pb.mb, hh, pb.kb, ctypes.c_double(1.0), ops._p(pb.V), pb.kb,
+              pb.mb * pb.kb, ops._p(G), hh, pb.kb * hh, ctypes.c_double(0.0), ops._p(R), hh,
+              pb.mb * hh, pb.nb, 1, ctypes.c_void_p(0), st)
+    _lib.call("nl_scatter_add_rows", pb.nb * pb.mb, hh, ops._p(pb.rows), ops._p(R),
+              ctypes.c_double(0.5), ops._p(S0), hh, st)
+
+
+def update(ham, t1, t2, B, t1n, t2n, mixing, prec=DEFAULT_PREC, H2out=None):
+    """One Jacobi step (coupled_cluster.py:602-623): writes t1n, t2n and B.E[0] = E(t1n, t2n)."""
+    p, h = ham.p, ham.h
+    t1_part(ham, t1, t2, B, prec)
+    _lib.call("nl_update_t1", p, h, ops._p(t1), ops._p(B.H1), ops._p(B.X1pp), ops._p(B.X1hh),
+              ctypes.c_double(mixing), ops._p(t1n), ops._stream())
+    t2_part(ham, t1, t2, B, prec)
+    assemble(ham, t2, B, t1n, t2n, mixing, H2out)
+
+
+def assemble(ham, t2, B, t1n, t2n, mixing, H2out=None):
+    p, h = ham.p, ham.h
+    st = ops._stream()
+    _lib.call("nl_assemble_t2", p, h, ops._p(B.S0), ops._p(B.Sab), ops._p(B.Sij), ops._p(B.Spp),
+              ops._p(B.X2pp), ops._p(B.X2hh), ops._p(t2), ops._p(ham.v_pphh), ops._p(t1n),
+              ctypes.c_double(mixing), ops._p(t2n), ops._p(H2out), ops._p(B.part), st)
+    nparts = int(_lib.load().nl_partials(p * p * h * h))
+    _lib.call("nl_finalize", nparts, ops._p(B.part), p * h, ops._p(ham.f_ph), ops._p(t1n),
+              ops._p(B.E), st)
+
+
+def energy(ham, t1, t2, B):
+    """ccsd_energy (coupled_cluster.py:265-286) into B.E[0]."""
+    p, h = ham.p, ham.h
+    st = ops._stream()
+    _lib.call("nl_energy_part", p, h, ops._p(ham.v_pphh), ops._p(t2), ops._p(t1), ops._p(B.part),
+              st)
+    nparts = int(_lib.load().nl_partials(p * p * h * h))
+    _lib.call("nl_finalize", nparts, ops._p(B.part), p * h, ops._p(ham.f_ph), ops._p(t1),
+              ops._p(B.E), st)
+
+
+def init_amplitudes(ham, B, t1, t2, delta):
+    """t1Init / t2Init (coupled_cluster.py:319-340, 416-439) through the update kernels."""
+    p, h = ham.p, ham.h
+    st = ops._stream()
+    zero1 = torch.zeros_like(t1)
+    permute_into(ham.f_hh, B.X1hh, alpha=-1.0)
+    if delta:
+        _diag_add(B.X1hh, -delta)
+    _lib.call("nl_update_t1", p, h, ops._p(zero1), ops._p(ham.f_ph), ops._p(ham.f_pp),
+              ops._p(B.X1hh), ctypes.c_double(0.0), ops._p(t1), st)
+    permute_into(ham.f_hh, B.X2hh, alpha=-1.0)
+    if delta:
+        _diag_add(B.X2hh, -0.5 * delta)
+    zero2 = ops.WS.get("zero_t2", p * p * h * h, device=t2.device).view(p, p, h, h)
+    zero2.zero_()
+    _lib.call("nl_assemble_t2", p, h, ops._p(ham.v_pphh), None, None, None, ops._p(ham.f_pp),
+              ops._p(B.X2hh), ops._p(zero2), None, None, ctypes.c_double(0.0), ops._p(t2), None,
+              ops._p(B.part), st)
+
+
+def _diag_add(X, c):
+    n = X.shape[0]
+    eye = torch.eye(n, dtype=F64, device=X.device)
+    ops.axpby(c, eye, 1.0, X)

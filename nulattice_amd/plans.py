@@ -1,0 +1,161 @@
+"""Host-side packing of the sparse two-body blocks into device plans (built once per Hamiltonian).
+
+v_pppp (COO <ab||cd>) becomes *pair blocks*: the bipartite graph between row pairs (a,b) and
+column pairs (c,d) is split into connected components, each a dense block V_b; small blocks are
+packed block-diagonally and all are padded to one (rows, cols) shape so the pp ladder is one
+batched MFMA GEMM between a row gather and a row scatter.  For plane-wave inputs the components
+are the (total momentum, spin-isospin pair) sectors; for position-space contacts they are sites.
+
+v_ppph (COO <cd||ak>) becomes *segment plans* for nl_segsum: for each intermediate of
+v_ppph_dgrams (ccDgrams.py:13-45) the entries are grouped by output row so that every output row
+is owned by one wavefront (no atomics).
+"""
+import numpy as np
+import scipy.sparse as sp
+import scipy.sparse.csgraph as csg
+import torch
+
+
+def coo_arrays(v, pnum, hnum, kind):
+    """Normalise a reference-format sparse block to (idx (N,4) int64, val (N,) float64).
+
+    Accepts the legacy list [[i0,i1,i2,i3,val], ...] (coupled_cluster.py:256-258), an (N,5)
+    array, an (indices, values) pair, an object with ``.indices``/``.values``
+    (_torch_types.TwoBodyOperator, indices (N,4)), or a dense array (sparse=False path), whose
+    nonzeros are taken."""
+    shape = (pnum, pnum, pnum, pnum if kind == "pppp" else hnum)
+    if hasattr(v, "indices") and hasattr(v, "values"):
+        idx = np.asarray(_np(v.indices), dtype=np.int64).reshape(-1, 4)
+        val = np.asarray(_np(v.values), dtype=np.float64).reshape(-1)
+    elif isinstance(v, tuple) and len(v) == 2:
+        idx = np.asarray(_np(v[0]), dtype=np.int64)
+        if idx.shape[0] == 4 and idx.ndim == 2 and idx.shape[1] != 4:
+            idx = idx.T
+        idx = idx.reshape(-1, 4)
+        val = np.asarray(_np(v[1]), dtype=np.float64).reshape(-1)
+    else:
+        arr = _np(v)
+        arr = np.asarray(arr, dtype=np.float64)
+        if arr.ndim == 4:
+            if arr.shape != shape:
+                raise ValueError(f"dense {kind} block has shape {arr.shape}, expected {shape}")
+            nz = np.nonzero(arr)
+            idx = np.stack(nz, axis=1).astype(np.int64)
+            val = arr[nz]
+        else:
+            arr = arr.reshape(-1, 5) if arr.size else np.zeros((0, 5))
+            idx = arr[:, :4].astype(np.int64)
+            val = arr[:, 4].astype(np.float64)
+    if idx.size:
+        hi = np.array(shape)
+        if (idx < 0).any() or (idx >= hi).any():
+            raise ValueError(f"{kind} COO index out of range for p={pnum}, h={hnum}")
+    return idx, val
+
+
+def _np(x):
+    if isinstance(x, torch.Tensor):
+        return x.detach().cpu().numpy()
+    return x
+
+
+def _group(items, labels):
+    """{label: items with that label} via one sort."""
+    o = np.argsort(labels, kind="stable")
+    lab, it = labels[o], items[o]
+    u, start = np.unique(lab, return_index=True)
+    ends = np.append(start[1:], lab.size)
+    return {int(x): it[s:e] for x, s, e in zip(u, start, ends)}
+
+
+class PairBlocks:
+    """Padded pair-block form of a COO (N,4) matrix over (row pair, col pair)."""
+
+    def __init__(self, row, col, val, nrow, ncol, device, min_rows=128):
+        self.nb = 0
+        if row.size == 0:
+            return
+        g = sp.coo_matrix((np.ones(row.size), (row, nrow + col)), shape=(nrow + ncol,) * 2)
+        ncomp, lab = csg.connected_components(g, directed=False)
+        rlab, clab = lab[row], lab[nrow + col]
+        comps = [int(x) for x in np.unique(rlab)]
+        # rows / cols of each component
+        ur, ur_inv = np.unique(row, return_inverse=True)
+        uc, uc_inv = np.unique(col, return_inverse=True)
+        rcomp = lab[ur]
+        ccomp = lab[nrow + uc]
+        r_of = _group(ur, rcomp)
+        c_of = _group(uc, ccomp)
+        # greedy block-diagonal packing of small components into bins of >= min_rows rows
+        order = sorted(comps, key=lambda c: -len(r_of[c]))
+        bins, cur, cur_rows = [], [], 0
+        for c in order:
+            cur.append(c); cur_rows += len(r_of[c])
+            if cur_rows >= min_rows:
+                bins.append(cur); cur, cur_rows = [], 0
+        if cur:
+            bins.append(cur)
+        mb = max(sum(len(r_of[c]) for c in b) for b in bins)
+        kb = max(sum(len(c_of[c]) for c in b) for b in bins)
+        mb = max(1, mb); kb = max(1, kb)
+        nb = len(bins)
+        rows = -np.ones((nb, mb), np.int64)
+        cols = -np.ones((nb, kb), np.int64)
+        rpos = np.empty(nrow, np.int64)   # row pair -> (bin, local row)
+        cpos = np.empty(ncol, np.int64)
+        rbin = np.empty(nrow, np.int64)
+        for bi, b in enumerate(bins):
+            r = np.concatenate([r_of[c] for c in b]); cc = np.concatenate([c_of[c] for c in b])
+            rows[bi, :len(r)] = r; cols[bi, :len(cc)] = cc
+            rpos[r] = np.arange(len(r)); cpos[cc] = np.arange(len(cc)); rbin[r] = bi
+        flat = (rbin[row] * mb + rpos[row]) * kb + cpos[col]
+        V = np.bincount(flat, weights=val, minlength=nb * mb * kb).reshape(nb, mb, kb)
+        self.nb, self.mb, self.kb = nb, mb, kb
+        self.nnz = row.size
+        self.V = torch.tensor(V, dtype=torch.float64, device=device)
+        self.rows = torch.tensor(rows.reshape(-1), dtype=torch.int64, device=device)
+        self.cols = torch.tensor(cols.reshape(-1), dtype=torch.int64, device=device)
+
+
+class SegPlan:
+    """Segments of (key -> [(src_off, val)]) for nl_segsum, with per-segment destination offset."""
+
+    def __init__(self, key, src_off, val, dst_of_key, device):
+        order = np.argsort(key, kind="stable")
+        key, src_off, val = key[order], src_off[order], val[order]
+        uk, start = np.unique(key, return_index=True)
+        ptr = np.append(start, key.size).astype(np.int64)
+        self.S = int(uk.size)
+        self.ptr = torch.tensor(ptr, dtype=torch.int64, device=device)
+        self.src = torch.tensor(src_off.astype(np.int64), dtype=torch.int64, device=device)
+        self.val = torch.tensor(val, dtype=torch.float64, device=device)
+        self.dst = torch.tensor(np.asarray(dst_of_key(uk), dtype=np.int64), dtype=torch.int64,
+                                device=device)
+
+
+class PpphPlans:
+    """The five segment plans that replace v_ppph_dgrams' scalar loop (ccDgrams.py:36-44).
+
+    Entry (c, d, a, k, v) of <cd||ak>:
+      ret0[a,i]      -= 1/2 v t2[c,d,k,i]   key a        src (c,d,k,:) of t2
+      ret1[a,d]      -= v t1[c,k]           key (a,d)    src (c,k) of t1
+      ret2[c,d,i,k]  += v t1[a,i]           key (c,d,k)  src (a,:) of t1   (dest stride h)
+      ret4[a,d,i,k]  += v t1[c,i]           key (a,d,k)  src (c,:) of t1   (into the ring W)
+      R56[a,k,i,j]   += v tau[c,d,i,j]      key (a,k)    src (c,d,:,:) of tau
+    """
+
+    def __init__(self, idx, val, p, h, device):
+        self.n = val.size
+        if val.size == 0:
+            return
+        c, d, a, k = (idx[:, n].astype(np.int64) for n in range(4))
+        hh, ph = h * h, p * h
+        self.ret0 = SegPlan(a, ((c * p + d) * h + k) * h, val, lambda u: u * h, device)
+        self.ret1 = SegPlan(a * p + d, c * h + k, val, lambda u: u, device)
+        self.ret2 = SegPlan((c * p + d) * h + k, a * h, val,
+                            lambda u: (u // h) * hh + u % h, device)
+        # ret4[a,d,i,k] lands in W[(d,k),(a,i)] = W[(d*h + k)*ph + a*h + i]
+        self.ret4 = SegPlan((a * p + d) * h + k, c * h, val,
+                            lambda u: ((u // h) % p * h + u % h) * ph + (u // (p * h)) * h,
+                            device)
+        self.r56 = SegPlan(a * h + k, (c * p + d) * hh, val, lambda u: u * hh, device)

@@ -1,0 +1,138 @@
+"""GPU parity of the individual kernels (through the C ABI) against float64 references."""
+import ctypes
+
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+from nulattice_amd import _lib, ops  # noqa: E402
+
+
+def dev(x):
+    return torch.tensor(np.asarray(x, np.float64), dtype=torch.float64, device="cuda")
+
+
+def host(t):
+    return t.detach().cpu().numpy()
+
+
+SPECS = [
+    # (spec, shapes of A, B) covering batch, broadcast batch, split-K, transposed operands
+    ("ij,jk->ik", (37, 53), (53, 29)),
+    ("ji,jk->ik", (53, 37), (53, 29)),
+    ("ij,kj->ik", (37, 53), (29, 53)),
+    ("ji,kj->ki", (53, 37), (29, 53)),
+    ("bij,bjk->bik", (3, 17, 40), (3, 40, 19)),
+    ("akci,ck->ai", (9, 4, 9, 4), (9, 4)),
+    ("cdkl,cdij->klij", (9, 9, 4, 4), (9, 9, 4, 4)),
+    ("abkl,klij->abij", (9, 9, 4, 4), (4, 4, 4, 4)),
+    ("bkij,ak->abij", (9, 4, 4, 4), (9, 4)),
+    ("idkl,adkj->ilaj", (4, 9, 4, 4), (9, 9, 4, 4)),
+    ("cjkl,bl->ckbj", (9, 4, 4, 4), (9, 4)),
+    ("li,ablj->abij", (4, 4), (9, 9, 4, 4)),
+    ("dckl,dakl->ac", (9, 9, 4, 4), (9, 9, 4, 4)),
+    ("cdlk,cdli->ki", (9, 9, 4, 4), (9, 9, 4, 4)),
+    ("ij,jk->ik", (5, 4000), (4000, 3)),       # tall K: split-K path
+]
+
+
+@pytest.mark.parametrize("spec,sa,sb", SPECS)
+def test_contract_matches_einsum(spec, sa, sb):
+    rng = np.random.default_rng(hash(spec) % 2 ** 32)
+    A, B = rng.standard_normal(sa), rng.standard_normal(sb)
+    ref = np.einsum(spec, A, B)
+    C0 = rng.standard_normal(ref.shape)
+    out = dev(C0)
+    ops.contract(spec, dev(A), dev(B), out, alpha=0.7, beta=0.3)
+    np.testing.assert_allclose(host(out), 0.7 * ref + 0.3 * C0, rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("M,N,K,batch", [(1, 1, 1, 1), (130, 257, 65, 2), (128, 128, 16, 1),
+                                         (7, 300, 1000, 3)])
+@pytest.mark.parametrize("ak,bn", [(1, 1), (1, 0), (0, 1), (0, 0)])
+def test_gemm_f64_layouts(M, N, K, batch, ak, bn):
+    rng = np.random.default_rng(M * 7 + N + K)
+    A = rng.standard_normal((batch, M, K) if ak else (batch, K, M))
+    B = rng.standard_normal((batch, K, N) if bn else (batch, N, K))
+    Am = A if ak else A.transpose(0, 2, 1)
+    Bm = B if bn else B.transpose(0, 2, 1)
+    ref = Am @ Bm
+    C = dev(np.zeros((batch, M, N)))
+    Ad, Bd = dev(A), dev(B)      # keep the device buffers alive across the launch
+    for splits in (1, 3):
+        ws = torch.empty(batch * splits * M * N, dtype=torch.float64, device="cuda")
+        _lib.call("nl_gemm_f64", ak, bn, M, N, K, ctypes.c_double(1.0), ops._p(Ad),
+                  K if ak else M, A[0].size, ops._p(Bd), N if bn else K, B[0].size,
+                  ctypes.c_double(0.0), ops._p(C), N, M * N, batch, splits, ops._p(ws),
+                  ops._stream())
+        np.testing.assert_allclose(host(C), ref, rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("dt", [0, 1])
+def test_gemm_lp_exact_on_small_integers(dt):
+    """BF16/FP16 inputs that are small integers are exact; FP32 accumulation of products
+    below 2^24 is exact, so the MFMA operand/result lane layout is checked bit for bit."""
+    rng = np.random.default_rng(5 + dt)
+    M, N, K = 150, 70, 200
+    A = rng.integers(-4, 5, (M, K)).astype(np.float64)
+    B = rng.integers(-4, 5, (N, K)).astype(np.float64)
+    out = dev(np.zeros((M, N)))
+    ops.contract("mk,nk->mn", dev(A), dev(B), out, beta=0.0, precision=("bf16", "fp16")[dt])
+    np.testing.assert_array_equal(host(out), A @ B.T)
+
+
+@pytest.mark.parametrize("prec,tol", [("bf16", 2e-2), ("fp16", 4e-3)])
+def test_gemm_lp_scaled_accuracy(prec, tol):
+    """Per-row scaling keeps tiny-magnitude rows (amplitude-like values ~1e-6) accurate."""
+    rng = np.random.default_rng(9)
+    A = rng.standard_normal((96, 320)) * np.logspace(-8, 0, 96)[:, None]
+    B = rng.standard_normal((320, 80)) * 1e-3
+    out = dev(np.zeros((96, 80)))
+    ops.contract("mk,kn->mn", dev(A), dev(B), out, beta=0.0, precision=prec)
+    ref = A @ B
+    rel = np.abs(host(out) - ref).max(1) / np.abs(ref).max(1)
+    assert rel.max() < tol
+
+
+def test_permute_tau_segsum():
+    rng = np.random.default_rng(1)
+    x = rng.standard_normal((3, 4, 5, 6))
+    out = torch.empty((5, 3, 6, 4), dtype=torch.float64, device="cuda")
+    ops.permute_into(dev(x).permute(2, 0, 3, 1), out, alpha=2.0)
+    np.testing.assert_array_equal(host(out), 2.0 * x.transpose(2, 0, 3, 1))
+    p, h = 5, 3
+    t1, t2 = rng.standard_normal((p, h)), rng.standard_normal((p, p, h, h))
+    tau = torch.empty((p, p, h, h), dtype=torch.float64, device="cuda")
+    t1d, t2d = dev(t1), dev(t2)
+    _lib.call("nl_tau", p, h, ops._p(t1d), ops._p(t2d), ops._p(tau), ops._stream())
+    ref = t2 + np.einsum("ai,bj->abij", t1, t1) - np.einsum("aj,bi->abij", t1, t1)
+    np.testing.assert_allclose(host(tau), ref, rtol=1e-14, atol=1e-14)
+    # segsum: out[dst[s] + x*2] += 0.5 * sum val[e] * src[so[e] + x]
+    src = rng.standard_normal(50)
+    ptr = np.array([0, 2, 2, 5]); so = np.array([0, 10, 3, 7, 20]); val = rng.standard_normal(5)
+    dst = np.array([0, 1, 30])
+    o = np.zeros(40)
+    oref = o.copy()
+    for s in range(3):
+        for x in range(4):
+            oref[dst[s] + 2 * x] += 0.5 * sum(val[e] * src[so[e] + x] for e in range(ptr[s],
+                                                                                    ptr[s + 1]))
+    od = dev(o)
+    L = lambda a: torch.tensor(a, dtype=torch.int64, device="cuda")
+    keep = [L(ptr), L(so), dev(val), L(dst), dev(src)]   # alive until after the launch
+    _lib.call("nl_segsum", 3, 4, ops._p(keep[0]), ops._p(keep[1]), ops._p(keep[2]),
+              ops._p(keep[3]), 1, 2, ctypes.c_double(0.5), ops._p(keep[4]), ops._p(od),
+              ops._stream())
+    torch.cuda.synchronize()
+    np.testing.assert_allclose(host(od), oref, rtol=1e-14, atol=1e-14)
+
+
+def test_error_codes():
+    """Bad arguments come back as -1, never as a fault."""
+    lib = _lib.load()
+    assert lib.nl_gemm_f64(1, 1, -1, 1, 1, 1.0, None, 1, 0, None, 1, 0, 0.0, None, 1, 0, 1, 1,
+                           None, None) == -1
+    assert lib.nl_gemm_lp(0, 8, 8, 7, 1.0, None, 8, 0, None, 0, None, 8, 0, None, 0, 0.0, None,
+                          8, 0, 1, None) == -1
