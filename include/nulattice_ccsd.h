@@ -30,16 +30,28 @@ int nl_gemm_f64(int a_kcontig, int b_ncontig, int64_t M, int64_t N, int64_t K, d
                 int64_t strideB, double beta, double* C, int64_t ldc, int64_t strideC,
                 int64_t batch, int64_t splits, double* ws, void* stream);
 
-/* C[b] (FP64) = alpha * diag(ia) (A[b] B[b]^T) diag(ib) + beta * C[b] with BF16 (dtype 0) or
+/* General strided FP64 tensor contraction on the same MFMA tiles: every index group is at most
+ * two runs with arbitrary element strides,
+ *   m = m1*M2+m2, n = n1*N2+n2, k = k1*K2+k2, batch = b1*B2+b2,
+ *   C(m,n) = alpha * sum_k A(m,k) B(k,n) + beta * D(m,n)   (D has C's layout; may be C or null)
+ * dims = {M1,M2,N1,N2,K1,K2,B1,B2}; sa = A strides {m1,m2,k1,k2,b1,b2};
+ * sb = B strides {k1,k2,n1,n2,b1,b2}; sc = C/D strides {m1,m2,n1,n2,b1,b2}.
+ * This is how every dense einsum of the diagram library (ccDgrams.py:48-578) runs without an
+ * operand transpose.  splits>1 splits K (ws: batch*splits*M*N doubles). */
+int nl_contract_f64(const int64_t* dims, double alpha, const double* A, const int64_t* sa,
+                    const double* B, const int64_t* sb, double beta, const double* D, double* C,
+                    const int64_t* sc, int64_t splits, double* ws, void* stream);
+
+/* C[b] (FP64) = alpha * diag(ia) (A[b] B[b]^T) diag(ib) + beta * D[b] with BF16 (dtype 0) or
  * FP16 (dtype 1) operands, FP32 accumulation (MFMA 16x16x32).  The precision form of the ring
  * and ladder contractions (ccDgrams.py:240-303); no reference counterpart for the precision.
- * K, lda, ldb multiples of 8; A, B 16-byte aligned.  splits>1 splits K (FP64 partials in ws,
- * batch*splits*M*N doubles, then a fixed-order reduction). */
+ * K, lda, ldb multiples of 8; A, B 16-byte aligned.  D (addend, C's layout) may be C or null.
+ * splits>1 splits K (FP64 partials in ws, batch*splits*M*N doubles, fixed-order reduction). */
 int nl_gemm_lp(int dtype, int64_t M, int64_t N, int64_t K, double alpha, const void* A,
                int64_t lda, int64_t strideA, const float* ia, int64_t strideIa, const void* B,
                int64_t ldb, int64_t strideB, const float* ib, int64_t strideIb, double beta,
-               double* C, int64_t ldc, int64_t strideC, int64_t batch, int64_t splits,
-               double* ws, void* stream);
+               const double* D, double* C, int64_t ldc, int64_t strideC, int64_t batch,
+               int64_t splits, double* ws, void* stream);
 
 /* out (contiguous, out_shape) = alpha * in[sum_d idx_d * in_strides[d]] + beta * out.
  * Replaces the transposes inside pAB/pIJ (ccDgrams.py:193-213) and opt_einsum's operand
@@ -81,7 +93,7 @@ int nl_update_t1(int64_t p, int64_t h, const double* t1, const double* H1, const
                  const double* Xhh, double mixing, double* t1n, void* stream);
 
 /* Fused T2 assembly, Jacobi update, mixing and energy partials:
- *   H2  = S0 + pAB(Sab) + pIJ(Sij) + pIJ(pAB(Spp))   (Spp stored as [(a,i)][(b,j)])
+ *   H2  = S0 + pAB(Sab) + pIJ(Sij) + pIJ(pAB(Spp))   (Sij, Spp stored as [(a,i)][(b,j)])
  *   t2n = m*t2 + (1-m)*(t2 + H2 / (-(Xpp_aa + Xpp_bb + Xhh_ii + Xhh_jj)))
  *   part[blk] = partial of 1/4 v.t2n + 1/2 v t1n t1n; part[nblk+blk] = partial of |H2|^2
  * (coupled_cluster.py:536-545, :618, :265-286).  nblk = nl_partials(p*p*h*h). */

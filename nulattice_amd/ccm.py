@@ -178,8 +178,9 @@ def ccsd_solver(fock_mats, two_body_int, t1initial=None, eps=1e-8, maxSteps=1000
     ``sparse`` only says how v_pppp / v_ppph are given (COO lists or dense arrays); both are
     packed into the same block plans.  ``hamiltonian`` may pass a prebuilt engine.Hamiltonian.
     With a low ``precision`` the iteration first runs to |dE|/|E| < ``eps_lowprec`` (default
-    max(eps, 1e-7): the low-precision fixed point is only defined to about that level) and then,
-    if ``refine``, continues in FP64 to ``eps``.
+    max(eps, 1e-7)) or until |dE| stops decreasing (the low-precision fixed point is only defined
+    to the rounding level of the contractions) and then, if ``refine``, continues in FP64 to
+    ``eps``.
     """
     if diis not in ("legacy", "pulsed", "soa"):
         raise ValueError(f"diis must be 'legacy', 'pulsed' or 'soa', not {diis!r}")
@@ -209,6 +210,7 @@ def ccsd_solver(fock_mats, two_body_int, t1initial=None, eps=1e-8, maxSteps=1000
     prec = _PREC[precision]
     lp_phase = precision != "fp64"
     eps_lp = max(eps, 1e-7) if eps_lowprec is None else eps_lowprec
+    best_rel, since_best = math.inf, 0
     steps_lp = steps_fp64 = 0
     t_start = time.perf_counter()
     energy = prev
@@ -236,7 +238,15 @@ def ccsd_solver(fock_mats, two_body_int, t1initial=None, eps=1e-8, maxSteps=1000
         t2, t2n = t2n, t2
         rel = abs(energy - prev) / abs(energy) if energy != 0 else 0.0
         converged = (energy == 0 and prev == 0) or rel < eps
-        if lp_phase and refine and ((energy == 0 and prev == 0) or rel < eps_lp):
+        if lp_phase:
+            # the low-precision iteration settles into rounding jitter instead of converging:
+            # stop it at eps_lowprec or when |dE| has not halved for 6 steps
+            if rel < 0.5 * best_rel:
+                best_rel, since_best = rel, 0
+            else:
+                since_best += 1
+        if lp_phase and refine and ((energy == 0 and prev == 0) or rel < eps_lp
+                                    or since_best >= 6):
             # the low-precision fixed point is reached: continue in FP64 from here
             lp_phase, prec = False, _PREC["fp64"]
             if info is not None:

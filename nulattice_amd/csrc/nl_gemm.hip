@@ -36,19 +36,33 @@ __device__ inline TileCoord tile_of(int wg, int nwg, int tiles_m, int tiles_n) {
 }
 
 // ---------------------------------------------------------------------------------- FP64
+// Strided tensor contraction: every index group is up to two runs with arbitrary strides,
+//   m = m1*M2 + m2, n = n1*N2 + n2, k = k1*K2 + k2, batch = b1*B2 + b2,
+//   A(m,k) = A[m1*am1 + m2*am2 + k1*ak1 + k2*ak2 (+ b1*ab1 + b2*ab2)], B and C alike,
+// so a 4-index contraction maps onto the MFMA tiles without a transpose.
 constexpr int BK64 = 16;
 constexpr int PAD64 = 4;
 
+struct Idx2 { int64_t n2, s1, s2; };      // extent of the inner run, strides of outer/inner run
+
+__device__ inline int64_t off2(int64_t i, const Idx2& d) {
+  int64_t q = i / d.n2;
+  return q * d.s1 + (i - q * d.n2) * d.s2;
+}
+
 struct GemmF64Args {
-  int M, N, K;
+  int M, N, K, nbatch;
   double alpha, beta;
-  const double* A; int64_t lda, sA;
-  const double* B; int64_t ldb, sB;
-  double* C; int64_t ldc, sC;
+  const double* A; Idx2 am, ak, ab;
+  const double* B; Idx2 bk, bn, bb;
+  double* C; Idx2 cm, cn, cb;
+  const double* D;                          // addend (same layout as C); may equal C or be null
   int tiles_m, tiles_n, splits, kchunk;
   double* ws;  // split-K partials [batch][splits][M][N] (only when splits > 1)
 };
 
+// AK: a thread loads 8 consecutive k of one row (A unit-stride in k); else 8 consecutive m.
+// BNC: a thread loads 8 consecutive n of one k-row (B unit-stride in n); else 8 consecutive k.
 template <bool AK, bool BNC>
 __global__ __launch_bounds__(256) void gemm_f64_kernel(GemmF64Args g) {
   __shared__ double As[BK64][TM + PAD64];
@@ -58,8 +72,8 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(GemmF64Args g) {
   const int batch = blockIdx.z, split = blockIdx.y;
   TileCoord tc = tile_of(blockIdx.x, gridDim.x, g.tiles_m, g.tiles_n);
   const int m0 = tc.tm * TM, n0 = tc.tn * TN;
-  const double* A = g.A + (int64_t)batch * g.sA;
-  const double* B = g.B + (int64_t)batch * g.sB;
+  const double* A = g.A + off2(batch, g.ab);
+  const double* B = g.B + off2(batch, g.bb);
   const int kbeg = split * g.kchunk;
   const int kend = min(g.K, kbeg + g.kchunk);
 
@@ -69,44 +83,52 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(GemmF64Args g) {
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
 
+  // per-thread fixed row (AK) / column (BNC) offsets
+  const int ar = AK ? (tid >> 1) : 0;
+  const int bc = BNC ? 0 : (tid >> 1);
+  const int64_t a_row = (AK && m0 + ar < g.M) ? off2(m0 + ar, g.am) : 0;
+  const int64_t b_col = (!BNC && n0 + bc < g.N) ? off2(n0 + bc, g.bn) : 0;
+
   double ra[8], rb[8];
   auto load = [&](int k0) {
-    if (AK) {  // A[m*lda + k]
-      int r = tid >> 1, kk = (tid & 1) * 8, m = m0 + r;
+    if (AK) {
+      int kk = (tid & 1) * 8, m = m0 + ar;
 #pragma unroll
       for (int e = 0; e < 8; ++e) {
         int k = k0 + kk + e;
-        ra[e] = (m < g.M && k < kend) ? A[(int64_t)m * g.lda + k] : 0.0;
+        ra[e] = (m < g.M && k < kend) ? A[a_row + off2(k, g.ak)] : 0.0;
       }
-    } else {   // A[k*lda + m]
+    } else {
       int kk = tid >> 4, r = (tid & 15) * 8, k = k0 + kk;
+      int64_t ko = (k < kend) ? off2(k, g.ak) : 0;
 #pragma unroll
       for (int e = 0; e < 8; ++e) {
         int m = m0 + r + e;
-        ra[e] = (m < g.M && k < kend) ? A[(int64_t)k * g.lda + m] : 0.0;
+        ra[e] = (m < g.M && k < kend) ? A[ko + off2(m, g.am)] : 0.0;
       }
     }
-    if (BNC) {  // B[k*ldb + n]
+    if (BNC) {
       int kk = tid >> 4, c = (tid & 15) * 8, k = k0 + kk;
+      int64_t ko = (k < kend) ? off2(k, g.bk) : 0;
 #pragma unroll
       for (int e = 0; e < 8; ++e) {
         int n = n0 + c + e;
-        rb[e] = (n < g.N && k < kend) ? B[(int64_t)k * g.ldb + n] : 0.0;
+        rb[e] = (n < g.N && k < kend) ? B[ko + off2(n, g.bn)] : 0.0;
       }
-    } else {    // B[n*ldb + k]
-      int c = tid >> 1, kk = (tid & 1) * 8, n = n0 + c;
+    } else {
+      int kk = (tid & 1) * 8, n = n0 + bc;
 #pragma unroll
       for (int e = 0; e < 8; ++e) {
         int k = k0 + kk + e;
-        rb[e] = (n < g.N && k < kend) ? B[(int64_t)n * g.ldb + k] : 0.0;
+        rb[e] = (n < g.N && k < kend) ? B[b_col + off2(k, g.bk)] : 0.0;
       }
     }
   };
   auto store = [&]() {
     if (AK) {
-      int r = tid >> 1, kk = (tid & 1) * 8;
+      int kk = (tid & 1) * 8;
 #pragma unroll
-      for (int e = 0; e < 8; ++e) As[kk + e][r] = ra[e];
+      for (int e = 0; e < 8; ++e) As[kk + e][ar] = ra[e];
     } else {
       int kk = tid >> 4, r = (tid & 15) * 8;
 #pragma unroll
@@ -117,9 +139,9 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(GemmF64Args g) {
 #pragma unroll
       for (int e = 0; e < 8; ++e) Bs[kk][c + e] = rb[e];
     } else {
-      int c = tid >> 1, kk = (tid & 1) * 8;
+      int kk = (tid & 1) * 8;
 #pragma unroll
-      for (int e = 0; e < 8; ++e) Bs[kk + e][c] = rb[e];
+      for (int e = 0; e < 8; ++e) Bs[kk + e][bc] = rb[e];
     }
   };
 
@@ -152,6 +174,7 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(GemmF64Args g) {
     }
   }
 
+  // f64 16x16x4 accumulator layout: register r of lane l holds D[4r + l/16][l%16]
   if (g.splits > 1) {
     double* W = g.ws + ((int64_t)batch * g.splits + split) * (int64_t)g.M * g.N;
 #pragma unroll
@@ -160,31 +183,60 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(GemmF64Args g) {
       for (int j = 0; j < 4; ++j)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          int row = m0 + wm * 64 + i * 16 + r * 4 + (lane >> 4);  // f64 16x16x4: D[4r + l/16][l%16]
+          int row = m0 + wm * 64 + i * 16 + r * 4 + (lane >> 4);
           int col = n0 + wn * 64 + j * 16 + (lane & 15);
           if (row < g.M && col < g.N) W[(int64_t)row * g.N + col] = g.alpha * acc[i][j][r];
         }
     return;
   }
-  double* C = g.C + (int64_t)batch * g.sC;
+  const int64_t cb = off2(batch, g.cb);
+  int64_t coff[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    int col = n0 + wn * 64 + j * 16 + (lane & 15);
+    coff[j] = col < g.N ? off2(col, g.cn) : 0;
+  }
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
+    for (int r = 0; r < 4; ++r) {
+      int row = m0 + wm * 64 + i * 16 + r * 4 + (lane >> 4);
+      if (row >= g.M) continue;
+      const int64_t ro = cb + off2(row, g.cm);
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        int row = m0 + wm * 64 + i * 16 + r * 4 + (lane >> 4);
+      for (int j = 0; j < 4; ++j) {
         int col = n0 + wn * 64 + j * 16 + (lane & 15);
-        if (row < g.M && col < g.N) {
-          double* p = C + (int64_t)row * g.ldc + col;
+        if (col < g.N) {
           double v = g.alpha * acc[i][j][r];
-          *p = (g.beta != 0.0) ? v + g.beta * *p : v;
+          if (g.D && g.beta != 0.0) v += g.beta * g.D[ro + coff[j]];
+          g.C[ro + coff[j]] = v;
         }
       }
+    }
+}
+
+// C[b](m,n) = sum_s ws[b][s][m][n] + beta * D[b](m,n)
+__global__ void splitk_reduce_strided_kernel(int M, int N, int splits, int nbatch,
+                                             const double* ws, double beta, const double* D,
+                                             double* C, Idx2 cm, Idx2 cn, Idx2 cb) {
+  int64_t total = (int64_t)nbatch * M * N;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    int64_t b = t / ((int64_t)M * N);
+    int64_t mn = t - b * (int64_t)M * N;
+    int64_t m = mn / N, n = mn - m * N;
+    const double* W = ws + b * splits * (int64_t)M * N + mn;
+    double s = 0.0;
+    for (int q = 0; q < splits; ++q) s += W[(int64_t)q * M * N];
+    int64_t o = off2(b, cb) + off2(m, cm) + off2(n, cn);
+    if (D && beta != 0.0) s += beta * D[o];
+    C[o] = s;
+  }
 }
 
 __global__ void splitk_reduce_kernel(int M, int N, int splits, int batch, const double* ws,
-                                     double beta, double* C, int64_t ldc, int64_t sC) {
+                                     double beta, const double* D, double* C, int64_t ldc,
+                                     int64_t sC) {
   int64_t total = (int64_t)batch * M * N;
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
        t += (int64_t)gridDim.x * blockDim.x) {
@@ -194,8 +246,9 @@ __global__ void splitk_reduce_kernel(int M, int N, int splits, int batch, const 
     const double* W = ws + b * splits * (int64_t)M * N + mn;
     double s = 0.0;
     for (int q = 0; q < splits; ++q) s += W[(int64_t)q * M * N];
-    double* p = C + b * sC + m * ldc + n;
-    *p = (beta != 0.0) ? s + beta * *p : s;
+    int64_t o = b * sC + m * ldc + n;
+    if (D && beta != 0.0) s += beta * D[o];
+    C[o] = s;
   }
 }
 
@@ -211,6 +264,7 @@ struct GemmLPArgs {
   const float* ia; int64_t sia;         // per-row inverse scale of A (or null)
   const float* ib; int64_t sib;         // per-row inverse scale of B (or null)
   double* C; int64_t ldc, sC;
+  const double* D;                      // addend with C's layout (may be C or null)
   int tiles_m, tiles_n, splits, kchunk;
   double* ws;                           // split-K partials [batch][splits][M][N]
 };
@@ -298,6 +352,7 @@ __global__ __launch_bounds__(256) void gemm_lp_kernel(GemmLPArgs g) {
   const bool partial = g.splits > 1;
   double* C = partial ? g.ws + ((int64_t)batch * g.splits + split) * (int64_t)g.M * g.N
                       : g.C + (int64_t)batch * g.sC;
+  const double* Dp = (partial || !g.D) ? nullptr : g.D + (int64_t)batch * g.sC;
   const int64_t ldc = partial ? g.N : g.ldc;
   const double beta = partial ? 0.0 : g.beta;
   const float* ia = g.ia ? g.ia + (int64_t)batch * g.sia : nullptr;
@@ -314,8 +369,9 @@ __global__ __launch_bounds__(256) void gemm_lp_kernel(GemmLPArgs g) {
         int col = n0 + wn * 64 + j * 16 + (lane & 15);
         if (col < g.N) {
           double v = sr * (double)acc[i][j][r] * (ib ? (double)ib[col] : 1.0);
-          double* p = C + (int64_t)row * ldc + col;
-          *p = (beta != 0.0) ? v + beta * *p : v;
+          const int64_t o = (int64_t)row * ldc + col;
+          if (Dp && beta != 0.0) v += beta * Dp[o];
+          C[o] = v;
         }
       }
     }
@@ -325,6 +381,60 @@ __global__ __launch_bounds__(256) void gemm_lp_kernel(GemmLPArgs g) {
 
 extern "C" {
 
+static inline Idx2 mk(int64_t n2, int64_t s1, int64_t s2) {
+  Idx2 d; d.n2 = n2 > 0 ? n2 : 1; d.s1 = s1; d.s2 = s2; return d;
+}
+
+// General strided FP64 contraction (see the kernel comment).  dims[8] = {M1, M2, N1, N2, K1,
+// K2, B1, B2}; sa[6] = A strides {m1, m2, k1, k2, b1, b2}; sb[6] = B {k1, k2, n1, n2, b1, b2};
+// sc[6] = C (and D) {m1, m2, n1, n2, b1, b2}.  C = alpha*A*B + beta*D (D may be C or null).
+int nl_contract_f64(const int64_t* dims, double alpha, const double* A, const int64_t* sa,
+                    const double* B, const int64_t* sb, double beta, const double* D, double* C,
+                    const int64_t* sc, int64_t splits, double* ws, void* stream) {
+  if (!dims || !sa || !sb || !sc) return NL_EARG;
+  for (int d = 0; d < 8; ++d)
+    if (dims[d] < 0) return NL_EARG;
+  const int64_t M = dims[0] * dims[1], N = dims[2] * dims[3], K = dims[4] * dims[5];
+  const int64_t nb = dims[6] * dims[7];
+  if (M == 0 || N == 0 || nb == 0) return NL_OK;
+  if (!A || !B || !C || splits < 1 || (splits > 1 && !ws)) return NL_EARG;
+  if (M > INT32_MAX || N > INT32_MAX || K > INT32_MAX || nb > 65535 || splits > 65535)
+    return NL_EARG;
+  hipStream_t st = (hipStream_t)stream;
+  GemmF64Args g;
+  g.M = (int)M; g.N = (int)N; g.K = (int)K; g.nbatch = (int)nb; g.alpha = alpha;
+  g.beta = beta; g.D = D;
+  g.A = A; g.am = mk(dims[1], sa[0], sa[1]); g.ak = mk(dims[5], sa[2], sa[3]);
+  g.ab = mk(dims[7], sa[4], sa[5]);
+  g.B = B; g.bk = mk(dims[5], sb[0], sb[1]); g.bn = mk(dims[3], sb[2], sb[3]);
+  g.bb = mk(dims[7], sb[4], sb[5]);
+  g.C = C; g.cm = mk(dims[1], sc[0], sc[1]); g.cn = mk(dims[3], sc[2], sc[3]);
+  g.cb = mk(dims[7], sc[4], sc[5]);
+  g.tiles_m = (int)nl_cdiv(M, TM); g.tiles_n = (int)nl_cdiv(N, TN);
+  int64_t kchunk = nl_cdiv(nl_cdiv(K > 0 ? K : 1, splits), BK64) * BK64;
+  g.splits = K == 0 ? 1 : (int)splits; g.kchunk = (int)kchunk; g.ws = ws;
+  // thread mapping: unit-stride inner k run of A -> load along k; of B's n -> load along n
+  const bool ak = (dims[5] > 1 ? sa[3] == 1 : dims[4] > 1 ? sa[2] == 1 : true) ||
+                  !(dims[1] > 1 ? sa[1] == 1 : dims[0] > 1 ? sa[0] == 1 : false);
+  const bool bn = (dims[3] > 1 ? sb[3] == 1 : dims[2] > 1 ? sb[2] == 1 : true) ||
+                  !(dims[5] > 1 ? sb[1] == 1 : dims[4] > 1 ? sb[0] == 1 : false);
+  dim3 grid((unsigned)(g.tiles_m * g.tiles_n), (unsigned)g.splits, (unsigned)nb);
+  if (ak && bn)   hipLaunchKernelGGL((gemm_f64_kernel<true, true>), grid, dim3(256), 0, st, g);
+  else if (ak)    hipLaunchKernelGGL((gemm_f64_kernel<true, false>), grid, dim3(256), 0, st, g);
+  else if (bn)    hipLaunchKernelGGL((gemm_f64_kernel<false, true>), grid, dim3(256), 0, st, g);
+  else            hipLaunchKernelGGL((gemm_f64_kernel<false, false>), grid, dim3(256), 0, st, g);
+  NL_LAUNCH_CHECK();
+  if (g.splits > 1) {
+    int64_t total = nb * M * N;
+    int blocks = (int)std::min<int64_t>(nl_cdiv(total, 256), 8192);
+    hipLaunchKernelGGL(splitk_reduce_strided_kernel, dim3(blocks), dim3(256), 0, st, (int)M,
+                       (int)N, g.splits, (int)nb, (const double*)ws, beta, D, C, g.cm, g.cn,
+                       g.cb);
+    NL_LAUNCH_CHECK();
+  }
+  return NL_OK;
+}
+
 // C[b] = alpha * op(A[b]) * op(B[b]) + beta * C[b], row-major C (ldc), FP64.
 //   a_kcontig = 1: A(m,k) = A[m*lda + k];  0: A(m,k) = A[k*lda + m]
 //   b_ncontig = 1: B(k,n) = B[k*ldb + n];  0: B(k,n) = B[n*ldb + k]
@@ -333,36 +443,12 @@ int nl_gemm_f64(int a_kcontig, int b_ncontig, int64_t M, int64_t N, int64_t K, d
                 const double* A, int64_t lda, int64_t strideA, const double* B, int64_t ldb,
                 int64_t strideB, double beta, double* C, int64_t ldc, int64_t strideC,
                 int64_t batch, int64_t splits, double* ws, void* stream) {
-  if (M < 0 || N < 0 || K < 0 || batch < 0 || splits < 1) return NL_EARG;
-  if (M == 0 || N == 0 || batch == 0) return NL_OK;
-  if (!A || !B || !C || (splits > 1 && !ws)) return NL_EARG;
-  if (M > INT32_MAX || N > INT32_MAX || K > INT32_MAX || batch > 65535 || splits > 65535)
-    return NL_EARG;
-  hipStream_t st = (hipStream_t)stream;
-  GemmF64Args g;
-  g.M = (int)M; g.N = (int)N; g.K = (int)K; g.alpha = alpha; g.beta = beta;
-  g.A = A; g.lda = lda; g.sA = strideA; g.B = B; g.ldb = ldb; g.sB = strideB;
-  g.C = C; g.ldc = ldc; g.sC = strideC;
-  g.tiles_m = (int)nl_cdiv(M, TM); g.tiles_n = (int)nl_cdiv(N, TN);
-  int64_t kchunk = nl_cdiv(nl_cdiv(K > 0 ? K : 1, splits), BK64) * BK64;
-  g.splits = (int)splits; g.kchunk = (int)kchunk; g.ws = ws;
-  if (K == 0) {  // C = beta * C
-    g.splits = 1;
-  }
-  dim3 grid((unsigned)(g.tiles_m * g.tiles_n), (unsigned)g.splits, (unsigned)batch);
-  if (a_kcontig && b_ncontig) hipLaunchKernelGGL((gemm_f64_kernel<true, true>), grid, dim3(256), 0, st, g);
-  else if (a_kcontig)         hipLaunchKernelGGL((gemm_f64_kernel<true, false>), grid, dim3(256), 0, st, g);
-  else if (b_ncontig)         hipLaunchKernelGGL((gemm_f64_kernel<false, true>), grid, dim3(256), 0, st, g);
-  else                        hipLaunchKernelGGL((gemm_f64_kernel<false, false>), grid, dim3(256), 0, st, g);
-  NL_LAUNCH_CHECK();
-  if (g.splits > 1) {
-    int64_t total = batch * M * N;
-    int blocks = (int)std::min<int64_t>(nl_cdiv(total, 256), 8192);
-    hipLaunchKernelGGL(splitk_reduce_kernel, dim3(blocks), dim3(256), 0, st, (int)M, (int)N,
-                       g.splits, (int)batch, (const double*)ws, beta, C, ldc, strideC);
-    NL_LAUNCH_CHECK();
-  }
-  return NL_OK;
+  if (M < 0 || N < 0 || K < 0 || batch < 0) return NL_EARG;
+  const int64_t dims[8] = {1, M, 1, N, 1, K, 1, batch};
+  const int64_t sa[6] = {0, a_kcontig ? lda : 1, 0, a_kcontig ? 1 : lda, 0, strideA};
+  const int64_t sb[6] = {0, b_ncontig ? ldb : 1, 0, b_ncontig ? 1 : ldb, 0, strideB};
+  const int64_t sc[6] = {0, ldc, 0, 1, 0, strideC};
+  return nl_contract_f64(dims, alpha, A, sa, B, sb, beta, C, C, sc, splits, ws, stream);
 }
 
 // C[b] (FP64, row-major) = alpha * diag(ia) * (A[b] * B[b]^T) * diag(ib) + beta * C[b]
@@ -371,8 +457,8 @@ int nl_gemm_f64(int a_kcontig, int b_ncontig, int64_t M, int64_t N, int64_t K, d
 int nl_gemm_lp(int dtype, int64_t M, int64_t N, int64_t K, double alpha, const void* A,
                int64_t lda, int64_t strideA, const float* ia, int64_t strideIa, const void* B,
                int64_t ldb, int64_t strideB, const float* ib, int64_t strideIb, double beta,
-               double* C, int64_t ldc, int64_t strideC, int64_t batch, int64_t splits,
-               double* ws, void* stream) {
+               const double* D, double* C, int64_t ldc, int64_t strideC, int64_t batch,
+               int64_t splits, double* ws, void* stream) {
   if (M < 0 || N < 0 || K < 0 || batch < 0 || (dtype != 0 && dtype != 1)) return NL_EARG;
   if (M == 0 || N == 0 || batch == 0) return NL_OK;
   if (!A || !B || !C) return NL_EARG;
@@ -386,7 +472,7 @@ int nl_gemm_lp(int dtype, int64_t M, int64_t N, int64_t K, double alpha, const v
   g.A = (const uint16_t*)A; g.lda = lda; g.sA = strideA;
   g.B = (const uint16_t*)B; g.ldb = ldb; g.sB = strideB;
   g.ia = ia; g.sia = strideIa; g.ib = ib; g.sib = strideIb;
-  g.C = C; g.ldc = ldc; g.sC = strideC;
+  g.C = C; g.ldc = ldc; g.sC = strideC; g.D = D;
   g.tiles_m = (int)nl_cdiv(M, TM); g.tiles_n = (int)nl_cdiv(N, TN);
   g.splits = (int)splits; g.ws = ws;
   g.kchunk = (int)(nl_cdiv(nl_cdiv(K, splits), BKL) * BKL);   // multiple of 64 (hence of 8)
@@ -404,7 +490,7 @@ int nl_gemm_lp(int dtype, int64_t M, int64_t N, int64_t K, double alpha, const v
     int64_t total = batch * M * N;
     int blocks = (int)std::min<int64_t>(nl_cdiv(total, 256), 8192);
     hipLaunchKernelGGL(splitk_reduce_kernel, dim3(blocks), dim3(256), 0, st, (int)M, (int)N,
-                       (int)splits, (int)batch, (const double*)ws, beta, C, ldc, strideC);
+                       (int)splits, (int)batch, (const double*)ws, beta, D, C, ldc, strideC);
     NL_LAUNCH_CHECK();
   }
   return NL_OK;

@@ -260,7 +260,7 @@ __global__ void update_t1_kernel(int64_t p, int64_t h, const double* t1, const d
 
 struct AssembleArgs {
   int64_t p, h;
-  const double *S0, *Sab, *Sij, *Spp;   // Spp in (a,i),(b,j) layout; any may be null
+  const double *S0, *Sab, *Sij, *Spp;   // Sij, Spp in (a,i),(b,j) layout; any may be null
   const double *Xpp, *Xhh;              // dressed diagonals (full p x p, h x h matrices)
   const double *t2, *vpphh, *t1n;       // old t2, v_pphh, new t1 (for the fused energy)
   double mixing;
@@ -279,9 +279,9 @@ __global__ __launch_bounds__(NT) void assemble_t2_kernel(AssembleArgs g) {
     int64_t b = r % p, a = r / p;
     double H = g.S0 ? g.S0[t] : 0.0;
     if (g.Sab) H += g.Sab[t] - g.Sab[((b * p + a) * h + i) * h + j];
-    if (g.Sij) H += g.Sij[t] - g.Sij[((a * p + b) * h + j) * h + i];
+    const int64_t ph = p * h;
+    if (g.Sij) H += g.Sij[(a * h + i) * ph + b * h + j] - g.Sij[(a * h + j) * ph + b * h + i];
     if (g.Spp) {
-      const int64_t ph = p * h;
       H += g.Spp[(a * h + i) * ph + b * h + j] - g.Spp[(b * h + i) * ph + a * h + j]
          - g.Spp[(a * h + j) * ph + b * h + i] + g.Spp[(b * h + j) * ph + a * h + i];
     }

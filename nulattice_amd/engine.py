@@ -108,7 +108,8 @@ class Buffers:
         self.q_hh = z(h, h)
         self.ret1 = z(p, p)
         self.tau = z(p, p, h, h)
-        self.S0, self.Sab, self.Sij = z(p, p, h, h), z(p, p, h, h), z(p, p, h, h)
+        self.S0, self.Sab = z(p, p, h, h), z(p, p, h, h)
+        self.Sij = z(p, h, p, h)     # P(ij) group as [a][i][b][j]
         self.Spp = z(p, h, p, h)
         self.Tr = z(p, h, p, h)      # t2 as [a][i][c][k]
         self.W = z(p, h, p, h)       # ring intermediate [c][k][b][j]
@@ -136,12 +137,14 @@ def t1_part(ham, t1, t2, B, prec=DEFAULT_PREC):
     """H1, X1pp, X1hh of t1Iter (coupled_cluster.py:372-408) into B.H1/B.X1pp/B.X1hh."""
     vpphh = ham.vpphh_view()
     H1, Xpp, Xhh = B.H1, B.X1pp, B.X1hh
+    permute_into(t2.permute(0, 2, 1, 3), B.Tr)       # t2 as [a][i][c][k], shared by both parts
+    t2r = B.Tr.permute(0, 2, 1, 3)                   # the same t2[a,c,i,k] through that layout
     permute_into(ham.f_ph, H1)
     contract("akci,ck->ai", ham.v_phph, t1, H1, alpha=-1.0)                  # A11
-    contract("ck,acik->ai", ham.f_ph, t2, H1)                                 # A12
+    contract("ck,acik->ai", ham.f_ph, t2r, H1)                                # A12
     contract("cikl,cakl->ai", ham.v_phhh, t2, H1, alpha=-0.5)                # A13
     contract("cdkl,ck->dl", vpphh, t1, B.s_ph, beta=0.0)                     # A14
-    contract("dl,dali->ai", B.s_ph, t2, H1)
+    contract("dl,dali->ai", B.s_ph, t2r, H1)
     pl = ham.ppph
     if pl.n:
         _segsum(pl.ret0, ham.h, t2, H1, -0.5)                                # ret0
@@ -187,39 +190,35 @@ def t2_part(ham, t1, t2, B, prec=DEFAULT_PREC):
     contract("dk,dj->kj", B.u_ph, t1, Xhh, alpha=-1.0)                      # A45
     # ---- S0: v_pphh + 1/2 tau.Whh + 1/2 pp ladder(tau)
     S0 = B.S0
-    permute_into(ham.v_pphh, S0)
     permute_into(ham.v_hhhh, B.Whh)
     contract("ci,cjkl->klij", t1, ham.v_phhh, B.zz, beta=0.0)
     permute_into(B.zz, B.Whh, 1.0, 1.0)
     permute_into(B.zz.permute(0, 1, 3, 2), B.Whh, -1.0, 1.0)
     contract("cdkl,cdij->klij", ham.v_pphh, tau, B.Whh, alpha=0.5, precision=prec["hh"])
-    contract("abkl,klij->abij", tau, B.Whh, S0, alpha=0.5, precision=prec["hh"])
+    contract("abkl,klij->abij", tau, B.Whh, S0, alpha=0.5, precision=prec["hh"],
+             addend=ham.v_pphh)
     _pp_ladder(ham, tau, S0, prec)
     # ---- Sab: P(ab) group
     Sab = B.Sab
     permute_into(ham.v_phhh, B.N56)
     if pl.n:
-        _segsum(pl.r56, h * h, tau, B.N56, 0.5)
+        _block_gemm(pl.r56, tau, B.N56, 0.5, h * h, "r56")
     contract("bkij,ak->abij", B.N56, t1, Sab, beta=0.0)                      # A23 + A39
     if pl.n:
         contract("ad,dbij->abij", B.ret1, t2, Sab)                           # A38
     contract("bc,acij->abij", Xpp, t2, Sab, precision=prec["dress"])         # X_pp t2
-    # ---- Sij: P(ij) group
-    Sij = B.Sij
-    Sij.zero_()
-    if pl.n:
-        _segsum(pl.ret2, h, t1, Sij, 1.0, 1, h)                              # A36
+    # ---- Sij: P(ij) group, stored as [a][i][b][j] (like Spp)
+    t2r = B.Tr.permute(0, 2, 1, 3)                                           # t2 via [a][i][c][k]
+    Sij = B.Sij.permute(0, 2, 1, 3)                                          # view Sij[a,b,i,j]
     contract("cikl,ck->li", ham.v_phhh, t1, B.q_hh, beta=0.0)                # A28
-    contract("li,ablj->abij", B.q_hh, t2, Sij, alpha=-1.0)
-    contract("kj,abik->abij", Xhh, t2, Sij, precision=prec["dress"])         # X_hh t2
+    contract("li,ablj->abij", B.q_hh, t2r, Sij, alpha=-1.0, beta=0.0)
+    contract("kj,abik->abij", Xhh, t2r, Sij, precision=prec["dress"])        # X_hh t2
+    if pl.n:
+        _segsum(pl.ret2, h, t1, B.Sij, 1.0, 1, p * h)                        # A36
     # ---- Spp: P(ij)P(ab) group in [(a,i)][(b,j)] layout
-    Tr = B.Tr
-    permute_into(t2.permute(0, 2, 1, 3), Tr)                                 # t2 as [a][i][c][k]
-    t2r = Tr.permute(0, 2, 1, 3)                                             # view t2[a,c,i,k]
     W = B.W
-    permute_into(ham.Wc, W)                                                  # A22
     contract("cdkl,dblj->ckbj", vpphh, t2r, W, alpha=0.5, precision=prec["ring"],
-             tag="ring_W")                                                   # A24
+             tag="ring_W", addend=ham.Wc)                                    # A24 + A22
     if pl.n:
         _segsum(pl.ret4, h, t1, W, 1.0)                                      # A37
     contract("cjkl,bl->ckbj", ham.v_phhh, t1, W, alpha=-1.0)                 # A29
@@ -233,19 +232,24 @@ def t2_part(ham, t1, t2, B, prec=DEFAULT_PREC):
 
 def _pp_ladder(ham, tau, S0, prec):
     """S0 += 1/2 sum_cd v_pppp[a,b,c,d] tau[c,d,i,j] over packed pair blocks."""
-    pb = ham.pppp
+    _block_gemm(ham.pppp, tau, S0, 0.5, ham.h * ham.h, "ladder")
+
+
+def _block_gemm(pb, src, out, alpha, ncol, tag):
+    """out[rows_b] += alpha * V_b src[cols_b] for every packed block b (row gather, batched MFMA
+    GEMM, row scatter; rows of src/out have ncol contiguous elements)."""
     if pb.nb == 0:
         return
-    hh = ham.h * ham.h
-    G = ops.WS.get("ladder_G", pb.nb * pb.kb * hh, device=tau.device)
-    R = ops.WS.get("ladder_R", pb.nb * pb.mb * hh, device=tau.device)
+    G = ops.WS.get(tag + "_G", pb.nb * pb.kb * ncol, device=src.device)
+    R = ops.WS.get(tag + "_R", pb.nb * pb.mb * ncol, device=src.device)
     st = ops._stream()
-    _lib.call("nl_gather_rows", pb.nb * pb.kb, hh, ops._p(pb.cols), ops._p(tau), hh, ops._p(G), st)
-    _lib.call("nl_gemm_f64", 1, 1, pb.mb, hh, pb.kb, ctypes.c_double(1.0), ops._p(pb.V), pb.kb,
-              pb.mb * pb.kb, ops._p(G), hh, pb.kb * hh, ctypes.c_double(0.0), ops._p(R), hh,
-              pb.mb * hh, pb.nb, 1, ctypes.c_void_p(0), st)
-    _lib.call("nl_scatter_add_rows", pb.nb * pb.mb, hh, ops._p(pb.rows), ops._p(R),
-              ctypes.c_double(0.5), ops._p(S0), hh, st)
+    _lib.call("nl_gather_rows", pb.nb * pb.kb, ncol, ops._p(pb.cols), ops._p(src), ncol,
+              ops._p(G), st)
+    _lib.call("nl_gemm_f64", 1, 1, pb.mb, ncol, pb.kb, ctypes.c_double(1.0), ops._p(pb.V), pb.kb,
+              pb.mb * pb.kb, ops._p(G), ncol, pb.kb * ncol, ctypes.c_double(0.0), ops._p(R),
+              ncol, pb.mb * ncol, pb.nb, 1, ctypes.c_void_p(0), st)
+    _lib.call("nl_scatter_add_rows", pb.nb * pb.mb, ncol, ops._p(pb.rows), ops._p(R),
+              ctypes.c_double(alpha), ops._p(out), ncol, st)
 
 
 def update(ham, t1, t2, B, t1n, t2n, mixing, prec=DEFAULT_PREC, H2out=None):

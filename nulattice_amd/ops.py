@@ -8,6 +8,7 @@ the permute kernel; the result is written straight into the destination when its
 else through a scratch tensor and a permute-accumulate.
 """
 import ctypes
+import itertools
 import math
 
 import torch
@@ -47,6 +48,10 @@ class Workspace:
 
 
 WS = Workspace()
+
+# Low-precision MFMA is used only when both output dimensions are at least this large; thinner
+# products (the h x h dressings) are bandwidth-bound and stay FP64.
+LP_MIN_DIM = 256
 
 # Optional kernel timers: when TIMERS["on"] is set, a contract() call with tag=T records a pair
 # of events on the launch stream around its GEMM launch into TIMERS[T].
@@ -111,19 +116,12 @@ def _parse(spec):
     return a, b, out
 
 
-def _merge(t, dims):
-    """(size, stride) of dims merged into one index, or None if the strides do not allow it.
-    Empty group -> (1, 0)."""
-    dims = [d for d in dims if t.shape[d] != 1]
-    if not dims:
-        return 1, 0
-    size = 1
-    for x, y in zip(dims[:-1], dims[1:]):
-        if t.stride(x) != t.stride(y) * t.shape[y]:
-            return None
-    for d in dims:
-        size *= t.shape[d]
-    return size, t.stride(dims[-1])
+def _splits(M, N, K, batch):
+    """Split-K factor so that a launch has enough workgroups (deterministic reduction)."""
+    tiles = math.ceil(M / 128) * math.ceil(N / 128) * batch
+    if tiles >= 512 or K <= 512:
+        return 1
+    return int(max(1, min(256, 1024 // tiles, K // 256)))
 
 
 def _order(t, labels, group):
@@ -131,91 +129,113 @@ def _order(t, labels, group):
     return sorted(group, key=lambda c: (-t.stride(labels.index(c)), labels.index(c)))
 
 
-def _layout(t, labels, outer, g1, g2):
-    """Try t as [outer][g1][g2] with g2 unit-stride. Returns (s_outer, s_g1) or None."""
-    pos = lambda g: [labels.index(c) for c in g]
-    mo, m1, m2 = _merge(t, pos(outer)), _merge(t, pos(g1)), _merge(t, pos(g2))
-    if mo is None or m1 is None or m2 is None:
+def _splits_at(t, labels, order):
+    """Positions i (0 < i < len) where order[i-1], order[i] cannot merge in t."""
+    out = set()
+    for i in range(1, len(order)):
+        x, y = labels.index(order[i - 1]), labels.index(order[i])
+        if t.shape[x] == 1 or t.shape[y] == 1:
+            continue
+        if t.stride(x) != t.stride(y) * t.shape[y]:
+            out.add(i)
+    return out
+
+
+def _group_runs(order, tensors, size):
+    """Split a group into <= 2 runs valid for every tensor.  tensors: [(t, labels)] holding all
+    of the group's labels, or (None, None) for a tensor the group broadcasts over.
+    Returns (n1, n2, [(s1, s2) per tensor]) or None."""
+    cuts = set()
+    for t, lab in tensors:
+        if t is not None:
+            cuts |= _splits_at(t, lab, order)
+    if len(cuts) > 1:
         return None
-    n2 = math.prod(t.shape[labels.index(c)] for c in g2)
-    if n2 > 1 and m2[1] != 1:
-        return None
-    return mo[1], m1[1]
+    cut = min(cuts) if cuts else 0
+    r1, r2 = order[:cut], order[cut:]
+    n1 = math.prod(size[c] for c in r1)
+    n2 = math.prod(size[c] for c in r2)
+    strides = []
+    for t, lab in tensors:
+        if t is None:
+            strides.append((0, 0))
+            continue
+        st = lambda run: t.stride(lab.index(run[-1])) if run else 0
+        strides.append((st(r1), st(r2)))
+    return n1, n2, strides
 
 
-def _as_layout(t, labels, order):
-    """Contiguous copy of t with its dims in ``order``."""
-    perm = [labels.index(c) for c in order]
-    return copy(t.permute(*perm))
-
-
-def _splits(M, N, K, batch):
-    tiles = math.ceil(M / 128) * math.ceil(N / 128) * batch
-    if tiles >= 512 or K <= 512:
-        return 1
-    return int(max(1, min(256, 1024 // tiles, K // 256)))
-
-
-def _operand(t, labels, bat, g1, g2):
-    """Strides of operand t as [bat][g1][g2] (g2 unit-stride) or [bat][g2][g1] (g1 unit-stride).
-
-    Returns (inner_is_g2, s_bat, ld) or None.  Batch labels absent from t get stride 0 (only a
-    single broadcast batch label is allowed)."""
-    present = [c for c in bat if c in labels]
-    if len(present) != len(bat) and len(bat) != 1:
-        return None
-    for first, second, flag in ((g1, g2, True), (g2, g1, False)):
-        lay = _layout(t, labels, present, first, second)
-        if lay is not None:
-            s_bat = lay[0] if present else 0
-            return flag, s_bat, lay[1]
-    return None
+def _best_runs(group, tensors, size):
+    """Try every order of a (small) group; prefer orders whose inner run is unit-stride in the
+    first tensor."""
+    best = None
+    for order in (itertools.permutations(group) if len(group) <= 4 else [tuple(group)]):
+        r = _group_runs(list(order), tensors, size)
+        if r is None:
+            continue
+        score = 0 if (tensors[0][0] is not None and r[2][0][1] == 1) else 1
+        if best is None or score < best[0]:
+            best = (score, list(order), r)
+            if score == 0:
+                break
+    return None if best is None else (best[1], best[2])
 
 
 def _plan(A, la, B, lb, out, lo, size):
-    """GEMM plan for out[lo] = sum A[la] B[lb]: groups, operand layouts and the bytes that would
-    have to be copied (operands that need a permute, an out that needs a scratch)."""
     bat = [c for c in lo if c in la and c in lb]
     Mi = [c for c in lo if c in la and c not in lb]
     Ni = [c for c in lo if c in lb and c not in la]
     Ki = [c for c in la if c in lb and c not in lo]
-    big, lbig = (A, la) if A.numel() >= B.numel() else (B, lb)
-    bat = _order(big, lbig, bat)
-    Ki = _order(big, lbig, Ki)
-    Mi = _order(A, la, Mi)
-    Ni = _order(B, lb, Ni)
-    opA = _operand(A, la, bat, Mi, Ki)
-    opB = _operand(B, lb, bat, Ki, Ni)
-    # an unmergeable outer M (or N) index becomes a broadcast batch index
-    if not bat and (opA is None or opB is None):
-        cand = None
-        if opA is None and len(Mi) > 1:
-            cand = ([Mi[0]], Mi[1:], Ni)
-        elif opB is None and len(Ni) > 1:
-            cand = ([Ni[0]], Mi, Ni[1:])
-        if cand is not None:
-            b2, M2, N2 = cand
-            oa, ob = _operand(A, la, b2, M2, Ki), _operand(B, lb, b2, Ki, N2)
-            if oa is not None and ob is not None and size[b2[0]] <= 65535:
-                bat, Mi, Ni, opA, opB = b2, M2, N2, oa, ob
-    nbat = math.prod(size[c] for c in bat)
-    outok = _layout(out, lo, bat, Mi, Ni) is not None and nbat <= 65535
-    cost = ((A.numel() if opA is None else 0) + (B.numel() if opB is None else 0)
-            + (0 if outok else 2 * out.numel()))
-    return dict(A=A, la=la, B=B, lb=lb, bat=bat, Mi=Mi, Ni=Ni, Ki=Ki, opA=opA, opB=opB,
-                outok=outok, cost=cost)
+    opt = [(bat, Mi, Ni)]
+    # an outer M (or N) index may become a broadcast batch index when it cannot merge
+    if not bat:
+        if len(Mi) > 1:
+            opt += [([c], [x for x in Mi if x != c], Ni) for c in Mi]
+        if len(Ni) > 1:
+            opt += [([c], Mi, [x for x in Ni if x != c]) for c in Ni]
+    for bat_, M_, N_ in opt:
+        if math.prod(size[c] for c in bat_) > 65535:
+            continue
+        tb = lambda t, lab: (t, lab) if all(c in lab for c in bat_) else (None, None)
+        rb = _best_runs(bat_, [tb(out, lo), tb(A, la), tb(B, lb)], size) if bat_ else \
+            ([], (1, 1, [(0, 0)] * 3))
+        rk = _best_runs(Ki, [(A, la), (B, lb)], size) if Ki else ([], (1, 1, [(0, 0)] * 2))
+        rm = _best_runs(M_, [(out, lo), (A, la)], size) if M_ else ([], (1, 1, [(0, 0)] * 2))
+        rn = _best_runs(N_, [(out, lo), (B, lb)], size) if N_ else ([], (1, 1, [(0, 0)] * 2))
+        if None in (rb, rk, rm, rn):
+            continue
+        (b1, b2, bs), (k1, k2, ks), (m1, m2, ms), (n1, n2, ns) = rb[1], rk[1], rm[1], rn[1]
+        dims = [m1, m2, n1, n2, k1, k2, b1, b2]
+        sa = [ms[1][0], ms[1][1], ks[0][0], ks[0][1], bs[1][0], bs[1][1]]
+        sb = [ks[1][0], ks[1][1], ns[1][0], ns[1][1], bs[2][0], bs[2][1]]
+        sc = [ms[0][0], ms[0][1], ns[0][0], ns[0][1], bs[0][0], bs[0][1]]
+        # coalescing score: unit inner strides for A (k or m), B (n or k) and C (n)
+        score = (sa[3] != 1 and sa[1] != 1) + (sb[3] != 1 and sb[1] != 1) + 2 * (sc[3] != 1)
+        return dict(dims=dims, sa=sa, sb=sb, sc=sc, score=score)
+    return None
 
 
-def contract(spec, A, B, out, alpha=1.0, beta=1.0, precision="fp64", tag=None):
-    """out[spec] = alpha * einsum(spec, A, B) + beta * out   (one GEMM launch).
+def _copy_layout(t, labels, order):
+    perm = [labels.index(c) for c in order]
+    return copy(t.permute(*perm)), "".join(order)
 
+
+def contract(spec, A, B, out, alpha=1.0, beta=1.0, precision="fp64", tag=None, addend=None):
+    """out = alpha * einsum(spec, A, B) + beta * (addend if given else out)  (one MFMA launch).
+
+    Every operand may be any strided view: the index groups (batch, M, N, K) are split into at
+    most two stride runs per tensor, which the contraction kernel addresses directly.  An operand
+    that cannot be expressed so is first laid out contiguously by the permute kernel.
     precision: "fp64" (MFMA f64), "bf16" or "fp16" (MFMA with FP32 accumulation, per-row
-    power-of-two scaling of both operands; FP64 output).  The operand order is chosen so that the
-    fewest bytes need a layout change.  ``out`` must be contiguous unless its strides already
-    match the GEMM's output layout.
+    power-of-two scaling of both operands; FP64 output; used only when both output dimensions
+    are >= LP_MIN_DIM).  ``addend`` must have ``out``'s shape.
     """
     la, lb, lo = _parse(spec)
     _check(A, "A"); _check(B, "B"); _check(out, "out")
+    if addend is not None:
+        _check(addend, "addend")
+        if tuple(addend.shape) != tuple(out.shape) or addend.stride() != out.stride():
+            raise _lib.NLError(f"contract {spec}: addend must match out's shape and strides")
     if len(la) != A.dim() or len(lb) != B.dim() or len(lo) != out.dim():
         raise _lib.NLError(f"contract {spec}: rank mismatch {A.shape} {B.shape} {out.shape}")
     size = {}
@@ -231,55 +251,64 @@ def contract(spec, A, B, out, alpha=1.0, beta=1.0, precision="fp64", tag=None):
     if precision != "fp64":
         Mi = [c for c in lo if c in la and c not in lb]
         Ni = [c for c in lo if c in lb and c not in la]
-        Ki = _order(A, la, [c for c in la if c in lb and c not in lo])
-        if any(c in la and c in lb for c in lo):
-            raise _lib.NLError("contract: batched low-precision contraction not supported")
         M = math.prod(size[c] for c in Mi); N = math.prod(size[c] for c in Ni)
+        if min(M, N) < LP_MIN_DIM or any(c in la and c in lb for c in lo):
+            precision = "fp64"      # thin or batched products stay exact
+    if precision != "fp64":
+        Ki = _order(A, la, [c for c in la if c in lb and c not in lo])
         K = math.prod(size[c] for c in Ki)
         return _contract_lp(A, la, B, lb, out, lo, Mi, Ni, Ki, M, N, K, alpha, beta, precision,
-                            tag)
-    p1 = _plan(A, la, B, lb, out, lo, size)
-    p2 = _plan(B, lb, A, la, out, lo, size)
-    pl = p1 if p1["cost"] <= p2["cost"] else p2
-    A, la, B, lb = pl["A"], pl["la"], pl["B"], pl["lb"]
-    bat, Mi, Ni, Ki, opA, opB = pl["bat"], pl["Mi"], pl["Ni"], pl["Ki"], pl["opA"], pl["opB"]
-    if opA is None:
-        present = [c for c in bat if c in la]
-        A = _as_layout(A, la, present + Mi + Ki); la = "".join(present + Mi + Ki)
-        opA = _operand(A, la, bat, Mi, Ki)
-    if opB is None:
-        present = [c for c in bat if c in lb]
-        B = _as_layout(B, lb, present + Ki + Ni); lb = "".join(present + Ki + Ni)
-        opB = _operand(B, lb, bat, Ki, Ni)
-    a_k, sA, lda = opA          # a_k: A is [M][K] (K unit-stride)
-    b_n, sB, ldb = opB          # b_n: B is [K][N] (N unit-stride)
-    M = math.prod(size[c] for c in Mi); N = math.prod(size[c] for c in Ni)
-    K = math.prod(size[c] for c in Ki); nb = math.prod(size[c] for c in bat)
-    if nb > 65535:
-        raise _lib.NLError(f"contract {spec}: batch {nb} too large")
-    if pl["outok"]:
-        C, (sC, ldc) = out, _layout(out, lo, bat, Mi, Ni)
-        direct = True
+                            tag, addend)
+    # choose the operand order with the best coalescing among the copy-free plans
+    cands = []
+    for (X, lx, Y, ly) in ((A, la, B, lb), (B, lb, A, la)):
+        pl = _plan(X, lx, Y, ly, out, lo, size)
+        if pl is not None:
+            cands.append((pl["score"], X, Y, pl))
+    direct = True
+    if cands:
+        cands.sort(key=lambda c: c[0])
+        _, X, Y, pl = cands[0]
     else:
-        if not out.is_contiguous():
-            raise _lib.NLError(f"contract {spec}: destination must be contiguous")
-        C = WS.get("contract_out", nb * M * N, device=out.device)
-        sC, ldc, direct = M * N, N, False
-    lda = max(lda, 1); ldb = max(ldb, 1); ldc = max(ldc, 1)
+        # lay out the operands contiguously as [bat][M][K] and [bat][K][N]
+        bat = [c for c in lo if c in la and c in lb]
+        Mi = [c for c in lo if c in la and c not in lb]
+        Ni = [c for c in lo if c in lb and c not in la]
+        Ki = [c for c in la if c in lb and c not in lo]
+        X, lx = _copy_layout(A, la, bat + Mi + Ki)
+        Y, ly = _copy_layout(B, lb, bat + Ki + Ni)
+        pl = _plan(X, lx, Y, ly, out, lo, size)
+        if pl is None:
+            if not out.is_contiguous():
+                raise _lib.NLError(f"contract {spec}: destination must be contiguous")
+            order = bat + Mi + Ni
+            tmp = WS.get("contract_out", out.numel(), device=out.device).view(
+                *[size[c] for c in order])
+            tv = tmp.permute(*[order.index(c) for c in lo])
+            pl = _plan(X, lx, Y, ly, tv, lo, size)
+            direct = False
+    dims, sa, sb, sc = pl["dims"], pl["sa"], pl["sb"], pl["sc"]
+    M, N, K = dims[0] * dims[1], dims[2] * dims[3], dims[4] * dims[5]
+    nb = dims[6] * dims[7]
     splits = _splits(M, N, K, nb)
     ws = WS.get("splitk", nb * splits * M * N, device=out.device) if splits > 1 else None
+    C = out if direct else tv
+    D = (addend if addend is not None else out) if (direct and beta != 0.0) else None
+    arr = lambda v: (ctypes.c_int64 * len(v))(*[int(x) for x in v])
     ev = _timed(tag)
     if ev:
         ev[0].record()
-    _lib.call("nl_gemm_f64", int(a_k), int(b_n), M, N, K, ctypes.c_double(alpha), _p(A), lda, sA,
-              _p(B), ldb, sB, ctypes.c_double(beta if direct else 0.0), _p(C), ldc, sC, nb,
-              splits, _p(ws), _stream())
+    _lib.call("nl_contract_f64", arr(dims), ctypes.c_double(alpha), _p(X), arr(sa), _p(Y),
+              arr(sb), ctypes.c_double(beta if direct else 0.0), _p(D), _p(C), arr(sc), splits,
+              _p(ws), _stream())
     if ev:
         ev[1].record()
     if not direct:
-        order = bat + Mi + Ni
-        Cv = C.view(*[size[c] for c in order])
-        permute_into(Cv.permute(*[order.index(c) for c in lo]), out, 1.0, beta)
+        if addend is not None and beta != 0.0:
+            permute_into(addend, out, beta, 0.0)
+            permute_into(tv, out, 1.0, 1.0)
+        else:
+            permute_into(tv, out, 1.0, beta)
     return out
 
 
@@ -355,28 +384,47 @@ def to_lp_rows(t, labels, rows, cols, dtype, tag):
     return buf, isc, ldk
 
 
-def _contract_lp(A, la, B, lb, out, lo, Mi, Ni, Ki, M, N, K, alpha, beta, precision, tag=None):
+def _rowmajor(t, labels, Mi, Ni):
+    """ldc if t seen as [M][N] (M = Mi merged, N = Ni merged and unit-stride), else None."""
+    if _splits_at(t, labels, Mi) or _splits_at(t, labels, Ni):
+        return None
+    nn = [c for c in Ni if t.shape[labels.index(c)] != 1]
+    mm = [c for c in Mi if t.shape[labels.index(c)] != 1]
+    if nn and t.stride(labels.index(nn[-1])) != 1:
+        return None
+    return t.stride(labels.index(mm[-1])) if mm else 1
+
+
+def _contract_lp(A, la, B, lb, out, lo, Mi, Ni, Ki, M, N, K, alpha, beta, precision, tag=None,
+                 addend=None):
     dt = BF16 if precision == "bf16" else FP16
     Ab, ia, lda = to_lp_rows(A, la, Mi, Ki, dt, "lpA")
     Bb, ib, ldb = to_lp_rows(B, lb, Ni, Ki, dt, "lpB")
-    lay = _layout(out, lo, [], Mi, Ni)
-    if lay is not None:
-        C, (_, ldc), direct = out, lay, True
-    else:
+    ldc = _rowmajor(out, lo, Mi, Ni)
+    direct = ldc is not None
+    if not direct:
         C = WS.get("contract_out", M * N, device=out.device).view(M, N)
-        ldc, direct = N, False
+        ldc = N
+    else:
+        C = out
+    D = (addend if addend is not None else out) if (direct and beta != 0.0) else None
     splits = _splits(M, N, lda, 1)
     ws = WS.get("splitk", splits * M * N, device=out.device) if splits > 1 else None
     ev = _timed(tag)
     if ev:
         ev[0].record()
     _lib.call("nl_gemm_lp", dt, M, N, lda, ctypes.c_double(alpha), _p(Ab), lda, 0, _p(ia), 0,
-              _p(Bb), ldb, 0, _p(ib), 0, ctypes.c_double(beta if direct else 0.0), _p(C),
+              _p(Bb), ldb, 0, _p(ib), 0, ctypes.c_double(beta if direct else 0.0), _p(D), _p(C),
               max(ldc, 1), 0, 1, splits, _p(ws), _stream())
     if ev:
         ev[1].record()
     if not direct:
         shp = [out.shape[lo.index(c)] for c in Mi + Ni]
         perm = [(Mi + Ni).index(c) for c in lo]
-        permute_into(C.view(*shp).permute(*perm), out, 1.0, beta)
+        tv = C.view(*shp).permute(*perm)
+        if addend is not None and beta != 0.0:
+            permute_into(addend, out, beta, 0.0)
+            permute_into(tv, out, 1.0, 1.0)
+        else:
+            permute_into(tv, out, 1.0, beta)
     return out

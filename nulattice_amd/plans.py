@@ -139,9 +139,9 @@ class PpphPlans:
     Entry (c, d, a, k, v) of <cd||ak>:
       ret0[a,i]      -= 1/2 v t2[c,d,k,i]   key a        src (c,d,k,:) of t2
       ret1[a,d]      -= v t1[c,k]           key (a,d)    src (c,k) of t1
-      ret2[c,d,i,k]  += v t1[a,i]           key (c,d,k)  src (a,:) of t1   (dest stride h)
+      ret2[c,d,i,k]  += v t1[a,i]           key (c,d,k)  src (a,:) of t1   (dest [c][i][d][k])
       ret4[a,d,i,k]  += v t1[c,i]           key (a,d,k)  src (c,:) of t1   (into the ring W)
-      R56[a,k,i,j]   += v tau[c,d,i,j]      key (a,k)    src (c,d,:,:) of tau
+      R56[a,k,i,j]   += v tau[c,d,i,j]      pair blocks over rows (a,k), columns (c,d)
     """
 
     def __init__(self, idx, val, p, h, device):
@@ -152,10 +152,13 @@ class PpphPlans:
         hh, ph = h * h, p * h
         self.ret0 = SegPlan(a, ((c * p + d) * h + k) * h, val, lambda u: u * h, device)
         self.ret1 = SegPlan(a * p + d, c * h + k, val, lambda u: u, device)
+        # ret2[c,d,i,k] lands in Sij stored as [c][i][d][k]: offset c*h*ph + i*ph + d*h + k
         self.ret2 = SegPlan((c * p + d) * h + k, a * h, val,
-                            lambda u: (u // h) * hh + u % h, device)
+                            lambda u: (u // (p * h)) * h * ph + ((u // h) % p) * h + u % h,
+                            device)
         # ret4[a,d,i,k] lands in W[(d,k),(a,i)] = W[(d*h + k)*ph + a*h + i]
         self.ret4 = SegPlan((a * p + d) * h + k, c * h, val,
                             lambda u: ((u // h) % p * h + u % h) * ph + (u // (p * h)) * h,
                             device)
-        self.r56 = SegPlan(a * h + k, (c * p + d) * hh, val, lambda u: u * hh, device)
+        # R56[(a,k),(i,j)] = sum_(c,d) v[(c,d),(a,k)] tau[(c,d),(i,j)]: a pair-block GEMM
+        self.r56 = PairBlocks(a * h + k, c * p + d, val, p * h, p * p, device)

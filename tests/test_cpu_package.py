@@ -178,9 +178,20 @@ def test_ppph_plans_reproduce_v_ppph_dgrams():
     W = np.zeros((p, h, p, h)); _segsum_host(pl.ret4, h, t1, W, 1.0)
     np.testing.assert_allclose(W, r4.transpose(1, 3, 0, 2), atol=1e-13)
     tau = t2 + np.einsum("ci,dj->cdij", t1, t1) - np.einsum("cj,di->cdij", t1, t1)
-    o = np.zeros((p, h, h, h)); _segsum_host(pl.r56, h * h, tau, o, 1.0)
+    o = _blocks_host(pl.r56, tau.reshape(p * p, h * h), p * h).reshape(p, h, h, h)
     np.testing.assert_allclose(o, (r5 + r6 - r6.transpose(0, 2, 1, 3)).transpose(0, 3, 1, 2),
                                atol=1e-13)
+
+
+def _blocks_host(pb, src, nrow):
+    cols, rows = pb.cols.numpy(), pb.rows.numpy()
+    G = np.where(cols[:, None] >= 0, src[np.maximum(cols, 0)], 0.0)
+    R = pb.V.numpy() @ G.reshape(pb.nb, pb.kb, src.shape[1])
+    out = np.zeros((nrow, src.shape[1]))
+    for r, i in zip(R.reshape(-1, src.shape[1]), rows):
+        if i >= 0:
+            out[i] += r
+    return out
 
 
 def test_pair_blocks_reproduce_pp_ladder():
