@@ -37,7 +37,10 @@ int nl_gemm_f64(int a_kcontig, int b_ncontig, int64_t M, int64_t N, int64_t K, d
  * dims = {M1,M2,N1,N2,K1,K2,B1,B2}; sa = A strides {m1,m2,k1,k2,b1,b2};
  * sb = B strides {k1,k2,n1,n2,b1,b2}; sc = C/D strides {m1,m2,n1,n2,b1,b2}.
  * This is how every dense einsum of the diagram library (ccDgrams.py:48-578) runs without an
- * operand transpose.  splits>1 splits K (ws: batch*splits*M*N doubles). */
+ * operand transpose.  splits>1 forces the MFMA path with that K split (ws: batch*splits*M*N
+ * doubles); splits<=0 chooses automatically between the 128x128 MFMA tiles and the skinny
+ * (N<=64 or M<=64) bandwidth kernel and their split, with ws of nl_contract_ws(dims) doubles. */
+int64_t nl_contract_ws(const int64_t* dims);
 int nl_contract_f64(const int64_t* dims, double alpha, const double* A, const int64_t* sa,
                     const double* B, const int64_t* sb, double beta, const double* D, double* C,
                     const int64_t* sc, int64_t splits, double* ws, void* stream);

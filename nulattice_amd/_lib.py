@@ -16,6 +16,7 @@ _vp = ctypes.c_void_p
 SIGNATURES = {
     "nl_gemm_f64": (_i32, [_i32, _i32, _i64, _i64, _i64, _dbl, _vp, _i64, _i64, _vp, _i64, _i64,
                            _dbl, _vp, _i64, _i64, _i64, _i64, _vp, _vp]),
+    "nl_contract_ws": (_i64, [_vp]),
     "nl_contract_f64": (_i32, [_vp, _dbl, _vp, _vp, _vp, _vp, _dbl, _vp, _vp, _vp, _i64, _vp,
                                _vp]),
     "nl_gemm_lp": (_i32, [_i32, _i64, _i64, _i64, _dbl, _vp, _i64, _i64, _vp, _i64, _vp, _i64,

@@ -43,12 +43,25 @@ __device__ inline TileCoord tile_of(int wg, int nwg, int tiles_m, int tiles_n) {
 constexpr int BK64 = 16;
 constexpr int PAD64 = 4;
 
-struct Idx2 { int64_t n2, s1, s2; };      // extent of the inner run, strides of outer/inner run
+struct Idx2 { int n2; int64_t s1, s2; };  // extent of the inner run, strides of outer/inner run
 
-__device__ inline int64_t off2(int64_t i, const Idx2& d) {
-  int64_t q = i / d.n2;
-  return q * d.s1 + (i - q * d.n2) * d.s2;
+__device__ inline int64_t off2(int i, const Idx2& d) {
+  int q = i / d.n2;
+  return q * d.s1 + (int64_t)(i - q * d.n2) * d.s2;
 }
+
+// Walks consecutive indices i, i+1, ... of a two-run index, carrying into the outer run.
+struct Walk2 {
+  int q, r;
+  int64_t off;
+  __device__ inline Walk2(int i, const Idx2& d) {
+    q = i / d.n2; r = i - q * d.n2; off = q * d.s1 + (int64_t)r * d.s2;
+  }
+  __device__ inline void next(const Idx2& d) {
+    ++r; off += d.s2;
+    if (r == d.n2) { r = 0; ++q; off = q * d.s1; }
+  }
+};
 
 struct GemmF64Args {
   int M, N, K, nbatch;
@@ -89,38 +102,47 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(GemmF64Args g) {
   const int64_t a_row = (AK && m0 + ar < g.M) ? off2(m0 + ar, g.am) : 0;
   const int64_t b_col = (!BNC && n0 + bc < g.N) ? off2(n0 + bc, g.bn) : 0;
 
+  // per-thread index walkers: the m (or n) run of a non-AK (BNC) thread is fixed per kernel
+  const int am0 = AK ? 0 : m0 + (tid & 15) * 8;
+  const int bn0 = BNC ? n0 + (tid & 15) * 8 : 0;
+  const Walk2 wa(am0, g.am), wb(bn0, g.bn);
+
   double ra[8], rb[8];
   auto load = [&](int k0) {
     if (AK) {
-      int kk = (tid & 1) * 8, m = m0 + ar;
+      int k = k0 + (tid & 1) * 8, m = m0 + ar;
+      Walk2 wk(k, g.ak);
 #pragma unroll
       for (int e = 0; e < 8; ++e) {
-        int k = k0 + kk + e;
-        ra[e] = (m < g.M && k < kend) ? A[a_row + off2(k, g.ak)] : 0.0;
+        ra[e] = (m < g.M && k + e < kend) ? A[a_row + wk.off] : 0.0;
+        wk.next(g.ak);
       }
     } else {
-      int kk = tid >> 4, r = (tid & 15) * 8, k = k0 + kk;
+      int k = k0 + (tid >> 4);
       int64_t ko = (k < kend) ? off2(k, g.ak) : 0;
+      Walk2 wm = wa;
 #pragma unroll
       for (int e = 0; e < 8; ++e) {
-        int m = m0 + r + e;
-        ra[e] = (m < g.M && k < kend) ? A[ko + off2(m, g.am)] : 0.0;
+        ra[e] = (am0 + e < g.M && k < kend) ? A[ko + wm.off] : 0.0;
+        wm.next(g.am);
       }
     }
     if (BNC) {
-      int kk = tid >> 4, c = (tid & 15) * 8, k = k0 + kk;
+      int k = k0 + (tid >> 4);
       int64_t ko = (k < kend) ? off2(k, g.bk) : 0;
+      Walk2 wn = wb;
 #pragma unroll
       for (int e = 0; e < 8; ++e) {
-        int n = n0 + c + e;
-        rb[e] = (n < g.N && k < kend) ? B[ko + off2(n, g.bn)] : 0.0;
+        rb[e] = (bn0 + e < g.N && k < kend) ? B[ko + wn.off] : 0.0;
+        wn.next(g.bn);
       }
     } else {
-      int kk = (tid & 1) * 8, n = n0 + bc;
+      int k = k0 + (tid & 1) * 8, n = n0 + bc;
+      Walk2 wk(k, g.bk);
 #pragma unroll
       for (int e = 0; e < 8; ++e) {
-        int k = k0 + kk + e;
-        rb[e] = (n < g.N && k < kend) ? B[b_col + off2(k, g.bk)] : 0.0;
+        rb[e] = (n < g.N && k + e < kend) ? B[b_col + wk.off] : 0.0;
+        wk.next(g.bk);
       }
     }
   };
@@ -212,6 +234,88 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(GemmF64Args g) {
           g.C[ro + coff[j]] = v;
         }
       }
+    }
+}
+
+// Skinny contraction (N <= NB <= 64): one output row per thread, the N outputs in registers,
+// K streamed through LDS in tiles of 16 (A tile 256 x 16 loaded along its unit-stride index).
+// Bandwidth-bound shapes (Fock dressings, h x h products, matrix-vector terms) run here instead
+// of on 128x128 MFMA tiles that would be mostly padding.
+constexpr int SKM = 256, SKK = 16;
+
+template <int NB, bool AM>   // AM: A unit-stride along m (threads along m), else along k
+__global__ __launch_bounds__(256) void skinny_f64_kernel(GemmF64Args g) {
+  __shared__ double As[SKK][SKM + 1];
+  __shared__ double Bs[SKK][NB];
+  const int tid = threadIdx.x;
+  const int batch = blockIdx.z, split = blockIdx.y;
+  const int m0 = blockIdx.x * SKM;
+  const double* A = g.A + off2(batch, g.ab);
+  const double* B = g.B + off2(batch, g.bb);
+  const int kbeg = split * g.kchunk;
+  const int kend = min(g.K, kbeg + g.kchunk);
+  double acc[NB];
+#pragma unroll
+  for (int n = 0; n < NB; ++n) acc[n] = 0.0;
+  // AM mapping: thread owns row m0+tid for the loads too; else thread loads k = tid%16 of rows
+  // m0 + tid/16 + 16e (e < 16), whose offsets are fixed for the whole kernel.
+  const int64_t arow = (AM && m0 + tid < g.M) ? off2(m0 + tid, g.am) : 0;
+  int64_t rowoff[AM ? 1 : 16];
+  if (!AM) {
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+      int m = m0 + tid / 16 + 16 * e;
+      rowoff[e] = m < g.M ? off2(m, g.am) : 0;
+    }
+  }
+  for (int k0 = kbeg; k0 < kend; k0 += SKK) {
+    if (AM) {
+      Walk2 wk(k0, g.ak);
+#pragma unroll
+      for (int e = 0; e < SKK; ++e) {
+        As[e][tid] = (m0 + tid < g.M && k0 + e < kend) ? A[arow + wk.off] : 0.0;
+        wk.next(g.ak);
+      }
+    } else {
+      const int kk = tid & 15, k = k0 + kk;
+      const int64_t ko = k < kend ? off2(k, g.ak) : 0;
+#pragma unroll
+      for (int e = 0; e < 16; ++e) {
+        int mm = tid / 16 + 16 * e;
+        As[kk][mm] = (m0 + mm < g.M && k < kend) ? A[rowoff[e] + ko] : 0.0;
+      }
+    }
+    for (int q = tid; q < SKK * NB; q += 256) {
+      int kk = q / NB, n = q - kk * NB, k = k0 + kk;
+      Bs[kk][n] = (n < g.N && k < kend) ? B[off2(k, g.bk) + off2(n, g.bn)] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < SKK; ++kk) {
+      const double a = As[kk][tid];
+#pragma unroll
+      for (int n = 0; n < NB; ++n) acc[n] = fma(a, Bs[kk][n], acc[n]);
+    }
+    __syncthreads();
+  }
+  const int m = m0 + tid;
+  if (m >= g.M) return;
+  if (g.splits > 1) {
+    double* W = g.ws + ((int64_t)batch * g.splits + split) * (int64_t)g.M * g.N +
+                (int64_t)m * g.N;
+#pragma unroll
+    for (int n = 0; n < NB; ++n)
+      if (n < g.N) W[n] = g.alpha * acc[n];
+    return;
+  }
+  const int64_t ro = off2(batch, g.cb) + off2(m, g.cm);
+#pragma unroll
+  for (int n = 0; n < NB; ++n)
+    if (n < g.N) {
+      const int64_t o = ro + off2(n, g.cn);
+      double v = g.alpha * acc[n];
+      if (g.D && g.beta != 0.0) v += g.beta * g.D[o];
+      g.C[o] = v;
     }
 }
 
@@ -382,7 +486,34 @@ __global__ __launch_bounds__(256) void gemm_lp_kernel(GemmLPArgs g) {
 extern "C" {
 
 static inline Idx2 mk(int64_t n2, int64_t s1, int64_t s2) {
-  Idx2 d; d.n2 = n2 > 0 ? n2 : 1; d.s1 = s1; d.s2 = s2; return d;
+  Idx2 d; d.n2 = n2 > 0 ? (int)n2 : 1; d.s1 = s1; d.s2 = s2; return d;
+}
+
+// Kernel choice and split-K factor for a contraction of the given extents; returns the number of
+// workspace doubles nl_contract_f64 needs (0 if no split).
+static void contract_plan(int64_t M, int64_t N, int64_t K, int64_t nb, int* kind, int64_t* splits) {
+  int64_t tiles;
+  if (N <= 64 || M <= 64) {
+    *kind = 1;   // skinny (operands swapped when M is the small side)
+    int64_t big = N <= 64 ? M : N;
+    tiles = nl_cdiv(big, SKM) * nb;
+    int64_t s = (tiles >= 2048 || K <= 256) ? 1 : std::min<int64_t>(2048 / tiles, K / 128);
+    *splits = std::max<int64_t>(1, std::min<int64_t>(s, 4096));
+  } else {
+    *kind = 0;
+    tiles = nl_cdiv(M, TM) * nl_cdiv(N, TN) * nb;
+    int64_t s = (tiles >= 512 || K <= 512) ? 1 : std::min<int64_t>(1024 / tiles, K / 256);
+    *splits = std::max<int64_t>(1, std::min<int64_t>(s, 256));
+  }
+}
+
+int64_t nl_contract_ws(const int64_t* dims) {
+  if (!dims) return -1;
+  const int64_t M = dims[0] * dims[1], N = dims[2] * dims[3], K = dims[4] * dims[5];
+  const int64_t nb = dims[6] * dims[7];
+  int kind; int64_t splits;
+  contract_plan(M, N, K, nb, &kind, &splits);
+  return splits > 1 ? nb * splits * M * N : 0;
 }
 
 // General strided FP64 contraction (see the kernel comment).  dims[8] = {M1, M2, N1, N2, K1,
@@ -392,6 +523,11 @@ int nl_contract_f64(const int64_t* dims, double alpha, const double* A, const in
                     const double* B, const int64_t* sb, double beta, const double* D, double* C,
                     const int64_t* sc, int64_t splits, double* ws, void* stream) {
   if (!dims || !sa || !sb || !sc) return NL_EARG;
+  int kind = 0;
+  if (splits <= 0) {   // automatic kernel choice and split (ws sized by nl_contract_ws)
+    contract_plan(dims[0] * dims[1], dims[2] * dims[3], dims[4] * dims[5], dims[6] * dims[7],
+                  &kind, &splits);
+  }
   for (int d = 0; d < 8; ++d)
     if (dims[d] < 0) return NL_EARG;
   const int64_t M = dims[0] * dims[1], N = dims[2] * dims[3], K = dims[4] * dims[5];
@@ -400,6 +536,7 @@ int nl_contract_f64(const int64_t* dims, double alpha, const double* A, const in
   if (!A || !B || !C || splits < 1 || (splits > 1 && !ws)) return NL_EARG;
   if (M > INT32_MAX || N > INT32_MAX || K > INT32_MAX || nb > 65535 || splits > 65535)
     return NL_EARG;
+  if (M + TM > INT32_MAX || N + TN > INT32_MAX || K + BK64 > INT32_MAX) return NL_EARG;
   hipStream_t st = (hipStream_t)stream;
   GemmF64Args g;
   g.M = (int)M; g.N = (int)N; g.K = (int)K; g.nbatch = (int)nb; g.alpha = alpha;
@@ -413,6 +550,34 @@ int nl_contract_f64(const int64_t* dims, double alpha, const double* A, const in
   g.tiles_m = (int)nl_cdiv(M, TM); g.tiles_n = (int)nl_cdiv(N, TN);
   int64_t kchunk = nl_cdiv(nl_cdiv(K > 0 ? K : 1, splits), BK64) * BK64;
   g.splits = K == 0 ? 1 : (int)splits; g.kchunk = (int)kchunk; g.ws = ws;
+  if (kind == 1) {
+    if (N > 64) {   // M is the small side: C^T = B^T A^T
+      std::swap(g.M, g.N); std::swap(g.tiles_m, g.tiles_n);
+      std::swap(g.A, g.B);
+      Idx2 t = g.am; g.am = g.bn; g.bn = t;
+      t = g.ak; g.ak = g.bk; g.bk = t;
+      t = g.ab; g.ab = g.bb; g.bb = t;
+      t = g.cm; g.cm = g.cn; g.cn = t;
+    }
+    const bool am = (g.am.n2 > 1 ? g.am.s2 == 1 : g.am.s1 == 1);
+    dim3 sg((unsigned)nl_cdiv(g.M, SKM), (unsigned)g.splits, (unsigned)nb);
+    const int n = g.N;
+#define NL_SK(NB_)                                                                        \
+    if (am) hipLaunchKernelGGL((skinny_f64_kernel<NB_, true>), sg, dim3(256), 0, st, g);  \
+    else    hipLaunchKernelGGL((skinny_f64_kernel<NB_, false>), sg, dim3(256), 0, st, g);
+    if (n <= 8) { NL_SK(8) } else if (n <= 16) { NL_SK(16) } else if (n <= 32) { NL_SK(32) }
+    else { NL_SK(64) }
+#undef NL_SK
+    NL_LAUNCH_CHECK();
+    if (g.splits > 1) {
+      int64_t total = nb * (int64_t)g.M * g.N;
+      int blocks = (int)std::min<int64_t>(nl_cdiv(total, 256), 8192);
+      hipLaunchKernelGGL(splitk_reduce_strided_kernel, dim3(blocks), dim3(256), 0, st, g.M, g.N,
+                         g.splits, (int)nb, (const double*)ws, beta, D, C, g.cm, g.cn, g.cb);
+      NL_LAUNCH_CHECK();
+    }
+    return NL_OK;
+  }
   // thread mapping: unit-stride inner k run of A -> load along k; of B's n -> load along n
   const bool ak = (dims[5] > 1 ? sa[3] == 1 : dims[4] > 1 ? sa[2] == 1 : true) ||
                   !(dims[1] > 1 ? sa[1] == 1 : dims[0] > 1 ? sa[0] == 1 : false);

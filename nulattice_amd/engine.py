@@ -82,6 +82,7 @@ class Hamiltonian:
         # v_phph as [b][k][i][c] (A27) and v_pphh as [c][d][k][l] = v_pphh[d,c,k,l] (A19)
         self.v_phph_bkic = ops.copy(self.v_phph.permute(0, 1, 3, 2))
         self.v_pphh_T = ops.copy(self.v_pphh.permute(1, 0, 2, 3))
+        self.ones_h = torch.ones(h, dtype=F64, device=device)
         consts = (self.f_pp, self.f_ph, self.f_hh, self.v_pphh, self.v_phph, self.v_phhh,
                   self.v_hhhh, self.vP, self.Wc, self.v_phph_bkic, self.v_pphh_T)
         ops.mark_constant(*consts)
@@ -107,6 +108,7 @@ class Buffers:
         self.s_ph, self.u_ph, self.w_ph, self.z_ph = z(p, h), z(p, h), z(p, h), z(p, h)
         self.q_hh = z(h, h)
         self.ret1 = z(p, p)
+        self.T0 = z(p, h, h)
         self.tau = z(p, p, h, h)
         self.S0, self.Sab = z(p, p, h, h), z(p, p, h, h)
         self.Sij = z(p, h, p, h)     # P(ij) group as [a][i][b][j]
@@ -147,7 +149,10 @@ def t1_part(ham, t1, t2, B, prec=DEFAULT_PREC):
     contract("dl,dali->ai", B.s_ph, t2r, H1)
     pl = ham.ppph
     if pl.n:
-        _segsum(pl.ret0, ham.h, t2, H1, -0.5)                                # ret0
+        T0 = B.T0
+        T0.zero_()
+        _segsum(pl.ret0, ham.h, t2, T0, 1.0)                                 # ret0 partials
+        contract("aki,k->ai", T0, ham.ones_h, H1, alpha=-0.5)               # ret0
         B.ret1.zero_()
         _segsum(pl.ret1, 1, t1, B.ret1, -1.0)                                # ret1
     permute_into(ham.f_hh, Xhh, alpha=-1.0)
@@ -224,7 +229,7 @@ def t2_part(ham, t1, t2, B, prec=DEFAULT_PREC):
     contract("cjkl,bl->ckbj", ham.v_phhh, t1, W, alpha=-1.0)                 # A29
     contract("acik,ckbj->aibj", t2r, W, B.Spp, beta=0.0, precision=prec["ring"], tag="ring")
     contract("cdkl,ci->idkl", ham.v_pphh, t1, B.Xidkl, beta=0.0)             # A34
-    contract("idkl,adkj->ilaj", B.Xidkl, t2, B.Yilaj, beta=0.0)
+    contract("idkl,adkj->ilaj", B.Xidkl, t2, B.Yilaj, beta=0.0, precision=prec["hh"])
     contract("ilaj,bl->aibj", B.Yilaj, t1, B.Spp)
     contract("bkci,cj->bkij", ham.v_phph_bkic.permute(0, 1, 3, 2), t1, B.Ubkij, beta=0.0)  # A27
     contract("ak,bkij->aibj", t1, B.Ubkij, B.Spp, alpha=-1.0)

@@ -288,13 +288,12 @@ def contract(spec, A, B, out, alpha=1.0, beta=1.0, precision="fp64", tag=None, a
             pl = _plan(X, lx, Y, ly, tv, lo, size)
             direct = False
     dims, sa, sb, sc = pl["dims"], pl["sa"], pl["sb"], pl["sc"]
-    M, N, K = dims[0] * dims[1], dims[2] * dims[3], dims[4] * dims[5]
-    nb = dims[6] * dims[7]
-    splits = _splits(M, N, K, nb)
-    ws = WS.get("splitk", nb * splits * M * N, device=out.device) if splits > 1 else None
+    arr = lambda v: (ctypes.c_int64 * len(v))(*[int(x) for x in v])
+    nws = int(_lib.load().nl_contract_ws(arr(dims)))
+    ws = WS.get("splitk", nws, device=out.device) if nws > 0 else None
+    splits = -1           # kernel and K split chosen by the library
     C = out if direct else tv
     D = (addend if addend is not None else out) if (direct and beta != 0.0) else None
-    arr = lambda v: (ctypes.c_int64 * len(v))(*[int(x) for x in v])
     ev = _timed(tag)
     if ev:
         ev[0].record()

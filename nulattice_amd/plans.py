@@ -137,7 +137,7 @@ class PpphPlans:
     """The five segment plans that replace v_ppph_dgrams' scalar loop (ccDgrams.py:36-44).
 
     Entry (c, d, a, k, v) of <cd||ak>:
-      ret0[a,i]      -= 1/2 v t2[c,d,k,i]   key a        src (c,d,k,:) of t2
+      ret0[a,i]      -= 1/2 v t2[c,d,k,i]   key (a,k)    src (c,d,k,:) of t2 (then sum over k)
       ret1[a,d]      -= v t1[c,k]           key (a,d)    src (c,k) of t1
       ret2[c,d,i,k]  += v t1[a,i]           key (c,d,k)  src (a,:) of t1   (dest [c][i][d][k])
       ret4[a,d,i,k]  += v t1[c,i]           key (a,d,k)  src (c,:) of t1   (into the ring W)
@@ -150,7 +150,8 @@ class PpphPlans:
             return
         c, d, a, k = (idx[:, n].astype(np.int64) for n in range(4))
         hh, ph = h * h, p * h
-        self.ret0 = SegPlan(a, ((c * p + d) * h + k) * h, val, lambda u: u * h, device)
+        # ret0 partials T0[a][k][i] (segments (a,k)); ret0[a,i] = sum_k T0[a,k,i]
+        self.ret0 = SegPlan(a * h + k, ((c * p + d) * h + k) * h, val, lambda u: u * h, device)
         self.ret1 = SegPlan(a * p + d, c * h + k, val, lambda u: u, device)
         # ret2[c,d,i,k] lands in Sij stored as [c][i][d][k]: offset c*h*ph + i*ph + d*h + k
         self.ret2 = SegPlan((c * p + d) * h + k, a * h, val,

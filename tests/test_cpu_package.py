@@ -167,8 +167,8 @@ def test_ppph_plans_reproduce_v_ppph_dgrams():
     idx, val = coo_arrays(coo, p, h, "ppph")
     pl = PpphPlans(idx, val, p, h, "cpu")
     r0, r1, r2, r3, r4, r5, r6 = cc.v_ppph_dgrams(coo, t1, t2)
-    o = np.zeros((p, h)); _segsum_host(pl.ret0, h, t2, o, -0.5)
-    np.testing.assert_allclose(o, r0, atol=1e-13)
+    o = np.zeros((p, h, h)); _segsum_host(pl.ret0, h, t2, o, -0.5)
+    np.testing.assert_allclose(o.sum(1), r0, atol=1e-13)
     o = np.zeros((p, p)); _segsum_host(pl.ret1, 1, t1, o, -1.0)
     np.testing.assert_allclose(o, r1, atol=1e-13)
     np.testing.assert_allclose(-o.T, r3, atol=1e-13)

@@ -135,4 +135,6 @@ def test_error_codes():
     assert lib.nl_gemm_f64(1, 1, -1, 1, 1, 1.0, None, 1, 0, None, 1, 0, 0.0, None, 1, 0, 1, 1,
                            None, None) == -1
     assert lib.nl_gemm_lp(0, 8, 8, 7, 1.0, None, 8, 0, None, 0, None, 8, 0, None, 0, 0.0, None,
-                          8, 0, 1, 1, None, None) == -1
+                          None, 8, 0, 1, 1, None, None) == -1
+    assert lib.nl_contract_f64(None, 1.0, None, None, None, None, 0.0, None, None, None, 1, None,
+                               None) == -1
