@@ -29,8 +29,8 @@ F64 = torch.float64
 
 _PREC = {
     "fp64": engine.DEFAULT_PREC,
-    "bf16": {"ring": "bf16", "hh": "bf16", "dress": "bf16"},
-    "fp16": {"ring": "fp16", "hh": "fp16", "dress": "fp16"},
+    "bf16": {"ring": "bf16", "hh": "bf16", "dress": "bf16", "ladder": "bf16"},
+    "fp16": {"ring": "fp16", "hh": "fp16", "dress": "fp16", "ladder": "fp16"},
 }
 
 

@@ -319,6 +319,64 @@ __global__ __launch_bounds__(256) void skinny_f64_kernel(GemmF64Args g) {
     }
 }
 
+// Small output (M, N <= 64): all M*N outputs of a K-chunk in one workgroup, each thread owning
+// up to 16 of them; K streamed through LDS in tiles of 32 with coalesced tile loads.  Serves the
+// tall-K h x h dressings (sum over p*p*h elements) at streaming bandwidth.
+constexpr int SMK = 32;
+
+template <bool AM, bool BN>   // AM: A unit-stride along m; BN: B unit-stride along n
+__global__ __launch_bounds__(256) void smallmn_f64_kernel(GemmF64Args g) {
+  __shared__ double As[SMK][65];
+  __shared__ double Bs[SMK][65];
+  const int tid = threadIdx.x;
+  const int batch = blockIdx.z, split = blockIdx.y;
+  const double* A = g.A + off2(batch, g.ab);
+  const double* B = g.B + off2(batch, g.bb);
+  const int kbeg = split * g.kchunk;
+  const int kend = min(g.K, kbeg + g.kchunk);
+  const int MN = g.M * g.N;
+  double acc[16];
+#pragma unroll
+  for (int j = 0; j < 16; ++j) acc[j] = 0.0;
+  for (int k0 = kbeg; k0 < kend; k0 += SMK) {
+    for (int q = tid; q < SMK * 64; q += 256) {
+      int k, m;
+      if (AM) { k = q >> 6; m = q & 63; } else { m = q / SMK; k = q - m * SMK; }
+      As[k][m] = (m < g.M && k0 + k < kend) ? A[off2(m, g.am) + off2(k0 + k, g.ak)] : 0.0;
+      int n;
+      if (BN) { k = q >> 6; n = q & 63; } else { n = q / SMK; k = q - n * SMK; }
+      Bs[k][n] = (n < g.N && k0 + k < kend) ? B[off2(k0 + k, g.bk) + off2(n, g.bn)] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      int o = tid + 256 * j;
+      if (o < MN) {
+        int m = o / g.N, n = o - m * g.N;
+        double s = acc[j];
+#pragma unroll 8
+        for (int kk = 0; kk < SMK; ++kk) s = fma(As[kk][m], Bs[kk][n], s);
+        acc[j] = s;
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    int o = tid + 256 * j;
+    if (o >= MN) continue;
+    int m = o / g.N, n = o - m * g.N;
+    if (g.splits > 1) {
+      g.ws[((int64_t)batch * g.splits + split) * (int64_t)MN + o] = g.alpha * acc[j];
+    } else {
+      const int64_t off = off2(batch, g.cb) + off2(m, g.cm) + off2(n, g.cn);
+      double v = g.alpha * acc[j];
+      if (g.D && g.beta != 0.0) v += g.beta * g.D[off];
+      g.C[off] = v;
+    }
+  }
+}
+
 // C[b](m,n) = sum_s ws[b][s][m][n] + beta * D[b](m,n)
 __global__ void splitk_reduce_strided_kernel(int M, int N, int splits, int nbatch,
                                              const double* ws, double beta, const double* D,
@@ -493,7 +551,11 @@ static inline Idx2 mk(int64_t n2, int64_t s1, int64_t s2) {
 // workspace doubles nl_contract_f64 needs (0 if no split).
 static void contract_plan(int64_t M, int64_t N, int64_t K, int64_t nb, int* kind, int64_t* splits) {
   int64_t tiles;
-  if (N <= 64 || M <= 64) {
+  if (N <= 64 && M <= 64) {
+    *kind = 2;   // small output: split K widely
+    int64_t s = K <= 256 ? 1 : std::min<int64_t>(2048 / std::max<int64_t>(nb, 1), K / 256);
+    *splits = std::max<int64_t>(1, std::min<int64_t>(s, 4096));
+  } else if (N <= 64 || M <= 64) {
     *kind = 1;   // skinny (operands swapped when M is the small side)
     int64_t big = N <= 64 ? M : N;
     tiles = nl_cdiv(big, SKM) * nb;
@@ -550,6 +612,26 @@ int nl_contract_f64(const int64_t* dims, double alpha, const double* A, const in
   g.tiles_m = (int)nl_cdiv(M, TM); g.tiles_n = (int)nl_cdiv(N, TN);
   int64_t kchunk = nl_cdiv(nl_cdiv(K > 0 ? K : 1, splits), BK64) * BK64;
   g.splits = K == 0 ? 1 : (int)splits; g.kchunk = (int)kchunk; g.ws = ws;
+  if (kind == 2) {
+    const bool am = (g.am.n2 > 1 ? g.am.s2 == 1 : g.am.s1 == 1);
+    const bool bn = (g.bn.n2 > 1 ? g.bn.s2 == 1 : g.bn.s1 == 1);
+    g.kchunk = (int)(nl_cdiv(nl_cdiv(K > 0 ? K : 1, g.splits), SMK) * SMK);
+    dim3 sg(1u, (unsigned)g.splits, (unsigned)nb);
+    if (am && bn)   hipLaunchKernelGGL((smallmn_f64_kernel<true, true>), sg, dim3(256), 0, st, g);
+    else if (am)    hipLaunchKernelGGL((smallmn_f64_kernel<true, false>), sg, dim3(256), 0, st, g);
+    else if (bn)    hipLaunchKernelGGL((smallmn_f64_kernel<false, true>), sg, dim3(256), 0, st, g);
+    else            hipLaunchKernelGGL((smallmn_f64_kernel<false, false>), sg, dim3(256), 0, st, g);
+    NL_LAUNCH_CHECK();
+    if (g.splits > 1) {
+      int64_t total = nb * M * N;
+      int blocks = (int)std::min<int64_t>(nl_cdiv(total, 256), 8192);
+      hipLaunchKernelGGL(splitk_reduce_strided_kernel, dim3(blocks), dim3(256), 0, st, (int)M,
+                         (int)N, g.splits, (int)nb, (const double*)ws, beta, D, C, g.cm, g.cn,
+                         g.cb);
+      NL_LAUNCH_CHECK();
+    }
+    return NL_OK;
+  }
   if (kind == 1) {
     if (N > 64) {   // M is the small side: C^T = B^T A^T
       std::swap(g.M, g.N); std::swap(g.tiles_m, g.tiles_n);

@@ -404,7 +404,7 @@ int nl_convert_rows(int dtype, int64_t R1, int64_t R2, int64_t C1, int64_t C2,
   a.in = in; a.out = (uint16_t*)out; a.iscale = iscale; a.target = target; a.bf16 = dtype == 0;
   hipStream_t st = (hipStream_t)stream;
   const int64_t R = R1 * R2;
-  if (a.s[1] == 1 && R2 >= 64 && a.s[3] != 1 && nl_cdiv(ldo, 64) <= 65535) {
+  if (a.s[1] == 1 && R2 >= 8 && a.s[3] != 1 && nl_cdiv(ldo, 64) <= 65535) {
     // rows are the unit-stride direction of the source: transpose through LDS tiles
     unsigned* bits = (unsigned*)iscale;   // holds amax bits until the last pass
     hipLaunchKernelGGL(zero_u32_kernel, dim3(grid_for(R)), dim3(NT), 0, st, R, bits);

@@ -132,7 +132,7 @@ def _segsum(plan, X, src, out, alpha, src_stride=1, dst_stride=1):
               ops._p(out), ops._stream())
 
 
-DEFAULT_PREC = {"ring": "fp64", "hh": "fp64", "dress": "fp64"}
+DEFAULT_PREC = {"ring": "fp64", "hh": "fp64", "dress": "fp64", "ladder": "fp64"}
 
 
 def t1_part(ham, t1, t2, B, prec=DEFAULT_PREC):
@@ -207,10 +207,10 @@ def t2_part(ham, t1, t2, B, prec=DEFAULT_PREC):
     Sab = B.Sab
     permute_into(ham.v_phhh, B.N56)
     if pl.n:
-        _block_gemm(pl.r56, tau, B.N56, 0.5, h * h, "r56")
+        _block_gemm(pl.r56, tau, B.N56, 0.5, h * h, "r56", prec.get("ladder", "fp64"))
     contract("bkij,ak->abij", B.N56, t1, Sab, beta=0.0)                      # A23 + A39
     if pl.n:
-        contract("ad,dbij->abij", B.ret1, t2, Sab)                           # A38
+        contract("ad,dbij->abij", B.ret1, t2, Sab, precision=prec["dress"])  # A38
     contract("bc,acij->abij", Xpp, t2, Sab, precision=prec["dress"])         # X_pp t2
     # ---- Sij: P(ij) group, stored as [a][i][b][j] (like Spp)
     t2r = B.Tr.permute(0, 2, 1, 3)                                           # t2 via [a][i][c][k]
@@ -237,24 +237,53 @@ def t2_part(ham, t1, t2, B, prec=DEFAULT_PREC):
 
 def _pp_ladder(ham, tau, S0, prec):
     """S0 += 1/2 sum_cd v_pppp[a,b,c,d] tau[c,d,i,j] over packed pair blocks."""
-    _block_gemm(ham.pppp, tau, S0, 0.5, ham.h * ham.h, "ladder")
+    _block_gemm(ham.pppp, tau, S0, 0.5, ham.h * ham.h, "ladder", prec.get("ladder", "fp64"))
 
 
-def _block_gemm(pb, src, out, alpha, ncol, tag):
+def _block_gemm(pb, src, out, alpha, ncol, tag, precision="fp64"):
     """out[rows_b] += alpha * V_b src[cols_b] for every packed block b (row gather, batched MFMA
-    GEMM, row scatter; rows of src/out have ncol contiguous elements)."""
+    GEMM, row scatter; rows of src/out have ncol contiguous elements).  With a low precision
+    the blocks V_b (constant, converted once) and the gathered rows (transposed per block) are
+    BF16/FP16 with per-row scales and the batched GEMM accumulates in FP32."""
     if pb.nb == 0:
         return
     G = ops.WS.get(tag + "_G", pb.nb * pb.kb * ncol, device=src.device)
     R = ops.WS.get(tag + "_R", pb.nb * pb.mb * ncol, device=src.device)
     st = ops._stream()
+    ev = ops._timed(tag)
+    if ev:
+        ev[0].record()
     _lib.call("nl_gather_rows", pb.nb * pb.kb, ncol, ops._p(pb.cols), ops._p(src), ncol,
               ops._p(G), st)
-    _lib.call("nl_gemm_f64", 1, 1, pb.mb, ncol, pb.kb, ctypes.c_double(1.0), ops._p(pb.V), pb.kb,
-              pb.mb * pb.kb, ops._p(G), ncol, pb.kb * ncol, ctypes.c_double(0.0), ops._p(R),
-              ncol, pb.mb * ncol, pb.nb, 1, ctypes.c_void_p(0), st)
+    if precision == "fp64":
+        _lib.call("nl_gemm_f64", 1, 1, pb.mb, ncol, pb.kb, ctypes.c_double(1.0), ops._p(pb.V),
+                  pb.kb, pb.mb * pb.kb, ops._p(G), ncol, pb.kb * ncol, ctypes.c_double(0.0),
+                  ops._p(R), ncol, pb.mb * ncol, pb.nb, 1, ctypes.c_void_p(0), st)
+    else:
+        dt = ops.BF16 if precision == "bf16" else ops.FP16
+        target = 1.0 if dt == ops.BF16 else 2.0 ** 14
+        ldk = (pb.kb + 7) // 8 * 8
+        lp = getattr(pb, "lp", None)
+        if lp is None or lp[0] != dt:
+            Vb = torch.empty(pb.nb * pb.mb * ldk, dtype=torch.int16, device=src.device)
+            iv = torch.empty(pb.nb * pb.mb, dtype=torch.float32, device=src.device)
+            s4 = (ctypes.c_int64 * 4)(pb.mb * pb.kb, pb.kb, 0, 1)
+            _lib.call("nl_convert_rows", dt, pb.nb, pb.mb, 1, pb.kb, s4, ops._p(pb.V),
+                      ops._p(Vb), ldk, ops._p(iv), ctypes.c_float(target), st)
+            pb.lp = lp = (dt, Vb, iv)
+        _, Vb, iv = lp
+        Gt = ops.WS.get(tag + "_Gt", pb.nb * ncol * ldk, dtype=torch.int16, device=src.device)
+        ig = ops.WS.get(tag + "_Gs", pb.nb * ncol, dtype=torch.float32, device=src.device)
+        s4 = (ctypes.c_int64 * 4)(pb.kb * ncol, 1, 0, ncol)
+        _lib.call("nl_convert_rows", dt, pb.nb, ncol, 1, pb.kb, s4, ops._p(G), ops._p(Gt), ldk,
+                  ops._p(ig), ctypes.c_float(target), st)
+        _lib.call("nl_gemm_lp", dt, pb.mb, ncol, ldk, ctypes.c_double(1.0), ops._p(Vb), ldk,
+                  pb.mb * ldk, ops._p(iv), pb.mb, ops._p(Gt), ldk, ncol * ldk, ops._p(ig), ncol,
+                  ctypes.c_double(0.0), None, ops._p(R), ncol, pb.mb * ncol, pb.nb, 1, None, st)
     _lib.call("nl_scatter_add_rows", pb.nb * pb.mb, ncol, ops._p(pb.rows), ops._p(R),
               ctypes.c_double(alpha), ops._p(out), ncol, st)
+    if ev:
+        ev[1].record()
 
 
 def update(ham, t1, t2, B, t1n, t2n, mixing, prec=DEFAULT_PREC, H2out=None):

@@ -59,6 +59,8 @@ TIMERS = {"on": False}
 
 
 def _timed(tag):
+    if TIMERS.get("all"):
+        tag = tag or _timed.spec
     if not (tag and TIMERS["on"]):
         return None
     s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -231,6 +233,7 @@ def contract(spec, A, B, out, alpha=1.0, beta=1.0, precision="fp64", tag=None, a
     are >= LP_MIN_DIM).  ``addend`` must have ``out``'s shape.
     """
     la, lb, lo = _parse(spec)
+    _timed.spec = spec
     _check(A, "A"); _check(B, "B"); _check(out, "out")
     if addend is not None:
         _check(addend, "addend")

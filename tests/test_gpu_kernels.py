@@ -35,6 +35,12 @@ SPECS = [
     ("dckl,dakl->ac", (9, 9, 4, 4), (9, 9, 4, 4)),
     ("cdlk,cdli->ki", (9, 9, 4, 4), (9, 9, 4, 4)),
     ("ij,jk->ik", (5, 4000), (4000, 3)),       # tall K: split-K path
+    # skinny kernel: small N, small M (operand swap), huge K with small M and N, batched
+    ("mk,kn->mn", (3000, 40), (40, 7)),
+    ("km,kn->mn", (5000, 20), (5000, 30)),
+    ("ab,bc->ac", (20, 3000), (3000, 900)),
+    ("bmk,bkn->bmn", (4, 700, 33), (4, 33, 9)),
+    ("mk,nk->mn", (1000, 50), (64, 50)),
 ]
 
 
