@@ -176,12 +176,15 @@ def main():
     # roofline of the dominant kernel: the T2 ring GEMM Spp = T[(a,i),(c,k)] W[(c,k),(b,j)]
     ring_flop = 2.0 * (p * h) ** 3
     ring_s = klp.get("ring")
-    peak = 2.5e15 if True else None
+    peak = 2.5e15
     roof = None
+    # HBM bytes per launch of this kernel at L=5 A=28 (bf16), from the calibrated rocprofv3
+    # FETCH_SIZE / WRITE_SIZE passes recorded with the kernel (see the profile summaries)
+    traffic = 19.2 * 2 ** 30 if (a.L, a.A, a.prec) == (5, 28, "bf16") else None
     if ring_s:
         ach = ring_flop / ring_s
         roof = {"bound": "mfma", "achieved": ach / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
-                "frac": ach / peak, "traffic": None, "kernel": f"nl_gemm_lp ({a.prec}) ring",
+                "frac": ach / peak, "traffic": traffic, "kernel": f"nl_gemm_lp ({a.prec}) ring",
                 "kernel_ms": ring_s * 1e3,
                 "fp64_ring_ms": (k64.get("ring") or 0) * 1e3,
                 "fp64_ring_frac_of_78.6TF": (ring_flop / k64["ring"] / 78.6e12
