@@ -105,6 +105,13 @@ int nl_assemble_t2(int64_t p, int64_t h, const double* S0, const double* Sab, co
                    const double* vpphh, const double* t1n, double mixing, double* t2n,
                    double* H2out, double* part, void* stream);
 
+/* Transpose of nl_assemble_t2's antisymmetrisers, for the Lambda (adjoint) equations of t2Iter
+ * (coupled_cluster.py:536-545): given the adjoint L[a,b,i,j] of H2,
+ *   gSab = pAB(L) as [a][b][i][j];  gSij = pIJ(L), gSpp = pIJ(pAB(L)) as [(a,i)][(b,j)].
+ * Any output may be null. */
+int nl_assemble_t2_adj(int64_t p, int64_t h, const double* L, double* gSab, double* gSij,
+                       double* gSpp, void* stream);
+
 /* Number of partial slots used by nl_assemble_t2 / nl_energy_part / nl_dot_part for n elements. */
 int64_t nl_partials(int64_t n);
 

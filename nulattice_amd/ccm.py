@@ -171,7 +171,7 @@ def _solver_ham(fock_mats, two_body_int, device):
 def ccsd_solver(fock_mats, two_body_int, t1initial=None, eps=1e-8, maxSteps=1000, max_diis=10,
                 delta=0, mixing=0.5, verbose=False, sparse=True, ccs=False, *, device=None,
                 diis="legacy", precision="fp64", refine=True, eps_lowprec=None, history=None,
-                info=None, hamiltonian=None):
+                info=None, hamiltonian=None, adjoint=False, lambda_eps=1e-10):
     """Damped-Jacobi CCSD (coupled_cluster.py:547-687) on one MI355X.
 
     Returns (E_corr, t1, t2); like the reference, returns None after a divergence or NaN.
@@ -180,8 +180,11 @@ def ccsd_solver(fock_mats, two_body_int, t1initial=None, eps=1e-8, maxSteps=1000
     With a low ``precision`` the iteration first runs to |dE|/|E| < ``eps_lowprec`` (default
     max(eps, 1e-7)) or until |dE| stops decreasing (the low-precision fixed point is only defined
     to the rounding level of the contractions) and then, if ``refine``, continues in FP64 to
-    ``eps``.
+    ``eps``.  ``adjoint=True`` also solves the CC Lambda equations at the returned amplitudes
+    (to |residual| <= ``lambda_eps`` |dE/dt|) and stores (lambda1, lambda2) in info["lambda"].
     """
+    if adjoint and info is None:
+        raise ValueError("adjoint=True needs an info dict to receive the Lambda amplitudes")
     if diis not in ("legacy", "pulsed", "soa"):
         raise ValueError(f"diis must be 'legacy', 'pulsed' or 'soa', not {diis!r}")
     if precision not in _PREC:
@@ -257,7 +260,9 @@ def ccsd_solver(fock_mats, two_body_int, t1initial=None, eps=1e-8, maxSteps=1000
         if converged:
             if verbose:
                 print(f"Energy found in {i + 1} iterations")
-            _fill_info(info, steps_lp, steps_fp64, t_start, energy)
+            _fill_info(info, steps_lp, steps_fp64, t_start, energy, True)
+            if adjoint:
+                _lambda(ham, t1, t2, B, mixing, lambda_eps, info)
             return energy, t1, t2
         if dstate is not None:
             ext = dstate.step(t1, t2, old1, old2, B)
@@ -275,11 +280,19 @@ def ccsd_solver(fock_mats, two_body_int, t1initial=None, eps=1e-8, maxSteps=1000
         prev = energy
         i += 1
     print("Max Iterations Reached")
-    _fill_info(info, steps_lp, steps_fp64, t_start, energy)
+    _fill_info(info, steps_lp, steps_fp64, t_start, energy, False)
+    if adjoint:
+        _lambda(ham, t1, t2, B, mixing, lambda_eps, info)
     return energy, t1, t2
 
 
-def _fill_info(info, steps_lp, steps_fp64, t_start, energy):
+def _fill_info(info, steps_lp, steps_fp64, t_start, energy, converged=False):
     if info is not None:
         info.update(steps_lowprec=steps_lp, steps_fp64=steps_fp64,
-                    seconds=time.perf_counter() - t_start, energy=energy)
+                    seconds=time.perf_counter() - t_start, energy=energy, converged=converged)
+
+
+def _lambda(ham, t1, t2, B, mixing, eps, info):
+    from . import adjoint as adj
+    l1, l2 = adj.solve_lambda(ham, t1, t2, eps=eps, mixing=mixing, B=B, info=info)
+    info["lambda"] = (l1.clone(), l2.clone())

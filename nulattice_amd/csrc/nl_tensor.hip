@@ -305,6 +305,30 @@ __global__ __launch_bounds__(NT) void assemble_t2_kernel(AssembleArgs g) {
   }
 }
 
+// Transpose of the assembly's antisymmetrisers: for the adjoint L of H2,
+//   gSab[a,b,i,j] = L[a,b,i,j] - L[b,a,i,j]                         (layout [a][b][i][j])
+//   gSij[a,i,b,j] = L[a,b,i,j] - L[a,b,j,i]                         (layout [a][i][b][j])
+//   gSpp[a,i,b,j] = L[a,b,i,j] - L[b,a,i,j] - L[a,b,j,i] + L[b,a,j,i] (layout [a][i][b][j])
+// Threads walk the [a][i][b][j] order; j is innermost in every output and in two of the reads.
+__global__ __launch_bounds__(NT) void assemble_adj_kernel(int64_t p, int64_t h, const double* L,
+                                                          double* gSab, double* gSij,
+                                                          double* gSpp) {
+  const int64_t total = p * p * h * h;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    int64_t j = t % h, r = t / h, b = r % p;
+    r /= p;
+    int64_t i = r % h, a = r / h;
+    double v0 = L[((a * p + b) * h + i) * h + j];
+    double v1 = L[((b * p + a) * h + i) * h + j];
+    double v2 = L[((a * p + b) * h + j) * h + i];
+    double v3 = L[((b * p + a) * h + j) * h + i];
+    if (gSab) gSab[((a * p + b) * h + i) * h + j] = v0 - v1;
+    if (gSij) gSij[t] = v0 - v2;
+    if (gSpp) gSpp[t] = v0 - v1 - v2 + v3;
+  }
+}
+
 // E2 partials for E = f.t1 + 1/4 v.t2 + 1/2 v t1 t1 (the f.t1 part is added by finalize).
 __global__ __launch_bounds__(NT) void energy_part_kernel(int64_t p, int64_t h, const double* v,
                                                          const double* t2, const double* t1,
@@ -496,6 +520,17 @@ int nl_assemble_t2(int64_t p, int64_t h, const double* S0, const double* Sab, co
   g.H2out = H2out;
   hipLaunchKernelGGL(assemble_t2_kernel, dim3(grid_for(n, 4096)), dim3(NT), 0, (hipStream_t)stream,
                      g);
+  NL_LAUNCH_CHECK();
+  return NL_OK;
+}
+
+int nl_assemble_t2_adj(int64_t p, int64_t h, const double* L, double* gSab, double* gSij,
+                       double* gSpp, void* stream) {
+  if (p < 0 || h < 0 || !L) return NL_EARG;
+  int64_t n = p * p * h * h;
+  if (n == 0) return NL_OK;
+  hipLaunchKernelGGL(assemble_adj_kernel, dim3(grid_for(n)), dim3(NT), 0, (hipStream_t)stream, p,
+                     h, L, gSab, gSij, gSpp);
   NL_LAUNCH_CHECK();
   return NL_OK;
 }

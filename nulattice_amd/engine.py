@@ -97,6 +97,34 @@ class Hamiltonian:
         """v_pphh[c,d,k,l] as a view of the [c][k][d][l] copy."""
         return self.vP.permute(0, 2, 1, 3)
 
+    _TWO_BODY = ("v_pphh", "v_phph", "v_phhh", "v_hhhh", "vP", "Wc", "v_phph_bkic", "v_pphh_T")
+
+    def rescaled(self, fock_mats, scale):
+        """Hamiltonian with Fock blocks ``fock_mats`` and every two-body block ``scale`` times this
+        one's (same sparsity plans).  For a contact theory whose couplings enter the normal-ordered
+        blocks affinely this is the derivative Hamiltonian dh/dtheta."""
+        new = Hamiltonian.__new__(Hamiltonian)
+        new.device, new.p, new.h = self.device, self.p, self.h
+        f_pp, f_ph, f_hh = fock_mats
+        new.f_pp, new.f_ph, new.f_hh = (_dev(x, self.device) for x in (f_pp, f_ph, f_hh))
+        for name, t, shp in (("f_pp", new.f_pp, (self.p, self.p)),
+                             ("f_ph", new.f_ph, (self.p, self.h)),
+                             ("f_hh", new.f_hh, (self.h, self.h))):
+            if tuple(t.shape) != shp:
+                raise ValueError(f"{name} has shape {tuple(t.shape)}, expected {shp}")
+        for name in self._TWO_BODY:
+            setattr(new, name, _scaled(getattr(self, name), scale))
+        new.ones_h = self.ones_h
+        new.pppp = self.pppp.scaled(scale, _scaled)
+        new.ppph = self.ppph.scaled(scale, _scaled)
+        return new
+
+
+def _scaled(t, s):
+    out = torch.empty_like(t)
+    ops.axpby(float(s), t, 0.0, out)
+    return out
+
 
 class Buffers:
     """Per-iteration device buffers (allocated once per problem size)."""

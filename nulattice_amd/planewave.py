@@ -167,6 +167,32 @@ class PlaneWave:
                                                 hh[None, :]].sum()
         return e + 4.0 * self.D0 * self.nk ** 3 / self.L ** 6
 
+    def coupling_derivative(self, name):
+        """d/dC0 or d/dD0 of the normal-ordered Hamiltonian: (dE_ref, [df_pp, df_ph, df_hh], scale).
+
+        Every block is affine in (C0, D0): the two-body blocks are g times a fixed pattern with
+        g = (C0 + 2 D0 rho) / L^3, the Fock diagonals carry shift = 3 n_k C0 / L^3 + 3 D0 rho^2,
+        and E_ref = sum_i (eps_i + shift) - 1/2 sum_ij v_ijij + 4 D0 n_k^3 / L^6.  The derivative
+        Hamiltonian therefore has constant Fock diagonals d(shift) and two-body blocks equal to
+        ``scale`` = (dg/dtheta) / g times this Hamiltonian's."""
+        L, nk, rho = self.L, self.nk, self.rho
+        if name == "C0":
+            dg, dshift, de3 = 1.0 / L ** 3, 3.0 * nk / L ** 3, 0.0
+        elif name == "D0":
+            dg, dshift, de3 = 2.0 * rho / L ** 3, 3.0 * rho ** 2, 4.0 * nk ** 3 / L ** 6
+        else:
+            raise ValueError(f"unknown coupling {name!r} (C0 or D0)")
+        if self.g == 0.0:
+            raise ValueError("the normal-ordered contact strength is zero: the derivative "
+                             "two-body blocks are not a multiple of the Hamiltonian's")
+        scale = dg / self.g
+        H = self.holes
+        hh = np.arange(self.h)
+        vsum = self.block(H, H, H, H)[hh[:, None], hh[None, :], hh[:, None], hh[None, :]].sum()
+        dEref = self.h * dshift - 0.5 * scale * vsum + de3
+        fock = [dshift * np.eye(self.p), np.zeros((self.p, self.h)), dshift * np.eye(self.h)]
+        return dEref, fock, scale
+
     def hamiltonian(self, sparse=True):
         """(E_ref, [f_pp, f_ph, f_hh], [v_pppp, v_ppph, v_pphh, v_phph, v_phhh, v_hhhh]) with the
         two COO blocks as (indices, values) pairs (sparse=True) or dense arrays."""

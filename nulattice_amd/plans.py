@@ -116,6 +116,26 @@ class PairBlocks:
         self.rows = torch.tensor(rows.reshape(-1), dtype=torch.int64, device=device)
         self.cols = torch.tensor(cols.reshape(-1), dtype=torch.int64, device=device)
 
+    def _like(self, V, rows, cols, mb, kb):
+        new = PairBlocks.__new__(PairBlocks)
+        new.nb, new.mb, new.kb, new.nnz = self.nb, mb, kb, getattr(self, "nnz", 0)
+        new.V, new.rows, new.cols = V, rows, cols
+        return new
+
+    def transposed(self, transpose_blocks):
+        """Blocks of the transposed matrix (rows and columns swapped, V_b -> V_b^T); a column
+        pair belongs to exactly one block, so the transposed scatter is conflict-free too.
+        ``transpose_blocks(V)`` returns V (nb, mb, kb) as a contiguous (nb, kb, mb) tensor."""
+        if self.nb == 0:
+            return self
+        return self._like(transpose_blocks(self.V), self.cols, self.rows, self.kb, self.mb)
+
+    def scaled(self, scale, scale_values):
+        """The same blocks times ``scale``; ``scale_values(V, s)`` returns s * V."""
+        if self.nb == 0:
+            return self
+        return self._like(scale_values(self.V, scale), self.rows, self.cols, self.mb, self.kb)
+
 
 class SegPlan:
     """Segments of (key -> [(src_off, val)]) for nl_segsum, with per-segment destination offset."""
@@ -132,6 +152,12 @@ class SegPlan:
         self.dst = torch.tensor(np.asarray(dst_of_key(uk), dtype=np.int64), dtype=torch.int64,
                                 device=device)
 
+    def scaled(self, scale, scale_values):
+        new = SegPlan.__new__(SegPlan)
+        new.S, new.ptr, new.src, new.dst = self.S, self.ptr, self.src, self.dst
+        new.val = scale_values(self.val, scale)
+        return new
+
 
 class PpphPlans:
     """The five segment plans that replace v_ppph_dgrams' scalar loop (ccDgrams.py:36-44).
@@ -146,9 +172,12 @@ class PpphPlans:
 
     def __init__(self, idx, val, p, h, device):
         self.n = val.size
+        self.p, self.h, self.device = p, h, device
+        self._adj = None
         if val.size == 0:
             return
         c, d, a, k = (idx[:, n].astype(np.int64) for n in range(4))
+        self._coo = (c, d, a, k, np.asarray(val, dtype=np.float64))
         hh, ph = h * h, p * h
         # ret0 partials T0[a][k][i] (segments (a,k)); ret0[a,i] = sum_k T0[a,k,i]
         self.ret0 = SegPlan(a * h + k, ((c * p + d) * h + k) * h, val, lambda u: u * h, device)
@@ -163,3 +192,37 @@ class PpphPlans:
                             device)
         # R56[(a,k),(i,j)] = sum_(c,d) v[(c,d),(a,k)] tau[(c,d),(i,j)]: a pair-block GEMM
         self.r56 = PairBlocks(a * h + k, c * p + d, val, p * h, p * p, device)
+
+    def scaled(self, scale, scale_values):
+        """The same plans with every value times ``scale`` (adjoint plans are not carried)."""
+        new = PpphPlans.__new__(PpphPlans)
+        new.n, new.p, new.h, new.device, new._adj = self.n, self.p, self.h, self.device, None
+        if self.n:
+            new._coo = self._coo
+            for name in ("ret0", "ret1", "ret2", "ret4"):
+                setattr(new, name, getattr(self, name).scaled(scale, scale_values))
+            new.r56 = self.r56.scaled(scale, scale_values)
+        return new
+
+    def adjoint(self, transpose_blocks):
+        """Transposed plans for the Lambda equations (built once, on first use).
+
+        Entry (c, d, a, k, v); l1 / gS* / gW are the adjoints of H1 / Sij / W / ret1:
+          ret0T: gt2[c,d,k,i]  += -1/2 v l1[a,i]           key (c,d,k)  src (a,:) of l1
+          ret1T: gt1[c,k]      -= v gret1[a,d]             key (c,k)    src (a,d) of gret1
+          ret2T: gt1[a,i]      += v gSij[c][i][d][k]       key a        src (c,.,d,k), stride ph
+          ret4T: gt1[c,i]      += v gW[(d,k),(a,i)]        key c        src ((d,k),(a,.))
+          r56T:  gtau[(c,d),:] += 1/2 v gN56[(a,k),:]      transposed pair blocks
+        """
+        if self._adj is None and self.n:
+            c, d, a, k, val = self._coo
+            p, h, dev = self.p, self.h, self.device
+            ph = p * h
+            self._adj = {
+                "ret0T": SegPlan((c * p + d) * h + k, a * h, val, lambda u: u * h, dev),
+                "ret1T": SegPlan(c * h + k, a * p + d, val, lambda u: u, dev),
+                "ret2T": SegPlan(a, c * h * ph + d * h + k, val, lambda u: u * h, dev),
+                "ret4T": SegPlan(c, (d * h + k) * ph + a * h, val, lambda u: u * h, dev),
+                "r56T": self.r56.transposed(transpose_blocks),
+            }
+        return self._adj

@@ -1,0 +1,173 @@
+"""Differentiation form on the GPU: the Lambda (adjoint) sweep and dE/d(coupling).
+
+Pins (no reference code exists for the Lambda equations):
+* the reverse sweep J^T l against the oracle's Jacobian: <l, J d> by complex-step
+  differentiation of the oracle's dense-path residual (exact to rounding) = <J^T l, d>;
+* dE/dtheta against central finite differences (h = 1e-4) of converged energies, both of the
+  CPU oracle (the FP64 reference gradient) and of the GPU FP64 solver.
+"""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+from nulattice_amd import adjoint, ccm, engine  # noqa: E402
+from nulattice_amd.planewave import planewave_hamiltonian  # noqa: E402
+from oracle import ccsd_ref as cc  # noqa: E402
+from tests.helpers import random_amplitudes, random_antisym_hamiltonian, to_coo  # noqa: E402
+
+
+def dev(x):
+    return torch.tensor(np.asarray(x, np.float64), dtype=torch.float64, device="cuda")
+
+
+def _oracle_residual(fock, dense, t1, t2):
+    f_pp, f_ph, f_hh = (x.astype(complex) for x in fock)
+    v_pppp, v_ppph, v_pphh, v_phph, v_phhh, v_hhhh = (x.astype(complex) for x in dense)
+    H1 = cc.t1_residual(t1, t2, f_ph, f_pp, f_hh, v_phph, v_phhh, v_pphh, v_ppph,
+                        sparse=False)[0]
+    H2 = cc.t2_residual(t1, t2, f_ph, f_hh, f_pp, v_pppp, v_phph, v_phhh, v_pphh, v_ppph, v_hhhh,
+                        sparse=False)[0]
+    return H1, H2
+
+
+def _gpu_ham(fock, dense):
+    v = [to_coo(dense[0]), to_coo(dense[1])] + list(dense[2:])
+    return engine.Hamiltonian(fock, v, device="cuda", min_block_rows=8)
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_reverse_sweep_is_the_oracle_jacobian_transpose(seed):
+    """<l, J d> (complex step of the oracle residual) == <J^T l, d> (GPU sweep) for every block
+    pairing (l1/l2 x d1/d2) at random amplitudes with every sparse term active."""
+    p, h = 7, 4
+    fock, dense, _, _ = random_antisym_hamiltonian(p, h, seed=seed)
+    t1, t2 = random_amplitudes(p, h, seed=seed + 10)
+    l1, l2 = random_amplitudes(p, h, seed=seed + 20)
+    d1, d2 = random_amplitudes(p, h, seed=seed + 30)
+    H = _gpu_ham(fock, dense)
+    B = engine.Buffers(p, h, H.device)
+    G = adjoint.AdjointBuffers(p, h, H.device)
+    T1, T2 = dev(t1), dev(t2)
+    adjoint.forward_intermediates(H, T1, T2, B)
+    eps = 1e-20
+    for m1, m2 in ((l1, 0 * l2), (0 * l1, l2)):
+        G.gt1.zero_()
+        G.gt2.zero_()
+        L1, L2 = dev(m1), dev(m2)
+        adjoint.residual_vjp(H, T1, T2, B, L1, L2, G)
+        g1, g2 = G.gt1.cpu().numpy(), G.gt2.cpu().numpy()
+        for a1, a2 in ((d1, 0 * d2), (0 * d1, d2)):
+            H1c, H2c = _oracle_residual(fock, dense, t1 + 1j * eps * a1, t2 + 1j * eps * a2)
+            lhs = (m1 * H1c.imag / eps).sum() + (m2 * H2c.imag / eps).sum()
+            rhs = (g1 * a1).sum() + (g2 * a2).sum()
+            assert abs(lhs - rhs) <= 1e-12 * max(abs(lhs), 1e-3), (lhs, rhs)
+
+
+def _path(theta, seed=3):
+    fock0, dense0, _, _ = random_antisym_hamiltonian(6, 3, seed=seed, scale=0.1)
+    fock1, dense1, _, _ = random_antisym_hamiltonian(6, 3, seed=seed + 1, scale=0.1)
+    F1 = [fock1[0] - np.diag(np.diag(fock1[0])), fock1[1].copy(),
+          fock1[2] - np.diag(np.diag(fock1[2]))]
+    fock = [a + theta * b for a, b in zip(fock0, F1)]
+    dense = [a + theta * b for a, b in zip(dense0, dense1)]
+    return fock, dense, F1, dense1
+
+
+def test_lambda_gradient_matches_oracle_finite_differences():
+    """h(theta) = h0 + theta h1 (random, nonzero T1, all v_ppph / v_pppp terms): the adjoint
+    dE_corr/dtheta equals central FD (h = 1e-4) of the CPU oracle's converged energies."""
+    theta, hstep = 0.2, 1e-4
+
+    def oracle_e(th):
+        fock, dense, _, _ = _path(th)
+        v = [to_coo(dense[0]), to_coo(dense[1])] + dense[2:]
+        return cc.ccsd_solver(fock, v, eps=1e-14, maxSteps=800, max_diis=0, mixing=0.3)[0]
+
+    fock, dense, F1, dense1 = _path(theta)
+    H = _gpu_ham(fock, dense)
+    e, t1, t2 = ccm.ccsd_solver(fock, [to_coo(dense[0]), to_coo(dense[1])] + dense[2:],
+                                eps=1e-14, maxSteps=800, max_diis=0, mixing=0.3, hamiltonian=H)
+    assert abs(e - oracle_e(theta)) < 1e-12 * abs(e)
+    info = {}
+    g = adjoint.coupling_gradient(H, t1, t2, [(0.0, _gpu_ham(F1, dense1))], info=info,
+                                  eps=1e-12, mixing=0.3)[0]
+    fd = (oracle_e(theta + hstep) - oracle_e(theta - hstep)) / (2 * hstep)
+    assert info["lambda_residual"] <= 1e-12
+    assert abs(g - fd) <= 1e-7 * abs(fd), (g, fd)
+
+
+def _fd_total(L, A, C0, D0, which, hstep=1e-4, oracle=False):
+    def total(c0, d0):
+        E0, fock, v = planewave_hamiltonian(L, A, c0, d0)
+        if oracle:
+            vl = [np.concatenate([v[0][0], v[0][1][:, None]], 1).tolist(),
+                  np.concatenate([v[1][0], v[1][1][:, None]], 1).tolist()] + v[2:]
+            e = cc.ccsd_solver(fock, vl, eps=1e-14, maxSteps=800, max_diis=0)[0]
+        else:
+            e = ccm.ccsd_solver(fock, v, eps=1e-14, maxSteps=800, max_diis=0)[0]
+        return E0 + e
+    if which == "C0":
+        return (total(C0 + hstep, D0) - total(C0 - hstep, D0)) / (2 * hstep)
+    return (total(C0, D0 + hstep) - total(C0, D0 - hstep)) / (2 * hstep)
+
+
+def test_planewave_gradient_L2_matches_oracle_fd():
+    """L=2, A=4, C0=-4, D0=1 through the autograd Function: dE/dC0, dE/dD0 equal central FD of
+    the CPU oracle's total energies (the FP64 reference gradient) to 1e-8."""
+    C0 = torch.tensor(-4.0, dtype=torch.float64, requires_grad=True)
+    D0 = torch.tensor(1.0, dtype=torch.float64, requires_grad=True)
+    info = {}
+    E = adjoint.planewave_ccsd_energy(C0, D0, 2, 4, eps=1e-14, info=info)
+    E.backward()
+    for name, g in (("C0", C0.grad), ("D0", D0.grad)):
+        fd = _fd_total(2, 4, -4.0, 1.0, name, oracle=True)
+        assert abs(float(g) - fd) <= 1e-8 * abs(fd), (name, float(g), fd)
+    assert info["recomputed_E_corr_diff"] == 0.0      # deterministic re-solve
+
+
+@pytest.mark.parametrize("A", [4, 28])
+def test_planewave_gradient_L3_matches_gpu_fd(A):
+    """L=3 (A=4: 104 particles; A=28: 80 particles x 28 holes), C0=-4, D0=1: adjoint gradients
+    equal central FD (h=1e-4) of the GPU FP64 energies to 1e-7; keeping T2 or re-solving for it
+    in backward gives the same gradient."""
+    grads = {}
+    for recompute in (True, False):
+        C0 = torch.tensor(-4.0, dtype=torch.float64, requires_grad=True)
+        D0 = torch.tensor(1.0, dtype=torch.float64, requires_grad=True)
+        E = adjoint.planewave_ccsd_energy(C0, D0, 3, A, eps=1e-14, recompute=recompute)
+        E.backward()
+        grads[recompute] = (float(C0.grad), float(D0.grad))
+    assert grads[True] == pytest.approx(grads[False], rel=1e-13, abs=0)
+    for n, name in enumerate(("C0", "D0")):
+        fd = _fd_total(3, A, -4.0, 1.0, name)
+        assert abs(grads[True][n] - fd) <= 1e-7 * abs(fd), (name, grads[True][n], fd)
+
+
+def test_low_precision_forward_gives_the_fp64_gradient():
+    """precision='bf16' forward (low-precision iteration + FP64 refinement): the gradient at the
+    refined fixed point equals the FP64 one to 1e-9."""
+    out = {}
+    for prec in ("fp64", "bf16"):
+        C0 = torch.tensor(-4.0, dtype=torch.float64, requires_grad=True)
+        D0 = torch.tensor(1.0, dtype=torch.float64, requires_grad=True)
+        E = adjoint.planewave_ccsd_energy(C0, D0, 3, 28, eps=1e-13, precision=prec)
+        E.backward()
+        out[prec] = (float(E.detach()), float(C0.grad), float(D0.grad))
+    for a, b in zip(out["fp64"], out["bf16"]):
+        assert abs(a - b) <= 1e-9 * abs(a)
+
+
+def test_solver_adjoint_flag_returns_lambda():
+    """ccsd_solver(adjoint=True) solves the Lambda equations at the converged point; the
+    Lagrangian stationarity residual is below the requested tolerance."""
+    E0, fock, v = planewave_hamiltonian(3, 4, -4.0, 1.0)
+    info = {}
+    e, t1, t2 = ccm.ccsd_solver(fock, v, eps=1e-12, max_diis=0, adjoint=True, info=info)
+    l1, l2 = info["lambda"]
+    assert tuple(l2.shape) == tuple(t2.shape) and tuple(l1.shape) == tuple(t1.shape)
+    assert info["lambda_residual"] <= 1e-10
+    l2h = l2.cpu().numpy()
+    np.testing.assert_allclose(l2h, -l2h.transpose(1, 0, 2, 3), atol=1e-15)
+    np.testing.assert_allclose(l2h, -l2h.transpose(0, 1, 3, 2), atol=1e-15)

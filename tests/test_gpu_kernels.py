@@ -41,6 +41,9 @@ SPECS = [
     ("ab,bc->ac", (20, 3000), (3000, 900)),
     ("bmk,bkn->bmn", (4, 700, 33), (4, 33, 9)),
     ("mk,nk->mn", (1000, 50), (64, 50)),
+    # no summed index (outer products of the adjoint sweep)
+    ("ai,ck->acik", (9, 4), (9, 4)),
+    ("dl,ai->dali", (9, 4), (9, 4)),
 ]
 
 
