@@ -36,6 +36,8 @@ def parse():
     ap.add_argument("--prec", default="bf16", choices=["bf16", "fp16"])
     ap.add_argument("--no-converge", action="store_true", help="skip the |dE| convergence runs")
     ap.add_argument("--cpu-sample", default="L4", choices=["L4", "none", "full"])
+    ap.add_argument("--grad", default="on", choices=["on", "off"],
+                    help="adjoint dE/dC0, dE/dD0 at C0=-4, D0=1 checked against central FD")
     return ap.parse_args()
 
 
@@ -77,6 +79,64 @@ def time_iterations(ccm_mod, engine, ops, H, prec, steps, warmup, torch):
             continue
         kern[tag] = sum(s.elapsed_time(e) for s, e in evs) / len(evs) / 1e3   # seconds
     return dt / steps, kern, energies
+
+
+def gradient_check(L, A, prec, engine, ccm, adjoint):
+    """dE/dC0 and dE/dD0 at (L, A, C0=-4, D0=1) from the Lambda equations, against central
+    finite differences (h=1e-4) of converged total energies.  The normal-ordered blocks are
+    affine in (C0, D0), so the shifted Hamiltonians h(theta +- h) = h +- h dh/dtheta are built
+    from the derivative Hamiltonian (exact; the derivative blocks themselves are pinned against
+    differences of freshly generated blocks in the tests)."""
+    from nulattice_amd.planewave import PlaneWave, shell_holes
+    import torch
+    C0, D0, hstep, eps = -4.0, 1.0, 1e-4, 1e-13
+    t0 = time.perf_counter()
+    pw = PlaneWave(L, shell_holes(L, A), C0, D0)
+    E0, fock, v = pw.hamiltonian()
+    H = engine.Hamiltonian(fock, v, device="cuda")
+    torch.cuda.synchronize()
+    setup = time.perf_counter() - t0
+
+    def solve(ham):
+        inf = {}
+        out = ccm.ccsd_solver(fock, v, eps=eps, maxSteps=600, max_diis=10, diis="pulsed",
+                              hamiltonian=ham, precision=prec, info=inf)
+        if out is None or not inf.get("converged"):
+            raise RuntimeError("CCSD did not converge in the gradient check")
+        return out, inf
+
+    t0 = time.perf_counter()
+    (e, t1, t2), finfo = solve(H)
+    t_fwd = time.perf_counter() - t0
+    parts = {n: pw.coupling_derivative(n) for n in ("C0", "D0")}
+    derivs = [(dEref, H.rescaled(dfock, scale)) for dEref, dfock, scale in parts.values()]
+    lam = {}
+    t0 = time.perf_counter()
+    grads = adjoint.coupling_gradient(H, t1, t2, derivs, info=lam, eps=1e-10)
+    torch.cuda.synchronize()
+    t_grad = time.perf_counter() - t0
+    del derivs, t1, t2
+    print(f"# gradient L={L} A={A}: adjoint {grads} ({t_grad:.1f} s, "
+          f"{lam['lambda_steps']} Lambda steps)", file=sys.stderr, flush=True)
+    res = {}
+    for (name, (dEref, dfock, scale)), g in zip(parts.items(), grads):
+        Es = []
+        for sgn in (1.0, -1.0):
+            Hs = H.rescaled([f + sgn * hstep * df for f, df in zip(fock, dfock)],
+                            1.0 + sgn * hstep * scale)
+            (es, _, _), _ = solve(Hs)
+            Es.append(E0 + sgn * hstep * dEref + es)
+            del Hs
+        fd = (Es[0] - Es[1]) / (2 * hstep)
+        res[name] = {"adjoint": g, "fd": fd, "rel_err": abs(g - fd) / abs(fd)}
+        print(f"# d/d{name}: adjoint {g:.12g} fd {fd:.12g} rel {res[name]['rel_err']:.2e}",
+              file=sys.stderr, flush=True)
+    return {"L": L, "A": A, "C0": C0, "D0": D0, "fd_step": hstep, "E_total": E0 + e,
+            "grad": res, "max_rel_err": max(r["rel_err"] for r in res.values()),
+            "forward_prec": prec, "forward_s": t_fwd, "forward_steps":
+            (finfo.get("steps_lowprec"), finfo.get("steps_fp64")),
+            "lambda_steps": lam["lambda_steps"], "lambda_residual": lam["lambda_residual"],
+            "lambda_s": lam["lambda_seconds"], "gradient_s": t_grad, "setup_s": setup}
 
 
 def cpu_baseline(mode, seed, D0):
@@ -139,7 +199,7 @@ def main():
         sys.exit(2)
     import numpy as np
     import torch
-    from nulattice_amd import ccm, engine, ops
+    from nulattice_amd import adjoint, ccm, engine, ops
     from nulattice_amd.planewave import planewave_hamiltonian
 
     rng = np.random.default_rng(a.seed)
@@ -189,6 +249,10 @@ def main():
                 "fp64_ring_ms": (k64.get("ring") or 0) * 1e3,
                 "fp64_ring_frac_of_78.6TF": (ring_flop / k64["ring"] / 78.6e12
                                              if k64.get("ring") else None)}
+    grad = None
+    if a.grad == "on":
+        del H
+        grad = gradient_check(a.L, a.A, a.prec, engine, ccm, adjoint)
     cpu = cpu_baseline(a.cpu_sample, a.seed, a.D0)
     out = {
         "metric": "CCSD iters/sec at L=5 A=28: mixed-prec vs FP64 GPU, |dE| vs FP64 oracle",
@@ -202,7 +266,7 @@ def main():
         "fp64_iters_per_s": 1.0 / s64, "fp64_ms_per_step": s64 * 1e3,
         "speedup_vs_fp64": s64 / slp,
         "canonical_flop_per_iter": canonical_flops(p, h, len(v[0][1]), len(v[1][1])),
-        "roofline": roof, "cpu_baseline": cpu,
+        "roofline": roof, "cpu_baseline": cpu, "adjoint_gradient": grad,
     }
     out.update(dE)
     print(json.dumps(out))

@@ -219,14 +219,17 @@ def _dot(x, y, G):
 
 
 def solve_lambda(ham, t1, t2, eps=1e-10, maxSteps=500, mixing=0.5, recompute=False, B=None,
-                 G=None, info=None, verbose=False):
+                 G=None, info=None, verbose=False, max_diis=10):
     """Solve J^T lambda = -dE_corr/dt at converged amplitudes (t1, t2) by damped Jacobi.
 
     lambda_new = m lambda + (1-m) (lambda + P(J^T lambda + g) / D) with the dressed denominators
     D of the amplitude iteration at t and P the antisymmetriser of the T2 block; stops when
-    |P(J^T lambda + g)| <= eps |g|.  ``recompute=True`` re-derives the forward intermediates
-    from (t1, t2) in every iteration instead of keeping them between iterations.
-    Returns (l1, l2) (the tensors G.l1, G.l2)."""
+    |P(J^T lambda + g)| <= eps |g|.  With ``max_diis`` > 0 the steps are extrapolated by the
+    pulsed DIIS of the amplitude solver every ``max_diis`` steps (the equations are linear, so
+    this is a restarted Krylov acceleration of the same fixed point).  ``recompute=True``
+    re-derives the forward intermediates from (t1, t2) in every iteration instead of keeping
+    them between iterations.  Returns (l1, l2) (the tensors G.l1, G.l2)."""
+    from .ccm import _Diis
     p, h = ham.p, ham.h
     dev = t1.device
     B = B if B is not None else engine.Buffers(p, h, dev)
@@ -238,6 +241,7 @@ def solve_lambda(ham, t1, t2, eps=1e-10, maxSteps=500, mixing=0.5, recompute=Fal
     gnorm = math.sqrt(_dot(G.gt1, G.gt1, G) + _dot(G.gt2, G.gt2, G))
     G.l1.zero_()
     G.l2.zero_()
+    dstate = _Diis("pulsed", max_diis, G.l1, G.l2, False) if max_diis > 0 else None
     t0 = time.perf_counter()
     res, steps = math.inf, 0
     for steps in range(1, maxSteps + 1):
@@ -263,6 +267,11 @@ def solve_lambda(ham, t1, t2, eps=1e-10, maxSteps=500, mixing=0.5, recompute=Fal
             break        # the residual was taken at the current lambda: keep it
         G.l1, G.l1n = G.l1n, G.l1
         G.l2, G.l2n = G.l2n, G.l2
+        if dstate is not None:
+            ext = dstate.step(G.l1, G.l2, G.l1n, G.l2n, B)
+            if ext is not None:
+                permute_into(ext[0], G.l1)
+                permute_into(ext[1], G.l2)
     if info is not None:
         info.update(lambda_steps=steps, lambda_residual=res,
                     lambda_seconds=time.perf_counter() - t0)
@@ -314,8 +323,9 @@ class PlaneWaveCCSDEnergy(torch.autograd.Function):
     """E(C0, D0) = E_ref + E_corr of closed-shell plane-wave CCSD at (L, A); ``backward`` returns
     dE/dC0 and dE/dD0 from the Lambda equations.
 
-    opts: eps (amplitude convergence, default 1e-12), maxSteps, mixing, precision ("fp64",
-    "bf16", "fp16": low-precision iteration followed by FP64 refinement), lambda_eps,
+    opts: eps (amplitude convergence, default 1e-12), maxSteps, mixing, max_diis (pulsed DIIS
+    of the amplitude and Lambda iterations, default 10; 0 = plain damped Jacobi), precision
+    ("fp64", "bf16", "fp16": low-precision iteration followed by FP64 refinement), lambda_eps,
     lambda_maxSteps, recompute (default True: the converged T2 is not kept between forward and
     backward; the backward re-solves the amplitude equations, whose kernels are deterministic,
     to regain it), device, info (dict receiving energies, steps and timings)."""
@@ -360,7 +370,8 @@ class PlaneWaveCCSDEnergy(torch.autograd.Function):
         vals = iter(coupling_gradient(ham, t1, t2, [d for d in derivs if d is not None],
                                       info=lam, eps=opts.get("lambda_eps", 1e-10),
                                       maxSteps=opts.get("lambda_maxSteps", 500),
-                                      mixing=opts.get("mixing", 0.5)))
+                                      mixing=opts.get("mixing", 0.5),
+                                      max_diis=opts.get("max_diis", 10)))
         grads = [None if d is None else next(vals) for d in derivs]
         if info is not None:
             info.update(lam)
@@ -373,7 +384,8 @@ class PlaneWaveCCSDEnergy(torch.autograd.Function):
 def _solve(ccm, ham, fock, v, opts, info, phase):
     sinfo = {}
     out = ccm.ccsd_solver(fock, v, eps=opts.get("eps", 1e-12), maxSteps=opts.get("maxSteps", 1000),
-                          max_diis=0, mixing=opts.get("mixing", 0.5), hamiltonian=ham,
+                          max_diis=opts.get("max_diis", 10), diis="pulsed",
+                          mixing=opts.get("mixing", 0.5), hamiltonian=ham,
                           precision=opts.get("precision", "fp64"), info=sinfo)
     if out is None or not sinfo.get("converged"):
         raise _lib.NLError("CCSD amplitude iteration did not converge: no gradient at an "

@@ -117,6 +117,9 @@ class Hamiltonian:
         new.ones_h = self.ones_h
         new.pppp = self.pppp.scaled(scale, _scaled)
         new.ppph = self.ppph.scaled(scale, _scaled)
+        consts = [new.f_pp, new.f_ph, new.f_hh] + [getattr(new, n) for n in self._TWO_BODY]
+        ops.mark_constant(*consts)
+        weakref.finalize(new, ops.forget_ptrs, [t.data_ptr() for t in consts])
         return new
 
 
