@@ -108,6 +108,87 @@ class _Diis:
         return out
 
 
+def t1Iter(t1, t2, f_ph, f_pp, f_hh, v_phph, v_phhh, v_pphh, v_ppph_results, sparse=True):
+    """One unfused T1 Jacobi step, t1 + H1 / (-(X_pp[a,a] + X_hh[i,i])) (coupled_cluster.py:
+    342-414), built from the diagram functions of ccdgrams.  ``v_ppph_results`` is the 7-tuple of
+    ``v_ppph_dgrams`` (sparse=True) or the dense v_ppph[a,b,c,i] (sparse=False).  Returns a new
+    float64 HIP tensor; inputs are not modified."""
+    from . import ccdgrams as dg
+    t1, t2 = dg._d(t1), dg._d(t2)
+    H1 = dg.start(f_ph)
+    dg.accumulate(H1, dg.dgram_akci_ck(v_phph, t1), dg.dgram_ck_acik(f_ph, t2),
+                  dg.dgram_cikl_cakl(v_phhh, t2), dg.dgram_cdkl_ck_dali(v_pphh, t1, t2))
+    X_hh = dg.start(f_hh, -1.0)
+    X_pp = dg.start(f_pp)
+    dg.accumulate(X_hh, dg.dgram_ck_ci(f_ph, t1), dg.dgram_bijk_bj(v_phhh, t1),
+                  dg.dgram_cdlk_cdli(v_pphh, t2), dg.dgram_cdlk_cl_di(v_pphh, t1))
+    dg.accumulate(X_pp, dg.dgram_ck_ak(f_ph, t1), dg.dgram_dckl_dakl(v_pphh, t2),
+                  dg.dgram_cdkl_dk_al(v_pphh, t1))
+    if sparse:
+        dg.accumulate(H1, v_ppph_results[0])
+        dg.accumulate(X_pp, v_ppph_results[1])
+    else:
+        dg.accumulate(H1, dg._c("cdak,cdki->ai", -0.5, v_ppph_results, t2))
+        dg.accumulate(X_pp, dg._c("cdak,ck->ad", -1.0, v_ppph_results, t1))
+    dg.accumulate(H1, dg._c("ac,ci->ai", 1.0, X_pp, t1), dg._c("ki,ak->ai", 1.0, X_hh, t1))
+    return dg.jacobi_t1(t1, H1, X_pp, X_hh)
+
+
+def t2Iter(t1, t2, f_ph, f_hh, f_pp, v_pppp, v_phph, v_phhh, v_pphh, v_ppph_results, v_hhhh,
+           sparse=True):
+    """One unfused T2 Jacobi step, t2 + H2 / (-(X_aa + X_bb + X_ii + X_jj)) (coupled_cluster.py:
+    442-545; note the reference's argument order f_ph, f_hh, f_pp), built from the diagram
+    functions of ccdgrams.  Sparse: v_pppp is COO and ``v_ppph_results`` the 7-tuple of
+    ``v_ppph_dgrams``; dense: both blocks are arrays.  Returns a new float64 HIP tensor."""
+    from . import ccdgrams as dg
+    t1, t2 = dg._d(t1), dg._d(t2)
+    H2 = dg.start(v_pphh)
+    dg.accumulate(H2, dg.dgram_klij_abkl(v_hhhh, t2), dg.dgram_bkcj_acik(v_phph, t2),
+                  dg.dgram_bkij_ak(v_phhh, t1), dg.dgram_cdkl_acik_dblj(v_pphh, t2),
+                  dg.dgram_cdkl_cdij_abkl(v_pphh, t2), dg.dgram_klij_ak_bl(v_hhhh, t1),
+                  dg.dgram_bkci_ak_cj(v_phph, t1), dg.dgram_cikl_ck_ablj(v_phhh, t1, t2),
+                  dg.dgram_cikl_al_bcjk(v_phhh, t1, t2), dg.dgram_cjkl_ci_abkl(v_phhh, t1, t2),
+                  dg.dgram_cjkl_ci_ak_bl(v_phhh, t1), dg.dgram_cdkl_ci_dj_abkl(v_pphh, t1, t2),
+                  dg.dgram_cdkl_ak_bl_cdij(v_pphh, t1, t2),
+                  dg.dgram_cdkl_ci_bl_adkj(v_pphh, t1, t2), dg.dgram_cdkl_ci_ak_dj_bl(v_pphh, t1))
+    X_hh = dg.start(f_hh, -1.0)
+    X_pp = dg.start(f_pp)
+    dg.accumulate(X_pp, dg.dgram_cdkl_bdkl(v_pphh, t2), dg.dgram_ck_bk(f_ph, t1),
+                  dg.dgram_cdlk_dk_bl(v_pphh, t1))
+    dg.accumulate(X_hh, dg.dgram_cdkl_cdjl(v_pphh, t2), dg.dgram_ck_cj(f_ph, t1),
+                  dg.dgram_cdlk_cl_dj(v_pphh, t1))
+    if sparse:
+        r = v_ppph_results
+        dg.accumulate(H2, dg.pIJ(r[2]), dg.dgram_da_dbij(r[3], t2), dg.dgram_acik_bcjk(r[4], t2),
+                      dg.dgram_bijk_ak1(r[5], t1), dg.dgram_bijk_ak2(r[6], t1))
+        lad1, lad2 = dg.v_pppp_dgrams(v_pppp, t1, t2)
+        dg.accumulate(H2, lad1, dg.pIJ(lad2), scale=0.5)
+    else:
+        v3 = dg._d(v_ppph_results)
+        dg.accumulate(H2, dg.pIJ(dg._c("abcj,ci->abij", 1.0, v3, t1)),
+                      dg.pAB(dg._c("cdak,ck,dbij->abij", 1.0, v3, t1, t2), -1.0),
+                      dg._pp(dg._c("dcak,di,bcjk->abij", 1.0, v3, t1, t2)),
+                      dg.pAB(dg._c("cdbk,cdij,ak->abij", 1.0, v3, t2, t1), 0.5),
+                      dg._pp(dg._c("cdbk,ci,dj,ak->abij", 1.0, v3, t1, t1, t1), 0.5),
+                      dg._c("abcd,cdij->abij", 0.5, v_pppp, t2),
+                      dg.pIJ(dg._c("abcd,ci,dj->abij", 1.0, v_pppp, t1, t1), 0.5))
+    dg.accumulate(H2, dg.pAB(dg._c("bc,acij->abij", 1.0, X_pp, t2)),
+                  dg.pIJ(dg._c("kj,abik->abij", 1.0, X_hh, t2)))
+    return dg.jacobi_t2(t2, H2, X_pp, X_hh)
+
+
+def v_ppph_dgrams(v_ppph, t1, t2):
+    """The seven v_ppph intermediates (ccDgrams.py:13-45) on the GPU; see ccdgrams."""
+    from . import ccdgrams as dg
+    return dg.v_ppph_dgrams(v_ppph, t1, t2)
+
+
+def v_pppp_dgrams(v_pppp, t1, t2):
+    """pp ladder intermediates (ccDgrams.py:215-237) on the GPU; see ccdgrams."""
+    from . import ccdgrams as dg
+    return dg.v_pppp_dgrams(v_pppp, t1, t2)
+
+
 def ccsd_energy(f_ph, v_pphh, t2, t1, device=None):
     """E = sum f_ai t_ai + 1/4 sum v_abij t_abij + 1/2 sum v_abij t_ai t_bj
     (coupled_cluster.py:265-286), evaluated on the GPU; returns a Python float."""

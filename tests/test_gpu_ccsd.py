@@ -129,3 +129,16 @@ def test_precision_form_refines_to_fp64(prec):
     assert abs(info["E_lowprec"] - e64) < 1e-3 * abs(e64)
     assert abs(e_lp - e64) < 1e-9 * abs(e64)
     assert info["steps_lowprec"] > 0
+
+
+def test_product_hamiltonian_on_gpu_solver():
+    """nulattice_amd.hamiltonian (the package's own input step) + the GPU solver reproduce the
+    notebook trajectory's steps 0-10 (sparse CooList blocks) and the dense form agrees."""
+    from nulattice_amd import hamiltonian as hm
+    g = GOLD["16O_L2"]
+    for sparse in (True, False):
+        vac, fock, v = hm.get_norm_ord_int(2, lat.ref_16O_gs, g["vT1"], g["vS1"], g["w3"],
+                                           sparse)
+        hist = []
+        ccm.ccsd_solver(fock, v, maxSteps=10, sparse=sparse, history=hist)
+        np.testing.assert_allclose(hist, g["energies"][:11], rtol=1e-12)

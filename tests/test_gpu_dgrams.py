@@ -1,0 +1,75 @@
+"""The reference-named diagram library and unfused t1Iter / t2Iter on the GPU against the oracle
+(the reference's per-diagram tests compare implementations the same way, at 1e-10; here 1e-12).
+"""
+import inspect
+
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+from nulattice_amd import ccdgrams as dg  # noqa: E402
+from nulattice_amd import ccm  # noqa: E402
+from oracle import ccsd_ref as cc  # noqa: E402
+from tests.helpers import random_amplitudes, random_antisym_hamiltonian, to_coo  # noqa: E402
+
+P, H = 6, 4
+FOCK, DENSE, _, _ = random_antisym_hamiltonian(P, H, seed=11)
+T1, T2 = random_amplitudes(P, H, seed=12)
+_rng = np.random.default_rng(13)
+ARGS = dict(v_phph=DENSE[3], v_phhh=DENSE[4], v_pphh=DENSE[2], v_hhhh=DENSE[5], f_ph=FOCK[1],
+            t1=T1, t2=T2, r3=_rng.standard_normal((P, P)), r4=_rng.standard_normal((P, P, H, H)),
+            r5=_rng.standard_normal((P, H, H, H)), r6=_rng.standard_normal((P, H, H, H)))
+NAMES = sorted(n for n in dir(cc) if n.startswith("dgram_"))
+
+
+def host(t):
+    return t.detach().cpu().numpy()
+
+
+def close(a, b, tol=1e-12):
+    return np.abs(a - b).max() <= tol * max(np.abs(a).max(), 1e-300)
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_diagram_matches_oracle(name):
+    ref_f = getattr(cc, name)
+    a = [ARGS[n] for n in inspect.signature(ref_f).parameters]
+    out = getattr(dg, name)(*a)
+    assert out.is_cuda and out.dtype == torch.float64
+    assert close(ref_f(*a), host(out))
+
+
+def test_permutators_and_coo_intermediates():
+    x = _rng.standard_normal((P, P, H, H))
+    assert close(cc.pAB(x), host(dg.pAB(x))) and close(cc.pIJ(x), host(dg.pIJ(x)))
+    v3, v4 = to_coo(DENSE[1]), to_coo(DENSE[0])
+    for a, b in zip(cc.v_ppph_dgrams(v3, T1, T2), ccm.v_ppph_dgrams(v3, T1, T2)):
+        assert close(a, host(b))
+    for a, b in zip(cc.v_pppp_dgrams(v4, T1, T2), ccm.v_pppp_dgrams(v4, T1, T2)):
+        assert close(a, host(b))
+    # empty COO input
+    r = ccm.v_ppph_dgrams([], T1, T2)
+    assert len(r) == 7 and all(float(t.abs().max()) == 0.0 for t in r)
+
+
+@pytest.mark.parametrize("sparse", [True, False])
+def test_t1iter_t2iter_match_oracle_and_leave_inputs_alone(sparse):
+    """Reference signatures and argument order (t2Iter takes f_ph, f_hh, f_pp), both input
+    forms; device inputs are not modified (the reference's tests check that of t1Iter)."""
+    f_pp, f_ph, f_hh = FOCK
+    v_pppp, v_ppph, v_pphh, v_phph, v_phhh, v_hhhh = DENSE
+    if sparse:
+        vp, vr = to_coo(v_pppp), cc.v_ppph_dgrams(to_coo(v_ppph), T1, T2)
+    else:
+        vp, vr = v_pppp, v_ppph
+    ref1 = cc.t1Iter(T1, T2, f_ph, f_pp, f_hh, v_phph, v_phhh, v_pphh, vr, sparse)
+    ref2 = cc.t2Iter(T1, T2, f_ph, f_hh, f_pp, vp, v_phph, v_phhh, v_pphh, vr, v_hhhh, sparse)
+    t1d = torch.tensor(T1, device="cuda")
+    t2d = torch.tensor(T2, device="cuda")
+    keep1, keep2 = t1d.clone(), t2d.clone()
+    new1 = ccm.t1Iter(t1d, t2d, f_ph, f_pp, f_hh, v_phph, v_phhh, v_pphh, vr, sparse)
+    new2 = ccm.t2Iter(t1d, t2d, f_ph, f_hh, f_pp, vp, v_phph, v_phhh, v_pphh, vr, v_hhhh, sparse)
+    assert close(ref1, host(new1)) and close(ref2, host(new2))
+    assert torch.equal(t1d, keep1) and torch.equal(t2d, keep2)
