@@ -539,6 +539,155 @@ __global__ __launch_bounds__(256) void gemm_lp_kernel(GemmLPArgs g) {
     }
 }
 
+// Large-tile BF16/FP16 GEMM for the big square products (the two ring GEMMs): 256x256 block
+// tile, eight waves (two per SIMD) each owning a 128x64 piece (8x4 MFMA 16x16x32 tiles, 128 FP32
+// accumulators per lane), K staged in 64-wide steps through two LDS buffers.  Global data goes
+// straight to LDS (global_load_lds_dwordx4: each wave instruction fills 1 KB = 8 rows of 128 B),
+// so no VGPRs hold staged tiles and the next step's loads run under the current step's MFMAs.
+// The 16-byte chunk c of tile row r is stored at position c ^ ((r >> 1) & 7) of its row: the 16
+// lanes of a fragment load (16 consecutive rows, one logical chunk) then hit 16 distinct bank
+// quads.  Rows beyond M or N keep stale LDS data, which only reaches outputs that are not
+// stored; K beyond the range is zero-filled (only the last step of a split can be partial).
+constexpr int BT = 256;
+constexpr int BKD = 64;
+
+__device__ inline void lds_dma16(const void* g, uint16_t* lds_row_base) {
+  __builtin_amdgcn_global_load_lds(g, (__attribute__((address_space(3))) void*)lds_row_base, 16,
+                                   0, 0);
+}
+
+using BigTile = uint16_t[2][BT][BKD];        // [A/B][tile row][k] of one K step
+
+// One K step of MFMAs for this wave's 128x64 piece from LDS buffer T.
+template <bool IS_BF16>
+__device__ inline void big_step(const BigTile& T, v4f (&acc)[8][4], int wm, int wn, int lane) {
+#pragma unroll
+  for (int s = 0; s < BKD / 32; ++s) {
+    const int cl = s * 4 + (lane >> 4);
+    const int ra = wm * 128 + (lane & 15), rb = wn * 64 + (lane & 15);
+    uint4 a[8], b[4];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int r = ra + i * 16;
+      a[i] = *(const uint4*)&T[0][r][(cl ^ ((r >> 1) & 7)) * 8];
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int r = rb + j * 16;
+      b[j] = *(const uint4*)&T[1][r][(cl ^ ((r >> 1) & 7)) * 8];
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        if (IS_BF16)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+              __builtin_bit_cast(v8bf, a[i]), __builtin_bit_cast(v8bf, b[j]), acc[i][j], 0, 0, 0);
+        else
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(
+              __builtin_bit_cast(v8h, a[i]), __builtin_bit_cast(v8h, b[j]), acc[i][j], 0, 0, 0);
+      }
+    // keep the next slice's fragment reads below these MFMAs: one slice of fragments (48
+    // VGPRs) live next to the 128 accumulators fits the two-waves-per-SIMD register budget
+    __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
+template <bool IS_BF16>
+__global__ __launch_bounds__(512) void gemm_lp_big_kernel(GemmLPArgs g) {
+  // two distinct LDS objects (not one array indexed by a step parity), so the compiler can tell
+  // the buffer being read from the buffer the in-flight loads are filling
+  __shared__ __attribute__((aligned(1024))) BigTile S0;
+  __shared__ __attribute__((aligned(1024))) BigTile S1;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int wm = w >> 2, wn = w & 3;
+  const int batch = blockIdx.z;
+  TileCoord tc = tile_of(blockIdx.x, gridDim.x, g.tiles_m, g.tiles_n);
+  const int m0 = tc.tm * BT, n0 = tc.tn * BT;
+  const uint16_t* A = g.A + (int64_t)batch * g.sA;
+  const uint16_t* B = g.B + (int64_t)batch * g.sB;
+  const int split = blockIdx.y;
+  const int kbeg = split * g.kchunk;
+  const int kend = min(g.K, kbeg + g.kchunk);
+
+  v4f acc[8][4];
+#pragma unroll
+  for (int i = 0; i < 8; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = (v4f){0.f, 0.f, 0.f, 0.f};
+
+  // this lane's place in a DMA instruction: row (lane >> 3) of an 8-row group, position lane & 7
+  const int drow = lane >> 3, dpos = lane & 7;
+  // wave w fills rows [32 w, 32 w + 32) of both operand tiles, 4 instructions of 8 rows each
+  auto issue = [&](BigTile& T, int k0) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int r0 = w * 32 + q * 8, r = r0 + drow;         // tile row
+      const int c = dpos ^ ((r >> 1) & 7);                  // logical chunk stored here
+      const int k = k0 + c * 8;
+      const bool kin = k < kend;
+      const int m = m0 + r, n = n0 + r;
+      if (m < g.M && kin) lds_dma16(A + (int64_t)m * g.lda + k, &T[0][r0][0]);
+      if (n < g.N && kin) lds_dma16(B + (int64_t)n * g.ldb + k, &T[1][r0][0]);
+      if (!kin) {
+        *(uint4*)&T[0][r][dpos * 8] = make_uint4(0, 0, 0, 0);
+        *(uint4*)&T[1][r][dpos * 8] = make_uint4(0, 0, 0, 0);
+      }
+    }
+  };
+
+  // Steps alternate S0, S1, in pairs (an odd step count gets one zero-filled step, so the loop
+  // has a single exit).  At the top of a step: wait for this wave's loads, then a barrier
+  // (everyone's loads of this step landed, everyone's reads of the other buffer are done), then
+  // start the next step's loads into the other buffer and compute this one.
+  const int npairs = kbeg < kend ? (kend - kbeg + 2 * BKD - 1) / (2 * BKD) : 0;
+  if (npairs > 0) issue(S0, kbeg);
+  for (int p = 0; p < npairs; ++p) {
+    const int k0 = kbeg + p * 2 * BKD;
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    issue(S1, k0 + BKD);
+    big_step<IS_BF16>(S0, acc, wm, wn, lane);
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (p + 1 < npairs) issue(S0, k0 + 2 * BKD);
+    big_step<IS_BF16>(S1, acc, wm, wn, lane);
+  }
+
+  const bool partial = g.splits > 1;
+  double* C = partial ? g.ws + ((int64_t)batch * g.splits + split) * (int64_t)g.M * g.N
+                      : g.C + (int64_t)batch * g.sC;
+  const double* Dp = (partial || !g.D) ? nullptr : g.D + (int64_t)batch * g.sC;
+  const int64_t ldc = partial ? g.N : g.ldc;
+  const double beta = partial ? 0.0 : g.beta;
+  const float* ia = g.ia ? g.ia + (int64_t)batch * g.sia : nullptr;
+  const float* ib = g.ib ? g.ib + (int64_t)batch * g.sib : nullptr;
+  double sc[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    int col = n0 + wn * 64 + j * 16 + (lane & 15);
+    sc[j] = (ib && col < g.N) ? (double)ib[col] : 1.0;
+  }
+#pragma unroll
+  for (int i = 0; i < 8; ++i)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      int row = m0 + wm * 128 + i * 16 + (lane >> 4) * 4 + r;
+      if (row >= g.M) continue;
+      double sr = g.alpha * (ia ? (double)ia[row] : 1.0);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        int col = n0 + wn * 64 + j * 16 + (lane & 15);
+        if (col < g.N) {
+          double v = sr * (double)acc[i][j][r] * sc[j];
+          const int64_t o = (int64_t)row * ldc + col;
+          if (Dp && beta != 0.0) v += beta * Dp[o];
+          C[o] = v;
+        }
+      }
+    }
+}
+
 }  // namespace
 
 extern "C" {
@@ -730,8 +879,20 @@ int nl_gemm_lp(int dtype, int64_t M, int64_t N, int64_t K, double alpha, const v
     // any caller; reject to keep the contract simple.
     return NL_EARG;
   }
-  if (dtype == 0) hipLaunchKernelGGL((gemm_lp_kernel<true>), grid, dim3(256), 0, st, g);
-  else            hipLaunchKernelGGL((gemm_lp_kernel<false>), grid, dim3(256), 0, st, g);
+  // 256x256 tiles when they alone give every CU a workgroup, else 128x128 tiles
+  const int64_t big_tiles = nl_cdiv(M, BT) * nl_cdiv(N, BT);
+  if (M >= BT && N >= BT && big_tiles * splits * batch >= 256) {
+    g.tiles_m = (int)nl_cdiv(M, BT); g.tiles_n = (int)nl_cdiv(N, BT);
+    grid = dim3((unsigned)(g.tiles_m * g.tiles_n), (unsigned)splits, (unsigned)batch);
+    if (dtype == 0)
+      hipLaunchKernelGGL((gemm_lp_big_kernel<true>), grid, dim3(512), 0, st, g);
+    else
+      hipLaunchKernelGGL((gemm_lp_big_kernel<false>), grid, dim3(512), 0, st, g);
+  } else if (dtype == 0) {
+    hipLaunchKernelGGL((gemm_lp_kernel<true>), grid, dim3(256), 0, st, g);
+  } else {
+    hipLaunchKernelGGL((gemm_lp_kernel<false>), grid, dim3(256), 0, st, g);
+  }
   NL_LAUNCH_CHECK();
   if (splits > 1) {
     int64_t total = batch * M * N;

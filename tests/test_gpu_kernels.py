@@ -92,6 +92,29 @@ def test_gemm_lp_exact_on_small_integers(dt):
     np.testing.assert_array_equal(host(out), A @ B.T)
 
 
+@pytest.mark.parametrize("dt", [0, 1])
+@pytest.mark.parametrize("M,N,K,splits", [(4000, 4100, 4136, 1), (4096, 4352, 1024, 1),
+                                          (1024, 1000, 4104, 16)])
+def test_gemm_lp_large_tile_path_exact(dt, M, N, K, splits):
+    """Shapes that take the 256x256-tile kernel (LDS DMA, swizzled LDS rows): edge tiles in M
+    and N, a partial last K step and an odd step count (K = 4136 = 64.6 x 64), split-K, per-row
+    scales and the addend, all exact on small-integer inputs."""
+    g = torch.Generator(device="cuda").manual_seed(M + K + dt)
+    tdt = (torch.bfloat16, torch.float16)[dt]
+    A = torch.randint(-4, 5, (M, K), device="cuda", generator=g).to(tdt)
+    B = torch.randint(-4, 5, (N, K), device="cuda", generator=g).to(tdt)
+    ia = torch.full((M,), 0.5, device="cuda")
+    ib = torch.full((N,), 2.0, device="cuda")
+    D = torch.randn(M, N, dtype=torch.float64, device="cuda", generator=g)
+    C = torch.empty(M, N, dtype=torch.float64, device="cuda")
+    ws = torch.empty(splits * M * N, dtype=torch.float64, device="cuda") if splits > 1 else None
+    _lib.call("nl_gemm_lp", dt, M, N, K, ctypes.c_double(3.0), ops._p(A), K, 0, ops._p(ia), 0,
+              ops._p(B), K, 0, ops._p(ib), 0, ctypes.c_double(0.5), ops._p(D), ops._p(C), N, 0,
+              1, splits, ops._p(ws), ops._stream())
+    ref = 3.0 * (A.double() @ B.double().t()) + 0.5 * D
+    assert float((C - ref).abs().max()) == 0.0
+
+
 @pytest.mark.parametrize("prec,tol", [("bf16", 2e-2), ("fp16", 4e-3)])
 def test_gemm_lp_scaled_accuracy(prec, tol):
     """Per-row scaling keeps tiny-magnitude rows (amplitude-like values ~1e-6) accurate."""
