@@ -240,7 +240,7 @@ def main():
     roof = None
     # HBM bytes per launch of this kernel at L=5 A=28 (bf16), from the calibrated rocprofv3
     # FETCH_SIZE / WRITE_SIZE passes recorded with the kernel (see the profile summaries)
-    traffic = 19.2 * 2 ** 30 if (a.L, a.A, a.prec) == (5, 28, "bf16") else None
+    traffic = 8.85 * 2 ** 30 if (a.L, a.A, a.prec) == (5, 28, "bf16") else None
     if ring_s:
         ach = ring_flop / ring_s
         roof = {"bound": "mfma", "achieved": ach / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
