@@ -65,7 +65,9 @@ int nl_permute(int rank, const int64_t* out_shape, const int64_t* in_strides, do
 /* Row-wise FP64 -> BF16 (dtype 0) / FP16 (dtype 1) conversion of a 4-index view
  * (rows r1*R2+r2, cols c1*C2+c2, element in[r1*s0+r2*s1+c1*s2+c2*s3]) with a per-row power-of-two
  * scale bringing the row max to <= target; iscale[r] receives the inverse scale; columns
- * [C1*C2, ldo) are zero-filled.  Precision-form operand preparation (no reference counterpart). */
+ * [C1*C2, ldo) are zero-filled.  target <= 0: plain rounding cast, no scales (iscale unused, may
+ * be null; BF16's exponent range makes a scale unnecessary).  Precision-form operand preparation
+ * (no reference counterpart). */
 int nl_convert_rows(int dtype, int64_t R1, int64_t R2, int64_t C1, int64_t C2,
                     const int64_t* strides4, const double* in, void* out, int64_t ldo,
                     float* iscale, float target, void* stream);

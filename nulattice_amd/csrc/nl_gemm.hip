@@ -298,25 +298,35 @@ __global__ __launch_bounds__(256) void skinny_f64_kernel(GemmF64Args g) {
     }
     __syncthreads();
   }
-  const int m = m0 + tid;
-  if (m >= g.M) return;
-  if (g.splits > 1) {
-    double* W = g.ws + ((int64_t)batch * g.splits + split) * (int64_t)g.M * g.N +
-                (int64_t)m * g.N;
+  // Outputs leave through LDS in SR-row slices (the A tile's storage is free now): a thread
+  // holds one row, but the stores run with consecutive lanes on consecutive columns, so rows of
+  // a row-major output (and the split-K partials) are written as contiguous runs.
+  constexpr int SR = NB <= 32 ? 64 : 32;
+  static_assert(SR * (NB + 1) <= SKK * (SKM + 1), "output staging must fit in the A tile");
+  double (*St)[NB + 1] = reinterpret_cast<double (*)[NB + 1]>(&As[0][0]);
+  const bool partial = g.splits > 1;
+  double* W = partial ? g.ws + ((int64_t)batch * g.splits + split) * (int64_t)g.M * g.N : nullptr;
+  const int64_t cbo = off2(batch, g.cb);
+  for (int sl = 0; sl < SKM / SR; ++sl) {
+    __syncthreads();
+    if (tid / SR == sl) {
 #pragma unroll
-    for (int n = 0; n < NB; ++n)
-      if (n < g.N) W[n] = g.alpha * acc[n];
-    return;
-  }
-  const int64_t ro = off2(batch, g.cb) + off2(m, g.cm);
-#pragma unroll
-  for (int n = 0; n < NB; ++n)
-    if (n < g.N) {
-      const int64_t o = ro + off2(n, g.cn);
-      double v = g.alpha * acc[n];
-      if (g.D && g.beta != 0.0) v += g.beta * g.D[o];
-      g.C[o] = v;
+      for (int n = 0; n < NB; ++n) St[tid % SR][n] = acc[n];
     }
+    __syncthreads();
+    for (int q = tid; q < SR * NB; q += 256) {
+      const int r = q / NB, n = q - r * NB, m = m0 + sl * SR + r;
+      if (m >= g.M || n >= g.N) continue;
+      double v = g.alpha * St[r][n];
+      if (partial) {
+        W[(int64_t)m * g.N + n] = v;
+      } else {
+        const int64_t o = cbo + off2(m, g.cm) + off2(n, g.cn);
+        if (g.D && g.beta != 0.0) v += g.beta * g.D[o];
+        g.C[o] = v;
+      }
+    }
+  }
 }
 
 // Small output (M, N <= 64): all M*N outputs of a K-chunk in one workgroup, each thread owning
@@ -373,6 +383,127 @@ __global__ __launch_bounds__(256) void smallmn_f64_kernel(GemmF64Args g) {
       double v = g.alpha * acc[j];
       if (g.D && g.beta != 0.0) v += g.beta * g.D[off];
       g.C[off] = v;
+    }
+  }
+}
+
+// Small output with M, N <= 32 (the h x h dressings): each thread owns a 4x4 register block of
+// the 32x32 output, so a k step costs 8 LDS reads for 16 FMAs (the kernel above reads two LDS
+// values per FMA), and the four 64-thread groups of the workgroup take interleaved quarters of
+// each 32-wide K tile; their partial blocks are added in a fixed order at the end.
+template <bool AM, bool BN>
+__global__ __launch_bounds__(256) void smallmn32_f64_kernel(GemmF64Args g) {
+  __shared__ double As[SMK][33];
+  __shared__ double Bs[SMK][33];
+  __shared__ double Red[3][32][33];
+  const int tid = threadIdx.x, grp = tid >> 6, lt = tid & 63;
+  const int bm = (lt >> 3) * 4, bn = (lt & 7) * 4;
+  const int batch = blockIdx.z, split = blockIdx.y;
+  const double* A = g.A + off2(batch, g.ab);
+  const double* B = g.B + off2(batch, g.bb);
+  const int kbeg = split * g.kchunk;
+  const int kend = min(g.K, kbeg + g.kchunk);
+  // load mapping (4 elements per thread per tile and operand): along the unit-stride index
+  const int lo = tid & 31, hi = tid >> 5;                  // hi + 8*it, it < 4
+  int64_t amo[4], bno[4];                                  // fixed m / n offsets
+#pragma unroll
+  for (int it = 0; it < 4; ++it) {
+    const int m = AM ? lo : hi + 8 * it, n = BN ? lo : hi + 8 * it;
+    amo[it] = m < g.M ? off2(m, g.am) : 0;
+    bno[it] = n < g.N ? off2(n, g.bn) : 0;
+  }
+  double acc[4][4];
+#pragma unroll
+  for (int x = 0; x < 4; ++x)
+#pragma unroll
+    for (int y = 0; y < 4; ++y) acc[x][y] = 0.0;
+  for (int k0 = kbeg; k0 < kend; k0 += SMK) {
+    const int64_t kfix = (k0 + lo < kend) ? off2(k0 + lo, g.ak) : 0;
+    const int64_t kfixb = (k0 + lo < kend) ? off2(k0 + lo, g.bk) : 0;
+#pragma unroll
+    for (int it = 0; it < 4; ++it) {
+      const int kv = hi + 8 * it;                          // k of the "along m/n" mappings
+      if (AM) {
+        const bool ok = lo < g.M && k0 + kv < kend;
+        As[kv][lo] = ok ? A[amo[it] + off2(k0 + kv, g.ak)] : 0.0;
+      } else {
+        const bool ok = kv < g.M && k0 + lo < kend;
+        As[lo][kv] = ok ? A[amo[it] + kfix] : 0.0;
+      }
+      if (BN) {
+        const bool ok = lo < g.N && k0 + kv < kend;
+        Bs[kv][lo] = ok ? B[off2(k0 + kv, g.bk) + bno[it]] : 0.0;
+      } else {
+        const bool ok = kv < g.N && k0 + lo < kend;
+        Bs[lo][kv] = ok ? B[kfixb + bno[it]] : 0.0;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kq = 0; kq < SMK / 4; ++kq) {
+      const int kk = kq * 4 + grp;
+      double a[4], b[4];
+#pragma unroll
+      for (int x = 0; x < 4; ++x) { a[x] = As[kk][bm + x]; b[x] = Bs[kk][bn + x]; }
+#pragma unroll
+      for (int x = 0; x < 4; ++x)
+#pragma unroll
+        for (int y = 0; y < 4; ++y) acc[x][y] = fma(a[x], b[y], acc[x][y]);
+    }
+    __syncthreads();
+  }
+  if (grp > 0) {
+#pragma unroll
+    for (int x = 0; x < 4; ++x)
+#pragma unroll
+      for (int y = 0; y < 4; ++y) Red[grp - 1][bm + x][bn + y] = acc[x][y];
+  }
+  __syncthreads();
+  if (grp != 0) return;
+  const int MN = g.M * g.N;
+#pragma unroll
+  for (int x = 0; x < 4; ++x)
+#pragma unroll
+    for (int y = 0; y < 4; ++y) {
+      const int m = bm + x, n = bn + y;
+      if (m >= g.M || n >= g.N) continue;
+      const double v = g.alpha * (((acc[x][y] + Red[0][m][n]) + Red[1][m][n]) + Red[2][m][n]);
+      if (g.splits > 1) {
+        g.ws[((int64_t)batch * g.splits + split) * (int64_t)MN + (int64_t)m * g.N + n] = v;
+      } else {
+        const int64_t off = off2(batch, g.cb) + off2(m, g.cm) + off2(n, g.cn);
+        g.C[off] = (g.D && g.beta != 0.0) ? v + g.beta * g.D[off] : v;
+      }
+    }
+}
+
+__device__ inline double wave_sum64(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+
+// C[b](m,n) = sum_s ws[b][s][m][n] + beta * D[b](m,n) with one wavefront per output: lanes take
+// the splits in a fixed interleave and a shuffle tree adds them (for large split counts, where
+// a thread-per-output loop would walk thousands of dependent loads).
+__global__ __launch_bounds__(256) void splitk_reduce_wave_kernel(int M, int N, int splits,
+                                                                 int nbatch, const double* ws,
+                                                                 double beta, const double* D,
+                                                                 double* C, Idx2 cm, Idx2 cn,
+                                                                 Idx2 cb) {
+  const int lane = threadIdx.x & 63;
+  const int64_t MN = (int64_t)M * N, total = (int64_t)nbatch * MN;
+  const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  for (int64_t t = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6; t < total; t += nw) {
+    const int64_t b = t / MN, mn = t - b * MN;
+    const double* W = ws + b * splits * MN + mn;
+    double s = 0.0;
+    for (int q = lane; q < splits; q += 64) s += W[(int64_t)q * MN];
+    s = wave_sum64(s);
+    if (lane == 0) {
+      const int64_t m = mn / N, n = mn - m * N;
+      const int64_t o = off2((int)b, cb) + off2((int)m, cm) + off2((int)n, cn);
+      C[o] = (D && beta != 0.0) ? s + beta * D[o] : s;
     }
   }
 }
@@ -730,6 +861,21 @@ int64_t nl_contract_ws(const int64_t* dims) {
 // General strided FP64 contraction (see the kernel comment).  dims[8] = {M1, M2, N1, N2, K1,
 // K2, B1, B2}; sa[6] = A strides {m1, m2, k1, k2, b1, b2}; sb[6] = B {k1, k2, n1, n2, b1, b2};
 // sc[6] = C (and D) {m1, m2, n1, n2, b1, b2}.  C = alpha*A*B + beta*D (D may be C or null).
+// Split-K reduction of the [batch][splits][M][N] partials into C (layout cm, cn, cb).
+static void reduce_splits(hipStream_t st, int M, int N, int splits, int nb, const double* ws,
+                          double beta, const double* D, double* C, Idx2 cm, Idx2 cn, Idx2 cb) {
+  const int64_t total = (int64_t)nb * M * N;
+  if (splits >= 64) {
+    int blocks = (int)std::min<int64_t>(nl_cdiv(total * 64, 256), 16384);
+    hipLaunchKernelGGL(splitk_reduce_wave_kernel, dim3(blocks), dim3(256), 0, st, M, N, splits,
+                       nb, ws, beta, D, C, cm, cn, cb);
+  } else {
+    int blocks = (int)std::min<int64_t>(nl_cdiv(total, 256), 8192);
+    hipLaunchKernelGGL(splitk_reduce_strided_kernel, dim3(blocks), dim3(256), 0, st, M, N,
+                       splits, nb, ws, beta, D, C, cm, cn, cb);
+  }
+}
+
 int nl_contract_f64(const int64_t* dims, double alpha, const double* A, const int64_t* sa,
                     const double* B, const int64_t* sb, double beta, const double* D, double* C,
                     const int64_t* sc, int64_t splits, double* ws, void* stream) {
@@ -766,17 +912,16 @@ int nl_contract_f64(const int64_t* dims, double alpha, const double* A, const in
     const bool bn = (g.bn.n2 > 1 ? g.bn.s2 == 1 : g.bn.s1 == 1);
     g.kchunk = (int)(nl_cdiv(nl_cdiv(K > 0 ? K : 1, g.splits), SMK) * SMK);
     dim3 sg(1u, (unsigned)g.splits, (unsigned)nb);
-    if (am && bn)   hipLaunchKernelGGL((smallmn_f64_kernel<true, true>), sg, dim3(256), 0, st, g);
-    else if (am)    hipLaunchKernelGGL((smallmn_f64_kernel<true, false>), sg, dim3(256), 0, st, g);
-    else if (bn)    hipLaunchKernelGGL((smallmn_f64_kernel<false, true>), sg, dim3(256), 0, st, g);
-    else            hipLaunchKernelGGL((smallmn_f64_kernel<false, false>), sg, dim3(256), 0, st, g);
+#define NL_SM(KERN)                                                                          \
+    if (am && bn)   hipLaunchKernelGGL((KERN<true, true>), sg, dim3(256), 0, st, g);       \
+    else if (am)    hipLaunchKernelGGL((KERN<true, false>), sg, dim3(256), 0, st, g);      \
+    else if (bn)    hipLaunchKernelGGL((KERN<false, true>), sg, dim3(256), 0, st, g);      \
+    else            hipLaunchKernelGGL((KERN<false, false>), sg, dim3(256), 0, st, g);
+    if (M <= 32 && N <= 32) { NL_SM(smallmn32_f64_kernel) } else { NL_SM(smallmn_f64_kernel) }
+#undef NL_SM
     NL_LAUNCH_CHECK();
     if (g.splits > 1) {
-      int64_t total = nb * M * N;
-      int blocks = (int)std::min<int64_t>(nl_cdiv(total, 256), 8192);
-      hipLaunchKernelGGL(splitk_reduce_strided_kernel, dim3(blocks), dim3(256), 0, st, (int)M,
-                         (int)N, g.splits, (int)nb, (const double*)ws, beta, D, C, g.cm, g.cn,
-                         g.cb);
+      reduce_splits(st, (int)M, (int)N, g.splits, (int)nb, ws, beta, D, C, g.cm, g.cn, g.cb);
       NL_LAUNCH_CHECK();
     }
     return NL_OK;
@@ -801,10 +946,7 @@ int nl_contract_f64(const int64_t* dims, double alpha, const double* A, const in
 #undef NL_SK
     NL_LAUNCH_CHECK();
     if (g.splits > 1) {
-      int64_t total = nb * (int64_t)g.M * g.N;
-      int blocks = (int)std::min<int64_t>(nl_cdiv(total, 256), 8192);
-      hipLaunchKernelGGL(splitk_reduce_strided_kernel, dim3(blocks), dim3(256), 0, st, g.M, g.N,
-                         g.splits, (int)nb, (const double*)ws, beta, D, C, g.cm, g.cn, g.cb);
+      reduce_splits(st, g.M, g.N, g.splits, (int)nb, ws, beta, D, C, g.cm, g.cn, g.cb);
       NL_LAUNCH_CHECK();
     }
     return NL_OK;
@@ -821,11 +963,7 @@ int nl_contract_f64(const int64_t* dims, double alpha, const double* A, const in
   else            hipLaunchKernelGGL((gemm_f64_kernel<false, false>), grid, dim3(256), 0, st, g);
   NL_LAUNCH_CHECK();
   if (g.splits > 1) {
-    int64_t total = nb * M * N;
-    int blocks = (int)std::min<int64_t>(nl_cdiv(total, 256), 8192);
-    hipLaunchKernelGGL(splitk_reduce_strided_kernel, dim3(blocks), dim3(256), 0, st, (int)M,
-                       (int)N, g.splits, (int)nb, (const double*)ws, beta, D, C, g.cm, g.cn,
-                       g.cb);
+    reduce_splits(st, (int)M, (int)N, g.splits, (int)nb, ws, beta, D, C, g.cm, g.cn, g.cb);
     NL_LAUNCH_CHECK();
   }
   return NL_OK;

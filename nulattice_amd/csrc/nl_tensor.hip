@@ -119,6 +119,24 @@ __global__ __launch_bounds__(NT) void convert_rows_kernel(ConvArgs a) {
   }
 }
 
+// Unscaled cast (BF16 keeps FP32's exponent range, so its relative precision does not depend on
+// a row scale): a 2D grid of (row, 2048-column chunk), so long rows are spread over workgroups.
+constexpr int CAST_CHUNK = 2048;
+__global__ __launch_bounds__(NT) void cast_rows_kernel(ConvArgs a) {
+  const int64_t r = blockIdx.x, r1 = r / a.R2, r2 = r - r1 * a.R2;
+  const double* base = a.in + r1 * a.s[0] + r2 * a.s[1];
+  uint16_t* o = a.out + r * a.ldo;
+  const int64_t c0 = (int64_t)blockIdx.y * CAST_CHUNK, c1e = min(a.ldo, c0 + CAST_CHUNK);
+  for (int64_t c = c0 + threadIdx.x; c < c1e; c += NT) {
+    float v = 0.f;
+    if (c < a.C) {
+      int64_t c1 = c / a.C2, c2 = c - c1 * a.C2;
+      v = (float)base[c1 * a.s[2] + c2 * a.s[3]];
+    }
+    o[c] = a.bf16 ? to_bf16_bits(v) : to_f16_bits(v);
+  }
+}
+
 // Transposing variant (the output rows are the unit-stride direction of the source): 64x64
 // tiles through LDS so that both the FP64 reads and the 16-bit writes are coalesced.
 //   pass 1: per-row amax (as float bits, atomicMax; |x| >= 0 orders like its bit pattern)
@@ -160,7 +178,7 @@ __global__ __launch_bounds__(NT) void convert_tiles_kernel(ConvArgs a, int64_t R
   const int tr = threadIdx.x & 63, tg = threadIdx.x >> 6;
   const int64_t r0 = blockIdx.x * 64, c0 = blockIdx.y * 64, r = r0 + tr;
   if (r < R) {
-    float sc = ldexpf(1.f, scale_exp(__uint_as_float(bits[r]), a.target));
+    float sc = bits ? ldexpf(1.f, scale_exp(__uint_as_float(bits[r]), a.target)) : 1.f;
     int64_t r1 = r / a.R2, r2 = r - r1 * a.R2;
     const double* base = a.in + r1 * a.s[0] + r2 * a.s[1];
     for (int cc = tg; cc < 64; cc += 4) {
@@ -305,6 +323,70 @@ __global__ __launch_bounds__(NT) void assemble_t2_kernel(AssembleArgs g) {
   }
 }
 
+// The same assembly with one workgroup per pair {a, b} (a <= b), for h <= AH.  The four h x h
+// blocks Sij[(a,i),(b,j)], Sij[(b,i),(a,j)], Spp[(a,i),(b,j)], Spp[(b,i),(a,j)] are read once
+// as h rows of h contiguous doubles and staged in LDS, where the transposed (i <-> j) reads of
+// the antisymmetrisers are served; S0, Sab, t2, v_pphh, t2n are contiguous h x h blocks at
+// (a,b) and (b,a).  Every tensor is therefore read once with row-contiguous accesses (the flat
+// kernel above reads two of the four Sij/Spp terms at a stride of p*h doubles).
+// H(b,a) uses the same staged blocks: its Spp part is minus that of H(a,b).
+constexpr int AH = 32;
+
+__global__ __launch_bounds__(NT) void assemble_pair_kernel(AssembleArgs g) {
+  __shared__ double SiA[AH][AH + 1], SiB[AH][AH + 1], SpA[AH][AH + 1], SpB[AH][AH + 1];
+  const int64_t p = g.p, ph = p * g.h;
+  const int h = (int)g.h, hh = h * h;
+  double e_acc = 0.0, r_acc = 0.0;
+  auto finish = [&](int64_t a, int64_t b, int i, int j, int64_t t, double H) {
+    double denom = -(g.Xpp[a * p + a] + g.Xpp[b * p + b] + g.Xhh[i * h + i] + g.Xhh[j * h + j]);
+    double told = g.t2[t];
+    double tn = g.mixing * told + (1.0 - g.mixing) * (told + H / denom);
+    if (g.t2n) g.t2n[t] = tn;
+    if (g.H2out) g.H2out[t] = H;
+    if (g.vpphh) {
+      double v = g.vpphh[t];
+      e_acc += 0.25 * v * tn + 0.5 * v * g.t1n[a * h + i] * g.t1n[b * h + j];
+    }
+    r_acc += H * H;
+  };
+  for (int64_t q = blockIdx.x; q < p * p; q += gridDim.x) {
+    const int64_t a = q / p, b = q - a * p;
+    if (a > b) continue;                                    // uniform in the block
+    for (int e = threadIdx.x; e < hh; e += NT) {
+      const int i = e / h, j = e - i * h;
+      const int64_t oab = (a * g.h + i) * ph + b * g.h + j, oba = (b * g.h + i) * ph + a * g.h + j;
+      SiA[i][j] = g.Sij ? g.Sij[oab] : 0.0;
+      SiB[i][j] = g.Sij ? g.Sij[oba] : 0.0;
+      SpA[i][j] = g.Spp ? g.Spp[oab] : 0.0;
+      SpB[i][j] = g.Spp ? g.Spp[oba] : 0.0;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < hh; e += NT) {
+      const int i = e / h, j = e - i * h;
+      const int64_t tab = (a * p + b) * hh + e, tba = (b * p + a) * hh + e;
+      double Hab = g.S0 ? g.S0[tab] : 0.0;
+      double Hba = g.S0 ? g.S0[tba] : 0.0;
+      if (g.Sab) {
+        const double x = g.Sab[tab], y = g.Sab[tba];
+        Hab += x - y;
+        Hba += y - x;
+      }
+      Hab += SiA[i][j] - SiA[j][i];
+      Hba += SiB[i][j] - SiB[j][i];
+      const double sp = SpA[i][j] - SpB[i][j] - SpA[j][i] + SpB[j][i];
+      finish(a, b, i, j, tab, Hab + sp);
+      if (a != b) finish(b, a, i, j, tba, Hba - sp);
+    }
+    __syncthreads();
+  }
+  double es = block_sum(e_acc);
+  double rs = block_sum(r_acc);
+  if (threadIdx.x == 0) {
+    g.part[blockIdx.x] = es;
+    g.part[gridDim.x + blockIdx.x] = rs;
+  }
+}
+
 // Transpose of the assembly's antisymmetrisers: for the adjoint L of H2,
 //   gSab[a,b,i,j] = L[a,b,i,j] - L[b,a,i,j]                         (layout [a][b][i][j])
 //   gSij[a,i,b,j] = L[a,b,i,j] - L[a,b,j,i]                         (layout [a][i][b][j])
@@ -418,7 +500,9 @@ int nl_permute(int rank, const int64_t* out_shape, const int64_t* in_strides, do
 int nl_convert_rows(int dtype, int64_t R1, int64_t R2, int64_t C1, int64_t C2,
                     const int64_t* strides4, const double* in, void* out, int64_t ldo,
                     float* iscale, float target, void* stream) {
-  if ((dtype != 0 && dtype != 1) || !strides4 || !in || !out || !iscale) return NL_EARG;
+  const bool scaled = target > 0.f;
+  if ((dtype != 0 && dtype != 1) || !strides4 || !in || !out || (scaled && !iscale))
+    return NL_EARG;
   if (R1 < 0 || R2 < 1 || C1 < 0 || C2 < 1 || ldo < C1 * C2 || (ldo % 8)) return NL_EARG;
   if (R1 * R2 == 0) return NL_OK;
   if (R1 * R2 > INT32_MAX) return NL_EARG;
@@ -430,6 +514,12 @@ int nl_convert_rows(int dtype, int64_t R1, int64_t R2, int64_t C1, int64_t C2,
   const int64_t R = R1 * R2;
   if (a.s[1] == 1 && R2 >= 8 && a.s[3] != 1 && nl_cdiv(ldo, 64) <= 65535) {
     // rows are the unit-stride direction of the source: transpose through LDS tiles
+    if (!scaled) {
+      dim3 g2((unsigned)nl_cdiv(R, 64), (unsigned)nl_cdiv(ldo, 64));
+      hipLaunchKernelGGL(convert_tiles_kernel, g2, dim3(NT), 0, st, a, R, (const unsigned*)nullptr);
+      NL_LAUNCH_CHECK();
+      return NL_OK;
+    }
     unsigned* bits = (unsigned*)iscale;   // holds amax bits until the last pass
     hipLaunchKernelGGL(zero_u32_kernel, dim3(grid_for(R)), dim3(NT), 0, st, R, bits);
     NL_LAUNCH_CHECK();
@@ -440,6 +530,13 @@ int nl_convert_rows(int dtype, int64_t R1, int64_t R2, int64_t C1, int64_t C2,
     hipLaunchKernelGGL(convert_tiles_kernel, g2, dim3(NT), 0, st, a, R, (const unsigned*)bits);
     NL_LAUNCH_CHECK();
     hipLaunchKernelGGL(iscale_kernel, dim3(grid_for(R)), dim3(NT), 0, st, R, target, iscale);
+    NL_LAUNCH_CHECK();
+    return NL_OK;
+  }
+  if (!scaled) {
+    if (nl_cdiv(ldo, CAST_CHUNK) > 65535) return NL_EARG;
+    dim3 g((unsigned)R, (unsigned)nl_cdiv(ldo, CAST_CHUNK));
+    hipLaunchKernelGGL(cast_rows_kernel, g, dim3(NT), 0, st, a);
     NL_LAUNCH_CHECK();
     return NL_OK;
   }
@@ -518,8 +615,12 @@ int nl_assemble_t2(int64_t p, int64_t h, const double* S0, const double* Sab, co
   g.p = p; g.h = h; g.S0 = S0; g.Sab = Sab; g.Sij = Sij; g.Spp = Spp; g.Xpp = Xpp; g.Xhh = Xhh;
   g.t2 = t2; g.vpphh = vpphh; g.t1n = t1n; g.mixing = mixing; g.t2n = t2n; g.part = part;
   g.H2out = H2out;
-  hipLaunchKernelGGL(assemble_t2_kernel, dim3(grid_for(n, 4096)), dim3(NT), 0, (hipStream_t)stream,
-                     g);
+  if (h <= AH && (Sij || Spp))
+    hipLaunchKernelGGL(assemble_pair_kernel, dim3(grid_for(n, 4096)), dim3(NT), 0,
+                       (hipStream_t)stream, g);
+  else
+    hipLaunchKernelGGL(assemble_t2_kernel, dim3(grid_for(n, 4096)), dim3(NT), 0,
+                       (hipStream_t)stream, g);
   NL_LAUNCH_CHECK();
   return NL_OK;
 }

@@ -25,9 +25,10 @@ reference's t1Iter and t2Iter (coupled_cluster.py:342-545, ccDgrams.py) regroupe
 Every regrouping is an exact identity of the reference's sums; only floating-point summation
 order changes.
 
-Precision form: ``prec`` maps the families "ring", "hh", "dress" to "fp64", "bf16" or "fp16";
-low-precision contractions use BF16/FP16 MFMA with FP32 accumulation and per-row power-of-two
-scales, everything else (Fock, denominators, update, energy, DIIS) stays FP64.
+Precision form: ``prec`` maps the families "ring", "hh", "dress", "ladder" to "fp64", "bf16" or
+"fp16"; low-precision contractions use BF16/FP16 MFMA with FP32 accumulation (FP16 operands with
+per-row power-of-two scales, BF16 operands plainly rounded), everything else (Fock, denominators,
+update, energy, DIIS) stays FP64.
 """
 import ctypes
 import weakref
@@ -292,19 +293,22 @@ def _block_gemm(pb, src, out, alpha, ncol, tag, precision="fp64"):
                   ops._p(R), ncol, pb.mb * ncol, pb.nb, 1, ctypes.c_void_p(0), st)
     else:
         dt = ops.BF16 if precision == "bf16" else ops.FP16
-        target = 1.0 if dt == ops.BF16 else 2.0 ** 14
+        scaled = dt != ops.BF16         # BF16: plain cast, no per-row scales
+        target = 2.0 ** 14 if scaled else 0.0
         ldk = (pb.kb + 7) // 8 * 8
         lp = getattr(pb, "lp", None)
         if lp is None or lp[0] != dt:
             Vb = torch.empty(pb.nb * pb.mb * ldk, dtype=torch.int16, device=src.device)
-            iv = torch.empty(pb.nb * pb.mb, dtype=torch.float32, device=src.device)
+            iv = (torch.empty(pb.nb * pb.mb, dtype=torch.float32, device=src.device) if scaled
+                  else None)
             s4 = (ctypes.c_int64 * 4)(pb.mb * pb.kb, pb.kb, 0, 1)
             _lib.call("nl_convert_rows", dt, pb.nb, pb.mb, 1, pb.kb, s4, ops._p(pb.V),
                       ops._p(Vb), ldk, ops._p(iv), ctypes.c_float(target), st)
             pb.lp = lp = (dt, Vb, iv)
         _, Vb, iv = lp
         Gt = ops.WS.get(tag + "_Gt", pb.nb * ncol * ldk, dtype=torch.int16, device=src.device)
-        ig = ops.WS.get(tag + "_Gs", pb.nb * ncol, dtype=torch.float32, device=src.device)
+        ig = (ops.WS.get(tag + "_Gs", pb.nb * ncol, dtype=torch.float32, device=src.device)
+              if scaled else None)
         s4 = (ctypes.c_int64 * 4)(pb.kb * ncol, 1, 0, ncol)
         _lib.call("nl_convert_rows", dt, pb.nb, ncol, 1, pb.kb, s4, ops._p(G), ops._p(Gt), ldk,
                   ops._p(ig), ctypes.c_float(target), st)

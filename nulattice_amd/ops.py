@@ -356,7 +356,8 @@ def forget_ptrs(ptrs):
 
 
 def to_lp_rows(t, labels, rows, cols, dtype, tag):
-    """Convert view t (rows x cols) to a 16-bit [R][ldk] buffer with per-row scales."""
+    """Convert view t (rows x cols) to a 16-bit [R][ldk] buffer: FP16 with per-row power-of-two
+    scales (returned as inverse scales), BF16 by a plain cast (inverse scales None)."""
     key = None
     if t.data_ptr() in CONST_PTRS:
         key = (t.data_ptr(), tuple(t.shape), tuple(t.stride()), labels, tuple(rows), tuple(cols),
@@ -371,13 +372,14 @@ def to_lp_rows(t, labels, rows, cols, dtype, tag):
     R1, R2, C1, C2, st = v
     R, Ck = R1 * R2, C1 * C2
     ldk = (Ck + 7) // 8 * 8
+    scaled = dtype != BF16          # BF16 has FP32's exponent range: a plain cast suffices
     if key is not None:
         buf = torch.empty(R * ldk, dtype=torch.int16, device=t.device)
-        isc = torch.empty(R, dtype=torch.float32, device=t.device)
+        isc = torch.empty(R, dtype=torch.float32, device=t.device) if scaled else None
     else:
         buf = WS.get(tag, R * ldk, dtype=torch.int16, device=t.device)
-        isc = WS.get(tag + "_s", R, dtype=torch.float32, device=t.device)
-    target = 1.0 if dtype == BF16 else 2.0 ** 14
+        isc = WS.get(tag + "_s", R, dtype=torch.float32, device=t.device) if scaled else None
+    target = 2.0 ** 14 if scaled else 0.0
     s4 = (ctypes.c_int64 * 4)(*st)
     _lib.call("nl_convert_rows", dtype, R1, R2, C1, C2, s4, ctypes.c_void_p(t.data_ptr()),
               _p(buf), ldk, _p(isc), ctypes.c_float(target), _stream())

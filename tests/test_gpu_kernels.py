@@ -41,6 +41,11 @@ SPECS = [
     ("ab,bc->ac", (20, 3000), (3000, 900)),
     ("bmk,bkn->bmn", (4, 700, 33), (4, 33, 9)),
     ("mk,nk->mn", (1000, 50), (64, 50)),
+    # small outputs: 4x4-blocked kernel with a wavefront split-K reduction (>= 64 splits), the
+    # general small-output kernel (M, N in (32, 64]), strided small outputs
+    ("ki,kj->ij", (20000, 7), (20000, 5)),
+    ("ki,kj->ij", (3000, 40), (3000, 50)),
+    ("abkl,abki->li", (40, 30, 6, 5), (40, 30, 6, 7)),
     # no summed index (outer products of the adjoint sweep)
     ("ai,ck->acik", (9, 4), (9, 4)),
     ("dl,ai->dali", (9, 4), (9, 4)),
