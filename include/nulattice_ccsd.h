@@ -56,6 +56,16 @@ int nl_gemm_lp(int dtype, int64_t M, int64_t N, int64_t K, double alpha, const v
                const double* D, double* C, int64_t ldc, int64_t strideC, int64_t batch,
                int64_t splits, double* ws, void* stream);
 
+/* nl_gemm_lp (batch 1) with C and D addressed like nl_contract_f64's output: cdims = {M1, M2,
+ * N1, N2} (M = M1*M2, N = N1*N2), sc = {m1, m2, n1, n2} strides, C(m,n) at
+ * C[(m/M2)*sc0 + (m%M2)*sc1 + (n/N2)*sc2 + (n%N2)*sc3].  Lets the precision form accumulate
+ * straight into strided T2 accumulators (the antisymmetriser groups of t2Iter,
+ * coupled_cluster.py:476-536) without a row-major staging copy. */
+int nl_gemm_lp_strided(int dtype, int64_t K, double alpha, const void* A, int64_t lda,
+                       const float* ia, const void* B, int64_t ldb, const float* ib, double beta,
+                       const double* D, double* C, const int64_t* cdims, const int64_t* sc,
+                       int64_t splits, double* ws, void* stream);
+
 /* out (contiguous, out_shape) = alpha * in[sum_d idx_d * in_strides[d]] + beta * out.
  * Replaces the transposes inside pAB/pIJ (ccDgrams.py:193-213) and opt_einsum's operand
  * permutations. */

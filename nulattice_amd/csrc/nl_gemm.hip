@@ -508,6 +508,74 @@ __global__ __launch_bounds__(256) void splitk_reduce_wave_kernel(int M, int N, i
   }
 }
 
+// Low-K products (K <= 32 with M, N > 64: outer-product-like accumulations into T2-sized
+// outputs).  Bandwidth lies in reading and writing C, so a workgroup stages the whole K of a 64x64
+// output tile's A and B slices in LDS once, each thread owns a 4x4 block of outputs spaced 16
+// apart, and the lanes run along C's unit-stride index (LN: n, else m), so the read-modify-write
+// of C moves in 128-byte runs.
+constexpr int LKT = 64, LKK = 32;
+
+template <bool LN>
+__global__ __launch_bounds__(256) void lowk_f64_kernel(GemmF64Args g) {
+  __shared__ double As[LKK][LKT + 1];
+  __shared__ double Bs[LKK][LKT + 1];
+  const int tid = threadIdx.x;
+  const int batch = blockIdx.z;
+  TileCoord tc = tile_of(blockIdx.x, gridDim.x, g.tiles_m, g.tiles_n);
+  const int m0 = tc.tm * LKT, n0 = tc.tn * LKT;
+  const double* A = g.A + off2(batch, g.ab);
+  const double* B = g.B + off2(batch, g.bb);
+  const int K = g.K;
+  // operand tiles: along k when that is the operand's unit stride, else along m / n
+  const bool ak1 = g.ak.n2 > 1 ? g.ak.s2 == 1 : g.ak.s1 == 1;
+  const bool bk1 = g.bk.n2 > 1 ? g.bk.s2 == 1 : g.bk.s1 == 1;
+  for (int q = tid; q < K * LKT; q += 256) {
+    int k, m;
+    if (ak1) { m = q / K; k = q - m * K; } else { k = q / LKT; m = q - k * LKT; }
+    As[k][m] = (m0 + m < g.M) ? A[off2(m0 + m, g.am) + off2(k, g.ak)] : 0.0;
+    int n;
+    if (bk1) { n = q / K; k = q - n * K; } else { k = q / LKT; n = q - k * LKT; }
+    Bs[k][n] = (n0 + n < g.N) ? B[off2(k, g.bk) + off2(n0 + n, g.bn)] : 0.0;
+  }
+  __syncthreads();
+  const int tm = LN ? (tid >> 4) : (tid & 15), tn = LN ? (tid & 15) : (tid >> 4);
+  double acc[4][4];
+#pragma unroll
+  for (int x = 0; x < 4; ++x)
+#pragma unroll
+    for (int y = 0; y < 4; ++y) acc[x][y] = 0.0;
+  for (int k = 0; k < K; ++k) {
+    double a[4], b[4];
+#pragma unroll
+    for (int x = 0; x < 4; ++x) { a[x] = As[k][tm + 16 * x]; b[x] = Bs[k][tn + 16 * x]; }
+#pragma unroll
+    for (int x = 0; x < 4; ++x)
+#pragma unroll
+      for (int y = 0; y < 4; ++y) acc[x][y] = fma(a[x], b[y], acc[x][y]);
+  }
+  const int64_t cb = off2(batch, g.cb);
+  int64_t coff[4];
+#pragma unroll
+  for (int y = 0; y < 4; ++y) {
+    const int col = n0 + tn + 16 * y;
+    coff[y] = col < g.N ? off2(col, g.cn) : 0;
+  }
+#pragma unroll
+  for (int x = 0; x < 4; ++x) {
+    const int row = m0 + tm + 16 * x;
+    if (row >= g.M) continue;
+    const int64_t ro = cb + off2(row, g.cm);
+#pragma unroll
+    for (int y = 0; y < 4; ++y) {
+      if (n0 + tn + 16 * y >= g.N) continue;
+      const int64_t o = ro + coff[y];
+      double v = g.alpha * acc[x][y];
+      if (g.D && g.beta != 0.0) v += g.beta * g.D[o];
+      g.C[o] = v;
+    }
+  }
+}
+
 // C[b](m,n) = sum_s ws[b][s][m][n] + beta * D[b](m,n)
 __global__ void splitk_reduce_strided_kernel(int M, int N, int splits, int nbatch,
                                              const double* ws, double beta, const double* D,
@@ -527,23 +595,6 @@ __global__ void splitk_reduce_strided_kernel(int M, int N, int splits, int nbatc
   }
 }
 
-__global__ void splitk_reduce_kernel(int M, int N, int splits, int batch, const double* ws,
-                                     double beta, const double* D, double* C, int64_t ldc,
-                                     int64_t sC) {
-  int64_t total = (int64_t)batch * M * N;
-  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
-       t += (int64_t)gridDim.x * blockDim.x) {
-    int64_t b = t / ((int64_t)M * N);
-    int64_t mn = t - b * (int64_t)M * N;
-    int64_t m = mn / N, n = mn - m * N;
-    const double* W = ws + b * splits * (int64_t)M * N + mn;
-    double s = 0.0;
-    for (int q = 0; q < splits; ++q) s += W[(int64_t)q * M * N];
-    int64_t o = b * sC + m * ldc + n;
-    if (D && beta != 0.0) s += beta * D[o];
-    C[o] = s;
-  }
-}
 
 // ---------------------------------------------------------------------------------- BF16/FP16
 constexpr int BKL = 64;
@@ -556,7 +607,7 @@ struct GemmLPArgs {
   const uint16_t* B; int64_t ldb, sB;   // [N][ldb] K-contiguous
   const float* ia; int64_t sia;         // per-row inverse scale of A (or null)
   const float* ib; int64_t sib;         // per-row inverse scale of B (or null)
-  double* C; int64_t ldc, sC;
+  double* C; Idx2 cm, cn; int64_t sC;   // C(m,n) = C[off2(m,cm) + off2(n,cn)] (+ batch * sC)
   const double* D;                      // addend with C's layout (may be C or null)
   int tiles_m, tiles_n, splits, kchunk;
   double* ws;                           // split-K partials [batch][splits][M][N]
@@ -646,10 +697,15 @@ __global__ __launch_bounds__(256) void gemm_lp_kernel(GemmLPArgs g) {
   double* C = partial ? g.ws + ((int64_t)batch * g.splits + split) * (int64_t)g.M * g.N
                       : g.C + (int64_t)batch * g.sC;
   const double* Dp = (partial || !g.D) ? nullptr : g.D + (int64_t)batch * g.sC;
-  const int64_t ldc = partial ? g.N : g.ldc;
   const double beta = partial ? 0.0 : g.beta;
   const float* ia = g.ia ? g.ia + (int64_t)batch * g.sia : nullptr;
   const float* ib = g.ib ? g.ib + (int64_t)batch * g.sib : nullptr;
+  int64_t coff[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    int col = n0 + wn * 64 + j * 16 + (lane & 15);
+    coff[j] = col < g.N ? (partial ? col : off2(col, g.cn)) : 0;
+  }
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -657,12 +713,13 @@ __global__ __launch_bounds__(256) void gemm_lp_kernel(GemmLPArgs g) {
       int row = m0 + wm * 64 + i * 16 + (lane >> 4) * 4 + r;
       if (row >= g.M) continue;
       double sr = g.alpha * (ia ? (double)ia[row] : 1.0);
+      const int64_t roff = partial ? (int64_t)row * g.N : off2(row, g.cm);
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         int col = n0 + wn * 64 + j * 16 + (lane & 15);
         if (col < g.N) {
           double v = sr * (double)acc[i][j][r] * (ib ? (double)ib[col] : 1.0);
-          const int64_t o = (int64_t)row * ldc + col;
+          const int64_t o = roff + coff[j];
           if (Dp && beta != 0.0) v += beta * Dp[o];
           C[o] = v;
         }
@@ -789,15 +846,16 @@ __global__ __launch_bounds__(512) void gemm_lp_big_kernel(GemmLPArgs g) {
   double* C = partial ? g.ws + ((int64_t)batch * g.splits + split) * (int64_t)g.M * g.N
                       : g.C + (int64_t)batch * g.sC;
   const double* Dp = (partial || !g.D) ? nullptr : g.D + (int64_t)batch * g.sC;
-  const int64_t ldc = partial ? g.N : g.ldc;
   const double beta = partial ? 0.0 : g.beta;
   const float* ia = g.ia ? g.ia + (int64_t)batch * g.sia : nullptr;
   const float* ib = g.ib ? g.ib + (int64_t)batch * g.sib : nullptr;
   double sc[4];
+  int64_t coff[4];
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
     int col = n0 + wn * 64 + j * 16 + (lane & 15);
     sc[j] = (ib && col < g.N) ? (double)ib[col] : 1.0;
+    coff[j] = col < g.N ? (partial ? col : off2(col, g.cn)) : 0;
   }
 #pragma unroll
   for (int i = 0; i < 8; ++i)
@@ -806,12 +864,13 @@ __global__ __launch_bounds__(512) void gemm_lp_big_kernel(GemmLPArgs g) {
       int row = m0 + wm * 128 + i * 16 + (lane >> 4) * 4 + r;
       if (row >= g.M) continue;
       double sr = g.alpha * (ia ? (double)ia[row] : 1.0);
+      const int64_t roff = partial ? (int64_t)row * g.N : off2(row, g.cm);
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         int col = n0 + wn * 64 + j * 16 + (lane & 15);
         if (col < g.N) {
           double v = sr * (double)acc[i][j][r] * sc[j];
-          const int64_t o = (int64_t)row * ldc + col;
+          const int64_t o = roff + coff[j];
           if (Dp && beta != 0.0) v += beta * Dp[o];
           C[o] = v;
         }
@@ -831,7 +890,10 @@ static inline Idx2 mk(int64_t n2, int64_t s1, int64_t s2) {
 // workspace doubles nl_contract_f64 needs (0 if no split).
 static void contract_plan(int64_t M, int64_t N, int64_t K, int64_t nb, int* kind, int64_t* splits) {
   int64_t tiles;
-  if (N <= 64 && M <= 64) {
+  if (K <= LKK && M > 64 && N > 64) {
+    *kind = 3;   // low K: 64x64 output tiles, whole K in LDS, lanes along C's unit stride
+    *splits = 1;
+  } else if (N <= 64 && M <= 64) {
     *kind = 2;   // small output: split K widely
     int64_t s = K <= 256 ? 1 : std::min<int64_t>(2048 / std::max<int64_t>(nb, 1), K / 256);
     *splits = std::max<int64_t>(1, std::min<int64_t>(s, 4096));
@@ -926,6 +988,17 @@ int nl_contract_f64(const int64_t* dims, double alpha, const double* A, const in
     }
     return NL_OK;
   }
+  if (kind == 3) {
+    g.tiles_m = (int)nl_cdiv(M, LKT); g.tiles_n = (int)nl_cdiv(N, LKT);
+    g.splits = 1;
+    const bool ln = (g.cn.n2 > 1 ? g.cn.s2 == 1 : g.cn.s1 == 1) ||
+                    !(g.cm.n2 > 1 ? g.cm.s2 == 1 : g.cm.s1 == 1);
+    dim3 lg((unsigned)(g.tiles_m * g.tiles_n), 1u, (unsigned)nb);
+    if (ln) hipLaunchKernelGGL((lowk_f64_kernel<true>), lg, dim3(256), 0, st, g);
+    else    hipLaunchKernelGGL((lowk_f64_kernel<false>), lg, dim3(256), 0, st, g);
+    NL_LAUNCH_CHECK();
+    return NL_OK;
+  }
   if (kind == 1) {
     if (N > 64) {   // M is the small side: C^T = B^T A^T
       std::swap(g.M, g.N); std::swap(g.tiles_m, g.tiles_n);
@@ -987,12 +1060,14 @@ int nl_gemm_f64(int a_kcontig, int b_ncontig, int64_t M, int64_t N, int64_t K, d
 
 // C[b] (FP64, row-major) = alpha * diag(ia) * (A[b] * B[b]^T) * diag(ib) + beta * C[b]
 // A: [M][lda], B: [N][ldb], 16-bit elements (dtype 0 = bf16, 1 = fp16), K-contiguous rows.
+// Shared body of the two LP entry points; C/D addressed through (cm, cn) and batch stride sC.
 // K, lda, ldb must be multiples of 8 and A, B 16-byte aligned.
-int nl_gemm_lp(int dtype, int64_t M, int64_t N, int64_t K, double alpha, const void* A,
-               int64_t lda, int64_t strideA, const float* ia, int64_t strideIa, const void* B,
-               int64_t ldb, int64_t strideB, const float* ib, int64_t strideIb, double beta,
-               const double* D, double* C, int64_t ldc, int64_t strideC, int64_t batch,
-               int64_t splits, double* ws, void* stream) {
+static int gemm_lp_impl(int dtype, int64_t M, int64_t N, int64_t K, double alpha, const void* A,
+                        int64_t lda, int64_t strideA, const float* ia, int64_t strideIa,
+                        const void* B, int64_t ldb, int64_t strideB, const float* ib,
+                        int64_t strideIb, double beta, const double* D, double* C, Idx2 cm,
+                        Idx2 cn, int64_t strideC, int64_t batch, int64_t splits, double* ws,
+                        void* stream) {
   if (M < 0 || N < 0 || K < 0 || batch < 0 || (dtype != 0 && dtype != 1)) return NL_EARG;
   if (M == 0 || N == 0 || batch == 0) return NL_OK;
   if (!A || !B || !C) return NL_EARG;
@@ -1001,22 +1076,18 @@ int nl_gemm_lp(int dtype, int64_t M, int64_t N, int64_t K, double alpha, const v
     return NL_EARG;
   if (M > INT32_MAX || N > INT32_MAX || K > INT32_MAX || batch > 65535) return NL_EARG;
   if (splits < 1 || splits > 65535 || (splits > 1 && !ws)) return NL_EARG;
+  if (K == 0) return NL_EARG;   // nothing to accumulate: no caller needs C = beta * C here
   GemmLPArgs g;
   g.M = (int)M; g.N = (int)N; g.K = (int)K; g.alpha = alpha; g.beta = beta;
   g.A = (const uint16_t*)A; g.lda = lda; g.sA = strideA;
   g.B = (const uint16_t*)B; g.ldb = ldb; g.sB = strideB;
   g.ia = ia; g.sia = strideIa; g.ib = ib; g.sib = strideIb;
-  g.C = C; g.ldc = ldc; g.sC = strideC; g.D = D;
+  g.C = C; g.cm = cm; g.cn = cn; g.sC = strideC; g.D = D;
   g.tiles_m = (int)nl_cdiv(M, TM); g.tiles_n = (int)nl_cdiv(N, TN);
   g.splits = (int)splits; g.ws = ws;
   g.kchunk = (int)(nl_cdiv(nl_cdiv(K, splits), BKL) * BKL);   // multiple of 64 (hence of 8)
   dim3 grid((unsigned)(g.tiles_m * g.tiles_n), (unsigned)splits, (unsigned)batch);
   hipStream_t st = (hipStream_t)stream;
-  if (K == 0) {
-    // nothing to accumulate: C = beta * C via a degenerate FP64 call path is not needed by
-    // any caller; reject to keep the contract simple.
-    return NL_EARG;
-  }
   // 256x256 tiles when they alone give every CU a workgroup, else 128x128 tiles
   const int64_t big_tiles = nl_cdiv(M, BT) * nl_cdiv(N, BT);
   if (M >= BT && N >= BT && big_tiles * splits * batch >= 256) {
@@ -1033,13 +1104,34 @@ int nl_gemm_lp(int dtype, int64_t M, int64_t N, int64_t K, double alpha, const v
   }
   NL_LAUNCH_CHECK();
   if (splits > 1) {
-    int64_t total = batch * M * N;
-    int blocks = (int)std::min<int64_t>(nl_cdiv(total, 256), 8192);
-    hipLaunchKernelGGL(splitk_reduce_kernel, dim3(blocks), dim3(256), 0, st, (int)M, (int)N,
-                       (int)splits, (int)batch, (const double*)ws, beta, D, C, ldc, strideC);
+    reduce_splits(st, (int)M, (int)N, (int)splits, (int)batch, ws, beta, D, C, cm, cn,
+                  mk(1, strideC, 0));
     NL_LAUNCH_CHECK();
   }
   return NL_OK;
+}
+
+int nl_gemm_lp(int dtype, int64_t M, int64_t N, int64_t K, double alpha, const void* A,
+               int64_t lda, int64_t strideA, const float* ia, int64_t strideIa, const void* B,
+               int64_t ldb, int64_t strideB, const float* ib, int64_t strideIb, double beta,
+               const double* D, double* C, int64_t ldc, int64_t strideC, int64_t batch,
+               int64_t splits, double* ws, void* stream) {
+  return gemm_lp_impl(dtype, M, N, K, alpha, A, lda, strideA, ia, strideIa, B, ldb, strideB, ib,
+                      strideIb, beta, D, C, mk(1, ldc, 0), mk(1, 1, 0), strideC, batch, splits,
+                      ws, stream);
+}
+
+int nl_gemm_lp_strided(int dtype, int64_t K, double alpha, const void* A, int64_t lda,
+                       const float* ia, const void* B, int64_t ldb, const float* ib, double beta,
+                       const double* D, double* C, const int64_t* cdims, const int64_t* sc,
+                       int64_t splits, double* ws, void* stream) {
+  if (!cdims || !sc) return NL_EARG;
+  for (int d = 0; d < 4; ++d)
+    if (cdims[d] < 1) return NL_EARG;
+  if (cdims[1] > INT32_MAX || cdims[3] > INT32_MAX) return NL_EARG;
+  return gemm_lp_impl(dtype, cdims[0] * cdims[1], cdims[2] * cdims[3], K, alpha, A, lda, 0, ia, 0,
+                      B, ldb, 0, ib, 0, beta, D, C, mk(cdims[1], sc[0], sc[1]),
+                      mk(cdims[3], sc[2], sc[3]), 0, 1, splits, ws, stream);
 }
 
 }  // extern "C"

@@ -388,6 +388,10 @@ def to_lp_rows(t, labels, rows, cols, dtype, tag):
     return buf, isc, ldk
 
 
+def _sizes(t, labels):
+    return {c: int(n) for c, n in zip(labels, t.shape)}
+
+
 def _rowmajor(t, labels, Mi, Ni):
     """ldc if t seen as [M][N] (M = Mi merged, N = Ni merged and unit-stride), else None."""
     if _splits_at(t, labels, Mi) or _splits_at(t, labels, Ni):
@@ -405,21 +409,32 @@ def _contract_lp(A, la, B, lb, out, lo, Mi, Ni, Ki, M, N, K, alpha, beta, precis
     Ab, ia, lda = to_lp_rows(A, la, Mi, Ki, dt, "lpA")
     Bb, ib, ldb = to_lp_rows(B, lb, Ni, Ki, dt, "lpB")
     ldc = _rowmajor(out, lo, Mi, Ni)
-    direct = ldc is not None
-    if not direct:
-        C = WS.get("contract_out", M * N, device=out.device).view(M, N)
-        ldc = N
-    else:
-        C = out
+    runs = None
+    if ldc is None:
+        # the output as <= 2 stride runs per group (in the group orders the operands used)
+        rm = _group_runs(Mi, [(out, lo)], _sizes(out, lo))
+        rn = _group_runs(Ni, [(out, lo)], _sizes(out, lo))
+        if rm is not None and rn is not None:
+            runs = (rm, rn)
+    direct = ldc is not None or runs is not None
+    C = out if direct else WS.get("contract_out", M * N, device=out.device).view(M, N)
     D = (addend if addend is not None else out) if (direct and beta != 0.0) else None
     splits = _splits(M, N, lda, 1)
     ws = WS.get("splitk", splits * M * N, device=out.device) if splits > 1 else None
     ev = _timed(tag)
     if ev:
         ev[0].record()
-    _lib.call("nl_gemm_lp", dt, M, N, lda, ctypes.c_double(alpha), _p(Ab), lda, 0, _p(ia), 0,
-              _p(Bb), ldb, 0, _p(ib), 0, ctypes.c_double(beta if direct else 0.0), _p(D), _p(C),
-              max(ldc, 1), 0, 1, splits, _p(ws), _stream())
+    if runs is not None:
+        (m1, m2, (sm,)), (n1, n2, (sn,)) = runs
+        arr = lambda v: (ctypes.c_int64 * 4)(*[int(x) for x in v])
+        _lib.call("nl_gemm_lp_strided", dt, lda, ctypes.c_double(alpha), _p(Ab), lda, _p(ia),
+                  _p(Bb), ldb, _p(ib), ctypes.c_double(beta), _p(D), _p(C),
+                  arr([m1, m2, n1, n2]), arr([sm[0], sm[1], sn[0], sn[1]]), splits, _p(ws),
+                  _stream())
+    else:
+        _lib.call("nl_gemm_lp", dt, M, N, lda, ctypes.c_double(alpha), _p(Ab), lda, 0, _p(ia), 0,
+                  _p(Bb), ldb, 0, _p(ib), 0, ctypes.c_double(beta if direct else 0.0), _p(D),
+                  _p(C), max(ldc if ldc is not None else N, 1), 0, 1, splits, _p(ws), _stream())
     if ev:
         ev[1].record()
     if not direct:

@@ -46,6 +46,11 @@ SPECS = [
     ("ki,kj->ij", (20000, 7), (20000, 5)),
     ("ki,kj->ij", (3000, 40), (3000, 50)),
     ("abkl,abki->li", (40, 30, 6, 5), (40, 30, 6, 7)),
+    # low K (<= 32) with large M and N: lanes along n, along m (unit-stride j of A's group),
+    # and an operand layout that needs the copy fallback
+    ("mk,kn->mn", (500, 16), (16, 700)),
+    ("cjkl,bl->ckbj", (70, 6, 5, 20), (90, 20)),
+    ("ak,bkij->aibj", (300, 20), (200, 20, 6, 5)),
     # no summed index (outer products of the adjoint sweep)
     ("ai,ck->acik", (9, 4), (9, 4)),
     ("dl,ai->dali", (9, 4), (9, 4)),
