@@ -878,6 +878,226 @@ __global__ __launch_bounds__(512) void gemm_lp_big_kernel(GemmLPArgs g) {
     }
 }
 
+// ---------------------------------------------------------------------------------- FP64, large
+// 256x128 FP64 tiles for the large contractions (the ring products of the plain form, the hh
+// ladder product): 8 waves of 64x64 (4x4 v_mfma_f64_16x16x4f64 tiles each, two waves per SIMD).
+// K advances in steps of 16, staged by LDS DMA (global_load_lds_dwordx4: 2 doubles per lane, no
+// register staging) into two distinct LDS buffers, so step s+1's loads run under step s's MFMAs.
+// Each operand tile is held along the operand's unit-stride index:
+//   k rows  (unit stride in k):        [row][16 k], 128-byte rows, 16-byte chunk c stored at
+//                                      position c ^ ((row >> 1) & 7) of its row;
+//   k-major (unit stride in m or n):   [k][rows], chunk c of k-row kr stored at c ^ ((kr & 1) << 3).
+// With 64 banks of 4 bytes, both let the 32 lanes of a fragment read half (16 rows x 2 k) hit 32
+// distinct 8-byte bank pairs.  The host checks that every 16-byte chunk is contiguous and aligned.
+// Rows beyond M or N keep stale LDS data, which reaches only outputs that are not stored; k beyond
+// the range is zero-filled in both operands.
+constexpr int FTM = 256, FTN = 128, FBK = 16;
+constexpr int FA_ELEMS = FTM * FBK, FB_ELEMS = FTN * FBK;
+using F64Tile = double[FA_ELEMS + FB_ELEMS];   // A tile, then B tile, of one K step
+
+__device__ inline void lds_dma16d(const double* g, double* lds) {
+  __builtin_amdgcn_global_load_lds(g, (__attribute__((address_space(3))) void*)lds, 16, 0, 0);
+}
+
+// LDS index of (row, k) in a k-rows tile, and in a k-major tile of ROWS rows.
+__device__ inline int krows_at(int r, int k) {
+  return r * FBK + ((((k >> 1) ^ ((r >> 1) & 7)) << 1) | (k & 1));
+}
+template <int ROWS>
+__device__ inline int kmajor_at(int r, int k) {
+  return k * ROWS + ((((r >> 1) ^ ((k & 1) << 3)) << 1) | (r & 1));
+}
+
+// Wave w's share of one operand tile per K step: ROWS/64 DMA instructions of 1 KB covering rows
+// [r0, r0 + ROWS) of an operand with nrows rows (index dr) and index k (dk).  Row offsets are
+// fixed per lane; the k offsets come from two walkers that advance by FBK per step along the
+// (possibly two-run) k index, so the loop carries no integer division.
+template <bool KROWS, int ROWS>
+struct F64Fill {
+  static constexpr int NI = ROWS / 64;             // instructions per wave and step
+  static constexpr int PER_K = ROWS / 128;         // k-major: instructions per k-row
+  int64_t roff[NI];
+  bool rin[NI];
+  int wk[2], wr[2];                                // walker: k, position in its run
+  int64_t woff[2];                                 // walker: offset of k
+  __device__ inline F64Fill(const Idx2& dr, const Idx2& dk, int r0, int nrows, int k0, int w,
+                            int lane) {
+#pragma unroll
+    for (int q = 0; q < NI; ++q) {
+      const int ins = w * NI + q;
+      int r;
+      if (KROWS) r = ins * 8 + (lane >> 3);
+      else r = 2 * (((ins % PER_K) * 64 + lane) ^ (((ins / PER_K) & 1) << 3));
+      rin[q] = r0 + r < nrows;
+      roff[q] = rin[q] ? off2(r0 + r, dr) : 0;
+    }
+#pragma unroll
+    for (int v = 0; v < 2; ++v) {
+      int k;
+      if (KROWS) {
+        const int r = (w * NI + v) * 8 + (lane >> 3);
+        k = k0 + 2 * ((lane & 7) ^ ((r >> 1) & 7));
+      } else {
+        k = k0 + (w * NI + v * PER_K) / PER_K;
+      }
+      const int qk = k / dk.n2;
+      wk[v] = k; wr[v] = k - qk * dk.n2;
+      woff[v] = qk * dk.s1 + (int64_t)wr[v] * dk.s2;
+    }
+  }
+  __device__ inline void issue(double* T, const double* P, const Idx2& dk, int kend, int w,
+                               int lane) {
+#pragma unroll
+    for (int q = 0; q < NI; ++q) {
+      const int ins = w * NI + q;
+      const int v = KROWS ? (q & 1) : q / PER_K;
+      if (KROWS) {
+        const int rb = ins * 8;
+        if (wk[v] < kend) {
+          if (rin[q]) lds_dma16d(P + roff[q] + woff[v], T + rb * FBK);
+        } else {
+          double* z = T + (rb + (lane >> 3)) * FBK + 2 * (lane & 7);
+          z[0] = 0.0; z[1] = 0.0;
+        }
+      } else {
+        const int kr = ins / PER_K, part = ins % PER_K;
+        if (wk[v] < kend) {
+          if (rin[q]) lds_dma16d(P + woff[v] + roff[q], T + kr * ROWS + part * 128);
+        } else {
+          double* z = T + kr * ROWS + 2 * (part * 64 + lane);
+          z[0] = 0.0; z[1] = 0.0;
+        }
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < 2; ++v) {
+      wk[v] += FBK;
+      if (dk.n2 == 1) {
+        woff[v] += FBK * dk.s1;
+      } else {
+        wr[v] += FBK; woff[v] += FBK * dk.s2;
+        while (wr[v] >= dk.n2) { wr[v] -= dk.n2; woff[v] += dk.s1 - (int64_t)dk.n2 * dk.s2; }
+      }
+    }
+  }
+};
+
+// One K step (16) of this wave's 64x64 piece from LDS buffer T.
+template <bool AK, bool BN>
+__device__ inline void f64_step(const F64Tile& T, v4d (&acc)[4][4], int wm, int wn, int lane) {
+  const double* TA = T;
+  const double* TB = T + FA_ELEMS;
+#pragma unroll
+  for (int kk = 0; kk < FBK / 4; ++kk) {
+    const int k = kk * 4 + (lane >> 4);
+    double a[4], b[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int r = wm * 64 + i * 16 + (lane & 15);
+      a[i] = TA[AK ? krows_at(r, k) : kmajor_at<FTM>(r, k)];
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int r = wn * 64 + j * 16 + (lane & 15);
+      b[j] = TB[BN ? kmajor_at<FTN>(r, k) : krows_at(r, k)];
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+  }
+}
+
+template <bool AK, bool BN>
+__global__ __launch_bounds__(512) void gemm_f64_big_kernel(GemmF64Args g) {
+  // two distinct LDS objects, so the compiler can tell the buffer being read from the buffer
+  // that the in-flight DMA is filling
+  __shared__ __attribute__((aligned(1024))) F64Tile S0;
+  __shared__ __attribute__((aligned(1024))) F64Tile S1;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int wm = w >> 1, wn = w & 1;
+  const int batch = blockIdx.z, split = blockIdx.y;
+  TileCoord tc = tile_of(blockIdx.x, gridDim.x, g.tiles_m, g.tiles_n);
+  const int m0 = tc.tm * FTM, n0 = tc.tn * FTN;
+  const double* A = g.A + off2(batch, g.ab);
+  const double* B = g.B + off2(batch, g.bb);
+  const int kbeg = split * g.kchunk;
+  const int kend = min(g.K, kbeg + g.kchunk);
+
+  v4d acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
+
+  F64Fill<AK, FTM> fa(g.am, g.ak, m0, g.M, kbeg, w, lane);
+  F64Fill<!BN, FTN> fb(g.bn, g.bk, n0, g.N, kbeg, w, lane);
+  // successive calls fill successive K steps starting at kbeg
+  auto issue = [&](F64Tile& T, int) {
+    fa.issue(T, A, g.ak, kend, w, lane);
+    fb.issue(T + FA_ELEMS, B, g.bk, kend, w, lane);
+  };
+
+  // Steps alternate S0, S1 in pairs (an odd step count gets one zero-filled step, so the loop
+  // has a single exit).  At the top of a step: wait for this wave's loads, a barrier (everyone's
+  // loads of this step landed, everyone's reads of the other buffer are done), then start the
+  // next step's loads into the other buffer and compute this one.
+  const int npairs = kbeg < kend ? (kend - kbeg + 2 * FBK - 1) / (2 * FBK) : 0;
+  if (npairs > 0) issue(S0, kbeg);
+  for (int p = 0; p < npairs; ++p) {
+    const int k0 = kbeg + p * 2 * FBK;
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    issue(S1, k0 + FBK);
+    f64_step<AK, BN>(S0, acc, wm, wn, lane);
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (p + 1 < npairs) issue(S0, k0 + 2 * FBK);
+    f64_step<AK, BN>(S1, acc, wm, wn, lane);
+  }
+
+  // f64 16x16x4 accumulator layout: register r of lane l holds D[4r + l/16][l%16]
+  if (g.splits > 1) {
+    double* W = g.ws + ((int64_t)batch * g.splits + split) * (int64_t)g.M * g.N;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          int row = m0 + wm * 64 + i * 16 + r * 4 + (lane >> 4);
+          int col = n0 + wn * 64 + j * 16 + (lane & 15);
+          if (row < g.M && col < g.N) W[(int64_t)row * g.N + col] = g.alpha * acc[i][j][r];
+        }
+    return;
+  }
+  const int64_t cb = off2(batch, g.cb);
+  int64_t coff[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    int col = n0 + wn * 64 + j * 16 + (lane & 15);
+    coff[j] = col < g.N ? off2(col, g.cn) : 0;
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      int row = m0 + wm * 64 + i * 16 + r * 4 + (lane >> 4);
+      if (row >= g.M) continue;
+      const int64_t ro = cb + off2(row, g.cm);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        int col = n0 + wn * 64 + j * 16 + (lane & 15);
+        if (col < g.N) {
+          double v = g.alpha * acc[i][j][r];
+          if (g.D && g.beta != 0.0) v += g.beta * g.D[ro + coff[j]];
+          g.C[ro + coff[j]] = v;
+        }
+      }
+    }
+}
+
 }  // namespace
 
 extern "C" {
@@ -938,12 +1158,22 @@ static void reduce_splits(hipStream_t st, int M, int N, int splits, int nb, cons
   }
 }
 
+// The 16-byte chunks (2 doubles) along an operand's unit-stride index group (extents n1, n2,
+// strides s1, s2) are contiguous and, given a 16-byte aligned base and even strides elsewhere,
+// aligned: the DMA condition of gemm_f64_big_kernel.
+static bool chunk_group(int64_t n1, int64_t n2, int64_t s1, int64_t s2) {
+  if (n2 > 1) return s2 == 1 && n2 % 2 == 0 && (n1 <= 1 || s1 % 2 == 0);
+  return n1 > 1 && s1 == 1 && n1 % 2 == 0;
+}
+static bool even_stride(int64_t n, int64_t s) { return n <= 1 || s % 2 == 0; }
+
 int nl_contract_f64(const int64_t* dims, double alpha, const double* A, const int64_t* sa,
                     const double* B, const int64_t* sb, double beta, const double* D, double* C,
                     const int64_t* sc, int64_t splits, double* ws, void* stream) {
   if (!dims || !sa || !sb || !sc) return NL_EARG;
   int kind = 0;
-  if (splits <= 0) {   // automatic kernel choice and split (ws sized by nl_contract_ws)
+  const bool auto_plan = splits <= 0;
+  if (auto_plan) {   // automatic kernel choice and split (ws sized by nl_contract_ws)
     contract_plan(dims[0] * dims[1], dims[2] * dims[3], dims[4] * dims[5], dims[6] * dims[7],
                   &kind, &splits);
   }
@@ -1029,6 +1259,31 @@ int nl_contract_f64(const int64_t* dims, double alpha, const double* A, const in
                   !(dims[1] > 1 ? sa[1] == 1 : dims[0] > 1 ? sa[0] == 1 : false);
   const bool bn = (dims[3] > 1 ? sb[3] == 1 : dims[2] > 1 ? sb[2] == 1 : true) ||
                   !(dims[5] > 1 ? sb[1] == 1 : dims[4] > 1 ? sb[0] == 1 : false);
+  // 256x128 DMA tiles when every chunk is aligned and they alone give each CU a workgroup
+  const int64_t big_tiles = nl_cdiv(M, FTM) * nl_cdiv(N, FTN);
+  const bool a_dma = (((uintptr_t)A) & 15) == 0 && even_stride(dims[6], sa[4]) &&
+                     even_stride(dims[7], sa[5]) &&
+                     (ak ? chunk_group(dims[4], dims[5], sa[2], sa[3]) &&
+                               even_stride(dims[0], sa[0]) && even_stride(dims[1], sa[1])
+                         : chunk_group(dims[0], dims[1], sa[0], sa[1]) &&
+                               even_stride(dims[4], sa[2]) && even_stride(dims[5], sa[3]));
+  const bool b_dma = (((uintptr_t)B) & 15) == 0 && even_stride(dims[6], sb[4]) &&
+                     even_stride(dims[7], sb[5]) &&
+                     (bn ? chunk_group(dims[2], dims[3], sb[2], sb[3]) &&
+                               even_stride(dims[4], sb[0]) && even_stride(dims[5], sb[1])
+                         : chunk_group(dims[4], dims[5], sb[0], sb[1]) &&
+                               even_stride(dims[2], sb[2]) && even_stride(dims[3], sb[3]));
+  if (auto_plan && g.splits == 1 && M >= FTM && N >= FTN && big_tiles * nb >= 256 && a_dma &&
+      b_dma) {
+    g.tiles_m = (int)nl_cdiv(M, FTM); g.tiles_n = (int)nl_cdiv(N, FTN);
+    dim3 bg((unsigned)big_tiles, 1u, (unsigned)nb);
+    if (ak && bn)   hipLaunchKernelGGL((gemm_f64_big_kernel<true, true>), bg, dim3(512), 0, st, g);
+    else if (ak)    hipLaunchKernelGGL((gemm_f64_big_kernel<true, false>), bg, dim3(512), 0, st, g);
+    else if (bn)    hipLaunchKernelGGL((gemm_f64_big_kernel<false, true>), bg, dim3(512), 0, st, g);
+    else            hipLaunchKernelGGL((gemm_f64_big_kernel<false, false>), bg, dim3(512), 0, st, g);
+    NL_LAUNCH_CHECK();
+    return NL_OK;
+  }
   dim3 grid((unsigned)(g.tiles_m * g.tiles_n), (unsigned)g.splits, (unsigned)nb);
   if (ak && bn)   hipLaunchKernelGGL((gemm_f64_kernel<true, true>), grid, dim3(256), 0, st, g);
   else if (ak)    hipLaunchKernelGGL((gemm_f64_kernel<true, false>), grid, dim3(256), 0, st, g);

@@ -89,6 +89,31 @@ def test_gemm_f64_layouts(M, N, K, batch, ak, bn):
         np.testing.assert_allclose(host(C), ref, rtol=1e-12, atol=1e-12)
 
 
+@pytest.mark.parametrize("spec,sa,sb", [
+    # the four operand layouts, edge tiles in M and N, a partial last K step
+    ("ik,kj->ij", (2000, 334), (334, 4000)),
+    ("ki,kj->ij", (333, 2000), (333, 4000)),
+    ("ik,jk->ij", (2000, 334), (4000, 334)),
+    ("ki,jk->ij", (334, 2000), (4000, 334)),
+    # two-run (strided) index groups: runs of 16 and of 14 (< the K step of 16), and a batch
+    ("acik,ckbj->aibj", (180, 180, 16, 16), (180, 16, 180, 16)),
+    ("cdkl,dblj->ckbj", (210, 210, 14, 14), (210, 210, 14, 14)),
+    ("bik,bkj->bij", (8, 1024, 200), (8, 200, 1024)),
+])
+def test_contract_f64_large_tile_path_exact(spec, sa, sb):
+    """Shapes that take the 256x128 FP64 tiles (LDS DMA, walkers along two-run k indices):
+    small-integer operands make every product and sum exact, so the result must equal the
+    float64 reference bit for bit, including alpha and the addend."""
+    g = torch.Generator(device="cuda").manual_seed(len(spec) + sa[0])
+    A = torch.randint(-3, 4, sa, device="cuda", generator=g).double()
+    B = torch.randint(-3, 4, sb, device="cuda", generator=g).double()
+    ref = torch.einsum(spec, A, B)
+    D = torch.randint(-3, 4, ref.shape, device="cuda", generator=g).double()
+    C = D.clone()
+    ops.contract(spec, A, B, C, alpha=2.0, beta=0.5)
+    assert float((C - (2.0 * ref + 0.5 * D)).abs().max()) == 0.0
+
+
 @pytest.mark.parametrize("dt", [0, 1])
 def test_gemm_lp_exact_on_small_integers(dt):
     """BF16/FP16 inputs that are small integers are exact; FP32 accumulation of products
