@@ -139,6 +139,8 @@ class Buffers:
         self.X2pp, self.X2hh = z(p, p), z(h, h)
         self.s_ph, self.u_ph, self.w_ph, self.z_ph = z(p, h), z(p, h), z(p, h), z(p, h)
         self.q_hh = z(h, h)
+        self.Dpp, self.Dhh = z(p, p), z(h, h)     # v.t2 dressings shared by both parts
+        self.Mpp, self.Mhh = z(p, p), z(h, h)     # combined Sab / Sij dressing matrices
         self.ret1 = z(p, p)
         self.T0 = z(p, h, h)
         self.tau = z(p, p, h, h)
@@ -190,13 +192,19 @@ def t1_part(ham, t1, t2, B, prec=DEFAULT_PREC):
     permute_into(ham.f_hh, Xhh, alpha=-1.0)
     contract("ck,ci->ki", ham.f_ph, t1, Xhh, alpha=-0.5)                     # A15
     contract("bijk,bj->ki", ham.v_phhh, t1, Xhh, alpha=-1.0)                 # A17
-    contract("cdlk,cdli->ki", ham.v_pphh, t2, Xhh, alpha=-0.5, precision=prec["dress"])  # A18
+    # A18 into Dhh: the same matrix is A42 of the T2 part (v_pphh antisymmetric in k,l and t2
+    # in i,j make -1/2 v[c,d,k,l] t2[c,d,j,l] = -1/2 v[c,d,l,k] t2[c,d,l,j])
+    contract("cdlk,cdli->ki", ham.v_pphh, t2, B.Dhh, alpha=-0.5, beta=0.0,
+             precision=prec["dress"])
+    ops.axpby(1.0, B.Dhh, 1.0, Xhh)
     contract("cdlk,cl->dk", vpphh, t1, B.u_ph, beta=0.0)                     # A20 (hh)
     contract("dk,di->ki", B.u_ph, t1, Xhh, alpha=-0.5)
     permute_into(ham.f_pp, Xpp)
     contract("ck,ak->ac", ham.f_ph, t1, Xpp, alpha=-0.5)                     # A16
-    contract("dckl,dakl->ac", ham.v_pphh_T.permute(1, 0, 2, 3), t2, Xpp, alpha=-0.5,
-             precision=prec["dress"])                                        # A19
+    # A19 into Dpp: equal to A41 of the T2 part (antisymmetry of v_pphh in c,d and of t2 in a,b)
+    contract("dckl,dakl->ac", ham.v_pphh_T.permute(1, 0, 2, 3), t2, B.Dpp, alpha=-0.5,
+             beta=0.0, precision=prec["dress"])
+    ops.axpby(1.0, B.Dpp, 1.0, Xpp)
     contract("cdkl,dk->cl", vpphh, t1, B.w_ph, beta=0.0)                     # A20 (pp)
     contract("cl,al->ac", B.w_ph, t1, Xpp, alpha=0.5)
     if pl.n:
@@ -208,7 +216,11 @@ def t1_part(ham, t1, t2, B, prec=DEFAULT_PREC):
 def t2_part(ham, t1, t2, B, prec=DEFAULT_PREC):
     """S0/Sab/Sij/Spp and X2pp/X2hh of t2Iter (coupled_cluster.py:476-536).
 
-    Requires B.ret1 and B.u_ph from t1_part at the same amplitudes."""
+    Requires B.ret1, B.u_ph, B.Dpp and B.Dhh from t1_part at the same amplitudes.  Uses that t2
+    is antisymmetric in (a,b) and in (i,j) and that Sab / Sij enter the residual only through
+    P(ab) / P(ij): X_pp[b,c] t2[a,c,i,j] and X_pp[a,c] t2[c,b,i,j] have the same P(ab) image, as
+    have -q[l,i] t2[a,b,l,j] and -q[k,j] t2[a,b,i,k] under P(ij), so each group's two dressings
+    run as one contraction with the summed matrix."""
     p, h = ham.p, ham.h
     vpphh = ham.vpphh_view()
     pl = ham.ppph
@@ -217,12 +229,12 @@ def t2_part(ham, t1, t2, B, prec=DEFAULT_PREC):
     # ---- dressed Fock matrices
     Xpp, Xhh = B.X2pp, B.X2hh
     permute_into(ham.f_pp, Xpp)
-    contract("cdkl,bdkl->bc", ham.v_pphh, t2, Xpp, alpha=-0.5, precision=prec["dress"])  # A41
+    ops.axpby(1.0, B.Dpp, 1.0, Xpp)                                          # A41 (= A19)
     contract("ck,bk->bc", ham.f_ph, t1, Xpp, alpha=-1.0)                     # A43
     contract("cdlk,dk->cl", ham.v_pphh, t1, B.z_ph, beta=0.0)               # A46
     contract("cl,bl->bc", B.z_ph, t1, Xpp, alpha=-1.0)
     permute_into(ham.f_hh, Xhh, alpha=-1.0)
-    contract("cdkl,cdjl->kj", ham.v_pphh, t2, Xhh, alpha=-0.5, precision=prec["dress"])  # A42
+    ops.axpby(1.0, B.Dhh, 1.0, Xhh)                                          # A42 (= A18)
     contract("ck,cj->kj", ham.f_ph, t1, Xhh, alpha=-1.0)                     # A44
     contract("dk,dj->kj", B.u_ph, t1, Xhh, alpha=-1.0)                      # A45
     # ---- S0: v_pphh + 1/2 tau.Whh + 1/2 pp ladder(tau)
@@ -241,15 +253,16 @@ def t2_part(ham, t1, t2, B, prec=DEFAULT_PREC):
     if pl.n:
         _block_gemm(pl.r56, tau, B.N56, 0.5, h * h, "r56", prec.get("ladder", "fp64"))
     contract("bkij,ak->abij", B.N56, t1, Sab, beta=0.0)                      # A23 + A39
+    permute_into(Xpp, B.Mpp)                                                 # X_pp t2 (P(ab) image)
     if pl.n:
-        contract("ad,dbij->abij", B.ret1, t2, Sab, precision=prec["dress"])  # A38
-    contract("bc,acij->abij", Xpp, t2, Sab, precision=prec["dress"])         # X_pp t2
+        ops.axpby(1.0, B.ret1, 1.0, B.Mpp)                                   # + A38
+    contract("ad,dbij->abij", B.Mpp, t2, Sab, precision=prec["dress"])
     # ---- Sij: P(ij) group, stored as [a][i][b][j] (like Spp)
     t2r = B.Tr.permute(0, 2, 1, 3)                                           # t2 via [a][i][c][k]
     Sij = B.Sij.permute(0, 2, 1, 3)                                          # view Sij[a,b,i,j]
     contract("cikl,ck->li", ham.v_phhh, t1, B.q_hh, beta=0.0)                # A28
-    contract("li,ablj->abij", B.q_hh, t2r, Sij, alpha=-1.0, beta=0.0)
-    contract("kj,abik->abij", Xhh, t2r, Sij, precision=prec["dress"])        # X_hh t2
+    _lib.call("nl_sub", h * h, ops._p(Xhh), ops._p(B.q_hh), ops._p(B.Mhh), ops._stream())
+    contract("kj,abik->abij", B.Mhh, t2r, Sij, beta=0.0)                     # X_hh t2 - A28 t2
     if pl.n:
         _segsum(pl.ret2, h, t1, B.Sij, 1.0, 1, p * h)                        # A36
     # ---- Spp: P(ij)P(ab) group in [(a,i)][(b,j)] layout
