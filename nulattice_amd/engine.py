@@ -153,6 +153,8 @@ class Buffers:
         self.N56 = z(p, h, h, h)
         self.Xidkl = z(h, p, h, h)
         self.Yilaj = z(h, h, p, h)
+        self.Zilaj = z(h, h, p, h)   # Yilaj - U[a,l,j,i]: the two Spp tails as one operand
+        self.F_ph = z(p, h)          # f_ph + s: the A12 + A14 dressing of H1
         self.Ubkij = z(p, h, h, h)
         self.part = z(2 * int(_lib.load().nl_partials(p * p * h * h)) + 8)
         self.E = z(1)
@@ -177,10 +179,18 @@ def t1_part(ham, t1, t2, B, prec=DEFAULT_PREC):
     t2r = B.Tr.permute(0, 2, 1, 3)                   # the same t2[a,c,i,k] through that layout
     permute_into(ham.f_ph, H1)
     contract("akci,ck->ai", ham.v_phph, t1, H1, alpha=-1.0)                  # A11
-    contract("ck,acik->ai", ham.f_ph, t2r, H1)                                # A12
     contract("cikl,cakl->ai", ham.v_phhh, t2, H1, alpha=-0.5)                # A13
-    contract("cdkl,ck->dl", vpphh, t1, B.s_ph, beta=0.0)                     # A14
-    contract("dl,dali->ai", B.s_ph, t2r, H1)
+    # A14: s[d,l] = v[c,d,k,l] t1[c,k].  With v antisymmetric in (c,d) and in (k,l), the other
+    # v_pphh.t1 products of both parts are the same matrix: u (A20, hh) = s, w (A20, pp) = -s,
+    # z (A46) = s; and s[d,l] t2[d,a,l,i] = s[d,l] t2[a,d,i,l], so A12 and A14 are one product
+    # with f_ph + s.
+    contract("cdkl,ck->dl", vpphh, t1, B.s_ph, beta=0.0)
+    permute_into(B.s_ph, B.u_ph)
+    permute_into(B.s_ph, B.w_ph, alpha=-1.0)
+    permute_into(B.s_ph, B.z_ph)
+    permute_into(ham.f_ph, B.F_ph)
+    ops.axpby(1.0, B.s_ph, 1.0, B.F_ph)
+    contract("ck,acik->ai", B.F_ph, t2r, H1)                                 # A12 + A14
     pl = ham.ppph
     if pl.n:
         T0 = B.T0
@@ -197,16 +207,14 @@ def t1_part(ham, t1, t2, B, prec=DEFAULT_PREC):
     contract("cdlk,cdli->ki", ham.v_pphh, t2, B.Dhh, alpha=-0.5, beta=0.0,
              precision=prec["dress"])
     ops.axpby(1.0, B.Dhh, 1.0, Xhh)
-    contract("cdlk,cl->dk", vpphh, t1, B.u_ph, beta=0.0)                     # A20 (hh)
-    contract("dk,di->ki", B.u_ph, t1, Xhh, alpha=-0.5)
+    contract("dk,di->ki", B.u_ph, t1, Xhh, alpha=-0.5)                      # A20 (hh)
     permute_into(ham.f_pp, Xpp)
     contract("ck,ak->ac", ham.f_ph, t1, Xpp, alpha=-0.5)                     # A16
     # A19 into Dpp: equal to A41 of the T2 part (antisymmetry of v_pphh in c,d and of t2 in a,b)
     contract("dckl,dakl->ac", ham.v_pphh_T.permute(1, 0, 2, 3), t2, B.Dpp, alpha=-0.5,
              beta=0.0, precision=prec["dress"])
     ops.axpby(1.0, B.Dpp, 1.0, Xpp)
-    contract("cdkl,dk->cl", vpphh, t1, B.w_ph, beta=0.0)                     # A20 (pp)
-    contract("cl,al->ac", B.w_ph, t1, Xpp, alpha=0.5)
+    contract("cl,al->ac", B.w_ph, t1, Xpp, alpha=0.5)                       # A20 (pp)
     if pl.n:
         ops.axpby(1.0, B.ret1, 1.0, Xpp)                                     # X_pp += ret1
     contract("ac,ci->ai", Xpp, t1, H1)
@@ -231,8 +239,7 @@ def t2_part(ham, t1, t2, B, prec=DEFAULT_PREC):
     permute_into(ham.f_pp, Xpp)
     ops.axpby(1.0, B.Dpp, 1.0, Xpp)                                          # A41 (= A19)
     contract("ck,bk->bc", ham.f_ph, t1, Xpp, alpha=-1.0)                     # A43
-    contract("cdlk,dk->cl", ham.v_pphh, t1, B.z_ph, beta=0.0)               # A46
-    contract("cl,bl->bc", B.z_ph, t1, Xpp, alpha=-1.0)
+    contract("cl,bl->bc", B.z_ph, t1, Xpp, alpha=-1.0)                      # A46
     permute_into(ham.f_hh, Xhh, alpha=-1.0)
     ops.axpby(1.0, B.Dhh, 1.0, Xhh)                                          # A42 (= A18)
     contract("ck,cj->kj", ham.f_ph, t1, Xhh, alpha=-1.0)                     # A44
@@ -267,17 +274,23 @@ def t2_part(ham, t1, t2, B, prec=DEFAULT_PREC):
         _segsum(pl.ret2, h, t1, B.Sij, 1.0, 1, p * h)                        # A36
     # ---- Spp: P(ij)P(ab) group in [(a,i)][(b,j)] layout
     W = B.W
-    contract("cdkl,dblj->ckbj", vpphh, t2r, W, alpha=0.5, precision=prec["ring"],
-             tag="ring_W", addend=ham.Wc)                                    # A24 + A22
-    if pl.n:
-        _segsum(pl.ret4, h, t1, W, 1.0)                                      # A37
-    contract("cjkl,bl->ckbj", ham.v_phhh, t1, W, alpha=-1.0)                 # A29
-    contract("acik,ckbj->aibj", t2r, W, B.Spp, beta=0.0, precision=prec["ring"], tag="ring")
+    # t2[d,b,l,j] = t2[b,d,j,l]: ring_W reads t2 in the ring's own [(b,j)][(d,l)] rows, so the
+    # two ring GEMMs share one 16-bit conversion of it
+    with ops.lp_reuse(B.Tr):
+        contract("cdkl,bdjl->ckbj", vpphh, t2r, W, alpha=0.5, precision=prec["ring"],
+                 tag="ring_W", addend=ham.Wc)                                # A24 + A22
+        if pl.n:
+            _segsum(pl.ret4, h, t1, W, 1.0)                                  # A37
+        contract("cjkl,bl->ckbj", ham.v_phhh, t1, W, alpha=-1.0)             # A29
+        contract("acik,ckbj->aibj", t2r, W, B.Spp, beta=0.0, precision=prec["ring"], tag="ring")
     contract("cdkl,ci->idkl", ham.v_pphh, t1, B.Xidkl, beta=0.0)             # A34
     contract("idkl,adkj->ilaj", B.Xidkl, t2, B.Yilaj, beta=0.0, precision=prec["hh"])
-    contract("ilaj,bl->aibj", B.Yilaj, t1, B.Spp)
     contract("bkci,cj->bkij", ham.v_phph_bkic.permute(0, 1, 3, 2), t1, B.Ubkij, beta=0.0)  # A27
-    contract("ak,bkij->aibj", t1, B.Ubkij, B.Spp, alpha=-1.0)
+    # Spp enters only through P(ij)P(ab), which is invariant under (a,i) <-> (b,j): the A27 term
+    # -t1[a,k] U[b,k,i,j] may stand as -U[a,l,j,i] t1[b,l], so both tails are one product
+    permute_into(B.Yilaj, B.Zilaj)
+    permute_into(B.Ubkij.permute(3, 1, 0, 2), B.Zilaj, alpha=-1.0, beta=1.0)
+    contract("ilaj,bl->aibj", B.Zilaj, t1, B.Spp)
 
 
 def _pp_ladder(ham, tau, S0, prec):

@@ -355,17 +355,35 @@ def forget_ptrs(ptrs):
             del _LP_CACHE[k]
 
 
+class lp_reuse:
+    """Context in which the given tensors do not change: their 16-bit conversions are cached as
+    those of constants are, and dropped on exit.  Lets two products that read one operand in the
+    same layout (the two ring GEMMs and t2) share a single conversion."""
+
+    def __init__(self, *ts):
+        self.ptrs = [t.data_ptr() for t in ts if t.data_ptr() not in CONST_PTRS]
+
+    def __enter__(self):
+        CONST_PTRS.update(self.ptrs)
+        return self
+
+    def __exit__(self, *exc):
+        forget_ptrs(self.ptrs)
+        return False
+
+
 def to_lp_rows(t, labels, rows, cols, dtype, tag):
     """Convert view t (rows x cols) to a 16-bit [R][ldk] buffer: FP16 with per-row power-of-two
-    scales (returned as inverse scales), BF16 by a plain cast (inverse scales None)."""
+    scales (returned as inverse scales), BF16 by a plain cast (inverse scales None).  Conversions
+    of constant storage are cached by the memory view they read (pointer, extents, strides), so
+    differently labelled views of the same rows share one."""
+    v = _view4(t, labels, rows, cols)
     key = None
-    if t.data_ptr() in CONST_PTRS:
-        key = (t.data_ptr(), tuple(t.shape), tuple(t.stride()), labels, tuple(rows), tuple(cols),
-               dtype)
+    if v is not None and t.data_ptr() in CONST_PTRS:
+        key = (t.data_ptr(), v[0], v[1], v[2], v[3], tuple(v[4]), dtype)
         hit = _LP_CACHE.get(key)
         if hit is not None:
             return hit
-    v = _view4(t, labels, rows, cols)
     if v is None:
         t = _as_layout(t, labels, rows + cols); labels = "".join(rows + cols)
         v = _view4(t, labels, rows, cols)
