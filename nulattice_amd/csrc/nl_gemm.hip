@@ -268,28 +268,51 @@ __global__ __launch_bounds__(256) void skinny_f64_kernel(GemmF64Args g) {
       rowoff[e] = m < g.M ? off2(m, g.am) : 0;
     }
   }
-  for (int k0 = kbeg; k0 < kend; k0 += SKK) {
+  // register prefetch: the next tile's loads are in flight while this tile is multiplied
+  constexpr int NBQ = (SKK * NB + 255) / 256;
+  double ra[16], rb[NBQ];
+  auto load = [&](int k0) {
     if (AM) {
       Walk2 wk(k0, g.ak);
 #pragma unroll
       for (int e = 0; e < SKK; ++e) {
-        As[e][tid] = (m0 + tid < g.M && k0 + e < kend) ? A[arow + wk.off] : 0.0;
+        ra[e] = (m0 + tid < g.M && k0 + e < kend) ? A[arow + wk.off] : 0.0;
         wk.next(g.ak);
       }
     } else {
-      const int kk = tid & 15, k = k0 + kk;
+      const int k = k0 + (tid & 15);
       const int64_t ko = k < kend ? off2(k, g.ak) : 0;
 #pragma unroll
       for (int e = 0; e < 16; ++e) {
-        int mm = tid / 16 + 16 * e;
-        As[kk][mm] = (m0 + mm < g.M && k < kend) ? A[rowoff[e] + ko] : 0.0;
+        const int mm = tid / 16 + 16 * e;
+        ra[e] = (m0 + mm < g.M && k < kend) ? A[rowoff[e] + ko] : 0.0;
       }
     }
-    for (int q = tid; q < SKK * NB; q += 256) {
-      int kk = q / NB, n = q - kk * NB, k = k0 + kk;
-      Bs[kk][n] = (n < g.N && k < kend) ? B[off2(k, g.bk) + off2(n, g.bn)] : 0.0;
+#pragma unroll
+    for (int j = 0; j < NBQ; ++j) {
+      const int q = tid + 256 * j, kk = q / NB, n = q - kk * NB, k = k0 + kk;
+      rb[j] = (q < SKK * NB && n < g.N && k < kend) ? B[off2(k, g.bk) + off2(n, g.bn)] : 0.0;
     }
+  };
+  auto store = [&]() {
+    if (AM) {
+#pragma unroll
+      for (int e = 0; e < SKK; ++e) As[e][tid] = ra[e];
+    } else {
+#pragma unroll
+      for (int e = 0; e < 16; ++e) As[tid & 15][tid / 16 + 16 * e] = ra[e];
+    }
+#pragma unroll
+    for (int j = 0; j < NBQ; ++j) {
+      const int q = tid + 256 * j;
+      if (q < SKK * NB) Bs[q / NB][q % NB] = rb[j];
+    }
+  };
+  if (kbeg < kend) load(kbeg);
+  for (int k0 = kbeg; k0 < kend; k0 += SKK) {
+    store();
     __syncthreads();
+    if (k0 + SKK < kend) load(k0 + SKK);
 #pragma unroll
     for (int kk = 0; kk < SKK; ++kk) {
       const double a = As[kk][tid];
@@ -417,7 +440,9 @@ __global__ __launch_bounds__(256) void smallmn32_f64_kernel(GemmF64Args g) {
   for (int x = 0; x < 4; ++x)
 #pragma unroll
     for (int y = 0; y < 4; ++y) acc[x][y] = 0.0;
-  for (int k0 = kbeg; k0 < kend; k0 += SMK) {
+  // register prefetch: the next K tile's loads are in flight while this one is multiplied
+  double ra[4], rb[4];
+  auto load = [&](int k0) {
     const int64_t kfix = (k0 + lo < kend) ? off2(k0 + lo, g.ak) : 0;
     const int64_t kfixb = (k0 + lo < kend) ? off2(k0 + lo, g.bk) : 0;
 #pragma unroll
@@ -425,20 +450,30 @@ __global__ __launch_bounds__(256) void smallmn32_f64_kernel(GemmF64Args g) {
       const int kv = hi + 8 * it;                          // k of the "along m/n" mappings
       if (AM) {
         const bool ok = lo < g.M && k0 + kv < kend;
-        As[kv][lo] = ok ? A[amo[it] + off2(k0 + kv, g.ak)] : 0.0;
+        ra[it] = ok ? A[amo[it] + off2(k0 + kv, g.ak)] : 0.0;
       } else {
         const bool ok = kv < g.M && k0 + lo < kend;
-        As[lo][kv] = ok ? A[amo[it] + kfix] : 0.0;
+        ra[it] = ok ? A[amo[it] + kfix] : 0.0;
       }
       if (BN) {
         const bool ok = lo < g.N && k0 + kv < kend;
-        Bs[kv][lo] = ok ? B[off2(k0 + kv, g.bk) + bno[it]] : 0.0;
+        rb[it] = ok ? B[off2(k0 + kv, g.bk) + bno[it]] : 0.0;
       } else {
         const bool ok = kv < g.N && k0 + lo < kend;
-        Bs[lo][kv] = ok ? B[kfixb + bno[it]] : 0.0;
+        rb[it] = ok ? B[kfixb + bno[it]] : 0.0;
       }
     }
+  };
+  if (kbeg < kend) load(kbeg);
+  for (int k0 = kbeg; k0 < kend; k0 += SMK) {
+#pragma unroll
+    for (int it = 0; it < 4; ++it) {
+      const int kv = hi + 8 * it;
+      if (AM) As[kv][lo] = ra[it]; else As[lo][kv] = ra[it];
+      if (BN) Bs[kv][lo] = rb[it]; else Bs[lo][kv] = rb[it];
+    }
     __syncthreads();
+    if (k0 + SMK < kend) load(k0 + SMK);
 #pragma unroll
     for (int kq = 0; kq < SMK / 4; ++kq) {
       const int kk = kq * 4 + grp;
@@ -539,6 +574,27 @@ __global__ __launch_bounds__(256) void lowk_f64_kernel(GemmF64Args g) {
   }
   __syncthreads();
   const int tm = LN ? (tid >> 4) : (tid & 15), tn = LN ? (tid & 15) : (tid >> 4);
+  // output offsets and the addend are read before the operand staging, so their latency overlaps
+  // it (each output is read and written by this thread only, so D may be C)
+  const int64_t cb = off2(batch, g.cb);
+  int64_t coff[4], roff[4];
+#pragma unroll
+  for (int y = 0; y < 4; ++y) {
+    const int col = n0 + tn + 16 * y;
+    coff[y] = col < g.N ? off2(col, g.cn) : 0;
+  }
+#pragma unroll
+  for (int x = 0; x < 4; ++x) {
+    const int row = m0 + tm + 16 * x;
+    roff[x] = row < g.M ? cb + off2(row, g.cm) : -1;
+  }
+  const bool addend = g.D && g.beta != 0.0;
+  double dv[4][4];
+#pragma unroll
+  for (int x = 0; x < 4; ++x)
+#pragma unroll
+    for (int y = 0; y < 4; ++y)
+      dv[x][y] = (addend && roff[x] >= 0 && n0 + tn + 16 * y < g.N) ? g.D[roff[x] + coff[y]] : 0.0;
   double acc[4][4];
 #pragma unroll
   for (int x = 0; x < 4; ++x)
@@ -553,25 +609,15 @@ __global__ __launch_bounds__(256) void lowk_f64_kernel(GemmF64Args g) {
 #pragma unroll
       for (int y = 0; y < 4; ++y) acc[x][y] = fma(a[x], b[y], acc[x][y]);
   }
-  const int64_t cb = off2(batch, g.cb);
-  int64_t coff[4];
-#pragma unroll
-  for (int y = 0; y < 4; ++y) {
-    const int col = n0 + tn + 16 * y;
-    coff[y] = col < g.N ? off2(col, g.cn) : 0;
-  }
 #pragma unroll
   for (int x = 0; x < 4; ++x) {
-    const int row = m0 + tm + 16 * x;
-    if (row >= g.M) continue;
-    const int64_t ro = cb + off2(row, g.cm);
+    if (roff[x] < 0) continue;
 #pragma unroll
     for (int y = 0; y < 4; ++y) {
       if (n0 + tn + 16 * y >= g.N) continue;
-      const int64_t o = ro + coff[y];
       double v = g.alpha * acc[x][y];
-      if (g.D && g.beta != 0.0) v += g.beta * g.D[o];
-      g.C[o] = v;
+      if (addend) v += g.beta * dv[x][y];
+      g.C[roff[x] + coff[y]] = v;
     }
   }
 }

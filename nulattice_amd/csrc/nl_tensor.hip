@@ -219,20 +219,30 @@ __global__ void tau_kernel(int64_t p, int64_t h, const double* t1, const double*
 }
 
 // out[dst[s] + x*ds] += alpha * sum_{e in [ptr[s], ptr[s+1])} val[e] * src[so[e] + x*ss]
-// one wavefront per segment.
-__global__ __launch_bounds__(NT) void segsum_kernel(int64_t S, int64_t X, const int64_t* ptr,
-                                                  const int64_t* so, const double* val,
-                                                  const int64_t* dst, int64_t ss, int64_t ds,
-                                                  double alpha, const double* src, double* out) {
-  int lane = threadIdx.x & 63;
-  for (int64_t s = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6; s < S;
-       s += ((int64_t)gridDim.x * blockDim.x) >> 6) {
-    int64_t e0 = ptr[s], e1 = ptr[s + 1];
-    for (int64_t x = lane; x < X; x += 64) {
-      double acc = 0.0;
-      for (int64_t e = e0; e < e1; ++e) acc += val[e] * src[so[e] + x * ss];
-      double* o = out + dst[s] + x * ds;
-      *o += alpha * acc;
+// G lanes per segment (G = the power of two >= X, at most 64), so a wavefront serves 64/G
+// segments; each lane keeps two partial sums (entries alternate between them, added in a fixed
+// order), which keeps two independent loads in flight along a segment.
+__global__ __launch_bounds__(NT) void segsum_kernel(int64_t S, int64_t X, int G,
+                                                  const int64_t* ptr, const int64_t* so,
+                                                  const double* val, const int64_t* dst,
+                                                  int64_t ss, int64_t ds, double alpha,
+                                                  const double* src, double* out) {
+  const int lane = threadIdx.x & 63, sub = lane / G, xl = lane - sub * G, per = 64 / G;
+  const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  for (int64_t wv = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6; wv * per < S;
+       wv += nw) {
+    const int64_t s = wv * per + sub;
+    if (s >= S) continue;
+    const int64_t e0 = ptr[s], e1 = ptr[s + 1], d = dst[s];
+    for (int64_t x = xl; x < X; x += G) {
+      double acc0 = 0.0, acc1 = 0.0;
+      int64_t e = e0;
+      for (; e + 1 < e1; e += 2) {
+        acc0 += val[e] * src[so[e] + x * ss];
+        acc1 += val[e + 1] * src[so[e + 1] + x * ss];
+      }
+      if (e < e1) acc0 += val[e] * src[so[e] + x * ss];
+      out[d + x * ds] += alpha * (acc0 + acc1);
     }
   }
 }
@@ -561,9 +571,12 @@ int nl_segsum(int64_t S, int64_t X, const int64_t* ptr, const int64_t* src_off, 
   if (S < 0 || X < 0) return NL_EARG;
   if (S == 0 || X == 0) return NL_OK;
   if (!ptr || !src_off || !val || !dst_off || !src || !out) return NL_EARG;
-  int blocks = grid_for(S * 64);
-  hipLaunchKernelGGL(segsum_kernel, dim3(blocks), dim3(NT), 0, (hipStream_t)stream, S, X, ptr,
-                     src_off, val, dst_off, src_stride, dst_stride, alpha, src, out);
+  int G = 1;
+  while (G < X && G < 64) G *= 2;
+  const int64_t waves = (S + 64 / G - 1) / (64 / G);
+  int blocks = grid_for(waves * 64);
+  hipLaunchKernelGGL(segsum_kernel, dim3(blocks), dim3(NT), 0, (hipStream_t)stream, S, X, G,
+                     ptr, src_off, val, dst_off, src_stride, dst_stride, alpha, src, out);
   NL_LAUNCH_CHECK();
   return NL_OK;
 }

@@ -271,7 +271,7 @@ def t2_part(ham, t1, t2, B, prec=DEFAULT_PREC):
     _lib.call("nl_sub", h * h, ops._p(Xhh), ops._p(B.q_hh), ops._p(B.Mhh), ops._stream())
     contract("kj,abik->abij", B.Mhh, t2r, Sij, beta=0.0)                     # X_hh t2 - A28 t2
     if pl.n:
-        _segsum(pl.ret2, h, t1, B.Sij, 1.0, 1, p * h)                        # A36
+        _segsum(pl.ret2p, h, t1, B.Sij, 1.0)                                 # A36
     # ---- Spp: P(ij)P(ab) group in [(a,i)][(b,j)] layout
     W = B.W
     # t2[d,b,l,j] = t2[b,d,j,l]: ring_W reads t2 in the ring's own [(b,j)][(d,l)] rows, so the

@@ -196,6 +196,34 @@ def test_permute_tau_segsum():
     np.testing.assert_allclose(host(od), oref, rtol=1e-14, atol=1e-14)
 
 
+@pytest.mark.parametrize("X", [1, 3, 28, 70])
+def test_segsum_lane_packing(X):
+    """Segments share a wavefront (G = 1 ... 64 lanes each, one X = 70 pass of two rounds);
+    uneven segment lengths (0 to 7 entries) and strided sources and destinations."""
+    rng = np.random.default_rng(X)
+    S = 300
+    lens = rng.integers(0, 8, S)
+    ptr = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    E = int(ptr[-1])
+    so = rng.integers(0, 50, E).astype(np.int64)
+    val = rng.standard_normal(E)
+    ss, ds = 2, 3
+    dst = (np.arange(S) * X * ds).astype(np.int64)
+    src = rng.standard_normal(50 + X * ss)
+    out0 = rng.standard_normal(S * X * ds)
+    ref = out0.copy()
+    for s in range(S):
+        for x in range(X):
+            ref[dst[s] + x * ds] += 0.7 * sum(val[e] * src[so[e] + x * ss]
+                                              for e in range(ptr[s], ptr[s + 1]))
+    L = lambda a: torch.tensor(a, dtype=torch.int64, device="cuda")
+    keep = [L(ptr), L(so), dev(val), L(dst), dev(src), dev(out0)]
+    _lib.call("nl_segsum", S, X, ops._p(keep[0]), ops._p(keep[1]), ops._p(keep[2]),
+              ops._p(keep[3]), ss, ds, ctypes.c_double(0.7), ops._p(keep[4]), ops._p(keep[5]),
+              ops._stream())
+    np.testing.assert_allclose(host(keep[5]), ref, rtol=1e-13, atol=1e-13)
+
+
 def test_error_codes():
     """Bad arguments come back as -1, never as a fault."""
     lib = _lib.load()
