@@ -212,29 +212,35 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(GemmF64Args g) {
     return;
   }
   const int64_t cb = off2(batch, g.cb);
+  const bool addend = g.D && g.beta != 0.0;
   int64_t coff[4];
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
     int col = n0 + wn * 64 + j * 16 + (lane & 15);
-    coff[j] = col < g.N ? off2(col, g.cn) : 0;
+    coff[j] = col < g.N ? off2(col, g.cn) : -1;
   }
+  // per 16-row group: all addend loads first, then the stores (D may be C; each element is read
+  // and written by this lane only), so the loads overlap instead of waiting on each store
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < 4; ++i) {
+    int64_t ro[4];
+    double dv[4][4];
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      int row = m0 + wm * 64 + i * 16 + r * 4 + (lane >> 4);
-      if (row >= g.M) continue;
-      const int64_t ro = cb + off2(row, g.cm);
+      const int row = m0 + wm * 64 + i * 16 + r * 4 + (lane >> 4);
+      ro[r] = row < g.M ? cb + off2(row, g.cm) : -1;
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        int col = n0 + wn * 64 + j * 16 + (lane & 15);
-        if (col < g.N) {
-          double v = g.alpha * acc[i][j][r];
-          if (g.D && g.beta != 0.0) v += g.beta * g.D[ro + coff[j]];
-          g.C[ro + coff[j]] = v;
-        }
-      }
+      for (int j = 0; j < 4; ++j)
+        dv[r][j] = (addend && ro[r] >= 0 && coff[j] >= 0) ? g.D[ro[r] + coff[j]] : 0.0;
     }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      if (ro[r] < 0) continue;
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        if (coff[j] >= 0) g.C[ro[r] + coff[j]] = g.alpha * acc[i][j][r] + g.beta * dv[r][j];
+    }
+  }
 }
 
 // Skinny contraction (N <= NB <= 64): one output row per thread, the N outputs in registers,
@@ -372,13 +378,26 @@ __global__ __launch_bounds__(256) void smallmn_f64_kernel(GemmF64Args g) {
 #pragma unroll
   for (int j = 0; j < 16; ++j) acc[j] = 0.0;
   for (int k0 = kbeg; k0 < kend; k0 += SMK) {
-    for (int q = tid; q < SMK * 64; q += 256) {
-      int k, m;
+    // all tile loads are issued before any LDS store, so their latencies overlap
+    constexpr int SQ = SMK * 64 / 256;
+    double ra[SQ], rb[SQ];
+#pragma unroll
+    for (int u = 0; u < SQ; ++u) {
+      const int q = tid + 256 * u;
+      int k, m, n;
       if (AM) { k = q >> 6; m = q & 63; } else { m = q / SMK; k = q - m * SMK; }
-      As[k][m] = (m < g.M && k0 + k < kend) ? A[off2(m, g.am) + off2(k0 + k, g.ak)] : 0.0;
-      int n;
+      ra[u] = (m < g.M && k0 + k < kend) ? A[off2(m, g.am) + off2(k0 + k, g.ak)] : 0.0;
       if (BN) { k = q >> 6; n = q & 63; } else { n = q / SMK; k = q - n * SMK; }
-      Bs[k][n] = (n < g.N && k0 + k < kend) ? B[off2(k0 + k, g.bk) + off2(n, g.bn)] : 0.0;
+      rb[u] = (n < g.N && k0 + k < kend) ? B[off2(k0 + k, g.bk) + off2(n, g.bn)] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < SQ; ++u) {
+      const int q = tid + 256 * u;
+      int k, m, n;
+      if (AM) { k = q >> 6; m = q & 63; } else { m = q / SMK; k = q - m * SMK; }
+      As[k][m] = ra[u];
+      if (BN) { k = q >> 6; n = q & 63; } else { n = q / SMK; k = q - n * SMK; }
+      Bs[k][n] = rb[u];
     }
     __syncthreads();
 #pragma unroll
@@ -564,13 +583,27 @@ __global__ __launch_bounds__(256) void lowk_f64_kernel(GemmF64Args g) {
   // operand tiles: along k when that is the operand's unit stride, else along m / n
   const bool ak1 = g.ak.n2 > 1 ? g.ak.s2 == 1 : g.ak.s1 == 1;
   const bool bk1 = g.bk.n2 > 1 ? g.bk.s2 == 1 : g.bk.s1 == 1;
-  for (int q = tid; q < K * LKT; q += 256) {
-    int k, m;
+  // all staging loads are issued before any LDS store, so their latencies overlap
+  constexpr int LQ = LKK * LKT / 256;
+  double ra[LQ], rb[LQ];
+#pragma unroll
+  for (int u = 0; u < LQ; ++u) {
+    const int q = tid + 256 * u;
+    int k, m, n;
     if (ak1) { m = q / K; k = q - m * K; } else { k = q / LKT; m = q - k * LKT; }
-    As[k][m] = (m0 + m < g.M) ? A[off2(m0 + m, g.am) + off2(k, g.ak)] : 0.0;
-    int n;
+    ra[u] = (q < K * LKT && m0 + m < g.M) ? A[off2(m0 + m, g.am) + off2(k, g.ak)] : 0.0;
     if (bk1) { n = q / K; k = q - n * K; } else { k = q / LKT; n = q - k * LKT; }
-    Bs[k][n] = (n0 + n < g.N) ? B[off2(k, g.bk) + off2(n0 + n, g.bn)] : 0.0;
+    rb[u] = (q < K * LKT && n0 + n < g.N) ? B[off2(k, g.bk) + off2(n0 + n, g.bn)] : 0.0;
+  }
+#pragma unroll
+  for (int u = 0; u < LQ; ++u) {
+    const int q = tid + 256 * u;
+    if (q >= K * LKT) continue;
+    int k, m, n;
+    if (ak1) { m = q / K; k = q - m * K; } else { k = q / LKT; m = q - k * LKT; }
+    As[k][m] = ra[u];
+    if (bk1) { n = q / K; k = q - n * K; } else { k = q / LKT; n = q - k * LKT; }
+    Bs[k][n] = rb[u];
   }
   __syncthreads();
   const int tm = LN ? (tid >> 4) : (tid & 15), tn = LN ? (tid & 15) : (tid >> 4);
@@ -746,31 +779,38 @@ __global__ __launch_bounds__(256) void gemm_lp_kernel(GemmLPArgs g) {
   const double beta = partial ? 0.0 : g.beta;
   const float* ia = g.ia ? g.ia + (int64_t)batch * g.sia : nullptr;
   const float* ib = g.ib ? g.ib + (int64_t)batch * g.sib : nullptr;
+  double sc[4];
   int64_t coff[4];
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
     int col = n0 + wn * 64 + j * 16 + (lane & 15);
-    coff[j] = col < g.N ? (partial ? col : off2(col, g.cn)) : 0;
+    sc[j] = (ib && col < g.N) ? (double)ib[col] : 1.0;
+    coff[j] = col < g.N ? (partial ? col : off2(col, g.cn)) : -1;
   }
+  // per 16-row group: all addend loads first, then the stores (D may be C, and each element is
+  // read and written by this lane only), so the loads overlap instead of waiting on each store
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < 4; ++i) {
+    int64_t ro[4];
+    double dv[4][4];
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      int row = m0 + wm * 64 + i * 16 + (lane >> 4) * 4 + r;
-      if (row >= g.M) continue;
-      double sr = g.alpha * (ia ? (double)ia[row] : 1.0);
-      const int64_t roff = partial ? (int64_t)row * g.N : off2(row, g.cm);
+      const int row = m0 + wm * 64 + i * 16 + (lane >> 4) * 4 + r;
+      ro[r] = row < g.M ? (partial ? (int64_t)row * g.N : off2(row, g.cm)) : -1;
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        int col = n0 + wn * 64 + j * 16 + (lane & 15);
-        if (col < g.N) {
-          double v = sr * (double)acc[i][j][r] * (ib ? (double)ib[col] : 1.0);
-          const int64_t o = roff + coff[j];
-          if (Dp && beta != 0.0) v += beta * Dp[o];
-          C[o] = v;
-        }
-      }
+      for (int j = 0; j < 4; ++j)
+        dv[r][j] = (Dp && beta != 0.0 && ro[r] >= 0 && coff[j] >= 0) ? Dp[ro[r] + coff[j]] : 0.0;
     }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      if (ro[r] < 0) continue;
+      const int row = m0 + wm * 64 + i * 16 + (lane >> 4) * 4 + r;
+      const double sr = g.alpha * (ia ? (double)ia[row] : 1.0);
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        if (coff[j] >= 0) C[ro[r] + coff[j]] = sr * (double)acc[i][j][r] * sc[j] + beta * dv[r][j];
+    }
+  }
 }
 
 // Large-tile BF16/FP16 GEMM for the big square products (the two ring GEMMs): 256x256 block
@@ -901,27 +941,32 @@ __global__ __launch_bounds__(512) void gemm_lp_big_kernel(GemmLPArgs g) {
   for (int j = 0; j < 4; ++j) {
     int col = n0 + wn * 64 + j * 16 + (lane & 15);
     sc[j] = (ib && col < g.N) ? (double)ib[col] : 1.0;
-    coff[j] = col < g.N ? (partial ? col : off2(col, g.cn)) : 0;
+    coff[j] = col < g.N ? (partial ? col : off2(col, g.cn)) : -1;
   }
+  // per 16-row group: all addend loads first, then the stores (D may be C; each element is read
+  // and written by this lane only), so the loads overlap instead of waiting on each store
 #pragma unroll
-  for (int i = 0; i < 8; ++i)
+  for (int i = 0; i < 8; ++i) {
+    int64_t ro[4];
+    double dv[4][4];
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      int row = m0 + wm * 128 + i * 16 + (lane >> 4) * 4 + r;
-      if (row >= g.M) continue;
-      double sr = g.alpha * (ia ? (double)ia[row] : 1.0);
-      const int64_t roff = partial ? (int64_t)row * g.N : off2(row, g.cm);
+      const int row = m0 + wm * 128 + i * 16 + (lane >> 4) * 4 + r;
+      ro[r] = row < g.M ? (partial ? (int64_t)row * g.N : off2(row, g.cm)) : -1;
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        int col = n0 + wn * 64 + j * 16 + (lane & 15);
-        if (col < g.N) {
-          double v = sr * (double)acc[i][j][r] * sc[j];
-          const int64_t o = roff + coff[j];
-          if (Dp && beta != 0.0) v += beta * Dp[o];
-          C[o] = v;
-        }
-      }
+      for (int j = 0; j < 4; ++j)
+        dv[r][j] = (Dp && beta != 0.0 && ro[r] >= 0 && coff[j] >= 0) ? Dp[ro[r] + coff[j]] : 0.0;
     }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      if (ro[r] < 0) continue;
+      const int row = m0 + wm * 128 + i * 16 + (lane >> 4) * 4 + r;
+      const double sr = g.alpha * (ia ? (double)ia[row] : 1.0);
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        if (coff[j] >= 0) C[ro[r] + coff[j]] = sr * (double)acc[i][j][r] * sc[j] + beta * dv[r][j];
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------------- FP64, large
@@ -1119,29 +1164,35 @@ __global__ __launch_bounds__(512) void gemm_f64_big_kernel(GemmF64Args g) {
     return;
   }
   const int64_t cb = off2(batch, g.cb);
+  const bool addend = g.D && g.beta != 0.0;
   int64_t coff[4];
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
     int col = n0 + wn * 64 + j * 16 + (lane & 15);
-    coff[j] = col < g.N ? off2(col, g.cn) : 0;
+    coff[j] = col < g.N ? off2(col, g.cn) : -1;
   }
+  // per 16-row group: all addend loads first, then the stores (D may be C; each element is read
+  // and written by this lane only), so the loads overlap instead of waiting on each store
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < 4; ++i) {
+    int64_t ro[4];
+    double dv[4][4];
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      int row = m0 + wm * 64 + i * 16 + r * 4 + (lane >> 4);
-      if (row >= g.M) continue;
-      const int64_t ro = cb + off2(row, g.cm);
+      const int row = m0 + wm * 64 + i * 16 + r * 4 + (lane >> 4);
+      ro[r] = row < g.M ? cb + off2(row, g.cm) : -1;
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        int col = n0 + wn * 64 + j * 16 + (lane & 15);
-        if (col < g.N) {
-          double v = g.alpha * acc[i][j][r];
-          if (g.D && g.beta != 0.0) v += g.beta * g.D[ro + coff[j]];
-          g.C[ro + coff[j]] = v;
-        }
-      }
+      for (int j = 0; j < 4; ++j)
+        dv[r][j] = (addend && ro[r] >= 0 && coff[j] >= 0) ? g.D[ro[r] + coff[j]] : 0.0;
     }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      if (ro[r] < 0) continue;
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        if (coff[j] >= 0) g.C[ro[r] + coff[j]] = g.alpha * acc[i][j][r] + g.beta * dv[r][j];
+    }
+  }
 }
 
 }  // namespace
