@@ -43,10 +43,19 @@ __device__ inline TileCoord tile_of(int wg, int nwg, int tiles_m, int tiles_n) {
 constexpr int BK64 = 16;
 constexpr int PAD64 = 4;
 
-struct Idx2 { int n2; int64_t s1, s2; };  // extent of the inner run, strides of outer/inner run
+// Extent of the inner run and strides of the outer / inner run, with a multiplier for dividing
+// by the extent: for 0 <= i < 2^31, i / n2 = (i * mg) >> sh where sh = 31 + ceil(log2 n2) and
+// mg = ceil(2^sh / n2) (Granlund and Montgomery, "Division by invariant integers using
+// multiplication", PLDI 1994: mg * n2 lies in [2^sh, 2^sh + 2^(sh-31)), which makes the product
+// exact for 31-bit dividends).  Kernels index with it instead of a hardware-less integer divide.
+struct Idx2 { int n2; int64_t s1, s2; uint64_t mg; int sh; };
+
+__device__ inline int div2(int i, const Idx2& d) {
+  return (int)(((uint64_t)(uint32_t)i * d.mg) >> d.sh);
+}
 
 __device__ inline int64_t off2(int i, const Idx2& d) {
-  int q = i / d.n2;
+  int q = div2(i, d);
   return q * d.s1 + (int64_t)(i - q * d.n2) * d.s2;
 }
 
@@ -55,7 +64,7 @@ struct Walk2 {
   int q, r;
   int64_t off;
   __device__ inline Walk2(int i, const Idx2& d) {
-    q = i / d.n2; r = i - q * d.n2; off = q * d.s1 + (int64_t)r * d.s2;
+    q = div2(i, d); r = i - q * d.n2; off = q * d.s1 + (int64_t)r * d.s2;
   }
   __device__ inline void next(const Idx2& d) {
     ++r; off += d.s2;
@@ -583,29 +592,6 @@ __global__ __launch_bounds__(256) void lowk_f64_kernel(GemmF64Args g) {
   // operand tiles: along k when that is the operand's unit stride, else along m / n
   const bool ak1 = g.ak.n2 > 1 ? g.ak.s2 == 1 : g.ak.s1 == 1;
   const bool bk1 = g.bk.n2 > 1 ? g.bk.s2 == 1 : g.bk.s1 == 1;
-  // all staging loads are issued before any LDS store, so their latencies overlap
-  constexpr int LQ = LKK * LKT / 256;
-  double ra[LQ], rb[LQ];
-#pragma unroll
-  for (int u = 0; u < LQ; ++u) {
-    const int q = tid + 256 * u;
-    int k, m, n;
-    if (ak1) { m = q / K; k = q - m * K; } else { k = q / LKT; m = q - k * LKT; }
-    ra[u] = (q < K * LKT && m0 + m < g.M) ? A[off2(m0 + m, g.am) + off2(k, g.ak)] : 0.0;
-    if (bk1) { n = q / K; k = q - n * K; } else { k = q / LKT; n = q - k * LKT; }
-    rb[u] = (q < K * LKT && n0 + n < g.N) ? B[off2(k, g.bk) + off2(n0 + n, g.bn)] : 0.0;
-  }
-#pragma unroll
-  for (int u = 0; u < LQ; ++u) {
-    const int q = tid + 256 * u;
-    if (q >= K * LKT) continue;
-    int k, m, n;
-    if (ak1) { m = q / K; k = q - m * K; } else { k = q / LKT; m = q - k * LKT; }
-    As[k][m] = ra[u];
-    if (bk1) { n = q / K; k = q - n * K; } else { k = q / LKT; n = q - k * LKT; }
-    Bs[k][n] = rb[u];
-  }
-  __syncthreads();
   const int tm = LN ? (tid >> 4) : (tid & 15), tn = LN ? (tid & 15) : (tid >> 4);
   // output offsets and the addend are read before the operand staging, so their latency overlaps
   // it (each output is read and written by this thread only, so D may be C)
@@ -628,6 +614,25 @@ __global__ __launch_bounds__(256) void lowk_f64_kernel(GemmF64Args g) {
 #pragma unroll
     for (int y = 0; y < 4; ++y)
       dv[x][y] = (addend && roff[x] >= 0 && n0 + tn + 16 * y < g.N) ? g.D[roff[x] + coff[y]] : 0.0;
+  // all staging loads are issued before any LDS store, so their latencies overlap; lanes run
+  // along each operand's unit-stride index (k: 32 lanes x 8 rows, else 64 lanes x 4 k)
+  constexpr int LQ = LKK * LKT / 256;
+  double ra[LQ], rb[LQ];
+#pragma unroll
+  for (int u = 0; u < LQ; ++u) {
+    const int ka = ak1 ? (tid & 31) : (tid >> 6) + 4 * u, m = ak1 ? (tid >> 5) + 8 * u : tid & 63;
+    const int kb = bk1 ? (tid & 31) : (tid >> 6) + 4 * u, n = bk1 ? (tid >> 5) + 8 * u : tid & 63;
+    ra[u] = (ka < K && m0 + m < g.M) ? A[off2(m0 + m, g.am) + off2(ka, g.ak)] : 0.0;
+    rb[u] = (kb < K && n0 + n < g.N) ? B[off2(kb, g.bk) + off2(n0 + n, g.bn)] : 0.0;
+  }
+#pragma unroll
+  for (int u = 0; u < LQ; ++u) {
+    const int ka = ak1 ? (tid & 31) : (tid >> 6) + 4 * u, m = ak1 ? (tid >> 5) + 8 * u : tid & 63;
+    const int kb = bk1 ? (tid & 31) : (tid >> 6) + 4 * u, n = bk1 ? (tid >> 5) + 8 * u : tid & 63;
+    As[ka][m] = ra[u];
+    Bs[kb][n] = rb[u];
+  }
+  __syncthreads();
   double acc[4][4];
 #pragma unroll
   for (int x = 0; x < 4; ++x)
@@ -1031,7 +1036,7 @@ struct F64Fill {
       } else {
         k = k0 + (w * NI + v * PER_K) / PER_K;
       }
-      const int qk = k / dk.n2;
+      const int qk = div2(k, dk);
       wk[v] = k; wr[v] = k - qk * dk.n2;
       woff[v] = qk * dk.s1 + (int64_t)wr[v] * dk.s2;
     }
@@ -1200,7 +1205,12 @@ __global__ __launch_bounds__(512) void gemm_f64_big_kernel(GemmF64Args g) {
 extern "C" {
 
 static inline Idx2 mk(int64_t n2, int64_t s1, int64_t s2) {
-  Idx2 d; d.n2 = n2 > 0 ? (int)n2 : 1; d.s1 = s1; d.s2 = s2; return d;
+  Idx2 d; d.n2 = n2 > 0 ? (int)n2 : 1; d.s1 = s1; d.s2 = s2;
+  int l = 0;
+  while ((int64_t(1) << l) < d.n2) ++l;
+  d.sh = 31 + l;
+  d.mg = ((uint64_t(1) << d.sh) + (uint64_t)d.n2 - 1) / (uint64_t)d.n2;
+  return d;
 }
 
 // Kernel choice and split-K factor for a contraction of the given extents; returns the number of
