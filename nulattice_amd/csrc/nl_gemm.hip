@@ -43,11 +43,8 @@ __device__ inline TileCoord tile_of(int wg, int nwg, int tiles_m, int tiles_n) {
 constexpr int BK64 = 16;
 constexpr int PAD64 = 4;
 
-// Extent of the inner run and strides of the outer / inner run, with a multiplier for dividing
-// by the extent: for 0 <= i < 2^31, i / n2 = (i * mg) >> sh where sh = 31 + ceil(log2 n2) and
-// mg = ceil(2^sh / n2) (Granlund and Montgomery, "Division by invariant integers using
-// multiplication", PLDI 1994: mg * n2 lies in [2^sh, 2^sh + 2^(sh-31)), which makes the product
-// exact for 31-bit dividends).  Kernels index with it instead of a hardware-less integer divide.
+// Extent of the inner run and strides of the outer / inner run, with the multiply-shift divisor of
+// the extent (as FastDiv: i / n2 = (i * mg) >> sh for 0 <= i < 2^31).
 struct Idx2 { int n2; int64_t s1, s2; uint64_t mg; int sh; };
 
 __device__ inline int div2(int i, const Idx2& d) {
@@ -1205,11 +1202,8 @@ __global__ __launch_bounds__(512) void gemm_f64_big_kernel(GemmF64Args g) {
 extern "C" {
 
 static inline Idx2 mk(int64_t n2, int64_t s1, int64_t s2) {
-  Idx2 d; d.n2 = n2 > 0 ? (int)n2 : 1; d.s1 = s1; d.s2 = s2;
-  int l = 0;
-  while ((int64_t(1) << l) < d.n2) ++l;
-  d.sh = 31 + l;
-  d.mg = ((uint64_t(1) << d.sh) + (uint64_t)d.n2 - 1) / (uint64_t)d.n2;
+  const FastDiv f = nl_fastdiv(n2);
+  Idx2 d; d.n2 = f.d; d.s1 = s1; d.s2 = s2; d.mg = f.mg; d.sh = f.sh;
   return d;
 }
 

@@ -39,20 +39,32 @@ struct PermArgs {
   int rank;
   int64_t shape[6];    // output shape (row-major, contiguous output)
   int64_t istr[6];     // input element stride of each output dimension
+  FastDiv fd[6];       // divisors of the shape (used when total < 2^31)
   int64_t total;
   double alpha, beta;
   const double* in;
   double* out;
 };
 
+template <bool SMALL>   // SMALL: total < 2^31, coordinates by multiply-shift division
 __global__ void permute_kernel(PermArgs a) {
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < a.total;
        t += (int64_t)gridDim.x * blockDim.x) {
-    int64_t rem = t, off = 0;
-    for (int d = a.rank - 1; d >= 0; --d) {
-      int64_t q = rem / a.shape[d];
-      off += (rem - q * a.shape[d]) * a.istr[d];
-      rem = q;
+    int64_t off = 0;
+    if (SMALL) {
+      int rem = (int)t;
+      for (int d = a.rank - 1; d >= 0; --d) {
+        const int q = fdiv(rem, a.fd[d]);
+        off += (int64_t)(rem - q * a.fd[d].d) * a.istr[d];
+        rem = q;
+      }
+    } else {
+      int64_t rem = t;
+      for (int d = a.rank - 1; d >= 0; --d) {
+        int64_t q = rem / a.shape[d];
+        off += (rem - q * a.shape[d]) * a.istr[d];
+        rem = q;
+      }
     }
     double v = a.alpha * a.in[off];
     a.out[t] = (a.beta != 0.0) ? v + a.beta * a.out[t] : v;
@@ -61,6 +73,7 @@ __global__ void permute_kernel(PermArgs a) {
 
 struct ConvArgs {
   int64_t R2, C, C2, ldo;   // rows = R1*R2 (grid), cols = C1*C2 = C
+  FastDiv fr2, fc2;         // divisors R2, C2 (rows and columns are below 2^31)
   int64_t s[4];             // strides of (r1, r2, c1, c2)
   const double* in;
   uint16_t* out;
@@ -94,11 +107,11 @@ __device__ inline int scale_exp(float amax, float target) {
 __global__ __launch_bounds__(NT) void convert_rows_kernel(ConvArgs a) {
   __shared__ float red[NT / 64];
   int64_t r = blockIdx.x;
-  int64_t r1 = r / a.R2, r2 = r - r1 * a.R2;
+  int64_t r1 = fdiv((int)r, a.fr2), r2 = r - r1 * a.R2;
   const double* base = a.in + r1 * a.s[0] + r2 * a.s[1];
   float m = 0.f;
   for (int64_t c = threadIdx.x; c < a.C; c += NT) {
-    int64_t c1 = c / a.C2, c2 = c - c1 * a.C2;
+    int64_t c1 = fdiv((int)c, a.fc2), c2 = c - c1 * a.C2;
     m = fmaxf(m, (float)fabs(base[c1 * a.s[2] + c2 * a.s[3]]));
   }
   m = wave_max(m);
@@ -112,7 +125,7 @@ __global__ __launch_bounds__(NT) void convert_rows_kernel(ConvArgs a) {
   for (int64_t c = threadIdx.x; c < a.ldo; c += NT) {
     float v = 0.f;
     if (c < a.C) {
-      int64_t c1 = c / a.C2, c2 = c - c1 * a.C2;
+      int64_t c1 = fdiv((int)c, a.fc2), c2 = c - c1 * a.C2;
       v = (float)base[c1 * a.s[2] + c2 * a.s[3]] * sc;
     }
     o[c] = a.bf16 ? to_bf16_bits(v) : to_f16_bits(v);
@@ -123,14 +136,14 @@ __global__ __launch_bounds__(NT) void convert_rows_kernel(ConvArgs a) {
 // a row scale): a 2D grid of (row, 2048-column chunk), so long rows are spread over workgroups.
 constexpr int CAST_CHUNK = 2048;
 __global__ __launch_bounds__(NT) void cast_rows_kernel(ConvArgs a) {
-  const int64_t r = blockIdx.x, r1 = r / a.R2, r2 = r - r1 * a.R2;
+  const int64_t r = blockIdx.x, r1 = fdiv((int)r, a.fr2), r2 = r - r1 * a.R2;
   const double* base = a.in + r1 * a.s[0] + r2 * a.s[1];
   uint16_t* o = a.out + r * a.ldo;
   const int64_t c0 = (int64_t)blockIdx.y * CAST_CHUNK, c1e = min(a.ldo, c0 + CAST_CHUNK);
   for (int64_t c = c0 + threadIdx.x; c < c1e; c += NT) {
     float v = 0.f;
     if (c < a.C) {
-      int64_t c1 = c / a.C2, c2 = c - c1 * a.C2;
+      int64_t c1 = fdiv((int)c, a.fc2), c2 = c - c1 * a.C2;
       v = (float)base[c1 * a.s[2] + c2 * a.s[3]];
     }
     o[c] = a.bf16 ? to_bf16_bits(v) : to_f16_bits(v);
@@ -154,12 +167,12 @@ __global__ __launch_bounds__(NT) void amax_tiles_kernel(ConvArgs a, int64_t R, u
   const int64_t r = blockIdx.x * 64 + tr, c0 = blockIdx.y * 64;
   float m = 0.f;
   if (r < R) {
-    int64_t r1 = r / a.R2, r2 = r - r1 * a.R2;
+    int64_t r1 = fdiv((int)r, a.fr2), r2 = r - r1 * a.R2;
     const double* base = a.in + r1 * a.s[0] + r2 * a.s[1];
     for (int cc = tg; cc < 64; cc += 4) {
       int64_t c = c0 + cc;
       if (c < a.C) {
-        int64_t c1 = c / a.C2, c2 = c - c1 * a.C2;
+        int64_t c1 = fdiv((int)c, a.fc2), c2 = c - c1 * a.C2;
         m = fmaxf(m, (float)fabs(base[c1 * a.s[2] + c2 * a.s[3]]));
       }
     }
@@ -179,13 +192,13 @@ __global__ __launch_bounds__(NT) void convert_tiles_kernel(ConvArgs a, int64_t R
   const int64_t r0 = blockIdx.x * 64, c0 = blockIdx.y * 64, r = r0 + tr;
   if (r < R) {
     float sc = bits ? ldexpf(1.f, scale_exp(__uint_as_float(bits[r]), a.target)) : 1.f;
-    int64_t r1 = r / a.R2, r2 = r - r1 * a.R2;
+    int64_t r1 = fdiv((int)r, a.fr2), r2 = r - r1 * a.R2;
     const double* base = a.in + r1 * a.s[0] + r2 * a.s[1];
     for (int cc = tg; cc < 64; cc += 4) {
       int64_t c = c0 + cc;
       float v = 0.f;
       if (c < a.C) {
-        int64_t c1 = c / a.C2, c2 = c - c1 * a.C2;
+        int64_t c1 = fdiv((int)c, a.fc2), c2 = c - c1 * a.C2;
         v = (float)base[c1 * a.s[2] + c2 * a.s[3]] * sc;
       }
       tile[tr][cc] = a.bf16 ? to_bf16_bits(v) : to_f16_bits(v);
@@ -206,15 +219,20 @@ __global__ __launch_bounds__(NT) void iscale_kernel(int64_t R, float target, flo
   }
 }
 
-// tau[a,b,i,j] = t2[a,b,i,j] + t1[a,i] t1[b,j] - t1[a,j] t1[b,i]
-__global__ void tau_kernel(int64_t p, int64_t h, const double* t1, const double* t2, double* tau) {
-  int64_t total = p * p * h * h;
-  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
-       t += (int64_t)gridDim.x * blockDim.x) {
-    int64_t j = t % h, r = t / h, i = r % h;
-    r /= h;
-    int64_t b = r % p, a = r / p;
-    tau[t] = t2[t] + t1[a * h + i] * t1[b * h + j] - t1[a * h + j] * t1[b * h + i];
+// tau[a,b,i,j] = t2[a,b,i,j] + t1[a,i] t1[b,j] - t1[a,j] t1[b,i]: workgroups walk the pairs
+// (a, b), threads the h x h block (contiguous in t2 and tau).
+__global__ __launch_bounds__(NT) void tau_kernel(int p, int h, FastDiv fp, FastDiv fh,
+                                                 const double* t1, const double* t2,
+                                                 double* tau) {
+  const int hh = h * h;
+  for (int q = blockIdx.x; q < p * p; q += gridDim.x) {
+    const int a = fdiv(q, fp), b = q - a * p;
+    const double* x = t2 + (int64_t)q * hh;
+    double* y = tau + (int64_t)q * hh;
+    for (int e = threadIdx.x; e < hh; e += NT) {
+      const int i = fdiv(e, fh), j = e - i * h;
+      y[e] = x[e] + t1[a * h + i] * t1[b * h + j] - t1[a * h + j] * t1[b * h + i];
+    }
   }
 }
 
@@ -248,27 +266,27 @@ __global__ __launch_bounds__(NT) void segsum_kernel(int64_t S, int64_t X, int G,
 }
 
 // out[r][c] = idx[r] >= 0 ? src[idx[r]*lds + c] : 0   (r < R, c < Cn), out contiguous
-__global__ void gather_rows_kernel(int64_t R, int64_t Cn, const int64_t* idx, const double* src,
-                                   int64_t lds, double* out) {
-  int64_t total = R * Cn;
-  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
-       t += (int64_t)gridDim.x * blockDim.x) {
-    int64_t r = t / Cn, c = t - r * Cn;
-    int64_t k = idx[r];
-    out[t] = (k >= 0) ? src[k * lds + c] : 0.0;
+__global__ __launch_bounds__(NT) void gather_rows_kernel(int64_t R, int64_t Cn,
+                                                         const int64_t* idx, const double* src,
+                                                         int64_t lds, double* out) {
+  for (int64_t r = blockIdx.x; r < R; r += gridDim.x) {
+    const int64_t k = idx[r];
+    double* o = out + r * Cn;
+    for (int64_t c = threadIdx.x; c < Cn; c += NT) o[c] = (k >= 0) ? src[k * lds + c] : 0.0;
   }
 }
 
 // out[idx[r]*ldo + c] += alpha * src[r*Cn + c]   (rows with idx < 0 skipped; idx unique)
-__global__ void scatter_add_rows_kernel(int64_t R, int64_t Cn, const int64_t* idx,
-                                        const double* src, double alpha, double* out,
-                                        int64_t ldo) {
-  int64_t total = R * Cn;
-  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
-       t += (int64_t)gridDim.x * blockDim.x) {
-    int64_t r = t / Cn, c = t - r * Cn;
-    int64_t k = idx[r];
-    if (k >= 0) out[k * ldo + c] += alpha * src[t];
+__global__ __launch_bounds__(NT) void scatter_add_rows_kernel(int64_t R, int64_t Cn,
+                                                              const int64_t* idx,
+                                                              const double* src, double alpha,
+                                                              double* out, int64_t ldo) {
+  for (int64_t r = blockIdx.x; r < R; r += gridDim.x) {
+    const int64_t k = idx[r];
+    if (k < 0) continue;
+    const double* x = src + r * Cn;
+    double* o = out + k * ldo;
+    for (int64_t c = threadIdx.x; c < Cn; c += NT) o[c] += alpha * x[c];
   }
 }
 
@@ -288,6 +306,7 @@ __global__ void update_t1_kernel(int64_t p, int64_t h, const double* t1, const d
 
 struct AssembleArgs {
   int64_t p, h;
+  FastDiv fp, fh;                       // divisors p, h
   const double *S0, *Sab, *Sij, *Spp;   // Sij, Spp in (a,i),(b,j) layout; any may be null
   const double *Xpp, *Xhh;              // dressed diagonals (full p x p, h x h matrices)
   const double *t2, *vpphh, *t1n;       // old t2, v_pphh, new t1 (for the fused energy)
@@ -360,10 +379,10 @@ __global__ __launch_bounds__(NT) void assemble_pair_kernel(AssembleArgs g) {
     r_acc += H * H;
   };
   for (int64_t q = blockIdx.x; q < p * p; q += gridDim.x) {
-    const int64_t a = q / p, b = q - a * p;
+    const int64_t a = fdiv((int)q, g.fp), b = q - a * p;
     if (a > b) continue;                                    // uniform in the block
     for (int e = threadIdx.x; e < hh; e += NT) {
-      const int i = e / h, j = e - i * h;
+      const int i = fdiv(e, g.fh), j = e - i * h;
       const int64_t oab = (a * g.h + i) * ph + b * g.h + j, oba = (b * g.h + i) * ph + a * g.h + j;
       SiA[i][j] = g.Sij ? g.Sij[oab] : 0.0;
       SiB[i][j] = g.Sij ? g.Sij[oba] : 0.0;
@@ -372,7 +391,7 @@ __global__ __launch_bounds__(NT) void assemble_pair_kernel(AssembleArgs g) {
     }
     __syncthreads();
     for (int e = threadIdx.x; e < hh; e += NT) {
-      const int i = e / h, j = e - i * h;
+      const int i = fdiv(e, g.fh), j = e - i * h;
       const int64_t tab = (a * p + b) * hh + e, tba = (b * p + a) * hh + e;
       double Hab = g.S0 ? g.S0[tab] : 0.0;
       double Hba = g.S0 ? g.S0[tba] : 0.0;
@@ -402,15 +421,21 @@ __global__ __launch_bounds__(NT) void assemble_pair_kernel(AssembleArgs g) {
 //   gSij[a,i,b,j] = L[a,b,i,j] - L[a,b,j,i]                         (layout [a][i][b][j])
 //   gSpp[a,i,b,j] = L[a,b,i,j] - L[b,a,i,j] - L[a,b,j,i] + L[b,a,j,i] (layout [a][i][b][j])
 // Threads walk the [a][i][b][j] order; j is innermost in every output and in two of the reads.
-__global__ __launch_bounds__(NT) void assemble_adj_kernel(int64_t p, int64_t h, const double* L,
+template <bool SMALL>   // SMALL: p*p*h*h < 2^31, coordinates by multiply-shift division
+__global__ __launch_bounds__(NT) void assemble_adj_kernel(int64_t p, int64_t h, FastDiv fp,
+                                                          FastDiv fh, const double* L,
                                                           double* gSab, double* gSij,
                                                           double* gSpp) {
   const int64_t total = p * p * h * h;
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
        t += (int64_t)gridDim.x * blockDim.x) {
-    int64_t j = t % h, r = t / h, b = r % p;
-    r /= p;
-    int64_t i = r % h, a = r / h;
+    int64_t a, i, b, j;
+    if (SMALL) {
+      const int r = fdiv((int)t, fh), r2 = fdiv(r, fp), a_ = fdiv(r2, fh);
+      j = t - (int64_t)r * h; b = r - (int64_t)r2 * p; a = a_; i = r2 - (int64_t)a_ * h;
+    } else {
+      j = t % h; int64_t r = t / h; b = r % p; r /= p; i = r % h; a = r / h;
+    }
     double v0 = L[((a * p + b) * h + i) * h + j];
     double v1 = L[((b * p + a) * h + i) * h + j];
     double v2 = L[((a * p + b) * h + j) * h + i];
@@ -422,16 +447,22 @@ __global__ __launch_bounds__(NT) void assemble_adj_kernel(int64_t p, int64_t h, 
 }
 
 // E2 partials for E = f.t1 + 1/4 v.t2 + 1/2 v t1 t1 (the f.t1 part is added by finalize).
-__global__ __launch_bounds__(NT) void energy_part_kernel(int64_t p, int64_t h, const double* v,
+template <bool SMALL>   // SMALL: p*p*h*h < 2^31, coordinates by multiply-shift division
+__global__ __launch_bounds__(NT) void energy_part_kernel(int64_t p, int64_t h, FastDiv fp,
+                                                         FastDiv fh, const double* v,
                                                          const double* t2, const double* t1,
                                                          double* part) {
   const int64_t total = p * p * h * h;
   double acc = 0.0;
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
        t += (int64_t)gridDim.x * blockDim.x) {
-    int64_t j = t % h, r = t / h, i = r % h;
-    r /= h;
-    int64_t b = r % p, a = r / p;
+    int64_t a, i, b, j;
+    if (SMALL) {
+      const int r = fdiv((int)t, fh), r2 = fdiv(r, fh), a_ = fdiv(r2, fp);
+      j = t - (int64_t)r * h; i = r - (int64_t)r2 * h; a = a_; b = r2 - (int64_t)a_ * p;
+    } else {
+      j = t % h; int64_t r = t / h; i = r % h; r /= h; b = r % p; a = r / p;
+    }
     double x = v[t];
     acc += 0.25 * x * t2[t] + 0.5 * x * t1[a * h + i] * t1[b * h + j];
   }
@@ -498,11 +529,17 @@ int nl_permute(int rank, const int64_t* out_shape, const int64_t* in_strides, do
     if (out_shape[d] < 0) return NL_EARG;
     a.shape[d] = out_shape[d];
     a.istr[d] = in_strides[d];
+    a.fd[d] = nl_fastdiv(out_shape[d] < INT32_MAX ? out_shape[d] : 1);
     a.total *= out_shape[d];
   }
   if (a.total == 0) return NL_OK;
   a.alpha = alpha; a.beta = beta; a.in = in; a.out = out;
-  hipLaunchKernelGGL(permute_kernel, dim3(grid_for(a.total)), dim3(NT), 0, (hipStream_t)stream, a);
+  if (a.total < INT32_MAX)
+    hipLaunchKernelGGL((permute_kernel<true>), dim3(grid_for(a.total)), dim3(NT), 0,
+                       (hipStream_t)stream, a);
+  else
+    hipLaunchKernelGGL((permute_kernel<false>), dim3(grid_for(a.total)), dim3(NT), 0,
+                       (hipStream_t)stream, a);
   NL_LAUNCH_CHECK();
   return NL_OK;
 }
@@ -515,9 +552,10 @@ int nl_convert_rows(int dtype, int64_t R1, int64_t R2, int64_t C1, int64_t C2,
     return NL_EARG;
   if (R1 < 0 || R2 < 1 || C1 < 0 || C2 < 1 || ldo < C1 * C2 || (ldo % 8)) return NL_EARG;
   if (R1 * R2 == 0) return NL_OK;
-  if (R1 * R2 > INT32_MAX) return NL_EARG;
+  if (R1 * R2 > INT32_MAX || ldo > INT32_MAX - CAST_CHUNK) return NL_EARG;
   ConvArgs a;
   a.R2 = R2; a.C = C1 * C2; a.C2 = C2; a.ldo = ldo;
+  a.fr2 = nl_fastdiv(R2); a.fc2 = nl_fastdiv(C2);
   for (int d = 0; d < 4; ++d) a.s[d] = strides4[d];
   a.in = in; a.out = (uint16_t*)out; a.iscale = iscale; a.target = target; a.bf16 = dtype == 0;
   hipStream_t st = (hipStream_t)stream;
@@ -559,8 +597,10 @@ int nl_tau(int64_t p, int64_t h, const double* t1, const double* t2, double* tau
   if (p < 0 || h < 0 || !t1 || !t2 || !tau) return NL_EARG;
   int64_t n = p * p * h * h;
   if (n == 0) return NL_OK;
-  hipLaunchKernelGGL(tau_kernel, dim3(grid_for(n)), dim3(NT), 0, (hipStream_t)stream, p, h, t1,
-                     t2, tau);
+  if (p * p > INT32_MAX || h * h > INT32_MAX || p * h > INT32_MAX) return NL_EARG;
+  hipLaunchKernelGGL(tau_kernel, dim3((unsigned)std::min<int64_t>(p * p, 65536)), dim3(NT), 0,
+                     (hipStream_t)stream, (int)p, (int)h, nl_fastdiv(p), nl_fastdiv(h), t1, t2,
+                     tau);
   NL_LAUNCH_CHECK();
   return NL_OK;
 }
@@ -586,8 +626,8 @@ int nl_gather_rows(int64_t R, int64_t Cn, const int64_t* idx, const double* src,
   if (R < 0 || Cn < 0) return NL_EARG;
   if (R * Cn == 0) return NL_OK;
   if (!idx || !src || !out) return NL_EARG;
-  hipLaunchKernelGGL(gather_rows_kernel, dim3(grid_for(R * Cn)), dim3(NT), 0, (hipStream_t)stream,
-                     R, Cn, idx, src, lds, out);
+  hipLaunchKernelGGL(gather_rows_kernel, dim3((unsigned)std::min<int64_t>(R, 65536)), dim3(NT), 0,
+                     (hipStream_t)stream, R, Cn, idx, src, lds, out);
   NL_LAUNCH_CHECK();
   return NL_OK;
 }
@@ -597,8 +637,8 @@ int nl_scatter_add_rows(int64_t R, int64_t Cn, const int64_t* idx, const double*
   if (R < 0 || Cn < 0) return NL_EARG;
   if (R * Cn == 0) return NL_OK;
   if (!idx || !src || !out) return NL_EARG;
-  hipLaunchKernelGGL(scatter_add_rows_kernel, dim3(grid_for(R * Cn)), dim3(NT), 0,
-                     (hipStream_t)stream, R, Cn, idx, src, alpha, out, ldo);
+  hipLaunchKernelGGL(scatter_add_rows_kernel, dim3((unsigned)std::min<int64_t>(R, 65536)),
+                     dim3(NT), 0, (hipStream_t)stream, R, Cn, idx, src, alpha, out, ldo);
   NL_LAUNCH_CHECK();
   return NL_OK;
 }
@@ -628,7 +668,8 @@ int nl_assemble_t2(int64_t p, int64_t h, const double* S0, const double* Sab, co
   g.p = p; g.h = h; g.S0 = S0; g.Sab = Sab; g.Sij = Sij; g.Spp = Spp; g.Xpp = Xpp; g.Xhh = Xhh;
   g.t2 = t2; g.vpphh = vpphh; g.t1n = t1n; g.mixing = mixing; g.t2n = t2n; g.part = part;
   g.H2out = H2out;
-  if (h <= AH && (Sij || Spp))
+  g.fp = nl_fastdiv(p); g.fh = nl_fastdiv(h);
+  if (h <= AH && (Sij || Spp) && p * p < INT32_MAX)
     hipLaunchKernelGGL(assemble_pair_kernel, dim3(grid_for(n, 4096)), dim3(NT), 0,
                        (hipStream_t)stream, g);
   else
@@ -643,8 +684,14 @@ int nl_assemble_t2_adj(int64_t p, int64_t h, const double* L, double* gSab, doub
   if (p < 0 || h < 0 || !L) return NL_EARG;
   int64_t n = p * p * h * h;
   if (n == 0) return NL_OK;
-  hipLaunchKernelGGL(assemble_adj_kernel, dim3(grid_for(n)), dim3(NT), 0, (hipStream_t)stream, p,
-                     h, L, gSab, gSij, gSpp);
+  if (n < INT32_MAX)
+    hipLaunchKernelGGL((assemble_adj_kernel<true>), dim3(grid_for(n)), dim3(NT), 0,
+                       (hipStream_t)stream, p, h, nl_fastdiv(p), nl_fastdiv(h), L, gSab, gSij,
+                       gSpp);
+  else
+    hipLaunchKernelGGL((assemble_adj_kernel<false>), dim3(grid_for(n)), dim3(NT), 0,
+                       (hipStream_t)stream, p, h, nl_fastdiv(1), nl_fastdiv(1), L, gSab, gSij,
+                       gSpp);
   NL_LAUNCH_CHECK();
   return NL_OK;
 }
@@ -654,8 +701,14 @@ int nl_energy_part(int64_t p, int64_t h, const double* vpphh, const double* t2, 
   if (p < 0 || h < 0 || !vpphh || !t2 || !t1 || !part) return NL_EARG;
   int64_t n = p * p * h * h;
   if (n == 0) return NL_OK;
-  hipLaunchKernelGGL(energy_part_kernel, dim3(grid_for(n, 4096)), dim3(NT), 0, (hipStream_t)stream,
-                     p, h, vpphh, t2, t1, part);
+  if (n < INT32_MAX)
+    hipLaunchKernelGGL((energy_part_kernel<true>), dim3(grid_for(n, 4096)), dim3(NT), 0,
+                       (hipStream_t)stream, p, h, nl_fastdiv(p), nl_fastdiv(h), vpphh, t2, t1,
+                       part);
+  else
+    hipLaunchKernelGGL((energy_part_kernel<false>), dim3(grid_for(n, 4096)), dim3(NT), 0,
+                       (hipStream_t)stream, p, h, nl_fastdiv(1), nl_fastdiv(1), vpphh, t2, t1,
+                       part);
   NL_LAUNCH_CHECK();
   return NL_OK;
 }
