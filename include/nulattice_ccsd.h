@@ -56,6 +56,17 @@ int nl_gemm_lp(int dtype, int64_t M, int64_t N, int64_t K, double alpha, const v
                const double* D, double* C, int64_t ldc, int64_t strideC, int64_t batch,
                int64_t splits, double* ws, void* stream);
 
+/* Row-scatter form of nl_gemm_lp: for each batch b and row m with crow[b*M + m] >= 0,
+ *   C[crow[b*M + m]*ldc + n] += alpha * ia[b][m] * (A[b] B[b]^T)[m][n] * ib[b][n]   (n < N).
+ * The packed pair-block products of the pp ladder and of R56 (ccDgrams.py:215-237 and :13-45,
+ * regrouped by connected pair blocks) add their rows straight into S0 / N56 with it, with no
+ * FP64 block result and no separate scatter pass.  crow must not name a row twice (over all
+ * batches); ia / ib may be null. */
+int nl_gemm_lp_rows(int dtype, int64_t M, int64_t N, int64_t K, double alpha, const void* A,
+                    int64_t lda, int64_t strideA, const float* ia, int64_t strideIa, const void* B,
+                    int64_t ldb, int64_t strideB, const float* ib, int64_t strideIb, double* C,
+                    int64_t ldc, const int64_t* crow, int64_t batch, void* stream);
+
 /* nl_gemm_lp (batch 1) with C and D addressed like nl_contract_f64's output: cdims = {M1, M2,
  * N1, N2} (M = M1*M2, N = N1*N2), sc = {m1, m2, n1, n2} strides, C(m,n) at
  * C[(m/M2)*sc0 + (m%M2)*sc1 + (n/N2)*sc2 + (n%N2)*sc3].  Lets the precision form accumulate
@@ -81,6 +92,12 @@ int nl_permute(int rank, const int64_t* out_shape, const int64_t* in_strides, do
 int nl_convert_rows(int dtype, int64_t R1, int64_t R2, int64_t C1, int64_t C2,
                     const int64_t* strides4, const double* in, void* out, int64_t ldo,
                     float* iscale, float target, void* stream);
+
+/* out[b][n][k] (BF16) = src[idx[b*kb + k]*lds + n] for n < ncol, k < kb (0 where idx < 0), and 0
+ * for kb <= k < ldk (ldk a multiple of 8): the row gather of the packed pair-block products
+ * (ccDgrams.py:215-237, :13-45) fused with the transposing BF16 conversion of their operand. */
+int nl_gather_convert(int64_t nb, int64_t kb, int64_t ncol, const int64_t* idx, const double* src,
+                      int64_t lds, void* out, int64_t ldk, void* stream);
 
 /* tau[a,b,i,j] = t2[a,b,i,j] + t1[a,i] t1[b,j] - t1[a,j] t1[b,i]: the t2 + pIJ(t1 t1) combination
  * that the pp ladder applies to (ccDgrams.py:215-237 with coupled_cluster.py:524-526). */

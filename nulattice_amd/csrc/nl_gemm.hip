@@ -692,6 +692,8 @@ struct GemmLPArgs {
   const double* D;                      // addend with C's layout (may be C or null)
   int tiles_m, tiles_n, splits, kchunk;
   double* ws;                           // split-K partials [batch][splits][M][N]
+  const int64_t* crow;                  // row scatter: row m of batch b goes to C row
+                                        // crow[b*M + m] (< 0: dropped), no batch offset
 };
 
 template <bool IS_BF16>
@@ -775,9 +777,10 @@ __global__ __launch_bounds__(256) void gemm_lp_kernel(GemmLPArgs g) {
   }
 
   const bool partial = g.splits > 1;
+  const int64_t cbo = g.crow ? 0 : (int64_t)batch * g.sC;
   double* C = partial ? g.ws + ((int64_t)batch * g.splits + split) * (int64_t)g.M * g.N
-                      : g.C + (int64_t)batch * g.sC;
-  const double* Dp = (partial || !g.D) ? nullptr : g.D + (int64_t)batch * g.sC;
+                      : g.C + cbo;
+  const double* Dp = (partial || !g.D) ? nullptr : g.D + cbo;
   const double beta = partial ? 0.0 : g.beta;
   const float* ia = g.ia ? g.ia + (int64_t)batch * g.sia : nullptr;
   const float* ib = g.ib ? g.ib + (int64_t)batch * g.sib : nullptr;
@@ -798,7 +801,11 @@ __global__ __launch_bounds__(256) void gemm_lp_kernel(GemmLPArgs g) {
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int row = m0 + wm * 64 + i * 16 + (lane >> 4) * 4 + r;
-      ro[r] = row < g.M ? (partial ? (int64_t)row * g.N : off2(row, g.cm)) : -1;
+      ro[r] = row >= g.M ? -1
+              : partial ? (int64_t)row * g.N
+              : g.crow ? (g.crow[(int64_t)batch * g.M + row] >= 0
+                              ? g.crow[(int64_t)batch * g.M + row] * g.cm.s1 : -1)
+              : off2(row, g.cm);
 #pragma unroll
       for (int j = 0; j < 4; ++j)
         dv[r][j] = (Dp && beta != 0.0 && ro[r] >= 0 && coff[j] >= 0) ? Dp[ro[r] + coff[j]] : 0.0;
@@ -931,9 +938,10 @@ __global__ __launch_bounds__(512) void gemm_lp_big_kernel(GemmLPArgs g) {
   }
 
   const bool partial = g.splits > 1;
+  const int64_t cbo = g.crow ? 0 : (int64_t)batch * g.sC;
   double* C = partial ? g.ws + ((int64_t)batch * g.splits + split) * (int64_t)g.M * g.N
-                      : g.C + (int64_t)batch * g.sC;
-  const double* Dp = (partial || !g.D) ? nullptr : g.D + (int64_t)batch * g.sC;
+                      : g.C + cbo;
+  const double* Dp = (partial || !g.D) ? nullptr : g.D + cbo;
   const double beta = partial ? 0.0 : g.beta;
   const float* ia = g.ia ? g.ia + (int64_t)batch * g.sia : nullptr;
   const float* ib = g.ib ? g.ib + (int64_t)batch * g.sib : nullptr;
@@ -954,7 +962,11 @@ __global__ __launch_bounds__(512) void gemm_lp_big_kernel(GemmLPArgs g) {
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int row = m0 + wm * 128 + i * 16 + (lane >> 4) * 4 + r;
-      ro[r] = row < g.M ? (partial ? (int64_t)row * g.N : off2(row, g.cm)) : -1;
+      ro[r] = row >= g.M ? -1
+              : partial ? (int64_t)row * g.N
+              : g.crow ? (g.crow[(int64_t)batch * g.M + row] >= 0
+                              ? g.crow[(int64_t)batch * g.M + row] * g.cm.s1 : -1)
+              : off2(row, g.cm);
 #pragma unroll
       for (int j = 0; j < 4; ++j)
         dv[r][j] = (Dp && beta != 0.0 && ro[r] >= 0 && coff[j] >= 0) ? Dp[ro[r] + coff[j]] : 0.0;
@@ -1423,7 +1435,7 @@ static int gemm_lp_impl(int dtype, int64_t M, int64_t N, int64_t K, double alpha
                         const void* B, int64_t ldb, int64_t strideB, const float* ib,
                         int64_t strideIb, double beta, const double* D, double* C, Idx2 cm,
                         Idx2 cn, int64_t strideC, int64_t batch, int64_t splits, double* ws,
-                        void* stream) {
+                        void* stream, const int64_t* crow = nullptr) {
   if (M < 0 || N < 0 || K < 0 || batch < 0 || (dtype != 0 && dtype != 1)) return NL_EARG;
   if (M == 0 || N == 0 || batch == 0) return NL_OK;
   if (!A || !B || !C) return NL_EARG;
@@ -1438,7 +1450,8 @@ static int gemm_lp_impl(int dtype, int64_t M, int64_t N, int64_t K, double alpha
   g.A = (const uint16_t*)A; g.lda = lda; g.sA = strideA;
   g.B = (const uint16_t*)B; g.ldb = ldb; g.sB = strideB;
   g.ia = ia; g.sia = strideIa; g.ib = ib; g.sib = strideIb;
-  g.C = C; g.cm = cm; g.cn = cn; g.sC = strideC; g.D = D;
+  g.C = C; g.cm = cm; g.cn = cn; g.sC = strideC; g.D = D; g.crow = crow;
+  if (crow && splits != 1) return NL_EARG;
   g.tiles_m = (int)nl_cdiv(M, TM); g.tiles_n = (int)nl_cdiv(N, TN);
   g.splits = (int)splits; g.ws = ws;
   g.kchunk = (int)(nl_cdiv(nl_cdiv(K, splits), BKL) * BKL);   // multiple of 64 (hence of 8)
@@ -1475,6 +1488,16 @@ int nl_gemm_lp(int dtype, int64_t M, int64_t N, int64_t K, double alpha, const v
   return gemm_lp_impl(dtype, M, N, K, alpha, A, lda, strideA, ia, strideIa, B, ldb, strideB, ib,
                       strideIb, beta, D, C, mk(1, ldc, 0), mk(1, 1, 0), strideC, batch, splits,
                       ws, stream);
+}
+
+int nl_gemm_lp_rows(int dtype, int64_t M, int64_t N, int64_t K, double alpha, const void* A,
+                    int64_t lda, int64_t strideA, const float* ia, int64_t strideIa, const void* B,
+                    int64_t ldb, int64_t strideB, const float* ib, int64_t strideIb, double* C,
+                    int64_t ldc, const int64_t* crow, int64_t batch, void* stream) {
+  if (!crow || ldc < N) return NL_EARG;
+  return gemm_lp_impl(dtype, M, N, K, alpha, A, lda, strideA, ia, strideIa, B, ldb, strideB, ib,
+                      strideIb, 1.0, C, C, mk(1, ldc, 0), mk(1, 1, 0), 0, batch, 1, nullptr,
+                      stream, crow);
 }
 
 int nl_gemm_lp_strided(int dtype, int64_t K, double alpha, const void* A, int64_t lda,

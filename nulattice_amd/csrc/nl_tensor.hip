@@ -211,6 +211,37 @@ __global__ __launch_bounds__(NT) void convert_tiles_kernel(ConvArgs a, int64_t R
   }
 }
 
+// out[b][n][k] = bf16(src[idx[b*kb + k]*lds + n]) (0 where idx < 0 or k >= kb, k < ldk):
+// the row gather of a packed-block product fused with its transposing BF16 conversion.  Rows
+// r = b*ncol + n of the output are tiled 64 x 64 with k through LDS, so the FP64 reads run along
+// n (source rows) and the 16-bit writes along k.
+__global__ __launch_bounds__(NT) void gather_convert_kernel(int64_t R, int64_t ncol, FastDiv fn,
+                                                            int64_t kb, int64_t ldk,
+                                                            const int64_t* idx,
+                                                            const double* src, int64_t lds,
+                                                            uint16_t* out) {
+  __shared__ uint16_t tile[64][66];
+  const int tr = threadIdx.x & 63, tg = threadIdx.x >> 6;
+  const int64_t r0 = blockIdx.x * 64, c0 = blockIdx.y * 64, r = r0 + tr;
+  if (r < R) {
+    const int64_t b = fdiv((int)r, fn), n = r - b * ncol;
+    for (int cc = tg; cc < 64; cc += 4) {
+      const int64_t k = c0 + cc;
+      float v = 0.f;
+      if (k < kb) {
+        const int64_t row = idx[b * kb + k];
+        if (row >= 0) v = (float)src[row * lds + n];
+      }
+      tile[tr][cc] = to_bf16_bits(v);
+    }
+  }
+  __syncthreads();
+  for (int rr = tg; rr < 64; rr += 4) {
+    const int64_t row = r0 + rr, c = c0 + tr;
+    if (row < R && c < ldk) out[row * ldk + c] = tile[rr][tr];
+  }
+}
+
 __global__ __launch_bounds__(NT) void iscale_kernel(int64_t R, float target, float* iscale) {
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < R;
        t += (int64_t)gridDim.x * blockDim.x) {
@@ -589,6 +620,19 @@ int nl_convert_rows(int dtype, int64_t R1, int64_t R2, int64_t C1, int64_t C2,
     return NL_OK;
   }
   hipLaunchKernelGGL(convert_rows_kernel, dim3((unsigned)R), dim3(NT), 0, st, a);
+  NL_LAUNCH_CHECK();
+  return NL_OK;
+}
+
+int nl_gather_convert(int64_t nb, int64_t kb, int64_t ncol, const int64_t* idx, const double* src,
+                      int64_t lds, void* out, int64_t ldk, void* stream) {
+  if (nb < 0 || kb < 0 || ncol < 0 || ldk < kb || (ldk % 8) || lds < ncol) return NL_EARG;
+  const int64_t R = nb * ncol;
+  if (R == 0 || ldk == 0) return NL_OK;
+  if (!idx || !src || !out || R >= INT32_MAX || nl_cdiv(ldk, 64) > 65535) return NL_EARG;
+  dim3 g((unsigned)nl_cdiv(R, 64), (unsigned)nl_cdiv(ldk, 64));
+  hipLaunchKernelGGL(gather_convert_kernel, g, dim3(NT), 0, (hipStream_t)stream, R, ncol,
+                     nl_fastdiv(ncol), kb, ldk, idx, src, lds, (uint16_t*)out);
   NL_LAUNCH_CHECK();
   return NL_OK;
 }

@@ -302,17 +302,20 @@ def _block_gemm(pb, src, out, alpha, ncol, tag, precision="fp64"):
     """out[rows_b] += alpha * V_b src[cols_b] for every packed block b (row gather, batched MFMA
     GEMM, row scatter; rows of src/out have ncol contiguous elements).  With a low precision
     the blocks V_b (constant, converted once) and the gathered rows (transposed per block) are
-    BF16/FP16 with per-row scales and the batched GEMM accumulates in FP32."""
+    16-bit and the batched GEMM accumulates in FP32: BF16 gathers and casts in one pass and adds
+    the block rows straight into out from the GEMM; FP16 (per-row scales) goes through FP64
+    gathered rows and block results."""
     if pb.nb == 0:
         return
-    G = ops.WS.get(tag + "_G", pb.nb * pb.kb * ncol, device=src.device)
-    R = ops.WS.get(tag + "_R", pb.nb * pb.mb * ncol, device=src.device)
     st = ops._stream()
     ev = ops._timed(tag)
     if ev:
         ev[0].record()
-    _lib.call("nl_gather_rows", pb.nb * pb.kb, ncol, ops._p(pb.cols), ops._p(src), ncol,
-              ops._p(G), st)
+    if precision != "bf16":
+        G = ops.WS.get(tag + "_G", pb.nb * pb.kb * ncol, device=src.device)
+        R = ops.WS.get(tag + "_R", pb.nb * pb.mb * ncol, device=src.device)
+        _lib.call("nl_gather_rows", pb.nb * pb.kb, ncol, ops._p(pb.cols), ops._p(src), ncol,
+                  ops._p(G), st)
     if precision == "fp64":
         _lib.call("nl_gemm_f64", 1, 1, pb.mb, ncol, pb.kb, ctypes.c_double(1.0), ops._p(pb.V),
                   pb.kb, pb.mb * pb.kb, ops._p(G), ncol, pb.kb * ncol, ctypes.c_double(0.0),
@@ -333,6 +336,16 @@ def _block_gemm(pb, src, out, alpha, ncol, tag, precision="fp64"):
             pb.lp = lp = (dt, Vb, iv)
         _, Vb, iv = lp
         Gt = ops.WS.get(tag + "_Gt", pb.nb * ncol * ldk, dtype=torch.int16, device=src.device)
+        if not scaled:
+            # BF16: gather + transposing cast in one pass, block rows added straight into out
+            _lib.call("nl_gather_convert", pb.nb, pb.kb, ncol, ops._p(pb.cols), ops._p(src), ncol,
+                      ops._p(Gt), ldk, st)
+            _lib.call("nl_gemm_lp_rows", dt, pb.mb, ncol, ldk, ctypes.c_double(alpha),
+                      ops._p(Vb), ldk, pb.mb * ldk, None, 0, ops._p(Gt), ldk, ncol * ldk, None,
+                      0, ops._p(out), ncol, ops._p(pb.rows), pb.nb, st)
+            if ev:
+                ev[1].record()
+            return
         ig = (ops.WS.get(tag + "_Gs", pb.nb * ncol, dtype=torch.float32, device=src.device)
               if scaled else None)
         s4 = (ctypes.c_int64 * 4)(pb.kb * ncol, 1, 0, ncol)

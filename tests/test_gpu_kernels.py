@@ -224,6 +224,45 @@ def test_segsum_lane_packing(X):
     np.testing.assert_allclose(host(keep[5]), ref, rtol=1e-13, atol=1e-13)
 
 
+def test_gather_convert_matches_gather_then_cast():
+    """Fused row gather + transposing BF16 cast of the packed-block products: bit-equal to
+    torch's round-to-nearest-even cast of the gathered, transposed rows; idx < 0 rows and the
+    k padding up to ldk are zero."""
+    g = torch.Generator(device="cuda").manual_seed(3)
+    nb, kb, ncol, ldk, nsrc = 5, 37, 70, 40, 90
+    src = torch.randn(nsrc, ncol, dtype=torch.float64, device="cuda", generator=g)
+    idx = torch.randint(-1, nsrc, (nb * kb,), device="cuda", generator=g)
+    out = torch.full((nb, ncol, ldk), 7, dtype=torch.int16, device="cuda")
+    _lib.call("nl_gather_convert", nb, kb, ncol, ops._p(idx), ops._p(src), ncol, ops._p(out), ldk,
+              ops._stream())
+    G = torch.where(idx[:, None] >= 0, src[idx.clamp(min=0)], torch.zeros_like(src[:1]))
+    ref = torch.zeros(nb, ncol, ldk, dtype=torch.bfloat16, device="cuda")
+    ref[:, :, :kb] = G.view(nb, kb, ncol).transpose(1, 2).float().to(torch.bfloat16)
+    assert torch.equal(out, ref.view(torch.int16))
+
+
+@pytest.mark.parametrize("M,N", [(40, 70), (300, 784)])
+def test_gemm_lp_rows_scatter_accumulates_exactly(M, N):
+    """Row-scatter LP GEMM: block rows land on C rows crow[b*M + m] (several batches, dropped
+    rows), accumulated onto C's values; small integers keep every sum exact."""
+    g = torch.Generator(device="cuda").manual_seed(M)
+    nbat, K, nrow = 3, 64, 1000
+    A = torch.randint(-3, 4, (nbat, M, K), device="cuda", generator=g).to(torch.bfloat16)
+    B = torch.randint(-3, 4, (nbat, N, K), device="cuda", generator=g).to(torch.bfloat16)
+    perm = torch.randperm(nrow, device="cuda", generator=g)[: nbat * M]
+    crow = torch.where(torch.rand(nbat * M, device="cuda", generator=g) < 0.2,
+                       torch.full_like(perm, -1), perm)
+    C0 = torch.randint(-5, 6, (nrow, N), device="cuda", generator=g).double()
+    C = C0.clone()
+    _lib.call("nl_gemm_lp_rows", 0, M, N, K, ctypes.c_double(0.5), ops._p(A), K, M * K, None, 0,
+              ops._p(B), K, N * K, None, 0, ops._p(C), N, ops._p(crow), nbat, ops._stream())
+    R = 0.5 * torch.bmm(A.double(), B.double().transpose(1, 2).contiguous()).reshape(nbat * M, N)
+    ref = C0.clone()
+    keep = crow >= 0
+    ref[crow[keep]] += R[keep]
+    assert float((C - ref).abs().max()) == 0.0
+
+
 def test_error_codes():
     """Bad arguments come back as -1, never as a fault."""
     lib = _lib.load()
