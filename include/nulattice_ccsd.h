@@ -99,6 +99,21 @@ int nl_convert_rows(int dtype, int64_t R1, int64_t R2, int64_t C1, int64_t C2,
 int nl_gather_convert(int64_t nb, int64_t kb, int64_t ncol, const int64_t* idx, const double* src,
                       int64_t lds, void* out, int64_t ldk, void* stream);
 
+/* nl_gather_convert with BF16 source rows (src: 16-bit elements, row stride lds): the same
+ * gather + transpose, no rounding. */
+int nl_gather_rows16(int64_t nb, int64_t kb, int64_t ncol, const int64_t* idx, const void* src,
+                     int64_t lds, void* out, int64_t ldk, void* stream);
+
+/* tau of nl_tau, rounded to BF16 and stored as [a][b][i][j] 16-bit elements: when every
+ * product that reads tau (ccDgrams.py:215-237 ladder, the hh intermediate of
+ * coupled_cluster.py:524-526, R56) runs in BF16, the FP64 tau is never formed. */
+int nl_tau16(int64_t p, int64_t h, const double* t1, const double* t2, void* tau16, void* stream);
+
+/* out[c*ldo + r] = in[r*ldi + c] for 16-bit elements (r < R, c < C), 0 for R <= r < ldo:
+ * re-lays a BF16 operand so that the contracted index is the unit-stride one. */
+int nl_transpose16(int64_t R, int64_t C, const void* in, int64_t ldi, void* out, int64_t ldo,
+                   void* stream);
+
 /* tau[a,b,i,j] = t2[a,b,i,j] + t1[a,i] t1[b,j] - t1[a,j] t1[b,i]: the t2 + pIJ(t1 t1) combination
  * that the pp ladder applies to (ccDgrams.py:215-237 with coupled_cluster.py:524-526). */
 int nl_tau(int64_t p, int64_t h, const double* t1, const double* t2, double* tau, void* stream);

@@ -215,11 +215,11 @@ __global__ __launch_bounds__(NT) void convert_tiles_kernel(ConvArgs a, int64_t R
 // the row gather of a packed-block product fused with its transposing BF16 conversion.  Rows
 // r = b*ncol + n of the output are tiled 64 x 64 with k through LDS, so the FP64 reads run along
 // n (source rows) and the 16-bit writes along k.
+template <typename S>   // source rows: double (cast here) or uint16_t (BF16 already)
 __global__ __launch_bounds__(NT) void gather_convert_kernel(int64_t R, int64_t ncol, FastDiv fn,
                                                             int64_t kb, int64_t ldk,
-                                                            const int64_t* idx,
-                                                            const double* src, int64_t lds,
-                                                            uint16_t* out) {
+                                                            const int64_t* idx, const S* src,
+                                                            int64_t lds, uint16_t* out) {
   __shared__ uint16_t tile[64][66];
   const int tr = threadIdx.x & 63, tg = threadIdx.x >> 6;
   const int64_t r0 = blockIdx.x * 64, c0 = blockIdx.y * 64, r = r0 + tr;
@@ -227,18 +227,40 @@ __global__ __launch_bounds__(NT) void gather_convert_kernel(int64_t R, int64_t n
     const int64_t b = fdiv((int)r, fn), n = r - b * ncol;
     for (int cc = tg; cc < 64; cc += 4) {
       const int64_t k = c0 + cc;
-      float v = 0.f;
+      uint16_t v = 0;
       if (k < kb) {
         const int64_t row = idx[b * kb + k];
-        if (row >= 0) v = (float)src[row * lds + n];
+        if (row >= 0) {
+          if constexpr (sizeof(S) == 8) v = to_bf16_bits((float)src[row * lds + n]);
+          else v = src[row * lds + n];
+        }
       }
-      tile[tr][cc] = to_bf16_bits(v);
+      tile[tr][cc] = v;
     }
   }
   __syncthreads();
   for (int rr = tg; rr < 64; rr += 4) {
     const int64_t row = r0 + rr, c = c0 + tr;
     if (row < R && c < ldk) out[row * ldk + c] = tile[rr][tr];
+  }
+}
+
+// out[c*ldo + r] = in[r*ldi + c] for 16-bit elements (r < R, c < C), and 0 for R <= r < ldo:
+// 64 x 64 tiles through LDS, reads along c and writes along r coalesced.
+__global__ __launch_bounds__(NT) void transpose16_kernel(int64_t R, int64_t C, const uint16_t* in,
+                                                         int64_t ldi, uint16_t* out,
+                                                         int64_t ldo) {
+  __shared__ uint16_t tile[64][66];
+  const int tc = threadIdx.x & 63, tg = threadIdx.x >> 6;
+  const int64_t r0 = blockIdx.y * 64, c0 = blockIdx.x * 64;
+  for (int rr = tg; rr < 64; rr += 4) {
+    const int64_t r = r0 + rr, c = c0 + tc;
+    tile[rr][tc] = (r < R && c < C) ? in[r * ldi + c] : (uint16_t)0;
+  }
+  __syncthreads();
+  for (int cc = tg; cc < 64; cc += 4) {
+    const int64_t c = c0 + cc, r = r0 + tc;
+    if (c < C && r < ldo) out[c * ldo + r] = tile[tc][cc];
   }
 }
 
@@ -252,17 +274,19 @@ __global__ __launch_bounds__(NT) void iscale_kernel(int64_t R, float target, flo
 
 // tau[a,b,i,j] = t2[a,b,i,j] + t1[a,i] t1[b,j] - t1[a,j] t1[b,i]: workgroups walk the pairs
 // (a, b), threads the h x h block (contiguous in t2 and tau).
+template <typename T>   // double, or uint16_t for a BF16 tau (the rounded FP64 value)
 __global__ __launch_bounds__(NT) void tau_kernel(int p, int h, FastDiv fp, FastDiv fh,
-                                                 const double* t1, const double* t2,
-                                                 double* tau) {
+                                                 const double* t1, const double* t2, T* tau) {
   const int hh = h * h;
   for (int q = blockIdx.x; q < p * p; q += gridDim.x) {
     const int a = fdiv(q, fp), b = q - a * p;
     const double* x = t2 + (int64_t)q * hh;
-    double* y = tau + (int64_t)q * hh;
+    T* y = tau + (int64_t)q * hh;
     for (int e = threadIdx.x; e < hh; e += NT) {
       const int i = fdiv(e, fh), j = e - i * h;
-      y[e] = x[e] + t1[a * h + i] * t1[b * h + j] - t1[a * h + j] * t1[b * h + i];
+      const double v = x[e] + t1[a * h + i] * t1[b * h + j] - t1[a * h + j] * t1[b * h + i];
+      if constexpr (sizeof(T) == 8) y[e] = v;
+      else y[e] = to_bf16_bits((float)v);
     }
   }
 }
@@ -631,8 +655,46 @@ int nl_gather_convert(int64_t nb, int64_t kb, int64_t ncol, const int64_t* idx, 
   if (R == 0 || ldk == 0) return NL_OK;
   if (!idx || !src || !out || R >= INT32_MAX || nl_cdiv(ldk, 64) > 65535) return NL_EARG;
   dim3 g((unsigned)nl_cdiv(R, 64), (unsigned)nl_cdiv(ldk, 64));
-  hipLaunchKernelGGL(gather_convert_kernel, g, dim3(NT), 0, (hipStream_t)stream, R, ncol,
-                     nl_fastdiv(ncol), kb, ldk, idx, src, lds, (uint16_t*)out);
+  hipLaunchKernelGGL((gather_convert_kernel<double>), g, dim3(NT), 0, (hipStream_t)stream, R,
+                     ncol, nl_fastdiv(ncol), kb, ldk, idx, src, lds, (uint16_t*)out);
+  NL_LAUNCH_CHECK();
+  return NL_OK;
+}
+
+int nl_gather_rows16(int64_t nb, int64_t kb, int64_t ncol, const int64_t* idx, const void* src,
+                     int64_t lds, void* out, int64_t ldk, void* stream) {
+  if (nb < 0 || kb < 0 || ncol < 0 || ldk < kb || (ldk % 8) || lds < ncol) return NL_EARG;
+  const int64_t R = nb * ncol;
+  if (R == 0 || ldk == 0) return NL_OK;
+  if (!idx || !src || !out || R >= INT32_MAX || nl_cdiv(ldk, 64) > 65535) return NL_EARG;
+  dim3 g((unsigned)nl_cdiv(R, 64), (unsigned)nl_cdiv(ldk, 64));
+  hipLaunchKernelGGL((gather_convert_kernel<uint16_t>), g, dim3(NT), 0, (hipStream_t)stream, R,
+                     ncol, nl_fastdiv(ncol), kb, ldk, idx, (const uint16_t*)src, lds,
+                     (uint16_t*)out);
+  NL_LAUNCH_CHECK();
+  return NL_OK;
+}
+
+int nl_tau16(int64_t p, int64_t h, const double* t1, const double* t2, void* tau16,
+             void* stream) {
+  if (p < 0 || h < 0 || !t1 || !t2 || !tau16) return NL_EARG;
+  if (p * p * h * h == 0) return NL_OK;
+  if (p * p > INT32_MAX || h * h > INT32_MAX || p * h > INT32_MAX) return NL_EARG;
+  hipLaunchKernelGGL((tau_kernel<uint16_t>), dim3((unsigned)std::min<int64_t>(p * p, 65536)),
+                     dim3(NT), 0, (hipStream_t)stream, (int)p, (int)h, nl_fastdiv(p),
+                     nl_fastdiv(h), t1, t2, (uint16_t*)tau16);
+  NL_LAUNCH_CHECK();
+  return NL_OK;
+}
+
+int nl_transpose16(int64_t R, int64_t C, const void* in, int64_t ldi, void* out, int64_t ldo,
+                   void* stream) {
+  if (R < 0 || C < 0 || ldi < C || ldo < R) return NL_EARG;
+  if (C == 0 || ldo == 0) return NL_OK;
+  if (!in || !out || nl_cdiv(ldo, 64) > 65535) return NL_EARG;
+  dim3 g((unsigned)nl_cdiv(C, 64), (unsigned)nl_cdiv(ldo, 64));
+  hipLaunchKernelGGL(transpose16_kernel, g, dim3(NT), 0, (hipStream_t)stream, R, C,
+                     (const uint16_t*)in, ldi, (uint16_t*)out, ldo);
   NL_LAUNCH_CHECK();
   return NL_OK;
 }
@@ -642,8 +704,8 @@ int nl_tau(int64_t p, int64_t h, const double* t1, const double* t2, double* tau
   int64_t n = p * p * h * h;
   if (n == 0) return NL_OK;
   if (p * p > INT32_MAX || h * h > INT32_MAX || p * h > INT32_MAX) return NL_EARG;
-  hipLaunchKernelGGL(tau_kernel, dim3((unsigned)std::min<int64_t>(p * p, 65536)), dim3(NT), 0,
-                     (hipStream_t)stream, (int)p, (int)h, nl_fastdiv(p), nl_fastdiv(h), t1, t2,
+  hipLaunchKernelGGL((tau_kernel<double>), dim3((unsigned)std::min<int64_t>(p * p, 65536)), dim3(NT),
+                     0, (hipStream_t)stream, (int)p, (int)h, nl_fastdiv(p), nl_fastdiv(h), t1, t2,
                      tau);
   NL_LAUNCH_CHECK();
   return NL_OK;

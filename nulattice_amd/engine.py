@@ -30,6 +30,7 @@ Precision form: ``prec`` maps the families "ring", "hh", "dress", "ladder" to "f
 per-row power-of-two scales, BF16 operands plainly rounded), everything else (Fock, denominators,
 update, energy, DIIS) stays FP64.
 """
+import contextlib
 import ctypes
 import weakref
 
@@ -170,6 +171,9 @@ def _segsum(plan, X, src, out, alpha, src_stride=1, dst_stride=1):
 
 DEFAULT_PREC = {"ring": "fp64", "hh": "fp64", "dress": "fp64", "ladder": "fp64"}
 
+# Form tau directly in BF16 when every product that reads it runs in BF16 (False: always FP64).
+TAU16 = True
+
 
 def t1_part(ham, t1, t2, B, prec=DEFAULT_PREC):
     """H1, X1pp, X1hh of t1Iter (coupled_cluster.py:372-408) into B.H1/B.X1pp/B.X1hh."""
@@ -232,8 +236,22 @@ def t2_part(ham, t1, t2, B, prec=DEFAULT_PREC):
     p, h = ham.p, ham.h
     vpphh = ham.vpphh_view()
     pl = ham.ppph
-    _lib.call("nl_tau", p, h, ops._p(t1), ops._p(t2), ops._p(B.tau), ops._stream())
+    # tau feeds only the hh products, the pp ladder and R56: when all of them run in BF16 (and
+    # are large enough not to fall back to FP64) it is formed directly in BF16, as [ab][ij] rows
+    # and, for the Whh product (summed over c,d), as [ij][cd] rows; the FP64 tau is not written
     tau = B.tau
+    lp_tau = (TAU16 and prec["hh"] == "bf16" and prec.get("ladder", "fp64") == "bf16"
+              and min(p * p, h * h) >= ops.LP_MIN_DIM and (h * h) % 8 == 0)
+    tau16 = None
+    if lp_tau:
+        ldT = (p * p + 7) // 8 * 8                   # [ij][cd] rows, zero-padded to 8
+        tau16 = ops.WS.get("tau16", p * p * h * h, dtype=torch.int16, device=t2.device)
+        tauT = ops.WS.get("tau16T", h * h * ldT, dtype=torch.int16, device=t2.device)
+        _lib.call("nl_tau16", p, h, ops._p(t1), ops._p(t2), ops._p(tau16), ops._stream())
+        _lib.call("nl_transpose16", p * p, h * h, ops._p(tau16), h * h, ops._p(tauT), ldT,
+                  ops._stream())
+    else:
+        _lib.call("nl_tau", p, h, ops._p(t1), ops._p(t2), ops._p(tau), ops._stream())
     # ---- dressed Fock matrices
     Xpp, Xhh = B.X2pp, B.X2hh
     permute_into(ham.f_pp, Xpp)
@@ -250,15 +268,20 @@ def t2_part(ham, t1, t2, B, prec=DEFAULT_PREC):
     contract("ci,cjkl->klij", t1, ham.v_phhh, B.zz, beta=0.0)
     permute_into(B.zz, B.Whh, 1.0, 1.0)
     permute_into(B.zz.permute(0, 1, 3, 2), B.Whh, -1.0, 1.0)
-    contract("cdkl,cdij->klij", ham.v_pphh, tau, B.Whh, alpha=0.5, precision=prec["hh"])
-    contract("abkl,klij->abij", tau, B.Whh, S0, alpha=0.5, precision=prec["hh"],
-             addend=ham.v_pphh)
-    _pp_ladder(ham, tau, S0, prec)
-    # ---- Sab: P(ab) group
     Sab = B.Sab
-    permute_into(ham.v_phhh, B.N56)
-    if pl.n:
-        _block_gemm(pl.r56, tau, B.N56, 0.5, h * h, "r56", prec.get("ladder", "fp64"))
+    with ops.lp_reuse(tau, absent=True) if lp_tau else contextlib.nullcontext():
+        if lp_tau:
+            ops.lp_provide(tau, "cdij", ["i", "j"], ["c", "d"], tauT, ldT)
+            ops.lp_provide(tau, "abkl", ["a", "b"], ["k", "l"], tau16, h * h)
+        contract("cdkl,cdij->klij", ham.v_pphh, tau, B.Whh, alpha=0.5, precision=prec["hh"])
+        contract("abkl,klij->abij", tau, B.Whh, S0, alpha=0.5, precision=prec["hh"],
+                 addend=ham.v_pphh)
+        _pp_ladder(ham, tau, S0, prec, tau16)
+        # ---- Sab: P(ab) group
+        permute_into(ham.v_phhh, B.N56)
+        if pl.n:
+            _block_gemm(pl.r56, tau, B.N56, 0.5, h * h, "r56", prec.get("ladder", "fp64"),
+                        tau16)
     contract("bkij,ak->abij", B.N56, t1, Sab, beta=0.0)                      # A23 + A39
     permute_into(Xpp, B.Mpp)                                                 # X_pp t2 (P(ab) image)
     if pl.n:
@@ -293,18 +316,19 @@ def t2_part(ham, t1, t2, B, prec=DEFAULT_PREC):
     contract("ilaj,bl->aibj", B.Zilaj, t1, B.Spp)
 
 
-def _pp_ladder(ham, tau, S0, prec):
+def _pp_ladder(ham, tau, S0, prec, tau16=None):
     """S0 += 1/2 sum_cd v_pppp[a,b,c,d] tau[c,d,i,j] over packed pair blocks."""
-    _block_gemm(ham.pppp, tau, S0, 0.5, ham.h * ham.h, "ladder", prec.get("ladder", "fp64"))
+    _block_gemm(ham.pppp, tau, S0, 0.5, ham.h * ham.h, "ladder", prec.get("ladder", "fp64"),
+                tau16)
 
 
-def _block_gemm(pb, src, out, alpha, ncol, tag, precision="fp64"):
+def _block_gemm(pb, src, out, alpha, ncol, tag, precision="fp64", src16=None):
     """out[rows_b] += alpha * V_b src[cols_b] for every packed block b (row gather, batched MFMA
     GEMM, row scatter; rows of src/out have ncol contiguous elements).  With a low precision
     the blocks V_b (constant, converted once) and the gathered rows (transposed per block) are
-    16-bit and the batched GEMM accumulates in FP32: BF16 gathers and casts in one pass and adds
-    the block rows straight into out from the GEMM; FP16 (per-row scales) goes through FP64
-    gathered rows and block results."""
+    16-bit and the batched GEMM accumulates in FP32: BF16 gathers and casts in one pass (or, given
+    src16, the BF16 rows of src, gathers them) and adds the block rows straight into out from
+    the GEMM; FP16 (per-row scales) goes through FP64 gathered rows and block results."""
     if pb.nb == 0:
         return
     st = ops._stream()
@@ -338,8 +362,12 @@ def _block_gemm(pb, src, out, alpha, ncol, tag, precision="fp64"):
         Gt = ops.WS.get(tag + "_Gt", pb.nb * ncol * ldk, dtype=torch.int16, device=src.device)
         if not scaled:
             # BF16: gather + transposing cast in one pass, block rows added straight into out
-            _lib.call("nl_gather_convert", pb.nb, pb.kb, ncol, ops._p(pb.cols), ops._p(src), ncol,
-                      ops._p(Gt), ldk, st)
+            if src16 is not None:
+                _lib.call("nl_gather_rows16", pb.nb, pb.kb, ncol, ops._p(pb.cols), ops._p(src16),
+                          ncol, ops._p(Gt), ldk, st)
+            else:
+                _lib.call("nl_gather_convert", pb.nb, pb.kb, ncol, ops._p(pb.cols), ops._p(src),
+                          ncol, ops._p(Gt), ldk, st)
             _lib.call("nl_gemm_lp_rows", dt, pb.mb, ncol, ldk, ctypes.c_double(alpha),
                       ops._p(Vb), ldk, pb.mb * ldk, None, 0, ops._p(Gt), ldk, ncol * ldk, None,
                       0, ops._p(out), ncol, ops._p(pb.rows), pb.nb, st)

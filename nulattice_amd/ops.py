@@ -257,6 +257,9 @@ def contract(spec, A, B, out, alpha=1.0, beta=1.0, precision="fp64", tag=None, a
         M = math.prod(size[c] for c in Mi); N = math.prod(size[c] for c in Ni)
         if min(M, N) < LP_MIN_DIM or any(c in la and c in lb for c in lo):
             precision = "fp64"      # thin or batched products stay exact
+    if ABSENT_PTRS and precision == "fp64" and (A.data_ptr() in ABSENT_PTRS or
+                                                B.data_ptr() in ABSENT_PTRS):
+        raise _lib.NLError(f"contract {spec}: FP64 read of a tensor whose FP64 values are absent")
     if precision != "fp64":
         Ki = _order(A, la, [c for c in la if c in lb and c not in lo])
         K = math.prod(size[c] for c in Ki)
@@ -340,6 +343,9 @@ def _view4(t, labels, rows, cols):
 # conversions are cached by (storage pointer, view, layout, dtype).
 CONST_PTRS = set()
 _LP_CACHE = {}
+# Storage whose FP64 values are not formed (only 16-bit images registered by lp_provide): any
+# other read of it raises instead of using stale data.
+ABSENT_PTRS = set()
 
 
 def mark_constant(*ts):
@@ -360,16 +366,30 @@ class lp_reuse:
     those of constants are, and dropped on exit.  Lets two products that read one operand in the
     same layout (the two ring GEMMs and t2) share a single conversion."""
 
-    def __init__(self, *ts):
+    def __init__(self, *ts, absent=False):
         self.ptrs = [t.data_ptr() for t in ts if t.data_ptr() not in CONST_PTRS]
+        self.absent = absent
 
     def __enter__(self):
         CONST_PTRS.update(self.ptrs)
+        if self.absent:
+            ABSENT_PTRS.update(self.ptrs)
         return self
 
     def __exit__(self, *exc):
         forget_ptrs(self.ptrs)
+        ABSENT_PTRS.difference_update(self.ptrs)
         return False
+
+
+def lp_provide(t, labels, rows, cols, buf, ldk, dtype=BF16):
+    """Register ``buf`` ([R][ldk] 16-bit rows, no scales) as the conversion of t seen as rows x
+    cols, for products inside an ``lp_reuse(t)`` block: they use it without reading t, so t's
+    FP64 values need not exist."""
+    v = _view4(t, labels, rows, cols)
+    if v is None or t.data_ptr() not in CONST_PTRS:
+        raise _lib.NLError("lp_provide: the view must be two-run and t under lp_reuse")
+    _LP_CACHE[(t.data_ptr(), v[0], v[1], v[2], v[3], tuple(v[4]), dtype)] = (buf, None, ldk)
 
 
 def to_lp_rows(t, labels, rows, cols, dtype, tag):
@@ -384,6 +404,9 @@ def to_lp_rows(t, labels, rows, cols, dtype, tag):
         hit = _LP_CACHE.get(key)
         if hit is not None:
             return hit
+    if t.data_ptr() in ABSENT_PTRS:
+        raise _lib.NLError("to_lp_rows: no 16-bit image registered for this view of a tensor "
+                           "whose FP64 values are not formed")
     if v is None:
         t = _as_layout(t, labels, rows + cols); labels = "".join(rows + cols)
         v = _view4(t, labels, rows, cols)

@@ -131,6 +131,32 @@ def test_precision_form_refines_to_fp64(prec):
     assert info["steps_lowprec"] > 0
 
 
+def test_bf16_tau_path_is_bitwise_the_fp64_tau_path():
+    """With every tau product in BF16 (L=3, A=28: h*h, p*p above the low-precision threshold),
+    forming tau directly in BF16 rounds exactly as converting the FP64 tau does, so one BF16
+    iteration gives bit-identical amplitudes and energy either way."""
+    E0, fock, v = planewave_hamiltonian(3, 28, -4.0)
+    H = engine.Hamiltonian(fock, v, device="cuda")
+    p, h = H.p, H.h
+    res = {}
+    for flag in (True, False):
+        engine.TAU16 = flag
+        try:
+            B = engine.Buffers(p, h, H.device)
+            t1 = torch.zeros(p, h, dtype=torch.float64, device="cuda")
+            t2 = torch.zeros(p, p, h, h, dtype=torch.float64, device="cuda")
+            engine.init_amplitudes(H, B, t1, t2, 0.0)
+            t1n, t2n = torch.empty_like(t1), torch.empty_like(t2)
+            for _ in range(2):
+                engine.update(H, t1, t2, B, t1n, t2n, 0.5, ccm._PREC["bf16"])
+                t1, t1n, t2, t2n = t1n, t1, t2n, t2
+            res[flag] = (t2.clone(), float(B.E.item()))
+        finally:
+            engine.TAU16 = True
+    assert torch.equal(res[True][0], res[False][0])
+    assert res[True][1] == res[False][1]
+
+
 def test_product_hamiltonian_on_gpu_solver():
     """nulattice_amd.hamiltonian (the package's own input step) + the GPU solver reproduce the
     notebook trajectory's steps 0-10 (sparse CooList blocks) and the dense form agrees."""

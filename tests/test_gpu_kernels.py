@@ -263,6 +263,35 @@ def test_gemm_lp_rows_scatter_accumulates_exactly(M, N):
     assert float((C - ref).abs().max()) == 0.0
 
 
+def test_tau16_transpose16_gather16():
+    """BF16 tau is the FP64 tau rounded (via FP32, as every FP64 -> BF16 cast here); the 16-bit
+    transpose and row gather move bits unchanged."""
+    g = torch.Generator(device="cuda").manual_seed(11)
+    p, h = 37, 6
+    t1 = torch.randn(p, h, dtype=torch.float64, device="cuda", generator=g)
+    t2 = torch.randn(p, p, h, h, dtype=torch.float64, device="cuda", generator=g)
+    tau = torch.empty_like(t2)
+    tau16 = torch.empty(p * p * h * h, dtype=torch.int16, device="cuda")
+    _lib.call("nl_tau", p, h, ops._p(t1), ops._p(t2), ops._p(tau), ops._stream())
+    _lib.call("nl_tau16", p, h, ops._p(t1), ops._p(t2), ops._p(tau16), ops._stream())
+    ref16 = tau.float().to(torch.bfloat16).view(torch.int16).reshape(-1)
+    assert torch.equal(tau16, ref16)
+    R, C, ldo = p * p, h * h, p * p + 7                       # rows padded with zeros
+    out = torch.full((C, ldo), 5, dtype=torch.int16, device="cuda")
+    _lib.call("nl_transpose16", R, C, ops._p(tau16), C, ops._p(out), ldo, ops._stream())
+    assert torch.equal(out[:, :R], tau16.view(R, C).t())
+    assert int(out[:, R:].abs().max()) == 0
+    nb, kb, ldk = 4, 9, 16
+    idx = torch.randint(-1, R, (nb * kb,), device="cuda", generator=g)
+    Gt = torch.empty(nb, C, ldk, dtype=torch.int16, device="cuda")
+    Gf = torch.empty(nb, C, ldk, dtype=torch.int16, device="cuda")
+    _lib.call("nl_gather_rows16", nb, kb, C, ops._p(idx), ops._p(tau16), C, ops._p(Gt), ldk,
+              ops._stream())
+    _lib.call("nl_gather_convert", nb, kb, C, ops._p(idx), ops._p(tau), C, ops._p(Gf), ldk,
+              ops._stream())
+    assert torch.equal(Gt, Gf)
+
+
 def test_error_codes():
     """Bad arguments come back as -1, never as a fault."""
     lib = _lib.load()
