@@ -264,6 +264,41 @@ __global__ __launch_bounds__(NT) void transpose16_kernel(int64_t R, int64_t C, c
   }
 }
 
+// BF16 source rows: out[b][n][k] = src[idx[b*kb + k]*lds + n].  Tiles of 64 k x 256 n: each
+// thread reads 4 consecutive n (8 bytes) of one source row, and writes 4 consecutive k of one
+// output row (8 bytes) from the LDS tile.  Needs ncol % 4 == 0, lds % 4 == 0 and 8-byte aligned
+// rows (checked on the host).
+__global__ __launch_bounds__(NT) void gather16_kernel(int64_t ncol, int64_t kb, int64_t ldk,
+                                                      const int64_t* idx, const uint16_t* src,
+                                                      int64_t lds, uint16_t* out) {
+  __shared__ uint16_t tile[64][256 + 4];
+  const int b = blockIdx.z, tid = threadIdx.x;
+  const int64_t n0 = (int64_t)blockIdx.x * 256, k0 = (int64_t)blockIdx.y * 64;
+  const int nl = (tid & 63) * 4;                 // 4 n per thread, 64 threads per k row
+#pragma unroll 4
+  for (int kk = tid >> 6; kk < 64; kk += 4) {
+    const int64_t k = k0 + kk, n = n0 + nl;
+    uint2 v = make_uint2(0u, 0u);
+    if (k < kb && n < ncol) {
+      const int64_t row = idx[(int64_t)b * kb + k];
+      if (row >= 0) v = *(const uint2*)(src + row * lds + n);
+    }
+    *(uint2*)&tile[kk][nl] = v;
+  }
+  __syncthreads();
+  const int kq = (tid & 15) * 4;                 // 4 k per thread, 16 threads per n row
+#pragma unroll 4
+  for (int nn = tid >> 4; nn < 256; nn += 16) {
+    const int64_t n = n0 + nn, k = k0 + kq;
+    if (n >= ncol || k >= ldk) continue;
+    uint16_t w[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) w[u] = tile[kq + u][nn];
+    *(uint2*)(out + ((int64_t)b * ncol + n) * ldk + k) =
+        make_uint2(w[0] | ((unsigned)w[1] << 16), w[2] | ((unsigned)w[3] << 16));
+  }
+}
+
 __global__ __launch_bounds__(NT) void iscale_kernel(int64_t R, float target, float* iscale) {
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < R;
        t += (int64_t)gridDim.x * blockDim.x) {
@@ -277,14 +312,22 @@ __global__ __launch_bounds__(NT) void iscale_kernel(int64_t R, float target, flo
 template <typename T>   // double, or uint16_t for a BF16 tau (the rounded FP64 value)
 __global__ __launch_bounds__(NT) void tau_kernel(int p, int h, FastDiv fp, FastDiv fh,
                                                  const double* t1, const double* t2, T* tau) {
+  // the pair's two t1 rows are staged in LDS (h <= NT), so an element costs one global load
+  __shared__ double ta[NT], tb[NT];
   const int hh = h * h;
   for (int q = blockIdx.x; q < p * p; q += gridDim.x) {
     const int a = fdiv(q, fp), b = q - a * p;
     const double* x = t2 + (int64_t)q * hh;
     T* y = tau + (int64_t)q * hh;
+    __syncthreads();
+    if (threadIdx.x < h) {
+      ta[threadIdx.x] = t1[a * h + threadIdx.x];
+      tb[threadIdx.x] = t1[b * h + threadIdx.x];
+    }
+    __syncthreads();
     for (int e = threadIdx.x; e < hh; e += NT) {
       const int i = fdiv(e, fh), j = e - i * h;
-      const double v = x[e] + t1[a * h + i] * t1[b * h + j] - t1[a * h + j] * t1[b * h + i];
+      const double v = x[e] + ta[i] * tb[j] - ta[j] * tb[i];
       if constexpr (sizeof(T) == 8) y[e] = v;
       else y[e] = to_bf16_bits((float)v);
     }
@@ -667,6 +710,14 @@ int nl_gather_rows16(int64_t nb, int64_t kb, int64_t ncol, const int64_t* idx, c
   const int64_t R = nb * ncol;
   if (R == 0 || ldk == 0) return NL_OK;
   if (!idx || !src || !out || R >= INT32_MAX || nl_cdiv(ldk, 64) > 65535) return NL_EARG;
+  if (ncol % 4 == 0 && lds % 4 == 0 && (((uintptr_t)src) & 7) == 0 &&
+      (((uintptr_t)out) & 7) == 0 && nb <= 65535) {
+    dim3 g((unsigned)nl_cdiv(ncol, 256), (unsigned)nl_cdiv(ldk, 64), (unsigned)nb);
+    hipLaunchKernelGGL(gather16_kernel, g, dim3(NT), 0, (hipStream_t)stream, ncol, kb, ldk, idx,
+                       (const uint16_t*)src, lds, (uint16_t*)out);
+    NL_LAUNCH_CHECK();
+    return NL_OK;
+  }
   dim3 g((unsigned)nl_cdiv(R, 64), (unsigned)nl_cdiv(ldk, 64));
   hipLaunchKernelGGL((gather_convert_kernel<uint16_t>), g, dim3(NT), 0, (hipStream_t)stream, R,
                      ncol, nl_fastdiv(ncol), kb, ldk, idx, (const uint16_t*)src, lds,
@@ -679,7 +730,7 @@ int nl_tau16(int64_t p, int64_t h, const double* t1, const double* t2, void* tau
              void* stream) {
   if (p < 0 || h < 0 || !t1 || !t2 || !tau16) return NL_EARG;
   if (p * p * h * h == 0) return NL_OK;
-  if (p * p > INT32_MAX || h * h > INT32_MAX || p * h > INT32_MAX) return NL_EARG;
+  if (p * p > INT32_MAX || h * h > INT32_MAX || p * h > INT32_MAX || h > NT) return NL_EARG;
   hipLaunchKernelGGL((tau_kernel<uint16_t>), dim3((unsigned)std::min<int64_t>(p * p, 65536)),
                      dim3(NT), 0, (hipStream_t)stream, (int)p, (int)h, nl_fastdiv(p),
                      nl_fastdiv(h), t1, t2, (uint16_t*)tau16);
@@ -703,7 +754,7 @@ int nl_tau(int64_t p, int64_t h, const double* t1, const double* t2, double* tau
   if (p < 0 || h < 0 || !t1 || !t2 || !tau) return NL_EARG;
   int64_t n = p * p * h * h;
   if (n == 0) return NL_OK;
-  if (p * p > INT32_MAX || h * h > INT32_MAX || p * h > INT32_MAX) return NL_EARG;
+  if (p * p > INT32_MAX || h * h > INT32_MAX || p * h > INT32_MAX || h > NT) return NL_EARG;
   hipLaunchKernelGGL((tau_kernel<double>), dim3((unsigned)std::min<int64_t>(p * p, 65536)), dim3(NT),
                      0, (hipStream_t)stream, (int)p, (int)h, nl_fastdiv(p), nl_fastdiv(h), t1, t2,
                      tau);
