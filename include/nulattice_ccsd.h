@@ -114,6 +114,17 @@ int nl_tau16(int64_t p, int64_t h, const double* t1, const double* t2, void* tau
 int nl_transpose16(int64_t R, int64_t C, const void* in, int64_t ldi, void* out, int64_t ldo,
                    void* stream);
 
+/* nl_transpose16 for batch matrices at element strides stride_in / stride_out apart. */
+int nl_transpose16_batched(int64_t batch, int64_t R, int64_t C, const void* in, int64_t ldi,
+                           int64_t stride_in, void* out, int64_t ldo, int64_t stride_out,
+                           void* stream);
+
+/* out (contiguous, out_shape, 16-bit elements) = in[sum_d idx_d * in_strides[d]]: re-lays the
+ * BF16 image of t2 for the products that read it in another order (nl_permute's 16-bit form;
+ * fewer than 2^31 elements). */
+int nl_permute16(int rank, const int64_t* out_shape, const int64_t* in_strides, const void* in,
+                 void* out, void* stream);
+
 /* tau[a,b,i,j] = t2[a,b,i,j] + t1[a,i] t1[b,j] - t1[a,j] t1[b,i]: the t2 + pIJ(t1 t1) combination
  * that the pp ladder applies to (ccDgrams.py:215-237 with coupled_cluster.py:524-526). */
 int nl_tau(int64_t p, int64_t h, const double* t1, const double* t2, double* tau, void* stream);

@@ -248,11 +248,13 @@ __global__ __launch_bounds__(NT) void gather_convert_kernel(int64_t R, int64_t n
 // out[c*ldo + r] = in[r*ldi + c] for 16-bit elements (r < R, c < C), and 0 for R <= r < ldo:
 // 64 x 64 tiles through LDS, reads along c and writes along r coalesced.
 __global__ __launch_bounds__(NT) void transpose16_kernel(int64_t R, int64_t C, const uint16_t* in,
-                                                         int64_t ldi, uint16_t* out,
-                                                         int64_t ldo) {
+                                                         int64_t ldi, int64_t si, uint16_t* out,
+                                                         int64_t ldo, int64_t so) {
   __shared__ uint16_t tile[64][66];
   const int tc = threadIdx.x & 63, tg = threadIdx.x >> 6;
   const int64_t r0 = blockIdx.y * 64, c0 = blockIdx.x * 64;
+  in += blockIdx.z * si;
+  out += blockIdx.z * so;
   for (int rr = tg; rr < 64; rr += 4) {
     const int64_t r = r0 + rr, c = c0 + tc;
     tile[rr][tc] = (r < R && c < C) ? in[r * ldi + c] : (uint16_t)0;
@@ -296,6 +298,66 @@ __global__ __launch_bounds__(NT) void gather16_kernel(int64_t ncol, int64_t kb, 
     for (int u = 0; u < 4; ++u) w[u] = tile[kq + u][nn];
     *(uint2*)(out + ((int64_t)b * ncol + n) * ldk + k) =
         make_uint2(w[0] | ((unsigned)w[1] << 16), w[2] | ((unsigned)w[3] << 16));
+  }
+}
+
+// transpose16 with 8-byte accesses (R, C, ldi, ldo, strides multiples of 4; 8-byte aligned):
+// reads 4 consecutive c of one row per lane, writes 4 consecutive r of one output row per lane.
+__global__ __launch_bounds__(NT) void transpose16v_kernel(int64_t R, int64_t C, const uint16_t* in,
+                                                          int64_t ldi, int64_t si, uint16_t* out,
+                                                          int64_t ldo, int64_t so) {
+  __shared__ uint16_t tile[64][64 + 4];
+  const int tid = threadIdx.x, q4 = (tid & 15) * 4, row = tid >> 4;
+  const int64_t r0 = blockIdx.y * 64, c0 = blockIdx.x * 64;
+  in += blockIdx.z * si;
+  out += blockIdx.z * so;
+#pragma unroll
+  for (int rr = row; rr < 64; rr += 16) {
+    const int64_t r = r0 + rr, c = c0 + q4;
+    uint2 v = make_uint2(0u, 0u);
+    if (r < R && c < C) v = *(const uint2*)(in + r * ldi + c);
+    tile[rr][q4 + 0] = (uint16_t)(v.x & 0xffffu);
+    tile[rr][q4 + 1] = (uint16_t)(v.x >> 16);
+    tile[rr][q4 + 2] = (uint16_t)(v.y & 0xffffu);
+    tile[rr][q4 + 3] = (uint16_t)(v.y >> 16);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int cc = row; cc < 64; cc += 16) {
+    const int64_t c = c0 + cc, r = r0 + q4;
+    if (c >= C || r >= ldo) continue;
+    *(uint2*)(out + c * ldo + r) =
+        make_uint2(tile[q4][cc] | ((unsigned)tile[q4 + 1][cc] << 16),
+                   tile[q4 + 2][cc] | ((unsigned)tile[q4 + 3][cc] << 16));
+  }
+}
+
+// 16-bit permute: out (contiguous, shape fd[].d) = in[sum_d idx_d * istr[d]], total < 2^31.
+// V = 4: elements move as 4-element (8-byte) vectors along a last dimension that is unit-stride
+// in both (extent and strides divided by 4 on the host).
+template <int V>
+struct Perm16Args {
+  int rank;
+  int64_t istr[6];
+  FastDiv fd[6];
+  int64_t total;
+  const uint16_t* in;
+  uint16_t* out;
+};
+
+template <int V>
+__global__ __launch_bounds__(NT) void permute16_kernel(Perm16Args<V> a) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < a.total;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    int rem = (int)t;
+    int64_t off = 0;
+    for (int d = a.rank - 1; d >= 0; --d) {
+      const int q = fdiv(rem, a.fd[d]);
+      off += (int64_t)(rem - q * a.fd[d].d) * a.istr[d];
+      rem = q;
+    }
+    if (V == 4) *(uint2*)(a.out + t * 4) = *(const uint2*)(a.in + off * 4);
+    else a.out[t] = a.in[off];
   }
 }
 
@@ -738,14 +800,76 @@ int nl_tau16(int64_t p, int64_t h, const double* t1, const double* t2, void* tau
   return NL_OK;
 }
 
+int nl_transpose16_batched(int64_t batch, int64_t R, int64_t C, const void* in, int64_t ldi,
+                           int64_t stride_in, void* out, int64_t ldo, int64_t stride_out,
+                           void* stream);
+
 int nl_transpose16(int64_t R, int64_t C, const void* in, int64_t ldi, void* out, int64_t ldo,
                    void* stream) {
   if (R < 0 || C < 0 || ldi < C || ldo < R) return NL_EARG;
   if (C == 0 || ldo == 0) return NL_OK;
   if (!in || !out || nl_cdiv(ldo, 64) > 65535) return NL_EARG;
-  dim3 g((unsigned)nl_cdiv(C, 64), (unsigned)nl_cdiv(ldo, 64));
-  hipLaunchKernelGGL(transpose16_kernel, g, dim3(NT), 0, (hipStream_t)stream, R, C,
-                     (const uint16_t*)in, ldi, (uint16_t*)out, ldo);
+  return nl_transpose16_batched(1, R, C, in, ldi, 0, out, ldo, 0, stream);
+}
+
+int nl_transpose16_batched(int64_t batch, int64_t R, int64_t C, const void* in, int64_t ldi,
+                           int64_t stride_in, void* out, int64_t ldo, int64_t stride_out,
+                           void* stream) {
+  if (batch < 0 || R < 0 || C < 0 || ldi < C || ldo < R) return NL_EARG;
+  if (batch == 0 || C == 0 || ldo == 0) return NL_OK;
+  if (!in || !out || nl_cdiv(ldo, 64) > 65535 || batch > 65535) return NL_EARG;
+  dim3 g((unsigned)nl_cdiv(C, 64), (unsigned)nl_cdiv(ldo, 64), (unsigned)batch);
+  const bool vec = C % 4 == 0 && ldi % 4 == 0 && ldo % 4 == 0 && stride_in % 4 == 0 &&
+                   stride_out % 4 == 0 && (((uintptr_t)in) & 7) == 0 && (((uintptr_t)out) & 7) == 0;
+  if (vec)
+    hipLaunchKernelGGL(transpose16v_kernel, g, dim3(NT), 0, (hipStream_t)stream, R, C,
+                       (const uint16_t*)in, ldi, stride_in, (uint16_t*)out, ldo, stride_out);
+  else
+    hipLaunchKernelGGL(transpose16_kernel, g, dim3(NT), 0, (hipStream_t)stream, R, C,
+                       (const uint16_t*)in, ldi, stride_in, (uint16_t*)out, ldo, stride_out);
+  NL_LAUNCH_CHECK();
+  return NL_OK;
+}
+
+int nl_permute16(int rank, const int64_t* out_shape, const int64_t* in_strides, const void* in,
+                 void* out, void* stream) {
+  if (rank < 1 || rank > 6 || !out_shape || !in_strides || !in || !out) return NL_EARG;
+  int64_t total = 1;
+  for (int d = 0; d < rank; ++d) {
+    if (out_shape[d] < 0) return NL_EARG;
+    total *= out_shape[d];
+  }
+  if (total == 0) return NL_OK;
+  if (total >= INT32_MAX) return NL_EARG;
+  // 4-element vectors when the last dimension is unit-stride, of extent and strides divisible
+  // by 4, and both buffers 8-byte aligned
+  bool vec = in_strides[rank - 1] == 1 && out_shape[rank - 1] % 4 == 0 &&
+             (((uintptr_t)in) & 7) == 0 && (((uintptr_t)out) & 7) == 0;
+  for (int d = 0; d + 1 < rank; ++d) vec = vec && in_strides[d] % 4 == 0;
+  if (vec) {
+    Perm16Args<4> a;
+    a.rank = rank; a.total = total / 4;
+    for (int d = 0; d < rank; ++d) {
+      const int64_t ext = d + 1 < rank ? out_shape[d] : out_shape[d] / 4;
+      a.istr[d] = d + 1 < rank ? in_strides[d] / 4 : 1;
+      a.fd[d] = nl_fastdiv(ext > 0 ? ext : 1);
+    }
+    a.in = (const uint16_t*)in; a.out = (uint16_t*)out;
+    hipLaunchKernelGGL((permute16_kernel<4>), dim3(grid_for(a.total)), dim3(NT), 0,
+                       (hipStream_t)stream, a);
+    NL_LAUNCH_CHECK();
+    return NL_OK;
+  }
+  Perm16Args<1> a;
+  a.rank = rank;
+  a.total = total;
+  for (int d = 0; d < rank; ++d) {
+    a.istr[d] = in_strides[d];
+    a.fd[d] = nl_fastdiv(out_shape[d] > 0 ? out_shape[d] : 1);
+  }
+  a.in = (const uint16_t*)in; a.out = (uint16_t*)out;
+  hipLaunchKernelGGL((permute16_kernel<1>), dim3(grid_for(a.total)), dim3(NT), 0,
+                     (hipStream_t)stream, a);
   NL_LAUNCH_CHECK();
   return NL_OK;
 }

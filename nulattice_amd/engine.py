@@ -215,8 +215,8 @@ def t1_part(ham, t1, t2, B, prec=DEFAULT_PREC):
     permute_into(ham.f_pp, Xpp)
     contract("ck,ak->ac", ham.f_ph, t1, Xpp, alpha=-0.5)                     # A16
     # A19 into Dpp: equal to A41 of the T2 part (antisymmetry of v_pphh in c,d and of t2 in a,b)
-    contract("dckl,dakl->ac", ham.v_pphh_T.permute(1, 0, 2, 3), t2, B.Dpp, alpha=-0.5,
-             beta=0.0, precision=prec["dress"])
+    contract("dckl,adkl->ac", ham.v_pphh_T.permute(1, 0, 2, 3), t2, B.Dpp, alpha=0.5,
+             beta=0.0, precision=prec["dress"])         # t2[d,a,k,l] = -t2[a,d,k,l]
     ops.axpby(1.0, B.Dpp, 1.0, Xpp)
     contract("cl,al->ac", B.w_ph, t1, Xpp, alpha=0.5)                       # A20 (pp)
     if pl.n:
@@ -391,11 +391,53 @@ def _block_gemm(pb, src, out, alpha, ncol, tag, precision="fp64", src16=None):
 def update(ham, t1, t2, B, t1n, t2n, mixing, prec=DEFAULT_PREC, H2out=None):
     """One Jacobi step (coupled_cluster.py:602-623): writes t1n, t2n and B.E[0] = E(t1n, t2n)."""
     p, h = ham.p, ham.h
-    t1_part(ham, t1, t2, B, prec)
-    _lib.call("nl_update_t1", p, h, ops._p(t1), ops._p(B.H1), ops._p(B.X1pp), ops._p(B.X1hh),
-              ctypes.c_double(mixing), ops._p(t1n), ops._stream())
-    t2_part(ham, t1, t2, B, prec)
+    with ops.lp_reuse(t2, B.Tr):
+        _t2_images(ham, t2, B, prec)
+        t1_part(ham, t1, t2, B, prec)
+        _lib.call("nl_update_t1", p, h, ops._p(t1), ops._p(B.H1), ops._p(B.X1pp),
+                  ops._p(B.X1hh), ctypes.c_double(mixing), ops._p(t1n), ops._stream())
+        t2_part(ham, t1, t2, B, prec)
     assemble(ham, t2, B, t1n, t2n, mixing, H2out)
+
+
+# BF16 images of t2 for the products that read it in BF16 (all from one cast of t2):
+#   A19 (dress)  t2 rows [a][(d,k,l)]   the cast itself
+#   X_pp t2      t2 rows [(b,i,j)][d]   16-bit transpose
+#   Y (hh)       t2 rows [(a,j)][(d,k)] batched 16-bit transpose
+#   ring (both)  Tr rows [(a,i)][(c,k)] 16-bit permute
+T2_IMAGES = True
+
+
+def _t2_images(ham, t2, B, prec):
+    """Register the BF16 images of t2 (and of Tr) that this iteration's BF16 products read, made
+    from a single FP64 -> BF16 pass over t2 instead of one conversion of t2 per product; the
+    FP64 t2 and Tr stay valid for the FP64 products.  Needs t2 and Tr under ops.lp_reuse."""
+    p, h = ham.p, ham.h
+    use = {k for k in ("dress", "hh", "ring") if prec.get(k) == "bf16"}
+    if not (T2_IMAGES and use) or p % 8 or (p * h) % 8 or p * p * h * h >= 2 ** 31 - 1:
+        return
+    n, ph, hh = p * p * h * h, p * h, h * h
+    dev, st = t2.device, ops._stream()
+    S = ops.WS.get("t2_bf16", n, dtype=torch.int16, device=dev)       # [a][d][k][l]
+    s4 = (ctypes.c_int64 * 4)(0, p * hh, 0, 1)
+    _lib.call("nl_convert_rows", ops.BF16, 1, p, 1, p * hh, s4, ops._p(t2), ops._p(S), p * hh,
+              None, ctypes.c_float(0.0), st)
+    if "dress" in use:
+        ops.lp_provide(t2, "adkl", ["a"], ["d", "k", "l"], S, p * hh)
+        T1 = ops.WS.get("t2_bf16_bijd", n, dtype=torch.int16, device=dev)
+        _lib.call("nl_transpose16", p, p * hh, ops._p(S), p * hh, ops._p(T1), p, st)
+        ops.lp_provide(t2, "dbij", ["b", "i", "j"], ["d"], T1, p)
+    if "hh" in use:
+        T2 = ops.WS.get("t2_bf16_ajdk", n, dtype=torch.int16, device=dev)
+        _lib.call("nl_transpose16_batched", p, ph, h, ops._p(S), h, p * hh, ops._p(T2), ph,
+                  p * hh, st)
+        ops.lp_provide(t2, "adkj", ["a", "j"], ["d", "k"], T2, ph)
+    if "ring" in use:
+        T3 = ops.WS.get("t2_bf16_aick", n, dtype=torch.int16, device=dev)   # Tr's rows
+        shp = (ctypes.c_int64 * 4)(p, h, p, h)
+        istr = (ctypes.c_int64 * 4)(p * hh, h, hh, 1)                 # out [a][i][c][k]
+        _lib.call("nl_permute16", 4, shp, istr, ops._p(S), ops._p(T3), st)
+        ops.lp_provide(B.Tr.permute(0, 2, 1, 3), "acik", ["a", "i"], ["c", "k"], T3, ph)
 
 
 def assemble(ham, t2, B, t1n, t2n, mixing, H2out=None):

@@ -131,16 +131,18 @@ def test_precision_form_refines_to_fp64(prec):
     assert info["steps_lowprec"] > 0
 
 
-def test_bf16_tau_path_is_bitwise_the_fp64_tau_path():
+def test_bf16_images_are_bitwise_the_per_product_conversions():
     """With every tau product in BF16 (L=3, A=28: h*h, p*p above the low-precision threshold),
-    forming tau directly in BF16 rounds exactly as converting the FP64 tau does, so one BF16
-    iteration gives bit-identical amplitudes and energy either way."""
-    E0, fock, v = planewave_hamiltonian(3, 28, -4.0)
+    forming tau directly in BF16, and taking every BF16 layout of t2 from one cast of t2, round
+    exactly as converting the FP64 tensors per product does, so two BF16 iterations give
+    bit-identical amplitudes and energy either way."""
+    E0, fock, v = planewave_hamiltonian(3, 28, -4.0)        # p = 80, h = 28: p, p*h multiples of 8
     H = engine.Hamiltonian(fock, v, device="cuda")
     p, h = H.p, H.h
+    assert p % 8 == 0 and (h * h) % 8 == 0 and p * p >= ops.LP_MIN_DIM
     res = {}
     for flag in (True, False):
-        engine.TAU16 = flag
+        engine.TAU16 = engine.T2_IMAGES = flag
         try:
             B = engine.Buffers(p, h, H.device)
             t1 = torch.zeros(p, h, dtype=torch.float64, device="cuda")
@@ -152,7 +154,7 @@ def test_bf16_tau_path_is_bitwise_the_fp64_tau_path():
                 t1, t1n, t2, t2n = t1n, t1, t2n, t2
             res[flag] = (t2.clone(), float(B.E.item()))
         finally:
-            engine.TAU16 = True
+            engine.TAU16 = engine.T2_IMAGES = True
     assert torch.equal(res[True][0], res[False][0])
     assert res[True][1] == res[False][1]
 

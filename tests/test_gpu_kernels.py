@@ -292,6 +292,25 @@ def test_tau16_transpose16_gather16():
     assert torch.equal(Gt, Gf)
 
 
+@pytest.mark.parametrize("h", [4, 6])
+def test_permute16_and_batched_transpose16(h):
+    """16-bit re-layouts move bits unchanged: the [a][c][i][k] -> [a][i][c][k] permute (4-element
+    vectors when h % 4 == 0, single elements otherwise) and the per-a transpose
+    [a][(d,k)][j] -> [a][j][(d,k)]."""
+    g = torch.Generator(device="cuda").manual_seed(h)
+    p = 13
+    S = torch.randint(-30000, 30000, (p, p, h, h), device="cuda", generator=g).to(torch.int16)
+    out = torch.empty(p, h, p, h, dtype=torch.int16, device="cuda")
+    shp = (ctypes.c_int64 * 4)(p, h, p, h)
+    istr = (ctypes.c_int64 * 4)(p * h * h, h, h * h, 1)
+    _lib.call("nl_permute16", 4, shp, istr, ops._p(S), ops._p(out), ops._stream())
+    assert torch.equal(out, S.permute(0, 2, 1, 3))
+    T = torch.empty(p, h, p * h, dtype=torch.int16, device="cuda")
+    _lib.call("nl_transpose16_batched", p, p * h, h, ops._p(S), h, p * h * h, ops._p(T), p * h,
+              p * h * h, ops._stream())
+    assert torch.equal(T, S.reshape(p, p * h, h).transpose(1, 2))
+
+
 def test_error_codes():
     """Bad arguments come back as -1, never as a fault."""
     lib = _lib.load()
