@@ -85,6 +85,7 @@ class Hamiltonian:
         self.v_phph_bkic = ops.copy(self.v_phph.permute(0, 1, 3, 2))
         self.v_pphh_T = ops.copy(self.v_pphh.permute(1, 0, 2, 3))
         self.ones_h = torch.ones(h, dtype=F64, device=device)
+        self.eye_h = torch.eye(h, dtype=F64, device=device)
         consts = (self.f_pp, self.f_ph, self.f_hh, self.v_pphh, self.v_phph, self.v_phhh,
                   self.v_hhhh, self.vP, self.Wc, self.v_phph_bkic, self.v_pphh_T)
         ops.mark_constant(*consts)
@@ -116,7 +117,7 @@ class Hamiltonian:
                 raise ValueError(f"{name} has shape {tuple(t.shape)}, expected {shp}")
         for name in self._TWO_BODY:
             setattr(new, name, _scaled(getattr(self, name), scale))
-        new.ones_h = self.ones_h
+        new.ones_h, new.eye_h = self.ones_h, self.eye_h
         new.pppp = self.pppp.scaled(scale, _scaled)
         new.ppph = self.ppph.scaled(scale, _scaled)
         consts = [new.f_pp, new.f_ph, new.f_hh] + [getattr(new, n) for n in self._TWO_BODY]
@@ -182,13 +183,18 @@ def t1_part(ham, t1, t2, B, prec=DEFAULT_PREC):
     permute_into(t2.permute(0, 2, 1, 3), B.Tr)       # t2 as [a][i][c][k], shared by both parts
     t2r = B.Tr.permute(0, 2, 1, 3)                   # the same t2[a,c,i,k] through that layout
     permute_into(ham.f_ph, H1)
-    contract("akci,ck->ai", ham.v_phph, t1, H1, alpha=-1.0)                  # A11
+    # U[b,k,i,j] = v_phph[b,k,c,i] t1[c,j] (A27) and X[i,d,k,l] = v_pphh[c,d,k,l] t1[c,i] (A34)
+    # serve the T2 part; A11 is U's (k = j) trace and A14's s its X's (i = k) trace, so each of
+    # the two blocks is read once per iteration
+    contract("bkci,cj->bkij", ham.v_phph_bkic.permute(0, 1, 3, 2), t1, B.Ubkij, beta=0.0)  # A27
+    contract("akij,kj->ai", B.Ubkij, ham.eye_h, H1, alpha=-1.0)              # A11
     contract("cikl,cakl->ai", ham.v_phhh, t2, H1, alpha=-0.5)                # A13
+    contract("cdkl,ci->idkl", ham.v_pphh, t1, B.Xidkl, beta=0.0)             # A34
     # A14: s[d,l] = v[c,d,k,l] t1[c,k].  With v antisymmetric in (c,d) and in (k,l), the other
     # v_pphh.t1 products of both parts are the same matrix: u (A20, hh) = s, w (A20, pp) = -s,
     # z (A46) = s; and s[d,l] t2[d,a,l,i] = s[d,l] t2[a,d,i,l], so A12 and A14 are one product
     # with f_ph + s.
-    contract("cdkl,ck->dl", vpphh, t1, B.s_ph, beta=0.0)
+    contract("idkl,ik->dl", B.Xidkl, ham.eye_h, B.s_ph, beta=0.0)
     permute_into(B.s_ph, B.u_ph)
     permute_into(B.s_ph, B.w_ph, alpha=-1.0)
     permute_into(B.s_ph, B.z_ph)
@@ -228,7 +234,8 @@ def t1_part(ham, t1, t2, B, prec=DEFAULT_PREC):
 def t2_part(ham, t1, t2, B, prec=DEFAULT_PREC):
     """S0/Sab/Sij/Spp and X2pp/X2hh of t2Iter (coupled_cluster.py:476-536).
 
-    Requires B.ret1, B.u_ph, B.Dpp and B.Dhh from t1_part at the same amplitudes.  Uses that t2
+    Requires B.ret1, B.u_ph, B.Dpp, B.Dhh, B.Ubkij and B.Xidkl from t1_part at the same
+    amplitudes.  Uses that t2
     is antisymmetric in (a,b) and in (i,j) and that Sab / Sij enter the residual only through
     P(ab) / P(ij): X_pp[b,c] t2[a,c,i,j] and X_pp[a,c] t2[c,b,i,j] have the same P(ab) image, as
     have -q[l,i] t2[a,b,l,j] and -q[k,j] t2[a,b,i,k] under P(ij), so each group's two dressings
@@ -306,9 +313,7 @@ def t2_part(ham, t1, t2, B, prec=DEFAULT_PREC):
             _segsum(pl.ret4, h, t1, W, 1.0)                                  # A37
         contract("cjkl,bl->ckbj", ham.v_phhh, t1, W, alpha=-1.0)             # A29
         contract("acik,ckbj->aibj", t2r, W, B.Spp, beta=0.0, precision=prec["ring"], tag="ring")
-    contract("cdkl,ci->idkl", ham.v_pphh, t1, B.Xidkl, beta=0.0)             # A34
-    contract("idkl,adkj->ilaj", B.Xidkl, t2, B.Yilaj, beta=0.0, precision=prec["hh"])
-    contract("bkci,cj->bkij", ham.v_phph_bkic.permute(0, 1, 3, 2), t1, B.Ubkij, beta=0.0)  # A27
+    contract("idkl,adkj->ilaj", B.Xidkl, t2, B.Yilaj, beta=0.0, precision=prec["hh"])  # A34
     # Spp enters only through P(ij)P(ab), which is invariant under (a,i) <-> (b,j): the A27 term
     # -t1[a,k] U[b,k,i,j] may stand as -U[a,l,j,i] t1[b,l], so both tails are one product
     permute_into(B.Yilaj, B.Zilaj)

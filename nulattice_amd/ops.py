@@ -126,6 +126,21 @@ def _splits(M, N, K, batch):
     return int(max(1, min(256, 1024 // tiles, K // 256)))
 
 
+def _lp_splits(M, N, K, batch):
+    """Split-K for nl_gemm_lp, which runs its 256x256 tiles when tiles x splits >= 256: for a
+    long K (>= 4096) whose 256x256 tiles alone do not fill the 256 CUs, the smallest split giving
+    >= 256 workgroups with the last round >= 80% busy (each split keeping >= 512 of K)."""
+    big = math.ceil(M / 256) * math.ceil(N / 256) * batch
+    if M >= 256 and N >= 256 and K >= 4096 and big < 256:
+        for s in range(2, 65):
+            w = big * s
+            if K // s < 512:
+                break
+            if w >= 256 and w >= 0.8 * 256 * math.ceil(w / 256):
+                return s
+    return _splits(M, N, K, batch)
+
+
 def _order(t, labels, group):
     """Indices of ``group`` sorted by decreasing stride of ``t`` (its memory order)."""
     return sorted(group, key=lambda c: (-t.stride(labels.index(c)), labels.index(c)))
@@ -460,7 +475,7 @@ def _contract_lp(A, la, B, lb, out, lo, Mi, Ni, Ki, M, N, K, alpha, beta, precis
     direct = ldc is not None or runs is not None
     C = out if direct else WS.get("contract_out", M * N, device=out.device).view(M, N)
     D = (addend if addend is not None else out) if (direct and beta != 0.0) else None
-    splits = _splits(M, N, lda, 1)
+    splits = _lp_splits(M, N, lda, 1)
     ws = WS.get("splitk", splits * M * N, device=out.device) if splits > 1 else None
     ev = _timed(tag)
     if ev:
