@@ -119,6 +119,18 @@ int nl_transpose16_batched(int64_t batch, int64_t R, int64_t C, const void* in, 
                            int64_t stride_in, void* out, int64_t ldo, int64_t stride_out,
                            void* stream);
 
+/* nl_transpose16_batched writing rows of rlen elements (zeros for R <= r < rlen) at pitch ldo:
+ * fills a column range of wider operand rows (the t2 part of the fused Sab operand). */
+int nl_transpose16_into(int64_t batch, int64_t R, int64_t C, const void* in, int64_t ldi,
+                        int64_t stride_in, void* out, int64_t ldo, int64_t stride_out,
+                        int64_t rlen, void* stream);
+
+/* BF16 cast of a 4-index view (as nl_convert_rows, target <= 0) into rows of pitch ldo, writing
+ * columns [0, C1*C2) and zeros up to cend: builds operands whose K is a concatenation (the
+ * fused Sab product of t2Iter's P(ab) group, coupled_cluster.py:476-536). */
+int nl_convert_rows_into(int64_t R1, int64_t R2, int64_t C1, int64_t C2, const int64_t* strides4,
+                         const double* in, void* out, int64_t ldo, int64_t cend, void* stream);
+
 /* out (contiguous, out_shape, 16-bit elements) = in[sum_d idx_d * in_strides[d]]: re-lays the
  * BF16 image of t2 for the products that read it in another order (nl_permute's 16-bit form;
  * fewer than 2^31 elements). */
@@ -151,7 +163,8 @@ int nl_update_t1(int64_t p, int64_t h, const double* t1, const double* H1, const
                  const double* Xhh, double mixing, double* t1n, void* stream);
 
 /* Fused T2 assembly, Jacobi update, mixing and energy partials:
- *   H2  = S0 + pAB(Sab) + pIJ(Sij) + pIJ(pAB(Spp))   (Sij, Spp stored as [(a,i)][(b,j)])
+ *   H2  = S0 + pAB(Sab) + pIJ(Sij) + pIJ(pAB(Spp))   (S0, Sab, Sij as [a][b][i][j], Spp as
+ *                                                     [(a,i)][(b,j)])
  *   t2n = m*t2 + (1-m)*(t2 + H2 / (-(Xpp_aa + Xpp_bb + Xhh_ii + Xhh_jj)))
  *   part[blk] = partial of 1/4 v.t2n + 1/2 v t1n t1n; part[nblk+blk] = partial of |H2|^2
  * (coupled_cluster.py:536-545, :618, :265-286).  nblk = nl_partials(p*p*h*h). */

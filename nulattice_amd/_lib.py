@@ -29,6 +29,9 @@ SIGNATURES = {
     "nl_transpose16": (_i32, [_i64, _i64, _vp, _i64, _vp, _i64, _vp]),
     "nl_transpose16_batched": (_i32, [_i64, _i64, _i64, _vp, _i64, _i64, _vp, _i64, _i64, _vp]),
     "nl_permute16": (_i32, [_i32, _vp, _vp, _vp, _vp, _vp]),
+    "nl_transpose16_into": (_i32, [_i64, _i64, _i64, _vp, _i64, _i64, _vp, _i64, _i64, _i64,
+                                   _vp]),
+    "nl_convert_rows_into": (_i32, [_i64, _i64, _i64, _i64, _vp, _vp, _vp, _i64, _i64, _vp]),
     "nl_gemm_lp_strided": (_i32, [_i32, _i64, _dbl, _vp, _i64, _vp, _vp, _i64, _vp, _dbl, _vp,
                                   _vp, _vp, _vp, _i64, _vp, _vp]),
     "nl_permute": (_i32, [_i32, _vp, _vp, _dbl, _vp, _dbl, _vp, _vp]),

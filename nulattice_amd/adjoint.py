@@ -102,7 +102,7 @@ def residual_vjp(ham, t1, t2, B, l1, l2, G):
     # ---- Spp[a,i,b,j] = t2[a,c,i,k] W[c,k,b,j] + Y[i,l,a,j] t1[b,l] - t1[a,k] U[b,k,i,j]
     gSpp = G.gSpp
     contract("aibj,ckbj->aick", gSpp, B.W, G.gTr, beta=0.0, tag="adj_ring_t")
-    contract("aick,aibj->ckbj", B.Tr, gSpp, G.gW, beta=0.0, tag="adj_ring_W")
+    contract("acik,aibj->ckbj", t2, gSpp, G.gW, beta=0.0, tag="adj_ring_W")
     contract("aibj,bl->ilaj", gSpp, t1, G.gY, beta=0.0)
     contract("ilaj,aibj->bl", B.Yilaj, gSpp, gt1)
     contract("aibj,bkij->ak", gSpp, B.Ubkij, gt1, alpha=-1.0)

@@ -72,7 +72,8 @@ __global__ void permute_kernel(PermArgs a) {
 }
 
 struct ConvArgs {
-  int64_t R2, C, C2, ldo;   // rows = R1*R2 (grid), cols = C1*C2 = C
+  int64_t R2, C, C2, ldo;   // rows = R1*R2 (grid), cols = C1*C2 = C, output row pitch ldo
+  int64_t cend;             // columns written per row: the C values, then zeros up to cend
   FastDiv fr2, fc2;         // divisors R2, C2 (rows and columns are below 2^31)
   int64_t s[4];             // strides of (r1, r2, c1, c2)
   const double* in;
@@ -122,7 +123,7 @@ __global__ __launch_bounds__(NT) void convert_rows_kernel(ConvArgs a) {
   float sc = ldexpf(1.f, e);
   if (threadIdx.x == 0) a.iscale[r] = ldexpf(1.f, -e);
   uint16_t* o = a.out + r * a.ldo;
-  for (int64_t c = threadIdx.x; c < a.ldo; c += NT) {
+  for (int64_t c = threadIdx.x; c < a.cend; c += NT) {
     float v = 0.f;
     if (c < a.C) {
       int64_t c1 = fdiv((int)c, a.fc2), c2 = c - c1 * a.C2;
@@ -139,7 +140,7 @@ __global__ __launch_bounds__(NT) void cast_rows_kernel(ConvArgs a) {
   const int64_t r = blockIdx.x, r1 = fdiv((int)r, a.fr2), r2 = r - r1 * a.R2;
   const double* base = a.in + r1 * a.s[0] + r2 * a.s[1];
   uint16_t* o = a.out + r * a.ldo;
-  const int64_t c0 = (int64_t)blockIdx.y * CAST_CHUNK, c1e = min(a.ldo, c0 + CAST_CHUNK);
+  const int64_t c0 = (int64_t)blockIdx.y * CAST_CHUNK, c1e = min(a.cend, c0 + CAST_CHUNK);
   for (int64_t c = c0 + threadIdx.x; c < c1e; c += NT) {
     float v = 0.f;
     if (c < a.C) {
@@ -207,7 +208,7 @@ __global__ __launch_bounds__(NT) void convert_tiles_kernel(ConvArgs a, int64_t R
   __syncthreads();
   for (int rr = tg; rr < 64; rr += 4) {
     int64_t row = r0 + rr, c = c0 + tr;
-    if (row < R && c < a.ldo) a.out[row * a.ldo + c] = tile[rr][tr];
+    if (row < R && c < a.cend) a.out[row * a.ldo + c] = tile[rr][tr];
   }
 }
 
@@ -245,11 +246,11 @@ __global__ __launch_bounds__(NT) void gather_convert_kernel(int64_t R, int64_t n
   }
 }
 
-// out[c*ldo + r] = in[r*ldi + c] for 16-bit elements (r < R, c < C), and 0 for R <= r < ldo:
+// out[c*ldo + r] = in[r*ldi + c] for 16-bit elements (r < R, c < C), and 0 for R <= r < rlen:
 // 64 x 64 tiles through LDS, reads along c and writes along r coalesced.
 __global__ __launch_bounds__(NT) void transpose16_kernel(int64_t R, int64_t C, const uint16_t* in,
                                                          int64_t ldi, int64_t si, uint16_t* out,
-                                                         int64_t ldo, int64_t so) {
+                                                         int64_t ldo, int64_t so, int64_t rlen) {
   __shared__ uint16_t tile[64][66];
   const int tc = threadIdx.x & 63, tg = threadIdx.x >> 6;
   const int64_t r0 = blockIdx.y * 64, c0 = blockIdx.x * 64;
@@ -262,7 +263,7 @@ __global__ __launch_bounds__(NT) void transpose16_kernel(int64_t R, int64_t C, c
   __syncthreads();
   for (int cc = tg; cc < 64; cc += 4) {
     const int64_t c = c0 + cc, r = r0 + tc;
-    if (c < C && r < ldo) out[c * ldo + r] = tile[tc][cc];
+    if (c < C && r < rlen) out[c * ldo + r] = tile[tc][cc];
   }
 }
 
@@ -305,7 +306,7 @@ __global__ __launch_bounds__(NT) void gather16_kernel(int64_t ncol, int64_t kb, 
 // reads 4 consecutive c of one row per lane, writes 4 consecutive r of one output row per lane.
 __global__ __launch_bounds__(NT) void transpose16v_kernel(int64_t R, int64_t C, const uint16_t* in,
                                                           int64_t ldi, int64_t si, uint16_t* out,
-                                                          int64_t ldo, int64_t so) {
+                                                          int64_t ldo, int64_t so, int64_t rlen) {
   __shared__ uint16_t tile[64][64 + 4];
   const int tid = threadIdx.x, q4 = (tid & 15) * 4, row = tid >> 4;
   const int64_t r0 = blockIdx.y * 64, c0 = blockIdx.x * 64;
@@ -325,7 +326,7 @@ __global__ __launch_bounds__(NT) void transpose16v_kernel(int64_t R, int64_t C, 
 #pragma unroll
   for (int cc = row; cc < 64; cc += 16) {
     const int64_t c = c0 + cc, r = r0 + q4;
-    if (c >= C || r >= ldo) continue;
+    if (c >= C || r >= rlen) continue;
     *(uint2*)(out + c * ldo + r) =
         make_uint2(tile[q4][cc] | ((unsigned)tile[q4 + 1][cc] << 16),
                    tile[q4 + 2][cc] | ((unsigned)tile[q4 + 3][cc] << 16));
@@ -467,7 +468,7 @@ __global__ void update_t1_kernel(int64_t p, int64_t h, const double* t1, const d
 struct AssembleArgs {
   int64_t p, h;
   FastDiv fp, fh;                       // divisors p, h
-  const double *S0, *Sab, *Sij, *Spp;   // Sij, Spp in (a,i),(b,j) layout; any may be null
+  const double *S0, *Sab, *Sij, *Spp;   // S0, Sab, Sij [a][b][i][j]; Spp [(a,i)][(b,j)]; any null
   const double *Xpp, *Xhh;              // dressed diagonals (full p x p, h x h matrices)
   const double *t2, *vpphh, *t1n;       // old t2, v_pphh, new t1 (for the fused energy)
   double mixing;
@@ -487,7 +488,7 @@ __global__ __launch_bounds__(NT) void assemble_t2_kernel(AssembleArgs g) {
     double H = g.S0 ? g.S0[t] : 0.0;
     if (g.Sab) H += g.Sab[t] - g.Sab[((b * p + a) * h + i) * h + j];
     const int64_t ph = p * h;
-    if (g.Sij) H += g.Sij[(a * h + i) * ph + b * h + j] - g.Sij[(a * h + j) * ph + b * h + i];
+    if (g.Sij) H += g.Sij[t] - g.Sij[((a * p + b) * h + j) * h + i];
     if (g.Spp) {
       H += g.Spp[(a * h + i) * ph + b * h + j] - g.Spp[(b * h + i) * ph + a * h + j]
          - g.Spp[(a * h + j) * ph + b * h + i] + g.Spp[(b * h + j) * ph + a * h + i];
@@ -512,12 +513,12 @@ __global__ __launch_bounds__(NT) void assemble_t2_kernel(AssembleArgs g) {
   }
 }
 
-// The same assembly with one workgroup per pair {a, b} (a <= b), for h <= AH.  The four h x h
-// blocks Sij[(a,i),(b,j)], Sij[(b,i),(a,j)], Spp[(a,i),(b,j)], Spp[(b,i),(a,j)] are read once
-// as h rows of h contiguous doubles and staged in LDS, where the transposed (i <-> j) reads of
-// the antisymmetrisers are served; S0, Sab, t2, v_pphh, t2n are contiguous h x h blocks at
-// (a,b) and (b,a).  Every tensor is therefore read once with row-contiguous accesses (the flat
-// kernel above reads two of the four Sij/Spp terms at a stride of p*h doubles).
+// The same assembly with one workgroup per pair {a, b} (a <= b), for h <= AH.  The h x h blocks
+// Sij[a,b,:,:], Sij[b,a,:,:] (contiguous) and Spp[(a,i),(b,j)], Spp[(b,i),(a,j)] (h rows of h
+// contiguous doubles) are staged in LDS, where the transposed (i <-> j) reads of the
+// antisymmetrisers are served; S0, Sab, t2, v_pphh, t2n are contiguous h x h blocks at (a,b) and
+// (b,a).  Every tensor is therefore read once with row-contiguous accesses (the flat kernel
+// above reads two of the four Spp terms at a stride of p*h doubles).
 // H(b,a) uses the same staged blocks: its Spp part is minus that of H(a,b).
 constexpr int AH = 32;
 
@@ -544,8 +545,8 @@ __global__ __launch_bounds__(NT) void assemble_pair_kernel(AssembleArgs g) {
     for (int e = threadIdx.x; e < hh; e += NT) {
       const int i = fdiv(e, g.fh), j = e - i * h;
       const int64_t oab = (a * g.h + i) * ph + b * g.h + j, oba = (b * g.h + i) * ph + a * g.h + j;
-      SiA[i][j] = g.Sij ? g.Sij[oab] : 0.0;
-      SiB[i][j] = g.Sij ? g.Sij[oba] : 0.0;
+      SiA[i][j] = g.Sij ? g.Sij[(a * p + b) * hh + e] : 0.0;
+      SiB[i][j] = g.Sij ? g.Sij[(b * p + a) * hh + e] : 0.0;
       SpA[i][j] = g.Spp ? g.Spp[oab] : 0.0;
       SpB[i][j] = g.Spp ? g.Spp[oba] : 0.0;
     }
@@ -714,7 +715,7 @@ int nl_convert_rows(int dtype, int64_t R1, int64_t R2, int64_t C1, int64_t C2,
   if (R1 * R2 == 0) return NL_OK;
   if (R1 * R2 > INT32_MAX || ldo > INT32_MAX - CAST_CHUNK) return NL_EARG;
   ConvArgs a;
-  a.R2 = R2; a.C = C1 * C2; a.C2 = C2; a.ldo = ldo;
+  a.R2 = R2; a.C = C1 * C2; a.C2 = C2; a.ldo = ldo; a.cend = ldo;
   a.fr2 = nl_fastdiv(R2); a.fc2 = nl_fastdiv(C2);
   for (int d = 0; d < 4; ++d) a.s[d] = strides4[d];
   a.in = in; a.out = (uint16_t*)out; a.iscale = iscale; a.target = target; a.bf16 = dtype == 0;
@@ -804,29 +805,39 @@ int nl_transpose16_batched(int64_t batch, int64_t R, int64_t C, const void* in, 
                            int64_t stride_in, void* out, int64_t ldo, int64_t stride_out,
                            void* stream);
 
+int nl_transpose16_into(int64_t batch, int64_t R, int64_t C, const void* in, int64_t ldi,
+                        int64_t stride_in, void* out, int64_t ldo, int64_t stride_out,
+                        int64_t rlen, void* stream);
+
 int nl_transpose16(int64_t R, int64_t C, const void* in, int64_t ldi, void* out, int64_t ldo,
                    void* stream) {
-  if (R < 0 || C < 0 || ldi < C || ldo < R) return NL_EARG;
-  if (C == 0 || ldo == 0) return NL_OK;
-  if (!in || !out || nl_cdiv(ldo, 64) > 65535) return NL_EARG;
-  return nl_transpose16_batched(1, R, C, in, ldi, 0, out, ldo, 0, stream);
+  return nl_transpose16_into(1, R, C, in, ldi, 0, out, ldo, 0, ldo, stream);
 }
 
 int nl_transpose16_batched(int64_t batch, int64_t R, int64_t C, const void* in, int64_t ldi,
                            int64_t stride_in, void* out, int64_t ldo, int64_t stride_out,
                            void* stream) {
-  if (batch < 0 || R < 0 || C < 0 || ldi < C || ldo < R) return NL_EARG;
-  if (batch == 0 || C == 0 || ldo == 0) return NL_OK;
-  if (!in || !out || nl_cdiv(ldo, 64) > 65535 || batch > 65535) return NL_EARG;
-  dim3 g((unsigned)nl_cdiv(C, 64), (unsigned)nl_cdiv(ldo, 64), (unsigned)batch);
-  const bool vec = C % 4 == 0 && ldi % 4 == 0 && ldo % 4 == 0 && stride_in % 4 == 0 &&
-                   stride_out % 4 == 0 && (((uintptr_t)in) & 7) == 0 && (((uintptr_t)out) & 7) == 0;
+  return nl_transpose16_into(batch, R, C, in, ldi, stride_in, out, ldo, stride_out, ldo, stream);
+}
+
+int nl_transpose16_into(int64_t batch, int64_t R, int64_t C, const void* in, int64_t ldi,
+                        int64_t stride_in, void* out, int64_t ldo, int64_t stride_out,
+                        int64_t rlen, void* stream) {
+  if (batch < 0 || R < 0 || C < 0 || ldi < C || rlen < R || rlen > ldo) return NL_EARG;
+  if (batch == 0 || C == 0 || rlen == 0) return NL_OK;
+  if (!in || !out || nl_cdiv(rlen, 64) > 65535 || batch > 65535) return NL_EARG;
+  dim3 g((unsigned)nl_cdiv(C, 64), (unsigned)nl_cdiv(rlen, 64), (unsigned)batch);
+  const bool vec = C % 4 == 0 && ldi % 4 == 0 && ldo % 4 == 0 && rlen % 4 == 0 &&
+                   stride_in % 4 == 0 && stride_out % 4 == 0 && (((uintptr_t)in) & 7) == 0 &&
+                   (((uintptr_t)out) & 7) == 0;
   if (vec)
     hipLaunchKernelGGL(transpose16v_kernel, g, dim3(NT), 0, (hipStream_t)stream, R, C,
-                       (const uint16_t*)in, ldi, stride_in, (uint16_t*)out, ldo, stride_out);
+                       (const uint16_t*)in, ldi, stride_in, (uint16_t*)out, ldo, stride_out,
+                       rlen);
   else
     hipLaunchKernelGGL(transpose16_kernel, g, dim3(NT), 0, (hipStream_t)stream, R, C,
-                       (const uint16_t*)in, ldi, stride_in, (uint16_t*)out, ldo, stride_out);
+                       (const uint16_t*)in, ldi, stride_in, (uint16_t*)out, ldo, stride_out,
+                       rlen);
   NL_LAUNCH_CHECK();
   return NL_OK;
 }
@@ -870,6 +881,31 @@ int nl_permute16(int rank, const int64_t* out_shape, const int64_t* in_strides, 
   a.in = (const uint16_t*)in; a.out = (uint16_t*)out;
   hipLaunchKernelGGL((permute16_kernel<1>), dim3(grid_for(a.total)), dim3(NT), 0,
                      (hipStream_t)stream, a);
+  NL_LAUNCH_CHECK();
+  return NL_OK;
+}
+
+int nl_convert_rows_into(int64_t R1, int64_t R2, int64_t C1, int64_t C2, const int64_t* strides4,
+                         const double* in, void* out, int64_t ldo, int64_t cend, void* stream) {
+  if (!strides4 || !in || !out) return NL_EARG;
+  if (R1 < 0 || R2 < 1 || C1 < 0 || C2 < 1 || cend < C1 * C2 || ldo < cend) return NL_EARG;
+  if (R1 * R2 == 0 || cend == 0) return NL_OK;
+  if (R1 * R2 > INT32_MAX || ldo > INT32_MAX - CAST_CHUNK) return NL_EARG;
+  ConvArgs a;
+  a.R2 = R2; a.C = C1 * C2; a.C2 = C2; a.ldo = ldo; a.cend = cend;
+  a.fr2 = nl_fastdiv(R2); a.fc2 = nl_fastdiv(C2);
+  for (int d = 0; d < 4; ++d) a.s[d] = strides4[d];
+  a.in = in; a.out = (uint16_t*)out; a.iscale = nullptr; a.target = 0.f; a.bf16 = 1;
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t R = R1 * R2;
+  if (a.s[1] == 1 && R2 >= 8 && a.s[3] != 1 && nl_cdiv(cend, 64) <= 65535) {
+    dim3 g2((unsigned)nl_cdiv(R, 64), (unsigned)nl_cdiv(cend, 64));
+    hipLaunchKernelGGL(convert_tiles_kernel, g2, dim3(NT), 0, st, a, R, (const unsigned*)nullptr);
+  } else {
+    if (nl_cdiv(cend, CAST_CHUNK) > 65535) return NL_EARG;
+    dim3 g((unsigned)R, (unsigned)nl_cdiv(cend, CAST_CHUNK));
+    hipLaunchKernelGGL(cast_rows_kernel, g, dim3(NT), 0, st, a);
+  }
   NL_LAUNCH_CHECK();
   return NL_OK;
 }

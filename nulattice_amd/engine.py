@@ -147,9 +147,8 @@ class Buffers:
         self.T0 = z(p, h, h)
         self.tau = z(p, p, h, h)
         self.S0, self.Sab = z(p, p, h, h), z(p, p, h, h)
-        self.Sij = z(p, h, p, h)     # P(ij) group as [a][i][b][j]
+        self.Sij = z(p, p, h, h)     # P(ij) group as [a][b][i][j]
         self.Spp = z(p, h, p, h)
-        self.Tr = z(p, h, p, h)      # t2 as [a][i][c][k]
         self.W = z(p, h, p, h)       # ring intermediate [c][k][b][j]
         self.Whh, self.zz = z(h, h, h, h), z(h, h, h, h)
         self.N56 = z(p, h, h, h)
@@ -180,8 +179,6 @@ def t1_part(ham, t1, t2, B, prec=DEFAULT_PREC):
     """H1, X1pp, X1hh of t1Iter (coupled_cluster.py:372-408) into B.H1/B.X1pp/B.X1hh."""
     vpphh = ham.vpphh_view()
     H1, Xpp, Xhh = B.H1, B.X1pp, B.X1hh
-    permute_into(t2.permute(0, 2, 1, 3), B.Tr)       # t2 as [a][i][c][k], shared by both parts
-    t2r = B.Tr.permute(0, 2, 1, 3)                   # the same t2[a,c,i,k] through that layout
     permute_into(ham.f_ph, H1)
     # U[b,k,i,j] = v_phph[b,k,c,i] t1[c,j] (A27) and X[i,d,k,l] = v_pphh[c,d,k,l] t1[c,i] (A34)
     # serve the T2 part; A11 is U's (k = j) trace and A14's s its X's (i = k) trace, so each of
@@ -200,7 +197,7 @@ def t1_part(ham, t1, t2, B, prec=DEFAULT_PREC):
     permute_into(B.s_ph, B.z_ph)
     permute_into(ham.f_ph, B.F_ph)
     ops.axpby(1.0, B.s_ph, 1.0, B.F_ph)
-    contract("ck,acik->ai", B.F_ph, t2r, H1)                                 # A12 + A14
+    contract("ck,acik->ai", B.F_ph, t2, H1)                                  # A12 + A14
     pl = ham.ppph
     if pl.n:
         T0 = B.T0
@@ -289,36 +286,73 @@ def t2_part(ham, t1, t2, B, prec=DEFAULT_PREC):
         if pl.n:
             _block_gemm(pl.r56, tau, B.N56, 0.5, h * h, "r56", prec.get("ladder", "fp64"),
                         tau16)
-    contract("bkij,ak->abij", B.N56, t1, Sab, beta=0.0)                      # A23 + A39
     permute_into(Xpp, B.Mpp)                                                 # X_pp t2 (P(ab) image)
     if pl.n:
         ops.axpby(1.0, B.ret1, 1.0, B.Mpp)                                   # + A38
-    contract("ad,dbij->abij", B.Mpp, t2, Sab, precision=prec["dress"])
-    # ---- Sij: P(ij) group, stored as [a][i][b][j] (like Spp)
-    t2r = B.Tr.permute(0, 2, 1, 3)                                           # t2 via [a][i][c][k]
-    Sij = B.Sij.permute(0, 2, 1, 3)                                          # view Sij[a,b,i,j]
+    if getattr(B, "t2_16", None) is not None and p >= ops.LP_MIN_DIM:
+        _sab_bf16(ham, t1, B, Sab)
+    else:
+        contract("bkij,ak->abij", B.N56, t1, Sab, beta=0.0)                  # A23 + A39
+        contract("ad,dbij->abij", B.Mpp, t2, Sab, precision=prec["dress"])
+    # ---- Sij: P(ij) group, [a][b][i][j]
+    Sij = B.Sij
     contract("cikl,ck->li", ham.v_phhh, t1, B.q_hh, beta=0.0)                # A28
     _lib.call("nl_sub", h * h, ops._p(Xhh), ops._p(B.q_hh), ops._p(B.Mhh), ops._stream())
-    contract("kj,abik->abij", B.Mhh, t2r, Sij, beta=0.0)                     # X_hh t2 - A28 t2
+    contract("kj,abik->abij", B.Mhh, t2, Sij, beta=0.0)                      # X_hh t2 - A28 t2
     if pl.n:
         _segsum(pl.ret2p, h, t1, B.Sij, 1.0)                                 # A36
     # ---- Spp: P(ij)P(ab) group in [(a,i)][(b,j)] layout
     W = B.W
     # t2[d,b,l,j] = t2[b,d,j,l]: ring_W reads t2 in the ring's own [(b,j)][(d,l)] rows, so the
     # two ring GEMMs share one 16-bit conversion of it
-    with ops.lp_reuse(B.Tr):
-        contract("cdkl,bdjl->ckbj", vpphh, t2r, W, alpha=0.5, precision=prec["ring"],
+    # the FP64 ring GEMMs read t2 as [(a,i)][(c,k)] rows of a permuted copy (long contiguous K
+    # runs); the BF16 ones read t2's 16-bit image, so there t2 itself is the operand
+    t2ring = t2
+    if prec["ring"] == "fp64":
+        Tr = ops.WS.get("Tr", p * p * h * h, device=t2.device).view(p, h, p, h)
+        permute_into(t2.permute(0, 2, 1, 3), Tr)
+        t2ring = Tr.permute(0, 2, 1, 3)
+    with ops.lp_reuse(t2ring):
+        contract("cdkl,bdjl->ckbj", vpphh, t2ring, W, alpha=0.5, precision=prec["ring"],
                  tag="ring_W", addend=ham.Wc)                                # A24 + A22
         if pl.n:
             _segsum(pl.ret4, h, t1, W, 1.0)                                  # A37
         contract("cjkl,bl->ckbj", ham.v_phhh, t1, W, alpha=-1.0)             # A29
-        contract("acik,ckbj->aibj", t2r, W, B.Spp, beta=0.0, precision=prec["ring"], tag="ring")
+        contract("acik,ckbj->aibj", t2ring, W, B.Spp, beta=0.0, precision=prec["ring"],
+                 tag="ring")
     contract("idkl,adkj->ilaj", B.Xidkl, t2, B.Yilaj, beta=0.0, precision=prec["hh"])  # A34
     # Spp enters only through P(ij)P(ab), which is invariant under (a,i) <-> (b,j): the A27 term
     # -t1[a,k] U[b,k,i,j] may stand as -U[a,l,j,i] t1[b,l], so both tails are one product
     permute_into(B.Yilaj, B.Zilaj)
     permute_into(B.Ubkij.permute(3, 1, 0, 2), B.Zilaj, alpha=-1.0, beta=1.0)
     contract("ilaj,bl->aibj", B.Zilaj, t1, B.Spp)
+
+
+def _sab_bf16(ham, t1, B, Sab):
+    """Sab[a,(b,i,j)] = sum_k t1[a,k] N56[b,k,i,j] + sum_d Mpp[a,d] t2[d,b,i,j] as ONE BF16 GEMM
+    over K = (k | d): A = [t1 | Mpp] rows a, B = [N56 | t2^T] rows (b,i,j) (t2^T from the
+    iteration's BF16 image of t2), written once with beta = 0 instead of a low-K FP64 pass that
+    writes Sab and a BF16 GEMM that reads and rewrites it."""
+    p, h = ham.p, ham.h
+    ldk = (h + p + 7) // 8 * 8
+    dev, st = t1.device, ops._stream()
+    A16 = ops.WS.get("sab_A16", p * ldk, dtype=torch.int16, device=dev)
+    B16 = ops.WS.get("sab_B16", p * h * h * ldk, dtype=torch.int16, device=dev)
+    i4 = lambda *v: (ctypes.c_int64 * 4)(*v)
+    off = lambda t, n: ctypes.c_void_p(t.data_ptr() + 2 * n)       # 16-bit element offset
+    _lib.call("nl_convert_rows_into", 1, p, 1, h, i4(0, h, 0, 1), ops._p(t1), ops._p(A16), ldk,
+              ldk, st)                                              # [t1 | 0 ...]
+    _lib.call("nl_convert_rows_into", 1, p, 1, p, i4(0, p, 0, 1), ops._p(B.Mpp), off(A16, h),
+              ldk, p, st)                                           # [t1 | Mpp | 0]
+    _lib.call("nl_convert_rows_into", p, h * h, 1, h, i4(h ** 3, 1, 0, h * h), ops._p(B.N56),
+              ops._p(B16), ldk, ldk, st)                            # [N56 | 0 ...]
+    _lib.call("nl_transpose16_into", 1, p, p * h * h, ops._p(B.t2_16), p * h * h, 0,
+              off(B16, h), ldk, 0, p, st)                           # [N56 | t2^T | 0]
+    splits = ops._lp_splits(p, p * h * h, ldk, 1)
+    ws = ops.WS.get("splitk", splits * p * p * h * h, device=dev) if splits > 1 else None
+    _lib.call("nl_gemm_lp", ops.BF16, p, p * h * h, ldk, ctypes.c_double(1.0), ops._p(A16), ldk,
+              0, None, 0, ops._p(B16), ldk, 0, None, 0, ctypes.c_double(0.0), None, ops._p(Sab),
+              p * h * h, 0, 1, splits, ops._p(ws), st)
 
 
 def _pp_ladder(ham, tau, S0, prec, tau16=None):
@@ -396,27 +430,31 @@ def _block_gemm(pb, src, out, alpha, ncol, tag, precision="fp64", src16=None):
 def update(ham, t1, t2, B, t1n, t2n, mixing, prec=DEFAULT_PREC, H2out=None):
     """One Jacobi step (coupled_cluster.py:602-623): writes t1n, t2n and B.E[0] = E(t1n, t2n)."""
     p, h = ham.p, ham.h
-    with ops.lp_reuse(t2, B.Tr):
-        _t2_images(ham, t2, B, prec)
-        t1_part(ham, t1, t2, B, prec)
-        _lib.call("nl_update_t1", p, h, ops._p(t1), ops._p(B.H1), ops._p(B.X1pp),
-                  ops._p(B.X1hh), ctypes.c_double(mixing), ops._p(t1n), ops._stream())
-        t2_part(ham, t1, t2, B, prec)
+    with ops.lp_reuse(t2):
+        B.t2_16 = None
+        try:
+            _t2_images(ham, t2, B, prec)
+            t1_part(ham, t1, t2, B, prec)
+            _lib.call("nl_update_t1", p, h, ops._p(t1), ops._p(B.H1), ops._p(B.X1pp),
+                      ops._p(B.X1hh), ctypes.c_double(mixing), ops._p(t1n), ops._stream())
+            t2_part(ham, t1, t2, B, prec)
+        finally:
+            B.t2_16 = None
     assemble(ham, t2, B, t1n, t2n, mixing, H2out)
 
 
 # BF16 images of t2 for the products that read it in BF16 (all from one cast of t2):
 #   A19 (dress)  t2 rows [a][(d,k,l)]   the cast itself
-#   X_pp t2      t2 rows [(b,i,j)][d]   16-bit transpose
+#   Sab          t2 rows [(b,i,j)][d]   16-bit transpose into the fused Sab operand
 #   Y (hh)       t2 rows [(a,j)][(d,k)] batched 16-bit transpose
-#   ring (both)  Tr rows [(a,i)][(c,k)] 16-bit permute
+#   ring (both)  t2 rows [(a,i)][(c,k)] 16-bit permute
 T2_IMAGES = True
 
 
 def _t2_images(ham, t2, B, prec):
-    """Register the BF16 images of t2 (and of Tr) that this iteration's BF16 products read, made
-    from a single FP64 -> BF16 pass over t2 instead of one conversion of t2 per product; the
-    FP64 t2 and Tr stay valid for the FP64 products.  Needs t2 and Tr under ops.lp_reuse."""
+    """Register the BF16 images of t2 that this iteration's BF16 products read, made from a single
+    FP64 -> BF16 pass over t2 instead of one conversion of t2 per product; the FP64 t2 stays valid
+    for the FP64 products.  Needs t2 under ops.lp_reuse."""
     p, h = ham.p, ham.h
     use = {k for k in ("dress", "hh", "ring") if prec.get(k) == "bf16"}
     if not (T2_IMAGES and use) or p % 8 or (p * h) % 8 or p * p * h * h >= 2 ** 31 - 1:
@@ -429,20 +467,18 @@ def _t2_images(ham, t2, B, prec):
               None, ctypes.c_float(0.0), st)
     if "dress" in use:
         ops.lp_provide(t2, "adkl", ["a"], ["d", "k", "l"], S, p * hh)
-        T1 = ops.WS.get("t2_bf16_bijd", n, dtype=torch.int16, device=dev)
-        _lib.call("nl_transpose16", p, p * hh, ops._p(S), p * hh, ops._p(T1), p, st)
-        ops.lp_provide(t2, "dbij", ["b", "i", "j"], ["d"], T1, p)
+        B.t2_16 = S                      # the fused Sab product transposes it into its operand
     if "hh" in use:
         T2 = ops.WS.get("t2_bf16_ajdk", n, dtype=torch.int16, device=dev)
         _lib.call("nl_transpose16_batched", p, ph, h, ops._p(S), h, p * hh, ops._p(T2), ph,
                   p * hh, st)
         ops.lp_provide(t2, "adkj", ["a", "j"], ["d", "k"], T2, ph)
     if "ring" in use:
-        T3 = ops.WS.get("t2_bf16_aick", n, dtype=torch.int16, device=dev)   # Tr's rows
+        T3 = ops.WS.get("t2_bf16_aick", n, dtype=torch.int16, device=dev)   # [a][i][c][k]
         shp = (ctypes.c_int64 * 4)(p, h, p, h)
         istr = (ctypes.c_int64 * 4)(p * hh, h, hh, 1)                 # out [a][i][c][k]
         _lib.call("nl_permute16", 4, shp, istr, ops._p(S), ops._p(T3), st)
-        ops.lp_provide(B.Tr.permute(0, 2, 1, 3), "acik", ["a", "i"], ["c", "k"], T3, ph)
+        ops.lp_provide(t2, "acik", ["a", "i"], ["c", "k"], T3, ph)
 
 
 def assemble(ham, t2, B, t1n, t2n, mixing, H2out=None):

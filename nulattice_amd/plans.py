@@ -166,7 +166,7 @@ class PpphPlans:
       ret0[a,i]      -= 1/2 v t2[c,d,k,i]   key (a,k)    src (c,d,k,:) of t2 (then sum over k)
       ret1[a,d]      -= v t1[c,k]           key (a,d)    src (c,k) of t1
       ret2[c,d,i,k]  += v t1[a,i]           key (c,d,k)  src (a,:) of t1   (dest [c][i][d][k])
-      ret2p: ret2 as P(ij) sees it, -v t1[a,x] at [c][k][d][x] (unit-stride destination)
+      ret2p: ret2 as P(ij) sees it, -v t1[a,x] at [c][d][k][x] (unit-stride destination)
       ret4[a,d,i,k]  += v t1[c,i]           key (a,d,k)  src (c,:) of t1   (into the ring W)
       R56[a,k,i,j]   += v tau[c,d,i,j]      pair blocks over rows (a,k), columns (c,d)
     """
@@ -187,11 +187,9 @@ class PpphPlans:
         self.ret2 = SegPlan((c * p + d) * h + k, a * h, val,
                             lambda u: (u // (p * h)) * h * ph + ((u // h) % p) * h + u % h,
                             device)
-        # the same term as the residual uses it, under P(ik): -v t1[a,x] at Sij[c][k][d][x], so
-        # x runs along Sij's unit stride (same segments, offset c*h*ph + k*ph + d*h + x)
-        self.ret2p = SegPlan((c * p + d) * h + k, a * h, -val,
-                             lambda u: (u // (p * h)) * h * ph + (u % h) * ph + ((u // h) % p) * h,
-                             device)
+        # the same term as the residual uses it, under P(ik): -v t1[a,x] at Sij[c,d,k,x] of the
+        # [a][b][i][j] accumulator, so x runs along its unit stride (offset ((c*p+d)*h+k)*h + x)
+        self.ret2p = SegPlan((c * p + d) * h + k, a * h, -val, lambda u: u * h, device)
         # ret4[a,d,i,k] lands in W[(d,k),(a,i)] = W[(d*h + k)*ph + a*h + i]
         self.ret4 = SegPlan((a * p + d) * h + k, c * h, val,
                             lambda u: ((u // h) % p * h + u % h) * ph + (u // (p * h)) * h,

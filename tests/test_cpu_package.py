@@ -176,9 +176,9 @@ def test_ppph_plans_reproduce_v_ppph_dgrams():
     np.testing.assert_allclose(o.transpose(0, 2, 1, 3), r2, atol=1e-13)
     # ret2p: the P(ij) form with a unit-stride destination differs from ret2 element by element
     # but has the same P(ij) image
-    q = np.zeros((p, h, p, h)); _segsum_host(pl.ret2p, h, t1, q, 1.0)
-    assert np.abs(q - o).max() > 1e-3
-    np.testing.assert_allclose(cc.pIJ(q.transpose(0, 2, 1, 3)), cc.pIJ(r2), atol=1e-13)
+    q = np.zeros((p, p, h, h)); _segsum_host(pl.ret2p, h, t1, q, 1.0)     # [c][d][k][x]
+    assert np.abs(q - r2).max() > 1e-3
+    np.testing.assert_allclose(cc.pIJ(q), cc.pIJ(r2), atol=1e-13)
     W = np.zeros((p, h, p, h)); _segsum_host(pl.ret4, h, t1, W, 1.0)
     np.testing.assert_allclose(W, r4.transpose(1, 3, 0, 2), atol=1e-13)
     tau = t2 + np.einsum("ci,dj->cdij", t1, t1) - np.einsum("cj,di->cdij", t1, t1)
