@@ -311,6 +311,26 @@ def test_permute16_and_batched_transpose16(h):
     assert torch.equal(T, S.reshape(p, p * h, h).transpose(1, 2))
 
 
+@pytest.mark.parametrize("h", [6, 8])
+def test_fused_sab_bf16_product_is_exact_on_small_integers(h):
+    """The P(ab) group as one BF16 GEMM over K = (k | d): t1 and N56 columns, then Mpp and the
+    transposed BF16 image of t2 (column-range casts and transposes into padded operand rows; h=6
+    takes the 2-byte transpose, h=8 the 8-byte one).  Small integers keep every product and FP32
+    sum exact."""
+    from types import SimpleNamespace
+    from nulattice_amd import engine
+    g = torch.Generator(device="cuda").manual_seed(17 + h)
+    p = 40
+    ri = lambda *s: torch.randint(-3, 4, s, device="cuda", generator=g).double()
+    t1, Mpp, N56, t2 = ri(p, h), ri(p, p), ri(p, h, h, h), ri(p, p, h, h)
+    B = SimpleNamespace(Mpp=Mpp, N56=N56,
+                        t2_16=t2.float().to(torch.bfloat16).view(torch.int16).reshape(-1))
+    Sab = torch.full((p, p, h, h), 9.0, dtype=torch.float64, device="cuda")
+    engine._sab_bf16(SimpleNamespace(p=p, h=h), t1, B, Sab)
+    ref = torch.einsum("ak,bkij->abij", t1, N56) + torch.einsum("ad,dbij->abij", Mpp, t2)
+    assert float((Sab - ref).abs().max()) == 0.0
+
+
 def test_error_codes():
     """Bad arguments come back as -1, never as a fault."""
     lib = _lib.load()
