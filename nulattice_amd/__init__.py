@@ -12,4 +12,5 @@ Hamiltonian.
 """
 from . import _lib  # noqa: F401
 
-__all__ = ["adjoint", "ccdgrams", "ccm", "engine", "hamiltonian", "ops", "planewave", "plans"]
+__all__ = ["adjoint", "ccdgrams", "ccm", "engine", "hamiltonian", "ops", "planewave", "plans",
+           "textio"]

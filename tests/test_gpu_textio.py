@@ -1,0 +1,39 @@
+"""GPU solver on the reference's deuteron benchmark files (benchmarks/CCSDBenchmark.py), read by
+the text loader: the stored dense and sparse energy trajectories, and the oracle step for step."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+from nulattice_amd import ccm  # noqa: E402
+from oracle import ccsd_ref as cc  # noqa: E402
+from tests.test_cpu_textio import FIXTURE, check_trajectory, deuteron  # noqa: E402
+from nulattice_amd.textio import read_energy_fixture  # noqa: E402
+
+
+@pytest.mark.parametrize("form", ["dense", "sparse"])
+def test_deuteron_trajectory_on_gpu(form):
+    """benchmarks/CCSDBenchmark.ipynb cell 2: ccsd_solver(verbose=True) with sparse=False and
+    sparse=True, 33 energies each, pulsed DIIS."""
+    _, fock, dense, sparse = deuteron()
+    blocks = dense if form == "dense" else sparse
+    hist = []
+    e, t1, t2 = ccm.ccsd_solver(fock, blocks, sparse=(form == "sparse"), diis="pulsed",
+                                history=hist)
+    check_trajectory(hist, read_energy_fixture(FIXTURE, form)["energies"], e)
+    # the oracle on the same arrays: identical through the Jacobi steps, and the same fixed point
+    href = []
+    eref, t1r, t2r = cc.ccsd_solver(fock, dense, sparse=False, diis="pulsed", history=href)
+    np.testing.assert_allclose(hist[:11], href[:11], rtol=1e-12)
+    assert abs(e - eref) < 5e-7 * abs(eref)
+    np.testing.assert_allclose(t2.cpu().numpy(), t2r, atol=1e-5)
+
+
+def test_deuteron_bf16_form_refines():
+    """The low-precision form on the same files lands on the FP64 fixed point after refinement."""
+    _, fock, dense, _ = deuteron()
+    e64, _, _ = ccm.ccsd_solver(fock, dense, sparse=False, eps=1e-11, max_diis=0)
+    info = {}
+    e16, _, _ = ccm.ccsd_solver(fock, dense, sparse=False, eps=1e-11, max_diis=0,
+                                precision="bf16", info=info)
+    assert abs(e16 - e64) < 1e-9 * abs(e64)
