@@ -81,6 +81,26 @@ def time_iterations(ccm_mod, engine, ops, H, prec, steps, warmup, torch):
     return dt / steps, kern, energies
 
 
+def time_sectors(SH, steps, warmup, torch):
+    """Per-iteration time of the symmetry-sector form (same step: update, mixing, energy read
+    back to the host)."""
+    Bf = SH.buffers()
+    t2 = SH.init_amplitudes(Bf)
+    t2n = torch.empty_like(t2)
+    for _ in range(warmup):
+        SH.update(t2, t2n, Bf, 0.5)
+        float(Bf["E"].item())
+        t2, t2n = t2n, t2
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        SH.update(t2, t2n, Bf, 0.5)
+        float(Bf["E"].item())
+        t2, t2n = t2n, t2
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps
+
+
 def gradient_check(L, A, prec, engine, ccm, adjoint):
     """dE/dC0 and dE/dD0 at (L, A, C0=-4, D0=1) from the Lambda equations, against central
     finite differences (h=1e-4) of converged total energies.  The normal-ordered blocks are
@@ -233,6 +253,25 @@ def main():
               "steps_refine_fp64": infolp.get("steps_fp64")}
         print(f"# convergence: {dE}", file=sys.stderr, flush=True)
 
+    # the symmetry-sector form of the same iteration (FP64, block-sparse T2)
+    from nulattice_amd.planewave import PlaneWave, shell_holes
+    from nulattice_amd.sectors import SectorHamiltonian
+    t0 = time.perf_counter()
+    SH = SectorHamiltonian(fock, v, *PlaneWave(a.L, shell_holes(a.L, a.A), C0, a.D0).labels())
+    torch.cuda.synchronize()
+    sec_setup = time.perf_counter() - t0
+    ssec = time_sectors(SH, max(a.steps, 20), a.warmup, torch)
+    sector = {"ms_per_step": ssec * 1e3, "iters_per_s": 1.0 / ssec,
+              "speedup_vs_dense_fp64": s64 / ssec, "speedup_vs_dense_lowprec": slp / ssec,
+              "amplitudes": SH.layout.n, "dense_amplitudes": p * p * h * h,
+              "mac_per_iter": SH.layout.macs(), "setup_s": sec_setup}
+    if not a.no_converge:
+        es, _, _ = ccm.ccsd_solver(fock, v, eps=1e-10, maxSteps=300, max_diis=0, hamiltonian=SH)
+        sector.update(E_corr=es, dE_rel_vs_dense_fp64=abs(es - dE["E_corr_fp64"]) /
+                      abs(dE["E_corr_fp64"]))
+    del SH
+    print(f"# sectors: {sector}", file=sys.stderr, flush=True)
+
     # roofline of the dominant kernel: the T2 ring GEMM Spp = T[(a,i),(c,k)] W[(c,k),(b,j)]
     ring_flop = 2.0 * (p * h) ** 3
     ring_s = klp.get("ring")
@@ -267,6 +306,7 @@ def main():
         "speedup_vs_fp64": s64 / slp,
         "canonical_flop_per_iter": canonical_flops(p, h, len(v[0][1]), len(v[1][1])),
         "roofline": roof, "cpu_baseline": cpu, "adjoint_gradient": grad,
+        "sector_form": sector,
     }
     out.update(dE)
     print(json.dumps(out))

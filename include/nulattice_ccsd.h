@@ -198,6 +198,46 @@ int nl_dot_part(int64_t n, const double* x, const double* y, double* part, void*
 int nl_axpby(int64_t n, double a, const double* x, double b, double* y, void* stream);
 int nl_sub(int64_t n, const double* x, const double* y, double* c, void* stream);
 
+/* ---- Symmetry-sector (block-sparse) form of the same iteration ----
+ * For Hamiltonians whose single-particle states carry additive conserved labels that identify
+ * them (plane waves: momentum mod L, spin and isospin projections), t2[a,b,i,j] is nonzero only
+ * in the pair sector l_a + l_b = l_i + l_j and every contraction of t2Iter
+ * (coupled_cluster.py:442-545; ccDgrams.py:240-578) is block diagonal; t1 and f_ph vanish.  The
+ * packed amplitudes hold the sectors' [ab][ij] blocks back to back. */
+
+/* Grouped FP64 GEMM over a device-side problem list: for problem q,
+ *   C(m,n) = alpha * sum_k A(m,k) B(k,n) + beta * D(m,n)   (D has C's layout; null = C)
+ * with A(m,k) = A[oA + m*sAm + k*sAk], B(k,n) = B[oB + k*sBk + n*sBn], C(m,n) = C[oC + m*sCm +
+ * n*sCn].  desc: 12 int64 per problem (M N K oA sAm sAk oB sBk sBn oC sCm sCn); tiles: 3 int32
+ * per 64x32 output tile (problem, m0, n0), one workgroup each.  Replaces the per-sector products
+ * of the pp ladder (v_pppp_dgrams, ccDgrams.py:215-237), the hh ladders (dgram_klij_abkl :240-251,
+ * dgram_cdkl_cdij_abkl :292-303) and the ring terms (dgram_bkcj_acik :253-264,
+ * dgram_cdkl_acik_dblj :279-290). */
+int nl_gemm_grouped_f64(int64_t ntile, const int64_t* desc, const int* tiles, double alpha,
+                        const double* A, const double* B, double beta, const double* D, double* C,
+                        void* stream);
+
+/* out[e] = idx[e] >= 0 ? in[idx[e]] : 0 (crossed-channel image of the packed t2, and the packed
+ * form of reference-layout tensors); scatter: out[idx[e]] = in[e] for idx[e] >= 0 (packed t2 back
+ * to the reference layout t2[a,b,i,j] that ccsd_solver returns, coupled_cluster.py:687). */
+int nl_sector_gather(int64_t n, const int64_t* idx, const double* in, double* out, void* stream);
+int nl_sector_scatter(int64_t n, const int64_t* idx, const double* in, double* out, void* stream);
+
+/* Diagonal dressings X_pp[a,a], X_hh[i,i] of t2Iter (coupled_cluster.py:501-515; dgram_cdkl_bdkl
+ * and dgram_cdkl_cdjl, ccDgrams.py:502-526): X[o] = base[o] + alpha * sum_{e in seg[o]..seg[o+1]} V[elem[e]] * T[elem[e]]. */
+int nl_sector_dress(int64_t no, const int64_t* seg, const int* elem, const double* V,
+                    const double* T, const double* base, double alpha, double* X, void* stream);
+
+/* Fused H2 assembly, Jacobi step, mixing and energy partials on the packed amplitudes
+ * (coupled_cluster.py:475-545, 618-620, 623, 265-286): with (a,b,i,j) = orb[0..3][e],
+ *   H = V + L + S[sg0] - S[sg1] - S[sg2] + S[sg3] + (X[a] + X[b] + X[p+i] + X[p+j]) t,
+ *   tn = mixing t + (1 - mixing) (t + H / -(X[a] + X[b] + X[p+i] + X[p+j])),
+ * part[0:g] = partial sums of 1/4 V tn, part[g:2g] = of H^2, g = nl_partials(n).  L, S (then sg),
+ * tn and H2out may be null. */
+int nl_sector_assemble(int64_t n, int64_t p, const int* orb, const int* sg, const double* V,
+                       const double* L, const double* S, const double* X, const double* t,
+                       double mixing, double* tn, double* H2out, double* part, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

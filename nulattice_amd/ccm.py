@@ -252,12 +252,15 @@ def _solver_ham(fock_mats, two_body_int, device):
 def ccsd_solver(fock_mats, two_body_int, t1initial=None, eps=1e-8, maxSteps=1000, max_diis=10,
                 delta=0, mixing=0.5, verbose=False, sparse=True, ccs=False, *, device=None,
                 diis="legacy", precision="fp64", refine=True, eps_lowprec=None, history=None,
-                info=None, hamiltonian=None, adjoint=False, lambda_eps=1e-10):
+                info=None, hamiltonian=None, adjoint=False, lambda_eps=1e-10, sectors=None):
     """Damped-Jacobi CCSD (coupled_cluster.py:547-687) on one MI355X.
 
     Returns (E_corr, t1, t2); like the reference, returns None after a divergence or NaN.
     ``sparse`` only says how v_pppp / v_ppph are given (COO lists or dense arrays); both are
-    packed into the same block plans.  ``hamiltonian`` may pass a prebuilt engine.Hamiltonian.
+    packed into the same block plans.  ``hamiltonian`` may pass a prebuilt engine.Hamiltonian,
+    or a sectors.SectorHamiltonian for the symmetry-sector form; ``sectors=(part_labels,
+    hole_labels, moduli)`` (e.g. ``PlaneWave.labels()``) builds the latter from the blocks.  The
+    sector form is FP64 and returns the same (E_corr, t1, t2) in the reference layout.
     With a low ``precision`` the iteration first runs to |dE|/|E| < ``eps_lowprec`` (default
     max(eps, 1e-7)) or until |dE| stops decreasing (the low-precision fixed point is only defined
     to the rounding level of the contractions) and then, if ``refine``, continues in FP64 to
@@ -271,6 +274,14 @@ def ccsd_solver(fock_mats, two_body_int, t1initial=None, eps=1e-8, maxSteps=1000
     if precision not in _PREC:
         raise ValueError(f"precision must be one of {sorted(_PREC)}")
     dev = _device(device)
+    from . import sectors as sec
+    if sectors is not None and hamiltonian is None:
+        hamiltonian = sec.SectorHamiltonian(fock_mats, two_body_int, *sectors, device=dev)
+    if isinstance(hamiltonian, sec.SectorHamiltonian):
+        if precision != "fp64" or adjoint:
+            raise ValueError("the sector form runs in FP64 without the adjoint")
+        return sec.solve(hamiltonian, t1initial, eps, maxSteps, max_diis, delta, mixing, verbose,
+                         ccs, diis, history, info)
     ham = hamiltonian if hamiltonian is not None else _solver_ham(fock_mats, two_body_int, dev)
     p, h = ham.p, ham.h
     B = Buffers(p, h, dev)

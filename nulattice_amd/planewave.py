@@ -75,6 +75,17 @@ class PlaneWave:
         self.f_hh = np.diag(e_h + shift)
         self.f_ph = np.zeros((self.p, self.h))
 
+    def labels(self):
+        """(particle labels (p, 5), hole labels (h, 5), moduli): the additive conserved labels
+        (kx, ky, kz, sz, tz) of every state, momenta taken mod L.  They identify each state, so
+        the Hamiltonian is block diagonal in them (``nulattice_amd.sectors``)."""
+        def lab(orbs):
+            o = np.asarray(orbs)
+            s = self._orb_s[o]
+            return np.column_stack([self._kv[self._orb_k[o]], s % 2, s // 2]).astype(np.int64)
+        L = self.L
+        return lab(self.parts), lab(self.holes), np.array([L, L, L, 0, 0], np.int64)
+
     # -------------------------------------------------------------- helpers
     def _K(self, a, b):
         """Encoded total momentum (k_a + k_b mod L) for orbital arrays a, b (broadcasting)."""

@@ -1,0 +1,525 @@
+"""Symmetry-sector (block-sparse) CCSD iteration.
+
+When every single-particle state carries additive conserved labels that identify it -- for plane
+waves the momentum k mod L and the spin and isospin projections (``PlaneWave.labels()``) -- the
+normal-ordered Hamiltonian conserves the summed labels of each pair and:
+
+* t2[a,b,i,j] is nonzero only in the *pair sector* l_a + l_b = l_i + l_j; the packed amplitudes
+  hold, sector after sector, the dense [(a,b)][(i,j)] block over the ordered pairs a != b,
+  i != j of that sector (L=5 A=28: 200,376 amplitudes instead of the 1.75e8 of the dense t2);
+* the ring products of t2Iter are block diagonal in the *crossed sector* l_a - l_i;
+* the dressings X_pp, X_hh and the Fock matrices are diagonal, and t1 stays exactly zero: every
+  term of the T1 residual (t1Iter, coupled_cluster.py:342-414) needs l_a = l_i, which labels that
+  identify the states forbid.
+
+With t1 = 0 the T2 residual of t2Iter (coupled_cluster.py:475-545) is
+  H2 = v_pphh + 1/2 v_pppp t2 (v_pppp_dgrams, ccDgrams.py:215-237)
+       + t2 (1/2 v_hhhh + 1/4 v_pphh^T t2)          (A21 :240-251, A25 :292-303)
+       + P(ij)P(ab) t2 (-v_phph + 1/2 v_pphh t2)   (A22 :253-264, A24 :279-290; crossed channel)
+       + (X_aa + X_bb + X_ii + X_jj) t2             (the X_pp / X_hh terms, coupled_cluster.py:536)
+and every product is a set of small dense GEMMs, one per sector, run by one grouped-GEMM launch.
+An iteration is seven launches (dressing, three ladder GEMM groups, the crossed-channel gather,
+two ring GEMM groups) plus the fused assembly and the energy reduction, all FP64; results equal
+the dense path's to rounding (only summation order differs).
+
+``SectorLayout`` is the host-side index structure (NumPy only); ``SectorHamiltonian`` packs a
+reference-format Hamiltonian onto the GPU; ``ccm.ccsd_solver(..., hamiltonian=SectorHamiltonian)``
+runs the reference solver loop on it and returns t2 in the reference layout.
+"""
+import ctypes
+import math
+import time
+from types import SimpleNamespace
+
+import numpy as np
+import torch
+
+from . import _lib, ops
+from .plans import _np, coo_arrays
+
+F64 = torch.float64
+I32 = torch.int32
+I64 = torch.int64
+
+
+def _norm(x, moduli):
+    x = np.array(x, dtype=np.int64, copy=True)
+    m = moduli > 0
+    if m.any():
+        x[..., m] = np.mod(x[..., m], moduli[m])
+    return x
+
+
+def _group(keys, nkeys):
+    """(position of each item within its key's group (-1 for key -1), group sizes, items sorted
+    by key (stable), group start offsets)."""
+    valid = keys >= 0
+    order = np.argsort(np.where(valid, keys, nkeys), kind="stable")
+    order = order[valid[order]]
+    counts = np.bincount(keys[valid], minlength=nkeys).astype(np.int64)
+    start = np.cumsum(counts) - counts
+    pos = np.full(keys.size, -1, dtype=np.int64)
+    pos[order] = np.arange(order.size) - start[keys[order]]
+    return pos, counts, order, start
+
+
+def _outer(nr, nc):
+    """For blocks of nr[s] x nc[s] laid out back to back (row-major): (block, row, col) of every
+    entry."""
+    sizes = nr * nc
+    blk = np.repeat(np.arange(nr.size), sizes)
+    local = np.arange(int(sizes.sum()), dtype=np.int64) - (np.cumsum(sizes) - sizes)[blk]
+    ncb = nc[blk]
+    return blk, local // np.maximum(ncb, 1), local % np.maximum(ncb, 1)
+
+
+def _tiles(desc, tm=64, tn=32):
+    """Tile table (problem, m0, n0) of a grouped-GEMM problem list."""
+    out = []
+    for q, d in enumerate(desc):
+        M, N = int(d[0]), int(d[1])
+        for m0 in range(0, M, tm):
+            for n0 in range(0, N, tn):
+                out.append((q, m0, n0))
+    return np.asarray(out, dtype=np.int32).reshape(-1, 3)
+
+
+class SectorLayout:
+    """Pair and crossed sectors, packed offsets, index maps and grouped-GEMM problem lists for
+    particles with labels ``part_labels`` (p, d) and holes with ``hole_labels`` (h, d); label
+    component c is taken modulo ``moduli[c]`` (0 = no modulus).
+
+    Attributes: n (packed amplitudes), orb (4, n) = (a, b, i, j) of each packed element,
+    dense_index (n,) = ((a*p + b)*h + i)*h + j, ring_map (ring image -> packed element or -1),
+    sg (4, n) ring-image offsets of Spp[a,i,b,j], Spp[a,j,b,i], Spp[b,i,a,j], Spp[b,j,a,i],
+    seg / elem (dressing segments: elements by a, then by i), and the problem lists
+    ``desc_pp``, ``desc_whh``, ``desc_hh``, ``desc_w``, ``desc_s``.
+    """
+
+    def __init__(self, part_labels, hole_labels, moduli=None):
+        lp = np.asarray(part_labels, dtype=np.int64)
+        lh = np.asarray(hole_labels, dtype=np.int64)
+        lp = lp.reshape(lp.shape[0], -1)
+        lh = lh.reshape(lh.shape[0], -1)
+        if lp.shape[1] != lh.shape[1]:
+            raise ValueError("particle and hole labels need the same number of components")
+        d = lp.shape[1]
+        moduli = np.zeros(d, np.int64) if moduli is None else np.asarray(moduli, np.int64)
+        if moduli.shape != (d,) or (moduli < 0).any():
+            raise ValueError("moduli: one non-negative integer per label component")
+        lp, lh = _norm(lp, moduli), _norm(lh, moduli)
+        allx = np.concatenate([lp, lh])
+        if len(np.unique(allx, axis=0)) != len(allx):
+            raise ValueError("the labels must identify every single-particle state (no two "
+                             "states may share all label components)")
+        p, h = len(lp), len(lh)
+        self.p, self.h, self.d = p, h, d
+        self.labels_p, self.labels_h, self.moduli = lp, lh, moduli
+
+        # ---------------- pair sectors (l_a + l_b = l_i + l_j), ordered pairs a != b, i != j
+        pl = _norm(lp[:, None, :] + lp[None, :, :], moduli).reshape(p * p, d)
+        hl = _norm(lh[:, None, :] + lh[None, :, :], moduli).reshape(h * h, d)
+        u, inv = np.unique(np.concatenate([pl, hl]).reshape(-1, d), axis=0, return_inverse=True)
+        inv = inv.reshape(-1)
+        kp, kh = inv[:p * p].copy(), inv[p * p:].copy()
+        kp[np.arange(p * p) % (p + 1) == 0] = -1          # a == b
+        kh[np.arange(h * h) % (h + 1) == 0] = -1          # i == j
+        present = np.intersect1d(kp[kp >= 0], kh[kh >= 0])
+        S = present.size
+        sec_of = np.full(len(u), -1, np.int64)
+        sec_of[present] = np.arange(S)
+        self.sp = np.where(kp >= 0, sec_of[np.maximum(kp, 0)], -1)     # sector of pair a*p+b
+        self.sh = np.where(kh >= 0, sec_of[np.maximum(kh, 0)], -1)     # sector of pair i*h+j
+        self.pp_pos, self.nP, self.pp_list, self.pp_start = _group(self.sp, S)
+        self.hh_pos, self.nH, self.hh_list, self.hh_start = _group(self.sh, S)
+        self.S = S
+        sizes = self.nP * self.nH
+        self.toff = np.cumsum(sizes) - sizes
+        self.n = int(sizes.sum())
+        self.ppoff = np.cumsum(self.nP ** 2) - self.nP ** 2
+        self.npp = int((self.nP ** 2).sum())
+        self.hhoff = np.cumsum(self.nH ** 2) - self.nH ** 2
+        self.nhh = int((self.nH ** 2).sum())
+        blk, r, c = _outer(self.nP, self.nH)
+        ab = self.pp_list[self.pp_start[blk] + r]
+        ij = self.hh_list[self.hh_start[blk] + c]
+        a, b, i, j = ab // p, ab % p, ij // h, ij % h
+        self.orb = np.stack([a, b, i, j]).astype(np.int32)
+        self.dense_index = ((a * p + b) * h + i) * h + j
+
+        # ---------------- crossed sectors (l_a - l_i)
+        ql = _norm(lp[:, None, :] - lh[None, :, :], moduli).reshape(p * h, d)
+        nql = _norm(-ql, moduli)
+        u2, inv2 = np.unique(np.concatenate([ql, nql]).reshape(-1, d), axis=0,
+                             return_inverse=True)
+        inv2 = inv2.reshape(-1)
+        kq, knq = inv2[:p * h], inv2[p * h:]
+        neg = np.full(len(u2), -1, np.int64)
+        neg[kq] = knq
+        self.rpos, nR_all, self.r_list, self.r_start = _group(kq, len(u2))
+        self.kq = kq
+        # ring storage: one block [R_x][R_neg(x)] per crossed sector x with pairs
+        xs = np.nonzero(nR_all)[0]
+        nx = neg[xs]
+        nRx = nR_all[xs]
+        nRn = nR_all[nx]
+        self.roff_all = np.full(len(u2), -1, np.int64)
+        rsz = nRx * nRn
+        self.roff_all[xs] = np.cumsum(rsz) - rsz
+        self.nring = int(rsz.sum())
+        self.woff_all = np.full(len(u2), -1, np.int64)
+        wsz = nRx * nRx
+        self.woff_all[xs] = np.cumsum(wsz) - wsz
+        self.nw = int(wsz.sum())
+        self.nR_all, self.neg = nR_all, neg
+        # ring image: Tr[(a,i),(c,k)] = t2[a,c,i,k]
+        blk, r, c = _outer(nRx, nRn)
+        ai = self.r_list[self.r_start[xs[blk]] + r]
+        ck = self.r_list[self.r_start[nx[blk]] + c]
+        ra, ri, rc, rk = ai // h, ai % h, ck // h, ck % h
+        self.ring_map = self.element(ra, rc, ri, rk)
+        # W image of v_phph: W[(c,k),(b,j)] = v_phph[b,k,c,j]
+        blk, r, c = _outer(nRx, nRx)
+        ck = self.r_list[self.r_start[xs[blk]] + r]
+        bj = self.r_list[self.r_start[xs[blk]] + c]
+        wc, wk, wb, wj = ck // h, ck % h, bj // h, bj % h
+        self.phph_index = ((wb * h + wk) * p + wc) * h + wj
+        # assembly gathers of Spp[x,y,z,w] = Tr-layout offset of row (x,y), column (z,w)
+        def ro(x, y, z, w):
+            return (self.roff_all[kq[x * h + y]] + self.rpos[x * h + y] * nR_all[kq[z * h + w]]
+                    + self.rpos[z * h + w])
+        self.sg = np.stack([ro(a, i, b, j), ro(a, j, b, i), ro(b, i, a, j),
+                            ro(b, j, a, i)]).astype(np.int32)
+
+        # ---------------- dressing segments: elements by a (X_pp), then by i (X_hh)
+        _, cnt_a, ord_a, _ = _group(a.astype(np.int64), p)
+        _, cnt_i, ord_i, _ = _group(i.astype(np.int64), h)
+        self.seg = np.concatenate([[0], np.cumsum(np.concatenate([cnt_a, cnt_i]))]).astype(
+            np.int64)
+        self.elem = np.concatenate([ord_a, ord_i]).astype(np.int32)
+
+        # ---------------- grouped-GEMM problem lists (M N K oA sAm sAk oB sBk sBn oC sCm sCn)
+        nP, nH, to, po, ho = self.nP, self.nH, self.toff, self.ppoff, self.hhoff
+        one = np.ones(S, np.int64)
+        self.desc_pp = np.stack([nP, nH, nP, po, nP, one, to, nH, one, to, nH, one], 1)
+        self.desc_whh = np.stack([nH, nH, nP, to, one, nH, to, nH, one, ho, nH, one], 1)
+        self.desc_hh = np.stack([nP, nH, nH, to, nH, one, ho, nH, one, to, nH, one], 1)
+        keep = (nRx > 0) & (nRn > 0)
+        xk, nk = xs[keep], nx[keep]
+        Rx, Rn = nR_all[xk], nR_all[nk]
+        ro_x, ro_n = self.roff_all[xk], self.roff_all[nk]
+        wo_x, wo_n = self.woff_all[xk], self.woff_all[nk]
+        o = np.ones(xk.size, np.int64)
+        # W_x = 1/2 VPr_x Tr_neg(x) - VPHPH_x: [R_x][R_x], K = |R_neg(x)|
+        self.desc_w = np.stack([Rx, Rx, Rn, ro_x, Rn, o, ro_n, Rx, o, wo_x, Rx, o], 1)
+        # Spp_x = Tr_x W_neg(x): [R_x][R_neg(x)], K = |R_neg(x)|
+        self.desc_s = np.stack([Rx, Rn, Rn, ro_x, Rn, o, wo_n, Rn, o, ro_x, Rn, o], 1)
+        for name in ("desc_pp", "desc_whh", "desc_hh", "desc_w", "desc_s"):
+            dsc = getattr(self, name).astype(np.int64).reshape(-1, 12)
+            dsc = dsc[(dsc[:, 0] > 0) & (dsc[:, 1] > 0)]
+            setattr(self, name, dsc)
+            setattr(self, name.replace("desc", "tiles"), _tiles(dsc))
+
+    def element(self, a, b, i, j):
+        """Packed index of t2[a,b,i,j] (-1 where the element lies outside every sector)."""
+        p, h = self.p, self.h
+        s1 = self.sp[a * p + b]
+        s2 = self.sh[i * h + j]
+        ok = (s1 >= 0) & (s1 == s2)
+        s = np.maximum(s1, 0)
+        e = self.toff[s] + self.pp_pos[a * p + b] * self.nH[s] + self.hh_pos[i * h + j]
+        return np.where(ok, e, -1).astype(np.int64)
+
+    def pack(self, t2):
+        """Packed form of a reference-layout t2[a,b,i,j] (NumPy)."""
+        return np.asarray(t2, np.float64).reshape(-1)[self.dense_index]
+
+    def unpack(self, x):
+        """Reference-layout t2[a,b,i,j] of packed amplitudes (NumPy)."""
+        out = np.zeros(self.p * self.p * self.h * self.h)
+        out[self.dense_index] = x
+        return out.reshape(self.p, self.p, self.h, self.h)
+
+    def macs(self):
+        """Multiply-adds of one iteration's sector products."""
+        tot = 0
+        for dsc in (self.desc_pp, self.desc_whh, self.desc_hh, self.desc_w, self.desc_s):
+            tot += int((dsc[:, 0] * dsc[:, 1] * dsc[:, 2]).sum())
+        return tot
+
+
+class SectorViolation(ValueError):
+    """The Hamiltonian has a nonzero element that does not conserve the labels."""
+
+
+def _conserves(labels_of, idx, val, moduli, sign=(1, 1, -1, -1)):
+    """True where sum_k sign_k l(idx_k) == 0 (mod) or val == 0."""
+    tot = sum(s * labels_of[k](idx[:, k]) for k, s in enumerate(sign))
+    tot = _norm(tot, moduli)
+    return (~np.any(tot != 0, axis=1)) | (val == 0)
+
+
+def pack_hamiltonian(lay, fock_mats, two_body_int):
+    """Host (NumPy) sector images of a reference-format Hamiltonian: dict with V (packed
+    v_pphh), VPP (v_pppp sector blocks [(a,b)][(c,d)]), VHH (v_hhhh sector blocks
+    [(k,l)][(i,j)]), VPH (W_x[(c,k),(b,j)] = v_phph[b,k,c,j]) and fdiag = (f_aa, -f_ii).
+    Raises SectorViolation if any nonzero element does not conserve the labels, if a Fock matrix
+    is not diagonal or if f_ph is not zero."""
+    f_pp, f_ph, f_hh = (np.asarray(_np(x), np.float64) for x in fock_mats)
+    v_pppp, v_ppph, v_pphh, v_phph, v_phhh, v_hhhh = two_body_int
+    p, h = f_ph.shape
+    if (lay.p, lay.h) != (p, h):
+        raise ValueError(f"labels are for p={lay.p}, h={lay.h}; the Hamiltonian has p={p}, "
+                         f"h={h}")
+    if np.any(f_ph != 0):
+        raise SectorViolation("f_ph must vanish (t1 = 0 in the sector form)")
+    for name, f in (("f_pp", f_pp), ("f_hh", f_hh)):
+        if np.count_nonzero(f - np.diag(np.diag(f))):
+            raise SectorViolation(f"{name} must be diagonal for state-identifying labels")
+    v_pphh = np.asarray(_np(v_pphh), np.float64)
+    v_phph = np.asarray(_np(v_phph), np.float64)
+    v_phhh = np.asarray(_np(v_phhh), np.float64)
+    v_hhhh = np.asarray(_np(v_hhhh), np.float64)
+    V = lay.pack(v_pphh)
+    if np.count_nonzero(V) != np.count_nonzero(v_pphh):
+        raise SectorViolation("v_pphh has nonzero elements outside the pair sectors")
+    VPH = v_phph.reshape(-1)[lay.phph_index]
+    if np.count_nonzero(VPH) != np.count_nonzero(v_phph):
+        raise SectorViolation("v_phph has nonzero elements outside the crossed sectors")
+    blk, r, c = _outer(lay.nH, lay.nH)
+    kl = lay.hh_list[lay.hh_start[blk] + r]
+    ij = lay.hh_list[lay.hh_start[blk] + c]
+    VHH = v_hhhh.reshape(-1)[kl * h * h + ij]
+    if np.count_nonzero(VHH) != np.count_nonzero(v_hhhh):
+        raise SectorViolation("v_hhhh has nonzero elements outside the pair sectors")
+    lpf = lambda x: lay.labels_p[x]
+    lhf = lambda x: lay.labels_h[x]
+    nz = np.nonzero(v_phhh)
+    if nz[0].size and not _conserves([lpf, lhf, lhf, lhf], np.stack(nz, 1), v_phhh[nz],
+                                     lay.moduli).all():
+        raise SectorViolation("v_phhh does not conserve the labels")
+    idx, val = coo_arrays(v_ppph, p, h, "ppph")
+    if idx.size and not _conserves([lpf, lpf, lpf, lhf], idx, val, lay.moduli).all():
+        raise SectorViolation("v_ppph does not conserve the labels")
+    idx, val = coo_arrays(v_pppp, p, h, "pppp")
+    if idx.size and not _conserves([lpf, lpf, lpf, lpf], idx, val, lay.moduli).all():
+        raise SectorViolation("v_pppp does not conserve the labels")
+    rows = idx[:, 0] * p + idx[:, 1]
+    cols = idx[:, 2] * p + idx[:, 3]
+    if (((idx[:, 0] == idx[:, 1]) | (idx[:, 2] == idx[:, 3])) & (val != 0)).any():
+        raise SectorViolation("v_pppp is not antisymmetric (nonzero <aa||cd> or <ab||cc>)")
+    s1, s2 = lay.sp[rows], lay.sp[cols]
+    use = (s1 >= 0) & (s1 == s2)
+    s = s1[use]
+    flat = lay.ppoff[s] + lay.pp_pos[rows[use]] * lay.nP[s] + lay.pp_pos[cols[use]]
+    VPP = np.bincount(flat, weights=val[use], minlength=lay.npp)
+    fdiag = np.concatenate([np.diag(f_pp), -np.diag(f_hh)])
+    return {"V": V, "VPP": VPP, "VHH": VHH, "VPH": VPH, "fdiag": fdiag}
+
+
+class SectorHamiltonian:
+    """A normal-ordered Hamiltonian in the reference's block format (fock_mats, two_body_int; the
+    sparse blocks as COO lists / (indices, values) or dense) packed into symmetry sectors on one
+    GPU.  ``part_labels`` / ``hole_labels`` give every particle / hole state its conserved labels
+    (see ``PlaneWave.labels``).  Raises SectorViolation if any nonzero element does not conserve
+    them, if a Fock matrix is not diagonal or f_ph is not zero."""
+
+    def __init__(self, fock_mats, two_body_int, part_labels, hole_labels, moduli=None,
+                 device="cuda"):
+        device = torch.device(device)
+        if device.type != "cuda":
+            raise _lib.NLError("SectorHamiltonian: a HIP device is required (no CPU fallback)")
+        _lib.load()
+        lay = SectorLayout(part_labels, hole_labels, moduli)
+        hst = pack_hamiltonian(lay, fock_mats, two_body_int)
+        p, h = lay.p, lay.h
+        self.layout, self.p, self.h, self.device = lay, p, h, device
+        V, VPP, VHH, VPH = hst["V"], hst["VPP"], hst["VHH"], hst["VPH"]
+        t = lambda x, dt=F64: torch.tensor(np.ascontiguousarray(x), dtype=dt, device=device)
+        self.V = t(V)
+        self.VPP = t(VPP)
+        self.VHH = t(VHH)
+        self.VPHPH = t(VPH)
+        self.orb = t(lay.orb, I32)
+        self.sg = t(lay.sg, I32)
+        self.ring_map = t(lay.ring_map, I64)
+        self.dense_index = t(lay.dense_index, I64)
+        self.seg = t(lay.seg, I64)
+        self.elem = t(lay.elem, I32)
+        self.fdiag = hst["fdiag"]
+        self.base = t(self.fdiag)
+        self.VPr = torch.empty(lay.nring, dtype=F64, device=device)
+        self._gather(self.ring_map, self.V, self.VPr)
+        self.problems = {}
+        for name in ("pp", "whh", "hh", "w", "s"):
+            dsc, til = getattr(lay, "desc_" + name), getattr(lay, "tiles_" + name)
+            self.problems[name] = (t(dsc, I64), t(til, I32), til.shape[0])
+        self.nparts = int(_lib.load().nl_partials(lay.n))
+        self._dot_parts = max(self.nparts, int(_lib.load().nl_partials(p * h)))
+
+    # ------------------------------------------------------------------ launches
+    def _gather(self, idx, src, out):
+        _lib.call("nl_sector_gather", idx.numel(), ops._p(idx), ops._p(src), ops._p(out),
+                  ops._stream())
+
+    def _gemm(self, name, alpha, A, B, beta, C, D=None):
+        dsc, til, nt = self.problems[name]
+        _lib.call("nl_gemm_grouped_f64", nt, ops._p(dsc), ops._p(til), ctypes.c_double(alpha),
+                  ops._p(A), ops._p(B), ctypes.c_double(beta), ops._p(D), ops._p(C),
+                  ops._stream())
+
+    def buffers(self):
+        lay, dev = self.layout, self.device
+        e = lambda n: torch.empty(max(n, 1), dtype=F64, device=dev)
+        return {"L": e(lay.n), "Whh": e(lay.nhh), "Tr": e(lay.nring), "W": e(lay.nw),
+                "S": e(lay.nring), "X": e(self.p + self.h),
+                "part": e(2 * self._dot_parts + 1), "E": e(1)}
+
+    def init_amplitudes(self, Bf, delta=0.0):
+        """t2 = v_pphh / (f_ii + f_jj - f_aa - f_bb + delta) in packed form (t2Init,
+        coupled_cluster.py:416-439) and its energy in Bf["E"]."""
+        lay = self.layout
+        X = self.fdiag.copy()
+        X[self.p:] -= 0.5 * delta
+        Bf["X"].copy_(torch.tensor(X, dtype=F64))
+        zero = torch.zeros(max(lay.n, 1), dtype=F64, device=self.device)
+        t2 = torch.empty(max(lay.n, 1), dtype=F64, device=self.device)
+        _lib.call("nl_sector_assemble", lay.n, self.p, ops._p(self.orb), None, ops._p(self.V),
+                  None, None, ops._p(Bf["X"]), ops._p(zero), ctypes.c_double(0.0), ops._p(t2),
+                  None, ops._p(Bf["part"]), ops._stream())
+        self._energy(Bf)
+        return t2
+
+    def _energy(self, Bf):
+        _lib.call("nl_finalize", self.nparts, ops._p(Bf["part"]), 0, None, None, ops._p(Bf["E"]),
+                  ops._stream())
+
+    def energy(self, t2, Bf):
+        """E = 1/4 sum v_pphh t2 (ccsd_energy with t1 = 0, coupled_cluster.py:265-286)."""
+        _lib.call("nl_dot_part", self.layout.n, ops._p(self.V), ops._p(t2), ops._p(Bf["part"]),
+                  ops._stream())
+        _lib.call("nl_finalize", self.nparts, ops._p(Bf["part"]), 0, None, None, ops._p(Bf["E"]),
+                  ops._stream())
+        return 0.25 * float(Bf["E"].item())
+
+    def residual(self, t2, Bf):
+        """Everything of H2 except v_pphh, the X t2 terms and the antisymmetrisers: Bf["L"]
+        (ladders), Bf["S"] (ring, crossed layout) and Bf["X"] (dressed diagonals)."""
+        lay = self.layout
+        _lib.call("nl_sector_dress", self.p + self.h, ops._p(self.seg), ops._p(self.elem),
+                  ops._p(self.V), ops._p(t2), ops._p(self.base), ctypes.c_double(-0.5),
+                  ops._p(Bf["X"]), ops._stream())
+        self._gemm("pp", 0.5, self.VPP, t2, 0.0, Bf["L"])
+        self._gemm("whh", 0.25, self.V, t2, 0.5, Bf["Whh"], self.VHH)
+        self._gemm("hh", 1.0, t2, Bf["Whh"], 1.0, Bf["L"])
+        self._gather(self.ring_map, t2, Bf["Tr"])
+        self._gemm("w", 0.5, self.VPr, Bf["Tr"], -1.0, Bf["W"], self.VPHPH)
+        self._gemm("s", 1.0, Bf["Tr"], Bf["W"], 0.0, Bf["S"])
+        return lay
+
+    def update(self, t2, t2n, Bf, mixing, H2out=None):
+        """One damped Jacobi step t2n = mixing t2 + (1 - mixing) t2Iter(t2) (coupled_cluster.py:
+        618-620) and its energy in Bf["E"]."""
+        self.residual(t2, Bf)
+        lay = self.layout
+        _lib.call("nl_sector_assemble", lay.n, self.p, ops._p(self.orb), ops._p(self.sg),
+                  ops._p(self.V), ops._p(Bf["L"]), ops._p(Bf["S"]), ops._p(Bf["X"]), ops._p(t2),
+                  ctypes.c_double(mixing), ops._p(t2n), ops._p(H2out), ops._p(Bf["part"]),
+                  ops._stream())
+        self._energy(Bf)
+
+    def unpack(self, t2):
+        """Reference-layout t2[a,b,i,j] (p, p, h, h) HIP tensor of packed amplitudes."""
+        out = torch.zeros((self.p, self.p, self.h, self.h), dtype=F64, device=self.device)
+        _lib.call("nl_sector_scatter", self.layout.n, ops._p(self.dense_index), ops._p(t2),
+                  ops._p(out), ops._stream())
+        return out
+
+    def pack(self, t2):
+        """Packed amplitudes of a reference-layout (p, p, h, h) tensor or array."""
+        src = t2 if isinstance(t2, torch.Tensor) else torch.tensor(np.asarray(t2), dtype=F64)
+        src = src.to(device=self.device, dtype=F64).contiguous()
+        out = torch.empty(max(self.layout.n, 1), dtype=F64, device=self.device)
+        _lib.call("nl_sector_gather", self.layout.n, ops._p(self.dense_index), ops._p(src),
+                  ops._p(out), ops._stream())
+        return out
+
+
+def solve(SH, t1initial=None, eps=1e-8, maxSteps=1000, max_diis=10, delta=0, mixing=0.5,
+          verbose=False, ccs=False, diis="legacy", history=None, info=None):
+    """The reference solver loop (coupled_cluster.py:547-687) on a SectorHamiltonian: same
+    stopping rule, messages and return values (E_corr, t1, t2), t2 in the reference layout."""
+    from .ccm import _Diis, _fill_info
+    p, h, dev = SH.p, SH.h, SH.device
+    Bf = SH.buffers()
+    t1 = torch.zeros((p, h), dtype=F64, device=dev)
+    if t1initial is not None:
+        t1i = np.asarray(_np(t1initial), np.float64)
+        if t1i.shape != (p, h):
+            raise ValueError(f"t1initial has shape {t1i.shape}, expected {(p, h)}")
+        if np.any(t1i != 0):
+            raise SectorViolation("a nonzero t1initial breaks the sector structure (t1 = 0)")
+    n = max(SH.layout.n, 1)
+    if ccs or t1initial is not None:
+        t2 = torch.zeros(n, dtype=F64, device=dev)
+        Bf["E"].zero_()
+        prev = 0.0
+    else:
+        t2 = SH.init_amplitudes(Bf, delta)
+        prev = float(Bf["E"].item())
+    t2n = torch.empty_like(t2)
+    pulsed = diis != "legacy" and max_diis > 0
+    dstate = _Diis(diis, max_diis, t1, t2, ccs) if pulsed else None
+    if history is not None:
+        history.append(prev)
+    if verbose:
+        print(f"Step {0}: {prev}")
+    t_start = time.perf_counter()
+    energy = prev
+    i = 0
+    while i < maxSteps:
+        if ccs:
+            t2n.copy_(t2)
+            energy = 0.0
+        else:
+            SH.update(t2, t2n, Bf, mixing)
+            energy = float(Bf["E"].item())
+        if history is not None:
+            history.append(energy)
+        if verbose:
+            print(f"Step {i + 1}: {energy}", "difference =",
+                  abs(energy - prev) / abs(energy) if energy != 0 else 0.0)
+        old2 = t2
+        t2, t2n = t2n, t2
+        rel = abs(energy - prev) / abs(energy) if energy != 0 else 0.0
+        if (energy == 0 and prev == 0) or rel < eps:
+            if verbose:
+                print(f"Energy found in {i + 1} iterations")
+            _fill_info(info, 0, i + 1, t_start, energy, True)
+            _sector_info(info, SH, t2)
+            return energy, t1, SH.unpack(t2)
+        if dstate is not None:
+            ext = dstate.step(t1, t2, t1, old2, SimpleNamespace(part=Bf["part"], E=Bf["E"]))
+            if ext is not None:
+                t2 = ext[1]
+                t2n = torch.empty_like(t2)
+        if abs(energy) > 1e10:
+            print("Iteration Diverged")
+            _fill_info(info, 0, i + 1, t_start, energy)
+            return None
+        if math.isnan(energy):
+            print("nan error")
+            _fill_info(info, 0, i + 1, t_start, energy)
+            return None
+        prev = energy
+        i += 1
+    print("Max Iterations Reached")
+    _fill_info(info, 0, maxSteps, t_start, energy, False)
+    _sector_info(info, SH, t2)
+    return energy, t1, SH.unpack(t2)
+
+
+def _sector_info(info, SH, t2):
+    if info is not None:
+        info["sector_t2"] = t2
+        info["sector_amplitudes"] = SH.layout.n

@@ -1,0 +1,52 @@
+"""NumPy statement of the sector-form C-ABI calls (nl_gemm_grouped_f64, nl_sector_gather,
+nl_sector_dress, nl_sector_assemble), following their header comments, for testing the host
+index structures of nulattice_amd.sectors without a GPU."""
+import numpy as np
+
+
+def gemm(desc, alpha, A, B, beta, C, D=None):
+    D = C if D is None else D
+    out = C.copy()
+    for M, N, K, oA, sAm, sAk, oB, sBk, sBn, oC, sCm, sCn in desc:
+        m = np.arange(M)[:, None]
+        n = np.arange(N)[None, :]
+        k = np.arange(K)
+        a = A[oA + m * sAm + k[None, :] * sAk]                  # (M, K)
+        b = B[oB + k[:, None] * sBk + n * sBn]                  # (K, N)
+        c = oC + m * sCm + n * sCn
+        out[c] = alpha * (a @ b) + beta * D[c]
+    C[:] = out
+
+
+def gather(idx, src):
+    return np.where(idx >= 0, src[np.maximum(idx, 0)], 0.0)
+
+
+def dress(seg, elem, V, T, base, alpha):
+    prod = V * T
+    return np.array([base[o] + alpha * prod[elem[seg[o]:seg[o + 1]]].sum()
+                     for o in range(len(base))])
+
+
+def step(lay, hst, t, mixing):
+    """One packed Jacobi step: (t_new, energy, H2)."""
+    V, n = hst["V"], lay.n
+    X = dress(lay.seg, lay.elem, V, t, hst["fdiag"], -0.5)
+    L = np.zeros(n)
+    Whh = np.zeros(lay.nhh)
+    gemm(lay.desc_pp, 0.5, hst["VPP"], t, 0.0, L)
+    gemm(lay.desc_whh, 0.25, V, t, 0.5, Whh, hst["VHH"])
+    gemm(lay.desc_hh, 1.0, t, Whh, 1.0, L)
+    Tr = gather(lay.ring_map, t)
+    VPr = gather(lay.ring_map, V)
+    W = np.zeros(lay.nw)
+    gemm(lay.desc_w, 0.5, VPr, Tr, -1.0, W, hst["VPH"])
+    S = np.zeros(lay.nring)
+    gemm(lay.desc_s, 1.0, Tr, W, 0.0, S)
+    a, b, i, j = lay.orb
+    p = lay.p
+    H = V + L + S[lay.sg[0]] - S[lay.sg[1]] - S[lay.sg[2]] + S[lay.sg[3]]
+    dsum = X[a] + X[b] + X[p + i] + X[p + j]
+    H = H + dsum * t
+    tn = mixing * t + (1.0 - mixing) * (t + H / (-dsum))
+    return tn, 0.25 * (V * tn).sum(), H

@@ -1,0 +1,126 @@
+"""Host index structures of the symmetry-sector form (no GPU): the packed layout, the crossed-
+channel maps and the grouped-GEMM problem lists, stepped by a NumPy statement of the sector
+kernels, against the oracle's dense t1Iter / t2Iter on plane-wave Hamiltonians."""
+import numpy as np
+import pytest
+
+from nulattice_amd.planewave import PlaneWave, shell_holes
+from nulattice_amd.sectors import SectorLayout, SectorViolation, pack_hamiltonian
+from oracle import ccsd_ref as cc
+from tests import sector_emulator as emu
+
+
+def _setup(L, A, C0=-4.0, D0=1.0):
+    pw = PlaneWave(L, shell_holes(L, A), C0, D0)
+    E0, fock, v = pw.hamiltonian(sparse=False)
+    lay = SectorLayout(*pw.labels())
+    return pw, fock, v, lay
+
+
+def _random_t2(lay, seed):
+    """Random antisymmetric amplitudes inside the pair sectors."""
+    rng = np.random.default_rng(seed)
+    p, h = lay.p, lay.h
+    R = lay.unpack(rng.standard_normal(lay.n) * 1e-2)
+    R = R - R.transpose(1, 0, 2, 3)
+    return R - R.transpose(0, 1, 3, 2)
+
+
+@pytest.mark.parametrize("L,A", [(2, 4), (3, 28)])
+def test_layout_is_a_bijection_onto_the_sectors(L, A):
+    pw, fock, v, lay = _setup(L, A)
+    p, h = lay.p, lay.h
+    assert np.unique(lay.dense_index).size == lay.n
+    # the sectors hold exactly the elements allowed by the labels (a != b, i != j)
+    lp, lh, mod = pw.labels()
+    tot = (lp[:, None, None, None] + lp[None, :, None, None] - lh[None, None, :, None]
+           - lh[None, None, None, :])
+    tot[..., :3] %= L
+    allowed = ~np.any(tot != 0, axis=-1)
+    ar, hr = np.arange(p), np.arange(h)
+    allowed &= (ar[:, None] != ar[None, :])[:, :, None, None]
+    allowed &= (hr[:, None] != hr[None, :])[None, None, :, :]
+    assert allowed.sum() == lay.n
+    assert allowed.reshape(-1)[lay.dense_index].all()
+    # element() inverts dense_index
+    a, b, i, j = lay.orb
+    np.testing.assert_array_equal(lay.element(a, b, i, j), np.arange(lay.n))
+    x = np.random.default_rng(0).standard_normal(lay.n)
+    np.testing.assert_array_equal(lay.pack(lay.unpack(x)), x)
+    # every Spp gather lands in ring storage
+    assert lay.sg.min() >= 0 and lay.sg.max() < lay.nring
+    # the t2 of the plane-wave Hamiltonian lies in the sectors
+    v_pphh = v[2]
+    assert np.count_nonzero(v_pphh.reshape(-1)[lay.dense_index]) == np.count_nonzero(v_pphh)
+
+
+@pytest.mark.parametrize("L,A,seed", [(2, 4, 0), (3, 4, 1), (3, 28, 2)])
+def test_sector_step_matches_oracle_dense_step(L, A, seed):
+    """One damped Jacobi step of the sector statement equals the oracle's dense t2Iter step at
+    random antisymmetric sector amplitudes (1e-12), and the oracle keeps t1 = 0 and t2 inside
+    the sectors."""
+    pw, fock, v, lay = _setup(L, A)
+    f_pp, f_ph, f_hh = fock
+    v_pppp, v_ppph, v_pphh, v_phph, v_phhh, v_hhhh = v
+    hst = pack_hamiltonian(lay, fock, v)
+    t2 = _random_t2(lay, seed)
+    t1 = np.zeros((lay.p, lay.h))
+    new1 = cc.t1Iter(t1, t2, f_ph, f_pp, f_hh, v_phph, v_phhh, v_pphh, v_ppph, sparse=False)
+    assert not new1.any()
+    new2 = cc.t2Iter(t1, t2, f_ph, f_hh, f_pp, v_pppp, v_phph, v_phhh, v_pphh, v_ppph, v_hhhh,
+                     sparse=False)
+    ref = 0.5 * t2 + 0.5 * new2
+    assert np.count_nonzero(ref.reshape(-1)[lay.dense_index]) == np.count_nonzero(ref)
+    tn, e, _ = emu.step(lay, hst, lay.pack(t2), 0.5)
+    np.testing.assert_allclose(tn, lay.pack(ref), rtol=1e-12, atol=1e-15 * np.abs(ref).max())
+    assert e == pytest.approx(cc.ccsd_energy(f_ph, v_pphh, ref, t1), rel=1e-12)
+
+
+def test_sector_solve_matches_oracle_trajectory():
+    """Six legacy solver steps in the sector statement against the oracle (L=3, A=4)."""
+    pw, fock, v, lay = _setup(3, 4)
+    hst = pack_hamiltonian(lay, fock, v)
+    href = []
+    cc.ccsd_solver(fock, v, maxSteps=6, max_diis=0, sparse=False, history=href)
+    # t2Init in packed form
+    f = hst["fdiag"]
+    a, b, i, j = lay.orb
+    t = hst["V"] / (-(f[a] + f[b] + f[lay.p + i] + f[lay.p + j]))
+    hist = [0.25 * (hst["V"] * t).sum()]
+    for _ in range(6):
+        t, e, _ = emu.step(lay, hst, t, 0.5)
+        hist.append(e)
+    np.testing.assert_allclose(hist, href, rtol=1e-12)
+
+
+def test_violations_are_refused():
+    pw, fock, v, lay = _setup(2, 4)
+    # a v_pphh element outside every pair sector
+    a, b, i, j = (x.reshape(-1) for x in np.meshgrid(*(np.arange(n) for n in
+                                                       (lay.p, lay.p, lay.h, lay.h)),
+                                                     indexing="ij"))
+    out = np.nonzero((lay.element(a, b, i, j) < 0) & (a != b) & (i != j))[0][0]
+    bad = [x.copy() for x in v]
+    bad[2][a[out], b[out], i[out], j[out]] = 1e-3
+    with pytest.raises(SectorViolation, match="v_pphh"):
+        pack_hamiltonian(lay, fock, bad)
+    # a v_phhh element that does not conserve the labels
+    lp, lh, mod = pw.labels()
+    tot = (lp[:, None, None, None] + lh[None, :, None, None] - lh[None, None, :, None]
+           - lh[None, None, None, :])
+    tot[..., :3] %= 2
+    viol = tuple(np.argwhere(np.any(tot != 0, axis=-1))[0])
+    bad = [x.copy() for x in v]
+    bad[4][viol] = 1e-3
+    with pytest.raises(SectorViolation, match="v_phhh"):
+        pack_hamiltonian(lay, fock, bad)
+    f2 = [fock[0].copy(), fock[1].copy(), fock[2].copy()]
+    f2[0][0, 1] = 0.1
+    with pytest.raises(SectorViolation, match="diagonal"):
+        pack_hamiltonian(lay, f2, v)
+    f3 = [fock[0], fock[1] + 0.1, fock[2]]
+    with pytest.raises(SectorViolation, match="f_ph"):
+        pack_hamiltonian(lay, f3, v)
+    lp, lh, mod = pw.labels()
+    with pytest.raises(ValueError, match="identify"):
+        SectorLayout(lp[:, :3], lh[:, :3], mod[:3])      # momentum alone does not
