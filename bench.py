@@ -36,6 +36,8 @@ def parse():
     ap.add_argument("--prec", default="bf16", choices=["bf16", "fp16"])
     ap.add_argument("--no-converge", action="store_true", help="skip the |dE| convergence runs")
     ap.add_argument("--cpu-sample", default="L4", choices=["L4", "none", "full"])
+    ap.add_argument("--form", default="all", choices=["all", "sectors"],
+                    help="sectors: time only the symmetry-sector form and print its line")
     ap.add_argument("--grad", default="on", choices=["on", "off"],
                     help="adjoint dE/dC0, dE/dD0 at C0=-4, D0=1 checked against central FD")
     return ap.parse_args()
@@ -81,24 +83,41 @@ def time_iterations(ccm_mod, engine, ops, H, prec, steps, warmup, torch):
     return dt / steps, kern, energies
 
 
-def time_sectors(SH, steps, warmup, torch):
+def time_sectors(SH, steps, warmup, torch, graph=True):
     """Per-iteration time of the symmetry-sector form (same step: update, mixing, energy read
-    back to the host)."""
+    back to the host), with the step replayed as a HIP graph or launched kernel by kernel."""
+    from nulattice_amd.sectors import SectorStepper
     Bf = SH.buffers()
-    t2 = SH.init_amplitudes(Bf)
-    t2n = torch.empty_like(t2)
+    st = SectorStepper(SH, Bf, SH.init_amplitudes(Bf), 0.5, graph=graph)
     for _ in range(warmup):
-        SH.update(t2, t2n, Bf, 0.5)
-        float(Bf["E"].item())
-        t2, t2n = t2n, t2
+        st.step()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
-        SH.update(t2, t2n, Bf, 0.5)
-        float(Bf["E"].item())
-        t2, t2n = t2n, t2
+        st.step()
     torch.cuda.synchronize()
     return (time.perf_counter() - t0) / steps
+
+
+def sectors_only(a, fock, v, C0, torch, ccm):
+    """The symmetry-sector form alone: iteration rate, work and the converged energy."""
+    from nulattice_amd.planewave import PlaneWave, shell_holes
+    from nulattice_amd.sectors import SectorHamiltonian
+    t0 = time.perf_counter()
+    SH = SectorHamiltonian(fock, v, *PlaneWave(a.L, shell_holes(a.L, a.A), C0, a.D0).labels())
+    torch.cuda.synchronize()
+    setup = time.perf_counter() - t0
+    s = time_sectors(SH, a.steps, a.warmup, torch)
+    s_eager = time_sectors(SH, a.steps, a.warmup, torch, graph=False)
+    out = {"metric": f"CCSD iters/sec at L={a.L} A={a.A}, symmetry-sector form (FP64)",
+           "value": 1.0 / s, "unit": "iters/s", "n_gpus": 1, "steps": a.steps,
+           "warmup": a.warmup, "ms_per_step": s * 1e3, "higher_is_better": True,
+           "dtype": "f64", "amplitudes": SH.layout.n, "mac_per_iter": SH.layout.macs(),
+           "ms_per_step_without_graph": s_eager * 1e3, "setup_s": setup}
+    if not a.no_converge:
+        e, _, _ = ccm.ccsd_solver(fock, v, eps=1e-10, maxSteps=300, max_diis=0, hamiltonian=SH)
+        out["E_corr"] = e
+    return out
 
 
 def gradient_check(L, A, prec, engine, ccm, adjoint):
@@ -226,6 +245,9 @@ def main():
     C0 = float(rng.uniform(-4.5, -3.5))
     t0 = time.perf_counter()
     E0, fock, v = planewave_hamiltonian(a.L, a.A, C0, a.D0)
+    if a.form == "sectors":
+        print(json.dumps(sectors_only(a, fock, v, C0, torch, ccm)))
+        return
     H = engine.Hamiltonian(fock, v, device="cuda")
     torch.cuda.synchronize()
     setup_s = time.perf_counter() - t0
@@ -261,7 +283,7 @@ def main():
     torch.cuda.synchronize()
     sec_setup = time.perf_counter() - t0
     ssec = time_sectors(SH, max(a.steps, 20), a.warmup, torch)
-    sector = {"ms_per_step": ssec * 1e3, "iters_per_s": 1.0 / ssec,
+    sector = {"ms_per_step": ssec * 1e3, "iters_per_s": 1.0 / ssec, "hip_graph": True,
               "speedup_vs_dense_fp64": s64 / ssec, "speedup_vs_dense_lowprec": slp / ssec,
               "amplitudes": SH.layout.n, "dense_amplitudes": p * p * h * h,
               "mac_per_iter": SH.layout.macs(), "setup_s": sec_setup}

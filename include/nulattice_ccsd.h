@@ -209,13 +209,14 @@ int nl_sub(int64_t n, const double* x, const double* y, double* c, void* stream)
  *   C(m,n) = alpha * sum_k A(m,k) B(k,n) + beta * D(m,n)   (D has C's layout; null = C)
  * with A(m,k) = A[oA + m*sAm + k*sAk], B(k,n) = B[oB + k*sBk + n*sBn], C(m,n) = C[oC + m*sCm +
  * n*sCn].  desc: 12 int64 per problem (M N K oA sAm sAk oB sBk sBn oC sCm sCn); tiles: 3 int32
- * per 64x32 output tile (problem, m0, n0), one workgroup each.  Replaces the per-sector products
- * of the pp ladder (v_pppp_dgrams, ccDgrams.py:215-237), the hh ladders (dgram_klij_abkl :240-251,
- * dgram_cdkl_cdij_abkl :292-303) and the ring terms (dgram_bkcj_acik :253-264,
- * dgram_cdkl_acik_dblj :279-290). */
-int nl_gemm_grouped_f64(int64_t ntile, const int64_t* desc, const int* tiles, double alpha,
-                        const double* A, const double* B, double beta, const double* D, double* C,
-                        void* stream);
+ * per workgroup (problem, m0, n0) of the output tile of ``mode``: 0 = 64x32 tiles; 1 = 32x32 and
+ * 2 = 64x8 tiles with K split over the workgroup's four wavefronts (small-N products and long
+ * K).  Replaces the per-sector products of the pp ladder (v_pppp_dgrams,
+ * ccDgrams.py:215-237), the hh ladders (dgram_klij_abkl :240-251, dgram_cdkl_cdij_abkl
+ * :292-303) and the ring terms (dgram_bkcj_acik :253-264, dgram_cdkl_acik_dblj :279-290). */
+int nl_gemm_grouped_f64(int mode, int64_t ntile, const int64_t* desc, const int* tiles,
+                        double alpha, const double* A, const double* B, double beta,
+                        const double* D, double* C, void* stream);
 
 /* out[e] = idx[e] >= 0 ? in[idx[e]] : 0 (crossed-channel image of the packed t2, and the packed
  * form of reference-layout tensors); scatter: out[idx[e]] = in[e] for idx[e] >= 0 (packed t2 back
@@ -224,9 +225,13 @@ int nl_sector_gather(int64_t n, const int64_t* idx, const double* in, double* ou
 int nl_sector_scatter(int64_t n, const int64_t* idx, const double* in, double* out, void* stream);
 
 /* Diagonal dressings X_pp[a,a], X_hh[i,i] of t2Iter (coupled_cluster.py:501-515; dgram_cdkl_bdkl
- * and dgram_cdkl_cdjl, ccDgrams.py:502-526): X[o] = base[o] + alpha * sum_{e in seg[o]..seg[o+1]} V[elem[e]] * T[elem[e]]. */
-int nl_sector_dress(int64_t no, const int64_t* seg, const int* elem, const double* V,
-                    const double* T, const double* base, double alpha, double* X, void* stream);
+ * and dgram_cdkl_cdjl, ccDgrams.py:502-526): X[o] = base[o] + alpha * sum over the elem entries
+ * of output o of V[elem[e]] * T[elem[e]].  The entries are cut into nchunk chunks of consecutive
+ * entries (chunk c = cstart[c]..cstart[c+1]; chunks of output o = cptr[o]..cptr[o+1]), summed
+ * into ws[0:nchunk] and then added per output in chunk order. */
+int nl_sector_dress(int64_t no, const int64_t* cptr, int64_t nchunk, const int64_t* cstart,
+                    const int* elem, const double* V, const double* T, const double* base,
+                    double alpha, double* ws, double* X, void* stream);
 
 /* Fused H2 assembly, Jacobi step, mixing and energy partials on the packed amplitudes
  * (coupled_cluster.py:475-545, 618-620, 623, 265-286): with (a,b,i,j) = orb[0..3][e],

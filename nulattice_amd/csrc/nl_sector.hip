@@ -38,8 +38,31 @@ __device__ inline double block_sum(double v) {
 // Problem q: C(m,n) = alpha * sum_k A(m,k) B(k,n) + beta * D(m,n) with
 // A(m,k) = A[oA + m*sAm + k*sAk], B(k,n) = B[oB + k*sBk + n*sBn], C/D(m,n) = X[oC + m*sCm + n*sCn].
 // desc holds 12 int64 per problem: M N K oA sAm sAk oB sBk sBn oC sCm sCn.
-// tiles holds 3 int32 per workgroup: problem, m0, n0 (64 x 32 output tile).
-constexpr int GTM = 64, GTN = 32, GTK = 16;
+// tiles holds 3 int32 per workgroup: problem, m0, n0.
+//
+// The sector products are small (M <= a few hundred, N <= 32) and many, so a workgroup's time is
+// its chain of K steps: each step's global loads are issued into registers one step ahead
+// (they are in flight while the previous step is computed from LDS), and each step covers 32
+// values of k.  Operands are read along whichever index is unit-stride.
+struct GDesc {
+  int64_t M, N, K, oA, sAm, sAk, oB, sBk, sBn, oC, sCm, sCn;
+};
+
+__device__ inline GDesc load_desc(const int64_t* d) {
+  return GDesc{d[0], d[1], d[2], d[3], d[4], d[5], d[6], d[7], d[8], d[9], d[10], d[11]};
+}
+
+__device__ inline void grouped_epilogue(const GDesc& g, int64_t m, int64_t n, double acc,
+                                        double alpha, double beta, const double* D, double* C) {
+  if (m >= g.M || n >= g.N) return;
+  const int64_t off = g.oC + m * g.sCm + n * g.sCn;
+  double v = alpha * acc;
+  if (beta != 0.0) v += beta * (D ? D : C)[off];
+  C[off] = v;
+}
+
+// 64 x 32 output tile, 256 threads of 4 x 2 outputs, K steps of 32.
+constexpr int GTM = 64, GTN = 32, GTK = 32;
 
 __global__ __launch_bounds__(NT) void gemm_grouped_f64_kernel(const int64_t* __restrict__ desc,
                                                               const int* __restrict__ tiles,
@@ -50,34 +73,49 @@ __global__ __launch_bounds__(NT) void gemm_grouped_f64_kernel(const int64_t* __r
   __shared__ double Bs[GTK][GTN + 1];
   const int q = tiles[3 * blockIdx.x], m0 = tiles[3 * blockIdx.x + 1],
             n0 = tiles[3 * blockIdx.x + 2];
-  const int64_t* d = desc + 12 * (int64_t)q;
-  const int64_t M = d[0], N = d[1], K = d[2];
-  const int64_t oA = d[3], sAm = d[4], sAk = d[5], oB = d[6], sBk = d[7], sBn = d[8];
-  const int64_t oC = d[9], sCm = d[10], sCn = d[11];
+  const GDesc g = load_desc(desc + 12 * (int64_t)q);
   const int tid = threadIdx.x, ty = tid >> 4, tx = tid & 15;
-  const bool a_k1 = sAk == 1, b_n1 = sBn == 1;
-  double acc[4][2];
+  const bool a_k1 = g.sAk == 1, b_n1 = g.sBn == 1;
+  double ra[8], rb[4];
+  auto fetch = [&](int64_t k0) {
 #pragma unroll
-  for (int r = 0; r < 4; ++r) acc[r][0] = acc[r][1] = 0.0;
-  for (int64_t k0 = 0; k0 < K; k0 += GTK) {
+    for (int r = 0; r < 8; ++r) {
+      const int idx = tid + NT * r;
+      const int kk = a_k1 ? (idx & 31) : (idx >> 6);
+      const int mm = a_k1 ? (idx >> 5) : (idx & 63);
+      const int64_t m = m0 + mm, k = k0 + kk;
+      ra[r] = (m < g.M && k < g.K) ? A[g.oA + m * g.sAm + k * g.sAk] : 0.0;
+    }
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int idx = tid + NT * r;
-      const int kk = a_k1 ? (idx & 15) : (idx >> 6);
-      const int mm = a_k1 ? (idx >> 4) : (idx & 63);
-      const int64_t m = m0 + mm, k = k0 + kk;
-      As[kk][mm] = (m < M && k < K) ? A[oA + m * sAm + k * sAk] : 0.0;
+      const int nn = idx & 31, kk = idx >> 5;           // b_n1: n fastest
+      const int nb = b_n1 ? nn : kk, kb = b_n1 ? kk : nn;
+      const int64_t n = n0 + nb, k = k0 + kb;
+      rb[r] = (n < g.N && k < g.K) ? B[g.oB + k * g.sBk + n * g.sBn] : 0.0;
+    }
+  };
+  double acc[4][2];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) acc[r][0] = acc[r][1] = 0.0;
+  if (g.K > 0) fetch(0);
+  for (int64_t k0 = 0; k0 < g.K; k0 += GTK) {
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      const int idx = tid + NT * r;
+      const int kk = a_k1 ? (idx & 31) : (idx >> 6);
+      const int mm = a_k1 ? (idx >> 5) : (idx & 63);
+      As[kk][mm] = ra[r];
     }
 #pragma unroll
-    for (int r = 0; r < 2; ++r) {
+    for (int r = 0; r < 4; ++r) {
       const int idx = tid + NT * r;
-      const int nn = b_n1 ? (idx & 31) : (idx >> 4);
-      const int kk = b_n1 ? (idx >> 5) : (idx & 15);
-      const int64_t n = n0 + nn, k = k0 + kk;
-      Bs[kk][nn] = (n < N && k < K) ? B[oB + k * sBk + n * sBn] : 0.0;
+      const int nn = idx & 31, kk = idx >> 5;
+      Bs[b_n1 ? kk : nn][b_n1 ? nn : kk] = rb[r];
     }
     __syncthreads();
-#pragma unroll
+    if (k0 + GTK < g.K) fetch(k0 + GTK);
+#pragma unroll 8
     for (int kk = 0; kk < GTK; ++kk) {
       const double b0 = Bs[kk][tx], b1 = Bs[kk][tx + 16];
 #pragma unroll
@@ -89,20 +127,108 @@ __global__ __launch_bounds__(NT) void gemm_grouped_f64_kernel(const int64_t* __r
     }
     __syncthreads();
   }
-  const double* Dp = D ? D : C;
 #pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    const int64_t m = m0 + ty + 16 * r;
-    if (m >= M) continue;
+  for (int r = 0; r < 4; ++r)
 #pragma unroll
-    for (int c = 0; c < 2; ++c) {
-      const int64_t n = n0 + tx + 16 * c;
-      if (n >= N) continue;
-      const int64_t off = oC + m * sCm + n * sCn;
-      double v = alpha * acc[r][c];
-      if (beta != 0.0) v += beta * Dp[off];
-      C[off] = v;
+    for (int c = 0; c < 2; ++c)
+      grouped_epilogue(g, m0 + ty + 16 * r, n0 + tx + 16 * c, acc[r][c], alpha, beta, D, C);
+}
+
+// Split-K tiles for the many small-N products (most pair sectors hold 1-8 hole pairs): a
+// TM x TN output tile per workgroup, whose four wavefronts each own a quarter of K (RM x 4
+// outputs per lane, K steps of 16 with the same one-step register prefetch); the four partial
+// tiles are summed through LDS in a fixed order.  (TM, TN) = (32, 32) or (64, 8).
+constexpr int STK = 16;
+
+template <int TM, int TN>
+__global__ __launch_bounds__(NT) void gemm_grouped_split_kernel(const int64_t* __restrict__ desc,
+                                                                const int* __restrict__ tiles,
+                                                                double alpha, const double* A,
+                                                                const double* B, double beta,
+                                                                const double* D, double* C) {
+  constexpr int NL = TN / 4, ML = 64 / NL, RM = TM / ML;   // lanes along N, along M; rows/lane
+  constexpr int AE = TM * STK / 64, BE = STK * TN / 64;    // A, B elements per lane per step
+  __shared__ double As[4][STK][TM + 1];
+  __shared__ double Bs[4][STK][TN + 1];
+  __shared__ double Red[4][TM][TN + 1];
+  const int q = tiles[3 * blockIdx.x], m0 = tiles[3 * blockIdx.x + 1],
+            n0 = tiles[3 * blockIdx.x + 2];
+  const GDesc g = load_desc(desc + 12 * (int64_t)q);
+  const int tid = threadIdx.x, grp = tid >> 6, lane = tid & 63, ly = lane / NL, lx = lane % NL;
+  const bool a_k1 = g.sAk == 1, b_n1 = g.sBn == 1;
+  const int64_t kq = ((g.K + 3) / 4 + STK - 1) / STK * STK;   // K range per wavefront
+  const int64_t kb0 = grp * kq, kb1 = (kb0 + kq < g.K) ? kb0 + kq : g.K;
+  double ra[AE], rb[BE];
+  auto a_at = [&](int idx, int& mm, int& kk) {
+    if (a_k1) { kk = idx & (STK - 1); mm = idx / STK; } else { mm = idx % TM; kk = idx / TM; }
+  };
+  auto b_at = [&](int idx, int& nn, int& kk) {
+    if (b_n1) { nn = idx % TN; kk = idx / TN; } else { kk = idx & (STK - 1); nn = idx / STK; }
+  };
+  auto fetch = [&](int64_t k0) {
+#pragma unroll
+    for (int r = 0; r < AE; ++r) {
+      int mm, kk;
+      a_at(lane + 64 * r, mm, kk);
+      const int64_t m = m0 + mm, k = k0 + kk;
+      ra[r] = (m < g.M && k < kb1) ? A[g.oA + m * g.sAm + k * g.sAk] : 0.0;
     }
+#pragma unroll
+    for (int r = 0; r < BE; ++r) {
+      int nn, kk;
+      b_at(lane + 64 * r, nn, kk);
+      const int64_t n = n0 + nn, k = k0 + kk;
+      rb[r] = (n < g.N && k < kb1) ? B[g.oB + k * g.sBk + n * g.sBn] : 0.0;
+    }
+  };
+  double acc[RM][4];
+#pragma unroll
+  for (int r = 0; r < RM; ++r)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) acc[r][c] = 0.0;
+  const int64_t steps = kq / STK;                         // the same count in every wavefront
+  if (steps > 0) fetch(kb0);
+  for (int64_t s = 0; s < steps; ++s) {
+    const int64_t k0 = kb0 + s * STK;
+#pragma unroll
+    for (int r = 0; r < AE; ++r) {
+      int mm, kk;
+      a_at(lane + 64 * r, mm, kk);
+      As[grp][kk][mm] = ra[r];
+    }
+#pragma unroll
+    for (int r = 0; r < BE; ++r) {
+      int nn, kk;
+      b_at(lane + 64 * r, nn, kk);
+      Bs[grp][kk][nn] = rb[r];
+    }
+    __syncthreads();
+    if (s + 1 < steps) fetch(k0 + STK);
+#pragma unroll
+    for (int kk = 0; kk < STK; ++kk) {
+      double a[RM], b[4];
+#pragma unroll
+      for (int r = 0; r < RM; ++r) a[r] = As[grp][kk][ly + ML * r];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) b[c] = Bs[grp][kk][lx + NL * c];
+#pragma unroll
+      for (int r = 0; r < RM; ++r)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) acc[r][c] = fma(a[r], b[c], acc[r][c]);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int r = 0; r < RM; ++r)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) Red[grp][ly + ML * r][lx + NL * c] = acc[r][c];
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < TM * TN / NT; ++r) {
+    const int idx = tid + NT * r;
+    const int m = idx / TN, n = idx % TN;
+    const double v = ((Red[0][m][n] + Red[1][m][n]) + Red[2][m][n]) + Red[3][m][n];
+    grouped_epilogue(g, m0 + m, n0 + n, v, alpha, beta, D, C);
   }
 }
 
@@ -127,23 +253,38 @@ __global__ __launch_bounds__(NT) void sector_scatter_kernel(int64_t n,
 }
 
 // ---------------------------------------------------------------- dressing
-// X[o] = base[o] + alpha * sum_{e in segment o} V[elem[e]] * T[elem[e]]; one wavefront per o.
-__global__ __launch_bounds__(NT) void sector_dress_kernel(int64_t no, const int64_t* __restrict__ seg,
-                                                          const int* __restrict__ elem,
-                                                          const double* __restrict__ V,
-                                                          const double* __restrict__ T,
-                                                          const double* __restrict__ base,
-                                                          double alpha, double* __restrict__ X) {
-  const int64_t o = blockIdx.x * (int64_t)(NT / 64) + (threadIdx.x >> 6);
+// X[o] = base[o] + alpha * sum_{e in segment o} V[elem[e]] * T[elem[e]].  Segments are split into
+// chunks of consecutive elem entries (chunk c = cstart[c]..cstart[c+1], chunks of output o =
+// cptr[o]..cptr[o+1]); one wavefront sums a chunk into ws[c], then one lane per output adds its
+// chunks in order (the hole segments are ~p^2 long, so a wavefront per output would serialise).
+__global__ __launch_bounds__(NT) void sector_dress_chunks_kernel(int64_t nchunk,
+                                                                 const int64_t* __restrict__ cstart,
+                                                                 const int* __restrict__ elem,
+                                                                 const double* __restrict__ V,
+                                                                 const double* __restrict__ T,
+                                                                 double* __restrict__ ws) {
+  const int64_t c = blockIdx.x * (int64_t)(NT / 64) + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
-  if (o >= no) return;
+  if (c >= nchunk) return;
   double acc = 0.0;
-  for (int64_t e = seg[o] + lane; e < seg[o + 1]; e += 64) {
+  for (int64_t e = cstart[c] + lane; e < cstart[c + 1]; e += 64) {
     const int s = elem[e];
     acc = fma(V[s], T[s], acc);
   }
   acc = wave_sum(acc);
-  if (lane == 0) X[o] = base[o] + alpha * acc;
+  if (lane == 0) ws[c] = acc;
+}
+
+__global__ __launch_bounds__(NT) void sector_dress_sum_kernel(int64_t no,
+                                                              const int64_t* __restrict__ cptr,
+                                                              const double* __restrict__ ws,
+                                                              const double* __restrict__ base,
+                                                              double alpha, double* __restrict__ X) {
+  const int64_t o = blockIdx.x * (int64_t)NT + threadIdx.x;
+  if (o >= no) return;
+  double acc = 0.0;
+  for (int64_t c = cptr[o]; c < cptr[o + 1]; ++c) acc += ws[c];
+  X[o] = base[o] + alpha * acc;
 }
 
 // ---------------------------------------------------------------- assembly
@@ -195,14 +336,22 @@ inline int grid_of(int64_t n, int cap) {
 
 extern "C" {
 
-int nl_gemm_grouped_f64(int64_t ntile, const int64_t* desc, const int* tiles, double alpha,
-                        const double* A, const double* B, double beta, const double* D, double* C,
-                        void* stream) {
-  if (ntile < 0 || ntile > INT32_MAX) return NL_EARG;
+int nl_gemm_grouped_f64(int mode, int64_t ntile, const int64_t* desc, const int* tiles,
+                        double alpha, const double* A, const double* B, double beta,
+                        const double* D, double* C, void* stream) {
+  if (ntile < 0 || ntile > INT32_MAX || mode < 0 || mode > 2) return NL_EARG;
   if (ntile == 0) return NL_OK;
   if (!desc || !tiles || !A || !B || !C) return NL_EARG;
-  hipLaunchKernelGGL(gemm_grouped_f64_kernel, dim3((unsigned)ntile), dim3(NT), 0,
-                     (hipStream_t)stream, desc, tiles, alpha, A, B, beta, D, C);
+  hipStream_t st = (hipStream_t)stream;
+  if (mode == 0)
+    hipLaunchKernelGGL(gemm_grouped_f64_kernel, dim3((unsigned)ntile), dim3(NT), 0, st, desc,
+                       tiles, alpha, A, B, beta, D, C);
+  else if (mode == 1)
+    hipLaunchKernelGGL((gemm_grouped_split_kernel<32, 32>), dim3((unsigned)ntile), dim3(NT), 0,
+                       st, desc, tiles, alpha, A, B, beta, D, C);
+  else
+    hipLaunchKernelGGL((gemm_grouped_split_kernel<64, 8>), dim3((unsigned)ntile), dim3(NT), 0,
+                       st, desc, tiles, alpha, A, B, beta, D, C);
   NL_LAUNCH_CHECK();
   return NL_OK;
 }
@@ -227,15 +376,20 @@ int nl_sector_scatter(int64_t n, const int64_t* idx, const double* in, double* o
   return NL_OK;
 }
 
-int nl_sector_dress(int64_t no, const int64_t* seg, const int* elem, const double* V,
-                    const double* T, const double* base, double alpha, double* X, void* stream) {
-  if (no < 0) return NL_EARG;
+int nl_sector_dress(int64_t no, const int64_t* cptr, int64_t nchunk, const int64_t* cstart,
+                    const int* elem, const double* V, const double* T, const double* base,
+                    double alpha, double* ws, double* X, void* stream) {
+  if (no < 0 || nchunk < 0) return NL_EARG;
   if (no == 0) return NL_OK;
-  if (!seg || !V || !T || !base || !X) return NL_EARG;
-  const int64_t grid = nl_cdiv(no, NT / 64);
-  if (grid > INT32_MAX) return NL_EARG;
-  hipLaunchKernelGGL(sector_dress_kernel, dim3((unsigned)grid), dim3(NT), 0, (hipStream_t)stream,
-                     no, seg, elem, V, T, base, alpha, X);
+  if (!cptr || !base || !X || (nchunk > 0 && (!cstart || !elem || !V || !T || !ws)))
+    return NL_EARG;
+  const int64_t g1 = nl_cdiv(nchunk, NT / 64), g2 = nl_cdiv(no, NT);
+  if (g1 > INT32_MAX || g2 > INT32_MAX) return NL_EARG;
+  if (nchunk > 0)
+    hipLaunchKernelGGL(sector_dress_chunks_kernel, dim3((unsigned)g1), dim3(NT), 0,
+                       (hipStream_t)stream, nchunk, cstart, elem, V, T, ws);
+  hipLaunchKernelGGL(sector_dress_sum_kernel, dim3((unsigned)g2), dim3(NT), 0,
+                     (hipStream_t)stream, no, cptr, ws, base, alpha, X);
   NL_LAUNCH_CHECK();
   return NL_OK;
 }

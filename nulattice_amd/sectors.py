@@ -17,10 +17,11 @@ With t1 = 0 the T2 residual of t2Iter (coupled_cluster.py:475-545) is
        + t2 (1/2 v_hhhh + 1/4 v_pphh^T t2)          (A21 :240-251, A25 :292-303)
        + P(ij)P(ab) t2 (-v_phph + 1/2 v_pphh t2)   (A22 :253-264, A24 :279-290; crossed channel)
        + (X_aa + X_bb + X_ii + X_jj) t2             (the X_pp / X_hh terms, coupled_cluster.py:536)
-and every product is a set of small dense GEMMs, one per sector, run by one grouped-GEMM launch.
-An iteration is seven launches (dressing, three ladder GEMM groups, the crossed-channel gather,
-two ring GEMM groups) plus the fused assembly and the energy reduction, all FP64; results equal
-the dense path's to rounding (only summation order differs).
+and every product is a set of small dense GEMMs, one per sector, run by grouped-GEMM launches
+(one per kernel shape: short K in one pass, long K split over the wavefronts of a workgroup,
+64x8 tiles for the narrow sectors).  An iteration is the dressing, the ladder and ring products,
+the crossed-channel gather, the fused assembly and the energy reduction, all FP64, replayed as
+one HIP graph; results equal the dense path's to rounding (only summation order differs).
 
 ``SectorLayout`` is the host-side index structure (NumPy only); ``SectorHamiltonian`` packs a
 reference-format Hamiltonian onto the GPU; ``ccm.ccsd_solver(..., hamiltonian=SectorHamiltonian)``
@@ -73,8 +74,21 @@ def _outer(nr, nc):
     return blk, local // np.maximum(ncb, 1), local % np.maximum(ncb, 1)
 
 
-def _tiles(desc, tm=64, tn=32):
-    """Tile table (problem, m0, n0) of a grouped-GEMM problem list."""
+# output tile (rows, columns) of the grouped-GEMM kernel modes: 0 = 64x32 tiles, 1 = 32x32 and
+# 2 = 64x8 tiles with K split over the four wavefronts of a workgroup
+MODE_TILE = {0: (64, 32), 1: (32, 32), 2: (64, 8)}
+
+
+def _mode(M, N, K):
+    """Kernel mode for one product: short K in one pass, long K split, narrow N in 64x8 tiles."""
+    if K <= 32:
+        return 0
+    return 2 if N <= 8 else 1
+
+
+def _tiles(desc, mode=0):
+    """Tile table (problem, m0, n0) of a grouped-GEMM problem list for one kernel mode."""
+    tm, tn = MODE_TILE[mode]
     out = []
     for q, d in enumerate(desc):
         M, N = int(d[0]), int(d[1])
@@ -82,6 +96,19 @@ def _tiles(desc, tm=64, tn=32):
             for n0 in range(0, N, tn):
                 out.append((q, m0, n0))
     return np.asarray(out, dtype=np.int32).reshape(-1, 3)
+
+
+def _launch_lists(desc):
+    """The problem list split by kernel mode: [(mode, desc, tiles), ...]."""
+    if len(desc) == 0:
+        return []
+    modes = np.array([_mode(*map(int, d[:3])) for d in desc])
+    out = []
+    for m in (0, 1, 2):
+        sub = desc[modes == m]
+        if len(sub):
+            out.append((m, sub, _tiles(sub, m)))
+    return out
 
 
 class SectorLayout:
@@ -95,6 +122,8 @@ class SectorLayout:
     seg / elem (dressing segments: elements by a, then by i), and the problem lists
     ``desc_pp``, ``desc_whh``, ``desc_hh``, ``desc_w``, ``desc_s``.
     """
+
+    DRESS_CHUNK = 256
 
     def __init__(self, part_labels, hole_labels, moduli=None):
         lp = np.asarray(part_labels, dtype=np.int64)
@@ -197,6 +226,13 @@ class SectorLayout:
         self.seg = np.concatenate([[0], np.cumsum(np.concatenate([cnt_a, cnt_i]))]).astype(
             np.int64)
         self.elem = np.concatenate([ord_a, ord_i]).astype(np.int32)
+        # chunks of at most DRESS_CHUNK consecutive entries, never crossing a segment
+        nch = -(-np.diff(self.seg) // self.DRESS_CHUNK)
+        self.cptr = np.concatenate([[0], np.cumsum(nch)]).astype(np.int64)
+        owner = np.repeat(np.arange(p + h), nch)
+        first = self.seg[owner] + (np.arange(int(nch.sum())) - self.cptr[owner]) * \
+            self.DRESS_CHUNK
+        self.cstart = np.concatenate([first, [self.seg[-1]]]).astype(np.int64)
 
         # ---------------- grouped-GEMM problem lists (M N K oA sAm sAk oB sBk sBn oC sCm sCn)
         nP, nH, to, po, ho = self.nP, self.nH, self.toff, self.ppoff, self.hhoff
@@ -218,7 +254,7 @@ class SectorLayout:
             dsc = getattr(self, name).astype(np.int64).reshape(-1, 12)
             dsc = dsc[(dsc[:, 0] > 0) & (dsc[:, 1] > 0)]
             setattr(self, name, dsc)
-            setattr(self, name.replace("desc", "tiles"), _tiles(dsc))
+            setattr(self, name.replace("desc", "launch"), _launch_lists(dsc))
 
     def element(self, a, b, i, j):
         """Packed index of t2[a,b,i,j] (-1 where the element lies outside every sector)."""
@@ -344,16 +380,18 @@ class SectorHamiltonian:
         self.sg = t(lay.sg, I32)
         self.ring_map = t(lay.ring_map, I64)
         self.dense_index = t(lay.dense_index, I64)
-        self.seg = t(lay.seg, I64)
         self.elem = t(lay.elem, I32)
+        self.cptr = t(lay.cptr, I64)
+        self.cstart = t(lay.cstart, I64)
         self.fdiag = hst["fdiag"]
         self.base = t(self.fdiag)
         self.VPr = torch.empty(lay.nring, dtype=F64, device=device)
         self._gather(self.ring_map, self.V, self.VPr)
         self.problems = {}
         for name in ("pp", "whh", "hh", "w", "s"):
-            dsc, til = getattr(lay, "desc_" + name), getattr(lay, "tiles_" + name)
-            self.problems[name] = (t(dsc, I64), t(til, I32), til.shape[0])
+            self.problems[name] = [(mode, t(dsc, I64), t(til, I32), til.shape[0])
+                                   for mode, dsc, til in getattr(lay, "launch_" + name)]
+        self._side = (torch.cuda.Stream(device=device), torch.cuda.Stream(device=device))
         self.nparts = int(_lib.load().nl_partials(lay.n))
         self._dot_parts = max(self.nparts, int(_lib.load().nl_partials(p * h)))
 
@@ -363,16 +401,16 @@ class SectorHamiltonian:
                   ops._stream())
 
     def _gemm(self, name, alpha, A, B, beta, C, D=None):
-        dsc, til, nt = self.problems[name]
-        _lib.call("nl_gemm_grouped_f64", nt, ops._p(dsc), ops._p(til), ctypes.c_double(alpha),
-                  ops._p(A), ops._p(B), ctypes.c_double(beta), ops._p(D), ops._p(C),
-                  ops._stream())
+        for mode, dsc, til, nt in self.problems[name]:
+            _lib.call("nl_gemm_grouped_f64", mode, nt, ops._p(dsc), ops._p(til),
+                      ctypes.c_double(alpha), ops._p(A), ops._p(B), ctypes.c_double(beta),
+                      ops._p(D), ops._p(C), ops._stream())
 
     def buffers(self):
         lay, dev = self.layout, self.device
         e = lambda n: torch.empty(max(n, 1), dtype=F64, device=dev)
         return {"L": e(lay.n), "Whh": e(lay.nhh), "Tr": e(lay.nring), "W": e(lay.nw),
-                "S": e(lay.nring), "X": e(self.p + self.h),
+                "S": e(lay.nring), "X": e(self.p + self.h), "Xc": e(len(lay.cstart)),
                 "part": e(2 * self._dot_parts + 1), "E": e(1)}
 
     def init_amplitudes(self, Bf, delta=0.0):
@@ -404,17 +442,31 @@ class SectorHamiltonian:
 
     def residual(self, t2, Bf):
         """Everything of H2 except v_pphh, the X t2 terms and the antisymmetrisers: Bf["L"]
-        (ladders), Bf["S"] (ring, crossed layout) and Bf["X"] (dressed diagonals)."""
+        (ladders), Bf["S"] (ring, crossed layout) and Bf["X"] (dressed diagonals).
+
+        The three independent chains -- dressing, pp ladder and hh ladder; the Whh dressing; the
+        crossed-channel gather and the two ring products -- run on the caller's stream and two
+        side streams (default priority), joined before returning, so the small launches overlap
+        instead of queueing behind each other's tails."""
         lay = self.layout
-        _lib.call("nl_sector_dress", self.p + self.h, ops._p(self.seg), ops._p(self.elem),
-                  ops._p(self.V), ops._p(t2), ops._p(self.base), ctypes.c_double(-0.5),
-                  ops._p(Bf["X"]), ops._stream())
+        main = torch.cuda.current_stream()
+        sa, sb = self._side
+        sa.wait_stream(main)
+        sb.wait_stream(main)
+        with torch.cuda.stream(sa):
+            self._gemm("whh", 0.25, self.V, t2, 0.5, Bf["Whh"], self.VHH)
+        with torch.cuda.stream(sb):
+            self._gather(self.ring_map, t2, Bf["Tr"])
+            self._gemm("w", 0.5, self.VPr, Bf["Tr"], -1.0, Bf["W"], self.VPHPH)
+            self._gemm("s", 1.0, Bf["Tr"], Bf["W"], 0.0, Bf["S"])
+        _lib.call("nl_sector_dress", self.p + self.h, ops._p(self.cptr), len(lay.cstart) - 1,
+                  ops._p(self.cstart), ops._p(self.elem), ops._p(self.V), ops._p(t2),
+                  ops._p(self.base), ctypes.c_double(-0.5), ops._p(Bf["Xc"]), ops._p(Bf["X"]),
+                  ops._stream())
         self._gemm("pp", 0.5, self.VPP, t2, 0.0, Bf["L"])
-        self._gemm("whh", 0.25, self.V, t2, 0.5, Bf["Whh"], self.VHH)
+        main.wait_stream(sa)
         self._gemm("hh", 1.0, t2, Bf["Whh"], 1.0, Bf["L"])
-        self._gather(self.ring_map, t2, Bf["Tr"])
-        self._gemm("w", 0.5, self.VPr, Bf["Tr"], -1.0, Bf["W"], self.VPHPH)
-        self._gemm("s", 1.0, Bf["Tr"], Bf["W"], 0.0, Bf["S"])
+        main.wait_stream(sb)
         return lay
 
     def update(self, t2, t2n, Bf, mixing, H2out=None):
@@ -445,8 +497,52 @@ class SectorHamiltonian:
         return out
 
 
+class SectorStepper:
+    """Damped Jacobi steps on a fixed pair of packed amplitude buffers.  With ``graph`` each
+    direction of the step (buffer 0 -> 1 and 1 -> 0) is captured once as a HIP graph
+    (torch.cuda.CUDAGraph around the library's launches, which allocate nothing and record no
+    memset; the side-stream fork and join of ``residual`` become graph edges), so a step is one
+    graph launch plus the energy read-back instead of thirteen kernel launches from Python."""
+
+    def __init__(self, SH, Bf, t_init, mixing, graph=True):
+        self.SH, self.Bf, self.mixing = SH, Bf, mixing
+        self.T = [t_init, torch.empty_like(t_init)]
+        self.cur = 0
+        self.graphs = None
+        if graph:
+            self.graphs = []
+            for k in (0, 1):
+                g = torch.cuda.CUDAGraph()
+                with torch.cuda.graph(g):
+                    SH.update(self.T[k], self.T[1 - k], Bf, mixing)
+                self.graphs.append(g)
+
+    @property
+    def t(self):
+        """The current amplitudes."""
+        return self.T[self.cur]
+
+    @property
+    def previous(self):
+        """The amplitudes before the last step (valid until the next step)."""
+        return self.T[1 - self.cur]
+
+    def step(self):
+        """Advance the current amplitudes by one step; returns the new energy."""
+        if self.graphs is not None:
+            self.graphs[self.cur].replay()
+        else:
+            self.SH.update(self.T[self.cur], self.T[1 - self.cur], self.Bf, self.mixing)
+        self.cur ^= 1
+        return float(self.Bf["E"].item())
+
+    def set(self, t):
+        """Replace the current amplitudes (a DIIS extrapolation)."""
+        self.T[self.cur].copy_(t)
+
+
 def solve(SH, t1initial=None, eps=1e-8, maxSteps=1000, max_diis=10, delta=0, mixing=0.5,
-          verbose=False, ccs=False, diis="legacy", history=None, info=None):
+          verbose=False, ccs=False, diis="legacy", history=None, info=None, graph=True):
     """The reference solver loop (coupled_cluster.py:547-687) on a SectorHamiltonian: same
     stopping rule, messages and return values (E_corr, t1, t2), t2 in the reference layout."""
     from .ccm import _Diis, _fill_info
@@ -467,9 +563,9 @@ def solve(SH, t1initial=None, eps=1e-8, maxSteps=1000, max_diis=10, delta=0, mix
     else:
         t2 = SH.init_amplitudes(Bf, delta)
         prev = float(Bf["E"].item())
-    t2n = torch.empty_like(t2)
     pulsed = diis != "legacy" and max_diis > 0
     dstate = _Diis(diis, max_diis, t1, t2, ccs) if pulsed else None
+    stepper = SectorStepper(SH, Bf, t2, mixing, graph=graph and not ccs)
     if history is not None:
         history.append(prev)
     if verbose:
@@ -479,18 +575,17 @@ def solve(SH, t1initial=None, eps=1e-8, maxSteps=1000, max_diis=10, delta=0, mix
     i = 0
     while i < maxSteps:
         if ccs:
-            t2n.copy_(t2)
-            energy = 0.0
+            energy = 0.0            # t1 = 0 and t2 = 0 stay fixed
+            stepper.previous.copy_(stepper.t)
+            stepper.cur ^= 1
         else:
-            SH.update(t2, t2n, Bf, mixing)
-            energy = float(Bf["E"].item())
+            energy = stepper.step()
         if history is not None:
             history.append(energy)
         if verbose:
             print(f"Step {i + 1}: {energy}", "difference =",
                   abs(energy - prev) / abs(energy) if energy != 0 else 0.0)
-        old2 = t2
-        t2, t2n = t2n, t2
+        t2 = stepper.t
         rel = abs(energy - prev) / abs(energy) if energy != 0 else 0.0
         if (energy == 0 and prev == 0) or rel < eps:
             if verbose:
@@ -499,10 +594,11 @@ def solve(SH, t1initial=None, eps=1e-8, maxSteps=1000, max_diis=10, delta=0, mix
             _sector_info(info, SH, t2)
             return energy, t1, SH.unpack(t2)
         if dstate is not None:
-            ext = dstate.step(t1, t2, t1, old2, SimpleNamespace(part=Bf["part"], E=Bf["E"]))
+            ext = dstate.step(t1, t2, t1, stepper.previous,
+                              SimpleNamespace(part=Bf["part"], E=Bf["E"]))
             if ext is not None:
-                t2 = ext[1]
-                t2n = torch.empty_like(t2)
+                stepper.set(ext[1])
+                t2 = stepper.t
         if abs(energy) > 1e10:
             print("Iteration Diverged")
             _fill_info(info, 0, i + 1, t_start, energy)

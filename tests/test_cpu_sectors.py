@@ -49,6 +49,22 @@ def test_layout_is_a_bijection_onto_the_sectors(L, A):
     np.testing.assert_array_equal(lay.pack(lay.unpack(x)), x)
     # every Spp gather lands in ring storage
     assert lay.sg.min() >= 0 and lay.sg.max() < lay.nring
+    # dressing chunks tile every segment, in order, at most DRESS_CHUNK entries each
+    cs, cp, seg = lay.cstart, lay.cptr, lay.seg
+    assert cs[0] == 0 and cs[-1] == seg[-1] and np.all(np.diff(cs) > 0)
+    assert np.diff(cs).max() <= lay.DRESS_CHUNK
+    np.testing.assert_array_equal(cs[cp], seg)
+    # the launch lists of each product cover every output element exactly once
+    from nulattice_amd.sectors import MODE_TILE
+    for name in ("pp", "whh", "hh", "w", "s"):
+        dsc = getattr(lay, "desc_" + name)
+        covered = 0
+        for mode, sub, til in getattr(lay, "launch_" + name):
+            tm, tn = MODE_TILE[mode]
+            M, N = sub[til[:, 0], 0], sub[til[:, 0], 1]
+            assert (til[:, 1] < M).all() and (til[:, 2] < N).all()
+            covered += int((np.minimum(tm, M - til[:, 1]) * np.minimum(tn, N - til[:, 2])).sum())
+        assert covered == int((dsc[:, 0] * dsc[:, 1]).sum())
     # the t2 of the plane-wave Hamiltonian lies in the sectors
     v_pphh = v[2]
     assert np.count_nonzero(v_pphh.reshape(-1)[lay.dense_index]) == np.count_nonzero(v_pphh)

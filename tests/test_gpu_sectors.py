@@ -23,12 +23,12 @@ def host(t):
     return t.detach().cpu().numpy()
 
 
-def _problems(rng, nprob, integer):
+def _problems(rng, nprob, integer, small=False):
     """Random problem list over three flat buffers, with both unit-stride orientations of A and
-    B, sizes straddling the 64 x 32 x 16 tile."""
+    B, sizes straddling the 64 x 32 x 32 tile (small: outputs up to 32 x 32, K up to 300)."""
     desc, oa, ob, oc = [], 0, 0, 0
     for _ in range(nprob):
-        M, N, K = (int(rng.integers(1, x)) for x in (150, 70, 90))
+        M, N, K = (int(rng.integers(1, x)) for x in ((33, 33, 300) if small else (150, 70, 90)))
         ta, tb = rng.integers(0, 2, 2)
         sAm, sAk = (K, 1) if ta else (1, M)
         sBk, sBn = (N, 1) if tb else (1, K)
@@ -42,18 +42,21 @@ def _problems(rng, nprob, integer):
     return desc, gen(oa), gen(ob), gen(oc), gen(oc)
 
 
+@pytest.mark.parametrize("mode", [0, 1, 2])
 @pytest.mark.parametrize("integer,beta,use_d", [(True, 0.0, False), (True, -1.0, True),
                                                 (False, 0.5, False), (False, 1.0, True)])
-def test_grouped_gemm(integer, beta, use_d):
+def test_grouped_gemm(integer, beta, use_d, mode):
+    """Every kernel mode (64x32 tiles; 32x32 and 64x8 tiles with K split over wavefronts) on
+    the same random problem lists."""
     rng = np.random.default_rng(7)
-    desc, A, B, C, D = _problems(rng, 40, integer)
-    til = _tiles(desc)
+    desc, A, B, C, D = _problems(rng, 40, integer, mode == 1)
+    til = _tiles(desc, mode)
     alpha = 1.0 if integer else 0.75
     # every device buffer stays bound until the call has been read back: a temporary freed
     # before the launch hands its memory to the next upload, which would overwrite it first
     Cg, Dg, Ag, Bg = dev(C), dev(D), dev(A), dev(B)
     descg, tilg = dev(desc, torch.int64), dev(til, torch.int32)
-    _lib.call("nl_gemm_grouped_f64", til.shape[0], ops._p(descg), ops._p(tilg),
+    _lib.call("nl_gemm_grouped_f64", mode, til.shape[0], ops._p(descg), ops._p(tilg),
               ctypes.c_double(alpha), ops._p(Ag), ops._p(Bg), ctypes.c_double(beta),
               ops._p(Dg if use_d else None), ops._p(Cg), ops._stream())
     torch.cuda.synchronize()
@@ -150,3 +153,20 @@ def test_sector_ccs_and_t1initial():
         ccm.ccsd_solver(fock, v, t1initial=np.ones((pw.p, pw.h)), sectors=pw.labels())
     with pytest.raises(ValueError):
         ccm.ccsd_solver(fock, v, precision="bf16", sectors=pw.labels())
+
+
+def test_graph_replays_equal_kernel_launches():
+    """The HIP-graph step (two captured graphs, replayed alternately, with the DIIS copies, dots
+    and host read-backs between replays) gives bit-identical trajectories and amplitudes to the
+    same kernels launched one by one (L=3, A=28, pulsed DIIS, 40 steps)."""
+    from nulattice_amd import sectors
+    pw, fock, v = _pw(3, 28)
+    SH = SectorHamiltonian(fock, v, *pw.labels())
+    res = {}
+    for graph in (False, True):
+        hist = []
+        e, _, t2 = sectors.solve(SH, eps=1e-14, maxSteps=40, diis="pulsed", history=hist,
+                                 graph=graph)
+        res[graph] = (hist, host(t2))
+    assert res[True][0] == res[False][0]
+    np.testing.assert_array_equal(res[True][1], res[False][1])
