@@ -12,5 +12,5 @@ Hamiltonian.
 """
 from . import _lib  # noqa: F401
 
-__all__ = ["adjoint", "ccdgrams", "ccm", "engine", "hamiltonian", "ops", "planewave", "plans",
-           "textio"]
+__all__ = ["adjoint", "ccdgrams", "ccm", "engine", "hamiltonian", "hfbasis", "ops", "planewave",
+           "plans", "sectors", "textio"]
