@@ -99,12 +99,12 @@ def time_sectors(SH, steps, warmup, torch, graph=True):
     return (time.perf_counter() - t0) / steps
 
 
-def sectors_only(a, fock, v, C0, torch, ccm):
+def sectors_only(a, C0, torch, ccm):
     """The symmetry-sector form alone: iteration rate, work and the converged energy."""
     from nulattice_amd.planewave import PlaneWave, shell_holes
     from nulattice_amd.sectors import SectorHamiltonian
     t0 = time.perf_counter()
-    SH = SectorHamiltonian(fock, v, *PlaneWave(a.L, shell_holes(a.L, a.A), C0, a.D0).labels())
+    SH = SectorHamiltonian.from_planewave(PlaneWave(a.L, shell_holes(a.L, a.A), C0, a.D0))
     torch.cuda.synchronize()
     setup = time.perf_counter() - t0
     s = time_sectors(SH, a.steps, a.warmup, torch)
@@ -115,7 +115,8 @@ def sectors_only(a, fock, v, C0, torch, ccm):
            "dtype": "f64", "amplitudes": SH.layout.n, "mac_per_iter": SH.layout.macs(),
            "ms_per_step_without_graph": s_eager * 1e3, "setup_s": setup}
     if not a.no_converge:
-        e, _, _ = ccm.ccsd_solver(fock, v, eps=1e-10, maxSteps=300, max_diis=0, hamiltonian=SH)
+        e, _, _ = ccm.ccsd_solver(None, None, eps=1e-10, maxSteps=300, max_diis=0,
+                                  hamiltonian=SH)
         out["E_corr"] = e
     return out
 
@@ -244,10 +245,10 @@ def main():
     rng = np.random.default_rng(a.seed)
     C0 = float(rng.uniform(-4.5, -3.5))
     t0 = time.perf_counter()
-    E0, fock, v = planewave_hamiltonian(a.L, a.A, C0, a.D0)
     if a.form == "sectors":
-        print(json.dumps(sectors_only(a, fock, v, C0, torch, ccm)))
+        print(json.dumps(sectors_only(a, C0, torch, ccm)))
         return
+    E0, fock, v = planewave_hamiltonian(a.L, a.A, C0, a.D0)
     H = engine.Hamiltonian(fock, v, device="cuda")
     torch.cuda.synchronize()
     setup_s = time.perf_counter() - t0
@@ -279,7 +280,7 @@ def main():
     from nulattice_amd.planewave import PlaneWave, shell_holes
     from nulattice_amd.sectors import SectorHamiltonian
     t0 = time.perf_counter()
-    SH = SectorHamiltonian(fock, v, *PlaneWave(a.L, shell_holes(a.L, a.A), C0, a.D0).labels())
+    SH = SectorHamiltonian.from_planewave(PlaneWave(a.L, shell_holes(a.L, a.A), C0, a.D0))
     torch.cuda.synchronize()
     sec_setup = time.perf_counter() - t0
     ssec = time_sectors(SH, max(a.steps, 20), a.warmup, torch)

@@ -233,6 +233,18 @@ int nl_sector_dress(int64_t no, const int64_t* cptr, int64_t nchunk, const int64
                     const int* elem, const double* V, const double* T, const double* base,
                     double alpha, double* ws, double* X, void* stream);
 
+/* Sector blocks of the plane-wave SU(4) contact, written on the GPU instead of gathered from
+ * host arrays (the NO2B two-body blocks of coupled_cluster.py:99-263 / 752-832 for the
+ * plane-wave transform of lattice.contacts): block b is out[off[b]..off[b+1]) (off has nblk+1
+ * entries, off[nblk] = total), row-major with ncol[b] columns; rows / columns carry the spin codes
+ * rs[roff[b]+r], cs[coff[b]+c] (4 s_1 + s_2) of their pairs, and
+ *   out = g [d(s_p,s_r) d(s_q,s_s) - d(s_p,s_s) d(s_q,s_r)]
+ * with (p,q) = the row pair, (r,s) = the column pair (crossed = 0), or (p,q) = (column first, row
+ * second), (r,s) = (row first, column second) (crossed = 1: W[(c,k),(b,j)] = <b k||c j>). */
+int nl_contact_fill(int64_t nblk, const int64_t* off, const int64_t* ncol, const int64_t* roff,
+                    const int64_t* coff, const unsigned char* rs, const unsigned char* cs,
+                    int crossed, double g, int64_t total, double* out, void* stream);
+
 /* Fused H2 assembly, Jacobi step, mixing and energy partials on the packed amplitudes
  * (coupled_cluster.py:475-545, 618-620, 623, 265-286): with (a,b,i,j) = orb[0..3][e],
  *   H = V + L + S[sg0] - S[sg1] - S[sg2] + S[sg3] + (X[a] + X[b] + X[p+i] + X[p+j]) t,

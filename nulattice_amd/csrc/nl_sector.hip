@@ -287,6 +287,38 @@ __global__ __launch_bounds__(NT) void sector_dress_sum_kernel(int64_t no,
   X[o] = base[o] + alpha * acc;
 }
 
+// ---------------------------------------------------------------- contact blocks
+// Sector blocks of the SU(4) contact <pq||rs> = g [d(s_p,s_r) d(s_q,s_s) - d(s_p,s_s) d(s_q,s_r)]
+// (momentum conservation holds inside every sector by construction).  Block b occupies
+// out[off[b] .. off[b+1]) as a row-major nrow x ncol matrix; row r carries the spin code
+// rs[roff[b] + r], column c the code cs[coff[b] + c] (code = 4 s_first + s_second).  crossed = 0:
+// bra = the row pair, ket = the column pair; crossed = 1 (W[(c,k),(b,j)] = <b k||c j>): bra =
+// (column first, row second), ket = (row first, column second).
+__global__ __launch_bounds__(NT) void contact_fill_kernel(int64_t nblk, const int64_t* __restrict__ off,
+                                                          const int64_t* __restrict__ ncol,
+                                                          const int64_t* __restrict__ roff,
+                                                          const int64_t* __restrict__ coff,
+                                                          const unsigned char* __restrict__ rs,
+                                                          const unsigned char* __restrict__ cs,
+                                                          int crossed, double g,
+                                                          double* __restrict__ out) {
+  const int64_t total = off[nblk];
+  for (int64_t t = blockIdx.x * (int64_t)NT + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * NT) {
+    int64_t lo = 0, hi = nblk - 1;                  // block b with off[b] <= t < off[b+1]
+    while (lo < hi) {
+      const int64_t mid = (lo + hi + 1) >> 1;
+      if (off[mid] <= t) lo = mid; else hi = mid - 1;
+    }
+    const int64_t local = t - off[lo], nc = ncol[lo];
+    const int r = (int)(local / nc), c = (int)(local % nc);
+    const int rc = rs[roff[lo] + r], cc = cs[coff[lo] + c];
+    const int r1 = rc >> 2, r2 = rc & 3, c1 = cc >> 2, c2 = cc & 3;
+    const int p = crossed ? c1 : r1, q = r2, r_ = crossed ? r1 : c1, s_ = c2;
+    out[t] = g * (double)((p == r_ && q == s_) - (p == s_ && q == r_));
+  }
+}
+
 // ---------------------------------------------------------------- assembly
 struct SectorAsm {
   int64_t n, p;
@@ -390,6 +422,18 @@ int nl_sector_dress(int64_t no, const int64_t* cptr, int64_t nchunk, const int64
                        (hipStream_t)stream, nchunk, cstart, elem, V, T, ws);
   hipLaunchKernelGGL(sector_dress_sum_kernel, dim3((unsigned)g2), dim3(NT), 0,
                      (hipStream_t)stream, no, cptr, ws, base, alpha, X);
+  NL_LAUNCH_CHECK();
+  return NL_OK;
+}
+
+int nl_contact_fill(int64_t nblk, const int64_t* off, const int64_t* ncol, const int64_t* roff,
+                    const int64_t* coff, const unsigned char* rs, const unsigned char* cs,
+                    int crossed, double g, int64_t total, double* out, void* stream) {
+  if (nblk < 0 || total < 0) return NL_EARG;
+  if (nblk == 0 || total == 0) return NL_OK;
+  if (!off || !ncol || !roff || !coff || !rs || !cs || !out) return NL_EARG;
+  hipLaunchKernelGGL(contact_fill_kernel, dim3(grid_of(total, 16384)), dim3(NT), 0,
+                     (hipStream_t)stream, nblk, off, ncol, roff, coff, rs, cs, crossed, g, out);
   NL_LAUNCH_CHECK();
   return NL_OK;
 }

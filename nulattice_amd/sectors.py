@@ -353,6 +353,32 @@ def pack_hamiltonian(lay, fock_mats, two_body_int):
     return {"V": V, "VPP": VPP, "VHH": VHH, "VPH": VPH, "fdiag": fdiag}
 
 
+def contact_block_specs(lay, pw):
+    """Arguments of ``nl_contact_fill`` for the four sector images of a plane-wave contact
+    Hamiltonian: {name: (off (nblk+1), ncol, roff, coff, row codes, column codes, crossed)} for
+    V (pp x hh pair blocks), VPP (pp x pp), VHH (hh x hh) and VPH (crossed ph x ph, <b k||c j> at
+    [(c,k),(b,j)]); a pair's code is 4 s_first + s_second."""
+    sp = pw._orb_s[np.asarray(pw.parts)]
+    sh = pw._orb_s[np.asarray(pw.holes)]
+    p, h = lay.p, lay.h
+    pp, hh, rr = lay.pp_list, lay.hh_list, lay.r_list
+    code = {"pp": (4 * sp[pp // p] + sp[pp % p]).astype(np.uint8),
+            "hh": (4 * sh[hh // h] + sh[hh % h]).astype(np.uint8),
+            "ph": (4 * sp[rr // h] + sh[rr % h]).astype(np.uint8)}
+    xs = np.nonzero(lay.nR_all)[0]
+    nR = lay.nR_all[xs]
+
+    def spec(nr, nc, roff, coff, rk, ck, crossed):
+        off = np.concatenate([[0], np.cumsum(nr * nc)]).astype(np.int64)
+        return (off, np.asarray(nc, np.int64), np.asarray(roff, np.int64),
+                np.asarray(coff, np.int64), code[rk], code[ck], crossed)
+
+    return {"V": spec(lay.nP, lay.nH, lay.pp_start, lay.hh_start, "pp", "hh", 0),
+            "VPP": spec(lay.nP, lay.nP, lay.pp_start, lay.pp_start, "pp", "pp", 0),
+            "VHH": spec(lay.nH, lay.nH, lay.hh_start, lay.hh_start, "hh", "hh", 0),
+            "VPH": spec(nR, nR, lay.r_start[xs], lay.r_start[xs], "ph", "ph", 1)}
+
+
 class SectorHamiltonian:
     """A normal-ordered Hamiltonian in the reference's block format (fock_mats, two_body_int; the
     sparse blocks as COO lists / (indices, values) or dense) packed into symmetry sectors on one
@@ -368,14 +394,42 @@ class SectorHamiltonian:
         _lib.load()
         lay = SectorLayout(part_labels, hole_labels, moduli)
         hst = pack_hamiltonian(lay, fock_mats, two_body_int)
+        t = lambda x: torch.tensor(np.ascontiguousarray(x), dtype=F64, device=device)
+        self._finish(lay, device, t(hst["V"]), t(hst["VPP"]), t(hst["VHH"]), t(hst["VPH"]),
+                     hst["fdiag"])
+
+    @classmethod
+    def from_planewave(cls, pw, device="cuda"):
+        """The sector Hamiltonian of a ``planewave.PlaneWave`` written directly on the GPU: every
+        element of a sector block conserves momentum, so it is the contact's spin factor times
+        the NO2B strength g (``nl_contact_fill``); no dense block or COO list is formed on the
+        host.  Equal to ``SectorHamiltonian(*pw.hamiltonian(), *pw.labels())`` element for
+        element."""
+        device = torch.device(device)
+        if device.type != "cuda":
+            raise _lib.NLError("SectorHamiltonian: a HIP device is required (no CPU fallback)")
+        _lib.load()
+        self = cls.__new__(cls)
+        lay = SectorLayout(*pw.labels())
+        specs = contact_block_specs(lay, pw)
+        out = {}
+        for name, (off, ncol, roff, coff, rcode, ccode, crossed) in specs.items():
+            buf = torch.empty(max(int(off[-1]), 1), dtype=F64, device=device)
+            arrs = [torch.tensor(x, device=device) for x in (off, ncol, roff, coff, rcode, ccode)]
+            _lib.call("nl_contact_fill", len(off) - 1, *(ops._p(x) for x in arrs),
+                      int(crossed), ctypes.c_double(pw.g), int(off[-1]), ops._p(buf),
+                      ops._stream())
+            torch.cuda.current_stream().synchronize()      # before arrs is released
+            out[name] = buf
+        fdiag = np.concatenate([np.diag(pw.f_pp), -np.diag(pw.f_hh)])
+        self._finish(lay, device, out["V"], out["VPP"], out["VHH"], out["VPH"], fdiag)
+        return self
+
+    def _finish(self, lay, device, V, VPP, VHH, VPH, fdiag):
         p, h = lay.p, lay.h
         self.layout, self.p, self.h, self.device = lay, p, h, device
-        V, VPP, VHH, VPH = hst["V"], hst["VPP"], hst["VHH"], hst["VPH"]
         t = lambda x, dt=F64: torch.tensor(np.ascontiguousarray(x), dtype=dt, device=device)
-        self.V = t(V)
-        self.VPP = t(VPP)
-        self.VHH = t(VHH)
-        self.VPHPH = t(VPH)
+        self.V, self.VPP, self.VHH, self.VPHPH = V, VPP, VHH, VPH
         self.orb = t(lay.orb, I32)
         self.sg = t(lay.sg, I32)
         self.ring_map = t(lay.ring_map, I64)
@@ -383,7 +437,7 @@ class SectorHamiltonian:
         self.elem = t(lay.elem, I32)
         self.cptr = t(lay.cptr, I64)
         self.cstart = t(lay.cstart, I64)
-        self.fdiag = hst["fdiag"]
+        self.fdiag = fdiag
         self.base = t(self.fdiag)
         self.VPr = torch.empty(lay.nring, dtype=F64, device=device)
         self._gather(self.ring_map, self.V, self.VPr)

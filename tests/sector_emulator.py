@@ -50,3 +50,20 @@ def step(lay, hst, t, mixing):
     H = H + dsum * t
     tn = mixing * t + (1.0 - mixing) * (t + H / (-dsum))
     return tn, 0.25 * (V * tn).sum(), H
+
+
+def contact_fill(off, ncol, roff, coff, rcode, ccode, crossed, g):
+    """nl_contact_fill: block b = out[off[b]:off[b+1]], row-major, ncol[b] columns."""
+    out = np.zeros(int(off[-1]))
+    for b in range(len(off) - 1):
+        nc = int(ncol[b])
+        if nc == 0:
+            continue
+        nr = (int(off[b + 1]) - int(off[b])) // nc
+        rc = rcode[roff[b]:roff[b] + nr].astype(np.int64)[:, None]
+        cc = ccode[coff[b]:coff[b] + nc].astype(np.int64)[None, :]
+        r1, r2, c1, c2 = rc >> 2, rc & 3, cc >> 2, cc & 3
+        p, q, r, s = (c1, r2, r1, c2) if crossed else (r1, r2, c1, c2)
+        out[off[b]:off[b + 1]] = (g * (((p == r) & (q == s)).astype(float)
+                                      - ((p == s) & (q == r)).astype(float))).reshape(-1)
+    return out

@@ -140,3 +140,16 @@ def test_violations_are_refused():
     lp, lh, mod = pw.labels()
     with pytest.raises(ValueError, match="identify"):
         SectorLayout(lp[:, :3], lh[:, :3], mod[:3])      # momentum alone does not
+
+
+@pytest.mark.parametrize("L,A", [(2, 4), (3, 28)])
+def test_contact_fill_specs_reproduce_the_packed_hamiltonian(L, A):
+    """The direct plane-wave build (nl_contact_fill's statement over contact_block_specs) equals
+    the packing of the generator's dense blocks, element for element."""
+    from nulattice_amd.sectors import contact_block_specs
+    pw, fock, v, lay = _setup(L, A)
+    hst = pack_hamiltonian(lay, fock, v)
+    specs = contact_block_specs(lay, pw)
+    for name in ("V", "VPP", "VHH", "VPH"):
+        got = emu.contact_fill(*specs[name], pw.g)
+        np.testing.assert_array_equal(got, hst[name])
