@@ -1,7 +1,7 @@
 """CPU FP64 oracle for the NuLattice CCSD amplitude iteration.
 
 TEST INFRASTRUCTURE ONLY.  Nothing in ``nulattice_amd`` imports this package.
-It may be imported by ``tests/``, by ``__graft_entry__.smoke()`` (as the checker)
+It may be imported by ``tests/``, by ``__graft_entry__.smoke()`` (to verify its result)
 and by ``bench.py``'s ``cpu_baseline`` leg (timed as the CPU reference), never as
 the thing measured or shipped.
 
