@@ -73,3 +73,46 @@ def test_t1iter_t2iter_match_oracle_and_leave_inputs_alone(sparse):
     new2 = ccm.t2Iter(t1d, t2d, f_ph, f_hh, f_pp, vp, v_phph, v_phhh, v_pphh, vr, v_hhhh, sparse)
     assert close(ref1, host(new1)) and close(ref2, host(new2))
     assert torch.equal(t1d, keep1) and torch.equal(t2d, keep2)
+
+
+def test_coo_known_answers_of_the_reference_tests():
+    """The reference's COO edge cases (tests/CCM/test_ccsd.py:307-428), hand-computed values:
+    a single <cd||ak> element lands where the dense contraction puts it; two elements that meet
+    in one ret0 entry add up (-1.0); ret2 entries of different (a, j) stay in their own places
+    (1.0 and 2.0, total 3.0)."""
+    p, h = 2, 2
+    t2 = np.zeros((p, p, h, h))
+    t2[0, 1, 0, 0] = 1.0
+    t1 = np.zeros((p, h))
+    v_dense = np.zeros((p, p, p, h))
+    v_dense[0, 1, 0, 0] = 1.0
+    r = ccm.v_ppph_dgrams((np.array([[0, 1, 0, 0]]), np.array([1.0])), t1, t2)
+    assert np.allclose(host(r[0]), -0.5 * np.einsum("cdak,cdki->ai", v_dense, t2))
+
+    t2 = np.zeros((p, p, h, h))
+    t2[0, 1, 0, 0] = 1.0
+    t2[1, 0, 1, 0] = 1.0
+    idx = np.array([[0, 1, 0, 0], [1, 0, 0, 1]])
+    r = ccm.v_ppph_dgrams((idx, np.array([1.0, 1.0])), t1, t2)
+    assert float(r[0][0, 0]) == -1.0
+
+    p, h = 4, 4
+    t1 = np.zeros((p, h))
+    t1[0, 0] = 1.0
+    t1[1, 1] = 2.0
+    t2 = np.zeros((p, p, h, h))
+    idx = np.array([[0, 0, 0, 0], [0, 0, 1, 1]])
+    r = ccm.v_ppph_dgrams((idx, np.array([1.0, 1.0])), t1, t2)
+    ret2 = host(r[2])
+    assert ret2[0, 0, 0, 0] == 1.0 and ret2[0, 0, 1, 1] == 2.0 and ret2.sum() == 3.0
+
+
+def test_solver_without_mixing_matches_oracle():
+    """mixing=0 (plain Jacobi), 10 legacy steps, dense blocks (tests/CCM/test_ccsd.py:355-376)."""
+    f_pp, f_ph, f_hh = FOCK
+    href, hgpu = [], []
+    cc.ccsd_solver(FOCK, DENSE, maxSteps=10, max_diis=0, mixing=0.0, sparse=False,
+                   history=href)
+    ccm.ccsd_solver(FOCK, DENSE, maxSteps=10, max_diis=0, mixing=0.0, sparse=False,
+                    history=hgpu)
+    np.testing.assert_allclose(hgpu, href, rtol=1e-12)
