@@ -218,9 +218,12 @@ int nl_gemm_grouped_f64(int mode, int64_t ntile, const int64_t* desc, const int*
                         double alpha, const double* A, const double* B, double beta,
                         const double* D, double* C, void* stream);
 
-/* out[e] = idx[e] >= 0 ? in[idx[e]] : 0 (crossed-channel image of the packed t2, and the packed
- * form of reference-layout tensors); scatter: out[idx[e]] = in[e] for idx[e] >= 0 (packed t2 back
- * to the reference layout t2[a,b,i,j] that ccsd_solver returns, coupled_cluster.py:687). */
+/* Signed index maps (m >= 0: +x at m; m <= -2: -x at -m-2; m = -1: zero / nothing):
+ * gather out[e] = in[idx[e]] with that sign (crossed-channel image of the packed t2, whose
+ * stored elements are a < b, i < j; packed form of reference-layout tensors); scatter
+ * out[idx[e]] = in[e] with that sign (the packed t2 back to the reference layout t2[a,b,i,j]
+ * that ccsd_solver returns, coupled_cluster.py:687: four calls for the four antisymmetric
+ * images). */
 int nl_sector_gather(int64_t n, const int64_t* idx, const double* in, double* out, void* stream);
 int nl_sector_scatter(int64_t n, const int64_t* idx, const double* in, double* out, void* stream);
 
@@ -249,11 +252,12 @@ int nl_contact_fill(int64_t nblk, const int64_t* off, const int64_t* ncol, const
  * (coupled_cluster.py:475-545, 618-620, 623, 265-286): with (a,b,i,j) = orb[0..3][e],
  *   H = V + L + S[sg0] - S[sg1] - S[sg2] + S[sg3] + (X[a] + X[b] + X[p+i] + X[p+j]) t,
  *   tn = mixing t + (1 - mixing) (t + H / -(X[a] + X[b] + X[p+i] + X[p+j])),
- * part[0:g] = partial sums of 1/4 V tn, part[g:2g] = of H^2, g = nl_partials(n).  L, S (then sg),
- * tn and H2out may be null. */
+ * part[0:g] = partial sums of escale V tn (1 when each antisymmetric element is stored once),
+ * part[g:2g] = of H^2, g = nl_partials(n).  L, S (then sg), tn and H2out may be null. */
 int nl_sector_assemble(int64_t n, int64_t p, const int* orb, const int* sg, const double* V,
                        const double* L, const double* S, const double* X, const double* t,
-                       double mixing, double* tn, double* H2out, double* part, void* stream);
+                       double mixing, double escale, double* tn, double* H2out, double* part,
+                       void* stream);
 
 #ifdef __cplusplus
 }

@@ -238,7 +238,7 @@ __global__ __launch_bounds__(NT) void sector_gather_kernel(int64_t n, const int6
                                                            double* __restrict__ out) {
   for (int64_t t = blockIdx.x * (int64_t)NT + threadIdx.x; t < n; t += (int64_t)gridDim.x * NT) {
     const int64_t s = idx[t];
-    out[t] = s >= 0 ? in[s] : 0.0;
+    out[t] = s >= 0 ? in[s] : (s <= -2 ? -in[-s - 2] : 0.0);
   }
 }
 
@@ -249,6 +249,7 @@ __global__ __launch_bounds__(NT) void sector_scatter_kernel(int64_t n,
   for (int64_t t = blockIdx.x * (int64_t)NT + threadIdx.x; t < n; t += (int64_t)gridDim.x * NT) {
     const int64_t s = idx[t];
     if (s >= 0) out[s] = in[t];
+    else if (s <= -2) out[-s - 2] = -in[t];
   }
 }
 
@@ -326,7 +327,7 @@ struct SectorAsm {
   const int* sg;         // [4][n]: ring-image offsets of S[a,i,b,j], S[a,j,b,i], S[b,i,a,j],
                          //         S[b,j,a,i]; null = no ring term
   const double *V, *L, *S, *X, *t;
-  double mixing;
+  double mixing, escale;
   double *tn, *H2out, *part;
 };
 
@@ -347,7 +348,7 @@ __global__ __launch_bounds__(NT) void sector_assemble_kernel(SectorAsm g) {
     const double tn = g.mixing * told + (1.0 - g.mixing) * nt;
     if (g.tn) g.tn[e] = tn;
     if (g.H2out) g.H2out[e] = H;
-    e_acc += 0.25 * v * tn;
+    e_acc += g.escale * v * tn;
     r_acc += H * H;
   }
   const double es = block_sum(e_acc);
@@ -440,9 +441,10 @@ int nl_contact_fill(int64_t nblk, const int64_t* off, const int64_t* ncol, const
 
 int nl_sector_assemble(int64_t n, int64_t p, const int* orb, const int* sg, const double* V,
                        const double* L, const double* S, const double* X, const double* t,
-                       double mixing, double* tn, double* H2out, double* part, void* stream) {
+                       double mixing, double escale, double* tn, double* H2out, double* part,
+                       void* stream) {
   if (n < 0 || p < 0 || !orb || !V || !X || !t || !part || (S && !sg)) return NL_EARG;
-  SectorAsm g{n, p, orb, sg, V, L, S, X, t, mixing, tn, H2out, part};
+  SectorAsm g{n, p, orb, sg, V, L, S, X, t, mixing, escale, tn, H2out, part};
   hipLaunchKernelGGL(sector_assemble_kernel, dim3((unsigned)nl_partials(n)), dim3(NT), 0,
                      (hipStream_t)stream, g);
   NL_LAUNCH_CHECK();

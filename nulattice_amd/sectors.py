@@ -5,8 +5,9 @@ waves the momentum k mod L and the spin and isospin projections (``PlaneWave.lab
 normal-ordered Hamiltonian conserves the summed labels of each pair and:
 
 * t2[a,b,i,j] is nonzero only in the *pair sector* l_a + l_b = l_i + l_j; the packed amplitudes
-  hold, sector after sector, the dense [(a,b)][(i,j)] block over the ordered pairs a != b,
-  i != j of that sector (L=5 A=28: 200,376 amplitudes instead of the 1.75e8 of the dense t2);
+  hold, sector after sector, the dense [(a,b)][(i,j)] block over the pairs a < b, i < j of that
+  sector, each antisymmetric element once (L=5 A=28: 49,268 amplitudes instead of the 1.75e8
+  of the dense t2);
 * the ring products of t2Iter are block diagonal in the *crossed sector* l_a - l_i;
 * the dressings X_pp, X_hh and the Fock matrices are diagonal, and t1 stays exactly zero: every
   term of the T1 residual (t1Iter, coupled_cluster.py:342-414) needs l_a = l_i, which labels that
@@ -17,6 +18,11 @@ With t1 = 0 the T2 residual of t2Iter (coupled_cluster.py:475-545) is
        + t2 (1/2 v_hhhh + 1/4 v_pphh^T t2)          (A21 :240-251, A25 :292-303)
        + P(ij)P(ab) t2 (-v_phph + 1/2 v_pphh t2)   (A22 :253-264, A24 :279-290; crossed channel)
        + (X_aa + X_bb + X_ii + X_jj) t2             (the X_pp / X_hh terms, coupled_cluster.py:536)
+with every sum over an ordered pair (c,d) or (k,l) equal to twice the sum over c < d or k < l, so
+on the stored elements the ladders are v_pppp t2 + t2 (v_hhhh + v_pphh^T t2) over a < b, i < j
+blocks and the dressings X_aa = f_aa - sum over stored elements containing a of v t2 (and
+X_ii likewise); the ring keeps ordered crossed pairs, read from the packed amplitudes through
+signed maps.
 and every product is a set of small dense GEMMs, one per sector, run by grouped-GEMM launches
 (one per kernel shape: short K in one pass, long K split over the wavefronts of a workgroup,
 64x8 tiles for the narrow sectors).  An iteration is the dressing, the ladder and ring products,
@@ -116,10 +122,11 @@ class SectorLayout:
     particles with labels ``part_labels`` (p, d) and holes with ``hole_labels`` (h, d); label
     component c is taken modulo ``moduli[c]`` (0 = no modulus).
 
-    Attributes: n (packed amplitudes), orb (4, n) = (a, b, i, j) of each packed element,
-    dense_index (n,) = ((a*p + b)*h + i)*h + j, ring_map (ring image -> packed element or -1),
+    Attributes: n (packed amplitudes, a < b and i < j), orb (4, n) = (a, b, i, j) of each packed
+    element, dense_index (n,) = ((a*p + b)*h + i)*h + j, image_maps (the four signed reference-
+    layout images), ring_map (ring image -> signed packed element),
     sg (4, n) ring-image offsets of Spp[a,i,b,j], Spp[a,j,b,i], Spp[b,i,a,j], Spp[b,j,a,i],
-    seg / elem (dressing segments: elements by a, then by i), and the problem lists
+    seg / elem (dressing segments: elements by a and b, then by i and j), and the problem lists
     ``desc_pp``, ``desc_whh``, ``desc_hh``, ``desc_w``, ``desc_s``.
     """
 
@@ -145,14 +152,16 @@ class SectorLayout:
         self.p, self.h, self.d = p, h, d
         self.labels_p, self.labels_h, self.moduli = lp, lh, moduli
 
-        # ---------------- pair sectors (l_a + l_b = l_i + l_j), ordered pairs a != b, i != j
+        # ---------------- pair sectors (l_a + l_b = l_i + l_j), pairs a < b, i < j
         pl = _norm(lp[:, None, :] + lp[None, :, :], moduli).reshape(p * p, d)
         hl = _norm(lh[:, None, :] + lh[None, :, :], moduli).reshape(h * h, d)
         u, inv = np.unique(np.concatenate([pl, hl]).reshape(-1, d), axis=0, return_inverse=True)
         inv = inv.reshape(-1)
         kp, kh = inv[:p * p].copy(), inv[p * p:].copy()
-        kp[np.arange(p * p) % (p + 1) == 0] = -1          # a == b
-        kh[np.arange(h * h) % (h + 1) == 0] = -1          # i == j
+        pa = np.arange(p * p)
+        kp[pa // p >= pa % p] = -1                       # keep a < b
+        ha = np.arange(h * h)
+        kh[ha // h >= ha % h] = -1                       # keep i < j
         present = np.intersect1d(kp[kp >= 0], kh[kh >= 0])
         S = present.size
         sec_of = np.full(len(u), -1, np.int64)
@@ -175,6 +184,11 @@ class SectorLayout:
         a, b, i, j = ab // p, ab % p, ij // h, ij % h
         self.orb = np.stack([a, b, i, j]).astype(np.int32)
         self.dense_index = ((a * p + b) * h + i) * h + j
+        # the four antisymmetric images of each stored element in the reference layout, as
+        # signed maps (m >= 0: +x at m; m <= -2: -x at -m-2)
+        dn = lambda w, x, y, z: ((w * p + x) * h + y) * h + z
+        self.image_maps = np.stack([dn(a, b, i, j), -dn(b, a, i, j) - 2, -dn(a, b, j, i) - 2,
+                                    dn(b, a, j, i)]).astype(np.int64)
 
         # ---------------- crossed sectors (l_a - l_i)
         ql = _norm(lp[:, None, :] - lh[None, :, :], moduli).reshape(p * h, d)
@@ -206,7 +220,7 @@ class SectorLayout:
         ai = self.r_list[self.r_start[xs[blk]] + r]
         ck = self.r_list[self.r_start[nx[blk]] + c]
         ra, ri, rc, rk = ai // h, ai % h, ck // h, ck % h
-        self.ring_map = self.element(ra, rc, ri, rk)
+        self.ring_map = self.signed_element(ra, rc, ri, rk)
         # W image of v_phph: W[(c,k),(b,j)] = v_phph[b,k,c,j]
         blk, r, c = _outer(nRx, nRx)
         ck = self.r_list[self.r_start[xs[blk]] + r]
@@ -220,12 +234,13 @@ class SectorLayout:
         self.sg = np.stack([ro(a, i, b, j), ro(a, j, b, i), ro(b, i, a, j),
                             ro(b, j, a, i)]).astype(np.int32)
 
-        # ---------------- dressing segments: elements by a (X_pp), then by i (X_hh)
-        _, cnt_a, ord_a, _ = _group(a.astype(np.int64), p)
-        _, cnt_i, ord_i, _ = _group(i.astype(np.int64), h)
+        # ---------------- dressing segments: elements by a and by b (X_pp), by i and by j (X_hh)
+        both = np.concatenate([np.arange(self.n), np.arange(self.n)])
+        _, cnt_a, ord_a, _ = _group(np.concatenate([a, b]).astype(np.int64), p)
+        _, cnt_i, ord_i, _ = _group(np.concatenate([i, j]).astype(np.int64), h)
         self.seg = np.concatenate([[0], np.cumsum(np.concatenate([cnt_a, cnt_i]))]).astype(
             np.int64)
-        self.elem = np.concatenate([ord_a, ord_i]).astype(np.int32)
+        self.elem = np.concatenate([both[ord_a], both[ord_i]]).astype(np.int32)
         # chunks of at most DRESS_CHUNK consecutive entries, never crossing a segment
         nch = -(-np.diff(self.seg) // self.DRESS_CHUNK)
         self.cptr = np.concatenate([[0], np.cumsum(nch)]).astype(np.int64)
@@ -257,7 +272,8 @@ class SectorLayout:
             setattr(self, name.replace("desc", "launch"), _launch_lists(dsc))
 
     def element(self, a, b, i, j):
-        """Packed index of t2[a,b,i,j] (-1 where the element lies outside every sector)."""
+        """Packed index of t2[a,b,i,j] for a < b, i < j (-1 for other orderings and where the
+        element lies outside every sector)."""
         p, h = self.p, self.h
         s1 = self.sp[a * p + b]
         s2 = self.sh[i * h + j]
@@ -266,14 +282,24 @@ class SectorLayout:
         e = self.toff[s] + self.pp_pos[a * p + b] * self.nH[s] + self.hh_pos[i * h + j]
         return np.where(ok, e, -1).astype(np.int64)
 
+    def signed_element(self, a, b, i, j):
+        """t2[a,b,i,j] in any index order as a signed packed reference: e (the value is +T[e]),
+        -e-2 (-T[e]) or -1 (zero: a == b, i == j or outside every sector)."""
+        a, b, i, j = (np.asarray(x, np.int64) for x in (a, b, i, j))
+        sg = np.where(a > b, -1, 1) * np.where(i > j, -1, 1)
+        e = self.element(np.minimum(a, b), np.maximum(a, b), np.minimum(i, j), np.maximum(i, j))
+        e = np.where((a == b) | (i == j), -1, e)
+        return np.where(e < 0, -1, np.where(sg > 0, e, -e - 2)).astype(np.int64)
+
     def pack(self, t2):
-        """Packed form of a reference-layout t2[a,b,i,j] (NumPy)."""
+        """Packed form (a < b, i < j) of a reference-layout t2[a,b,i,j] (NumPy)."""
         return np.asarray(t2, np.float64).reshape(-1)[self.dense_index]
 
     def unpack(self, x):
-        """Reference-layout t2[a,b,i,j] of packed amplitudes (NumPy)."""
+        """Reference-layout antisymmetric t2[a,b,i,j] of packed amplitudes (NumPy)."""
         out = np.zeros(self.p * self.p * self.h * self.h)
-        out[self.dense_index] = x
+        for m in self.image_maps:
+            out[np.where(m >= 0, m, -m - 2)] = np.where(m >= 0, x, -x)
         return out.reshape(self.p, self.p, self.h, self.h)
 
     def macs(self):
@@ -297,10 +323,10 @@ def _conserves(labels_of, idx, val, moduli, sign=(1, 1, -1, -1)):
 
 def pack_hamiltonian(lay, fock_mats, two_body_int):
     """Host (NumPy) sector images of a reference-format Hamiltonian: dict with V (packed
-    v_pphh), VPP (v_pppp sector blocks [(a,b)][(c,d)]), VHH (v_hhhh sector blocks
-    [(k,l)][(i,j)]), VPH (W_x[(c,k),(b,j)] = v_phph[b,k,c,j]) and fdiag = (f_aa, -f_ii).
-    Raises SectorViolation if any nonzero element does not conserve the labels, if a Fock matrix
-    is not diagonal or if f_ph is not zero."""
+    v_pphh, a < b, i < j), VPP (v_pppp sector blocks [(a<b)][(c<d)]), VHH (v_hhhh sector blocks
+    [(k<l)][(i<j)]), VPH (W_x[(c,k),(b,j)] = v_phph[b,k,c,j]) and fdiag = (f_aa, -f_ii).
+    Raises SectorViolation if any nonzero element does not conserve the labels, if a stored block
+    is not antisymmetric, if a Fock matrix is not diagonal or if f_ph is not zero."""
     f_pp, f_ph, f_hh = (np.asarray(_np(x), np.float64) for x in fock_mats)
     v_pppp, v_ppph, v_pphh, v_phph, v_phhh, v_hhhh = two_body_int
     p, h = f_ph.shape
@@ -317,7 +343,12 @@ def pack_hamiltonian(lay, fock_mats, two_body_int):
     v_phhh = np.asarray(_np(v_phhh), np.float64)
     v_hhhh = np.asarray(_np(v_hhhh), np.float64)
     V = lay.pack(v_pphh)
-    if np.count_nonzero(V) != np.count_nonzero(v_pphh):
+    flat = v_pphh.reshape(-1)
+    for m in lay.image_maps[1:]:
+        if not np.array_equal(np.where(m >= 0, flat[np.maximum(m, 0)], -flat[-np.minimum(m, -2) - 2]),
+                              V):
+            raise SectorViolation("v_pphh is not antisymmetric")
+    if 4 * np.count_nonzero(V) != np.count_nonzero(v_pphh):
         raise SectorViolation("v_pphh has nonzero elements outside the pair sectors")
     VPH = v_phph.reshape(-1)[lay.phph_index]
     if np.count_nonzero(VPH) != np.count_nonzero(v_phph):
@@ -326,7 +357,11 @@ def pack_hamiltonian(lay, fock_mats, two_body_int):
     kl = lay.hh_list[lay.hh_start[blk] + r]
     ij = lay.hh_list[lay.hh_start[blk] + c]
     VHH = v_hhhh.reshape(-1)[kl * h * h + ij]
-    if np.count_nonzero(VHH) != np.count_nonzero(v_hhhh):
+    k_, l_, i_, j_ = kl // h, kl % h, ij // h, ij % h
+    for x1, x2, y1, y2, sg in ((l_, k_, i_, j_, -1), (k_, l_, j_, i_, -1), (l_, k_, j_, i_, 1)):
+        if not np.array_equal(sg * v_hhhh[x1, x2, y1, y2], VHH):
+            raise SectorViolation("v_hhhh is not antisymmetric")
+    if 4 * np.count_nonzero(VHH) != np.count_nonzero(v_hhhh):
         raise SectorViolation("v_hhhh has nonzero elements outside the pair sectors")
     lpf = lambda x: lay.labels_p[x]
     lhf = lambda x: lay.labels_h[x]
@@ -340,15 +375,26 @@ def pack_hamiltonian(lay, fock_mats, two_body_int):
     idx, val = coo_arrays(v_pppp, p, h, "pppp")
     if idx.size and not _conserves([lpf, lpf, lpf, lpf], idx, val, lay.moduli).all():
         raise SectorViolation("v_pppp does not conserve the labels")
-    rows = idx[:, 0] * p + idx[:, 1]
-    cols = idx[:, 2] * p + idx[:, 3]
     if (((idx[:, 0] == idx[:, 1]) | (idx[:, 2] == idx[:, 3])) & (val != 0)).any():
         raise SectorViolation("v_pppp is not antisymmetric (nonzero <aa||cd> or <ab||cc>)")
+    # each entry lands, with the sign of its reordering, on the stored [(a<b)][(c<d)] element;
+    # a list holding all four images of an element (the reference's lists do) must give the
+    # same block from each image class, a list of a < b, c < d entries alone gives it directly
+    a0, b0 = np.minimum(idx[:, 0], idx[:, 1]), np.maximum(idx[:, 0], idx[:, 1])
+    c0, d0 = np.minimum(idx[:, 2], idx[:, 3]), np.maximum(idx[:, 2], idx[:, 3])
+    sgn = np.where(idx[:, 0] > idx[:, 1], -1.0, 1.0) * np.where(idx[:, 2] > idx[:, 3], -1.0, 1.0)
+    cls = (idx[:, 0] > idx[:, 1]).astype(np.int64) * 2 + (idx[:, 2] > idx[:, 3])
+    rows, cols = a0 * p + b0, c0 * p + d0
     s1, s2 = lay.sp[rows], lay.sp[cols]
     use = (s1 >= 0) & (s1 == s2)
     s = s1[use]
     flat = lay.ppoff[s] + lay.pp_pos[rows[use]] * lay.nP[s] + lay.pp_pos[cols[use]]
-    VPP = np.bincount(flat, weights=val[use], minlength=lay.npp)
+    part = [np.bincount(flat[cls[use] == k], weights=(sgn * val)[use][cls[use] == k],
+                        minlength=lay.npp) for k in range(4)]
+    VPP = part[0]
+    present = [k for k in (1, 2, 3) if np.any(cls[use] == k)]
+    if present and not all(np.array_equal(part[k], VPP) for k in (1, 2, 3)):
+        raise SectorViolation("v_pppp is not antisymmetric")
     fdiag = np.concatenate([np.diag(f_pp), -np.diag(f_hh)])
     return {"V": V, "VPP": VPP, "VHH": VHH, "VPH": VPH, "fdiag": fdiag}
 
@@ -434,6 +480,7 @@ class SectorHamiltonian:
         self.sg = t(lay.sg, I32)
         self.ring_map = t(lay.ring_map, I64)
         self.dense_index = t(lay.dense_index, I64)
+        self.image_maps = [t(m, I64) for m in lay.image_maps]
         self.elem = t(lay.elem, I32)
         self.cptr = t(lay.cptr, I64)
         self.cstart = t(lay.cstart, I64)
@@ -477,8 +524,8 @@ class SectorHamiltonian:
         zero = torch.zeros(max(lay.n, 1), dtype=F64, device=self.device)
         t2 = torch.empty(max(lay.n, 1), dtype=F64, device=self.device)
         _lib.call("nl_sector_assemble", lay.n, self.p, ops._p(self.orb), None, ops._p(self.V),
-                  None, None, ops._p(Bf["X"]), ops._p(zero), ctypes.c_double(0.0), ops._p(t2),
-                  None, ops._p(Bf["part"]), ops._stream())
+                  None, None, ops._p(Bf["X"]), ops._p(zero), ctypes.c_double(0.0),
+                  ctypes.c_double(1.0), ops._p(t2), None, ops._p(Bf["part"]), ops._stream())
         self._energy(Bf)
         return t2
 
@@ -487,12 +534,13 @@ class SectorHamiltonian:
                   ops._stream())
 
     def energy(self, t2, Bf):
-        """E = 1/4 sum v_pphh t2 (ccsd_energy with t1 = 0, coupled_cluster.py:265-286)."""
+        """E = 1/4 sum v_pphh t2 over all orderings = sum over the stored a < b, i < j elements
+        (ccsd_energy with t1 = 0, coupled_cluster.py:265-286)."""
         _lib.call("nl_dot_part", self.layout.n, ops._p(self.V), ops._p(t2), ops._p(Bf["part"]),
                   ops._stream())
         _lib.call("nl_finalize", self.nparts, ops._p(Bf["part"]), 0, None, None, ops._p(Bf["E"]),
                   ops._stream())
-        return 0.25 * float(Bf["E"].item())
+        return float(Bf["E"].item())
 
     def residual(self, t2, Bf):
         """Everything of H2 except v_pphh, the X t2 terms and the antisymmetrisers: Bf["L"]
@@ -502,43 +550,60 @@ class SectorHamiltonian:
         crossed-channel gather and the two ring products -- run on the caller's stream and two
         side streams (default priority), joined before returning, so the small launches overlap
         instead of queueing behind each other's tails."""
-        lay = self.layout
         main = torch.cuda.current_stream()
         sa, sb = self._side
         sa.wait_stream(main)
         sb.wait_stream(main)
         with torch.cuda.stream(sa):
-            self._gemm("whh", 0.25, self.V, t2, 0.5, Bf["Whh"], self.VHH)
+            self.chain_whh(t2, Bf)
         with torch.cuda.stream(sb):
-            self._gather(self.ring_map, t2, Bf["Tr"])
-            self._gemm("w", 0.5, self.VPr, Bf["Tr"], -1.0, Bf["W"], self.VPHPH)
-            self._gemm("s", 1.0, Bf["Tr"], Bf["W"], 0.0, Bf["S"])
-        _lib.call("nl_sector_dress", self.p + self.h, ops._p(self.cptr), len(lay.cstart) - 1,
-                  ops._p(self.cstart), ops._p(self.elem), ops._p(self.V), ops._p(t2),
-                  ops._p(self.base), ctypes.c_double(-0.5), ops._p(Bf["Xc"]), ops._p(Bf["X"]),
-                  ops._stream())
-        self._gemm("pp", 0.5, self.VPP, t2, 0.0, Bf["L"])
+            self.chain_ring(t2, Bf)
+        self.chain_pp(t2, Bf)
         main.wait_stream(sa)
         self._gemm("hh", 1.0, t2, Bf["Whh"], 1.0, Bf["L"])
         main.wait_stream(sb)
-        return lay
+        return self.layout
+
+    # the independent chains of one step
+    def chain_pp(self, t2, Bf):
+        """Dressed diagonals X and the pp ladder L = v_pppp t2."""
+        lay = self.layout
+        _lib.call("nl_sector_dress", self.p + self.h, ops._p(self.cptr), len(lay.cstart) - 1,
+                  ops._p(self.cstart), ops._p(self.elem), ops._p(self.V), ops._p(t2),
+                  ops._p(self.base), ctypes.c_double(-1.0), ops._p(Bf["Xc"]), ops._p(Bf["X"]),
+                  ops._stream())
+        self._gemm("pp", 1.0, self.VPP, t2, 0.0, Bf["L"])
+
+    def chain_whh(self, t2, Bf):
+        """Whh = v_hhhh + v_pphh^T t2."""
+        self._gemm("whh", 1.0, self.V, t2, 1.0, Bf["Whh"], self.VHH)
+
+    def chain_ring(self, t2, Bf):
+        """Crossed-channel image of t2, W = 1/2 v_pphh t2 - v_phph, Spp = t2 W."""
+        self._gather(self.ring_map, t2, Bf["Tr"])
+        self._gemm("w", 0.5, self.VPr, Bf["Tr"], -1.0, Bf["W"], self.VPHPH)
+        self._gemm("s", 1.0, Bf["Tr"], Bf["W"], 0.0, Bf["S"])
 
     def update(self, t2, t2n, Bf, mixing, H2out=None):
         """One damped Jacobi step t2n = mixing t2 + (1 - mixing) t2Iter(t2) (coupled_cluster.py:
         618-620) and its energy in Bf["E"]."""
         self.residual(t2, Bf)
+        self._assemble(t2, t2n, Bf, mixing, H2out)
+
+    def _assemble(self, t2, t2n, Bf, mixing, H2out=None):
         lay = self.layout
         _lib.call("nl_sector_assemble", lay.n, self.p, ops._p(self.orb), ops._p(self.sg),
                   ops._p(self.V), ops._p(Bf["L"]), ops._p(Bf["S"]), ops._p(Bf["X"]), ops._p(t2),
-                  ctypes.c_double(mixing), ops._p(t2n), ops._p(H2out), ops._p(Bf["part"]),
-                  ops._stream())
+                  ctypes.c_double(mixing), ctypes.c_double(1.0), ops._p(t2n), ops._p(H2out),
+                  ops._p(Bf["part"]), ops._stream())
         self._energy(Bf)
 
     def unpack(self, t2):
         """Reference-layout t2[a,b,i,j] (p, p, h, h) HIP tensor of packed amplitudes."""
         out = torch.zeros((self.p, self.p, self.h, self.h), dtype=F64, device=self.device)
-        _lib.call("nl_sector_scatter", self.layout.n, ops._p(self.dense_index), ops._p(t2),
-                  ops._p(out), ops._stream())
+        for m in self.image_maps:
+            _lib.call("nl_sector_scatter", self.layout.n, ops._p(m), ops._p(t2), ops._p(out),
+                      ops._stream())
         return out
 
     def pack(self, t2):
@@ -556,7 +621,9 @@ class SectorStepper:
     direction of the step (buffer 0 -> 1 and 1 -> 0) is captured once as a HIP graph
     (torch.cuda.CUDAGraph around the library's launches, which allocate nothing and record no
     memset; the side-stream fork and join of ``residual`` become graph edges), so a step is one
-    graph launch plus the energy read-back instead of thirteen kernel launches from Python."""
+    graph launch plus the energy read-back instead of thirteen kernel launches from Python.
+    (Replaying the three independent chains as separate graphs on three streams measured
+    slower: the launches then wait on the host instead of on each other.)"""
 
     def __init__(self, SH, Bf, t_init, mixing, graph=True):
         self.SH, self.Bf, self.mixing = SH, Bf, mixing

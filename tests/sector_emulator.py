@@ -19,7 +19,10 @@ def gemm(desc, alpha, A, B, beta, C, D=None):
 
 
 def gather(idx, src):
-    return np.where(idx >= 0, src[np.maximum(idx, 0)], 0.0)
+    """Signed map: m >= 0 -> +src[m], m <= -2 -> -src[-m-2], m = -1 -> 0."""
+    pos = src[np.maximum(idx, 0)]
+    neg = -src[np.maximum(-idx - 2, 0)]
+    return np.where(idx >= 0, pos, np.where(idx <= -2, neg, 0.0))
 
 
 def dress(seg, elem, V, T, base, alpha):
@@ -31,11 +34,11 @@ def dress(seg, elem, V, T, base, alpha):
 def step(lay, hst, t, mixing):
     """One packed Jacobi step: (t_new, energy, H2)."""
     V, n = hst["V"], lay.n
-    X = dress(lay.seg, lay.elem, V, t, hst["fdiag"], -0.5)
+    X = dress(lay.seg, lay.elem, V, t, hst["fdiag"], -1.0)
     L = np.zeros(n)
     Whh = np.zeros(lay.nhh)
-    gemm(lay.desc_pp, 0.5, hst["VPP"], t, 0.0, L)
-    gemm(lay.desc_whh, 0.25, V, t, 0.5, Whh, hst["VHH"])
+    gemm(lay.desc_pp, 1.0, hst["VPP"], t, 0.0, L)
+    gemm(lay.desc_whh, 1.0, V, t, 1.0, Whh, hst["VHH"])
     gemm(lay.desc_hh, 1.0, t, Whh, 1.0, L)
     Tr = gather(lay.ring_map, t)
     VPr = gather(lay.ring_map, V)
@@ -49,7 +52,7 @@ def step(lay, hst, t, mixing):
     dsum = X[a] + X[b] + X[p + i] + X[p + j]
     H = H + dsum * t
     tn = mixing * t + (1.0 - mixing) * (t + H / (-dsum))
-    return tn, 0.25 * (V * tn).sum(), H
+    return tn, (V * tn).sum(), H
 
 
 def contact_fill(off, ncol, roff, coff, rcode, ccode, crossed, g):

@@ -31,15 +31,15 @@ def test_layout_is_a_bijection_onto_the_sectors(L, A):
     pw, fock, v, lay = _setup(L, A)
     p, h = lay.p, lay.h
     assert np.unique(lay.dense_index).size == lay.n
-    # the sectors hold exactly the elements allowed by the labels (a != b, i != j)
+    # the sectors hold exactly the elements allowed by the labels (a < b, i < j)
     lp, lh, mod = pw.labels()
     tot = (lp[:, None, None, None] + lp[None, :, None, None] - lh[None, None, :, None]
            - lh[None, None, None, :])
     tot[..., :3] %= L
     allowed = ~np.any(tot != 0, axis=-1)
     ar, hr = np.arange(p), np.arange(h)
-    allowed &= (ar[:, None] != ar[None, :])[:, :, None, None]
-    allowed &= (hr[:, None] != hr[None, :])[None, None, :, :]
+    allowed &= (ar[:, None] < ar[None, :])[:, :, None, None]
+    allowed &= (hr[:, None] < hr[None, :])[None, None, :, :]
     assert allowed.sum() == lay.n
     assert allowed.reshape(-1)[lay.dense_index].all()
     # element() inverts dense_index
@@ -67,7 +67,19 @@ def test_layout_is_a_bijection_onto_the_sectors(L, A):
         assert covered == int((dsc[:, 0] * dsc[:, 1]).sum())
     # the t2 of the plane-wave Hamiltonian lies in the sectors
     v_pphh = v[2]
-    assert np.count_nonzero(v_pphh.reshape(-1)[lay.dense_index]) == np.count_nonzero(v_pphh)
+    assert 4 * np.count_nonzero(v_pphh.reshape(-1)[lay.dense_index]) == np.count_nonzero(v_pphh)
+    np.testing.assert_array_equal(lay.unpack(lay.pack(v_pphh)), v_pphh)
+    # signed_element agrees with the reference-layout images
+    x = np.random.default_rng(1).standard_normal(lay.n)
+    T = lay.unpack(x)
+    e = lay.signed_element(a, b, i, j)
+    np.testing.assert_array_equal(np.where(e >= 0, x[np.maximum(e, 0)],
+                                           np.where(e <= -2, -x[np.maximum(-e - 2, 0)], 0.0)),
+                                  T[a, b, i, j])
+    e = lay.signed_element(b, a, i, j)
+    np.testing.assert_array_equal(np.where(e >= 0, x[np.maximum(e, 0)],
+                                           np.where(e <= -2, -x[np.maximum(-e - 2, 0)], 0.0)),
+                                  T[b, a, i, j])
 
 
 @pytest.mark.parametrize("L,A,seed", [(2, 4, 0), (3, 4, 1), (3, 28, 2)])
@@ -86,7 +98,7 @@ def test_sector_step_matches_oracle_dense_step(L, A, seed):
     new2 = cc.t2Iter(t1, t2, f_ph, f_hh, f_pp, v_pppp, v_phph, v_phhh, v_pphh, v_ppph, v_hhhh,
                      sparse=False)
     ref = 0.5 * t2 + 0.5 * new2
-    assert np.count_nonzero(ref.reshape(-1)[lay.dense_index]) == np.count_nonzero(ref)
+    np.testing.assert_array_equal(lay.unpack(lay.pack(ref)), ref)    # antisymmetric, in sectors
     tn, e, _ = emu.step(lay, hst, lay.pack(t2), 0.5)
     np.testing.assert_allclose(tn, lay.pack(ref), rtol=1e-12, atol=1e-15 * np.abs(ref).max())
     assert e == pytest.approx(cc.ccsd_energy(f_ph, v_pphh, ref, t1), rel=1e-12)
@@ -102,7 +114,7 @@ def test_sector_solve_matches_oracle_trajectory():
     f = hst["fdiag"]
     a, b, i, j = lay.orb
     t = hst["V"] / (-(f[a] + f[b] + f[lay.p + i] + f[lay.p + j]))
-    hist = [0.25 * (hst["V"] * t).sum()]
+    hist = [(hst["V"] * t).sum()]
     for _ in range(6):
         t, e, _ = emu.step(lay, hst, t, 0.5)
         hist.append(e)
@@ -115,7 +127,7 @@ def test_violations_are_refused():
     a, b, i, j = (x.reshape(-1) for x in np.meshgrid(*(np.arange(n) for n in
                                                        (lay.p, lay.p, lay.h, lay.h)),
                                                      indexing="ij"))
-    out = np.nonzero((lay.element(a, b, i, j) < 0) & (a != b) & (i != j))[0][0]
+    out = np.nonzero((lay.element(a, b, i, j) < 0) & (a < b) & (i < j))[0][0]
     bad = [x.copy() for x in v]
     bad[2][a[out], b[out], i[out], j[out]] = 1e-3
     with pytest.raises(SectorViolation, match="v_pphh"):

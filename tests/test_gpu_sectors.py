@@ -142,7 +142,7 @@ def test_sector_solver_L5_A28_against_dense_fp64():
                                 sectors=pw.labels(), info=info)
     np.testing.assert_allclose(hs, hd, rtol=1e-12)
     assert float((t2s - t2d).abs().max()) < 1e-12 * float(t2d.abs().max())
-    assert info["sector_amplitudes"] == 197072
+    assert info["sector_amplitudes"] == 49268
 
 
 def test_sector_ccs_and_t1initial():
