@@ -236,6 +236,15 @@ int nl_sector_dress(int64_t no, const int64_t* cptr, int64_t nchunk, const int64
                     const int* elem, const double* V, const double* T, const double* base,
                     double alpha, double* ws, double* X, void* stream);
 
+/* The ring terms of t2Iter (dgram_bkcj_acik, dgram_cdkl_acik_dblj: ccDgrams.py:253-264,
+ * 279-290) for crossed sectors of at most 32 pairs, one workgroup per pair of sectors {x, -x}:
+ * pairs holds 7 int64 per workgroup (offsets of Tr_x and Tr_-x in the crossed layout, of VPH_x and
+ * VPH_-x, |R_x|, |R_-x|, self-conjugate flag).  Tr is gathered from the packed amplitudes T
+ * through the signed map rmap, W_x = 1/2 VPr_x Tr_-x - VPH_x stays in LDS, and
+ * S_x = Tr_x W_-x is written at the offset of Tr_x. */
+int nl_sector_ring(int64_t npair, const int64_t* pairs, const int64_t* rmap, const double* T,
+                   const double* VPr, const double* VPH, double* S, void* stream);
+
 /* Sector blocks of the plane-wave SU(4) contact, written on the GPU instead of gathered from
  * host arrays (the NO2B two-body blocks of coupled_cluster.py:99-263 / 752-832 for the
  * plane-wave transform of lattice.contacts): block b is out[off[b]..off[b+1]) (off has nblk+1

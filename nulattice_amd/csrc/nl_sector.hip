@@ -288,6 +288,93 @@ __global__ __launch_bounds__(NT) void sector_dress_sum_kernel(int64_t no,
   X[o] = base[o] + alpha * acc;
 }
 
+// ---------------------------------------------------------------- crossed-channel ring
+// One workgroup per pair of crossed sectors {x, y = -x} (|R_x|, |R_y| <= 32): the images
+// Tr_x [R_x][R_y] and Tr_y [R_y][R_x] are gathered from the packed amplitudes through the signed
+// map, and with VPr (the same images of v_pphh) and VPH staged in LDS it forms
+//   W_x = 1/2 VPr_x Tr_y - VPH_x,  W_y = 1/2 VPr_y Tr_x - VPH_y   (in LDS only)
+//   Spp_x = Tr_x W_y,  Spp_y = Tr_y W_x                          (written to S)
+// so the ring intermediate never reaches HBM.  A self-conjugate sector (y = x) is its own pair.
+constexpr int RR = 32;
+
+__device__ inline double signed_at(const int64_t* map, const double* v, int64_t k) {
+  const int64_t m = map[k];
+  return m >= 0 ? v[m] : (m <= -2 ? -v[-m - 2] : 0.0);
+}
+
+__global__ __launch_bounds__(NT) void sector_ring_kernel(const int64_t* __restrict__ pairs,
+                                                         const int64_t* __restrict__ rmap,
+                                                         const double* __restrict__ T,
+                                                         const double* __restrict__ VPr,
+                                                         const double* __restrict__ VPH,
+                                                         double* __restrict__ S) {
+  __shared__ double Tx[RR][RR + 1], Ty[RR][RR + 1], Px[RR][RR + 1], Py[RR][RR + 1];
+  __shared__ double Wx[RR][RR + 1], Wy[RR][RR + 1];
+  const int64_t* q = pairs + 7 * (int64_t)blockIdx.x;
+  const int64_t rx = q[0], ry = q[1], wx = q[2], wy = q[3];
+  const int n1 = (int)min(q[4], (int64_t)RR), n2 = (int)min(q[5], (int64_t)RR);
+  const bool self = q[6] != 0;
+  const int tid = threadIdx.x;
+  for (int e = tid; e < n1 * n2; e += NT) {
+    const int r = e / n2, c = e % n2;
+    Tx[r][c] = signed_at(rmap, T, rx + e);
+    Px[r][c] = VPr[rx + e];
+  }
+  for (int e = tid; e < n1 * n1; e += NT) Wx[e / n1][e % n1] = VPH[wx + e];
+  if (!self) {
+    for (int e = tid; e < n2 * n1; e += NT) {
+      const int r = e / n1, c = e % n1;
+      Ty[r][c] = signed_at(rmap, T, ry + e);
+      Py[r][c] = VPr[ry + e];
+    }
+    for (int e = tid; e < n2 * n2; e += NT) Wy[e / n2][e % n2] = VPH[wy + e];
+  }
+  __syncthreads();
+  // W_x (n1 x n1, K = n2) and W_y (n2 x n2, K = n1); each thread owns up to 4 + 4 entries
+  double wxv[4], wyv[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int e = tid + NT * u;
+    wxv[u] = 0.0;
+    if (e < n1 * n1) {
+      const int r = e / n1, c = e % n1;
+      double acc = 0.0;
+      for (int k = 0; k < n2; ++k) acc = fma(Px[r][k], self ? Tx[k][c] : Ty[k][c], acc);
+      wxv[u] = 0.5 * acc - Wx[r][c];
+    }
+    wyv[u] = 0.0;
+    if (!self && e < n2 * n2) {
+      const int r = e / n2, c = e % n2;
+      double acc = 0.0;
+      for (int k = 0; k < n1; ++k) acc = fma(Py[r][k], Tx[k][c], acc);
+      wyv[u] = 0.5 * acc - Wy[r][c];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int e = tid + NT * u;
+    if (e < n1 * n1) Wx[e / n1][e % n1] = wxv[u];
+    if (!self && e < n2 * n2) Wy[e / n2][e % n2] = wyv[u];
+  }
+  __syncthreads();
+  // Spp_x = Tr_x W_y (n1 x n2), Spp_y = Tr_y W_x (n2 x n1)
+  for (int e = tid; e < n1 * n2; e += NT) {
+    const int r = e / n2, c = e % n2;
+    double acc = 0.0;
+    for (int k = 0; k < n2; ++k) acc = fma(Tx[r][k], self ? Wx[k][c] : Wy[k][c], acc);
+    S[rx + e] = acc;
+  }
+  if (!self) {
+    for (int e = tid; e < n2 * n1; e += NT) {
+      const int r = e / n1, c = e % n1;
+      double acc = 0.0;
+      for (int k = 0; k < n1; ++k) acc = fma(Ty[r][k], Wx[k][c], acc);
+      S[ry + e] = acc;
+    }
+  }
+}
+
 // ---------------------------------------------------------------- contact blocks
 // Sector blocks of the SU(4) contact <pq||rs> = g [d(s_p,s_r) d(s_q,s_s) - d(s_p,s_s) d(s_q,s_r)]
 // (momentum conservation holds inside every sector by construction).  Block b occupies
@@ -423,6 +510,17 @@ int nl_sector_dress(int64_t no, const int64_t* cptr, int64_t nchunk, const int64
                        (hipStream_t)stream, nchunk, cstart, elem, V, T, ws);
   hipLaunchKernelGGL(sector_dress_sum_kernel, dim3((unsigned)g2), dim3(NT), 0,
                      (hipStream_t)stream, no, cptr, ws, base, alpha, X);
+  NL_LAUNCH_CHECK();
+  return NL_OK;
+}
+
+int nl_sector_ring(int64_t npair, const int64_t* pairs, const int64_t* rmap, const double* T,
+                   const double* VPr, const double* VPH, double* S, void* stream) {
+  if (npair < 0 || npair > INT32_MAX) return NL_EARG;
+  if (npair == 0) return NL_OK;
+  if (!pairs || !rmap || !T || !VPr || !VPH || !S) return NL_EARG;
+  hipLaunchKernelGGL(sector_ring_kernel, dim3((unsigned)npair), dim3(NT), 0, (hipStream_t)stream,
+                     pairs, rmap, T, VPr, VPH, S);
   NL_LAUNCH_CHECK();
   return NL_OK;
 }

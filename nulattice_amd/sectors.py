@@ -265,6 +265,11 @@ class SectorLayout:
         self.desc_w = np.stack([Rx, Rx, Rn, ro_x, Rn, o, ro_n, Rx, o, wo_x, Rx, o], 1)
         # Spp_x = Tr_x W_neg(x): [R_x][R_neg(x)], K = |R_neg(x)|
         self.desc_s = np.stack([Rx, Rn, Rn, ro_x, Rn, o, wo_n, Rn, o, ro_x, Rn, o], 1)
+        # the fused ring kernel: one entry per pair {x, neg(x)} of crossed sectors
+        first = xk <= nk
+        self.ring_pairs = np.stack([ro_x, ro_n, wo_x, wo_n, Rx, Rn, (xk == nk).astype(np.int64)],
+                                   1)[first].astype(np.int64).reshape(-1, 7)
+        self.ring_fused = bool(len(Rx) == 0 or max(Rx.max(), Rn.max()) <= 32)
         for name in ("desc_pp", "desc_whh", "desc_hh", "desc_w", "desc_s"):
             dsc = getattr(self, name).astype(np.int64).reshape(-1, 12)
             dsc = dsc[(dsc[:, 0] > 0) & (dsc[:, 1] > 0)]
@@ -488,6 +493,7 @@ class SectorHamiltonian:
         self.base = t(self.fdiag)
         self.VPr = torch.empty(lay.nring, dtype=F64, device=device)
         self._gather(self.ring_map, self.V, self.VPr)
+        self.ring_pairs = t(lay.ring_pairs, I64) if lay.ring_fused else None
         self.problems = {}
         for name in ("pp", "whh", "hh", "w", "s"):
             self.problems[name] = [(mode, t(dsc, I64), t(til, I32), til.shape[0])
@@ -579,7 +585,13 @@ class SectorHamiltonian:
         self._gemm("whh", 1.0, self.V, t2, 1.0, Bf["Whh"], self.VHH)
 
     def chain_ring(self, t2, Bf):
-        """Crossed-channel image of t2, W = 1/2 v_pphh t2 - v_phph, Spp = t2 W."""
+        """Crossed-channel image of t2, W = 1/2 v_pphh t2 - v_phph, Spp = t2 W (one fused
+        launch when every crossed sector has at most 32 pairs)."""
+        if self.ring_pairs is not None:
+            _lib.call("nl_sector_ring", self.layout.ring_pairs.shape[0], ops._p(self.ring_pairs),
+                      ops._p(self.ring_map), ops._p(t2), ops._p(self.VPr), ops._p(self.VPHPH),
+                      ops._p(Bf["S"]), ops._stream())
+            return
         self._gather(self.ring_map, t2, Bf["Tr"])
         self._gemm("w", 0.5, self.VPr, Bf["Tr"], -1.0, Bf["W"], self.VPHPH)
         self._gemm("s", 1.0, Bf["Tr"], Bf["W"], 0.0, Bf["S"])

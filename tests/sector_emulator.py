@@ -70,3 +70,24 @@ def contact_fill(off, ncol, roff, coff, rcode, ccode, crossed, g):
         out[off[b]:off[b + 1]] = (g * (((p == r) & (q == s)).astype(float)
                                       - ((p == s) & (q == r)).astype(float))).reshape(-1)
     return out
+
+
+def ring_fused(pairs, ring_map, T, VPr, VPH, S):
+    """nl_sector_ring: per pair {x, y}, W_x = 1/2 VPr_x Tr_y - VPH_x, Spp_x = Tr_x W_y (and the
+    same with x and y exchanged unless self-conjugate), written into S at the Tr offsets."""
+    S = S.copy()
+    Tr = gather(ring_map, T)
+    for rx, ry, wx, wy, n1, n2, self_ in pairs:
+        Tx = Tr[rx:rx + n1 * n2].reshape(n1, n2)
+        Px = VPr[rx:rx + n1 * n2].reshape(n1, n2)
+        if self_:
+            Wx = 0.5 * Px @ Tx - VPH[wx:wx + n1 * n1].reshape(n1, n1)
+            S[rx:rx + n1 * n2] = (Tx @ Wx).reshape(-1)
+            continue
+        Ty = Tr[ry:ry + n2 * n1].reshape(n2, n1)
+        Py = VPr[ry:ry + n2 * n1].reshape(n2, n1)
+        Wx = 0.5 * Px @ Ty - VPH[wx:wx + n1 * n1].reshape(n1, n1)
+        Wy = 0.5 * Py @ Tx - VPH[wy:wy + n2 * n2].reshape(n2, n2)
+        S[rx:rx + n1 * n2] = (Tx @ Wy).reshape(-1)
+        S[ry:ry + n2 * n1] = (Ty @ Wx).reshape(-1)
+    return S

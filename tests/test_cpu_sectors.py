@@ -165,3 +165,26 @@ def test_contact_fill_specs_reproduce_the_packed_hamiltonian(L, A):
     for name in ("V", "VPP", "VHH", "VPH"):
         got = emu.contact_fill(*specs[name], pw.g)
         np.testing.assert_array_equal(got, hst[name])
+
+
+@pytest.mark.parametrize("L,A", [(3, 4), (3, 28)])
+def test_fused_ring_statement_equals_the_grouped_products(L, A):
+    """nl_sector_ring's pair table reproduces the two grouped ring products (W, then Spp) on
+    every crossed sector, self-conjugate ones included."""
+    pw, fock, v, lay = _setup(L, A)
+    hst = pack_hamiltonian(lay, fock, v)
+    assert lay.ring_fused
+    t = lay.pack(_random_t2(lay, 5))
+    Tr = emu.gather(lay.ring_map, t)
+    VPr = emu.gather(lay.ring_map, hst["V"])
+    W = np.zeros(lay.nw)
+    emu.gemm(lay.desc_w, 0.5, VPr, Tr, -1.0, W, hst["VPH"])
+    S = np.zeros(lay.nring)
+    emu.gemm(lay.desc_s, 1.0, Tr, W, 0.0, S)
+    S2 = emu.ring_fused(lay.ring_pairs, lay.ring_map, t, VPr, hst["VPH"], np.zeros(lay.nring))
+    np.testing.assert_allclose(S2, S, rtol=1e-13, atol=1e-16)
+    covered = np.zeros(lay.nring, bool)
+    for rx, ry, wx, wy, n1, n2, self_ in lay.ring_pairs:
+        covered[rx:rx + n1 * n2] = True
+        covered[ry:ry + n1 * n2] = True
+    assert covered.all()
