@@ -245,6 +245,24 @@ int nl_sector_dress(int64_t no, const int64_t* cptr, int64_t nchunk, const int64
 int nl_sector_ring(int64_t npair, const int64_t* pairs, const int64_t* rmap, const double* T,
                    const double* VPr, const double* VPH, double* S, void* stream);
 
+/* Reverse sweep of nl_sector_ring for the Lambda equations of the sector form (no reference
+ * counterpart; the transpose of the ring terms of t2Iter): from the adjoint gS of Spp in the
+ * crossed layout, gTr_x = gS_x W_-x^T + 1/2 VPr_-x^T (Tr_x^T gS_x) for every crossed sector,
+ * written at the offsets of Tr_x (W recomputed in LDS). */
+int nl_sector_ring_adj(int64_t npair, const int64_t* pairs, const int64_t* rmap, const double* T,
+                       const double* VPr, const double* VPH, const double* gS, double* gTr,
+                       void* stream);
+
+/* out[e] += alpha * (signed in[idx[e]]) (signed map as above): the reverse sweep's gathers. */
+int nl_sector_gather_add(int64_t n, const int64_t* idx, const double* in, double alpha,
+                         double* out, void* stream);
+
+/* out[e] += alpha * W[e] * (G[a] + G[b] + G[p+i] + G[p+j]), (a,b,i,j) = orb[0..3][e]: the
+ * transpose of the dressed-diagonal terms of t2Iter (coupled_cluster.py:536-545) in the
+ * reverse sweep. */
+int nl_sector_diag_axpy(int64_t n, int64_t p, const int* orb, const double* W, const double* G,
+                        double alpha, double* out, void* stream);
+
 /* Sector blocks of the plane-wave SU(4) contact, written on the GPU instead of gathered from
  * host arrays (the NO2B two-body blocks of coupled_cluster.py:99-263 / 752-832 for the
  * plane-wave transform of lattice.contacts): block b is out[off[b]..off[b+1]) (off has nblk+1

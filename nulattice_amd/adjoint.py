@@ -328,7 +328,9 @@ class PlaneWaveCCSDEnergy(torch.autograd.Function):
     ("fp64", "bf16", "fp16": low-precision iteration followed by FP64 refinement), lambda_eps,
     lambda_maxSteps, recompute (default True: the converged T2 is not kept between forward and
     backward; the backward re-solves the amplitude equations, whose kernels are deterministic,
-    to regain it), device, info (dict receiving energies, steps and timings)."""
+    to regain it), device, info (dict receiving energies, steps and timings), form ("dense",
+    default, or "sectors": the symmetry-sector form of the amplitude and Lambda iterations, FP64,
+    with ``precision`` ignored)."""
 
     @staticmethod
     def forward(ctx, C0, D0, L, A, opts):
@@ -336,9 +338,19 @@ class PlaneWaveCCSDEnergy(torch.autograd.Function):
         from .planewave import PlaneWave, shell_holes
         dev = ccm._device(opts.get("device"))
         pw = PlaneWave(L, shell_holes(L, A), float(C0), float(D0))
+        info = opts.get("info")
+        if opts.get("form", "dense") == "sectors":
+            from . import sectors
+            SH = sectors.SectorHamiltonian.from_planewave(pw, device=dev)
+            E0 = pw.ref_energy()
+            e, t2 = _solve_sectors(sectors, SH, opts, info, "forward")
+            ctx.state = dict(pw=pw, SH=SH, opts=opts, e=e,
+                             t=t2 if not opts.get("recompute", True) else None)
+            if info is not None:
+                info.update(E_ref=E0, E_corr=e)
+            return torch.tensor(E0 + e, dtype=F64)
         E0, fock, v = pw.hamiltonian(sparse=True)
         ham = engine.Hamiltonian(fock, v, device=dev)
-        info = opts.get("info")
         e, t1, t2 = _solve(ccm, ham, fock, v, opts, info, "forward")
         keep = not opts.get("recompute", True)
         ctx.state = dict(pw=pw, ham=ham, fock=fock, v=v, opts=opts, e=e,
@@ -351,6 +363,8 @@ class PlaneWaveCCSDEnergy(torch.autograd.Function):
     def backward(ctx, gE):
         from . import ccm
         s = ctx.state
+        if "SH" in s:
+            return _backward_sectors(ctx, gE)
         ham, pw, opts = s["ham"], s["pw"], s["opts"]
         info = opts.get("info")
         if s["t"] is None:
@@ -394,6 +408,46 @@ def _solve(ccm, ham, fock, v, opts, info, phase):
     if info is not None:
         info[phase] = sinfo
     return e, t1, t2
+
+
+def _solve_sectors(sectors, SH, opts, info, phase):
+    sinfo = {}
+    out = sectors.solve(SH, eps=opts.get("eps", 1e-12), maxSteps=opts.get("maxSteps", 1000),
+                        max_diis=opts.get("max_diis", 10), diis="pulsed",
+                        mixing=opts.get("mixing", 0.5), info=sinfo, unpack=False)
+    if out is None or not sinfo.get("converged"):
+        raise _lib.NLError("CCSD amplitude iteration did not converge: no gradient at an "
+                           "unconverged point")
+    if info is not None:
+        info[phase] = {k: v for k, v in sinfo.items() if k != "sector_t2"}
+    return out[0], sinfo["sector_t2"]
+
+
+def _backward_sectors(ctx, gE):
+    from . import sectors
+    s = ctx.state
+    SH, pw, opts = s["SH"], s["pw"], s["opts"]
+    info = opts.get("info")
+    if s["t"] is None:
+        e, t2 = _solve_sectors(sectors, SH, opts, info, "backward")
+        if info is not None:
+            info["recomputed_E_corr_diff"] = abs(e - s["e"])
+    else:
+        t2 = s["t"]
+    names = [n for k, n in enumerate(("C0", "D0")) if ctx.needs_input_grad[k]]
+    derivs = [sectors.planewave_derivative(SH, pw, n) for n in names]
+    lam = {}
+    vals = dict(zip(names, sectors.coupling_gradient(
+        SH, t2, derivs, info=lam, eps=opts.get("lambda_eps", 1e-10),
+        maxSteps=opts.get("lambda_maxSteps", 500), mixing=opts.get("mixing", 0.5),
+        max_diis=opts.get("max_diis", 10))))
+    grads = [vals.get("C0"), vals.get("D0")]
+    if info is not None:
+        info.update(lam)
+        info["dE_dC0"], info["dE_dD0"] = grads
+    g = float(gE)
+    out = [None if x is None else torch.tensor(g * x, dtype=F64) for x in grads]
+    return out[0], out[1], None, None, None
 
 
 def planewave_ccsd_energy(C0, D0, L, A, **opts):

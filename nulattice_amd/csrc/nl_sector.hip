@@ -375,6 +375,122 @@ __global__ __launch_bounds__(NT) void sector_ring_kernel(const int64_t* __restri
   }
 }
 
+// Reverse sweep of the ring kernel (the Lambda equations): given the adjoint gS of Spp in the
+// crossed layout, recompute W_x, W_y in LDS and form
+//   gW_y = Tr_x^T gS_x,  gW_x = Tr_y^T gS_y,
+//   gTr_x = gS_x W_y^T + 1/2 VPr_y^T gW_y,  gTr_y = gS_y W_x^T + 1/2 VPr_x^T gW_x,
+// written to gTr at the offsets of Tr_x and Tr_y (self-conjugate: y = x, one set).
+__global__ __launch_bounds__(NT) void sector_ring_adj_kernel(const int64_t* __restrict__ pairs,
+                                                             const int64_t* __restrict__ rmap,
+                                                             const double* __restrict__ T,
+                                                             const double* __restrict__ VPr,
+                                                             const double* __restrict__ VPH,
+                                                             const double* __restrict__ gS,
+                                                             double* __restrict__ gTr) {
+  __shared__ double Tx[RR][RR + 1], Ty[RR][RR + 1], Px[RR][RR + 1], Py[RR][RR + 1];
+  __shared__ double Wx[RR][RR + 1], Wy[RR][RR + 1], Gx[RR][RR + 1], Gy[RR][RR + 1];
+  __shared__ double Hx[RR][RR + 1], Hy[RR][RR + 1];          // gW_x, gW_y
+  const int64_t* q = pairs + 7 * (int64_t)blockIdx.x;
+  const int64_t rx = q[0], ry = q[1], wx = q[2], wy = q[3];
+  const int n1 = (int)min(q[4], (int64_t)RR), n2 = (int)min(q[5], (int64_t)RR);
+  const bool self = q[6] != 0;
+  const int tid = threadIdx.x;
+  // with y = x the y-side arrays alias the x-side ones
+  double (*TY)[RR + 1] = self ? Tx : Ty;
+  double (*PY)[RR + 1] = self ? Px : Py;
+  double (*WY)[RR + 1] = self ? Wx : Wy;
+  double (*GY)[RR + 1] = self ? Gx : Gy;
+  double (*HY)[RR + 1] = self ? Hx : Hy;
+  for (int e = tid; e < n1 * n2; e += NT) {
+    const int r = e / n2, c = e % n2;
+    Tx[r][c] = signed_at(rmap, T, rx + e);
+    Px[r][c] = VPr[rx + e];
+    Gx[r][c] = gS[rx + e];
+  }
+  if (!self) {
+    for (int e = tid; e < n2 * n1; e += NT) {
+      const int r = e / n1, c = e % n1;
+      Ty[r][c] = signed_at(rmap, T, ry + e);
+      Py[r][c] = VPr[ry + e];
+      Gy[r][c] = gS[ry + e];
+    }
+  }
+  __syncthreads();
+  // W_x = 1/2 P_x T_y - VPH_x (n1 x n1), W_y = 1/2 P_y T_x - VPH_y (n2 x n2)
+  for (int e = tid; e < n1 * n1; e += NT) {
+    const int r = e / n1, c = e % n1;
+    double acc = 0.0;
+    for (int k = 0; k < n2; ++k) acc = fma(Px[r][k], TY[k][c], acc);
+    Wx[r][c] = 0.5 * acc - VPH[wx + e];
+  }
+  if (!self)
+    for (int e = tid; e < n2 * n2; e += NT) {
+      const int r = e / n2, c = e % n2;
+      double acc = 0.0;
+      for (int k = 0; k < n1; ++k) acc = fma(Py[r][k], Tx[k][c], acc);
+      Wy[r][c] = 0.5 * acc - VPH[wy + e];
+    }
+  // gW_y = T_x^T gS_x (n2 x n2), gW_x = T_y^T gS_y (n1 x n1)
+  for (int e = tid; e < n2 * n2; e += NT) {
+    const int r = e / n2, c = e % n2;
+    double acc = 0.0;
+    for (int k = 0; k < n1; ++k) acc = fma(Tx[k][r], Gx[k][c], acc);
+    HY[r][c] = acc;
+  }
+  if (!self)
+    for (int e = tid; e < n1 * n1; e += NT) {
+      const int r = e / n1, c = e % n1;
+      double acc = 0.0;
+      for (int k = 0; k < n2; ++k) acc = fma(Ty[k][r], Gy[k][c], acc);
+      Hx[r][c] = acc;
+    }
+  __syncthreads();
+  // gTr_x = gS_x W_y^T + 1/2 P_y^T gW_y (n1 x n2)
+  for (int e = tid; e < n1 * n2; e += NT) {
+    const int r = e / n2, c = e % n2;
+    double a1 = 0.0, a2 = 0.0;
+    for (int k = 0; k < n2; ++k) {
+      a1 = fma(Gx[r][k], WY[c][k], a1);
+      a2 = fma(PY[k][r], HY[k][c], a2);
+    }
+    gTr[rx + e] = a1 + 0.5 * a2;
+  }
+  if (!self)
+    for (int e = tid; e < n2 * n1; e += NT) {
+      const int r = e / n1, c = e % n1;
+      double a1 = 0.0, a2 = 0.0;
+      for (int k = 0; k < n1; ++k) {
+        a1 = fma(Gy[r][k], Wx[c][k], a1);
+        a2 = fma(Px[k][r], Hx[k][c], a2);
+      }
+      gTr[ry + e] = a1 + 0.5 * a2;
+    }
+}
+
+// out[e] += alpha * (signed in[idx[e]])
+__global__ __launch_bounds__(NT) void sector_gather_add_kernel(int64_t n, const int64_t* __restrict__ idx,
+                                                               const double* __restrict__ in,
+                                                               double alpha,
+                                                               double* __restrict__ out) {
+  for (int64_t t = blockIdx.x * (int64_t)NT + threadIdx.x; t < n; t += (int64_t)gridDim.x * NT) {
+    const int64_t s = idx[t];
+    if (s >= 0) out[t] += alpha * in[s];
+    else if (s <= -2) out[t] -= alpha * in[-s - 2];
+  }
+}
+
+// out[e] += alpha * W[e] * (G[a] + G[b] + G[p+i] + G[p+j])
+__global__ __launch_bounds__(NT) void sector_diag_axpy_kernel(int64_t n, int64_t p,
+                                                              const int* __restrict__ orb,
+                                                              const double* __restrict__ W,
+                                                              const double* __restrict__ G,
+                                                              double alpha, double* __restrict__ out) {
+  for (int64_t e = blockIdx.x * (int64_t)NT + threadIdx.x; e < n; e += (int64_t)gridDim.x * NT) {
+    const int a = orb[e], b = orb[n + e], i = orb[2 * n + e], j = orb[3 * n + e];
+    out[e] += alpha * W[e] * (G[a] + G[b] + G[p + i] + G[p + j]);
+  }
+}
+
 // ---------------------------------------------------------------- contact blocks
 // Sector blocks of the SU(4) contact <pq||rs> = g [d(s_p,s_r) d(s_q,s_s) - d(s_p,s_s) d(s_q,s_r)]
 // (momentum conservation holds inside every sector by construction).  Block b occupies
@@ -521,6 +637,40 @@ int nl_sector_ring(int64_t npair, const int64_t* pairs, const int64_t* rmap, con
   if (!pairs || !rmap || !T || !VPr || !VPH || !S) return NL_EARG;
   hipLaunchKernelGGL(sector_ring_kernel, dim3((unsigned)npair), dim3(NT), 0, (hipStream_t)stream,
                      pairs, rmap, T, VPr, VPH, S);
+  NL_LAUNCH_CHECK();
+  return NL_OK;
+}
+
+int nl_sector_ring_adj(int64_t npair, const int64_t* pairs, const int64_t* rmap, const double* T,
+                       const double* VPr, const double* VPH, const double* gS, double* gTr,
+                       void* stream) {
+  if (npair < 0 || npair > INT32_MAX) return NL_EARG;
+  if (npair == 0) return NL_OK;
+  if (!pairs || !rmap || !T || !VPr || !VPH || !gS || !gTr) return NL_EARG;
+  hipLaunchKernelGGL(sector_ring_adj_kernel, dim3((unsigned)npair), dim3(NT), 0,
+                     (hipStream_t)stream, pairs, rmap, T, VPr, VPH, gS, gTr);
+  NL_LAUNCH_CHECK();
+  return NL_OK;
+}
+
+int nl_sector_gather_add(int64_t n, const int64_t* idx, const double* in, double alpha,
+                         double* out, void* stream) {
+  if (n < 0) return NL_EARG;
+  if (n == 0) return NL_OK;
+  if (!idx || !in || !out) return NL_EARG;
+  hipLaunchKernelGGL(sector_gather_add_kernel, dim3(grid_of(n, 8192)), dim3(NT), 0,
+                     (hipStream_t)stream, n, idx, in, alpha, out);
+  NL_LAUNCH_CHECK();
+  return NL_OK;
+}
+
+int nl_sector_diag_axpy(int64_t n, int64_t p, const int* orb, const double* W, const double* G,
+                        double alpha, double* out, void* stream) {
+  if (n < 0 || p < 0) return NL_EARG;
+  if (n == 0) return NL_OK;
+  if (!orb || !W || !G || !out) return NL_EARG;
+  hipLaunchKernelGGL(sector_diag_axpy_kernel, dim3(grid_of(n, 8192)), dim3(NT), 0,
+                     (hipStream_t)stream, n, p, orb, W, G, alpha, out);
   NL_LAUNCH_CHECK();
   return NL_OK;
 }

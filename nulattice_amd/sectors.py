@@ -255,6 +255,9 @@ class SectorLayout:
         self.desc_pp = np.stack([nP, nH, nP, po, nP, one, to, nH, one, to, nH, one], 1)
         self.desc_whh = np.stack([nH, nH, nP, to, one, nH, to, nH, one, ho, nH, one], 1)
         self.desc_hh = np.stack([nP, nH, nH, to, nH, one, ho, nH, one, to, nH, one], 1)
+        # reverse sweep: VPP^T x and x Whh^T
+        self.desc_ppT = np.stack([nP, nH, nP, po, one, nP, to, nH, one, to, nH, one], 1)
+        self.desc_hhT = np.stack([nP, nH, nH, to, nH, one, ho, one, nH, to, nH, one], 1)
         keep = (nRx > 0) & (nRn > 0)
         xk, nk = xs[keep], nx[keep]
         Rx, Rn = nR_all[xk], nR_all[nk]
@@ -270,7 +273,28 @@ class SectorLayout:
         self.ring_pairs = np.stack([ro_x, ro_n, wo_x, wo_n, Rx, Rn, (xk == nk).astype(np.int64)],
                                    1)[first].astype(np.int64).reshape(-1, 7)
         self.ring_fused = bool(len(Rx) == 0 or max(Rx.max(), Rn.max()) <= 32)
-        for name in ("desc_pp", "desc_whh", "desc_hh", "desc_w", "desc_s"):
+        # reverse-sweep maps: the packed element (signed) whose assembly reads each Spp position,
+        # and the four signed crossed-image positions of each packed element
+        self.sg_inv = np.full(self.nring, -1, np.int64)
+        for k, sign in enumerate((1, -1, -1, 1)):
+            pos = self.sg[k].astype(np.int64)
+            if np.any(self.sg_inv[pos] != -1):
+                raise AssertionError("an Spp position is read by two assembly terms")
+            e = np.arange(self.n, dtype=np.int64)
+            self.sg_inv[pos] = e if sign > 0 else -e - 2
+        rm = self.ring_map
+        pos = np.nonzero(rm != -1)[0]
+        enc = rm[pos]
+        elem_of = np.where(enc >= 0, enc, -enc - 2)
+        sgn = np.where(enc >= 0, 1, -1)
+        order = np.argsort(elem_of, kind="stable")
+        if not np.array_equal(np.bincount(elem_of, minlength=self.n), np.full(self.n, 4)):
+            raise AssertionError("a packed element does not have four crossed images")
+        p4 = pos[order].reshape(self.n, 4)
+        s4 = sgn[order].reshape(self.n, 4)
+        self.ring_inv = np.where(s4 > 0, p4, -p4 - 2).T.copy().astype(np.int64)
+        for name in ("desc_pp", "desc_whh", "desc_hh", "desc_w", "desc_s", "desc_ppT",
+                     "desc_hhT"):
             dsc = getattr(self, name).astype(np.int64).reshape(-1, 12)
             dsc = dsc[(dsc[:, 0] > 0) & (dsc[:, 1] > 0)]
             setattr(self, name, dsc)
@@ -494,13 +518,31 @@ class SectorHamiltonian:
         self.VPr = torch.empty(lay.nring, dtype=F64, device=device)
         self._gather(self.ring_map, self.V, self.VPr)
         self.ring_pairs = t(lay.ring_pairs, I64) if lay.ring_fused else None
+        self.sg_inv = t(lay.sg_inv, I64)
+        self.ring_inv = [t(m, I64) for m in lay.ring_inv]
         self.problems = {}
-        for name in ("pp", "whh", "hh", "w", "s"):
+        for name in ("pp", "whh", "hh", "w", "s", "ppT", "hhT"):
             self.problems[name] = [(mode, t(dsc, I64), t(til, I32), til.shape[0])
                                    for mode, dsc, til in getattr(lay, "launch_" + name)]
         self._side = (torch.cuda.Stream(device=device), torch.cuda.Stream(device=device))
         self.nparts = int(_lib.load().nl_partials(lay.n))
         self._dot_parts = max(self.nparts, int(_lib.load().nl_partials(p * h)))
+
+    def scaled(self, scale, fdiag):
+        """The Hamiltonian whose two-body blocks are ``scale`` times these and whose Fock
+        diagonals are ``fdiag`` = (f_aa, -f_ii) (the derivative Hamiltonian of a coupling that
+        multiplies the two-body part and shifts the Fock diagonals); shares layout and maps."""
+        new = SectorHamiltonian.__new__(SectorHamiltonian)
+        new.__dict__.update(self.__dict__)
+        for name in ("V", "VPP", "VHH", "VPHPH", "VPr"):
+            x = getattr(self, name)
+            y = torch.zeros_like(x)
+            _lib.call("nl_axpby", x.numel(), ctypes.c_double(scale), ops._p(x),
+                      ctypes.c_double(0.0), ops._p(y), ops._stream())
+            setattr(new, name, y)
+        new.fdiag = np.asarray(fdiag, np.float64)
+        new.base = torch.tensor(new.fdiag, dtype=F64, device=self.device)
+        return new
 
     # ------------------------------------------------------------------ launches
     def _gather(self, idx, src, out):
@@ -675,9 +717,11 @@ class SectorStepper:
 
 
 def solve(SH, t1initial=None, eps=1e-8, maxSteps=1000, max_diis=10, delta=0, mixing=0.5,
-          verbose=False, ccs=False, diis="legacy", history=None, info=None, graph=True):
+          verbose=False, ccs=False, diis="legacy", history=None, info=None, graph=True,
+          unpack=True):
     """The reference solver loop (coupled_cluster.py:547-687) on a SectorHamiltonian: same
-    stopping rule, messages and return values (E_corr, t1, t2), t2 in the reference layout."""
+    stopping rule, messages and return values (E_corr, t1, t2), t2 in the reference layout
+    (``unpack=False``: t2 is returned packed, a < b, i < j)."""
     from .ccm import _Diis, _fill_info
     p, h, dev = SH.p, SH.h, SH.device
     Bf = SH.buffers()
@@ -725,7 +769,7 @@ def solve(SH, t1initial=None, eps=1e-8, maxSteps=1000, max_diis=10, delta=0, mix
                 print(f"Energy found in {i + 1} iterations")
             _fill_info(info, 0, i + 1, t_start, energy, True)
             _sector_info(info, SH, t2)
-            return energy, t1, SH.unpack(t2)
+            return energy, t1, SH.unpack(t2) if unpack else t2
         if dstate is not None:
             ext = dstate.step(t1, t2, t1, stepper.previous,
                               SimpleNamespace(part=Bf["part"], E=Bf["E"]))
@@ -745,10 +789,127 @@ def solve(SH, t1initial=None, eps=1e-8, maxSteps=1000, max_diis=10, delta=0, mix
     print("Max Iterations Reached")
     _fill_info(info, 0, maxSteps, t_start, energy, False)
     _sector_info(info, SH, t2)
-    return energy, t1, SH.unpack(t2)
+    return energy, t1, SH.unpack(t2) if unpack else t2
 
 
 def _sector_info(info, SH, t2):
     if info is not None:
         info["sector_t2"] = t2
         info["sector_amplitudes"] = SH.layout.n
+
+
+# ------------------------------------------------------------------ Lambda equations (reverse)
+def residual_vjp(SH, t2, lam, Bf, out):
+    """out = J^T lam of the packed residual at t2 without its diagonal part D lam (D = X_a + X_b
+    + X_i + X_j at t2, added by the assembly), each forward product transposed:
+      v_pppp^T lam + lam Whh^T + v_pphh (t^T lam)     (pp and hh ladders; Whh = v_hhhh + v_pphh^T t)
+      + the ring's reverse sweep gathered back onto the packed elements
+      - v * (gX_a + gX_b + gX_i + gX_j),  gX = sums of lam t over the elements of each state
+    Needs Bf["Whh"] of the forward step at t2 (``forward_intermediates``)."""
+    lay = SH.layout
+    SH._gemm("ppT", 1.0, SH.VPP, lam, 0.0, out)
+    SH._gemm("whh", 1.0, t2, lam, 0.0, Bf["G1"])
+    SH._gemm("hh", 1.0, SH.V, Bf["G1"], 1.0, out)
+    SH._gemm("hhT", 1.0, lam, Bf["Whh"], 1.0, out)
+    if SH.ring_pairs is None:
+        raise _lib.NLError("the sector reverse sweep needs crossed sectors of at most 32 pairs")
+    SH._gather(SH.sg_inv, lam, Bf["gS"])
+    _lib.call("nl_sector_ring_adj", lay.ring_pairs.shape[0], ops._p(SH.ring_pairs),
+              ops._p(SH.ring_map), ops._p(t2), ops._p(SH.VPr), ops._p(SH.VPHPH), ops._p(Bf["gS"]),
+              ops._p(Bf["gTr"]), ops._stream())
+    for m in SH.ring_inv:
+        _lib.call("nl_sector_gather_add", lay.n, ops._p(m), ops._p(Bf["gTr"]),
+                  ctypes.c_double(1.0), ops._p(out), ops._stream())
+    _lib.call("nl_sector_dress", SH.p + SH.h, ops._p(SH.cptr), len(lay.cstart) - 1,
+              ops._p(SH.cstart), ops._p(SH.elem), ops._p(lam), ops._p(t2), ops._p(Bf["zero_x"]),
+              ctypes.c_double(1.0), ops._p(Bf["Xc"]), ops._p(Bf["gX"]), ops._stream())
+    _lib.call("nl_sector_diag_axpy", lay.n, SH.p, ops._p(SH.orb), ops._p(SH.V), ops._p(Bf["gX"]),
+              ctypes.c_double(-1.0), ops._p(out), ops._stream())
+
+
+def lambda_buffers(SH):
+    lay, dev = SH.layout, SH.device
+    Bf = SH.buffers()
+    e = lambda n: torch.empty(max(n, 1), dtype=F64, device=dev)
+    Bf.update(G1=e(lay.nhh), gS=e(lay.nring), gTr=e(lay.nring), gX=e(SH.p + SH.h),
+              zero_x=torch.zeros(SH.p + SH.h, dtype=F64, device=dev), JT=e(lay.n),
+              R=e(lay.n))
+    return Bf
+
+
+def solve_lambda(SH, t2, eps=1e-10, maxSteps=500, mixing=0.5, max_diis=10, info=None,
+                 verbose=False):
+    """Packed Lambda amplitudes: J^T lam = -dE/dt at converged packed amplitudes t2 (dE/dt = v,
+    E = sum v t over the stored elements).  Damped Jacobi with the amplitude iteration's dressed
+    denominators, lam_new = m lam + (1-m) (lam + (J^T lam + v) / -D), pulsed DIIS every
+    ``max_diis`` steps; stops when |J^T lam + v| <= eps |v|."""
+    from .ccm import _Diis
+    lay = SH.layout
+    Bf = lambda_buffers(SH)
+    SH.residual(t2, Bf)                          # forward X and Whh at t2
+    n = max(lay.n, 1)
+    lam = torch.zeros(n, dtype=F64, device=SH.device)
+    lamn = torch.empty_like(lam)
+    z1 = torch.zeros((SH.p, SH.h), dtype=F64, device=SH.device)
+    ns = SimpleNamespace(part=Bf["part"], E=Bf["E"])
+    _lib.call("nl_dot_part", lay.n, ops._p(SH.V), ops._p(SH.V), ops._p(Bf["part"]),
+              ops._stream())
+    _lib.call("nl_finalize", SH.nparts, ops._p(Bf["part"]), 0, None, None, ops._p(Bf["E"]),
+              ops._stream())
+    gnorm = math.sqrt(float(Bf["E"].item()))
+    dstate = _Diis("pulsed", max_diis, z1, lam, False) if max_diis > 0 else None
+    t0 = time.perf_counter()
+    res, steps = math.inf, 0
+    for steps in range(1, maxSteps + 1):
+        residual_vjp(SH, t2, lam, Bf, Bf["JT"])
+        _lib.call("nl_sector_assemble", lay.n, SH.p, ops._p(SH.orb), None, ops._p(SH.V),
+                  ops._p(Bf["JT"]), None, ops._p(Bf["X"]), ops._p(lam), ctypes.c_double(mixing),
+                  ctypes.c_double(0.0), ops._p(lamn), ops._p(Bf["R"]), ops._p(Bf["part"]),
+                  ops._stream())
+        _lib.call("nl_finalize", SH.nparts, ops._p(Bf["part"][SH.nparts:]), 0, None, None,
+                  ops._p(Bf["E"]), ops._stream())
+        res = math.sqrt(float(Bf["E"].item())) / max(gnorm, 1e-300)
+        if verbose:
+            print(f"lambda step {steps}: |residual|/|v| = {res:.3e}")
+        if res <= eps or not math.isfinite(res):
+            break
+        lam, lamn = lamn, lam
+        if dstate is not None:
+            ext = dstate.step(z1, lam, z1, lamn, ns)
+            if ext is not None:
+                lam.copy_(ext[1])
+    if info is not None:
+        info.update(lambda_steps=steps, lambda_residual=res,
+                    lambda_seconds=time.perf_counter() - t0)
+    if not math.isfinite(res):
+        raise _lib.NLError("Lambda iteration diverged")
+    return lam
+
+
+def coupling_gradient(SH, t2, derivatives, info=None, **lambda_kw):
+    """dE/dtheta = dE_ref/dtheta + E_corr(t; h_theta) + lam . H(t; h_theta) for each
+    (dE_ref/dtheta, derivative SectorHamiltonian) in ``derivatives``, at converged packed t2."""
+    lam = solve_lambda(SH, t2, info=info, **lambda_kw)
+    out = []
+    for dEref, SD in derivatives:
+        Bd = SD.buffers()
+        tmp = torch.empty_like(t2)
+        H = torch.empty_like(t2)
+        SD.update(t2, tmp, Bd, 0.0, H2out=H)
+        out.append(dEref + SD.energy(t2, Bd) + _vdot(lam, H, Bd, SD))
+    return out
+
+
+def _vdot(x, y, Bf, SH):
+    _lib.call("nl_dot_part", SH.layout.n, ops._p(x), ops._p(y), ops._p(Bf["part"]), ops._stream())
+    _lib.call("nl_finalize", SH.nparts, ops._p(Bf["part"]), 0, None, None, ops._p(Bf["E"]),
+              ops._stream())
+    return float(Bf["E"].item())
+
+
+def planewave_derivative(SH, pw, name):
+    """(dE_ref/dtheta, derivative SectorHamiltonian) of the plane-wave coupling ``name`` ("C0" or
+    "D0"): two-body blocks times (dg/dtheta)/g, Fock diagonals shifted by d(shift)/dtheta."""
+    dEref, fock, scale = pw.coupling_derivative(name)
+    fdiag = np.concatenate([np.diag(fock[0]), -np.diag(fock[2])])
+    return dEref, SH.scaled(scale, fdiag)

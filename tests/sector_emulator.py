@@ -91,3 +91,50 @@ def ring_fused(pairs, ring_map, T, VPr, VPH, S):
         S[rx:rx + n1 * n2] = (Tx @ Wy).reshape(-1)
         S[ry:ry + n2 * n1] = (Ty @ Wx).reshape(-1)
     return S
+
+
+def ring_adj(pairs, ring_map, T, VPr, VPH, gS):
+    """nl_sector_ring_adj: gTr_x = gS_x W_y^T + 1/2 VPr_y^T (Tr_x^T gS_x) per crossed pair."""
+    Tr = gather(ring_map, T)
+    gTr = np.zeros_like(gS)
+    for rx, ry, wx, wy, n1, n2, self_ in pairs:
+        Tx = Tr[rx:rx + n1 * n2].reshape(n1, n2)
+        Px = VPr[rx:rx + n1 * n2].reshape(n1, n2)
+        Gx = gS[rx:rx + n1 * n2].reshape(n1, n2)
+        if self_:
+            W = 0.5 * Px @ Tx - VPH[wx:wx + n1 * n1].reshape(n1, n1)
+            gTr[rx:rx + n1 * n2] = (Gx @ W.T + 0.5 * Px.T @ (Tx.T @ Gx)).reshape(-1)
+            continue
+        Ty = Tr[ry:ry + n2 * n1].reshape(n2, n1)
+        Py = VPr[ry:ry + n2 * n1].reshape(n2, n1)
+        Gy = gS[ry:ry + n2 * n1].reshape(n2, n1)
+        Wx = 0.5 * Px @ Ty - VPH[wx:wx + n1 * n1].reshape(n1, n1)
+        Wy = 0.5 * Py @ Tx - VPH[wy:wy + n2 * n2].reshape(n2, n2)
+        gTr[rx:rx + n1 * n2] = (Gx @ Wy.T + 0.5 * Py.T @ (Tx.T @ Gx)).reshape(-1)
+        gTr[ry:ry + n2 * n1] = (Gy @ Wx.T + 0.5 * Px.T @ (Ty.T @ Gy)).reshape(-1)
+    return gTr
+
+
+def residual_vjp(lay, hst, t, lam):
+    """J^T lam of the packed residual H(t) (every term but the D t diagonal's D lam part is
+    returned in the first value; that part, D lam with D = X_a + X_b + X_i + X_j at t, in the
+    second)."""
+    V, n, p = hst["V"], lay.n, lay.p
+    X = dress(lay.seg, lay.elem, V, t, hst["fdiag"], -1.0)
+    Whh = np.zeros(lay.nhh)
+    gemm(lay.desc_whh, 1.0, V, t, 1.0, Whh, hst["VHH"])
+    out = np.zeros(n)
+    gemm(lay.desc_ppT, 1.0, hst["VPP"], lam, 0.0, out)
+    G1 = np.zeros(lay.nhh)
+    gemm(lay.desc_whh, 1.0, t, lam, 0.0, G1)
+    gemm(lay.desc_hh, 1.0, V, G1, 1.0, out)
+    gemm(lay.desc_hhT, 1.0, lam, Whh, 1.0, out)
+    gS = gather(lay.sg_inv, lam)
+    VPr = gather(lay.ring_map, V)
+    gTr = ring_adj(lay.ring_pairs, lay.ring_map, t, VPr, hst["VPH"], gS)
+    for k in range(4):
+        out += gather(lay.ring_inv[k], gTr)
+    gX = dress(lay.seg, lay.elem, lam, t, np.zeros(len(X)), 1.0)
+    a, b, i, j = lay.orb
+    out -= V * (gX[a] + gX[b] + gX[p + i] + gX[p + j])
+    return out, (X[a] + X[b] + X[p + i] + X[p + j]) * lam

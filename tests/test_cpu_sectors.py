@@ -188,3 +188,26 @@ def test_fused_ring_statement_equals_the_grouped_products(L, A):
         covered[rx:rx + n1 * n2] = True
         covered[ry:ry + n1 * n2] = True
     assert covered.all()
+
+
+@pytest.mark.parametrize("L,A", [(2, 4), (3, 28)])
+def test_reverse_sweep_is_the_transpose_of_the_residual(L, A):
+    """<lam, J d> = <J^T lam, d> for the packed residual (J d by central differences, exact for
+    the quadratic residual up to rounding), at random sector amplitudes and directions."""
+    pw, fock, v, lay = _setup(L, A)
+    hst = pack_hamiltonian(lay, fock, v)
+    rng = np.random.default_rng(7)
+    t = lay.pack(_random_t2(lay, 8))
+    d = rng.standard_normal(lay.n)
+    lam = rng.standard_normal(lay.n)
+    eps = 1e-3
+    Jd = (emu.step(lay, hst, t + eps * d, 0.5)[2] - emu.step(lay, hst, t - eps * d, 0.5)[2]) / (2 * eps)
+    a, b = emu.residual_vjp(lay, hst, t, lam)
+    lhs, rhs = lam @ Jd, (a + b) @ d
+    assert abs(lhs - rhs) <= 1e-10 * max(abs(lhs), 1e-300)
+    # reverse-sweep maps: each packed element has four crossed images, each Spp position is read
+    # by at most one assembly term
+    dec = lambda m: np.where(m >= 0, m, -m - 2)
+    assert np.all(lay.ring_map[dec(lay.ring_inv)] != -1)
+    used = lay.sg_inv[lay.sg_inv != -1]
+    assert np.unique(dec(used)).size * 4 == used.size
