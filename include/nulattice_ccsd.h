@@ -211,7 +211,7 @@ int nl_sub(int64_t n, const double* x, const double* y, double* c, void* stream)
  * n*sCn].  desc: 12 int64 per problem (M N K oA sAm sAk oB sBk sBn oC sCm sCn); tiles: 3 int32
  * per workgroup (problem, m0, n0) of the output tile of ``mode``: 0 = 64x32 tiles; 1 = 32x32 and
  * 2 = 64x8 tiles with K split over the workgroup's four wavefronts (small-N products and long
- * K).  Replaces the per-sector products of the pp ladder (v_pppp_dgrams,
+ * K); 3 = 32x32 tiles with K split over eight wavefronts (512 threads).  Replaces the per-sector products of the pp ladder (v_pppp_dgrams,
  * ccDgrams.py:215-237), the hh ladders (dgram_klij_abkl :240-251, dgram_cdkl_cdij_abkl
  * :292-303) and the ring terms (dgram_bkcj_acik :253-264, dgram_cdkl_acik_dblj :279-290). */
 int nl_gemm_grouped_f64(int mode, int64_t ntile, const int64_t* desc, const int* tiles,

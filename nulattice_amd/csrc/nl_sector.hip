@@ -135,28 +135,30 @@ __global__ __launch_bounds__(NT) void gemm_grouped_f64_kernel(const int64_t* __r
 }
 
 // Split-K tiles for the many small-N products (most pair sectors hold 1-8 hole pairs): a
-// TM x TN output tile per workgroup, whose four wavefronts each own a quarter of K (RM x 4
-// outputs per lane, K steps of 16 with the same one-step register prefetch); the four partial
-// tiles are summed through LDS in a fixed order.  (TM, TN) = (32, 32) or (64, 8).
+// TM x TN output tile per workgroup, whose NW wavefronts each own 1/NW of K (RM x 4 outputs per
+// lane, K steps of 16 with the same one-step register prefetch); the partial tiles are summed
+// through LDS in a fixed order.  (TM, TN, NW) = (32, 32, 4), (64, 8, 4), or (32, 32, 8) for the
+// few widest products, whose time is the length of their K chain.
 constexpr int STK = 16;
 
-template <int TM, int TN>
-__global__ __launch_bounds__(NT) void gemm_grouped_split_kernel(const int64_t* __restrict__ desc,
+template <int TM, int TN, int NW>
+__global__ __launch_bounds__(64 * NW) void gemm_grouped_split_kernel(const int64_t* __restrict__ desc,
                                                                 const int* __restrict__ tiles,
                                                                 double alpha, const double* A,
                                                                 const double* B, double beta,
                                                                 const double* D, double* C) {
   constexpr int NL = TN / 4, ML = 64 / NL, RM = TM / ML;   // lanes along N, along M; rows/lane
   constexpr int AE = TM * STK / 64, BE = STK * TN / 64;    // A, B elements per lane per step
-  __shared__ double As[4][STK][TM + 1];
-  __shared__ double Bs[4][STK][TN + 1];
-  __shared__ double Red[4][TM][TN + 1];
+  constexpr int NTH = 64 * NW;
+  __shared__ double As[NW][STK][TM + 1];
+  __shared__ double Bs[NW][STK][TN + 1];
+  __shared__ double Red[NW][TM][TN + 1];
   const int q = tiles[3 * blockIdx.x], m0 = tiles[3 * blockIdx.x + 1],
             n0 = tiles[3 * blockIdx.x + 2];
   const GDesc g = load_desc(desc + 12 * (int64_t)q);
   const int tid = threadIdx.x, grp = tid >> 6, lane = tid & 63, ly = lane / NL, lx = lane % NL;
   const bool a_k1 = g.sAk == 1, b_n1 = g.sBn == 1;
-  const int64_t kq = ((g.K + 3) / 4 + STK - 1) / STK * STK;   // K range per wavefront
+  const int64_t kq = ((g.K + NW - 1) / NW + STK - 1) / STK * STK;   // K range per wavefront
   const int64_t kb0 = grp * kq, kb1 = (kb0 + kq < g.K) ? kb0 + kq : g.K;
   double ra[AE], rb[BE];
   auto a_at = [&](int idx, int& mm, int& kk) {
@@ -224,10 +226,12 @@ __global__ __launch_bounds__(NT) void gemm_grouped_split_kernel(const int64_t* _
     for (int c = 0; c < 4; ++c) Red[grp][ly + ML * r][lx + NL * c] = acc[r][c];
   __syncthreads();
 #pragma unroll
-  for (int r = 0; r < TM * TN / NT; ++r) {
-    const int idx = tid + NT * r;
+  for (int r = 0; r < TM * TN / NTH; ++r) {
+    const int idx = tid + NTH * r;
     const int m = idx / TN, n = idx % TN;
-    const double v = ((Red[0][m][n] + Red[1][m][n]) + Red[2][m][n]) + Red[3][m][n];
+    double v = Red[0][m][n];
+#pragma unroll
+    for (int w = 1; w < NW; ++w) v += Red[w][m][n];
     grouped_epilogue(g, m0 + m, n0 + n, v, alpha, beta, D, C);
   }
 }
@@ -575,7 +579,7 @@ extern "C" {
 int nl_gemm_grouped_f64(int mode, int64_t ntile, const int64_t* desc, const int* tiles,
                         double alpha, const double* A, const double* B, double beta,
                         const double* D, double* C, void* stream) {
-  if (ntile < 0 || ntile > INT32_MAX || mode < 0 || mode > 2) return NL_EARG;
+  if (ntile < 0 || ntile > INT32_MAX || mode < 0 || mode > 3) return NL_EARG;
   if (ntile == 0) return NL_OK;
   if (!desc || !tiles || !A || !B || !C) return NL_EARG;
   hipStream_t st = (hipStream_t)stream;
@@ -583,11 +587,14 @@ int nl_gemm_grouped_f64(int mode, int64_t ntile, const int64_t* desc, const int*
     hipLaunchKernelGGL(gemm_grouped_f64_kernel, dim3((unsigned)ntile), dim3(NT), 0, st, desc,
                        tiles, alpha, A, B, beta, D, C);
   else if (mode == 1)
-    hipLaunchKernelGGL((gemm_grouped_split_kernel<32, 32>), dim3((unsigned)ntile), dim3(NT), 0,
+    hipLaunchKernelGGL((gemm_grouped_split_kernel<32, 32, 4>), dim3((unsigned)ntile), dim3(NT),
+                       0, st, desc, tiles, alpha, A, B, beta, D, C);
+  else if (mode == 2)
+    hipLaunchKernelGGL((gemm_grouped_split_kernel<64, 8, 4>), dim3((unsigned)ntile), dim3(NT), 0,
                        st, desc, tiles, alpha, A, B, beta, D, C);
   else
-    hipLaunchKernelGGL((gemm_grouped_split_kernel<64, 8>), dim3((unsigned)ntile), dim3(NT), 0,
-                       st, desc, tiles, alpha, A, B, beta, D, C);
+    hipLaunchKernelGGL((gemm_grouped_split_kernel<32, 32, 8>), dim3((unsigned)ntile),
+                       dim3(512), 0, st, desc, tiles, alpha, A, B, beta, D, C);
   NL_LAUNCH_CHECK();
   return NL_OK;
 }

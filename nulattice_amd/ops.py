@@ -119,11 +119,14 @@ def _parse(spec):
 
 
 def _splits(M, N, K, batch):
-    """Split-K factor so that a launch has enough workgroups (deterministic reduction)."""
+    """Split-K factor so that a launch has enough workgroups (deterministic reduction).  Tiny
+    outputs with a long K (the h x h dressings: one tile, K = p^2 h) get up to 1024 splits: each
+    workgroup's K loop is a chain of dependent steps, so their count, not the bytes, sets the
+    time."""
     tiles = math.ceil(M / 128) * math.ceil(N / 128) * batch
     if tiles >= 512 or K <= 512:
         return 1
-    return int(max(1, min(256, 1024 // tiles, K // 256)))
+    return int(max(1, min(1024, 2048 // tiles, K // 256)))
 
 
 def _lp_splits(M, N, K, batch):

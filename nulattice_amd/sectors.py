@@ -81,8 +81,9 @@ def _outer(nr, nc):
 
 
 # output tile (rows, columns) of the grouped-GEMM kernel modes: 0 = 64x32 tiles, 1 = 32x32 and
-# 2 = 64x8 tiles with K split over the four wavefronts of a workgroup
-MODE_TILE = {0: (64, 32), 1: (32, 32), 2: (64, 8)}
+# 2 = 64x8 tiles with K split over the four wavefronts of a workgroup, 3 = 32x32 tiles with K
+# split over eight wavefronts (lists of only a few long-K tiles, whose K chain sets their time)
+MODE_TILE = {0: (64, 32), 1: (32, 32), 2: (64, 8), 3: (32, 32)}
 
 
 def _mode(M, N, K):
@@ -113,7 +114,8 @@ def _launch_lists(desc):
     for m in (0, 1, 2):
         sub = desc[modes == m]
         if len(sub):
-            out.append((m, sub, _tiles(sub, m)))
+            til = _tiles(sub, m)
+            out.append((3 if m == 1 and len(til) < 64 else m, sub, til))
     return out
 
 

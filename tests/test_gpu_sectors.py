@@ -42,14 +42,14 @@ def _problems(rng, nprob, integer, small=False):
     return desc, gen(oa), gen(ob), gen(oc), gen(oc)
 
 
-@pytest.mark.parametrize("mode", [0, 1, 2])
+@pytest.mark.parametrize("mode", [0, 1, 2, 3])
 @pytest.mark.parametrize("integer,beta,use_d", [(True, 0.0, False), (True, -1.0, True),
                                                 (False, 0.5, False), (False, 1.0, True)])
 def test_grouped_gemm(integer, beta, use_d, mode):
     """Every kernel mode (64x32 tiles; 32x32 and 64x8 tiles with K split over wavefronts) on
     the same random problem lists."""
     rng = np.random.default_rng(7)
-    desc, A, B, C, D = _problems(rng, 40, integer, mode == 1)
+    desc, A, B, C, D = _problems(rng, 40, integer, mode in (1, 3))
     til = _tiles(desc, mode)
     alpha = 1.0 if integer else 0.75
     # every device buffer stays bound until the call has been read back: a temporary freed
