@@ -343,7 +343,7 @@ def main():
         sector["adjoint_gradient"] = sector_gradient(a.L, a.A, adjoint, torch, grad)
     cpu = cpu_baseline(a.cpu_sample, a.seed, a.D0)
     out = {
-        "metric": "CCSD iters/sec at L=5 A=28: mixed-prec vs FP64 GPU, |dE| vs FP64 oracle",
+        "metric": f"CCSD iters/sec at L={a.L} A={a.A}: mixed-prec vs FP64 GPU, |dE| vs FP64 oracle",
         "value": 1.0 / slp, "unit": "iters/s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": slp * 1e3, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": f"{a.prec}+fp64",
