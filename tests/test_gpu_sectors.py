@@ -245,3 +245,23 @@ def test_sector_gradient_L5_A28_against_finite_differences():
     assert float(C0.grad) == pytest.approx(fd_c, rel=1e-6)
     assert float(D0.grad) == pytest.approx(fd_d, rel=1e-6)
     assert info["lambda_residual"] <= 1e-10
+
+
+def test_grouped_ring_path_equals_fused_ring():
+    """The ring through the grouped GEMMs (the path for crossed sectors wider than 32 pairs)
+    and through the fused kernel give the same step (L=3, A=28)."""
+    pw, fock, v = _pw(3, 28)
+    SH = SectorHamiltonian.from_planewave(pw)
+    lay = SH.layout
+    rng = np.random.default_rng(9)
+    t = dev(lay.pack(lay.unpack(rng.standard_normal(lay.n) * 1e-2)))
+    out = []
+    for fused in (True, False):
+        if not fused:
+            SH.ring_pairs = None
+        Bf = SH.buffers()
+        tn = torch.empty_like(t)
+        SH.update(t, tn, Bf, 0.5)
+        out.append((host(tn), float(Bf["E"].item())))
+    np.testing.assert_allclose(out[1][0], out[0][0], rtol=1e-13, atol=1e-17)
+    assert out[1][1] == pytest.approx(out[0][1], rel=1e-13)
