@@ -392,10 +392,13 @@ def pack_hamiltonian(lay, fock_mats, two_body_int):
     for x1, x2, y1, y2, sg in ((l_, k_, i_, j_, -1), (k_, l_, j_, i_, -1), (l_, k_, j_, i_, 1)):
         if not np.array_equal(sg * v_hhhh[x1, x2, y1, y2], VHH):
             raise SectorViolation("v_hhhh is not antisymmetric")
-    if 4 * np.count_nonzero(VHH) != np.count_nonzero(v_hhhh):
-        raise SectorViolation("v_hhhh has nonzero elements outside the pair sectors")
     lpf = lambda x: lay.labels_p[x]
     lhf = lambda x: lay.labels_h[x]
+    # (elements of hole-pair sectors without particle pairs never meet t2 and are not stored)
+    nz = np.nonzero(v_hhhh)
+    if nz[0].size and not _conserves([lhf, lhf, lhf, lhf], np.stack(nz, 1), v_hhhh[nz],
+                                     lay.moduli).all():
+        raise SectorViolation("v_hhhh does not conserve the labels")
     nz = np.nonzero(v_phhh)
     if nz[0].size and not _conserves([lpf, lhf, lhf, lhf], np.stack(nz, 1), v_phhh[nz],
                                      lay.moduli).all():

@@ -211,3 +211,58 @@ def test_reverse_sweep_is_the_transpose_of_the_residual(L, A):
     assert np.all(lay.ring_map[dec(lay.ring_inv)] != -1)
     used = lay.sg_inv[lay.sg_inv != -1]
     assert np.unique(dec(used)).size * 4 == used.size
+
+
+def _random_labelled_hamiltonian(p, h, seed, mod=4):
+    """A random antisymmetric Hamiltonian conserving random state-identifying labels
+    (component 0 taken mod ``mod``, component 1 unbounded), holes first."""
+    from tests.helpers import random_antisym_hamiltonian
+    rng = np.random.default_rng(seed)
+    n = p + h
+    while True:
+        lab = np.stack([rng.integers(0, mod, n), rng.integers(-2, 3, n)], 1)
+        if len(np.unique(lab, axis=0)) == n:
+            break
+    _, _, V, F = random_antisym_hamiltonian(p, h, seed=seed + 100)
+    tot = lab[:, None, None, None, :] + lab[None, :, None, None, :] \
+        - lab[None, None, :, None, :] - lab[None, None, None, :, :]
+    tot[..., 0] %= mod
+    V = np.where(np.any(tot != 0, axis=-1), 0.0, V)
+    F = np.diag(np.diag(F))
+    o, v = slice(0, h), slice(h, n)
+    fock = [F[v, v].copy(), F[v, o].copy(), F[o, o].copy()]
+    dense = [V[v, v, v, v].copy(), V[v, v, v, o].copy(), V[v, v, o, o].copy(),
+             V[v, o, v, o].copy(), V[v, o, o, o].copy(), V[o, o, o, o].copy()]
+    return fock, dense, lab[h:], lab[:h], np.array([mod, 0])
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_sector_step_on_random_labelled_hamiltonians(seed):
+    """Beyond plane waves: random label-conserving Hamiltonians (labels mod 4 and unbounded,
+    self-conjugate crossed sectors included) stepped by the sector statement against the
+    oracle's dense t2Iter (1e-12), t1 staying exactly 0 in the oracle."""
+    p, h = 10, 4
+    fock, v, lp, lh, mod = _random_labelled_hamiltonian(p, h, seed)
+    lay = SectorLayout(lp, lh, mod)
+    hst = pack_hamiltonian(lay, fock, v)
+    t2 = _random_t2(lay, seed + 7)
+    t1 = np.zeros((p, h))
+    f_pp, f_ph, f_hh = fock
+    v_pppp, v_ppph, v_pphh, v_phph, v_phhh, v_hhhh = v
+    assert not cc.t1Iter(t1, t2, f_ph, f_pp, f_hh, v_phph, v_phhh, v_pphh, v_ppph,
+                         sparse=False).any()
+    new2 = cc.t2Iter(t1, t2, f_ph, f_hh, f_pp, v_pppp, v_phph, v_phhh, v_pphh, v_ppph, v_hhhh,
+                     sparse=False)
+    ref = 0.5 * t2 + 0.5 * new2
+    np.testing.assert_allclose(lay.unpack(lay.pack(ref)), ref, atol=1e-15)
+    tn, e, _ = emu.step(lay, hst, lay.pack(t2), 0.5)
+    np.testing.assert_allclose(tn, lay.pack(ref), rtol=1e-12, atol=1e-14 * np.abs(ref).max())
+    assert e == pytest.approx(cc.ccsd_energy(f_ph, v_pphh, ref, t1), rel=1e-12)
+    # and the reverse sweep is the transpose there too
+    rng = np.random.default_rng(seed)
+    d, lam = rng.standard_normal(lay.n), rng.standard_normal(lay.n)
+    eps = 1e-3
+    tp = lay.pack(t2)
+    Jd = (emu.step(lay, hst, tp + eps * d, 0.5)[2] - emu.step(lay, hst, tp - eps * d, 0.5)[2]) / (2 * eps)
+    a, b = emu.residual_vjp(lay, hst, tp, lam)
+    assert abs(lam @ Jd - (a + b) @ d) <= 1e-10 * abs(lam @ Jd)
