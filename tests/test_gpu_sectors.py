@@ -265,3 +265,17 @@ def test_grouped_ring_path_equals_fused_ring():
         out.append((host(tn), float(Bf["E"].item())))
     np.testing.assert_allclose(out[1][0], out[0][0], rtol=1e-13, atol=1e-17)
     assert out[1][1] == pytest.approx(out[0][1], rel=1e-13)
+
+
+@pytest.mark.parametrize("diis,delta", [("soa", 0.0), ("legacy", 0.3)])
+def test_sector_solver_options_match_dense_solver(diis, delta):
+    """The SoA DIIS variant and a nonzero delta (shifted initial denominators) give the dense
+    solver's trajectory (L=3, A=4)."""
+    pw, fock, v = _pw(3, 4)
+    hd, hs = [], []
+    ccm.ccsd_solver(fock, v, eps=1e-10, maxSteps=40, diis=diis, delta=delta, history=hd)
+    ccm.ccsd_solver(fock, v, eps=1e-10, maxSteps=40, diis=diis, delta=delta, history=hs,
+                    sectors=pw.labels())
+    n = min(len(hd), len(hs))
+    np.testing.assert_allclose(hs[:n], hd[:n], rtol=1e-9)
+    assert abs(len(hd) - len(hs)) <= 1
