@@ -193,8 +193,11 @@ def diis_cost(p, h, ccm, torch, n=10):
                         E=torch.empty(1, dtype=torch.float64, device=dev))
     d = ccm._Diis("pulsed", n, t1, t2, False)
     cur = t2.clone()
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(0)
     for k in range(n):
-        new = cur * (1.0 - 0.1 / (k + 2)) + 1e-6 * (k + 1)
+        new = cur + 1e-4 * 0.5 ** k * torch.rand(cur.shape, generator=gen, dtype=cur.dtype,
+                                                 device=dev)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         ext = d.step(t1, new, t1, cur, B)
