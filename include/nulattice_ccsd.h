@@ -198,6 +198,20 @@ int nl_dot_part(int64_t n, const double* x, const double* y, double* part, void*
 int nl_axpby(int64_t n, double a, const double* x, double b, double* y, void* stream);
 int nl_sub(int64_t n, const double* x, const double* y, double* c, void* stream);
 
+/* Gram matrix of k <= 12 vectors of n elements in one pass over each: the DIIS B matrix
+ * (coupled_cluster.py:648-656, soa/coupled_cluster.py:536-543), which the reference forms pair by
+ * pair.  x is a host array of k device pointers; part holds k(k+1)/2 * nl_partials(n) doubles,
+ * pair q (a <= b, row by row: q = a*k - a*(a-1)/2 + b - a) at part[q*nl_partials(n)].
+ * nl_gram_finalize sums each pair's partials in a fixed order into out[0:npair]. */
+int nl_gram_part(int64_t n, int k, const double* const* x, double* part, void* stream);
+int nl_gram_finalize(int64_t nparts, int npair, const double* part, double* out, void* stream);
+
+/* y = sum_k coef[k] * x_k for k <= 16 vectors in one pass: the DIIS extrapolation
+ * (coupled_cluster.py:663-669).  coef: host array of k doubles; x: host array of k device
+ * pointers. */
+int nl_lincomb(int64_t n, int k, const double* coef, const double* const* x, double* y,
+               void* stream);
+
 /* ---- Symmetry-sector (block-sparse) form of the same iteration ----
  * For Hamiltonians whose single-particle states carry additive conserved labels that identify
  * them (plane waves: momentum mod L, spin and isospin projections), t2[a,b,i,j] is nonzero only

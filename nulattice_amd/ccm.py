@@ -46,15 +46,6 @@ def _t(x, device):
     return engine._dev(x, device)
 
 
-def _dot(x, y, B):
-    """x.y over all elements, on the GPU, returned to the host."""
-    n = x.numel()
-    _lib.call("nl_dot_part", n, ops._p(x), ops._p(y), ops._p(B.part), ops._stream())
-    nparts = int(_lib.load().nl_partials(n))
-    _lib.call("nl_finalize", nparts, ops._p(B.part), 0, None, None, ops._p(B.E), ops._stream())
-    return float(B.E.item())
-
-
 class _Diis:
     """Pulsed DIIS on device vectors; B-matrix solve in FP64 on the host (a 10x10 system)."""
 
@@ -62,8 +53,9 @@ class _Diis:
         self.mode, self.n, self.ccs = mode, max_diis, ccs
         self.h1, self.h2, self.e1, self.e2 = [ops.copy(t1)], [ops.copy(t2)], [], []
 
-    def step(self, t1, t2, o1, o2, B):
-        """Record (t, t - old); return extrapolated (t1, t2) or None."""
+    def step(self, t1, t2, o1, o2, B=None):
+        """Record (t, t - old); return extrapolated (t1, t2) or None.  ``B`` is unused (kept for
+        callers that pass reduction buffers; the Gram kernel allocates its own partials)."""
         soa = self.mode == "soa"
         self.h1.append(ops.copy(t1)); self.h2.append(ops.copy(t2))
         e1 = torch.empty_like(t1); e2 = torch.empty_like(t2)
@@ -79,27 +71,16 @@ class _Diis:
             for L in (self.h1, self.h2, self.e1, self.e2):
                 L.pop(0)
         n = len(self.e1)
-        Bm = np.empty((n, n))
-        for a in range(n):
-            for b in range(a, n):
-                Bm[a, b] = Bm[b, a] = _dot(self.e1[a], self.e1[b], B) + _dot(self.e2[a],
-                                                                               self.e2[b], B)
+        # B[a, b] = e_a . e_b over the concatenated (t1, t2) errors, each list read in one pass
+        Bm = ops.gram(self.e1) + ops.gram(self.e2)
         Bm = Bm / (np.abs(Bm).max() + (1e-16 if soa else 0.0))
         A = -np.ones((n + 1, n + 1)); A[:n, :n] = Bm; A[n, n] = 0.0
         rhs = np.zeros(n + 1); rhs[n] = -1.0
         out = None
         try:
             c = np.linalg.solve(A, rhs)[:n]
-            n1 = torch.empty_like(t1); n2 = torch.empty_like(t2)
-            ops.axpby(float(c[0]), self.h1[1], 0.0, n1)
-            for k in range(1, n):
-                ops.axpby(float(c[k]), self.h1[k + 1], 1.0, n1)
-            if not self.ccs:
-                ops.axpby(float(c[0]), self.h2[1], 0.0, n2)
-                for k in range(1, n):
-                    ops.axpby(float(c[k]), self.h2[k + 1], 1.0, n2)
-            else:
-                n2 = t2
+            n1 = ops.lincomb(c, self.h1[1:], torch.empty_like(t1))
+            n2 = t2 if self.ccs else ops.lincomb(c, self.h2[1:], torch.empty_like(t2))
             out = (n1, n2)
         except np.linalg.LinAlgError:
             pass

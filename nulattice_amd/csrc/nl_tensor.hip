@@ -654,6 +654,74 @@ __global__ __launch_bounds__(NT) void dot_part_kernel(int64_t n, const double* x
   if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
+// Gram matrix of K vectors in one pass: every element of every vector is read once and feeds
+// all K(K+1)/2 products.  part[q * gridDim.x + blockIdx.x] is this workgroup's share of
+// x_a . x_b, with q running over a <= b row by row (a = 0: b = 0..K-1, then a = 1, ...).
+constexpr int GRAM_MAX = 12;
+constexpr int LINCOMB_MAX = 16;
+
+struct GramArgs {
+  const double* x[GRAM_MAX];
+};
+
+template <int K>
+__global__ __launch_bounds__(NT) void gram_part_kernel(int64_t n, GramArgs g, double* part) {
+  constexpr int NP = K * (K + 1) / 2;
+  double acc[NP];
+#pragma unroll
+  for (int q = 0; q < NP; ++q) acc[q] = 0.0;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    double v[K];
+#pragma unroll
+    for (int a = 0; a < K; ++a) v[a] = g.x[a][t];
+#pragma unroll
+    for (int a = 0; a < K; ++a)
+#pragma unroll
+      for (int b = a; b < K; ++b) acc[a * K - a * (a - 1) / 2 + (b - a)] += v[a] * v[b];
+  }
+  __shared__ double red[NT / 64][NP];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < NP; ++q) {
+    const double s = wave_sum(acc[q]);
+    if (lane == 0) red[w][q] = s;
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < NP; q += NT) {
+    double s = 0.0;
+    for (int i = 0; i < NT / 64; ++i) s += red[i][q];
+    part[(int64_t)q * gridDim.x + blockIdx.x] = s;
+  }
+}
+
+// out[q] = sum(part[q*nparts : (q+1)*nparts]) in a fixed order; one workgroup per q.
+__global__ __launch_bounds__(NT) void gram_finalize_kernel(int64_t nparts, const double* part,
+                                                           double* out) {
+  const double* p = part + (int64_t)blockIdx.x * nparts;
+  double acc = 0.0;
+  for (int64_t k = threadIdx.x; k < nparts; k += NT) acc += p[k];
+  const double s = block_sum(acc);
+  if (threadIdx.x == 0) out[blockIdx.x] = s;
+}
+
+struct LincombArgs {
+  const double* x[LINCOMB_MAX];
+  double c[LINCOMB_MAX];
+};
+
+// y = sum_k c[k] x_k, one pass (reads K vectors, writes one)
+template <int K>
+__global__ __launch_bounds__(NT) void lincomb_kernel(int64_t n, LincombArgs g, double* y) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    double s = g.c[0] * g.x[0][t];
+#pragma unroll
+    for (int k = 1; k < K; ++k) s += g.c[k] * g.x[k][t];
+    y[t] = s;
+  }
+}
+
 // y = a*x + b*y
 __global__ void axpby_kernel(int64_t n, double a, const double* x, double b, double* y) {
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n;
@@ -1061,6 +1129,60 @@ int nl_sub(int64_t n, const double* x, const double* y, double* c, void* stream)
   if (n < 0 || !x || !y || !c) return NL_EARG;
   if (n == 0) return NL_OK;
   hipLaunchKernelGGL(sub_kernel, dim3(grid_for(n)), dim3(NT), 0, (hipStream_t)stream, n, x, y, c);
+  NL_LAUNCH_CHECK();
+  return NL_OK;
+}
+
+int nl_gram_part(int64_t n, int k, const double* const* x, double* part, void* stream) {
+  if (n < 0 || k < 1 || k > GRAM_MAX || !x || !part) return NL_EARG;
+  GramArgs g = {};
+  for (int a = 0; a < k; ++a) {
+    if (n > 0 && !x[a]) return NL_EARG;
+    g.x[a] = x[a];
+  }
+  const dim3 grid(grid_for(n, 4096)), block(NT);
+  hipStream_t s = (hipStream_t)stream;
+  switch (k) {
+#define NL_GRAM_CASE(K) \
+    case K: hipLaunchKernelGGL((gram_part_kernel<K>), grid, block, 0, s, n, g, part); break;
+    NL_GRAM_CASE(1) NL_GRAM_CASE(2) NL_GRAM_CASE(3) NL_GRAM_CASE(4) NL_GRAM_CASE(5)
+    NL_GRAM_CASE(6) NL_GRAM_CASE(7) NL_GRAM_CASE(8) NL_GRAM_CASE(9) NL_GRAM_CASE(10)
+    NL_GRAM_CASE(11) NL_GRAM_CASE(12)
+#undef NL_GRAM_CASE
+  }
+  NL_LAUNCH_CHECK();
+  return NL_OK;
+}
+
+int nl_gram_finalize(int64_t nparts, int npair, const double* part, double* out, void* stream) {
+  if (nparts < 1 || npair < 1 || !part || !out) return NL_EARG;
+  hipLaunchKernelGGL(gram_finalize_kernel, dim3(npair), dim3(NT), 0, (hipStream_t)stream, nparts,
+                     part, out);
+  NL_LAUNCH_CHECK();
+  return NL_OK;
+}
+
+int nl_lincomb(int64_t n, int k, const double* coef, const double* const* x, double* y,
+               void* stream) {
+  if (n < 0 || k < 1 || k > LINCOMB_MAX || !coef || !x || !y) return NL_EARG;
+  if (n == 0) return NL_OK;
+  LincombArgs g = {};
+  for (int a = 0; a < k; ++a) {
+    if (!x[a]) return NL_EARG;
+    g.x[a] = x[a];
+    g.c[a] = coef[a];
+  }
+  const dim3 grid(grid_for(n)), block(NT);
+  hipStream_t s = (hipStream_t)stream;
+  switch (k) {
+#define NL_LINCOMB_CASE(K) \
+    case K: hipLaunchKernelGGL((lincomb_kernel<K>), grid, block, 0, s, n, g, y); break;
+    NL_LINCOMB_CASE(1) NL_LINCOMB_CASE(2) NL_LINCOMB_CASE(3) NL_LINCOMB_CASE(4)
+    NL_LINCOMB_CASE(5) NL_LINCOMB_CASE(6) NL_LINCOMB_CASE(7) NL_LINCOMB_CASE(8)
+    NL_LINCOMB_CASE(9) NL_LINCOMB_CASE(10) NL_LINCOMB_CASE(11) NL_LINCOMB_CASE(12)
+    NL_LINCOMB_CASE(13) NL_LINCOMB_CASE(14) NL_LINCOMB_CASE(15) NL_LINCOMB_CASE(16)
+#undef NL_LINCOMB_CASE
+  }
   NL_LAUNCH_CHECK();
   return NL_OK;
 }

@@ -11,6 +11,7 @@ import ctypes
 import itertools
 import math
 
+import numpy as np
 import torch
 
 from . import _lib
@@ -109,6 +110,73 @@ def axpby(a, x, b, y):
     _lib.call("nl_axpby", y.numel(), ctypes.c_double(a), _p(x), ctypes.c_double(b), _p(y),
               _stream())
     return y
+
+
+GRAM_MAX, LINCOMB_MAX = 12, 16
+
+
+def _ptrs(ts):
+    return (ctypes.c_void_p * len(ts))(*[t.data_ptr() for t in ts])
+
+
+def gram(vs):
+    """Host FP64 matrix G[a, b] = vs[a] . vs[b] of contiguous device vectors of one size: one
+    nl_gram_part pass over up to 12 vectors; larger sets run as pairs of groups of 6 (each pair
+    one pass over its 12 vectors).  Fixed summation order: bitwise reproducible."""
+    k = len(vs)
+    if k == 0:
+        return np.zeros((0, 0))
+    n = vs[0].numel()
+    for v in vs:
+        _check(v, "v")
+        if v.numel() != n or not v.is_contiguous():
+            raise ValueError("gram: vectors must be contiguous and of one size")
+    G = np.empty((k, k))
+    if k <= GRAM_MAX:
+        sets = [list(range(k))]
+    else:
+        g = [list(range(s, min(s + GRAM_MAX // 2, k))) for s in range(0, k, GRAM_MAX // 2)]
+        sets = [g[i] + g[j] for i in range(len(g)) for j in range(i + 1, len(g))]
+    nparts = int(_lib.load().nl_partials(n))
+    for idx in sets:
+        m = len(idx)
+        npair = m * (m + 1) // 2
+        part = torch.empty(npair * nparts, dtype=torch.float64, device=vs[0].device)
+        out = torch.empty(npair, dtype=torch.float64, device=vs[0].device)
+        _lib.call("nl_gram_part", n, m, _ptrs([vs[i] for i in idx]), _p(part), _stream())
+        _lib.call("nl_gram_finalize", nparts, npair, _p(part), _p(out), _stream())
+        o = out.cpu().numpy()
+        q = 0
+        for a in range(m):
+            for b in range(a, m):
+                G[idx[a], idx[b]] = G[idx[b], idx[a]] = o[q]
+                q += 1
+    return G
+
+
+def lincomb(coef, vs, out):
+    """out = sum_k coef[k] * vs[k] (contiguous, same numel): one nl_lincomb pass over up to 16
+    vectors; longer sums continue in chunks that add to out."""
+    _check(out, "out")
+    vs = list(vs)
+    if len(coef) != len(vs) or not vs:
+        raise ValueError("lincomb: need one coefficient per vector")
+    n = out.numel()
+    for v in vs + [out]:
+        _check(v, "v")
+        if v.numel() != n or not v.is_contiguous():
+            raise ValueError("lincomb: vectors must be contiguous and of one size")
+    k0 = 0
+    while k0 < len(vs):
+        if k0 == 0:
+            xs, cs = list(vs[:LINCOMB_MAX]), [float(c) for c in coef[:LINCOMB_MAX]]
+        else:
+            xs = [out] + list(vs[k0:k0 + LINCOMB_MAX - 1])
+            cs = [1.0] + [float(c) for c in coef[k0:k0 + LINCOMB_MAX - 1]]
+        _lib.call("nl_lincomb", n, len(xs), (ctypes.c_double * len(cs))(*cs), _ptrs(xs), _p(out),
+                  _stream())
+        k0 += len(xs) - (0 if k0 == 0 else 1)
+    return out
 
 
 # ----------------------------------------------------------------------------- contract

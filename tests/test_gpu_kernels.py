@@ -340,3 +340,52 @@ def test_error_codes():
                           None, 8, 0, 1, 1, None, None) == -1
     assert lib.nl_contract_f64(None, 1.0, None, None, None, None, 0.0, None, None, None, 1, None,
                                None) == -1
+
+
+@pytest.mark.parametrize("k,n", [(1, 1000), (5, 1), (10, 300_001), (12, 4096 * 256 + 3),
+                                 (14, 5000), (20, 777)])
+def test_gram_matches_float64_products(k, n):
+    """nl_gram_part / nl_gram_finalize (through ops.gram): every pair a <= b of up to 12 vectors in
+    one pass, larger sets as pairs of groups; equals V V^T of torch FP64 to rounding, symmetric,
+    and bitwise the same on a second call."""
+    g = torch.Generator(device="cuda").manual_seed(k * 7 + n)
+    vs = [torch.randn(n, generator=g, dtype=torch.float64, device="cuda") for _ in range(k)]
+    G = ops.gram(vs)
+    V = torch.stack(vs)
+    np.testing.assert_allclose(G, host(V @ V.T), rtol=1e-12, atol=1e-13 * n)
+    assert np.array_equal(G, G.T)
+    assert np.array_equal(G, ops.gram(vs))
+
+
+def test_gram_pair_order_on_exact_values():
+    """Integer vectors (exact sums): G[a, b] = (a+1)(b+1) n + a b, so a mixed-up pair index fails."""
+    n, k = 10_000, 12
+    ar = torch.arange(n, device="cuda", dtype=torch.float64)
+    vs = [(a + 1) * torch.ones(n, dtype=torch.float64, device="cuda") + a * (ar == 5).double()
+          for a in range(k)]
+    G = ops.gram(vs)
+    ref = np.array([[(a + 1) * (b + 1) * n + a * (b + 1) + b * (a + 1) + a * b
+                     for b in range(k)] for a in range(k)], dtype=np.float64)
+    assert np.array_equal(G, ref)
+
+
+@pytest.mark.parametrize("k,n", [(1, 5), (7, 100_003), (16, 4097), (17, 999), (33, 2049)])
+def test_lincomb_matches_float64_sum(k, n):
+    """nl_lincomb (through ops.lincomb): sum_k c_k x_k in one pass over up to 16 vectors, longer
+    sums in chained chunks; equals the torch FP64 sum to rounding."""
+    g = torch.Generator(device="cuda").manual_seed(k + 3 * n)
+    vs = [torch.randn(n, generator=g, dtype=torch.float64, device="cuda") for _ in range(k)]
+    c = np.linspace(-1.5, 2.0, k)
+    out = torch.full((n,), 7.0, dtype=torch.float64, device="cuda")
+    ops.lincomb(c, vs, out)
+    ref = sum(float(ck) * v for ck, v in zip(c, vs))
+    np.testing.assert_allclose(host(out), host(ref), rtol=1e-13, atol=1e-13 * k)
+
+
+def test_gram_and_lincomb_argument_errors():
+    lib = _lib.load()
+    assert lib.nl_gram_part(10, 13, None, None, None) == -1
+    assert lib.nl_gram_part(10, 0, None, None, None) == -1
+    assert lib.nl_gram_finalize(0, 1, None, None, None) == -1
+    assert lib.nl_lincomb(10, 17, None, None, None, None) == -1
+    assert lib.nl_lincomb(10, 0, None, None, None, None) == -1
