@@ -16,7 +16,6 @@ import argparse
 import json
 import math
 import os
-import platform
 import sys
 import time
 
@@ -300,8 +299,14 @@ def cpu_baseline(mode, seed, D0):
                        f"L={L} A=28 took {sec:.1f} s on {cores} host cores"
                        + ("" if L == 5 else f"; scaled to L=5 by the canonical FLOP ratio "
                                             f"{scale:.2f}")),
-            "host_cpu": platform.processor() or _cpu_model(),
-            "torch_threads": None}
+            "host_cpu": _cpu_model(), "os_cpu_count": os.cpu_count(), "affinity_cores": cores,
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS"),
+            "torch_threads": _torch_threads()}
+
+
+def _torch_threads():
+    import torch
+    return torch.get_num_threads()
 
 
 def _cpu_model():
@@ -360,6 +365,13 @@ def main():
         print(f"# convergence: {dE}", file=sys.stderr, flush=True)
         dE["seeds"] = seed_errors(a, ccm, engine, torch, e64_seed0=e64,
                                   lp_seed0=infolp.get("E_lowprec", elp))
+        # the other 16-bit type's fixed point, stopped as the seeds' runs are
+        other = "fp16" if a.prec == "bf16" else "bf16"
+        eo, _, _ = ccm.ccsd_solver(fock, v, eps=1e-7, maxSteps=40, max_diis=0, hamiltonian=H,
+                                   precision=other, refine=False)
+        dE[f"dE_rel_lowprec_{other}"] = abs(eo - e64) / abs(e64)
+        print(f"# {other} fixed point: |dE|/|E| = {dE[f'dE_rel_lowprec_{other}']:.2e}",
+              file=sys.stderr, flush=True)
         dE["diis_ms_per_extrapolation"] = diis_cost(p, h, ccm, torch)
         print(f"# DIIS extrapolation: {dE['diis_ms_per_extrapolation']:.1f} ms",
               file=sys.stderr, flush=True)
@@ -405,6 +417,17 @@ def main():
         grad = gradient_check(a.L, a.A, a.prec, engine, ccm, adjoint)
         sector["adjoint_gradient"] = sector_gradient(a.L, a.A, adjoint, torch, grad)
     cpu = cpu_baseline(a.cpu_sample, a.seed, a.D0)
+    # whole-iteration roofline fraction: sum over kernel classes of canonical work / peak, divided
+    # by the measured iteration time (canonical FLOP at the MFMA peak of the regime, plus the
+    # minimal T2 traffic: read R2, t2, v_pphh, write t2, R2 = 5 p^2 h^2 FP64 words at 8 TB/s)
+    cflop = canonical_flops(p, h, len(v[0][1]), len(v[1][1]))
+    cbytes = 5.0 * p * p * h * h * 8
+    whole = {"lowprec_frac": (cflop / 2.5e15 + cbytes / 8e12) / slp,
+             "fp64_frac": (cflop / 78.6e12 + cbytes / 8e12) / s64,
+             "canonical_bytes_per_iter": cbytes,
+             "note": "canonical work of the reference term list; the regrouped schedule does less "
+                     "(A22 and A37 enter the ring intermediate W), so the FP64 fraction can "
+                     "exceed 1"}
     out = {
         "metric": f"CCSD iters/sec at L={a.L} A={a.A}: mixed-prec vs FP64 GPU, |dE| vs FP64 oracle",
         "value": 1.0 / slp, "unit": "iters/s", "n_gpus": 1, "steps": a.steps,
@@ -416,7 +439,8 @@ def main():
                    "L": a.L, "A": a.A, "p": p, "h": h, "C0": C0, "D0": a.D0, "seed": a.seed},
         "fp64_iters_per_s": 1.0 / s64, "fp64_ms_per_step": s64 * 1e3,
         "speedup_vs_fp64": s64 / slp,
-        "canonical_flop_per_iter": canonical_flops(p, h, len(v[0][1]), len(v[1][1])),
+        "canonical_flop_per_iter": cflop,
+        "whole_iteration": whole,
         "roofline": roof, "cpu_baseline": cpu, "adjoint_gradient": grad,
         "sector_form": sector,
     }
