@@ -35,8 +35,9 @@ def parse():
     ap.add_argument("--prec", default="bf16", choices=["bf16", "fp16"])
     ap.add_argument("--no-converge", action="store_true", help="skip the |dE| convergence runs")
     ap.add_argument("--cpu-sample", default="L4", choices=["L4", "none", "full"])
-    ap.add_argument("--form", default="all", choices=["all", "sectors"],
-                    help="sectors: time only the symmetry-sector form and print its line")
+    ap.add_argument("--form", default="all", choices=["all", "sectors", "diis"],
+                    help="sectors: time only the symmetry-sector form and print its line; "
+                         "diis: time only one DIIS extrapolation over 10 T2-sized vectors")
     ap.add_argument("--grad", default="on", choices=["on", "off"],
                     help="adjoint dE/dC0, dE/dD0 at C0=-4, D0=1 checked against central FD")
     return ap.parse_args()
@@ -180,16 +181,12 @@ def gradient_check(L, A, prec, engine, ccm, adjoint):
 
 def diis_cost(p, h, ccm, torch, n=10):
     """Wall time of one pulsed-DIIS extrapolation over n T2-sized vectors (the n-th recorded
-    step: n(n+1)/2 device dot products, the host solve and the extrapolation), which the
-    converging runs pay once every n steps."""
-    from types import SimpleNamespace
-    from nulattice_amd import _lib
+    step: its copy and error vector, the n(n+1)/2 B-matrix entries in one Gram pass over the
+    error vectors, the host solve, the one-pass linear combination and the history restart),
+    which the converging runs pay once every n steps."""
     dev = "cuda"
     t1 = torch.zeros((p, h), dtype=torch.float64, device=dev)
     t2 = torch.full((p, p, h, h), 1e-3, dtype=torch.float64, device=dev)
-    parts = int(_lib.load().nl_partials(p * p * h * h))
-    B = SimpleNamespace(part=torch.empty(2 * parts + 1, dtype=torch.float64, device=dev),
-                        E=torch.empty(1, dtype=torch.float64, device=dev))
     d = ccm._Diis("pulsed", n, t1, t2, False)
     cur = t2.clone()
     gen = torch.Generator(device=dev)
@@ -199,7 +196,7 @@ def diis_cost(p, h, ccm, torch, n=10):
                                                  device=dev)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        ext = d.step(t1, new, t1, cur, B)
+        ext = d.step(t1, new, t1, cur)
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
         cur = new
@@ -335,6 +332,12 @@ def main():
     t0 = time.perf_counter()
     if a.form == "sectors":
         print(json.dumps(sectors_only(a, C0, torch, ccm)))
+        return
+    if a.form == "diis":
+        p, h = 4 * a.L ** 3 - a.A, a.A
+        ms = diis_cost(p, h, ccm, torch)
+        print(json.dumps({"diis_ms_per_extrapolation": ms, "vectors": 10, "L": a.L, "A": a.A,
+                          "t2_bytes": p * p * h * h * 8}))
         return
     E0, fock, v = planewave_hamiltonian(a.L, a.A, C0, a.D0)
     H = engine.Hamiltonian(fock, v, device="cuda")
