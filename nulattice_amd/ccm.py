@@ -233,7 +233,8 @@ def _solver_ham(fock_mats, two_body_int, device):
 def ccsd_solver(fock_mats, two_body_int, t1initial=None, eps=1e-8, maxSteps=1000, max_diis=10,
                 delta=0, mixing=0.5, verbose=False, sparse=True, ccs=False, *, device=None,
                 diis="legacy", precision="fp64", refine=True, eps_lowprec=None, history=None,
-                info=None, hamiltonian=None, adjoint=False, lambda_eps=1e-10, sectors=None):
+                info=None, hamiltonian=None, adjoint=False, lambda_eps=1e-10, sectors=None,
+                _soa_rules=False):
     """Damped-Jacobi CCSD (coupled_cluster.py:547-687) on one MI355X.
 
     Returns (E_corr, t1, t2); like the reference, returns None after a divergence or NaN.
@@ -247,6 +248,7 @@ def ccsd_solver(fock_mats, two_body_int, t1initial=None, eps=1e-8, maxSteps=1000
     to the rounding level of the contractions) and then, if ``refine``, continues in FP64 to
     ``eps``.  ``adjoint=True`` also solves the CC Lambda equations at the returned amplitudes
     (to |residual| <= ``lambda_eps`` |dE/dt|) and stores (lambda1, lambda2) in info["lambda"].
+    ``_soa_rules`` applies the SoA driver's stopping and failure rules instead (see soa.py).
     """
     if adjoint and info is None:
         raise ValueError("adjoint=True needs an info dict to receive the Lambda amplitudes")
@@ -259,6 +261,8 @@ def ccsd_solver(fock_mats, two_body_int, t1initial=None, eps=1e-8, maxSteps=1000
     if sectors is not None and hamiltonian is None:
         hamiltonian = sec.SectorHamiltonian(fock_mats, two_body_int, *sectors, device=dev)
     if isinstance(hamiltonian, sec.SectorHamiltonian):
+        if _soa_rules:
+            raise ValueError("the sector form follows the legacy driver's rules")
         if precision != "fp64" or adjoint:
             raise ValueError("the sector form runs in FP64 without the adjoint")
         return sec.solve(hamiltonian, t1initial, eps, maxSteps, max_diis, delta, mixing, verbose,
@@ -312,7 +316,10 @@ def ccsd_solver(fock_mats, two_body_int, t1initial=None, eps=1e-8, maxSteps=1000
         old1, old2 = t1, t2
         t1, t1n = t1n, t1
         t2, t2n = t2n, t2
-        rel = abs(energy - prev) / abs(energy) if energy != 0 else 0.0
+        # legacy: |dE|/|E| (coupled_cluster.py:629);
+        # SoA: |dE|/max(1, |E|) (soa/coupled_cluster.py:517)
+        den = max(1.0, abs(energy)) if _soa_rules else abs(energy)
+        rel = abs(energy - prev) / den if den != 0 else 0.0
         converged = (energy == 0 and prev == 0) or rel < eps
         if lp_phase:
             # the low-precision iteration settles into rounding jitter instead of converging:
@@ -331,7 +338,7 @@ def ccsd_solver(fock_mats, two_body_int, t1initial=None, eps=1e-8, maxSteps=1000
             i += 1
             continue
         if converged:
-            if verbose:
+            if verbose and not _soa_rules:
                 print(f"Energy found in {i + 1} iterations")
             _fill_info(info, steps_lp, steps_fp64, t_start, energy, True)
             if adjoint:
@@ -342,6 +349,11 @@ def ccsd_solver(fock_mats, two_body_int, t1initial=None, eps=1e-8, maxSteps=1000
             if ext is not None:
                 t1, t2 = ext
                 t1n, t2n = torch.empty_like(t1), torch.empty_like(t2)
+        if _soa_rules and (abs(energy) > 1e10 or math.isnan(energy)):
+            # SoA (soa/coupled_cluster.py:569-575): report, leave the loop, return the last
+            # energy and amplitudes
+            print("Diverged.")
+            break
         if abs(energy) > 1e10:
             print("Iteration Diverged")
             _fill_info(info, steps_lp, steps_fp64, t_start, energy)
@@ -352,7 +364,7 @@ def ccsd_solver(fock_mats, two_body_int, t1initial=None, eps=1e-8, maxSteps=1000
             return None
         prev = energy
         i += 1
-    print("Max Iterations Reached")
+    print("Max iterations reached." if _soa_rules else "Max Iterations Reached")
     _fill_info(info, steps_lp, steps_fp64, t_start, energy, False)
     if adjoint:
         _lambda(ham, t1, t2, B, mixing, lambda_eps, info)

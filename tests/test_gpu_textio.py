@@ -31,7 +31,7 @@ def test_deuteron_trajectory_on_gpu(form):
 
 def test_deuteron_bf16_form_refines():
     """The low-precision form on the same files lands on the FP64 fixed point after refinement
-    (plain Jacobi diverges on this Hamiltonian; the pulsed DIIS converges it).  The stopping rule
+    (the BF16 iteration without DIIS does not settle on this Hamiltonian; the pulsed DIIS does).  The stopping rule
     is a relative energy change below eps, which an extrapolation can meet a few 1e-9 short of the
     fixed point at eps = 1e-11 (the oracle's own solve stops 3e-11 away there); eps = 1e-13 puts
     both solves within ~1e-12 of it."""

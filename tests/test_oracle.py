@@ -231,3 +231,29 @@ def test_dense_and_sparse_solvers_agree():
     cc.ccsd_solver(fs, vs, maxSteps=6, max_diis=0, sparse=True, history=hs)
     cc.ccsd_solver(fd, vd, maxSteps=6, max_diis=0, sparse=False, history=hd)
     assert _rel(hs, hd).max() < 1e-13
+
+
+def _pw_lists(L, A, C0):
+    from nulattice_amd.planewave import planewave_hamiltonian
+    E0, fock, v = planewave_hamiltonian(L, A, C0)
+    vl = [[[int(x) for x in i] + [float(val)] for i, val in zip(*v[0])],
+          [[int(x) for x in i] + [float(val)] for i, val in zip(*v[1])]] + v[2:]
+    return fock, v, vl
+
+
+def test_oracle_failure_paths(capsys):
+    """The oracle's failure semantics are the reference driver's (coupled_cluster.py:675-687):
+    |E| > 1e10 prints "Iteration Diverged" and returns None (forced by over-relaxation,
+    mixing = -20, which makes t <- -20 t + 21 t_new); a NaN energy prints "nan error" and returns
+    None (forced by a degenerate Fock diagonal f_aa = f_ii with delta = 0: 0/0 in the initial
+    amplitudes).  The GPU drivers are tested on the same two inputs."""
+    fock, _, vl = _pw_lists(2, 4, -4.0)
+    h = []
+    assert cc.ccsd_solver(fock, vl, max_diis=0, maxSteps=100, mixing=-20.0, history=h) is None
+    assert abs(h[-1]) > 1e10 and len(h) < 100
+    assert "Iteration Diverged" in capsys.readouterr().out
+    f2 = [x.copy() for x in fock]
+    f2[0][0, 0] = f2[2][0, 0]
+    with np.errstate(invalid="ignore", divide="ignore"):
+        assert cc.ccsd_solver(f2, vl, max_diis=0, maxSteps=100) is None
+    assert "nan error" in capsys.readouterr().out
