@@ -78,3 +78,17 @@ def test_failure_semantics_of_both_drivers(case, capsys):
         assert abs(e) > 1e10 and e == hs[-1]
     else:
         assert np.isnan(e) and np.isnan(hs[-1])
+
+
+def test_legacy_and_soa_drivers_agree_on_16O():
+    """tests/CCM/test_ccm_equivalence.py:29-61 on the GPU: 16O at L=2 (vT1 = vS1 = -9, w3 = 6,
+    a = 2.5), eps = 1e-8, maxSteps = 500, max_diis = 10: the legacy driver (damped Jacobi, its
+    DIIS block never runs) and the SoA driver (pulsed DIIS) give total energies within 1e-5 MeV."""
+    from oracle import lattice_ref as lat
+    from nulattice_amd import hamiltonian as hm
+    E0, fock, v = hm.get_norm_ord_int(2, lat.ref_16O_gs, -9.0, -9.0, 6.0, sparse=True)
+    e_old, _, _ = ccm.ccsd_solver(fock, v, eps=1e-8, maxSteps=500, max_diis=10, sparse=True)
+    e_new, _, _ = soa.ccsd_solver(fock, v, eps=1e-8, maxSteps=500, max_diis=10, sparse=True,
+                                  device="cuda")
+    unit = lat.phys_unit(2.5)
+    assert abs((e_old + E0) - (e_new + E0)) * unit < 1e-5
