@@ -364,7 +364,11 @@ def main():
               "dE_rel_refined": abs(elp - e64) / abs(e64),
               "steps_fp64_converge": info64.get("steps_fp64"),
               "steps_lowprec": infolp.get("steps_lowprec"),
-              "steps_refine_fp64": infolp.get("steps_fp64")}
+              "steps_refine_fp64": infolp.get("steps_fp64"),
+              # wall time to the converged energy (eps 1e-10, plain Jacobi): FP64 from the
+              # start, and the low-precision iteration followed by its FP64 refinement
+              "seconds_fp64_converge": info64.get("seconds"),
+              "seconds_lowprec_refined": infolp.get("seconds")}
         print(f"# convergence: {dE}", file=sys.stderr, flush=True)
         dE["seeds"] = seed_errors(a, ccm, engine, torch, e64_seed0=e64,
                                   lp_seed0=infolp.get("E_lowprec", elp))
