@@ -60,16 +60,17 @@ def time_iterations(ccm_mod, engine, ops, H, prec, steps, warmup, torch):
     t1n, t2n = torch.empty_like(t1), torch.empty_like(t2)
     engine.init_amplitudes(H, B, t1, t2, 0.0)
     P = ccm_mod._PREC[prec]
+    img = engine.T2Image()        # as the solver does: the assembly writes the next BF16 image
     energies = []
     for _ in range(warmup):
-        engine.update(H, t1, t2, B, t1n, t2n, 0.5, P)
+        engine.update(H, t1, t2, B, t1n, t2n, 0.5, P, img=img)
         energies.append(float(B.E.item()))
         t1, t1n, t2, t2n = t1n, t1, t2n, t2
     torch.cuda.synchronize()
     ops.TIMERS.clear(); ops.TIMERS["on"] = True
     t0 = time.perf_counter()
     for _ in range(steps):
-        engine.update(H, t1, t2, B, t1n, t2n, 0.5, P)
+        engine.update(H, t1, t2, B, t1n, t2n, 0.5, P, img=img)
         energies.append(float(B.E.item()))
         t1, t1n, t2, t2n = t1n, t1, t2n, t2
     torch.cuda.synchronize()

@@ -173,6 +173,16 @@ int nl_assemble_t2(int64_t p, int64_t h, const double* S0, const double* Sab, co
                    const double* vpphh, const double* t1n, double mixing, double* t2n,
                    double* H2out, double* part, void* stream);
 
+/* nl_assemble_t2 that also writes t2n16[t] = BF16(t2n[t]) (16-bit elements, t2n's layout,
+ * rounded to nearest even through FP32, bit for bit the cast of nl_convert_rows): the BF16 image
+ * of the new amplitudes that the next iteration's BF16 products read (coupled_cluster.py:602-623
+ * feeding :442-545), written while the value is in a register instead of by a later cast pass
+ * over t2. */
+int nl_assemble_t2_16(int64_t p, int64_t h, const double* S0, const double* Sab,
+                      const double* Sij, const double* Spp, const double* Xpp, const double* Xhh,
+                      const double* t2, const double* vpphh, const double* t1n, double mixing,
+                      double* t2n, double* H2out, double* part, void* t2n16, void* stream);
+
 /* Transpose of nl_assemble_t2's antisymmetrisers, for the Lambda (adjoint) equations of t2Iter
  * (coupled_cluster.py:536-545): given the adjoint L[a,b,i,j] of H2,
  *   gSab = pAB(L) as [a][b][i][j];  gSij = pIJ(L), gSpp = pIJ(pAB(L)) as [(a,i)][(b,j)].

@@ -44,6 +44,8 @@ SIGNATURES = {
     "nl_update_t1": (_i32, [_i64, _i64, _vp, _vp, _vp, _vp, _dbl, _vp, _vp]),
     "nl_assemble_t2": (_i32, [_i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _dbl,
                               _vp, _vp, _vp, _vp]),
+    "nl_assemble_t2_16": (_i32, [_i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _dbl,
+                                 _vp, _vp, _vp, _vp, _vp]),
     "nl_assemble_t2_adj": (_i32, [_i64, _i64, _vp, _vp, _vp, _vp, _vp]),
     "nl_partials": (_i64, [_i64]),
     "nl_energy_part": (_i32, [_i64, _i64, _vp, _vp, _vp, _vp, _vp]),

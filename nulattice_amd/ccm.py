@@ -281,6 +281,7 @@ def ccsd_solver(fock_mats, two_body_int, t1initial=None, eps=1e-8, maxSteps=1000
         t2.zero_()
     pulsed = diis != "legacy" and max_diis > 0
     dstate = _Diis(diis, max_diis, t1, t2, ccs) if pulsed else None
+    img = engine.T2Image()      # BF16 image of t2 handed from one step's assembly to the next
     engine.energy(ham, t1, t2, B)
     prev = float(B.E.item())
     if history is not None:
@@ -303,7 +304,7 @@ def ccsd_solver(fock_mats, two_body_int, t1initial=None, eps=1e-8, maxSteps=1000
             ops.permute_into(t2, t2n)
             engine.energy(ham, t1n, t2n, B)
         else:
-            engine.update(ham, t1, t2, B, t1n, t2n, mixing, prec)
+            engine.update(ham, t1, t2, B, t1n, t2n, mixing, prec, img=img)
         energy = float(B.E.item())
         if lp_phase:
             steps_lp += 1

@@ -475,6 +475,7 @@ struct AssembleArgs {
   double* t2n;                          // new t2 (may alias nothing)
   double* part;                         // per-block partials: [3][gridDim]: E2, |H2|^2, spare
   double* H2out;                        // optional: raw residual H2 (for adjoint / diagnostics)
+  uint16_t* t2n16;                      // optional: BF16 image of t2n (same layout, RNE via FP32)
 };
 
 __global__ __launch_bounds__(NT) void assemble_t2_kernel(AssembleArgs g) {
@@ -498,6 +499,7 @@ __global__ __launch_bounds__(NT) void assemble_t2_kernel(AssembleArgs g) {
     double nt = told + H / denom;
     double tn = g.mixing * told + (1.0 - g.mixing) * nt;
     if (g.t2n) g.t2n[t] = tn;
+    if (g.t2n16) g.t2n16[t] = to_bf16_bits((float)tn);
     if (g.H2out) g.H2out[t] = H;
     if (g.vpphh) {
       double v = g.vpphh[t];
@@ -532,6 +534,7 @@ __global__ __launch_bounds__(NT) void assemble_pair_kernel(AssembleArgs g) {
     double told = g.t2[t];
     double tn = g.mixing * told + (1.0 - g.mixing) * (told + H / denom);
     if (g.t2n) g.t2n[t] = tn;
+    if (g.t2n16) g.t2n16[t] = to_bf16_bits((float)tn);
     if (g.H2out) g.H2out[t] = H;
     if (g.vpphh) {
       double v = g.vpphh[t];
@@ -1041,10 +1044,11 @@ int nl_update_t1(int64_t p, int64_t h, const double* t1, const double* H1, const
 // Number of partial-sum slots (per quantity) nl_assemble_t2 / nl_energy / nl_dot use.
 int64_t nl_partials(int64_t n) { return grid_for(n, 4096); }
 
-int nl_assemble_t2(int64_t p, int64_t h, const double* S0, const double* Sab, const double* Sij,
-                   const double* Spp, const double* Xpp, const double* Xhh, const double* t2,
-                   const double* vpphh, const double* t1n, double mixing, double* t2n,
-                   double* H2out, double* part, void* stream) {
+static int assemble_t2_impl(int64_t p, int64_t h, const double* S0, const double* Sab,
+                            const double* Sij, const double* Spp, const double* Xpp,
+                            const double* Xhh, const double* t2, const double* vpphh,
+                            const double* t1n, double mixing, double* t2n, double* H2out,
+                            double* part, uint16_t* t2n16, void* stream) {
   if (p < 0 || h < 0 || !Xpp || !Xhh || !t2 || !part) return NL_EARG;
   if (vpphh && !t1n) return NL_EARG;
   int64_t n = p * p * h * h;
@@ -1052,7 +1056,7 @@ int nl_assemble_t2(int64_t p, int64_t h, const double* S0, const double* Sab, co
   AssembleArgs g;
   g.p = p; g.h = h; g.S0 = S0; g.Sab = Sab; g.Sij = Sij; g.Spp = Spp; g.Xpp = Xpp; g.Xhh = Xhh;
   g.t2 = t2; g.vpphh = vpphh; g.t1n = t1n; g.mixing = mixing; g.t2n = t2n; g.part = part;
-  g.H2out = H2out;
+  g.H2out = H2out; g.t2n16 = t2n16;
   g.fp = nl_fastdiv(p); g.fh = nl_fastdiv(h);
   if (h <= AH && (Sij || Spp) && p * p < INT32_MAX)
     hipLaunchKernelGGL(assemble_pair_kernel, dim3(grid_for(n, 4096)), dim3(NT), 0,
@@ -1062,6 +1066,23 @@ int nl_assemble_t2(int64_t p, int64_t h, const double* S0, const double* Sab, co
                        (hipStream_t)stream, g);
   NL_LAUNCH_CHECK();
   return NL_OK;
+}
+
+int nl_assemble_t2(int64_t p, int64_t h, const double* S0, const double* Sab, const double* Sij,
+                   const double* Spp, const double* Xpp, const double* Xhh, const double* t2,
+                   const double* vpphh, const double* t1n, double mixing, double* t2n,
+                   double* H2out, double* part, void* stream) {
+  return assemble_t2_impl(p, h, S0, Sab, Sij, Spp, Xpp, Xhh, t2, vpphh, t1n, mixing, t2n, H2out,
+                          part, nullptr, stream);
+}
+
+int nl_assemble_t2_16(int64_t p, int64_t h, const double* S0, const double* Sab,
+                      const double* Sij, const double* Spp, const double* Xpp, const double* Xhh,
+                      const double* t2, const double* vpphh, const double* t1n, double mixing,
+                      double* t2n, double* H2out, double* part, void* t2n16, void* stream) {
+  if (!t2n16) return NL_EARG;
+  return assemble_t2_impl(p, h, S0, Sab, Sij, Spp, Xpp, Xhh, t2, vpphh, t1n, mixing, t2n, H2out,
+                          part, (uint16_t*)t2n16, stream);
 }
 
 int nl_assemble_t2_adj(int64_t p, int64_t h, const double* L, double* gSab, double* gSij,

@@ -427,20 +427,45 @@ def _block_gemm(pb, src, out, alpha, ncol, tag, precision="fp64", src16=None):
         ev[1].record()
 
 
-def update(ham, t1, t2, B, t1n, t2n, mixing, prec=DEFAULT_PREC, H2out=None):
-    """One Jacobi step (coupled_cluster.py:602-623): writes t1n, t2n and B.E[0] = E(t1n, t2n)."""
+class T2Image:
+    """The BF16 image of t2 carried from one step's assembly to the next step's BF16 products
+    (pass the same object to every ``update`` of a solve).  Two buffers alternate: the one the
+    current step reads and the one its assembly writes.  ``src`` is the tensor object whose
+    image the current buffer holds; any other t2 (a DIIS extrapolation, a fresh start) is cast
+    as before."""
+
+    def __init__(self):
+        self.bufs = [None, None]
+        self.cur = 0
+        self.src = None
+
+    def buffer(self, k, n, dev):
+        b = self.bufs[k]
+        if b is None or b.numel() < n or b.device != dev:
+            b = torch.empty(n, dtype=torch.int16, device=dev)
+            self.bufs[k] = b
+        return b[:n]
+
+
+def update(ham, t1, t2, B, t1n, t2n, mixing, prec=DEFAULT_PREC, H2out=None, img=None):
+    """One Jacobi step (coupled_cluster.py:602-623): writes t1n, t2n and B.E[0] = E(t1n, t2n).
+    ``img`` (a T2Image) lets the assembly write the BF16 image of t2n that the next step's BF16
+    products read, instead of that step casting t2 again."""
     p, h = ham.p, ham.h
     with ops.lp_reuse(t2):
         B.t2_16 = None
         try:
-            _t2_images(ham, t2, B, prec)
+            _t2_images(ham, t2, B, prec, img)
             t1_part(ham, t1, t2, B, prec)
             _lib.call("nl_update_t1", p, h, ops._p(t1), ops._p(B.H1), ops._p(B.X1pp),
                       ops._p(B.X1hh), ctypes.c_double(mixing), ops._p(t1n), ops._stream())
             t2_part(ham, t1, t2, B, prec)
         finally:
             B.t2_16 = None
-    assemble(ham, t2, B, t1n, t2n, mixing, H2out)
+    assemble(ham, t2, B, t1n, t2n, mixing, H2out,
+             img=img if _images_wanted(ham, prec) else None)
+    if img is not None and not _images_wanted(ham, prec):
+        img.src = None
 
 
 # BF16 images of t2 for the products that read it in BF16 (all from one cast of t2):
@@ -451,20 +476,31 @@ def update(ham, t1, t2, B, t1n, t2n, mixing, prec=DEFAULT_PREC, H2out=None):
 T2_IMAGES = True
 
 
-def _t2_images(ham, t2, B, prec):
-    """Register the BF16 images of t2 that this iteration's BF16 products read, made from a single
-    FP64 -> BF16 pass over t2 instead of one conversion of t2 per product; the FP64 t2 stays valid
-    for the FP64 products.  Needs t2 under ops.lp_reuse."""
+def _images_wanted(ham, prec):
     p, h = ham.p, ham.h
-    use = {k for k in ("dress", "hh", "ring") if prec.get(k) == "bf16"}
-    if not (T2_IMAGES and use) or p % 8 or (p * h) % 8 or p * p * h * h >= 2 ** 31 - 1:
+    use = any(prec.get(k) == "bf16" for k in ("dress", "hh", "ring"))
+    return bool(T2_IMAGES and use) and not (p % 8 or (p * h) % 8
+                                            or p * p * h * h >= 2 ** 31 - 1)
+
+
+def _t2_images(ham, t2, B, prec, img=None):
+    """Register the BF16 images of t2 that this iteration's BF16 products read, made from a single
+    FP64 -> BF16 pass over t2 instead of one conversion of t2 per product (or taken from ``img``
+    when the previous step's assembly wrote this t2's image); the FP64 t2 stays valid for the
+    FP64 products.  Needs t2 under ops.lp_reuse."""
+    p, h = ham.p, ham.h
+    if not _images_wanted(ham, prec):
         return
+    use = {k for k in ("dress", "hh", "ring") if prec.get(k) == "bf16"}
     n, ph, hh = p * p * h * h, p * h, h * h
     dev, st = t2.device, ops._stream()
-    S = ops.WS.get("t2_bf16", n, dtype=torch.int16, device=dev)       # [a][d][k][l]
-    s4 = (ctypes.c_int64 * 4)(0, p * hh, 0, 1)
-    _lib.call("nl_convert_rows", ops.BF16, 1, p, 1, p * hh, s4, ops._p(t2), ops._p(S), p * hh,
-              None, ctypes.c_float(0.0), st)
+    if img is not None and img.src is t2:
+        S = img.buffer(img.cur, n, dev)                               # [a][d][k][l]
+    else:
+        S = ops.WS.get("t2_bf16", n, dtype=torch.int16, device=dev)   # [a][d][k][l]
+        s4 = (ctypes.c_int64 * 4)(0, p * hh, 0, 1)
+        _lib.call("nl_convert_rows", ops.BF16, 1, p, 1, p * hh, s4, ops._p(t2), ops._p(S),
+                  p * hh, None, ctypes.c_float(0.0), st)
     if "dress" in use:
         ops.lp_provide(t2, "adkl", ["a"], ["d", "k", "l"], S, p * hh)
         B.t2_16 = S                      # the fused Sab product transposes it into its operand
@@ -481,12 +517,21 @@ def _t2_images(ham, t2, B, prec):
         ops.lp_provide(t2, "acik", ["a", "i"], ["c", "k"], T3, ph)
 
 
-def assemble(ham, t2, B, t1n, t2n, mixing, H2out=None):
+def assemble(ham, t2, B, t1n, t2n, mixing, H2out=None, img=None):
     p, h = ham.p, ham.h
     st = ops._stream()
-    _lib.call("nl_assemble_t2", p, h, ops._p(B.S0), ops._p(B.Sab), ops._p(B.Sij), ops._p(B.Spp),
-              ops._p(B.X2pp), ops._p(B.X2hh), ops._p(t2), ops._p(ham.v_pphh), ops._p(t1n),
-              ctypes.c_double(mixing), ops._p(t2n), ops._p(H2out), ops._p(B.part), st)
+    args = (p, h, ops._p(B.S0), ops._p(B.Sab), ops._p(B.Sij), ops._p(B.Spp), ops._p(B.X2pp),
+            ops._p(B.X2hh), ops._p(t2), ops._p(ham.v_pphh), ops._p(t1n),
+            ctypes.c_double(mixing), ops._p(t2n), ops._p(H2out), ops._p(B.part))
+    if img is not None and t2n is not None:
+        nxt = 1 - img.cur
+        buf = img.buffer(nxt, p * p * h * h, t2n.device)
+        _lib.call("nl_assemble_t2_16", *args, ops._p(buf), st)
+        img.cur, img.src = nxt, t2n
+    else:
+        _lib.call("nl_assemble_t2", *args, st)
+        if img is not None:
+            img.src = None
     nparts = int(_lib.load().nl_partials(p * p * h * h))
     _lib.call("nl_finalize", nparts, ops._p(B.part), p * h, ops._p(ham.f_ph), ops._p(t1n),
               ops._p(B.E), st)

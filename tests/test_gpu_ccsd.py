@@ -141,7 +141,8 @@ def test_bf16_images_are_bitwise_the_per_product_conversions():
     p, h = H.p, H.h
     assert p % 8 == 0 and (h * h) % 8 == 0 and p * p >= ops.LP_MIN_DIM
     res = {}
-    for flag in (True, False):
+    # (images on, carried by the assembly): the carried image is the cast's bit for bit
+    for flag, carry in ((True, False), (False, False), (True, True)):
         engine.TAU16 = engine.T2_IMAGES = flag
         try:
             B = engine.Buffers(p, h, H.device)
@@ -149,14 +150,18 @@ def test_bf16_images_are_bitwise_the_per_product_conversions():
             t2 = torch.zeros(p, p, h, h, dtype=torch.float64, device="cuda")
             engine.init_amplitudes(H, B, t1, t2, 0.0)
             t1n, t2n = torch.empty_like(t1), torch.empty_like(t2)
-            for _ in range(2):
-                engine.update(H, t1, t2, B, t1n, t2n, 0.5, ccm._PREC["bf16"])
+            img = engine.T2Image() if carry else None
+            for _ in range(3):
+                engine.update(H, t1, t2, B, t1n, t2n, 0.5, ccm._PREC["bf16"], img=img)
                 t1, t1n, t2, t2n = t1n, t1, t2n, t2
-            res[flag] = (t2.clone(), float(B.E.item()))
+            if carry:
+                assert img.src is t2
+            res[(flag, carry)] = (t2.clone(), float(B.E.item()))
         finally:
             engine.TAU16 = engine.T2_IMAGES = True
-    assert torch.equal(res[True][0], res[False][0])
-    assert res[True][1] == res[False][1]
+    for key in ((False, False), (True, True)):
+        assert torch.equal(res[(True, False)][0], res[key][0])
+        assert res[(True, False)][1] == res[key][1]
 
 
 def test_product_hamiltonian_on_gpu_solver():
