@@ -174,10 +174,10 @@ int nl_assemble_t2(int64_t p, int64_t h, const double* S0, const double* Sab, co
                    double* H2out, double* part, void* stream);
 
 /* nl_assemble_t2 that also writes t2n16[t] = BF16(t2n[t]) (16-bit elements, t2n's layout,
- * rounded to nearest even through FP32, bit for bit the cast of nl_convert_rows): the BF16 image
- * of the new amplitudes that the next iteration's BF16 products read (coupled_cluster.py:602-623
- * feeding :442-545), written while the value is in a register instead of by a later cast pass
- * over t2. */
+ * rounded FP64 -> FP32 -> BF16, each to nearest even: bit for bit the cast of nl_convert_rows):
+ * the BF16 image of the new amplitudes that the next iteration's BF16 products read
+ * (coupled_cluster.py:602-623 feeding :442-545), written while the value is in a register
+ * instead of by a later cast pass over t2. */
 int nl_assemble_t2_16(int64_t p, int64_t h, const double* S0, const double* Sab,
                       const double* Sij, const double* Spp, const double* Xpp, const double* Xhh,
                       const double* t2, const double* vpphh, const double* t1n, double mixing,

@@ -87,6 +87,16 @@ __device__ inline uint16_t to_bf16_bits(float f) {
   __bf16 b = (__bf16)f;  // round to nearest even
   return __builtin_bit_cast(uint16_t, b);
 }
+
+// BF16 of an FP64 value rounded twice, FP64 -> FP32 -> BF16 (each to nearest even), as the cast
+// kernels round it.  The FP32 value passes through an empty asm so that the compiler cannot fold
+// the two truncations into one FP64 -> BF16 rounding, which differs in rare ties-after-rounding
+// cases (an FP64 value whose FP32 rounding lands exactly halfway between two BF16 values).
+__device__ inline uint16_t to_bf16_bits_2r(double x) {
+  float f = (float)x;
+  asm volatile("" : "+v"(f));
+  return to_bf16_bits(f);
+}
 __device__ inline uint16_t to_f16_bits(float f) {
   _Float16 h = (_Float16)f;
   return __builtin_bit_cast(uint16_t, h);
@@ -370,8 +380,15 @@ __global__ __launch_bounds__(NT) void iscale_kernel(int64_t R, float target, flo
   }
 }
 
-// tau[a,b,i,j] = t2[a,b,i,j] + t1[a,i] t1[b,j] - t1[a,j] t1[b,i]: workgroups walk the pairs
-// (a, b), threads the h x h block (contiguous in t2 and tau).
+// tau[a,b,i,j] = t2[a,b,i,j] + t1[a,i] t1[b,j] - t1[a,j] t1[b,i] with the two fused
+// multiply-adds written out, so its rounding does not depend on how the compiler contracts the
+// expression.
+__device__ inline double tau_value(double x, double ai, double bj, double aj, double bi) {
+  return fma(-aj, bi, fma(ai, bj, x));
+}
+
+// The tau kernel: workgroups walk the pairs (a, b), threads the h x h block (contiguous in t2
+// and tau).
 template <typename T>   // double, or uint16_t for a BF16 tau (the rounded FP64 value)
 __global__ __launch_bounds__(NT) void tau_kernel(int p, int h, FastDiv fp, FastDiv fh,
                                                  const double* t1, const double* t2, T* tau) {
@@ -390,9 +407,9 @@ __global__ __launch_bounds__(NT) void tau_kernel(int p, int h, FastDiv fp, FastD
     __syncthreads();
     for (int e = threadIdx.x; e < hh; e += NT) {
       const int i = fdiv(e, fh), j = e - i * h;
-      const double v = x[e] + ta[i] * tb[j] - ta[j] * tb[i];
+      const double v = tau_value(x[e], ta[i], tb[j], ta[j], tb[i]);
       if constexpr (sizeof(T) == 8) y[e] = v;
-      else y[e] = to_bf16_bits((float)v);
+      else y[e] = to_bf16_bits_2r(v);
     }
   }
 }
@@ -499,7 +516,7 @@ __global__ __launch_bounds__(NT) void assemble_t2_kernel(AssembleArgs g) {
     double nt = told + H / denom;
     double tn = g.mixing * told + (1.0 - g.mixing) * nt;
     if (g.t2n) g.t2n[t] = tn;
-    if (g.t2n16) g.t2n16[t] = to_bf16_bits((float)tn);
+    if (g.t2n16) g.t2n16[t] = to_bf16_bits_2r(tn);
     if (g.H2out) g.H2out[t] = H;
     if (g.vpphh) {
       double v = g.vpphh[t];
@@ -534,7 +551,7 @@ __global__ __launch_bounds__(NT) void assemble_pair_kernel(AssembleArgs g) {
     double told = g.t2[t];
     double tn = g.mixing * told + (1.0 - g.mixing) * (told + H / denom);
     if (g.t2n) g.t2n[t] = tn;
-    if (g.t2n16) g.t2n16[t] = to_bf16_bits((float)tn);
+    if (g.t2n16) g.t2n16[t] = to_bf16_bits_2r(tn);
     if (g.H2out) g.H2out[t] = H;
     if (g.vpphh) {
       double v = g.vpphh[t];

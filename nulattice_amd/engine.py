@@ -228,6 +228,14 @@ def t1_part(ham, t1, t2, B, prec=DEFAULT_PREC):
     contract("ki,ak->ai", Xhh, t1, H1)
 
 
+def _lp_tau(ham, prec):
+    """tau feeds only the hh products, the pp ladder and R56: when all of them run in BF16 (and
+    are large enough not to fall back to FP64) it is formed directly in BF16."""
+    p, h = ham.p, ham.h
+    return (TAU16 and prec["hh"] == "bf16" and prec.get("ladder", "fp64") == "bf16"
+            and min(p * p, h * h) >= ops.LP_MIN_DIM and (h * h) % 8 == 0)
+
+
 def t2_part(ham, t1, t2, B, prec=DEFAULT_PREC):
     """S0/Sab/Sij/Spp and X2pp/X2hh of t2Iter (coupled_cluster.py:476-536).
 
@@ -244,8 +252,7 @@ def t2_part(ham, t1, t2, B, prec=DEFAULT_PREC):
     # are large enough not to fall back to FP64) it is formed directly in BF16, as [ab][ij] rows
     # and, for the Whh product (summed over c,d), as [ij][cd] rows; the FP64 tau is not written
     tau = B.tau
-    lp_tau = (TAU16 and prec["hh"] == "bf16" and prec.get("ladder", "fp64") == "bf16"
-              and min(p * p, h * h) >= ops.LP_MIN_DIM and (h * h) % 8 == 0)
+    lp_tau = _lp_tau(ham, prec)
     tau16 = None
     if lp_tau:
         ldT = (p * p + 7) // 8 * 8                   # [ij][cd] rows, zero-padded to 8
@@ -430,9 +437,10 @@ def _block_gemm(pb, src, out, alpha, ncol, tag, precision="fp64", src16=None):
 class T2Image:
     """The BF16 image of t2 carried from one step's assembly to the next step's BF16 products
     (pass the same object to every ``update`` of a solve).  Two buffers alternate: the one the
-    current step reads and the one its assembly writes.  ``src`` is the tensor object whose
+    current step reads and the one its assembly writes.  ``src`` is the t2 tensor object whose
     image the current buffer holds; any other t2 (a DIIS extrapolation, a fresh start) is cast
-    as before."""
+    as before.  (Carrying tau the same way was measured and left out: the two extra t1 reads
+    per element cost the assembly more than the tau pass they replace.)"""
 
     def __init__(self):
         self.bufs = [None, None]
@@ -445,6 +453,9 @@ class T2Image:
             b = torch.empty(n, dtype=torch.int16, device=dev)
             self.bufs[k] = b
         return b[:n]
+
+    def holds(self, t2):
+        return self.src is t2
 
 
 def update(ham, t1, t2, B, t1n, t2n, mixing, prec=DEFAULT_PREC, H2out=None, img=None):
@@ -462,10 +473,7 @@ def update(ham, t1, t2, B, t1n, t2n, mixing, prec=DEFAULT_PREC, H2out=None, img=
             t2_part(ham, t1, t2, B, prec)
         finally:
             B.t2_16 = None
-    assemble(ham, t2, B, t1n, t2n, mixing, H2out,
-             img=img if _images_wanted(ham, prec) else None)
-    if img is not None and not _images_wanted(ham, prec):
-        img.src = None
+    assemble(ham, t2, B, t1n, t2n, mixing, H2out, img=img, prec=prec)
 
 
 # BF16 images of t2 for the products that read it in BF16 (all from one cast of t2):
@@ -494,7 +502,7 @@ def _t2_images(ham, t2, B, prec, img=None):
     use = {k for k in ("dress", "hh", "ring") if prec.get(k) == "bf16"}
     n, ph, hh = p * p * h * h, p * h, h * h
     dev, st = t2.device, ops._stream()
-    if img is not None and img.src is t2:
+    if img is not None and img.holds(t2):
         S = img.buffer(img.cur, n, dev)                               # [a][d][k][l]
     else:
         S = ops.WS.get("t2_bf16", n, dtype=torch.int16, device=dev)   # [a][d][k][l]
@@ -517,13 +525,16 @@ def _t2_images(ham, t2, B, prec, img=None):
         ops.lp_provide(t2, "acik", ["a", "i"], ["c", "k"], T3, ph)
 
 
-def assemble(ham, t2, B, t1n, t2n, mixing, H2out=None, img=None):
+def assemble(ham, t2, B, t1n, t2n, mixing, H2out=None, img=None, prec=None):
+    """The assembly (nl_assemble_t2).  With ``img`` and ``prec`` it also writes the BF16 image of
+    t2n into ``img`` when ``prec`` reads t2 in BF16."""
     p, h = ham.p, ham.h
     st = ops._stream()
     args = (p, h, ops._p(B.S0), ops._p(B.Sab), ops._p(B.Sij), ops._p(B.Spp), ops._p(B.X2pp),
             ops._p(B.X2hh), ops._p(t2), ops._p(ham.v_pphh), ops._p(t1n),
             ctypes.c_double(mixing), ops._p(t2n), ops._p(H2out), ops._p(B.part))
-    if img is not None and t2n is not None:
+    if (img is not None and prec is not None and t2n is not None
+            and _images_wanted(ham, prec)):
         nxt = 1 - img.cur
         buf = img.buffer(nxt, p * p * h * h, t2n.device)
         _lib.call("nl_assemble_t2_16", *args, ops._p(buf), st)

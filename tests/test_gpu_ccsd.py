@@ -175,3 +175,31 @@ def test_product_hamiltonian_on_gpu_solver():
         hist = []
         ccm.ccsd_solver(fock, v, maxSteps=10, sparse=sparse, history=hist)
         np.testing.assert_allclose(hist, g["energies"][:11], rtol=1e-12)
+
+
+def test_bf16_operands_carried_by_the_assembly_are_bitwise_the_conversions():
+    """Random Hamiltonian with a T1 source, p = 32, h = 16, 262,144 amplitudes (enough for the
+    rare FP64 values whose FP32 rounding ties between two BF16 values, which a single FP64 ->
+    BF16 rounding would round differently): three BF16 steps that take t2's BF16 image from the
+    previous step's assembly (engine.T2Image, nl_assemble_t2_16) give bit-identical amplitudes
+    and energy to casting t2 at the start of each step."""
+    p, h = 32, 16
+    fock, dense, _, _ = random_antisym_hamiltonian(p, h, seed=21, scale=0.1)
+    H = engine.Hamiltonian(fock, dense, device="cuda")
+    t1_0, t2_0 = random_amplitudes(p, h, seed=22, scale=0.02)
+    res = {}
+    for carry in (False, True):
+        B = engine.Buffers(p, h, H.device)
+        t1, t2 = dev(t1_0), dev(t2_0)
+        t1n, t2n = torch.empty_like(t1), torch.empty_like(t2)
+        img = engine.T2Image() if carry else None
+        for _ in range(3):
+            engine.update(H, t1, t2, B, t1n, t2n, 0.5, ccm._PREC["bf16"], img=img)
+            t1, t1n, t2, t2n = t1n, t1, t2n, t2
+        if carry:
+            assert img.src is t2
+        res[carry] = (t2.clone(), t1.clone(), float(B.E.item()))
+    assert float(res[True][1].abs().max()) > 0
+    assert torch.equal(res[False][0], res[True][0])
+    assert torch.equal(res[False][1], res[True][1])
+    assert res[False][2] == res[True][2]
