@@ -2,6 +2,7 @@
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--L 5] [--A 28] [--seed 0]
                     [--prec bf16|fp16] [--no-converge] [--cpu-sample L4|none|full]
+                    [--form all|sectors|diis] [--grad on|off]
 
 Workload: plane-wave lattice contact theory, L=5 (500 orbitals), A=28 closed shell (k = 0 and
 the first momentum shell filled; h=28, p=472), SU(4) two-body contact C0 ~ U[-4.5,-3.5] from
@@ -10,7 +11,12 @@ dense T2 (p,p,h,h) as in the reference's layout.  A step is one CCSD Jacobi iter
 residuals, update with mixing 0.5, energy read back to the host), timed over K steps after W
 warm-ups with the device synchronised on both sides.  ``value`` is the mixed-precision
 iteration rate; the FP64 rate, the speed-up and |dE| (low-precision fixed point and after the
-FP64 refinement, both against the converged FP64 energy) are reported beside it.
+FP64 refinement, both against the converged FP64 energy) are reported beside it, with the
+roofline of the dominant kernel (the BF16 ring GEMM), the whole-iteration fraction, the wall
+time to the converged energy, |dE| over seeds 0..4 and for the other 16-bit type, the cost of
+one DIIS extrapolation, the symmetry-sector form of the same iteration, the adjoint gradient
+against central differences, and the CPU oracle's rate on this host.  ``--form sectors`` and
+``--form diis`` time only the sector form or only the DIIS extrapolation.
 """
 import argparse
 import json
