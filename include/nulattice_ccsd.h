@@ -208,6 +208,11 @@ int nl_dot_part(int64_t n, const double* x, const double* y, double* part, void*
 int nl_axpby(int64_t n, double a, const double* x, double b, double* y, void* stream);
 int nl_sub(int64_t n, const double* x, const double* y, double* c, void* stream);
 
+/* hcopy = x and c = x - y in one pass: a DIIS step's history entry (the deepcopy of t) and error
+ * vector (t - oldT) (coupled_cluster.py:636-641, soa/coupled_cluster.py:523-526). */
+int nl_copy_sub(int64_t n, const double* x, const double* y, double* hcopy, double* c,
+                void* stream);
+
 /* Gram matrix of k <= 12 vectors of n elements in one pass over each: the DIIS B matrix
  * (coupled_cluster.py:648-656, soa/coupled_cluster.py:536-543), which the reference forms pair by
  * pair.  x is a host array of k device pointers; part holds k(k+1)/2 * nl_partials(n) doubles,

@@ -53,6 +53,7 @@ SIGNATURES = {
     "nl_dot_part": (_i32, [_i64, _vp, _vp, _vp, _vp]),
     "nl_axpby": (_i32, [_i64, _dbl, _vp, _dbl, _vp, _vp]),
     "nl_sub": (_i32, [_i64, _vp, _vp, _vp, _vp]),
+    "nl_copy_sub": (_i32, [_i64, _vp, _vp, _vp, _vp, _vp]),
     "nl_gram_part": (_i32, [_i64, _i32, _vp, _vp, _vp]),
     "nl_gram_finalize": (_i32, [_i64, _i32, _vp, _vp, _vp]),
     "nl_lincomb": (_i32, [_i64, _i32, _vp, _vp, _vp, _vp]),

@@ -57,11 +57,14 @@ class _Diis:
         """Record (t, t - old); return extrapolated (t1, t2) or None.  ``B`` is unused (kept for
         callers that pass reduction buffers; the Gram kernel allocates its own partials)."""
         soa = self.mode == "soa"
-        self.h1.append(ops.copy(t1)); self.h2.append(ops.copy(t2))
-        e1 = torch.empty_like(t1); e2 = torch.empty_like(t2)
-        _lib.call("nl_sub", t1.numel(), ops._p(t1), ops._p(o1), ops._p(e1), ops._stream())
-        _lib.call("nl_sub", t2.numel(), ops._p(t2), ops._p(o2), ops._p(e2), ops._stream())
-        self.e1.append(e1); self.e2.append(e2)
+        # history entry (copy of t) and error vector (t - old) in one pass each
+        for t, o, hist, errs in ((t1, o1, self.h1, self.e1), (t2, o2, self.h2, self.e2)):
+            if not (t.is_contiguous() and o.is_contiguous()) or t.numel() != o.numel():
+                raise _lib.NLError("DIIS vectors must be contiguous and of one size")
+            hc, e = torch.empty_like(t), torch.empty_like(t)
+            _lib.call("nl_copy_sub", t.numel(), ops._p(t), ops._p(o), ops._p(hc), ops._p(e),
+                      ops._stream())
+            hist.append(hc); errs.append(e)
         if soa and len(self.e1) > self.n:
             for L in (self.h1, self.h2, self.e1, self.e2):
                 L.pop(0)

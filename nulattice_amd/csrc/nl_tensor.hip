@@ -742,6 +742,17 @@ __global__ __launch_bounds__(NT) void lincomb_kernel(int64_t n, LincombArgs g, d
   }
 }
 
+// hcopy = x, c = x - y in one pass (a DIIS step's history entry and error vector)
+__global__ void copy_sub_kernel(int64_t n, const double* x, const double* y, double* hcopy,
+                                double* c) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const double v = x[t];
+    hcopy[t] = v;
+    c[t] = v - y[t];
+  }
+}
+
 // y = a*x + b*y
 __global__ void axpby_kernel(int64_t n, double a, const double* x, double b, double* y) {
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n;
@@ -1167,6 +1178,16 @@ int nl_sub(int64_t n, const double* x, const double* y, double* c, void* stream)
   if (n < 0 || !x || !y || !c) return NL_EARG;
   if (n == 0) return NL_OK;
   hipLaunchKernelGGL(sub_kernel, dim3(grid_for(n)), dim3(NT), 0, (hipStream_t)stream, n, x, y, c);
+  NL_LAUNCH_CHECK();
+  return NL_OK;
+}
+
+int nl_copy_sub(int64_t n, const double* x, const double* y, double* hcopy, double* c,
+                void* stream) {
+  if (n < 0 || !x || !y || !hcopy || !c) return NL_EARG;
+  if (n == 0) return NL_OK;
+  hipLaunchKernelGGL(copy_sub_kernel, dim3(grid_for(n)), dim3(NT), 0, (hipStream_t)stream, n, x,
+                     y, hcopy, c);
   NL_LAUNCH_CHECK();
   return NL_OK;
 }

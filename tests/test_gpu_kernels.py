@@ -389,3 +389,16 @@ def test_gram_and_lincomb_argument_errors():
     assert lib.nl_gram_finalize(0, 1, None, None, None) == -1
     assert lib.nl_lincomb(10, 17, None, None, None, None) == -1
     assert lib.nl_lincomb(10, 0, None, None, None, None) == -1
+
+
+def test_copy_sub_is_a_copy_and_a_difference():
+    """nl_copy_sub (a DIIS step's history entry and error vector in one pass): bitwise the copy
+    and the FP64 difference."""
+    g = torch.Generator(device="cuda").manual_seed(5)
+    for n in (1, 1000, 1 << 20):
+        x = torch.randn(n, generator=g, dtype=torch.float64, device="cuda")
+        y = torch.randn(n, generator=g, dtype=torch.float64, device="cuda")
+        hc, c = torch.empty_like(x), torch.empty_like(x)
+        _lib.call("nl_copy_sub", n, ops._p(x), ops._p(y), ops._p(hc), ops._p(c), ops._stream())
+        assert torch.equal(hc, x) and torch.equal(c, x - y)
+    assert _lib.load().nl_copy_sub(-1, None, None, None, None, None) == -1
