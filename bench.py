@@ -358,6 +358,9 @@ def main():
     print(f"# fp64: {s64 * 1e3:.1f} ms/iter", file=sys.stderr, flush=True)
     slp, klp, _ = time_iterations(ccm, engine, ops, H, a.prec, a.steps, a.warmup, torch)
     print(f"# {a.prec}: {slp * 1e3:.1f} ms/iter", file=sys.stderr, flush=True)
+    other = "fp16" if a.prec == "bf16" else "bf16"
+    sother, _, _ = time_iterations(ccm, engine, ops, H, other, max(2, a.steps // 2), 1, torch)
+    print(f"# {other}: {sother * 1e3:.1f} ms/iter", file=sys.stderr, flush=True)
 
     dE = {}
     if not a.no_converge:
@@ -452,6 +455,7 @@ def main():
                                f"{a.prec} ring/ladder/dressing MFMA + FP64 update/energy",
                    "L": a.L, "A": a.A, "p": p, "h": h, "C0": C0, "D0": a.D0, "seed": a.seed},
         "fp64_iters_per_s": 1.0 / s64, "fp64_ms_per_step": s64 * 1e3,
+        f"{other}_ms_per_step": sother * 1e3,
         "speedup_vs_fp64": s64 / slp,
         "canonical_flop_per_iter": cflop,
         "whole_iteration": whole,
