@@ -83,15 +83,67 @@ int nl_gemm_lp_strided(int dtype, int64_t K, double alpha, const void* A, int64_
 int nl_permute(int rank, const int64_t* out_shape, const int64_t* in_strides, double alpha,
                const double* in, double beta, double* out, void* stream);
 
+/* nl_gemm_lp with split operands: A = Ahi + Alo, B = Bhi + Blo (16-bit each, the lo part at the
+ * hi part's layout; either lo may be null), C = alpha diag(ia) (Ahi Bhi^T + Ahi Blo^T +
+ * Alo Bhi^T) diag(ib) + beta D.  The lo.lo term is below the split's rounding.  The precision
+ * form's ring, ladder and dressing products (ccDgrams.py:240-303, coupled_cluster.py:476-536)
+ * with operands carried to ~17 bits. */
+int nl_gemm_lp_split(int dtype, int64_t M, int64_t N, int64_t K, double alpha, const void* A,
+                     const void* Alo, int64_t lda, int64_t strideA, const float* ia,
+                     int64_t strideIa, const void* B, const void* Blo, int64_t ldb,
+                     int64_t strideB, const float* ib, int64_t strideIb, double beta,
+                     const double* D, double* C, int64_t ldc, int64_t strideC, int64_t batch,
+                     int64_t splits, double* ws, void* stream);
+
+/* nl_gemm_lp_strided with split operands (see nl_gemm_lp_split). */
+int nl_gemm_lp_strided_split(int dtype, int64_t K, double alpha, const void* A, const void* Alo,
+                             int64_t lda, const float* ia, const void* B, const void* Blo,
+                             int64_t ldb, const float* ib, double beta, const double* D,
+                             double* C, const int64_t* cdims, const int64_t* sc, int64_t splits,
+                             double* ws, void* stream);
+
+/* nl_gemm_lp_rows with split operands (see nl_gemm_lp_split). */
+int nl_gemm_lp_rows_split(int dtype, int64_t M, int64_t N, int64_t K, double alpha,
+                          const void* A, const void* Alo, int64_t lda, int64_t strideA,
+                          const float* ia, int64_t strideIa, const void* B, const void* Blo,
+                          int64_t ldb, int64_t strideB, const float* ib, int64_t strideIb,
+                          double* C, int64_t ldc, const int64_t* crow, int64_t batch,
+                          void* stream);
+
 /* Row-wise FP64 -> BF16 (dtype 0) / FP16 (dtype 1) conversion of a 4-index view
- * (rows r1*R2+r2, cols c1*C2+c2, element in[r1*s0+r2*s1+c1*s2+c2*s3]) with a per-row power-of-two
- * scale bringing the row max to <= target; iscale[r] receives the inverse scale; columns
- * [C1*C2, ldo) are zero-filled.  target <= 0: plain rounding cast, no scales (iscale unused, may
- * be null; BF16's exponent range makes a scale unnecessary).  Precision-form operand preparation
- * (no reference counterpart). */
+ * (rows r1*R2+r2, cols c1*C2+c2, element in[r1*s0+r2*s1+c1*s2+c2*s3]) with a per-row scale;
+ * iscale[r] receives the inverse scale; columns [C1*C2, ldo) are zero-filled.
+ *   target > 0:  power-of-two scale bringing the row max to <= target;
+ *   target < 0:  scale exactly |target| / row max (a row whose nonzero elements share one
+ *                magnitude, as every block of a contact interaction, converts exactly);
+ *   target == 0: plain rounding cast, no scales (iscale unused, may be null).
+ * Precision-form operand preparation (no reference counterpart). */
 int nl_convert_rows(int dtype, int64_t R1, int64_t R2, int64_t C1, int64_t C2,
                     const int64_t* strides4, const double* in, void* out, int64_t ldo,
                     float* iscale, float target, void* stream);
+
+/* tau[a,b,i,j] = t2 + t1[a,i] t1[b,j] - t1[a,j] t1[b,i] (ccDgrams.py:215-237, 444-472) as the
+ * hi/lo BF16 split of the precision form with split operands: row (a,b) of out is
+ * [hi(h*h) | lo(h*h)]; tau (FP64, may be null) receives the value as well. */
+int nl_tau_split(int64_t p, int64_t h, const double* t1, const double* t2, double* tau,
+                 void* out, void* stream);
+
+/* nl_gather_rows16 writing k < ldk of each output row at row pitch ldo >= ldk (the hi and lo
+ * halves of a split operand row are two gathers into one [hi | lo] row). */
+int nl_gather_rows16_into(int64_t nb, int64_t kb, int64_t ncol, const int64_t* idx,
+                          const void* src, int64_t lds, void* out, int64_t ldk, int64_t ldo,
+                          void* stream);
+
+/* nl_convert_rows writing a hi/lo split: with y = x * scale (per-row scale as in nl_convert_rows;
+ * target 0: unscaled), hi = 16-bit(y), lo = 16-bit(y - hi), so hi + lo carries ~17 (BF16) or ~23
+ * (FP16) significant bits.  Row r gets nblk blocks of kpad columns, block q at
+ * out[r*ldo + q*kpad + c] holding lo when bit q of lomask is set and hi otherwise (columns
+ * C1*C2 <= c < kpad zero).  GEMM operands [hi|lo|hi] x [hi|hi|lo] over the K-concatenated blocks
+ * sum hi.hi + lo.hi + hi.lo.  Precision form with split operands (no reference counterpart). */
+int nl_convert_split(int dtype, int64_t R1, int64_t R2, int64_t C1, int64_t C2,
+                     const int64_t* strides4, const double* in, void* out, int64_t ldo,
+                     int64_t kpad, int nblk, int lomask, float* iscale, float target,
+                     void* stream);
 
 /* out[b][n][k] (BF16) = src[idx[b*kb + k]*lds + n] for n < ncol, k < kb (0 where idx < 0), and 0
  * for kb <= k < ldk (ldk a multiple of 8): the row gather of the packed pair-block products
@@ -182,6 +234,22 @@ int nl_assemble_t2_16(int64_t p, int64_t h, const double* S0, const double* Sab,
                       const double* Sij, const double* Spp, const double* Xpp, const double* Xhh,
                       const double* t2, const double* vpphh, const double* t1n, double mixing,
                       double* t2n, double* H2out, double* part, void* t2n16, void* stream);
+
+/* nl_assemble_t2 (+ an image of t2n when t2n16 is not null) with two optional FP64 diagonal
+ * vectors dpp (p) and dhh (h): H2 += (dpp[a] + dpp[b] + dhh[i] + dhh[j]) t2[a,b,i,j] before the
+ * update.  The precision form runs the X_pp t2 dressing (coupled_cluster.py:536) in low precision
+ * without X_pp's diagonal, whose product with t2 is the largest term of the residual and is
+ * added here exactly.  image_kind 0: t2n16 is the BF16 image of nl_assemble_t2_16; 1: the hi/lo
+ * split of t2n (as nl_convert_split, unscaled BF16), row a as [hi(p*h*h) | lo(p*h*h)]. */
+int nl_assemble_t2_ext(int64_t p, int64_t h, const double* S0, const double* Sab,
+                       const double* Sij, const double* Spp, const double* Xpp, const double* Xhh,
+                       const double* dpp, const double* dhh, const double* t2,
+                       const double* vpphh, const double* t1n, double mixing, double* t2n,
+                       double* H2out, double* part, void* t2n16, int image_kind, void* stream);
+
+/* d[i] = X[i,i], then X[i,i] = 0, for a row-major n x n FP64 matrix (the diagonal split of a
+ * dressing matrix before its low-precision product; coupled_cluster.py:536). */
+int nl_take_diag(int64_t n, double* X, double* d, void* stream);
 
 /* Transpose of nl_assemble_t2's antisymmetrisers, for the Lambda (adjoint) equations of t2Iter
  * (coupled_cluster.py:536-545): given the adjoint L[a,b,i,j] of H2,

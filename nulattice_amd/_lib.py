@@ -21,8 +21,18 @@ SIGNATURES = {
                                _vp]),
     "nl_gemm_lp": (_i32, [_i32, _i64, _i64, _i64, _dbl, _vp, _i64, _i64, _vp, _i64, _vp, _i64,
                           _i64, _vp, _i64, _dbl, _vp, _vp, _i64, _i64, _i64, _i64, _vp, _vp]),
+    "nl_gemm_lp_split": (_i32, [_i32, _i64, _i64, _i64, _dbl, _vp, _vp, _i64, _i64, _vp, _i64,
+                                _vp, _vp, _i64, _i64, _vp, _i64, _dbl, _vp, _vp, _i64, _i64,
+                                _i64, _i64, _vp, _vp]),
+    "nl_gemm_lp_strided_split": (_i32, [_i32, _i64, _dbl, _vp, _vp, _i64, _vp, _vp, _vp, _i64,
+                                        _vp, _dbl, _vp, _vp, _vp, _vp, _i64, _vp, _vp]),
+    "nl_gemm_lp_rows_split": (_i32, [_i32, _i64, _i64, _i64, _dbl, _vp, _vp, _i64, _i64, _vp,
+                                     _i64, _vp, _vp, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64,
+                                     _vp]),
     "nl_gemm_lp_rows": (_i32, [_i32, _i64, _i64, _i64, _dbl, _vp, _i64, _i64, _vp, _i64, _vp, _i64,
                                _i64, _vp, _i64, _vp, _i64, _vp, _i64, _vp]),
+    "nl_convert_split": (_i32, [_i32, _i64, _i64, _i64, _i64, _vp, _vp, _vp, _i64, _i64, _i32,
+                                _i32, _vp, _flt, _vp]),
     "nl_gather_convert": (_i32, [_i64, _i64, _i64, _vp, _vp, _i64, _vp, _i64, _vp]),
     "nl_gather_rows16": (_i32, [_i64, _i64, _i64, _vp, _vp, _i64, _vp, _i64, _vp]),
     "nl_tau16": (_i32, [_i64, _i64, _vp, _vp, _vp, _vp]),
@@ -46,6 +56,11 @@ SIGNATURES = {
                               _vp, _vp, _vp, _vp]),
     "nl_assemble_t2_16": (_i32, [_i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _dbl,
                                  _vp, _vp, _vp, _vp, _vp]),
+    "nl_assemble_t2_ext": (_i32, [_i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                                  _vp, _dbl, _vp, _vp, _vp, _vp, _i32, _vp]),
+    "nl_tau_split": (_i32, [_i64, _i64, _vp, _vp, _vp, _vp, _vp]),
+    "nl_gather_rows16_into": (_i32, [_i64, _i64, _i64, _vp, _vp, _i64, _vp, _i64, _i64, _vp]),
+    "nl_take_diag": (_i32, [_i64, _vp, _vp, _vp]),
     "nl_assemble_t2_adj": (_i32, [_i64, _i64, _vp, _vp, _vp, _vp, _vp]),
     "nl_partials": (_i64, [_i64]),
     "nl_energy_part": (_i32, [_i64, _i64, _vp, _vp, _vp, _vp, _vp]),

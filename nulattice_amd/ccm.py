@@ -31,6 +31,8 @@ _PREC = {
     "fp64": engine.DEFAULT_PREC,
     "bf16": {"ring": "bf16", "hh": "bf16", "dress": "bf16", "ladder": "bf16"},
     "fp16": {"ring": "fp16", "hh": "fp16", "dress": "fp16", "ladder": "fp16"},
+    "bf16x3": {"ring": "bf16x3", "hh": "bf16x3", "dress": "bf16x3", "ladder": "bf16x3"},
+    "fp16x3": {"ring": "fp16x3", "hh": "fp16x3", "dress": "fp16x3", "ladder": "fp16x3"},
 }
 
 

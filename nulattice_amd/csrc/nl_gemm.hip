@@ -686,6 +686,8 @@ struct GemmLPArgs {
   double alpha, beta;
   const uint16_t* A; int64_t lda, sA;   // [M][lda] K-contiguous, 16-bit elements
   const uint16_t* B; int64_t ldb, sB;   // [N][ldb] K-contiguous
+  const uint16_t* Alo;                  // split operands: lo parts with A's / B's layout (or
+  const uint16_t* Blo;                  // null); the product is hi.hi + hi.lo + lo.hi
   const float* ia; int64_t sia;         // per-row inverse scale of A (or null)
   const float* ib; int64_t sib;         // per-row inverse scale of B (or null)
   double* C; Idx2 cm, cn; int64_t sC;   // C(m,n) = C[off2(m,cm) + off2(n,cn)] (+ batch * sC)
@@ -955,6 +957,219 @@ __global__ __launch_bounds__(512) void gemm_lp_big_kernel(GemmLPArgs g) {
   }
   // per 16-row group: all addend loads first, then the stores (D may be C; each element is read
   // and written by this lane only), so the loads overlap instead of waiting on each store
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    int64_t ro[4];
+    double dv[4][4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int row = m0 + wm * 128 + i * 16 + (lane >> 4) * 4 + r;
+      ro[r] = row >= g.M ? -1
+              : partial ? (int64_t)row * g.N
+              : g.crow ? (g.crow[(int64_t)batch * g.M + row] >= 0
+                              ? g.crow[(int64_t)batch * g.M + row] * g.cm.s1 : -1)
+              : off2(row, g.cm);
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        dv[r][j] = (Dp && beta != 0.0 && ro[r] >= 0 && coff[j] >= 0) ? Dp[ro[r] + coff[j]] : 0.0;
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      if (ro[r] < 0) continue;
+      const int row = m0 + wm * 128 + i * 16 + (lane >> 4) * 4 + r;
+      const double sr = g.alpha * (ia ? (double)ia[row] : 1.0);
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        if (coff[j] >= 0) C[ro[r] + coff[j]] = sr * (double)acc[i][j][r] * sc[j] + beta * dv[r][j];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------------- split GEMM
+// 256x256 tiles like gemm_lp_big_kernel, for operands given as hi/lo pairs (x = hi + lo, both
+// 16-bit): C = alpha diag(ia) (Ahi Bhi^T + Ahi Blo^T + Alo Bhi^T) diag(ib) + beta D, with PA / PB
+// = 1 for an operand without a lo part (an exact Hamiltonian block).  One K step is 32 wide and
+// holds the PA + PB operand tiles; NS steps are in flight through NS distinct LDS stages:
+// at the top of step s a wave waits (vmcnt) for its own loads of stage s only, leaving the
+// later stages' loads running, and after the barrier issues stage s + NS - 1 into the buffer
+// step s - 1 read.  Every lane issues every DMA instruction (rows beyond M or N read a clamped
+// row, whose products reach only outputs that are not stored), so each wave's count of loads
+// per stage is the same constant and the vmcnt thresholds are exact.  K and the split-K chunk
+// are multiples of 32 (the host checks), so no step is partial.  Rows of a stage tile are 64 B
+// (four 16-byte chunks); chunk c of row r is stored at c ^ ((r >> 2) & 3), so the 16 lanes of a
+// fragment read (16 consecutive rows, one logical chunk) hit 16 distinct bank quads.
+constexpr int SBK = 32;
+
+template <int NP>
+using SStage = uint16_t[NP][BT][SBK];
+
+// s_waitcnt with vmcnt = N (gfx9 encoding: vmcnt bits [3:0] and [15:14]; expcnt and lgkmcnt
+// left at their maxima, so only vector-memory operations are waited for).
+template <int N>
+__device__ inline void wait_vmcnt() {
+  static_assert(N >= 0 && N < 64, "vmcnt range");
+  __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (7 << 4) | (15 << 8));
+}
+
+template <bool IS_BF16>
+__device__ inline void mfma_block(v4f (&acc)[8][4], const uint4 (&a)[8], const uint4 (&b)[4]) {
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      if (IS_BF16)
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+            __builtin_bit_cast(v8bf, a[i]), __builtin_bit_cast(v8bf, b[j]), acc[i][j], 0, 0, 0);
+      else
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(
+            __builtin_bit_cast(v8h, a[i]), __builtin_bit_cast(v8h, b[j]), acc[i][j], 0, 0, 0);
+    }
+}
+
+template <bool IS_BF16, int PA, int PB, int NS>
+__global__ __launch_bounds__(512) void gemm_lp_split_kernel(GemmLPArgs g) {
+  constexpr int NP = PA + PB;
+  constexpr int LPS = 2 * NP;          // DMA instructions per wave per stage
+  __shared__ __attribute__((aligned(1024))) SStage<NP> S0;
+  __shared__ __attribute__((aligned(1024))) SStage<NP> S1;
+  // (stages beyond NS are never referenced, and unreferenced LDS objects are not allocated)
+  __shared__ __attribute__((aligned(1024))) SStage<NP> S2[1];
+  __shared__ __attribute__((aligned(1024))) SStage<NP> S3[1];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int wm = w >> 2, wn = w & 3;
+  const int batch = blockIdx.z;
+  TileCoord tc = tile_of(blockIdx.x, gridDim.x, g.tiles_m, g.tiles_n);
+  const int m0 = tc.tm * BT, n0 = tc.tn * BT;
+  const int64_t oA = (int64_t)batch * g.sA, oB = (int64_t)batch * g.sB;
+  const uint16_t* Ah = g.A + oA;
+  const uint16_t* Al = PA == 2 ? g.Alo + oA : nullptr;
+  const uint16_t* Bh = g.B + oB;
+  const uint16_t* Bl = PB == 2 ? g.Blo + oB : nullptr;
+  const int split = blockIdx.y;
+  const int kbeg = split * g.kchunk;
+  const int kend = min(g.K, kbeg + g.kchunk);
+  const int nsteps = kbeg < kend ? (kend - kbeg + SBK - 1) / SBK : 0;
+
+  v4f acc[8][4];
+#pragma unroll
+  for (int i = 0; i < 8; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = (v4f){0.f, 0.f, 0.f, 0.f};
+
+  // this lane's place in a DMA instruction: row (lane >> 2) of a 16-row group, position lane & 3
+  const int drow = lane >> 2, dpos = lane & 3;
+  auto issue = [&](SStage<NP>& T, int k0) {
+#pragma unroll
+    for (int q = 0; q < NP; ++q) {
+      const bool isA = q < PA;
+      const uint16_t* base = isA ? (q == 0 ? Ah : Al) : (q == PA ? Bh : Bl);
+      const int64_t ld = isA ? g.lda : g.ldb;
+      const int lim = (isA ? g.M : g.N) - 1, o0 = isA ? m0 : n0;
+#pragma unroll
+      for (int hh = 0; hh < 2; ++hh) {
+        const int r0 = w * 32 + hh * 16, r = r0 + drow;
+        const int c = dpos ^ ((r >> 2) & 3);
+        const int row = min(o0 + r, lim);
+        if (NS > 2 || k0 + c * 8 < kend) {
+          lds_dma16(base + (int64_t)row * ld + k0 + c * 8, &T[q][r0][0]);
+        } else {
+          // two stages only: the last step of a K range that is not a multiple of 32 is
+          // zero-filled beyond it (every wait there is a full one, so the count of DMA
+          // instructions does not matter)
+          *(uint4*)&T[q][r][dpos * 8] = make_uint4(0, 0, 0, 0);
+        }
+      }
+    }
+  };
+  const int cl = lane >> 4;
+  const int ra = wm * 128 + (lane & 15), rb = wn * 64 + (lane & 15);
+  auto compute = [&](const SStage<NP>& T) {
+    uint4 a[8], b[4];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int r = ra + i * 16;
+      a[i] = *(const uint4*)&T[0][r][(cl ^ ((r >> 2) & 3)) * 8];
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int r = rb + j * 16;
+      b[j] = *(const uint4*)&T[PA][r][(cl ^ ((r >> 2) & 3)) * 8];
+    }
+    mfma_block<IS_BF16>(acc, a, b);                         // hi . hi
+    if constexpr (PB == 2) {
+      uint4 bl[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int r = rb + j * 16;
+        bl[j] = *(const uint4*)&T[PA + 1][r][(cl ^ ((r >> 2) & 3)) * 8];
+      }
+      mfma_block<IS_BF16>(acc, a, bl);                      // hi . lo
+    }
+    if constexpr (PA == 2) {
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        const int r = ra + i * 16;
+        a[i] = *(const uint4*)&T[1][r][(cl ^ ((r >> 2) & 3)) * 8];
+      }
+      mfma_block<IS_BF16>(acc, a, b);                       // lo . hi
+    }
+  };
+  // top of step s: stage s's loads done (the later issued stages may still be in flight), all
+  // waves past step s - 1 (so its buffer may be refilled), then issue stage s + NS - 1
+  auto step = [&](SStage<NP>& Tc, SStage<NP>& Ti, int s) {
+    if (s >= nsteps) return;
+    const int rem = nsteps - 1 - s;
+    if constexpr (NS == 4) {
+      if (rem >= 2) wait_vmcnt<2 * LPS>();
+      else if (rem == 1) wait_vmcnt<LPS>();
+      else wait_vmcnt<0>();
+    } else if constexpr (NS == 3) {
+      if (rem >= 1) wait_vmcnt<LPS>();
+      else wait_vmcnt<0>();
+    } else {
+      __builtin_amdgcn_s_waitcnt(0);     // loads and zero-fill stores of stage s landed
+    }
+    asm volatile("s_barrier" ::: "memory");
+    if (s + NS - 1 < nsteps) issue(Ti, kbeg + (s + NS - 1) * SBK);
+    compute(Tc);
+  };
+  if (0 < nsteps) issue(S0, kbeg);
+  if (NS > 2 && 1 < nsteps) issue(S1, kbeg + SBK);
+  if constexpr (NS > 3) {
+    if (2 < nsteps) issue(S2[0], kbeg + 2 * SBK);
+  }
+  for (int s = 0; s < nsteps; s += NS) {
+    if constexpr (NS == 2) {
+      step(S0, S1, s);
+      step(S1, S0, s + 1);
+    } else if constexpr (NS == 3) {
+      step(S0, S2[0], s);
+      step(S1, S0, s + 1);
+      step(S2[0], S1, s + 2);
+    } else {
+      step(S0, S3[0], s);
+      step(S1, S0, s + 1);
+      step(S2[0], S1, s + 2);
+      step(S3[0], S2[0], s + 3);
+    }
+  }
+
+  const bool partial = g.splits > 1;
+  const int64_t cbo = g.crow ? 0 : (int64_t)batch * g.sC;
+  double* C = partial ? g.ws + ((int64_t)batch * g.splits + split) * (int64_t)g.M * g.N
+                      : g.C + cbo;
+  const double* Dp = (partial || !g.D) ? nullptr : g.D + cbo;
+  const double beta = partial ? 0.0 : g.beta;
+  const float* ia = g.ia ? g.ia + (int64_t)batch * g.sia : nullptr;
+  const float* ib = g.ib ? g.ib + (int64_t)batch * g.sib : nullptr;
+  double sc[4];
+  int64_t coff[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    int col = n0 + wn * 64 + j * 16 + (lane & 15);
+    sc[j] = (ib && col < g.N) ? (double)ib[col] : 1.0;
+    coff[j] = col < g.N ? (partial ? col : off2(col, g.cn)) : -1;
+  }
 #pragma unroll
   for (int i = 0; i < 8; ++i) {
     int64_t ro[4];
@@ -1435,13 +1650,15 @@ static int gemm_lp_impl(int dtype, int64_t M, int64_t N, int64_t K, double alpha
                         const void* B, int64_t ldb, int64_t strideB, const float* ib,
                         int64_t strideIb, double beta, const double* D, double* C, Idx2 cm,
                         Idx2 cn, int64_t strideC, int64_t batch, int64_t splits, double* ws,
-                        void* stream, const int64_t* crow = nullptr) {
+                        void* stream, const int64_t* crow = nullptr, const void* Alo = nullptr,
+                        const void* Blo = nullptr, bool staged = false) {
   if (M < 0 || N < 0 || K < 0 || batch < 0 || (dtype != 0 && dtype != 1)) return NL_EARG;
   if (M == 0 || N == 0 || batch == 0) return NL_OK;
   if (!A || !B || !C) return NL_EARG;
   if ((K % 8) || (lda % 8) || (ldb % 8) || lda < K || ldb < K) return NL_EARG;
   if ((((uintptr_t)A) & 15) || (((uintptr_t)B) & 15) || (strideA % 8) || (strideB % 8))
     return NL_EARG;
+  if ((((uintptr_t)Alo) & 15) || (((uintptr_t)Blo) & 15)) return NL_EARG;
   if (M > INT32_MAX || N > INT32_MAX || K > INT32_MAX || batch > 65535) return NL_EARG;
   if (splits < 1 || splits > 65535 || (splits > 1 && !ws)) return NL_EARG;
   if (K == 0) return NL_EARG;   // nothing to accumulate: no caller needs C = beta * C here
@@ -1449,6 +1666,7 @@ static int gemm_lp_impl(int dtype, int64_t M, int64_t N, int64_t K, double alpha
   g.M = (int)M; g.N = (int)N; g.K = (int)K; g.alpha = alpha; g.beta = beta;
   g.A = (const uint16_t*)A; g.lda = lda; g.sA = strideA;
   g.B = (const uint16_t*)B; g.ldb = ldb; g.sB = strideB;
+  g.Alo = (const uint16_t*)Alo; g.Blo = (const uint16_t*)Blo;
   g.ia = ia; g.sia = strideIa; g.ib = ib; g.sib = strideIb;
   g.C = C; g.cm = cm; g.cn = cn; g.sC = strideC; g.D = D; g.crow = crow;
   if (crow && splits != 1) return NL_EARG;
@@ -1459,7 +1677,43 @@ static int gemm_lp_impl(int dtype, int64_t M, int64_t N, int64_t K, double alpha
   hipStream_t st = (hipStream_t)stream;
   // 256x256 tiles when they alone give every CU a workgroup, else 128x128 tiles
   const int64_t big_tiles = nl_cdiv(M, BT) * nl_cdiv(N, BT);
-  if (M >= BT && N >= BT && big_tiles * splits * batch >= 256) {
+  const bool big = M >= BT && N >= BT && big_tiles * splits * batch >= 256;
+  const bool split_ops = Alo || Blo;
+  if (big && (split_ops || staged) && (K % SBK == 0 || (Alo && Blo))) {
+    // hi/lo operands (or the staged pipeline on request): the multi-stage split kernel
+    g.tiles_m = (int)nl_cdiv(M, BT); g.tiles_n = (int)nl_cdiv(N, BT);
+    grid = dim3((unsigned)(g.tiles_m * g.tiles_n), (unsigned)splits, (unsigned)batch);
+    const int pa = Alo ? 2 : 1, pb = Blo ? 2 : 1;
+#define NL_SPLIT(BF, PA_, PB_, NS_)                                                          \
+  hipLaunchKernelGGL((gemm_lp_split_kernel<BF, PA_, PB_, NS_>), grid, dim3(512), 0, st, g)
+    if (dtype == 0) {
+      if (pa == 2 && pb == 2) NL_SPLIT(true, 2, 2, 2);
+      else if (pa == 2) NL_SPLIT(true, 2, 1, 3);
+      else if (pb == 2) NL_SPLIT(true, 1, 2, 3);
+      else NL_SPLIT(true, 1, 1, 4);
+    } else {
+      if (pa == 2 && pb == 2) NL_SPLIT(false, 2, 2, 2);
+      else if (pa == 2) NL_SPLIT(false, 2, 1, 3);
+      else if (pb == 2) NL_SPLIT(false, 1, 2, 3);
+      else NL_SPLIT(false, 1, 1, 4);
+    }
+#undef NL_SPLIT
+  } else if (split_ops) {
+    // shapes the split kernel does not take: the three products in turn, each accumulating
+    // into C (the first applies beta and D)
+    int rc = gemm_lp_impl(dtype, M, N, K, alpha, A, lda, strideA, ia, strideIa, B, ldb, strideB,
+                          ib, strideIb, beta, D, C, cm, cn, strideC, batch, splits, ws, stream,
+                          crow);
+    if (rc == NL_OK && Blo)
+      rc = gemm_lp_impl(dtype, M, N, K, alpha, A, lda, strideA, ia, strideIa, Blo, ldb, strideB,
+                        ib, strideIb, 1.0, crow ? D : C, C, cm, cn, strideC, batch, splits, ws,
+                        stream, crow);
+    if (rc == NL_OK && Alo)
+      rc = gemm_lp_impl(dtype, M, N, K, alpha, Alo, lda, strideA, ia, strideIa, B, ldb, strideB,
+                        ib, strideIb, 1.0, crow ? D : C, C, cm, cn, strideC, batch, splits, ws,
+                        stream, crow);
+    return rc;
+  } else if (big) {
     g.tiles_m = (int)nl_cdiv(M, BT); g.tiles_n = (int)nl_cdiv(N, BT);
     grid = dim3((unsigned)(g.tiles_m * g.tiles_n), (unsigned)splits, (unsigned)batch);
     if (dtype == 0)
@@ -1511,6 +1765,44 @@ int nl_gemm_lp_strided(int dtype, int64_t K, double alpha, const void* A, int64_
   return gemm_lp_impl(dtype, cdims[0] * cdims[1], cdims[2] * cdims[3], K, alpha, A, lda, 0, ia, 0,
                       B, ldb, 0, ib, 0, beta, D, C, mk(cdims[1], sc[0], sc[1]),
                       mk(cdims[3], sc[2], sc[3]), 0, 1, splits, ws, stream);
+}
+
+int nl_gemm_lp_split(int dtype, int64_t M, int64_t N, int64_t K, double alpha, const void* A,
+                     const void* Alo, int64_t lda, int64_t strideA, const float* ia,
+                     int64_t strideIa, const void* B, const void* Blo, int64_t ldb,
+                     int64_t strideB, const float* ib, int64_t strideIb, double beta,
+                     const double* D, double* C, int64_t ldc, int64_t strideC, int64_t batch,
+                     int64_t splits, double* ws, void* stream) {
+  return gemm_lp_impl(dtype, M, N, K, alpha, A, lda, strideA, ia, strideIa, B, ldb, strideB, ib,
+                      strideIb, beta, D, C, mk(1, ldc, 0), mk(1, 1, 0), strideC, batch, splits,
+                      ws, stream, nullptr, Alo, Blo, true);
+}
+
+int nl_gemm_lp_strided_split(int dtype, int64_t K, double alpha, const void* A, const void* Alo,
+                             int64_t lda, const float* ia, const void* B, const void* Blo,
+                             int64_t ldb, const float* ib, double beta, const double* D,
+                             double* C, const int64_t* cdims, const int64_t* sc, int64_t splits,
+                             double* ws, void* stream) {
+  if (!cdims || !sc) return NL_EARG;
+  for (int d = 0; d < 4; ++d)
+    if (cdims[d] < 1) return NL_EARG;
+  if (cdims[1] > INT32_MAX || cdims[3] > INT32_MAX) return NL_EARG;
+  return gemm_lp_impl(dtype, cdims[0] * cdims[1], cdims[2] * cdims[3], K, alpha, A, lda, 0, ia, 0,
+                      B, ldb, 0, ib, 0, beta, D, C, mk(cdims[1], sc[0], sc[1]),
+                      mk(cdims[3], sc[2], sc[3]), 0, 1, splits, ws, stream, nullptr, Alo, Blo,
+                      true);
+}
+
+int nl_gemm_lp_rows_split(int dtype, int64_t M, int64_t N, int64_t K, double alpha,
+                          const void* A, const void* Alo, int64_t lda, int64_t strideA,
+                          const float* ia, int64_t strideIa, const void* B, const void* Blo,
+                          int64_t ldb, int64_t strideB, const float* ib, int64_t strideIb,
+                          double* C, int64_t ldc, const int64_t* crow, int64_t batch,
+                          void* stream) {
+  if (!crow || ldc < N) return NL_EARG;
+  return gemm_lp_impl(dtype, M, N, K, alpha, A, lda, strideA, ia, strideIa, B, ldb, strideB, ib,
+                      strideIb, 1.0, C, C, mk(1, ldc, 0), mk(1, 1, 0), 0, batch, 1, nullptr,
+                      stream, crow, Alo, Blo, true);
 }
 
 }  // extern "C"

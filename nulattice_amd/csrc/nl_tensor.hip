@@ -113,6 +113,19 @@ __device__ inline int scale_exp(float amax, float target) {
   return e;
 }
 
+// Per-row scale and inverse scale.  target > 0: the power of two 2^e <= target/amax (exact
+// rescaling, the row max lands in [target/2, target]); target < 0: exactly |target|/amax, so a row
+// whose nonzero elements share one magnitude (a contact interaction's blocks: every element is
+// +-g) converts to exactly +-|target| and its magnitude lives in the FP32 inverse scale.
+__device__ inline float row_scale(float amax, float target) {
+  if (target > 0.f) return ldexpf(1.f, scale_exp(amax, target));
+  return (amax > 0.f && isfinite(amax)) ? -target / amax : 1.f;
+}
+__device__ inline float row_iscale(float amax, float target) {
+  if (target > 0.f) return ldexpf(1.f, -scale_exp(amax, target));
+  return (amax > 0.f && isfinite(amax)) ? amax / -target : 1.f;
+}
+
 // One workgroup per output row: amax pass, then scaled conversion (per-row power-of-two scale).
 // Used when the row's elements are unit-stride or when the rows themselves are strided.
 __global__ __launch_bounds__(NT) void convert_rows_kernel(ConvArgs a) {
@@ -129,9 +142,8 @@ __global__ __launch_bounds__(NT) void convert_rows_kernel(ConvArgs a) {
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
   __syncthreads();
   float amax = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
-  int e = scale_exp(amax, a.target);
-  float sc = ldexpf(1.f, e);
-  if (threadIdx.x == 0) a.iscale[r] = ldexpf(1.f, -e);
+  const float sc = row_scale(amax, a.target);
+  if (threadIdx.x == 0) a.iscale[r] = row_iscale(amax, a.target);
   uint16_t* o = a.out + r * a.ldo;
   for (int64_t c = threadIdx.x; c < a.cend; c += NT) {
     float v = 0.f;
@@ -159,6 +171,128 @@ __global__ __launch_bounds__(NT) void cast_rows_kernel(ConvArgs a) {
     }
     o[c] = a.bf16 ? to_bf16_bits(v) : to_f16_bits(v);
   }
+}
+
+// ---- hi/lo split conversion (precision form with split operands)
+// x*sc = hi + lo + r with hi = 16-bit(x*sc), lo = 16-bit(x*sc - hi): |r| <= 2^-9 |lo| (BF16) or
+// 2^-12 |lo| (FP16), so the pair carries about 17 (BF16) or 23 (FP16) significant bits.  A GEMM
+// over K-concatenated blocks [hi | lo | hi] x [hi | hi | lo] then sums hi.hi + lo.hi + hi.lo, the
+// product to that precision without the lo.lo term (Henry, Tang & Heinecke, ARITH 2019 for BF16;
+// the same split in FP16 with per-row scales).
+struct SplitArgs {
+  int64_t R2, C, C2, ldo, kpad;   // rows = R1*R2, cols C = C1*C2, row pitch ldo, block pitch kpad
+  FastDiv fr2, fc2;
+  int64_t s[4];
+  const double* in;
+  uint16_t* out;
+  float* iscale;                  // per-row inverse scale (null: unscaled)
+  float target;
+  int nblk, lomask, bf16;         // block q holds the lo part when lomask bit q is set, else hi
+};
+
+__device__ inline float from16(uint16_t b, int bf16) {
+  return bf16 ? __uint_as_float((unsigned)b << 16)
+              : (float)__builtin_bit_cast(_Float16, b);
+}
+
+__device__ inline void split16(double xs, int bf16, uint16_t& hi, uint16_t& lo) {
+  const float f = (float)xs;
+  hi = bf16 ? to_bf16_bits(f) : to_f16_bits(f);
+  const float r = (float)(xs - (double)from16(hi, bf16));
+  lo = bf16 ? to_bf16_bits(r) : to_f16_bits(r);
+}
+
+// One workgroup per output row: amax pass (when scaled), then the split written to every block.
+__global__ __launch_bounds__(NT) void split_rows_kernel(SplitArgs a) {
+  __shared__ float red[NT / 64];
+  const int64_t r = blockIdx.x;
+  const int64_t r1 = fdiv((int)r, a.fr2), r2 = r - r1 * a.R2;
+  const double* base = a.in + r1 * a.s[0] + r2 * a.s[1];
+  float sc = 1.f;
+  if (a.iscale) {
+    float m = 0.f;
+    for (int64_t c = threadIdx.x; c < a.C; c += NT) {
+      const int64_t c1 = fdiv((int)c, a.fc2), c2 = c - c1 * a.C2;
+      m = fmaxf(m, (float)fabs(base[c1 * a.s[2] + c2 * a.s[3]]));
+    }
+    m = wave_max(m);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    const float amax = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+    sc = row_scale(amax, a.target);
+    if (threadIdx.x == 0) a.iscale[r] = row_iscale(amax, a.target);
+  }
+  uint16_t* o = a.out + r * a.ldo;
+  for (int64_t c = threadIdx.x; c < a.kpad; c += NT) {
+    uint16_t hi = 0, lo = 0;
+    if (c < a.C) {
+      const int64_t c1 = fdiv((int)c, a.fc2), c2 = c - c1 * a.C2;
+      split16(base[c1 * a.s[2] + c2 * a.s[3]] * (double)sc, a.bf16, hi, lo);
+    }
+    for (int q = 0; q < a.nblk; ++q) o[q * a.kpad + c] = ((a.lomask >> q) & 1) ? lo : hi;
+  }
+}
+
+// Transposing split (output rows are the unit-stride direction of the source): 64x64 tiles
+// through LDS, per-row scales from amax bits gathered by amax_split_kernel.
+__global__ __launch_bounds__(NT) void amax_split_kernel(SplitArgs a, int64_t R, unsigned* bits) {
+  __shared__ float red[4][64];
+  const int tr = threadIdx.x & 63, tg = threadIdx.x >> 6;
+  const int64_t r = blockIdx.x * 64 + tr, c0 = blockIdx.y * 64;
+  float m = 0.f;
+  if (r < R) {
+    const int64_t r1 = fdiv((int)r, a.fr2), r2 = r - r1 * a.R2;
+    const double* base = a.in + r1 * a.s[0] + r2 * a.s[1];
+    for (int cc = tg; cc < 64; cc += 4) {
+      const int64_t c = c0 + cc;
+      if (c < a.C) {
+        const int64_t c1 = fdiv((int)c, a.fc2), c2 = c - c1 * a.C2;
+        m = fmaxf(m, (float)fabs(base[c1 * a.s[2] + c2 * a.s[3]]));
+      }
+    }
+  }
+  red[tg][tr] = m;
+  __syncthreads();
+  if (tg == 0 && r < R) {
+    m = fmaxf(fmaxf(red[0][tr], red[1][tr]), fmaxf(red[2][tr], red[3][tr]));
+    atomicMax(&bits[r], __float_as_uint(m));
+  }
+}
+
+__global__ __launch_bounds__(NT) void split_tiles_kernel(SplitArgs a, int64_t R,
+                                                         const unsigned* bits) {
+  __shared__ uint16_t thi[64][66], tlo[64][66];
+  const int tr = threadIdx.x & 63, tg = threadIdx.x >> 6;
+  const int64_t r0 = blockIdx.x * 64, c0 = blockIdx.y * 64, r = r0 + tr;
+  if (r < R) {
+    const float sc = bits ? row_scale(__uint_as_float(bits[r]), a.target) : 1.f;
+    const int64_t r1 = fdiv((int)r, a.fr2), r2 = r - r1 * a.R2;
+    const double* base = a.in + r1 * a.s[0] + r2 * a.s[1];
+    for (int cc = tg; cc < 64; cc += 4) {
+      const int64_t c = c0 + cc;
+      uint16_t hi = 0, lo = 0;
+      if (c < a.C) {
+        const int64_t c1 = fdiv((int)c, a.fc2), c2 = c - c1 * a.C2;
+        split16(base[c1 * a.s[2] + c2 * a.s[3]] * (double)sc, a.bf16, hi, lo);
+      }
+      thi[tr][cc] = hi;
+      tlo[tr][cc] = lo;
+    }
+  }
+  __syncthreads();
+  for (int rr = tg; rr < 64; rr += 4) {
+    const int64_t row = r0 + rr, c = c0 + tr;
+    if (row < R && c < a.kpad)
+      for (int q = 0; q < a.nblk; ++q)
+        a.out[row * a.ldo + q * a.kpad + c] = ((a.lomask >> q) & 1) ? tlo[rr][tr] : thi[rr][tr];
+  }
+}
+
+__global__ __launch_bounds__(NT) void bits_to_iscale_kernel(int64_t R, float target,
+                                                            float* iscale) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < R;
+       t += (int64_t)gridDim.x * blockDim.x)
+    iscale[t] = row_iscale(__uint_as_float(__float_as_uint(iscale[t])), target);
 }
 
 // Transposing variant (the output rows are the unit-stride direction of the source): 64x64
@@ -202,7 +336,7 @@ __global__ __launch_bounds__(NT) void convert_tiles_kernel(ConvArgs a, int64_t R
   const int tr = threadIdx.x & 63, tg = threadIdx.x >> 6;
   const int64_t r0 = blockIdx.x * 64, c0 = blockIdx.y * 64, r = r0 + tr;
   if (r < R) {
-    float sc = bits ? ldexpf(1.f, scale_exp(__uint_as_float(bits[r]), a.target)) : 1.f;
+    float sc = bits ? row_scale(__uint_as_float(bits[r]), a.target) : 1.f;
     int64_t r1 = fdiv((int)r, a.fr2), r2 = r - r1 * a.R2;
     const double* base = a.in + r1 * a.s[0] + r2 * a.s[1];
     for (int cc = tg; cc < 64; cc += 4) {
@@ -230,7 +364,8 @@ template <typename S>   // source rows: double (cast here) or uint16_t (BF16 alr
 __global__ __launch_bounds__(NT) void gather_convert_kernel(int64_t R, int64_t ncol, FastDiv fn,
                                                             int64_t kb, int64_t ldk,
                                                             const int64_t* idx, const S* src,
-                                                            int64_t lds, uint16_t* out) {
+                                                            int64_t lds, uint16_t* out,
+                                                            int64_t ldo) {
   __shared__ uint16_t tile[64][66];
   const int tr = threadIdx.x & 63, tg = threadIdx.x >> 6;
   const int64_t r0 = blockIdx.x * 64, c0 = blockIdx.y * 64, r = r0 + tr;
@@ -252,7 +387,7 @@ __global__ __launch_bounds__(NT) void gather_convert_kernel(int64_t R, int64_t n
   __syncthreads();
   for (int rr = tg; rr < 64; rr += 4) {
     const int64_t row = r0 + rr, c = c0 + tr;
-    if (row < R && c < ldk) out[row * ldk + c] = tile[rr][tr];
+    if (row < R && c < ldk) out[row * ldo + c] = tile[rr][tr];
   }
 }
 
@@ -283,7 +418,7 @@ __global__ __launch_bounds__(NT) void transpose16_kernel(int64_t R, int64_t C, c
 // rows (checked on the host).
 __global__ __launch_bounds__(NT) void gather16_kernel(int64_t ncol, int64_t kb, int64_t ldk,
                                                       const int64_t* idx, const uint16_t* src,
-                                                      int64_t lds, uint16_t* out) {
+                                                      int64_t lds, uint16_t* out, int64_t ldo) {
   __shared__ uint16_t tile[64][256 + 4];
   const int b = blockIdx.z, tid = threadIdx.x;
   const int64_t n0 = (int64_t)blockIdx.x * 256, k0 = (int64_t)blockIdx.y * 64;
@@ -307,7 +442,7 @@ __global__ __launch_bounds__(NT) void gather16_kernel(int64_t ncol, int64_t kb, 
     uint16_t w[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) w[u] = tile[kq + u][nn];
-    *(uint2*)(out + ((int64_t)b * ncol + n) * ldk + k) =
+    *(uint2*)(out + ((int64_t)b * ncol + n) * ldo + k) =
         make_uint2(w[0] | ((unsigned)w[1] << 16), w[2] | ((unsigned)w[3] << 16));
   }
 }
@@ -376,7 +511,7 @@ __global__ __launch_bounds__(NT) void iscale_kernel(int64_t R, float target, flo
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < R;
        t += (int64_t)gridDim.x * blockDim.x) {
     float amax = __uint_as_float(__float_as_uint(iscale[t]));
-    iscale[t] = ldexpf(1.f, -scale_exp(amax, target));
+    iscale[t] = row_iscale(amax, target);
   }
 }
 
@@ -410,6 +545,35 @@ __global__ __launch_bounds__(NT) void tau_kernel(int p, int h, FastDiv fp, FastD
       const double v = tau_value(x[e], ta[i], tb[j], ta[j], tb[i]);
       if constexpr (sizeof(T) == 8) y[e] = v;
       else y[e] = to_bf16_bits_2r(v);
+    }
+  }
+}
+
+// tau as the hi/lo split of the precision form with split operands: row (a,b) of out holds
+// [hi(h*h) | lo(h*h)] (row pitch 2*h*h), optionally with the FP64 tau beside it.
+__global__ __launch_bounds__(NT) void tau_split_kernel(int p, int h, FastDiv fp, FastDiv fh,
+                                                       const double* t1, const double* t2,
+                                                       double* tau, uint16_t* out) {
+  __shared__ double ta[NT], tb[NT];
+  const int hh = h * h;
+  for (int q = blockIdx.x; q < p * p; q += gridDim.x) {
+    const int a = fdiv(q, fp), b = q - a * p;
+    const double* x = t2 + (int64_t)q * hh;
+    uint16_t* y = out + (int64_t)q * 2 * hh;
+    __syncthreads();
+    if (threadIdx.x < h) {
+      ta[threadIdx.x] = t1[a * h + threadIdx.x];
+      tb[threadIdx.x] = t1[b * h + threadIdx.x];
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < hh; e += NT) {
+      const int i = fdiv(e, fh), j = e - i * h;
+      const double v = tau_value(x[e], ta[i], tb[j], ta[j], tb[i]);
+      if (tau) tau[(int64_t)q * hh + e] = v;
+      uint16_t hi, lo;
+      split16(v, 1, hi, lo);
+      y[e] = hi;
+      y[hh + e] = lo;
     }
   }
 }
@@ -487,13 +651,30 @@ struct AssembleArgs {
   FastDiv fp, fh;                       // divisors p, h
   const double *S0, *Sab, *Sij, *Spp;   // S0, Sab, Sij [a][b][i][j]; Spp [(a,i)][(b,j)]; any null
   const double *Xpp, *Xhh;              // dressed diagonals (full p x p, h x h matrices)
+  const double *dpp, *dhh;              // optional: (dpp[a] + dpp[b] + dhh[i] + dhh[j]) t2[a,b,i,j]
+                                        // is added to H2 in FP64 (the diagonal of a dressing
+                                        // product that ran in low precision without it)
   const double *t2, *vpphh, *t1n;       // old t2, v_pphh, new t1 (for the fused energy)
   double mixing;
   double* t2n;                          // new t2 (may alias nothing)
   double* part;                         // per-block partials: [3][gridDim]: E2, |H2|^2, spare
   double* H2out;                        // optional: raw residual H2 (for adjoint / diagnostics)
   uint16_t* t2n16;                      // optional: BF16 image of t2n (same layout, RNE via FP32)
+  int split16;                          // t2n16 holds the hi/lo split instead: row a of t2n as
+                                        // [hi(p*h*h) | lo(p*h*h)] (row pitch 2*p*h*h)
 };
+
+__device__ inline void write_image(const AssembleArgs& g, int64_t t, int64_t a, double tn) {
+  if (!g.split16) {
+    g.t2n16[t] = to_bf16_bits_2r(tn);
+    return;
+  }
+  const int64_t row = g.p * g.h * g.h, o = t + a * row;
+  uint16_t hi, lo;
+  split16(tn, 1, hi, lo);
+  g.t2n16[o] = hi;
+  g.t2n16[o + row] = lo;
+}
 
 __global__ __launch_bounds__(NT) void assemble_t2_kernel(AssembleArgs g) {
   const int64_t p = g.p, h = g.h, hh = h * h, total = p * p * hh;
@@ -513,10 +694,12 @@ __global__ __launch_bounds__(NT) void assemble_t2_kernel(AssembleArgs g) {
     }
     double denom = -(g.Xpp[a * p + a] + g.Xpp[b * p + b] + g.Xhh[i * h + i] + g.Xhh[j * h + j]);
     double told = g.t2[t];
+    if (g.dpp) H += (g.dpp[a] + g.dpp[b]) * told;
+    if (g.dhh) H += (g.dhh[i] + g.dhh[j]) * told;
     double nt = told + H / denom;
     double tn = g.mixing * told + (1.0 - g.mixing) * nt;
     if (g.t2n) g.t2n[t] = tn;
-    if (g.t2n16) g.t2n16[t] = to_bf16_bits_2r(tn);
+    if (g.t2n16) write_image(g, t, a, tn);
     if (g.H2out) g.H2out[t] = H;
     if (g.vpphh) {
       double v = g.vpphh[t];
@@ -549,9 +732,11 @@ __global__ __launch_bounds__(NT) void assemble_pair_kernel(AssembleArgs g) {
   auto finish = [&](int64_t a, int64_t b, int i, int j, int64_t t, double H) {
     double denom = -(g.Xpp[a * p + a] + g.Xpp[b * p + b] + g.Xhh[i * h + i] + g.Xhh[j * h + j]);
     double told = g.t2[t];
+    if (g.dpp) H += (g.dpp[a] + g.dpp[b]) * told;
+    if (g.dhh) H += (g.dhh[i] + g.dhh[j]) * told;
     double tn = g.mixing * told + (1.0 - g.mixing) * (told + H / denom);
     if (g.t2n) g.t2n[t] = tn;
-    if (g.t2n16) g.t2n16[t] = to_bf16_bits_2r(tn);
+    if (g.t2n16) write_image(g, t, a, tn);
     if (g.H2out) g.H2out[t] = H;
     if (g.vpphh) {
       double v = g.vpphh[t];
@@ -807,7 +992,7 @@ int nl_permute(int rank, const int64_t* out_shape, const int64_t* in_strides, do
 int nl_convert_rows(int dtype, int64_t R1, int64_t R2, int64_t C1, int64_t C2,
                     const int64_t* strides4, const double* in, void* out, int64_t ldo,
                     float* iscale, float target, void* stream) {
-  const bool scaled = target > 0.f;
+  const bool scaled = target != 0.f;
   if ((dtype != 0 && dtype != 1) || !strides4 || !in || !out || (scaled && !iscale))
     return NL_EARG;
   if (R1 < 0 || R2 < 1 || C1 < 0 || C2 < 1 || ldo < C1 * C2 || (ldo % 8)) return NL_EARG;
@@ -853,6 +1038,49 @@ int nl_convert_rows(int dtype, int64_t R1, int64_t R2, int64_t C1, int64_t C2,
   return NL_OK;
 }
 
+int nl_convert_split(int dtype, int64_t R1, int64_t R2, int64_t C1, int64_t C2,
+                     const int64_t* strides4, const double* in, void* out, int64_t ldo,
+                     int64_t kpad, int nblk, int lomask, float* iscale, float target,
+                     void* stream) {
+  const bool scaled = target != 0.f;
+  if ((dtype != 0 && dtype != 1) || !strides4 || !in || !out || (scaled && !iscale))
+    return NL_EARG;
+  if (nblk < 1 || nblk > 4 || lomask < 0 || lomask >= (1 << nblk)) return NL_EARG;
+  if (R1 < 0 || R2 < 1 || C1 < 0 || C2 < 1 || kpad < C1 * C2 || ldo < nblk * kpad) return NL_EARG;
+  if (R1 * R2 == 0) return NL_OK;
+  if (R1 * R2 > INT32_MAX || kpad > INT32_MAX || C1 * C2 > INT32_MAX) return NL_EARG;
+  SplitArgs a;
+  a.R2 = R2; a.C = C1 * C2; a.C2 = C2; a.ldo = ldo; a.kpad = kpad;
+  a.fr2 = nl_fastdiv(R2); a.fc2 = nl_fastdiv(C2);
+  for (int d = 0; d < 4; ++d) a.s[d] = strides4[d];
+  a.in = in; a.out = (uint16_t*)out; a.iscale = scaled ? iscale : nullptr; a.target = target;
+  a.nblk = nblk; a.lomask = lomask; a.bf16 = dtype == 0;
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t R = R1 * R2;
+  if (a.s[1] == 1 && R2 >= 8 && a.s[3] != 1 && nl_cdiv(kpad, 64) <= 65535) {
+    unsigned* bits = scaled ? (unsigned*)iscale : nullptr;
+    if (scaled) {
+      hipLaunchKernelGGL(zero_u32_kernel, dim3(grid_for(R)), dim3(NT), 0, st, R, bits);
+      NL_LAUNCH_CHECK();
+      dim3 g1((unsigned)nl_cdiv(R, 64), (unsigned)nl_cdiv(a.C, 64));
+      hipLaunchKernelGGL(amax_split_kernel, g1, dim3(NT), 0, st, a, R, bits);
+      NL_LAUNCH_CHECK();
+    }
+    dim3 g2((unsigned)nl_cdiv(R, 64), (unsigned)nl_cdiv(kpad, 64));
+    hipLaunchKernelGGL(split_tiles_kernel, g2, dim3(NT), 0, st, a, R, (const unsigned*)bits);
+    NL_LAUNCH_CHECK();
+    if (scaled) {
+      hipLaunchKernelGGL(bits_to_iscale_kernel, dim3(grid_for(R)), dim3(NT), 0, st, R, target,
+                         iscale);
+      NL_LAUNCH_CHECK();
+    }
+    return NL_OK;
+  }
+  hipLaunchKernelGGL(split_rows_kernel, dim3((unsigned)R), dim3(NT), 0, st, a);
+  NL_LAUNCH_CHECK();
+  return NL_OK;
+}
+
 int nl_gather_convert(int64_t nb, int64_t kb, int64_t ncol, const int64_t* idx, const double* src,
                       int64_t lds, void* out, int64_t ldk, void* stream) {
   if (nb < 0 || kb < 0 || ncol < 0 || ldk < kb || (ldk % 8) || lds < ncol) return NL_EARG;
@@ -861,29 +1089,54 @@ int nl_gather_convert(int64_t nb, int64_t kb, int64_t ncol, const int64_t* idx, 
   if (!idx || !src || !out || R >= INT32_MAX || nl_cdiv(ldk, 64) > 65535) return NL_EARG;
   dim3 g((unsigned)nl_cdiv(R, 64), (unsigned)nl_cdiv(ldk, 64));
   hipLaunchKernelGGL((gather_convert_kernel<double>), g, dim3(NT), 0, (hipStream_t)stream, R,
-                     ncol, nl_fastdiv(ncol), kb, ldk, idx, src, lds, (uint16_t*)out);
+                     ncol, nl_fastdiv(ncol), kb, ldk, idx, src, lds, (uint16_t*)out, ldk);
   NL_LAUNCH_CHECK();
   return NL_OK;
 }
 
-int nl_gather_rows16(int64_t nb, int64_t kb, int64_t ncol, const int64_t* idx, const void* src,
-                     int64_t lds, void* out, int64_t ldk, void* stream) {
-  if (nb < 0 || kb < 0 || ncol < 0 || ldk < kb || (ldk % 8) || lds < ncol) return NL_EARG;
+static int gather_rows16_impl(int64_t nb, int64_t kb, int64_t ncol, const int64_t* idx,
+                              const void* src, int64_t lds, void* out, int64_t ldk, int64_t ldo,
+                              void* stream) {
+  if (nb < 0 || kb < 0 || ncol < 0 || ldk < kb || (ldk % 8) || lds < ncol || ldo < ldk)
+    return NL_EARG;
   const int64_t R = nb * ncol;
   if (R == 0 || ldk == 0) return NL_OK;
   if (!idx || !src || !out || R >= INT32_MAX || nl_cdiv(ldk, 64) > 65535) return NL_EARG;
-  if (ncol % 4 == 0 && lds % 4 == 0 && (((uintptr_t)src) & 7) == 0 &&
+  if (ncol % 4 == 0 && lds % 4 == 0 && ldo % 4 == 0 && (((uintptr_t)src) & 7) == 0 &&
       (((uintptr_t)out) & 7) == 0 && nb <= 65535) {
     dim3 g((unsigned)nl_cdiv(ncol, 256), (unsigned)nl_cdiv(ldk, 64), (unsigned)nb);
     hipLaunchKernelGGL(gather16_kernel, g, dim3(NT), 0, (hipStream_t)stream, ncol, kb, ldk, idx,
-                       (const uint16_t*)src, lds, (uint16_t*)out);
+                       (const uint16_t*)src, lds, (uint16_t*)out, ldo);
     NL_LAUNCH_CHECK();
     return NL_OK;
   }
   dim3 g((unsigned)nl_cdiv(R, 64), (unsigned)nl_cdiv(ldk, 64));
   hipLaunchKernelGGL((gather_convert_kernel<uint16_t>), g, dim3(NT), 0, (hipStream_t)stream, R,
                      ncol, nl_fastdiv(ncol), kb, ldk, idx, (const uint16_t*)src, lds,
-                     (uint16_t*)out);
+                     (uint16_t*)out, ldo);
+  NL_LAUNCH_CHECK();
+  return NL_OK;
+}
+
+int nl_gather_rows16(int64_t nb, int64_t kb, int64_t ncol, const int64_t* idx, const void* src,
+                     int64_t lds, void* out, int64_t ldk, void* stream) {
+  return gather_rows16_impl(nb, kb, ncol, idx, src, lds, out, ldk, ldk, stream);
+}
+
+int nl_gather_rows16_into(int64_t nb, int64_t kb, int64_t ncol, const int64_t* idx,
+                          const void* src, int64_t lds, void* out, int64_t ldk, int64_t ldo,
+                          void* stream) {
+  return gather_rows16_impl(nb, kb, ncol, idx, src, lds, out, ldk, ldo, stream);
+}
+
+int nl_tau_split(int64_t p, int64_t h, const double* t1, const double* t2, double* tau,
+                 void* out, void* stream) {
+  if (p < 0 || h < 0 || !t1 || !t2 || !out) return NL_EARG;
+  if (p * p * h * h == 0) return NL_OK;
+  if (p * p > INT32_MAX || h * h > INT32_MAX || p * h > INT32_MAX || h > NT) return NL_EARG;
+  hipLaunchKernelGGL(tau_split_kernel, dim3((unsigned)std::min<int64_t>(p * p, 65536)),
+                     dim3(NT), 0, (hipStream_t)stream, (int)p, (int)h, nl_fastdiv(p),
+                     nl_fastdiv(h), t1, t2, tau, (uint16_t*)out);
   NL_LAUNCH_CHECK();
   return NL_OK;
 }
@@ -1076,7 +1329,9 @@ static int assemble_t2_impl(int64_t p, int64_t h, const double* S0, const double
                             const double* Sij, const double* Spp, const double* Xpp,
                             const double* Xhh, const double* t2, const double* vpphh,
                             const double* t1n, double mixing, double* t2n, double* H2out,
-                            double* part, uint16_t* t2n16, void* stream) {
+                            double* part, uint16_t* t2n16, void* stream,
+                            const double* dpp = nullptr, const double* dhh = nullptr,
+                            int split16 = 0) {
   if (p < 0 || h < 0 || !Xpp || !Xhh || !t2 || !part) return NL_EARG;
   if (vpphh && !t1n) return NL_EARG;
   int64_t n = p * p * h * h;
@@ -1084,7 +1339,7 @@ static int assemble_t2_impl(int64_t p, int64_t h, const double* S0, const double
   AssembleArgs g;
   g.p = p; g.h = h; g.S0 = S0; g.Sab = Sab; g.Sij = Sij; g.Spp = Spp; g.Xpp = Xpp; g.Xhh = Xhh;
   g.t2 = t2; g.vpphh = vpphh; g.t1n = t1n; g.mixing = mixing; g.t2n = t2n; g.part = part;
-  g.H2out = H2out; g.t2n16 = t2n16;
+  g.H2out = H2out; g.t2n16 = t2n16; g.dpp = dpp; g.dhh = dhh; g.split16 = split16;
   g.fp = nl_fastdiv(p); g.fh = nl_fastdiv(h);
   if (h <= AH && (Sij || Spp) && p * p < INT32_MAX)
     hipLaunchKernelGGL(assemble_pair_kernel, dim3(grid_for(n, 4096)), dim3(NT), 0,
@@ -1111,6 +1366,34 @@ int nl_assemble_t2_16(int64_t p, int64_t h, const double* S0, const double* Sab,
   if (!t2n16) return NL_EARG;
   return assemble_t2_impl(p, h, S0, Sab, Sij, Spp, Xpp, Xhh, t2, vpphh, t1n, mixing, t2n, H2out,
                           part, (uint16_t*)t2n16, stream);
+}
+
+int nl_assemble_t2_ext(int64_t p, int64_t h, const double* S0, const double* Sab,
+                       const double* Sij, const double* Spp, const double* Xpp, const double* Xhh,
+                       const double* dpp, const double* dhh, const double* t2,
+                       const double* vpphh, const double* t1n, double mixing, double* t2n,
+                       double* H2out, double* part, void* t2n16, int image_kind,
+                       void* stream) {
+  if (image_kind != 0 && image_kind != 1) return NL_EARG;
+  return assemble_t2_impl(p, h, S0, Sab, Sij, Spp, Xpp, Xhh, t2, vpphh, t1n, mixing, t2n, H2out,
+                          part, (uint16_t*)t2n16, stream, dpp, dhh, image_kind);
+}
+
+__global__ void take_diag_kernel(int64_t n, double* X, double* d) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    d[i] = X[i * n + i];
+    X[i * n + i] = 0.0;
+  }
+}
+
+int nl_take_diag(int64_t n, double* X, double* d, void* stream) {
+  if (n < 0 || (n > 0 && (!X || !d))) return NL_EARG;
+  if (n == 0) return NL_OK;
+  hipLaunchKernelGGL(take_diag_kernel, dim3(grid_for(n)), dim3(NT), 0, (hipStream_t)stream, n,
+                     X, d);
+  NL_LAUNCH_CHECK();
+  return NL_OK;
 }
 
 int nl_assemble_t2_adj(int64_t p, int64_t h, const double* L, double* gSab, double* gSij,

@@ -143,6 +143,8 @@ class Buffers:
         self.q_hh = z(h, h)
         self.Dpp, self.Dhh = z(p, p), z(h, h)     # v.t2 dressings shared by both parts
         self.Mpp, self.Mhh = z(p, p), z(h, h)     # combined Sab / Sij dressing matrices
+        self.dMpp = z(p)             # Mpp's diagonal when the Sab product runs in low precision
+        self.dpp = None              # the diagonal the assembly adds in FP64 (None: none)
         self.ret1 = z(p, p)
         self.T0 = z(p, h, h)
         self.tau = z(p, p, h, h)
@@ -170,6 +172,10 @@ def _segsum(plan, X, src, out, alpha, src_stride=1, dst_stride=1):
 
 
 DEFAULT_PREC = {"ring": "fp64", "hh": "fp64", "dress": "fp64", "ladder": "fp64"}
+
+# The diagonal of the Sab dressing matrix is applied in FP64 by the assembly when the Sab
+# product runs in low precision (exact regrouping: X t2 = diag(X) t2 + offdiag(X) t2).
+DIAG_SPLIT = True
 
 # Form tau directly in BF16 when every product that reads it runs in BF16 (False: always FP64).
 TAU16 = True
@@ -228,6 +234,19 @@ def t1_part(ham, t1, t2, B, prec=DEFAULT_PREC):
     contract("ki,ak->ai", Xhh, t1, H1)
 
 
+SPLIT = (2, 0b10)       # [hi | lo] rows of a split operand image
+
+
+def _fast_split(ham, prec):
+    """Every family on BF16 split operands (the "bf16x3" precision form): t2's and tau's split
+    images are made once per iteration (t2's by the previous step's assembly) and shared by the
+    products instead of one conversion per product."""
+    p, h = ham.p, ham.h
+    return (all(prec.get(k) == "bf16x3" for k in ("ring", "hh", "dress", "ladder"))
+            and not any(k in prec for k in ("ring_W", "ring_S", "hh_W", "hh_S"))
+            and p * p * h * h * 2 < 2 ** 31 - 1)
+
+
 def _lp_tau(ham, prec):
     """tau feeds only the hh products, the pp ladder and R56: when all of them run in BF16 (and
     are large enough not to fall back to FP64) it is formed directly in BF16."""
@@ -252,9 +271,21 @@ def t2_part(ham, t1, t2, B, prec=DEFAULT_PREC):
     # are large enough not to fall back to FP64) it is formed directly in BF16, as [ab][ij] rows
     # and, for the Whh product (summed over c,d), as [ij][cd] rows; the FP64 tau is not written
     tau = B.tau
-    lp_tau = _lp_tau(ham, prec)
+    split = _fast_split(ham, prec)
+    lp_tau = _lp_tau(ham, prec) or split
     tau16 = None
-    if lp_tau:
+    if split:
+        # tau only as its split image: [ab][hi(ij) | lo(ij)] rows, and [ij][hi(cd) | lo(cd)]
+        # rows (the Whh product sums over c, d) by two 16-bit transposes
+        n, hh, pp = p * p * h * h, h * h, p * p
+        tau16 = ops.WS.get("tau_split", 2 * n, dtype=torch.int16, device=t2.device)
+        tauT = ops.WS.get("tau_splitT", 2 * n, dtype=torch.int16, device=t2.device)
+        st = ops._stream()
+        _lib.call("nl_tau_split", p, h, ops._p(t1), ops._p(t2), None, ops._p(tau16), st)
+        for q in range(2):
+            _lib.call("nl_transpose16_into", 1, pp, hh, ops._off16(tau16, q * hh), 2 * hh, 0,
+                      ops._off16(tauT, q * pp), 2 * pp, 0, pp, st)
+    elif lp_tau:
         ldT = (p * p + 7) // 8 * 8                   # [ij][cd] rows, zero-padded to 8
         tau16 = ops.WS.get("tau16", p * p * h * h, dtype=torch.int16, device=t2.device)
         tauT = ops.WS.get("tau16T", h * h * ldT, dtype=torch.int16, device=t2.device)
@@ -281,12 +312,16 @@ def t2_part(ham, t1, t2, B, prec=DEFAULT_PREC):
     permute_into(B.zz.permute(0, 1, 3, 2), B.Whh, -1.0, 1.0)
     Sab = B.Sab
     with ops.lp_reuse(tau, absent=True) if lp_tau else contextlib.nullcontext():
-        if lp_tau:
+        if split:
+            ops.lp_provide(tau, "cdij", ["i", "j"], ["c", "d"], tauT, 2 * p * p, parts=SPLIT)
+            ops.lp_provide(tau, "abkl", ["a", "b"], ["k", "l"], tau16, 2 * h * h, parts=SPLIT)
+        elif lp_tau:
             ops.lp_provide(tau, "cdij", ["i", "j"], ["c", "d"], tauT, ldT)
             ops.lp_provide(tau, "abkl", ["a", "b"], ["k", "l"], tau16, h * h)
-        contract("cdkl,cdij->klij", ham.v_pphh, tau, B.Whh, alpha=0.5, precision=prec["hh"])
-        contract("abkl,klij->abij", tau, B.Whh, S0, alpha=0.5, precision=prec["hh"],
-                 addend=ham.v_pphh)
+        contract("cdkl,cdij->klij", ham.v_pphh, tau, B.Whh, alpha=0.5,
+                 precision=prec.get("hh_W", prec["hh"]))
+        contract("abkl,klij->abij", tau, B.Whh, S0, alpha=0.5,
+                 precision=prec.get("hh_S", prec["hh"]), addend=ham.v_pphh)
         _pp_ladder(ham, tau, S0, prec, tau16)
         # ---- Sab: P(ab) group
         permute_into(ham.v_phhh, B.N56)
@@ -296,6 +331,13 @@ def t2_part(ham, t1, t2, B, prec=DEFAULT_PREC):
     permute_into(Xpp, B.Mpp)                                                 # X_pp t2 (P(ab) image)
     if pl.n:
         ops.axpby(1.0, B.ret1, 1.0, B.Mpp)                                   # + A38
+    # Mpp's diagonal (the Fock diagonal: the largest term of the residual, the one the Jacobi
+    # denominator divides out) leaves a low-precision Sab product: the assembly adds
+    # (Mpp[a,a] + Mpp[b,b]) t2[a,b,i,j] in FP64, the P(ab) image of the diagonal part
+    B.dpp = None
+    if prec["dress"] != "fp64" and DIAG_SPLIT:
+        _lib.call("nl_take_diag", p, ops._p(B.Mpp), ops._p(B.dMpp), ops._stream())
+        B.dpp = B.dMpp
     if getattr(B, "t2_16", None) is not None and p >= ops.LP_MIN_DIM:
         _sab_bf16(ham, t1, B, Sab)
     else:
@@ -320,13 +362,14 @@ def t2_part(ham, t1, t2, B, prec=DEFAULT_PREC):
         permute_into(t2.permute(0, 2, 1, 3), Tr)
         t2ring = Tr.permute(0, 2, 1, 3)
     with ops.lp_reuse(t2ring):
-        contract("cdkl,bdjl->ckbj", vpphh, t2ring, W, alpha=0.5, precision=prec["ring"],
-                 tag="ring_W", addend=ham.Wc)                                # A24 + A22
+        contract("cdkl,bdjl->ckbj", vpphh, t2ring, W, alpha=0.5,
+                 precision=prec.get("ring_W", prec["ring"]), tag="ring_W",
+                 addend=ham.Wc)                                              # A24 + A22
         if pl.n:
             _segsum(pl.ret4, h, t1, W, 1.0)                                  # A37
         contract("cjkl,bl->ckbj", ham.v_phhh, t1, W, alpha=-1.0)             # A29
-        contract("acik,ckbj->aibj", t2ring, W, B.Spp, beta=0.0, precision=prec["ring"],
-                 tag="ring")
+        contract("acik,ckbj->aibj", t2ring, W, B.Spp, beta=0.0,
+                 precision=prec.get("ring_S", prec["ring"]), tag="ring")
     contract("idkl,adkj->ilaj", B.Xidkl, t2, B.Yilaj, beta=0.0, precision=prec["hh"])  # A34
     # Spp enters only through P(ij)P(ab), which is invariant under (a,i) <-> (b,j): the A27 term
     # -t1[a,k] U[b,k,i,j] may stand as -U[a,l,j,i] t1[b,l], so both tails are one product
@@ -371,17 +414,20 @@ def _pp_ladder(ham, tau, S0, prec, tau16=None):
 def _block_gemm(pb, src, out, alpha, ncol, tag, precision="fp64", src16=None):
     """out[rows_b] += alpha * V_b src[cols_b] for every packed block b (row gather, batched MFMA
     GEMM, row scatter; rows of src/out have ncol contiguous elements).  With a low precision
-    the blocks V_b (constant, converted once) and the gathered rows (transposed per block) are
-    16-bit and the batched GEMM accumulates in FP32: BF16 gathers and casts in one pass (or, given
-    src16, the BF16 rows of src, gathers them) and adds the block rows straight into out from
-    the GEMM; FP16 (per-row scales) goes through FP64 gathered rows and block results."""
+    the blocks V_b (constant, converted once with exact per-row scales) and the gathered rows
+    (transposed per block) are 16-bit and the batched GEMM accumulates in FP32: BF16 gathers and
+    casts in one pass (or, given src16, the BF16 rows of src, gathers them) and adds the block
+    rows straight into out from the GEMM; FP16 (per-row scales) and the split forms go through
+    FP64 gathered rows and block results (split: [V | V] x [hi | lo] of the gathered rows)."""
     if pb.nb == 0:
         return
     st = ops._stream()
     ev = ops._timed(tag)
     if ev:
         ev[0].record()
-    if precision != "bf16":
+    # split operands with src16 given: src16 rows are [hi(ncol) | lo(ncol)] of src
+    split16 = precision == "bf16x3" and src16 is not None
+    if precision != "bf16" and not split16:
         G = ops.WS.get(tag + "_G", pb.nb * pb.kb * ncol, device=src.device)
         R = ops.WS.get(tag + "_R", pb.nb * pb.mb * ncol, device=src.device)
         _lib.call("nl_gather_rows", pb.nb * pb.kb, ncol, ops._p(pb.cols), ops._p(src), ncol,
@@ -391,40 +437,58 @@ def _block_gemm(pb, src, out, alpha, ncol, tag, precision="fp64", src16=None):
                   pb.kb, pb.mb * pb.kb, ops._p(G), ncol, pb.kb * ncol, ctypes.c_double(0.0),
                   ops._p(R), ncol, pb.mb * ncol, pb.nb, 1, ctypes.c_void_p(0), st)
     else:
-        dt = ops.BF16 if precision == "bf16" else ops.FP16
-        scaled = dt != ops.BF16         # BF16: plain cast, no per-row scales
-        target = 2.0 ** 14 if scaled else 0.0
-        ldk = (pb.kb + 7) // 8 * 8
+        dt, split = ops.lp_kind(precision)
+        nblk = 2 if split else 1
+        kpad = (pb.kb + 7) // 8 * 8
+        ldk = nblk * kpad
         lp = getattr(pb, "lp", None)
-        if lp is None or lp[0] != dt:
+        if lp is None or lp[0] != (dt, split):
+            # the blocks are Hamiltonian elements: exact per-row scales in both 16-bit types
+            # (split: the block twice, [V | V], against [hi | lo] of the gathered rows)
             Vb = torch.empty(pb.nb * pb.mb * ldk, dtype=torch.int16, device=src.device)
-            iv = (torch.empty(pb.nb * pb.mb, dtype=torch.float32, device=src.device) if scaled
-                  else None)
+            iv = torch.empty(pb.nb * pb.mb, dtype=torch.float32, device=src.device)
             s4 = (ctypes.c_int64 * 4)(pb.mb * pb.kb, pb.kb, 0, 1)
-            _lib.call("nl_convert_rows", dt, pb.nb, pb.mb, 1, pb.kb, s4, ops._p(pb.V),
-                      ops._p(Vb), ldk, ops._p(iv), ctypes.c_float(target), st)
-            pb.lp = lp = (dt, Vb, iv)
+            tgt = ctypes.c_float(ops.lp_target(dt, True))
+            if split:
+                _lib.call("nl_convert_split", dt, pb.nb, pb.mb, 1, pb.kb, s4, ops._p(pb.V),
+                          ops._p(Vb), ldk, kpad, 2, 0, ops._p(iv), tgt, st)
+            else:
+                _lib.call("nl_convert_rows", dt, pb.nb, pb.mb, 1, pb.kb, s4, ops._p(pb.V),
+                          ops._p(Vb), ldk, ops._p(iv), tgt, st)
+            pb.lp = lp = ((dt, split), Vb, iv)
         _, Vb, iv = lp
+        target = ops.lp_target(dt, False)
+        scaled = target != 0.0
         Gt = ops.WS.get(tag + "_Gt", pb.nb * ncol * ldk, dtype=torch.int16, device=src.device)
-        if not scaled:
+        if precision == "bf16" or split16:
             # BF16: gather + transposing cast in one pass, block rows added straight into out
-            if src16 is not None:
+            # (split: the hi and lo halves of the gathered rows, [V | V] x [hi | lo])
+            if split16:
+                for q in range(2):
+                    _lib.call("nl_gather_rows16_into", pb.nb, pb.kb, ncol, ops._p(pb.cols),
+                              ops._off16(src16, q * ncol), 2 * ncol, ops._off16(Gt, q * kpad),
+                              kpad, ldk, st)
+            elif src16 is not None:
                 _lib.call("nl_gather_rows16", pb.nb, pb.kb, ncol, ops._p(pb.cols), ops._p(src16),
                           ncol, ops._p(Gt), ldk, st)
             else:
                 _lib.call("nl_gather_convert", pb.nb, pb.kb, ncol, ops._p(pb.cols), ops._p(src),
                           ncol, ops._p(Gt), ldk, st)
             _lib.call("nl_gemm_lp_rows", dt, pb.mb, ncol, ldk, ctypes.c_double(alpha),
-                      ops._p(Vb), ldk, pb.mb * ldk, None, 0, ops._p(Gt), ldk, ncol * ldk, None,
-                      0, ops._p(out), ncol, ops._p(pb.rows), pb.nb, st)
+                      ops._p(Vb), ldk, pb.mb * ldk, ops._p(iv), pb.mb, ops._p(Gt), ldk,
+                      ncol * ldk, None, 0, ops._p(out), ncol, ops._p(pb.rows), pb.nb, st)
             if ev:
                 ev[1].record()
             return
         ig = (ops.WS.get(tag + "_Gs", pb.nb * ncol, dtype=torch.float32, device=src.device)
               if scaled else None)
         s4 = (ctypes.c_int64 * 4)(pb.kb * ncol, 1, 0, ncol)
-        _lib.call("nl_convert_rows", dt, pb.nb, ncol, 1, pb.kb, s4, ops._p(G), ops._p(Gt), ldk,
-                  ops._p(ig), ctypes.c_float(target), st)
+        if split:
+            _lib.call("nl_convert_split", dt, pb.nb, ncol, 1, pb.kb, s4, ops._p(G), ops._p(Gt),
+                      ldk, kpad, 2, 0b10, ops._p(ig), ctypes.c_float(target), st)
+        else:
+            _lib.call("nl_convert_rows", dt, pb.nb, ncol, 1, pb.kb, s4, ops._p(G), ops._p(Gt),
+                      ldk, ops._p(ig), ctypes.c_float(target), st)
         _lib.call("nl_gemm_lp", dt, pb.mb, ncol, ldk, ctypes.c_double(1.0), ops._p(Vb), ldk,
                   pb.mb * ldk, ops._p(iv), pb.mb, ops._p(Gt), ldk, ncol * ldk, ops._p(ig), ncol,
                   ctypes.c_double(0.0), None, ops._p(R), ncol, pb.mb * ncol, pb.nb, 1, None, st)
@@ -446,6 +510,7 @@ class T2Image:
         self.bufs = [None, None]
         self.cur = 0
         self.src = None
+        self.kind = 0           # 0: BF16 image, 1: [hi | lo] split image (nl_assemble_t2_ext)
 
     def buffer(self, k, n, dev):
         b = self.bufs[k]
@@ -454,8 +519,8 @@ class T2Image:
             self.bufs[k] = b
         return b[:n]
 
-    def holds(self, t2):
-        return self.src is t2
+    def holds(self, t2, kind=0):
+        return self.src is t2 and self.kind == kind
 
 
 def update(ham, t1, t2, B, t1n, t2n, mixing, prec=DEFAULT_PREC, H2out=None, img=None):
@@ -466,7 +531,10 @@ def update(ham, t1, t2, B, t1n, t2n, mixing, prec=DEFAULT_PREC, H2out=None, img=
     with ops.lp_reuse(t2):
         B.t2_16 = None
         try:
-            _t2_images(ham, t2, B, prec, img)
+            if _fast_split(ham, prec):
+                _t2_split_images(ham, t2, img)
+            else:
+                _t2_images(ham, t2, B, prec, img)
             t1_part(ham, t1, t2, B, prec)
             _lib.call("nl_update_t1", p, h, ops._p(t1), ops._p(B.H1), ops._p(B.X1pp),
                       ops._p(B.X1hh), ctypes.c_double(mixing), ops._p(t1n), ops._stream())
@@ -502,7 +570,7 @@ def _t2_images(ham, t2, B, prec, img=None):
     use = {k for k in ("dress", "hh", "ring") if prec.get(k) == "bf16"}
     n, ph, hh = p * p * h * h, p * h, h * h
     dev, st = t2.device, ops._stream()
-    if img is not None and img.holds(t2):
+    if img is not None and img.holds(t2, 0):
         S = img.buffer(img.cur, n, dev)                               # [a][d][k][l]
     else:
         S = ops.WS.get("t2_bf16", n, dtype=torch.int16, device=dev)   # [a][d][k][l]
@@ -525,24 +593,65 @@ def _t2_images(ham, t2, B, prec, img=None):
         ops.lp_provide(t2, "acik", ["a", "i"], ["c", "k"], T3, ph)
 
 
+def _t2_split_images(ham, t2, img=None):
+    """Register the [hi | lo] split images of t2 that the bf16x3 products read: the rows
+    [a][(d,k,l)] (A19; written by the previous step's assembly, or split here from FP64), and from
+    them by 16-bit permutes [(a,i)][(c,k)] (both ring products), [(a,j)][(d,k)] (Y) and
+    [(b,i,j)][d] (Sab).  Needs t2 under ops.lp_reuse."""
+    p, h = ham.p, ham.h
+    n, row, ph = p * p * h * h, p * h * h, p * h
+    dev, st = t2.device, ops._stream()
+    if img is not None and img.holds(t2, 1):
+        N = img.buffer(img.cur, 2 * n, dev)
+    else:
+        N = ops.WS.get("t2_split", 2 * n, dtype=torch.int16, device=dev)
+        s4 = (ctypes.c_int64 * 4)(0, row, 0, 1)
+        _lib.call("nl_convert_split", ops.BF16, 1, p, 1, row, s4, ops._p(t2), ops._p(N), 2 * row,
+                  row, 2, 0b10, None, ctypes.c_float(0.0), st)
+    ops.lp_provide(t2, "adkl", ["a"], ["d", "k", "l"], N, 2 * row, parts=SPLIT)
+    # ring operand rows (a,i), per part the columns (c,k): out (a, i, part, c, k)
+    R = ops.WS.get("t2_split_ring", 2 * n, dtype=torch.int16, device=dev)
+    shp = (ctypes.c_int64 * 5)(p, h, 2, p, h)
+    istr = (ctypes.c_int64 * 5)(2 * row, h, row, h * h, 1)
+    _lib.call("nl_permute16", 5, shp, istr, ops._p(N), ops._p(R), st)
+    ops.lp_provide(t2, "acik", ["a", "i"], ["c", "k"], R, 2 * ph, parts=SPLIT)
+    # Y operand rows (a,j), columns (d,k): per a and part, the (d,k) x j block transposed
+    Y = ops.WS.get("t2_split_aj", 2 * n, dtype=torch.int16, device=dev)
+    for q in range(2):
+        _lib.call("nl_transpose16_into", p, ph, h, ops._off16(N, q * row), h, 2 * row,
+                  ops._off16(Y, q * ph), 2 * ph, 2 * ph * h, ph, st)
+    ops.lp_provide(t2, "adkj", ["a", "j"], ["d", "k"], Y, 2 * ph, parts=SPLIT)
+    # Sab operand rows (b,i,j), columns d: the d x (b,i,j) matrix transposed, per part
+    S = ops.WS.get("t2_split_T", 2 * n, dtype=torch.int16, device=dev)
+    for q in range(2):
+        _lib.call("nl_transpose16_into", 1, p, row, ops._off16(N, q * row), 2 * row, 0,
+                  ops._off16(S, q * p), 2 * p, 0, p, st)
+    ops.lp_provide(t2, "dbij", ["b", "i", "j"], ["d"], S, 2 * p, parts=SPLIT)
+
+
 def assemble(ham, t2, B, t1n, t2n, mixing, H2out=None, img=None, prec=None):
-    """The assembly (nl_assemble_t2).  With ``img`` and ``prec`` it also writes the BF16 image of
-    t2n into ``img`` when ``prec`` reads t2 in BF16."""
+    """The assembly (nl_assemble_t2_ext, with the diagonal t2_part split off in B.dpp).  With
+    ``img`` and ``prec`` it also writes the BF16 image of t2n into ``img`` when ``prec`` reads t2
+    in BF16."""
     p, h = ham.p, ham.h
     st = ops._stream()
     args = (p, h, ops._p(B.S0), ops._p(B.Sab), ops._p(B.Sij), ops._p(B.Spp), ops._p(B.X2pp),
-            ops._p(B.X2hh), ops._p(t2), ops._p(ham.v_pphh), ops._p(t1n),
-            ctypes.c_double(mixing), ops._p(t2n), ops._p(H2out), ops._p(B.part))
-    if (img is not None and prec is not None and t2n is not None
+            ops._p(B.X2hh), ops._p(getattr(B, "dpp", None)), None, ops._p(t2),
+            ops._p(ham.v_pphh), ops._p(t1n), ctypes.c_double(mixing), ops._p(t2n),
+            ops._p(H2out), ops._p(B.part))
+    buf, kind = None, 0
+    if img is not None and prec is not None and t2n is not None and _fast_split(ham, prec):
+        nxt = 1 - img.cur
+        buf, kind = img.buffer(nxt, 2 * p * p * h * h, t2n.device), 1
+        img.cur, img.src, img.kind = nxt, t2n, 1
+    elif (img is not None and prec is not None and t2n is not None
             and _images_wanted(ham, prec)):
         nxt = 1 - img.cur
         buf = img.buffer(nxt, p * p * h * h, t2n.device)
-        _lib.call("nl_assemble_t2_16", *args, ops._p(buf), st)
-        img.cur, img.src = nxt, t2n
-    else:
-        _lib.call("nl_assemble_t2", *args, st)
-        if img is not None:
-            img.src = None
+        img.cur, img.src, img.kind = nxt, t2n, 0
+    elif img is not None:
+        img.src = None
+    _lib.call("nl_assemble_t2_ext", *args, ops._p(buf), kind, st)
     nparts = int(_lib.load().nl_partials(p * p * h * h))
     _lib.call("nl_finalize", nparts, ops._p(B.part), p * h, ops._p(ham.f_ph), ops._p(t1n),
               ops._p(B.E), st)

@@ -23,6 +23,11 @@ def _p(t):
     return ctypes.c_void_p(t.data_ptr()) if t is not None else ctypes.c_void_p(0)
 
 
+def _off16(t, n):
+    """Pointer to 16-bit element n of t."""
+    return ctypes.c_void_p(t.data_ptr() + 2 * int(n))
+
+
 def _stream():
     return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
 
@@ -428,21 +433,27 @@ def _view4(t, labels, rows, cols):
 # Tensors that stay constant for the life of a solve (Hamiltonian blocks): their low-precision
 # conversions are cached by (storage pointer, view, layout, dtype).
 CONST_PTRS = set()
+# Hamiltonian blocks among them: converted with exact per-row scales (target < 0 in
+# nl_convert_rows), so blocks whose nonzero elements share one magnitude are exact in 16 bits.
+EXACT_PTRS = set()
 _LP_CACHE = {}
 # Storage whose FP64 values are not formed (only 16-bit images registered by lp_provide): any
 # other read of it raises instead of using stale data.
 ABSENT_PTRS = set()
 
 
-def mark_constant(*ts):
+def mark_constant(*ts, exact=True):
     for t in ts:
         CONST_PTRS.add(t.data_ptr())
+        if exact:
+            EXACT_PTRS.add(t.data_ptr())
 
 
 def forget_ptrs(ptrs):
     """Drop constant registrations (and cached conversions) of freed storage."""
     for ptr in ptrs:
         CONST_PTRS.discard(ptr)
+        EXACT_PTRS.discard(ptr)
         for k in [k for k in _LP_CACHE if k[0] == ptr]:
             del _LP_CACHE[k]
 
@@ -468,25 +479,63 @@ class lp_reuse:
         return False
 
 
-def lp_provide(t, labels, rows, cols, buf, ldk, dtype=BF16):
-    """Register ``buf`` ([R][ldk] 16-bit rows, no scales) as the conversion of t seen as rows x
-    cols, for products inside an ``lp_reuse(t)`` block: they use it without reading t, so t's
-    FP64 values need not exist."""
+def lp_provide(t, labels, rows, cols, buf, ldk, dtype=BF16, parts=(1, 0)):
+    """Register ``buf`` ([R][ldk] 16-bit rows, no scales; with parts=(2, 0b10) each row is
+    [hi | lo] of the split) as the conversion of t seen as rows x cols, for products inside an
+    ``lp_reuse(t)`` block: they use it without reading t, so t's FP64 values need not exist."""
     v = _view4(t, labels, rows, cols)
     if v is None or t.data_ptr() not in CONST_PTRS:
         raise _lib.NLError("lp_provide: the view must be two-run and t under lp_reuse")
-    _LP_CACHE[(t.data_ptr(), v[0], v[1], v[2], v[3], tuple(v[4]), dtype)] = (buf, None, ldk)
+    key = (t.data_ptr(), v[0], v[1], v[2], v[3], tuple(v[4]), dtype)
+    if parts != (1, 0):
+        key = key + (parts,)
+    _LP_CACHE[key] = (buf, None, ldk)
 
 
-def to_lp_rows(t, labels, rows, cols, dtype, tag):
-    """Convert view t (rows x cols) to a 16-bit [R][ldk] buffer: FP16 with per-row power-of-two
-    scales (returned as inverse scales), BF16 by a plain cast (inverse scales None).  Conversions
-    of constant storage are cached by the memory view they read (pointer, extents, strides), so
-    differently labelled views of the same rows share one."""
+def lp_target(dtype, exact):
+    """nl_convert_rows target: Hamiltonian blocks (exact) get exact per-row scales (a contact
+    interaction's +-g elements become +-1 in BF16, +-2^14 in FP16: its magnitude is not rounded
+    to 8 or 11 bits, which would bias every term the block enters by the same relative amount);
+    other FP16 operands power-of-two scales; other BF16 operands a plain cast."""
+    if exact:
+        return -1.0 if dtype == BF16 else -(2.0 ** 14)
+    return 2.0 ** 14 if dtype != BF16 else 0.0
+
+
+def lp_kind(precision):
+    """(dtype, split) of a low-precision name: "bf16", "fp16" (one rounding of each operand) or
+    "bf16x3", "fp16x3" (hi/lo split operands, three products hi.hi + lo.hi + hi.lo)."""
+    if precision not in ("bf16", "fp16", "bf16x3", "fp16x3", "bf16x2a", "bf16x2b"):
+        raise ValueError(f"unknown precision {precision!r}")
+    return (BF16 if precision.startswith("bf16") else FP16), precision[4:] or False
+
+
+def split_parts(exact_a, exact_b):
+    """Operand images of a split product, as (nblk, lomask) K-blocks per row: a rounded operand
+    is [hi | lo]; an exact (Hamiltonian) operand is its hi part alone, or [hi | hi] when it meets
+    a rounded one, so that the K-concatenated product is hi.hi + hi.lo.  Two rounded operands
+    stay [hi | lo] each and go to the split GEMM (hi.hi + hi.lo + lo.hi in one kernel)."""
+    if exact_a and exact_b:
+        return (1, 0), (1, 0)
+    if exact_a:
+        return (2, 0), (2, 0b10)
+    if exact_b:
+        return (2, 0b10), (2, 0)
+    return (2, 0b10), (2, 0b10)
+
+
+def to_lp_rows(t, labels, rows, cols, dtype, tag, parts=(1, 0)):
+    """Convert view t (rows x cols) to a 16-bit [R][ldk] buffer: Hamiltonian blocks with exact
+    per-row scales, other FP16 operands with per-row power-of-two scales (returned as inverse
+    scales), other BF16 operands by a plain cast (inverse scales None).  Conversions of constant
+    storage are cached by the memory view they read (pointer, extents, strides), so differently
+    labelled views of the same rows share one."""
     v = _view4(t, labels, rows, cols)
     key = None
     if v is not None and t.data_ptr() in CONST_PTRS:
         key = (t.data_ptr(), v[0], v[1], v[2], v[3], tuple(v[4]), dtype)
+        if parts != (1, 0):
+            key = key + (parts,)
         hit = _LP_CACHE.get(key)
         if hit is not None:
             return hit
@@ -498,18 +547,24 @@ def to_lp_rows(t, labels, rows, cols, dtype, tag):
         v = _view4(t, labels, rows, cols)
     R1, R2, C1, C2, st = v
     R, Ck = R1 * R2, C1 * C2
-    ldk = (Ck + 7) // 8 * 8
-    scaled = dtype != BF16          # BF16 has FP32's exponent range: a plain cast suffices
+    kpad = (Ck + 7) // 8 * 8
+    nblk, mask = parts
+    ldk = nblk * kpad
+    target = lp_target(dtype, key is not None and t.data_ptr() in EXACT_PTRS)
+    scaled = target != 0.0          # BF16 has FP32's exponent range: a plain cast suffices
     if key is not None:
         buf = torch.empty(R * ldk, dtype=torch.int16, device=t.device)
         isc = torch.empty(R, dtype=torch.float32, device=t.device) if scaled else None
     else:
         buf = WS.get(tag, R * ldk, dtype=torch.int16, device=t.device)
         isc = WS.get(tag + "_s", R, dtype=torch.float32, device=t.device) if scaled else None
-    target = 2.0 ** 14 if scaled else 0.0
     s4 = (ctypes.c_int64 * 4)(*st)
-    _lib.call("nl_convert_rows", dtype, R1, R2, C1, C2, s4, ctypes.c_void_p(t.data_ptr()),
-              _p(buf), ldk, _p(isc), ctypes.c_float(target), _stream())
+    if parts == (1, 0):
+        _lib.call("nl_convert_rows", dtype, R1, R2, C1, C2, s4, ctypes.c_void_p(t.data_ptr()),
+                  _p(buf), ldk, _p(isc), ctypes.c_float(target), _stream())
+    else:
+        _lib.call("nl_convert_split", dtype, R1, R2, C1, C2, s4, ctypes.c_void_p(t.data_ptr()),
+                  _p(buf), ldk, kpad, nblk, mask, _p(isc), ctypes.c_float(target), _stream())
     if key is not None:
         _LP_CACHE[key] = (buf, isc, ldk)
     return buf, isc, ldk
@@ -532,9 +587,19 @@ def _rowmajor(t, labels, Mi, Ni):
 
 def _contract_lp(A, la, B, lb, out, lo, Mi, Ni, Ki, M, N, K, alpha, beta, precision, tag=None,
                  addend=None):
-    dt = BF16 if precision == "bf16" else FP16
-    Ab, ia, lda = to_lp_rows(A, la, Mi, Ki, dt, "lpA")
-    Bb, ib, ldb = to_lp_rows(B, lb, Ni, Ki, dt, "lpB")
+    dt, split = lp_kind(precision)
+    pa = pb = (1, 0)
+    if split:
+        ex = lambda t, lab, rows: (t.data_ptr() in EXACT_PTRS
+                                   and _view4(t, lab, rows, Ki) is not None)
+        pa, pb = split_parts(ex(A, la, Mi) or split == "x2b", ex(B, lb, Ni) or split == "x2a")
+    Ab, ia, lda = to_lp_rows(A, la, Mi, Ki, dt, "lpA", pa)
+    Bb, ib, ldb = to_lp_rows(B, lb, Ni, Ki, dt, "lpB", pb)
+    # both operands rounded: [hi | lo] rows of each, lo parts passed to the split GEMM
+    both = pa == (2, 0b10) and pb == (2, 0b10)
+    Kg = lda // 2 if both else lda
+    lo_a = ctypes.c_void_p(Ab.data_ptr() + 2 * Kg) if both else None
+    lo_b = ctypes.c_void_p(Bb.data_ptr() + 2 * Kg) if both else None
     ldc = _rowmajor(out, lo, Mi, Ni)
     runs = None
     if ldc is None:
@@ -546,18 +611,28 @@ def _contract_lp(A, la, B, lb, out, lo, Mi, Ni, Ki, M, N, K, alpha, beta, precis
     direct = ldc is not None or runs is not None
     C = out if direct else WS.get("contract_out", M * N, device=out.device).view(M, N)
     D = (addend if addend is not None else out) if (direct and beta != 0.0) else None
-    splits = _lp_splits(M, N, lda, 1)
+    splits = _lp_splits(M, N, Kg, 1)
     ws = WS.get("splitk", splits * M * N, device=out.device) if splits > 1 else None
     ev = _timed(tag)
     if ev:
         ev[0].record()
+    arr = lambda v: (ctypes.c_int64 * 4)(*[int(x) for x in v])
     if runs is not None:
         (m1, m2, (sm,)), (n1, n2, (sn,)) = runs
-        arr = lambda v: (ctypes.c_int64 * 4)(*[int(x) for x in v])
-        _lib.call("nl_gemm_lp_strided", dt, lda, ctypes.c_double(alpha), _p(Ab), lda, _p(ia),
-                  _p(Bb), ldb, _p(ib), ctypes.c_double(beta), _p(D), _p(C),
-                  arr([m1, m2, n1, n2]), arr([sm[0], sm[1], sn[0], sn[1]]), splits, _p(ws),
-                  _stream())
+        dims, strides = arr([m1, m2, n1, n2]), arr([sm[0], sm[1], sn[0], sn[1]])
+        if both:
+            _lib.call("nl_gemm_lp_strided_split", dt, Kg, ctypes.c_double(alpha), _p(Ab), lo_a,
+                      lda, _p(ia), _p(Bb), lo_b, ldb, _p(ib), ctypes.c_double(beta), _p(D),
+                      _p(C), dims, strides, splits, _p(ws), _stream())
+        else:
+            _lib.call("nl_gemm_lp_strided", dt, lda, ctypes.c_double(alpha), _p(Ab), lda, _p(ia),
+                      _p(Bb), ldb, _p(ib), ctypes.c_double(beta), _p(D), _p(C), dims, strides,
+                      splits, _p(ws), _stream())
+    elif both:
+        _lib.call("nl_gemm_lp_split", dt, M, N, Kg, ctypes.c_double(alpha), _p(Ab), lo_a, lda, 0,
+                  _p(ia), 0, _p(Bb), lo_b, ldb, 0, _p(ib), 0,
+                  ctypes.c_double(beta if direct else 0.0), _p(D), _p(C),
+                  max(ldc if ldc is not None else N, 1), 0, 1, splits, _p(ws), _stream())
     else:
         _lib.call("nl_gemm_lp", dt, M, N, lda, ctypes.c_double(alpha), _p(Ab), lda, 0, _p(ia), 0,
                   _p(Bb), ldb, 0, _p(ib), 0, ctypes.c_double(beta if direct else 0.0), _p(D),
