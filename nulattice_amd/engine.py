@@ -400,6 +400,7 @@ def _sab_bf16(ham, t1, B, Sab):
               off(B16, h), ldk, 0, p, st)                           # [N56 | t2^T | 0]
     splits = ops._lp_splits(p, p * h * h, ldk, 1)
     ws = ops.WS.get("splitk", splits * p * p * h * h, device=dev) if splits > 1 else None
+    ops.count_flops("lp", 2.0 * p * p * h * h * ldk)
     _lib.call("nl_gemm_lp", ops.BF16, p, p * h * h, ldk, ctypes.c_double(1.0), ops._p(A16), ldk,
               0, None, 0, ops._p(B16), ldk, 0, None, 0, ctypes.c_double(0.0), None, ops._p(Sab),
               p * h * h, 0, 1, splits, ops._p(ws), st)
@@ -433,6 +434,7 @@ def _block_gemm(pb, src, out, alpha, ncol, tag, precision="fp64", src16=None):
         _lib.call("nl_gather_rows", pb.nb * pb.kb, ncol, ops._p(pb.cols), ops._p(src), ncol,
                   ops._p(G), st)
     if precision == "fp64":
+        ops.count_flops("fp64", 2.0 * pb.nb * pb.mb * ncol * pb.kb)
         _lib.call("nl_gemm_f64", 1, 1, pb.mb, ncol, pb.kb, ctypes.c_double(1.0), ops._p(pb.V),
                   pb.kb, pb.mb * pb.kb, ops._p(G), ncol, pb.kb * ncol, ctypes.c_double(0.0),
                   ops._p(R), ncol, pb.mb * ncol, pb.nb, 1, ctypes.c_void_p(0), st)
@@ -457,6 +459,7 @@ def _block_gemm(pb, src, out, alpha, ncol, tag, precision="fp64", src16=None):
                           ops._p(Vb), ldk, ops._p(iv), tgt, st)
             pb.lp = lp = ((dt, split), Vb, iv)
         _, Vb, iv = lp
+        ops.count_flops("lp", 2.0 * pb.nb * pb.mb * ncol * ldk)
         target = ops.lp_target(dt, False)
         scaled = target != 0.0
         Gt = ops.WS.get(tag + "_Gt", pb.nb * ncol * ldk, dtype=torch.int16, device=src.device)

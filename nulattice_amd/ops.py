@@ -63,6 +63,16 @@ LP_MIN_DIM = 256
 # of events on the launch stream around its GEMM launch into TIMERS[T].
 TIMERS = {"on": False}
 
+# Optional count of the matrix-core work launched: when FLOPS["on"] is set, every GEMM launch adds
+# its executed FLOP (2 M N K per product; a split product counts each of its partial products)
+# to FLOPS["fp64"] or FLOPS["lp"].
+FLOPS = {"on": False, "fp64": 0.0, "lp": 0.0}
+
+
+def count_flops(kind, flop):
+    if FLOPS["on"]:
+        FLOPS[kind] += float(flop)
+
 
 def _timed(tag):
     if TIMERS.get("all"):
@@ -394,6 +404,7 @@ def contract(spec, A, B, out, alpha=1.0, beta=1.0, precision="fp64", tag=None, a
     ev = _timed(tag)
     if ev:
         ev[0].record()
+    count_flops("fp64", 2.0 * math.prod(dims))
     _lib.call("nl_contract_f64", arr(dims), ctypes.c_double(alpha), _p(X), arr(sa), _p(Y),
               arr(sb), ctypes.c_double(beta if direct else 0.0), _p(D), _p(C), arr(sc), splits,
               _p(ws), _stream())
@@ -613,6 +624,7 @@ def _contract_lp(A, la, B, lb, out, lo, Mi, Ni, Ki, M, N, K, alpha, beta, precis
     D = (addend if addend is not None else out) if (direct and beta != 0.0) else None
     splits = _lp_splits(M, N, Kg, 1)
     ws = WS.get("splitk", splits * M * N, device=out.device) if splits > 1 else None
+    count_flops("lp", 2.0 * M * N * (3 * Kg if both else lda))
     ev = _timed(tag)
     if ev:
         ev[0].record()

@@ -1,0 +1,271 @@
+"""GPU parity of the split-operand precision form ("bf16x3") through the C ABI.
+
+BF16 rounding is restated in NumPy (round to nearest even on the FP32 bit pattern), so the
+hi/lo images are checked bit for bit; the split GEMM is checked on integer-valued operands,
+whose products and FP32 sums are exact; the solver is checked against the FP64 fixed point at
+the headline size.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+from nulattice_amd import _lib, ccm, engine, ops  # noqa: E402
+from nulattice_amd.planewave import PlaneWave, planewave_hamiltonian, shell_holes  # noqa: E402
+
+L_ = _lib.load
+
+
+def P(t):
+    return ctypes.c_void_p(0 if t is None else t.data_ptr())
+
+
+def S():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def i64(*v):
+    return (ctypes.c_int64 * len(v))(*v)
+
+
+def bf16_bits(x32):
+    """Round-to-nearest-even FP32 -> BF16 bit patterns (finite inputs)."""
+    u = np.ascontiguousarray(x32, np.float32).view(np.uint32).astype(np.uint64)
+    return ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint16)
+
+
+def bf16_value(bits):
+    return (bits.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
+
+
+def split_ref(x):
+    """hi = BF16(FP32(x)), lo = BF16(FP32(x - hi)) as the kernels define the split."""
+    hi = bf16_bits(x.astype(np.float32))
+    lo = bf16_bits((x - bf16_value(hi)).astype(np.float32))
+    return hi, lo
+
+
+def u16(t):
+    return t.detach().cpu().numpy().view(np.uint16)
+
+
+def test_bf16_reference_rounding():
+    """The NumPy restatement rounds like torch's own FP32 -> BF16 cast (ties to even)."""
+    rng = np.random.default_rng(0)
+    x = (rng.standard_normal(100000) * 10.0 ** rng.integers(-20, 20, 100000)).astype(np.float32)
+    x[:4] = [1.0 + 2 ** -8, 1.0 + 3 * 2 ** -8, -(1.0 + 2 ** -8), 1.00390625]   # exact ties
+    t = torch.tensor(x).to(torch.bfloat16).view(torch.int16).numpy().view(np.uint16)
+    np.testing.assert_array_equal(bf16_bits(x), t)
+
+
+@pytest.mark.parametrize("transposed", [False, True])
+def test_convert_split_blocks(transposed):
+    """nl_convert_split writes [hi | lo | hi] rows (lomask 0b010) bit for bit, zero padding,
+    and hi + lo reproduces x to 2^-16 relative; both the row kernel and the transposing tile
+    kernel (source rows unit-stride)."""
+    rng = np.random.default_rng(3)
+    R, C, kpad = 70, 100, 104
+    x = rng.standard_normal((R, C)) * 10.0 ** rng.integers(-6, 3, (R, C))
+    src = torch.tensor(x.T.copy() if transposed else x, device="cuda")
+    strides = (0, 1, 0, R) if transposed else (0, C, 0, 1)
+    out = torch.full((R * 3 * kpad,), -1, dtype=torch.int16, device="cuda")
+    rc = L_().nl_convert_split(0, 1, R, 1, C, i64(*strides), P(src), P(out), 3 * kpad, kpad, 3,
+                               0b010, None, ctypes.c_float(0.0), S())
+    assert rc == 0
+    torch.cuda.synchronize()
+    o = u16(out).reshape(R, 3, kpad)
+    hi, lo = split_ref(x)
+    np.testing.assert_array_equal(o[:, 0, :C], hi)
+    np.testing.assert_array_equal(o[:, 1, :C], lo)
+    np.testing.assert_array_equal(o[:, 2, :C], hi)
+    assert not o[:, :, C:].any()
+    rec = bf16_value(hi) + bf16_value(lo)
+    assert np.max(np.abs(rec - x) / np.abs(x)) < 2.0 ** -16
+
+
+def test_convert_exact_scales_make_contact_blocks_exact():
+    """target < 0: rows whose nonzero elements are +-g (a contact interaction's blocks) convert
+    to exactly +-1, the magnitude g carried by the FP32 inverse scale."""
+    g = -4.1234567 / 125.0
+    rng = np.random.default_rng(5)
+    x = g * rng.integers(-1, 2, (40, 64)).astype(np.float64)
+    x[3] = 0.0
+    src = torch.tensor(x, device="cuda")
+    out = torch.empty(40 * 64, dtype=torch.int16, device="cuda")
+    isc = torch.empty(40, dtype=torch.float32, device="cuda")
+    assert L_().nl_convert_rows(0, 1, 40, 1, 64, i64(0, 64, 0, 1), P(src), P(out), 64, P(isc),
+                                ctypes.c_float(-1.0), S()) == 0
+    torch.cuda.synchronize()
+    v = bf16_value(u16(out)).reshape(40, 64)
+    np.testing.assert_array_equal(v, np.sign(x))
+    s = isc.cpu().numpy().astype(np.float64)
+    assert s[3] == 1.0
+    rows = [r for r in range(40) if r != 3]
+    np.testing.assert_allclose(s[rows], abs(g), rtol=2 ** -23)
+
+
+def _bf16(a):
+    return torch.tensor(a, dtype=torch.float32, device="cuda").to(torch.bfloat16)
+
+
+@pytest.mark.parametrize("M,N,K", [(300, 513, 96), (4000, 4200, 600), (4100, 4000, 472),
+                                   (4096, 4097, 1024)])
+@pytest.mark.parametrize("parts", ["2x2", "1x2", "2x1"])
+def test_split_gemm_exact(M, N, K, parts):
+    """nl_gemm_lp_split = alpha (Ah Bh^T + Ah Bl^T + Al Bh^T) + beta D on integer operands
+    (exact in BF16 and in FP32 sums): the 256x256 multi-stage kernel (large shapes, K not a
+    multiple of 32 in the two-stage form) and the fallback of three products (small shape)."""
+    rng = np.random.default_rng(M + N + K)
+    Ah, Al, Bh, Bl = (rng.integers(-8, 9, s).astype(np.float64)
+                      for s in ((M, K), (M, K), (N, K), (N, K)))
+    use_al, use_bl = parts[0] == "2", parts[2] == "2"
+    ref = Ah @ Bh.T + (Ah @ Bl.T if use_bl else 0) + (Al @ Bh.T if use_al else 0)
+    D = rng.integers(-5, 6, (M, N)).astype(np.float64)
+    ta, tal, tb, tbl = _bf16(Ah), _bf16(Al), _bf16(Bh), _bf16(Bl)
+    C = torch.full((M, N), float("nan"), dtype=torch.float64, device="cuda")
+    Dd = torch.tensor(D, device="cuda")
+    rc = L_().nl_gemm_lp_split(0, M, N, K, ctypes.c_double(0.5), P(ta), P(tal if use_al else None),
+                               K, 0, None, 0, P(tb), P(tbl if use_bl else None), K, 0, None, 0,
+                               ctypes.c_double(2.0), P(Dd), P(C), N, 0, 1, 1, None, S())
+    assert rc == 0
+    np.testing.assert_array_equal(C.cpu().numpy(), 0.5 * ref + 2.0 * D)
+
+
+def test_split_gemm_strided_output_and_split_k():
+    """The strided-output split GEMM with a split-K reduction writes C[(m1,m2),(n1,n2)] at
+    two-run strides exactly."""
+    rng = np.random.default_rng(11)
+    m1, m2, n1, n2, K = 40, 28, 40, 28, 4096      # 5 x 5 tiles of 256, 16 K splits: 400 groups
+    M, N = m1 * m2, n1 * n2
+    Ah, Al, Bh, Bl = (rng.integers(-4, 5, s).astype(np.float64)
+                      for s in ((M, K), (M, K), (N, K), (N, K)))
+    ref = (Ah @ Bh.T + Ah @ Bl.T + Al @ Bh.T).reshape(m1, m2, n1, n2)
+    out = torch.zeros((m1, n1, m2, n2), dtype=torch.float64, device="cuda")   # [m1][n1][m2][n2]
+    ws = torch.empty(16 * M * N, dtype=torch.float64, device="cuda")
+    sc = i64(n1 * m2 * n2, n2, m2 * n2, 1)
+    ta, tal, tb, tbl = _bf16(Ah), _bf16(Al), _bf16(Bh), _bf16(Bl)   # kept alive for the launch
+    rc = L_().nl_gemm_lp_strided_split(0, K, ctypes.c_double(1.0), P(ta), P(tal), K, None,
+                                       P(tb), P(tbl), K, None, ctypes.c_double(0.0), None,
+                                       P(out), i64(m1, m2, n1, n2), sc, 16, P(ws), S())
+    assert rc == 0
+    np.testing.assert_array_equal(out.cpu().numpy().transpose(0, 2, 1, 3), ref)
+
+
+def test_tau_split():
+    """nl_tau_split: FP64 tau = t2 + t1 t1 - t1 t1 and its [hi | lo] rows bit for bit."""
+    rng = np.random.default_rng(7)
+    p, h = 24, 6
+    t1 = rng.standard_normal((p, h)) * 0.1
+    t2 = rng.standard_normal((p, p, h, h)) * 0.01
+    tau = t2 + np.einsum("ai,bj->abij", t1, t1) - np.einsum("aj,bi->abij", t1, t1)
+    g1, g2 = torch.tensor(t1, device="cuda"), torch.tensor(t2, device="cuda")
+    tau_d = torch.empty_like(g2)
+    img = torch.empty(2 * p * p * h * h, dtype=torch.int16, device="cuda")
+    assert L_().nl_tau_split(p, h, P(g1), P(g2), P(tau_d), P(img), S()) == 0
+    torch.cuda.synchronize()
+    np.testing.assert_allclose(tau_d.cpu().numpy(), tau, rtol=1e-14, atol=1e-17)
+    hi, lo = split_ref(tau_d.cpu().numpy().reshape(p * p, h * h))
+    o = u16(img).reshape(p * p, 2, h * h)
+    np.testing.assert_array_equal(o[:, 0], hi)
+    np.testing.assert_array_equal(o[:, 1], lo)
+
+
+def test_assemble_ext_diagonals_and_split_image():
+    """nl_assemble_t2_ext: H2 = S0 + pAB(Sab) + pIJ(Sij) + pIJ pAB(Spp) + (dpp_a + dpp_b + dhh_i +
+    dhh_j) t2, the Jacobi step with the dressed denominators, and the [hi | lo] image of t2n."""
+    rng = np.random.default_rng(9)
+    p, h = 20, 6
+    S0, Sab, Sij, t2, v = (rng.standard_normal((p, p, h, h)) for _ in range(5))
+    Spp = rng.standard_normal((p, h, p, h))
+    Xpp = rng.standard_normal((p, p)) + np.diag(5 + rng.random(p))
+    Xhh = rng.standard_normal((h, h)) - np.diag(5 + rng.random(h))
+    dpp, dhh = rng.standard_normal(p), rng.standard_normal(h)
+    t1n = rng.standard_normal((p, h)) * 0.1
+    pAB = lambda x: x - x.transpose(1, 0, 2, 3)
+    pIJ = lambda x: x - x.transpose(0, 1, 3, 2)
+    H = (S0 + pAB(Sab) + pIJ(Sij) + pIJ(pAB(Spp.transpose(0, 2, 1, 3)))
+         + (dpp[:, None, None, None] + dpp[None, :, None, None] + dhh[None, None, :, None]
+            + dhh[None, None, None, :]) * t2)
+    dp, dh = np.diag(Xpp), np.diag(Xhh)
+    den = -(dp[:, None, None, None] + dp[None, :, None, None] + dh[None, None, :, None]
+            + dh[None, None, None, :])
+    m = 0.3
+    tn = m * t2 + (1 - m) * (t2 + H / den)
+    d = lambda x: torch.tensor(x, device="cuda")
+    g = [d(x) for x in (S0, Sab, Sij, Spp, Xpp, Xhh, dpp, dhh, t2, v, t1n)]
+    t2n, H2 = torch.empty_like(g[0]), torch.empty_like(g[0])
+    part = torch.empty(2 * int(L_().nl_partials(p * p * h * h)) + 8, dtype=torch.float64,
+                       device="cuda")
+    img = torch.empty(2 * p * p * h * h, dtype=torch.int16, device="cuda")
+    rc = L_().nl_assemble_t2_ext(p, h, *[P(x) for x in g], ctypes.c_double(m), P(t2n), P(H2),
+                                 P(part), P(img), 1, S())
+    assert rc == 0
+    torch.cuda.synchronize()
+    np.testing.assert_allclose(H2.cpu().numpy(), H, rtol=1e-13, atol=1e-13)
+    tnd = t2n.cpu().numpy()
+    np.testing.assert_allclose(tnd, tn, rtol=1e-12, atol=1e-14)
+    hi, lo = split_ref(tnd.reshape(p, -1))
+    o = u16(img).reshape(p, 2, -1)
+    np.testing.assert_array_equal(o[:, 0], hi)
+    np.testing.assert_array_equal(o[:, 1], lo)
+
+
+def _steps(H, prec, n):
+    p, h = H.p, H.h
+    B = engine.Buffers(p, h, H.device)
+    t1 = torch.zeros(p, h, dtype=torch.float64, device="cuda")
+    t2 = torch.zeros(p, p, h, h, dtype=torch.float64, device="cuda")
+    engine.init_amplitudes(H, B, t1, t2, 0.0)
+    t1n, t2n = torch.empty_like(t1), torch.empty_like(t2)
+    img = engine.T2Image()
+    es = []
+    for _ in range(n):
+        engine.update(H, t1, t2, B, t1n, t2n, 0.5, prec, img=img)
+        es.append(float(B.E.item()))
+        t1, t1n, t2, t2n = t1n, t1, t2n, t2
+    return np.array(es), t2.clone()
+
+
+def test_split_images_equal_per_product_conversions():
+    """Plane-wave L=3, A=28 (p=80, h=28): the shared split images of t2 (written by the
+    assembly, permuted in 16 bits) and of tau give the same steps as converting every operand
+    per product (FP64 -> split), to summation order (1e-12)."""
+    E0, fock, v = planewave_hamiltonian(3, 28, -4.0, 1.0)
+    H = engine.Hamiltonian(fock, v, device="cuda")
+    fast = ccm._PREC["bf16x3"]
+    assert engine._fast_split(H, fast)
+    slow = dict(fast, ring_W="bf16x3")          # any sub-family key disables the shared images
+    assert not engine._fast_split(H, slow)
+    ef, tf = _steps(H, fast, 4)
+    es, ts = _steps(H, slow, 4)
+    np.testing.assert_allclose(ef, es, rtol=1e-12)
+    scale = float(ts.abs().max())
+    assert float((tf - ts).abs().max()) <= 1e-12 * scale
+
+
+def test_precision_form_L5_A28_meets_1e6():
+    """Headline size (L=5, A=28, C0 from seed 0, D0=1): the bf16x3 iteration converges to within
+    1e-6 relative of the FP64 energy (the symmetry-sector FP64 solve, itself equal to the dense
+    FP64 solve to 1e-12), with no FP64 refinement."""
+    from nulattice_amd.sectors import SectorHamiltonian
+    C0 = float(np.random.default_rng(0).uniform(-4.5, -3.5))
+    SH = SectorHamiltonian.from_planewave(PlaneWave(5, shell_holes(5, 28), C0, 1.0))
+    e64, _, _ = ccm.ccsd_solver(None, None, eps=1e-13, maxSteps=400, max_diis=0, hamiltonian=SH)
+    del SH
+    E0, fock, v = planewave_hamiltonian(5, 28, C0, 1.0)
+    H = engine.Hamiltonian(fock, v, device="cuda")
+    info = {}
+    e, _, _ = ccm.ccsd_solver(fock, v, eps=1e-10, maxSteps=120, max_diis=0, hamiltonian=H,
+                              precision="bf16x3", refine=False, info=info)
+    rel = abs(e - e64) / abs(e64)
+    print(f"L=5 A=28 seed 0: bf16x3 |dE|/|E| = {rel:.3e} after {info['steps_lowprec']} steps")
+    assert info["converged"]
+    assert rel <= 1e-6
+    # the single-rounding forms for the record (their fixed points miss 1e-6)
+    for prec in ("bf16", "fp16"):
+        el, _, _ = ccm.ccsd_solver(fock, v, eps=1e-9, maxSteps=60, max_diis=0, hamiltonian=H,
+                                   precision=prec, refine=False)
+        print(f"  {prec}: |dE|/|E| = {abs(el - e64) / abs(e64):.3e}")
