@@ -20,6 +20,8 @@ f_ph = 0 (momentum conservation), so T1 = 0 exactly for these references.
 """
 import numpy as np
 
+from .plans import CooBlock
+
 
 def kvectors(L):
     k = np.arange(L)
@@ -118,12 +120,12 @@ class PlaneWave:
         return v_pphh, v_phph, v_phhh, v_hhhh
 
     def coo_pppp(self):
-        """All nonzeros of v_pppp[a,b,c,d] as (idx (N,4) int64, val (N,))."""
-        return self._coo(self.parts, self.parts)
+        """All nonzeros of v_pppp[a,b,c,d] as a CooBlock (idx (N,4) int64, val (N,))."""
+        return CooBlock(*self._coo(self.parts, self.parts))
 
     def coo_ppph(self):
-        """All nonzeros of v_ppph[a,b,c,i] as (idx (N,4) int64, val (N,))."""
-        return self._coo(self.parts, self.holes)
+        """All nonzeros of v_ppph[a,b,c,i] as a CooBlock (idx (N,4) int64, val (N,))."""
+        return CooBlock(*self._coo(self.parts, self.holes))
 
     def _coo(self, cols1, cols2):
         P = np.asarray(self.parts)

@@ -10,29 +10,52 @@ v_ppph (COO <cd||ak>) becomes *segment plans* for nl_segsum: for each intermedia
 v_ppph_dgrams (ccDgrams.py:13-45) the entries are grouped by output row so that every output row
 is owned by one wavefront (no atomics).
 """
+from typing import NamedTuple
+
 import numpy as np
 import scipy.sparse as sp
 import scipy.sparse.csgraph as csg
 import torch
 
 
+class CooBlock(NamedTuple):
+    """A sparse two-body block as rows: indices (N, 4) int64 [a, b, c, d], values (N,) -- the
+    row layout of _torch_types.TwoBodyOperator (_torch_types.py:169-199), as a named pair."""
+    indices: np.ndarray
+    values: np.ndarray
+
+
 def coo_arrays(v, pnum, hnum, kind):
     """Normalise a reference-format sparse block to (idx (N,4) int64, val (N,) float64).
 
-    Accepts the legacy list [[i0,i1,i2,i3,val], ...] (coupled_cluster.py:256-258), an (N,5)
-    array, an (indices, values) pair, an object with ``.indices``/``.values``
-    (_torch_types.TwoBodyOperator, indices (N,4)), or a dense array (sparse=False path), whose
-    nonzeros are taken."""
+    Accepts, by container:
+      * the legacy list [[i0,i1,i2,i3,val], ...] (coupled_cluster.py:256-258) or an (N,5) array;
+      * an object with ``.indices`` (N,4) and ``.values`` (N,): _torch_types.TwoBodyOperator
+        (_torch_types.py:169-199), this package's CooBlock and hamiltonian.CooList;
+      * a bare (indices, values) pair as soa/coupled_cluster.py:18-46 (to_soa_sparse) returns
+        it: indices (4, N); (N, 4) indices are accepted only when N != 4, where the two
+        layouts cannot be confused;
+      * a dense array (sparse=False path), whose nonzeros are taken."""
     shape = (pnum, pnum, pnum, pnum if kind == "pppp" else hnum)
     if hasattr(v, "indices") and hasattr(v, "values"):
-        idx = np.asarray(_np(v.indices), dtype=np.int64).reshape(-1, 4)
+        idx = np.asarray(_np(v.indices), dtype=np.int64)
         val = np.asarray(_np(v.values), dtype=np.float64).reshape(-1)
+        if val.size == 0:
+            idx = idx.reshape(0, 4)
+        if idx.shape != (val.size, 4):
+            raise ValueError(f"{kind}: container indices have shape {idx.shape}, expected "
+                             f"({val.size}, 4)")
     elif isinstance(v, tuple) and len(v) == 2:
         idx = np.asarray(_np(v[0]), dtype=np.int64)
-        if idx.shape[0] == 4 and idx.ndim == 2 and idx.shape[1] != 4:
-            idx = idx.T
-        idx = idx.reshape(-1, 4)
         val = np.asarray(_np(v[1]), dtype=np.float64).reshape(-1)
+        n = val.size
+        if idx.size == 0 and n == 0:
+            idx = idx.reshape(0, 4)
+        elif idx.shape == (4, n):
+            idx = idx.T                      # to_soa_sparse layout (including n == 4)
+        elif idx.shape != (n, 4):
+            raise ValueError(f"{kind}: (indices, values) pair with indices {idx.shape} and "
+                             f"{n} values; expected (4, {n}) as to_soa_sparse returns")
     else:
         arr = _np(v)
         arr = np.asarray(arr, dtype=np.float64)
@@ -46,6 +69,7 @@ def coo_arrays(v, pnum, hnum, kind):
             arr = arr.reshape(-1, 5) if arr.size else np.zeros((0, 5))
             idx = arr[:, :4].astype(np.int64)
             val = arr[:, 4].astype(np.float64)
+    idx = np.ascontiguousarray(idx)
     if idx.size:
         hi = np.array(shape)
         if (idx < 0).any() or (idx >= hi).any():

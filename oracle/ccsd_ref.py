@@ -228,6 +228,40 @@ def t2Iter(t1, t2, f_ph, f_hh, f_pp, v_pppp, v_phph, v_phhh, v_pphh, v_ppph_resu
     return t2 + H2 / denom
 
 
+def diis_window_step(hist1, hist2, errs, max_diis, soa, ccs=False):
+    """One step of the pulsed DIIS window, after the newest amplitudes (hist1/hist2[-1]) and
+    error vector (errs[-1]) were appended.  Returns None while the window fills; when it holds
+    max_diis errors it returns the extrapolation sum_k c_k hist[k+1] with c minimising
+    |sum_k c_k e_k| subject to sum_k c_k = 1 (bordered system on B_ab = e_a . e_b / max|B|):
+      * pulsed (coupled_cluster.py:635-672, with errors appended every step): the oldest entry
+        of the window is dropped before the solve, so n = max_diis - 1 errors enter;
+      * soa (soa/coupled_cluster.py:520-567): all max_diis errors enter, B / (max|B| + 1e-16).
+    A singular solve returns the newest amplitudes unchanged.  The caller restarts the window
+    from the returned amplitudes."""
+    if soa and len(errs) > max_diis:
+        hist1.pop(0); hist2.pop(0); errs.pop(0)
+    if len(errs) != max_diis:
+        return None
+    if not soa:
+        # the window drops its oldest entry before the solve (coupled_cluster.py:642-647)
+        hist1.pop(0); hist2.pop(0); errs.pop(0)
+    n = len(errs)
+    B = np.empty((n, n))
+    for a in range(n):
+        for b in range(a, n):
+            B[a, b] = B[b, a] = np.dot(errs[a], errs[b])
+    B = B / (np.abs(B).max() + (1e-16 if soa else 0.0))
+    A = -np.ones((n + 1, n + 1)); A[:n, :n] = B; A[n, n] = 0.0
+    rhs = np.zeros(n + 1); rhs[n] = -1.0
+    try:
+        c = np.linalg.solve(A, rhs)[:n]
+    except np.linalg.LinAlgError:
+        return hist1[-1], hist2[-1]
+    t1 = sum(c[k] * hist1[k + 1] for k in range(n))
+    t2 = hist2[-1] if ccs else sum(c[k] * hist2[k + 1] for k in range(n))
+    return t1, t2
+
+
 def ccsd_solver(fock_mats, two_body_int, t1initial=None, eps=1e-8, maxSteps=1000, max_diis=10,
                 delta=0, mixing=0.5, verbose=False, sparse=True, ccs=False, diis="legacy",
                 history=None):
@@ -276,27 +310,9 @@ def ccsd_solver(fock_mats, two_body_int, t1initial=None, eps=1e-8, maxSteps=1000
         if pulsed:
             hist1.append(t1.copy()); hist2.append(t2.copy())
             errs.append(np.concatenate([(t1 - oldT1).ravel(), (t2 - oldT2).ravel()]))
-            if soa and len(errs) > max_diis:
-                hist1.pop(0); hist2.pop(0); errs.pop(0)
-            if len(errs) == max_diis:
-                if not soa:
-                    # the window drops its oldest entry before the solve (coupled_cluster.py:642-647)
-                    hist1.pop(0); hist2.pop(0); errs.pop(0)
-                n = len(errs)
-                B = np.empty((n, n))
-                for a in range(n):
-                    for b in range(a, n):
-                        B[a, b] = B[b, a] = np.dot(errs[a], errs[b])
-                B = B / (np.abs(B).max() + (1e-16 if soa else 0.0))
-                A = -np.ones((n + 1, n + 1)); A[:n, :n] = B; A[n, n] = 0.0
-                rhs = np.zeros(n + 1); rhs[n] = -1.0
-                try:
-                    c = np.linalg.solve(A, rhs)[:n]
-                    t1 = sum(c[k] * hist1[k + 1] for k in range(n))
-                    if not ccs:
-                        t2 = sum(c[k] * hist2[k + 1] for k in range(n))
-                except np.linalg.LinAlgError:
-                    pass
+            ext = diis_window_step(hist1, hist2, errs, max_diis, soa, ccs)
+            if ext is not None:
+                t1, t2 = ext
                 hist1, hist2, errs = [t1.copy()], [t2.copy()], []
         if abs(energy) > 1e10:
             print("Iteration Diverged")

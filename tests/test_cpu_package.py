@@ -7,6 +7,7 @@ import re
 
 import numpy as np
 import pytest
+import torch
 
 from nulattice_amd import _lib
 from nulattice_amd.planewave import PlaneWave, planewave_hamiltonian, shell_holes
@@ -218,3 +219,47 @@ def test_pair_blocks_reproduce_pp_ladder():
             out[i] += r
     l1, l2 = cc.v_pppp_dgrams(coo, t1, t2)
     np.testing.assert_allclose(out.reshape(p, p, h, h), l1 + cc.pIJ(l2), atol=1e-12)
+
+
+class _TwoBodyLike:
+    """The attribute interface of _torch_types.TwoBodyOperator (_torch_types.py:169-199):
+    indices (N, 4) int64 tensor, values (N,) tensor."""
+
+    def __init__(self, indices, values):
+        self.indices, self.values = indices, values
+
+
+@pytest.mark.parametrize("n", [4, 5, 37])
+def test_coo_arrays_containers(n):
+    """Every container of a sparse block gives the same (N,4) rows: the legacy list, a
+    TwoBodyOperator-like object, this package's CooBlock, and the (4, N) pair to_soa_sparse
+    returns (soa/coupled_cluster.py:18-46), including n = 4, where (4, N) and (N, 4) have the
+    same shape and only the container fixes the layout."""
+    from nulattice_amd.hamiltonian import CooList
+    from nulattice_amd.plans import CooBlock
+    rng = np.random.default_rng(n)
+    p, h = 9, 4
+    idx = np.stack([rng.integers(0, p, n), rng.integers(0, p, n), rng.integers(0, p, n),
+                    rng.integers(0, h, n)], 1)
+    val = rng.standard_normal(n)
+    ref_idx, ref_val = coo_arrays([list(map(int, i)) + [float(v)] for i, v in zip(idx, val)],
+                                  p, h, "ppph")
+    np.testing.assert_array_equal(ref_idx, idx)
+    soa = (torch.tensor(idx.T, dtype=torch.long), torch.tensor(val))       # (4, N) as SoA
+    for c in (soa, _TwoBodyLike(torch.tensor(idx), torch.tensor(val)), CooBlock(idx, val),
+              CooList(idx, val)):
+        i2, v2 = coo_arrays(c, p, h, "ppph")
+        np.testing.assert_array_equal(i2, idx)
+        np.testing.assert_array_equal(v2, val)
+    if n != 4:
+        i2, _ = coo_arrays((idx, val), p, h, "ppph")          # unambiguous (N, 4) pair
+        np.testing.assert_array_equal(i2, idx)
+
+
+def test_coo_arrays_rejects_mismatched_layouts():
+    from nulattice_amd.plans import CooBlock
+    idx = np.zeros((5, 4), np.int64)
+    with pytest.raises(ValueError):
+        coo_arrays((idx, np.zeros(6)), 9, 4, "ppph")
+    with pytest.raises(ValueError):
+        coo_arrays(CooBlock(idx.T.copy(), np.zeros(5)), 9, 4, "ppph")

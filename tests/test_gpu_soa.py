@@ -92,3 +92,30 @@ def test_legacy_and_soa_drivers_agree_on_16O():
                                   device="cuda")
     unit = lat.phys_unit(2.5)
     assert abs((e_old + E0) - (e_new + E0)) * unit < 1e-5
+
+
+def test_soa_diis_trajectory_matches_oracle():
+    """examples/Example_CCSD.ipynb's Hamiltonian (L=2 16O, three-body contact): the GPU solver
+    with diis="soa" (soa/coupled_cluster.py:520-567) follows the oracle's SoA variant, whose
+    window and extrapolation are pinned by the oracle's own tests: the steps before the first
+    extrapolation to 1e-12, the rest to the conditioning of the DIIS solve (1e-6)."""
+    import json
+    import os
+    from oracle import ccsd_ref as cc
+    from oracle import hamiltonian_ref as ham_ref
+    from oracle import lattice_ref as lat
+    g = json.load(open(os.path.join(os.path.dirname(__file__), "golden",
+                                    "trajectories.json")))["16O_L2"]
+    _, fock, v = ham_ref.get_norm_ord_int(2, lat.ref_16O_gs, g["vT1"], g["vS1"], g["w3"], True)
+    hr, hg = [], []
+    cc.ccsd_solver(fock, v, eps=1e-12, maxSteps=30, max_diis=6, diis="soa", history=hr)
+    ccm.ccsd_solver(fock, v, eps=1e-12, maxSteps=30, max_diis=6, diis="soa", history=hg)
+    n = min(len(hr), len(hg))
+    assert n >= 20
+    rel = np.abs(np.array(hg[:n]) - np.array(hr[:n])) / np.abs(np.array(hr[:n]))
+    assert rel[:7].max() < 1e-12          # steps 0..6: the first extrapolation follows step 6
+    assert rel.max() < 1e-6
+    # the extrapolations changed the path (a plain-Jacobi run differs after step 6)
+    hj = []
+    cc.ccsd_solver(fock, v, eps=1e-12, maxSteps=10, max_diis=0, history=hj)
+    assert abs(hj[8] - hr[8]) > 1e-8 * abs(hr[8])

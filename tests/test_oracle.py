@@ -257,3 +257,41 @@ def test_oracle_failure_paths(capsys):
     with np.errstate(invalid="ignore", divide="ignore"):
         assert cc.ccsd_solver(f2, vl, max_diis=0, maxSteps=100) is None
     assert "nan error" in capsys.readouterr().out
+
+
+@pytest.mark.parametrize("soa", [False, True])
+def test_oracle_diis_window_is_the_constrained_least_squares_fit(soa):
+    """On a linear fixed-point map x -> G x + b, the oracle's DIIS step returns sum_k c_k x_k
+    with c minimising |sum_k c_k e_k| subject to sum_k c_k = 1, computed here in closed form
+    (c proportional to (E^T E)^-1 1) over exactly the kept window: the last max_diis - 1 errors
+    for the pulsed variant (coupled_cluster.py:635-672), all max_diis for the SoA variant
+    (soa/coupled_cluster.py:520-567).  The other window gives a different vector, so a wrong
+    window or pairing of errors with amplitudes fails the check."""
+    rng = np.random.default_rng(17)
+    n, nd = 30, 6
+    Q = np.linalg.qr(rng.standard_normal((n, n)))[0]
+    G = Q @ np.diag(rng.uniform(-0.85, 0.85, n)) @ Q.T
+    b = rng.standard_normal(n)
+    x = rng.standard_normal(n)
+    split = lambda v: (v[:6].reshape(3, 2).copy(), v[6:].reshape(2, 2, 3, 2).copy())
+    x1, x2 = split(x)
+    h1, h2, errs, xs, es = [x1], [x2], [], [], []
+    ext = None
+    for _ in range(nd):
+        xn = G @ x + b
+        a1, a2 = split(xn)
+        h1.append(a1); h2.append(a2); errs.append(xn - x)
+        xs.append(xn); es.append(xn - x)
+        ext = cc.diis_window_step(h1, h2, errs, nd, soa)
+        x = xn
+    assert ext is not None
+    got = np.concatenate([ext[0].ravel(), ext[1].ravel()])
+
+    def fit(k):
+        E, X = np.stack(es[-k:], 1), np.stack(xs[-k:], 1)
+        c = np.linalg.solve(E.T @ E, np.ones(k))
+        return X @ (c / c.sum())
+    keep = nd if soa else nd - 1
+    np.testing.assert_allclose(got, fit(keep), rtol=1e-9, atol=1e-12)
+    other = fit(nd - 1 if soa else nd)
+    assert np.max(np.abs(other - got)) > 1e-6 * np.max(np.abs(got))
