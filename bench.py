@@ -10,7 +10,7 @@ numpy.random.default_rng(seed), three-body contact D0=1.0 normal-ordered at the 
 dense T2 (p,p,h,h) as in the reference's layout.  A step is one CCSD Jacobi iteration (T1 and T2
 residuals, update with mixing 0.5, energy read back to the host), timed over K steps after W
 warm-ups with the device synchronised on both sides.  ``value`` is the mixed-precision
-iteration rate (default bf16x3: BF16 matrix cores on hi/lo split operands, FP32 accumulation,
+iteration rate (default fp16x3: FP16 matrix cores on hi/lo split operands, FP32 accumulation,
 FP64 energy, denominators and update); the FP64 rate, the speed-up and |dE| of the
 mixed-precision fixed point against the converged FP64 energy (seeds 0..4) are reported beside
 it, with the single-rounding BF16 / FP16 forms for reference, the roofline of the dominant
@@ -39,16 +39,21 @@ def parse():
     ap.add_argument("--L", type=int, default=5)
     ap.add_argument("--A", type=int, default=28)
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--D0", type=float, default=1.0)
-    ap.add_argument("--prec", default="bf16x3", choices=["bf16x3", "fp16x3", "bf16", "fp16"])
+    ap.add_argument("--D0", type=float, default=None,
+                    help="three-body contact (default 1.0; drawn from U[0.5, 1.5] after C0 "
+                         "from the seed in the combined form)")
+    ap.add_argument("--prec", default="fp16x3", choices=["fp16x3", "bf16x3", "bf16", "fp16"])
     ap.add_argument("--no-converge", action="store_true", help="skip the |dE| convergence runs")
     ap.add_argument("--cpu-sample", default="sample", choices=["sample", "full", "none"],
                     help="sample: one oracle iteration at the bench size with every dense "
                          "contraction in full and the two COO loops over every 16th entry "
                          "(their time scaled by 16); full: the whole iteration")
-    ap.add_argument("--form", default="all", choices=["all", "sectors", "diis"],
+    ap.add_argument("--form", default="all", choices=["all", "sectors", "diis", "combined"],
                     help="sectors: time only the symmetry-sector form and print its line; "
-                         "diis: time only one DIIS extrapolation over 10 T2-sized vectors")
+                         "diis: time only one DIIS extrapolation over 10 T2-sized vectors; "
+                         "combined (the default for L >= 6): the largest-size line, FP32 "
+                         "storage through the SoA driver, mixed-precision residual, adjoint "
+                         "gradient, seeded C0 and D0")
     ap.add_argument("--grad", default="on", choices=["on", "off"],
                     help="adjoint dE/dC0, dE/dD0 at C0=-4, D0=1 checked against central FD")
     return ap.parse_args()
@@ -113,6 +118,86 @@ def time_sectors(SH, steps, warmup, torch, graph=True):
         st.step()
     torch.cuda.synchronize()
     return (time.perf_counter() - t0) / steps
+
+
+def combined(a, C0, D0, torch, ccm, engine, ops, adjoint):
+    """The combined configuration at the largest single-GPU size (L=6, A=28: T2 836^2 x 28^2 =
+    5.5e8 amplitudes): the dense mixed-precision iteration rate (and the FP64 one), the SoA
+    driver with FP32 storage (DIIS history and returned amplitudes in FP32) to convergence with
+    |dE| against the FP64 symmetry-sector solve, the matrix-core work per iteration against the
+    peaks, the assembly's HBM rate, and dE/dC0, dE/dD0 from the Lambda equations (sector form)
+    against central finite differences of converged energies."""
+    from nulattice_amd import soa
+    from nulattice_amd.planewave import PlaneWave, planewave_hamiltonian, shell_holes
+    from nulattice_amd.sectors import SectorHamiltonian
+    t0 = time.perf_counter()
+    E0, fock, v = planewave_hamiltonian(a.L, a.A, C0, D0)
+    H = engine.Hamiltonian(fock, v, device="cuda")
+    torch.cuda.synchronize()
+    setup = time.perf_counter() - t0
+    p, h = H.p, H.h
+    print(f"# combined L={a.L} A={a.A} p={p} h={h} C0={C0:.6f} D0={D0:.6f}: setup {setup:.1f} s",
+          file=sys.stderr, flush=True)
+    slp, klp, _ = time_iterations(ccm, engine, ops, H, a.prec, a.steps, a.warmup, torch)
+    print(f"# {a.prec}: {slp * 1e3:.1f} ms/iter", file=sys.stderr, flush=True)
+    s64, k64, _ = time_iterations(ccm, engine, ops, H, "fp64", 2, 1, torch)
+    print(f"# fp64: {s64 * 1e3:.1f} ms/iter", file=sys.stderr, flush=True)
+    n = p * p * h * h
+    asm = klp.get("assemble")
+    # assembly: reads S0, Sab, Sij, Spp, t2, v_pphh (FP64), writes t2n (FP64) and its 16-bit
+    # image (4 B for the split image, 2 B for a BF16 one), each element once
+    img_b = 4 if a.prec.endswith("x3") else 2
+    asm_gbs = (6 * 8 + 8 + img_b) * n / asm / 1e9 if asm else None
+    fl = klp["flops"]
+    out = {"metric": f"CCSD iters/sec at L={a.L} A={a.A}: mixed-prec, FP32 storage, "
+                     f"|dE| vs FP64, adjoint gradient",
+           "value": 1.0 / slp, "unit": "iters/s", "n_gpus": 1, "steps": a.steps,
+           "warmup": a.warmup, "ms_per_step": slp * 1e3, "higher_is_better": True,
+           "dtype": f"{a.prec}+fp64, fp32 storage",
+           "config": {"workload": f"CCSD L={a.L} A={a.A} plane-wave, dense T2 {p}x{p}x{h}x{h}",
+                      "L": a.L, "A": a.A, "p": p, "h": h, "C0": C0, "D0": D0, "seed": a.seed},
+           "fp64_ms_per_step": s64 * 1e3, "speedup_vs_fp64": s64 / slp,
+           "mfma_frac_executed": (fl["lp"] / 2.5e15 + fl["fp64"] / 78.6e12) / slp,
+           "executed_flop": fl, "assembly_ms": asm * 1e3 if asm else None,
+           "assembly_hbm_GBps": asm_gbs, "setup_s": setup}
+    SH = SectorHamiltonian.from_planewave(PlaneWave(a.L, shell_holes(a.L, a.A), C0, D0))
+    e64, _, _ = ccm.ccsd_solver(None, None, eps=1e-12, maxSteps=400, max_diis=0, hamiltonian=SH)
+    del SH
+    info = {}
+    e32, t1, t2 = soa.ccsd_solver(fock, v, eps=1e-10, maxSteps=200, dtype=torch.float32,
+                                  hamiltonian=H, info=info)
+    out.update(E_corr_fp64=e64, E_corr_fp32_storage=e32, dE_rel=abs(e32 - e64) / abs(e64),
+               amplitudes_dtype=str(t2.dtype).replace("torch.", ""),
+               steps=info.get("steps_lowprec", 0) + info.get("steps_fp64", 0),
+               seconds_to_solution=info.get("seconds"))
+    print(f"# FP32-storage SoA solve: E {e32!r} vs FP64 {e64!r}: |dE|/|E| {out['dE_rel']:.2e} "
+          f"({out['steps']} steps, {out['seconds_to_solution']:.1f} s)", file=sys.stderr,
+          flush=True)
+    del t1, t2, H
+    torch.cuda.empty_cache()
+    # adjoint gradient (symmetry-sector Lambda equations) against central differences
+    c = torch.tensor(C0, dtype=torch.float64, requires_grad=True)
+    d = torch.tensor(D0, dtype=torch.float64, requires_grad=True)
+    t0 = time.perf_counter()
+    E = adjoint.planewave_ccsd_energy(c, d, a.L, a.A, eps=1e-13, form="sectors")
+    E.backward()
+    tg = time.perf_counter() - t0
+    g = {"C0": float(c.grad), "D0": float(d.grad)}
+    hstep, fd = 1e-4, {}
+    for name in ("C0", "D0"):
+        Es = []
+        for sgn in (1.0, -1.0):
+            cc0 = C0 + (sgn * hstep if name == "C0" else 0.0)
+            dd0 = D0 + (sgn * hstep if name == "D0" else 0.0)
+            Es.append(float(adjoint.planewave_ccsd_energy(
+                torch.tensor(cc0, dtype=torch.float64), torch.tensor(dd0, dtype=torch.float64),
+                a.L, a.A, eps=1e-13, form="sectors")))
+        fd[name] = (Es[0] - Es[1]) / (2 * hstep)
+    out["adjoint_gradient"] = {"form": "symmetry-sector Lambda equations", "adjoint": g,
+                               "fd": fd, "fd_step": hstep, "seconds": tg,
+                               "max_rel_err": max(abs(g[k] - fd[k]) / abs(fd[k]) for k in g)}
+    print(f"# gradient: {out['adjoint_gradient']}", file=sys.stderr, flush=True)
+    return out
 
 
 def sectors_only(a, C0, torch, ccm):
@@ -366,7 +451,14 @@ def main():
 
     rng = np.random.default_rng(a.seed)
     C0 = float(rng.uniform(-4.5, -3.5))
+    if a.form == "all" and a.L >= 6:
+        a.form = "combined"
+    if a.D0 is None:
+        a.D0 = float(rng.uniform(0.5, 1.5)) if a.form == "combined" else 1.0
     t0 = time.perf_counter()
+    if a.form == "combined":
+        print(json.dumps(combined(a, C0, a.D0, torch, ccm, engine, ops, adjoint)))
+        return
     if a.form == "sectors":
         print(json.dumps(sectors_only(a, C0, torch, ccm)))
         return

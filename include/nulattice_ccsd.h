@@ -123,10 +123,10 @@ int nl_convert_rows(int dtype, int64_t R1, int64_t R2, int64_t C1, int64_t C2,
                     float* iscale, float target, void* stream);
 
 /* tau[a,b,i,j] = t2 + t1[a,i] t1[b,j] - t1[a,j] t1[b,i] (ccDgrams.py:215-237, 444-472) as the
- * hi/lo BF16 split of the precision form with split operands: row (a,b) of out is
- * [hi(h*h) | lo(h*h)]; tau (FP64, may be null) receives the value as well. */
+ * hi/lo split (dtype 0 BF16, 1 FP16) of tau * scale for the precision form with split operands:
+ * row (a,b) of out is [hi(h*h) | lo(h*h)]; tau (FP64, may be null) receives the value too. */
 int nl_tau_split(int64_t p, int64_t h, const double* t1, const double* t2, double* tau,
-                 void* out, void* stream);
+                 void* out, int dtype, double scale, void* stream);
 
 /* nl_gather_rows16 writing k < ldk of each output row at row pitch ldo >= ldk (the hi and lo
  * halves of a split operand row are two gathers into one [hi | lo] row). */
@@ -134,8 +134,8 @@ int nl_gather_rows16_into(int64_t nb, int64_t kb, int64_t ncol, const int64_t* i
                           const void* src, int64_t lds, void* out, int64_t ldk, int64_t ldo,
                           void* stream);
 
-/* nl_convert_rows writing a hi/lo split: with y = x * scale (per-row scale as in nl_convert_rows;
- * target 0: unscaled), hi = 16-bit(y), lo = 16-bit(y - hi), so hi + lo carries ~17 (BF16) or ~23
+/* nl_convert_rows writing a hi/lo split: with y = x * scale * (per-row scale as in
+ * nl_convert_rows; target 0: none), hi = 16-bit(y), lo = 16-bit(y - hi), so hi + lo carries ~17 (BF16) or ~23
  * (FP16) significant bits.  Row r gets nblk blocks of kpad columns, block q at
  * out[r*ldo + q*kpad + c] holding lo when bit q of lomask is set and hi otherwise (columns
  * C1*C2 <= c < kpad zero).  GEMM operands [hi|lo|hi] x [hi|hi|lo] over the K-concatenated blocks
@@ -143,7 +143,7 @@ int nl_gather_rows16_into(int64_t nb, int64_t kb, int64_t ncol, const int64_t* i
 int nl_convert_split(int dtype, int64_t R1, int64_t R2, int64_t C1, int64_t C2,
                      const int64_t* strides4, const double* in, void* out, int64_t ldo,
                      int64_t kpad, int nblk, int lomask, float* iscale, float target,
-                     void* stream);
+                     double scale, void* stream);
 
 /* out[b][n][k] (BF16) = src[idx[b*kb + k]*lds + n] for n < ncol, k < kb (0 where idx < 0), and 0
  * for kb <= k < ldk (ldk a multiple of 8): the row gather of the packed pair-block products
@@ -239,13 +239,14 @@ int nl_assemble_t2_16(int64_t p, int64_t h, const double* S0, const double* Sab,
  * vectors dpp (p) and dhh (h): H2 += (dpp[a] + dpp[b] + dhh[i] + dhh[j]) t2[a,b,i,j] before the
  * update.  The precision form runs the X_pp t2 dressing (coupled_cluster.py:536) in low precision
  * without X_pp's diagonal, whose product with t2 is the largest term of the residual and is
- * added here exactly.  image_kind 0: t2n16 is the BF16 image of nl_assemble_t2_16; 1: the hi/lo
- * split of t2n (as nl_convert_split, unscaled BF16), row a as [hi(p*h*h) | lo(p*h*h)]. */
+ * added here exactly.  image_kind 0: t2n16 is the BF16 image of nl_assemble_t2_16; 1 (BF16) or 2
+ * (FP16): the hi/lo split of t2n * image_scale, row a as [hi(p*h*h) | lo(p*h*h)]. */
 int nl_assemble_t2_ext(int64_t p, int64_t h, const double* S0, const double* Sab,
                        const double* Sij, const double* Spp, const double* Xpp, const double* Xhh,
                        const double* dpp, const double* dhh, const double* t2,
                        const double* vpphh, const double* t1n, double mixing, double* t2n,
-                       double* H2out, double* part, void* t2n16, int image_kind, void* stream);
+                       double* H2out, double* part, void* t2n16, int image_kind,
+                       double image_scale, void* stream);
 
 /* d[i] = X[i,i], then X[i,i] = 0, for a row-major n x n FP64 matrix (the diagonal split of a
  * dressing matrix before its low-precision product; coupled_cluster.py:536). */
@@ -294,6 +295,16 @@ int nl_gram_finalize(int64_t nparts, int npair, const double* part, double* out,
  * pointers. */
 int nl_lincomb(int64_t n, int k, const double* coef, const double* const* x, double* y,
                void* stream);
+
+/* The DIIS history kept in FP32, as the SoA driver keeps it with dtype=torch.float32
+ * (soa/coupled_cluster.py:453-460, 520-567): nl_copy_sub storing the entry and error vector as
+ * float, and the Gram pass and the extrapolation reading float vectors (products, sums and the
+ * extrapolated amplitudes in FP64). */
+int nl_copy_sub_f32(int64_t n, const double* x, const double* y, float* hcopy, float* c,
+                    void* stream);
+int nl_gram_part_f32(int64_t n, int k, const float* const* x, double* part, void* stream);
+int nl_lincomb_f32(int64_t n, int k, const double* coef, const float* const* x, double* y,
+                   void* stream);
 
 /* ---- Symmetry-sector (block-sparse) form of the same iteration ----
  * For Hamiltonians whose single-particle states carry additive conserved labels that identify

@@ -32,7 +32,7 @@ SIGNATURES = {
     "nl_gemm_lp_rows": (_i32, [_i32, _i64, _i64, _i64, _dbl, _vp, _i64, _i64, _vp, _i64, _vp, _i64,
                                _i64, _vp, _i64, _vp, _i64, _vp, _i64, _vp]),
     "nl_convert_split": (_i32, [_i32, _i64, _i64, _i64, _i64, _vp, _vp, _vp, _i64, _i64, _i32,
-                                _i32, _vp, _flt, _vp]),
+                                _i32, _vp, _flt, _dbl, _vp]),
     "nl_gather_convert": (_i32, [_i64, _i64, _i64, _vp, _vp, _i64, _vp, _i64, _vp]),
     "nl_gather_rows16": (_i32, [_i64, _i64, _i64, _vp, _vp, _i64, _vp, _i64, _vp]),
     "nl_tau16": (_i32, [_i64, _i64, _vp, _vp, _vp, _vp]),
@@ -57,8 +57,8 @@ SIGNATURES = {
     "nl_assemble_t2_16": (_i32, [_i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _dbl,
                                  _vp, _vp, _vp, _vp, _vp]),
     "nl_assemble_t2_ext": (_i32, [_i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
-                                  _vp, _dbl, _vp, _vp, _vp, _vp, _i32, _vp]),
-    "nl_tau_split": (_i32, [_i64, _i64, _vp, _vp, _vp, _vp, _vp]),
+                                  _vp, _dbl, _vp, _vp, _vp, _vp, _i32, _dbl, _vp]),
+    "nl_tau_split": (_i32, [_i64, _i64, _vp, _vp, _vp, _vp, _i32, _dbl, _vp]),
     "nl_gather_rows16_into": (_i32, [_i64, _i64, _i64, _vp, _vp, _i64, _vp, _i64, _i64, _vp]),
     "nl_take_diag": (_i32, [_i64, _vp, _vp, _vp]),
     "nl_assemble_t2_adj": (_i32, [_i64, _i64, _vp, _vp, _vp, _vp, _vp]),
@@ -70,6 +70,9 @@ SIGNATURES = {
     "nl_sub": (_i32, [_i64, _vp, _vp, _vp, _vp]),
     "nl_copy_sub": (_i32, [_i64, _vp, _vp, _vp, _vp, _vp]),
     "nl_gram_part": (_i32, [_i64, _i32, _vp, _vp, _vp]),
+    "nl_copy_sub_f32": (_i32, [_i64, _vp, _vp, _vp, _vp, _vp]),
+    "nl_gram_part_f32": (_i32, [_i64, _i32, _vp, _vp, _vp]),
+    "nl_lincomb_f32": (_i32, [_i64, _i32, _vp, _vp, _vp, _vp]),
     "nl_gram_finalize": (_i32, [_i64, _i32, _vp, _vp, _vp]),
     "nl_lincomb": (_i32, [_i64, _i32, _vp, _vp, _vp, _vp]),
     "nl_gemm_grouped_f64": (_i32, [_i32, _i64, _vp, _vp, _dbl, _vp, _vp, _dbl, _vp, _vp, _vp]),

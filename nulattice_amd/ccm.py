@@ -49,11 +49,24 @@ def _t(x, device):
 
 
 class _Diis:
-    """Pulsed DIIS on device vectors; B-matrix solve in FP64 on the host (a 10x10 system)."""
+    """Pulsed DIIS on device vectors; B-matrix solve in FP64 on the host (a 10x10 system).
+    ``store`` is the dtype of the kept history entries and error vectors: float64, or float32
+    as the SoA driver keeps them with dtype=torch.float32 (soa/coupled_cluster.py:453-460,
+    520-567); products, sums and the extrapolated amplitudes are FP64 either way."""
 
-    def __init__(self, mode, max_diis, t1, t2, ccs):
+    def __init__(self, mode, max_diis, t1, t2, ccs, store=torch.float64):
         self.mode, self.n, self.ccs = mode, max_diis, ccs
-        self.h1, self.h2, self.e1, self.e2 = [ops.copy(t1)], [ops.copy(t2)], [], []
+        self.f32 = store == torch.float32
+        self.h1, self.h2, self.e1, self.e2 = [self._entry(t1)], [self._entry(t2)], [], []
+
+    def _entry(self, t):
+        if not self.f32:
+            return ops.copy(t)
+        hc = torch.empty(t.shape, dtype=torch.float32, device=t.device)
+        junk = torch.empty_like(hc)
+        _lib.call("nl_copy_sub_f32", t.numel(), ops._p(t), ops._p(t), ops._p(hc), ops._p(junk),
+                  ops._stream())
+        return hc
 
     def step(self, t1, t2, o1, o2, B=None):
         """Record (t, t - old); return extrapolated (t1, t2) or None.  ``B`` is unused (kept for
@@ -63,9 +76,11 @@ class _Diis:
         for t, o, hist, errs in ((t1, o1, self.h1, self.e1), (t2, o2, self.h2, self.e2)):
             if not (t.is_contiguous() and o.is_contiguous()) or t.numel() != o.numel():
                 raise _lib.NLError("DIIS vectors must be contiguous and of one size")
-            hc, e = torch.empty_like(t), torch.empty_like(t)
-            _lib.call("nl_copy_sub", t.numel(), ops._p(t), ops._p(o), ops._p(hc), ops._p(e),
-                      ops._stream())
+            dt = torch.float32 if self.f32 else torch.float64
+            hc = torch.empty(t.shape, dtype=dt, device=t.device)
+            e = torch.empty_like(hc)
+            _lib.call("nl_copy_sub_f32" if self.f32 else "nl_copy_sub", t.numel(), ops._p(t),
+                      ops._p(o), ops._p(hc), ops._p(e), ops._stream())
             hist.append(hc); errs.append(e)
         if soa and len(self.e1) > self.n:
             for L in (self.h1, self.h2, self.e1, self.e2):
@@ -90,7 +105,7 @@ class _Diis:
         except np.linalg.LinAlgError:
             pass
         r1, r2 = out if out is not None else (t1, t2)
-        self.h1, self.h2, self.e1, self.e2 = [ops.copy(r1)], [ops.copy(r2)], [], []
+        self.h1, self.h2, self.e1, self.e2 = [self._entry(r1)], [self._entry(r2)], [], []
         return out
 
 
@@ -239,7 +254,7 @@ def ccsd_solver(fock_mats, two_body_int, t1initial=None, eps=1e-8, maxSteps=1000
                 delta=0, mixing=0.5, verbose=False, sparse=True, ccs=False, *, device=None,
                 diis="legacy", precision="fp64", refine=True, eps_lowprec=None, history=None,
                 info=None, hamiltonian=None, adjoint=False, lambda_eps=1e-10, sectors=None,
-                _soa_rules=False):
+                storage=torch.float64, _soa_rules=False):
     """Damped-Jacobi CCSD (coupled_cluster.py:547-687) on one MI355X.
 
     Returns (E_corr, t1, t2); like the reference, returns None after a divergence or NaN.
@@ -253,6 +268,8 @@ def ccsd_solver(fock_mats, two_body_int, t1initial=None, eps=1e-8, maxSteps=1000
     to the rounding level of the contractions) and then, if ``refine``, continues in FP64 to
     ``eps``.  ``adjoint=True`` also solves the CC Lambda equations at the returned amplitudes
     (to |residual| <= ``lambda_eps`` |dE/dt|) and stores (lambda1, lambda2) in info["lambda"].
+    ``storage=torch.float32`` keeps the DIIS history in FP32 and returns t1, t2 as float32
+    (the working amplitudes, energy and denominators stay FP64).
     ``_soa_rules`` applies the SoA driver's stopping and failure rules instead (see soa.py).
     """
     if adjoint and info is None:
@@ -285,7 +302,11 @@ def ccsd_solver(fock_mats, two_body_int, t1initial=None, eps=1e-8, maxSteps=1000
     if ccs or t1initial is not None:
         t2.zero_()
     pulsed = diis != "legacy" and max_diis > 0
-    dstate = _Diis(diis, max_diis, t1, t2, ccs) if pulsed else None
+    if storage not in (torch.float64, torch.float32):
+        raise ValueError("storage must be torch.float64 or torch.float32")
+    dstate = _Diis(diis, max_diis, t1, t2, ccs, storage) if pulsed else None
+    out = (lambda e, a, b: (e, a, b)) if storage == torch.float64 else \
+        (lambda e, a, b: (e, _as_f32(a), _as_f32(b)))
     img = engine.T2Image()      # BF16 image of t2 handed from one step's assembly to the next
     engine.energy(ham, t1, t2, B)
     prev = float(B.E.item())
@@ -349,7 +370,12 @@ def ccsd_solver(fock_mats, two_body_int, t1initial=None, eps=1e-8, maxSteps=1000
             _fill_info(info, steps_lp, steps_fp64, t_start, energy, True)
             if adjoint:
                 _lambda(ham, t1, t2, B, mixing, lambda_eps, info)
-            return energy, t1, t2
+            return out(energy, t1, t2)
+        if dstate is not None and dstate.f32 and rel < 1e-7:
+            # an FP32 history cannot carry the amplitudes below its rounding (each
+            # extrapolation would re-inject ~6e-8 relative noise): the last steps run as plain
+            # damped Jacobi in FP64
+            dstate = None
         if dstate is not None:
             ext = dstate.step(t1, t2, old1, old2, B)
             if ext is not None:
@@ -374,7 +400,17 @@ def ccsd_solver(fock_mats, two_body_int, t1initial=None, eps=1e-8, maxSteps=1000
     _fill_info(info, steps_lp, steps_fp64, t_start, energy, False)
     if adjoint:
         _lambda(ham, t1, t2, B, mixing, lambda_eps, info)
-    return energy, t1, t2
+    return out(energy, t1, t2)
+
+
+def _as_f32(t):
+    """float32 copy of an FP64 device tensor (the library's cast: the copy half of
+    nl_copy_sub_f32)."""
+    o = torch.empty(t.shape, dtype=torch.float32, device=t.device)
+    junk = torch.empty_like(o)
+    _lib.call("nl_copy_sub_f32", t.numel(), ops._p(t), ops._p(t), ops._p(o), ops._p(junk),
+              ops._stream())
+    return o
 
 
 def _fill_info(info, steps_lp, steps_fp64, t_start, energy, converged=False):

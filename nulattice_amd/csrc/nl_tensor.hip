@@ -188,6 +188,7 @@ struct SplitArgs {
   float* iscale;                  // per-row inverse scale (null: unscaled)
   float target;
   int nblk, lomask, bf16;         // block q holds the lo part when lomask bit q is set, else hi
+  double scale;                   // fixed factor applied before the split (with the row scale)
 };
 
 __device__ inline float from16(uint16_t b, int bf16) {
@@ -196,9 +197,14 @@ __device__ inline float from16(uint16_t b, int bf16) {
 }
 
 __device__ inline void split16(double xs, int bf16, uint16_t& hi, uint16_t& lo) {
-  const float f = (float)xs;
+  // FP64 -> FP32 -> 16 bits, each step a hardware conversion: the FP32 values pass through an
+  // empty asm so that the compiler cannot fold the two roundings into one FP64 -> FP16 rounding
+  // (a long software sequence on this target)
+  float f = (float)xs;
+  asm volatile("" : "+v"(f));
   hi = bf16 ? to_bf16_bits(f) : to_f16_bits(f);
-  const float r = (float)(xs - (double)from16(hi, bf16));
+  float r = (float)(xs - (double)from16(hi, bf16));
+  asm volatile("" : "+v"(r));
   lo = bf16 ? to_bf16_bits(r) : to_f16_bits(r);
 }
 
@@ -227,7 +233,7 @@ __global__ __launch_bounds__(NT) void split_rows_kernel(SplitArgs a) {
     uint16_t hi = 0, lo = 0;
     if (c < a.C) {
       const int64_t c1 = fdiv((int)c, a.fc2), c2 = c - c1 * a.C2;
-      split16(base[c1 * a.s[2] + c2 * a.s[3]] * (double)sc, a.bf16, hi, lo);
+      split16(base[c1 * a.s[2] + c2 * a.s[3]] * ((double)sc * a.scale), a.bf16, hi, lo);
     }
     for (int q = 0; q < a.nblk; ++q) o[q * a.kpad + c] = ((a.lomask >> q) & 1) ? lo : hi;
   }
@@ -273,7 +279,7 @@ __global__ __launch_bounds__(NT) void split_tiles_kernel(SplitArgs a, int64_t R,
       uint16_t hi = 0, lo = 0;
       if (c < a.C) {
         const int64_t c1 = fdiv((int)c, a.fc2), c2 = c - c1 * a.C2;
-        split16(base[c1 * a.s[2] + c2 * a.s[3]] * (double)sc, a.bf16, hi, lo);
+        split16(base[c1 * a.s[2] + c2 * a.s[3]] * ((double)sc * a.scale), a.bf16, hi, lo);
       }
       thi[tr][cc] = hi;
       tlo[tr][cc] = lo;
@@ -553,7 +559,8 @@ __global__ __launch_bounds__(NT) void tau_kernel(int p, int h, FastDiv fp, FastD
 // [hi(h*h) | lo(h*h)] (row pitch 2*h*h), optionally with the FP64 tau beside it.
 __global__ __launch_bounds__(NT) void tau_split_kernel(int p, int h, FastDiv fp, FastDiv fh,
                                                        const double* t1, const double* t2,
-                                                       double* tau, uint16_t* out) {
+                                                       double* tau, uint16_t* out, int bf16,
+                                                       double scale) {
   __shared__ double ta[NT], tb[NT];
   const int hh = h * h;
   for (int q = blockIdx.x; q < p * p; q += gridDim.x) {
@@ -571,7 +578,7 @@ __global__ __launch_bounds__(NT) void tau_split_kernel(int p, int h, FastDiv fp,
       const double v = tau_value(x[e], ta[i], tb[j], ta[j], tb[i]);
       if (tau) tau[(int64_t)q * hh + e] = v;
       uint16_t hi, lo;
-      split16(v, 1, hi, lo);
+      split16(v * scale, bf16, hi, lo);
       y[e] = hi;
       y[hh + e] = lo;
     }
@@ -661,7 +668,9 @@ struct AssembleArgs {
   double* H2out;                        // optional: raw residual H2 (for adjoint / diagnostics)
   uint16_t* t2n16;                      // optional: BF16 image of t2n (same layout, RNE via FP32)
   int split16;                          // t2n16 holds the hi/lo split instead: row a of t2n as
-                                        // [hi(p*h*h) | lo(p*h*h)] (row pitch 2*p*h*h)
+                                        // [hi(p*h*h) | lo(p*h*h)] (row pitch 2*p*h*h);
+                                        // 1: BF16, 2: FP16 of t2n * img_scale
+  double img_scale;
 };
 
 __device__ inline void write_image(const AssembleArgs& g, int64_t t, int64_t a, double tn) {
@@ -671,7 +680,7 @@ __device__ inline void write_image(const AssembleArgs& g, int64_t t, int64_t a, 
   }
   const int64_t row = g.p * g.h * g.h, o = t + a * row;
   uint16_t hi, lo;
-  split16(tn, 1, hi, lo);
+  split16(tn * g.img_scale, g.split16 == 1, hi, lo);
   g.t2n16[o] = hi;
   g.t2n16[o + row] = lo;
 }
@@ -865,12 +874,15 @@ __global__ __launch_bounds__(NT) void dot_part_kernel(int64_t n, const double* x
 constexpr int GRAM_MAX = 12;
 constexpr int LINCOMB_MAX = 16;
 
+// T: the stored vectors' type (double, or float for a history kept in FP32); the products and
+// sums are FP64 either way.
+template <typename T>
 struct GramArgs {
-  const double* x[GRAM_MAX];
+  const T* x[GRAM_MAX];
 };
 
-template <int K>
-__global__ __launch_bounds__(NT) void gram_part_kernel(int64_t n, GramArgs g, double* part) {
+template <int K, typename T>
+__global__ __launch_bounds__(NT) void gram_part_kernel(int64_t n, GramArgs<T> g, double* part) {
   constexpr int NP = K * (K + 1) / 2;
   double acc[NP];
 #pragma unroll
@@ -879,7 +891,7 @@ __global__ __launch_bounds__(NT) void gram_part_kernel(int64_t n, GramArgs g, do
        t += (int64_t)gridDim.x * blockDim.x) {
     double v[K];
 #pragma unroll
-    for (int a = 0; a < K; ++a) v[a] = g.x[a][t];
+    for (int a = 0; a < K; ++a) v[a] = (double)g.x[a][t];
 #pragma unroll
     for (int a = 0; a < K; ++a)
 #pragma unroll
@@ -910,31 +922,32 @@ __global__ __launch_bounds__(NT) void gram_finalize_kernel(int64_t nparts, const
   if (threadIdx.x == 0) out[blockIdx.x] = s;
 }
 
+template <typename T>
 struct LincombArgs {
-  const double* x[LINCOMB_MAX];
+  const T* x[LINCOMB_MAX];
   double c[LINCOMB_MAX];
 };
 
-// y = sum_k c[k] x_k, one pass (reads K vectors, writes one)
-template <int K>
-__global__ __launch_bounds__(NT) void lincomb_kernel(int64_t n, LincombArgs g, double* y) {
+// y = sum_k c[k] x_k, one pass (reads K vectors, writes one FP64 vector)
+template <int K, typename T>
+__global__ __launch_bounds__(NT) void lincomb_kernel(int64_t n, LincombArgs<T> g, double* y) {
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n;
        t += (int64_t)gridDim.x * blockDim.x) {
-    double s = g.c[0] * g.x[0][t];
+    double s = g.c[0] * (double)g.x[0][t];
 #pragma unroll
-    for (int k = 1; k < K; ++k) s += g.c[k] * g.x[k][t];
+    for (int k = 1; k < K; ++k) s += g.c[k] * (double)g.x[k][t];
     y[t] = s;
   }
 }
 
-// hcopy = x, c = x - y in one pass (a DIIS step's history entry and error vector)
-__global__ void copy_sub_kernel(int64_t n, const double* x, const double* y, double* hcopy,
-                                double* c) {
+// hcopy = x, c = x - y in one pass (a DIIS step's history entry and error vector), stored as T
+template <typename T>
+__global__ void copy_sub_kernel(int64_t n, const double* x, const double* y, T* hcopy, T* c) {
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n;
        t += (int64_t)gridDim.x * blockDim.x) {
     const double v = x[t];
-    hcopy[t] = v;
-    c[t] = v - y[t];
+    hcopy[t] = (T)v;
+    c[t] = (T)(v - y[t]);
   }
 }
 
@@ -1041,7 +1054,7 @@ int nl_convert_rows(int dtype, int64_t R1, int64_t R2, int64_t C1, int64_t C2,
 int nl_convert_split(int dtype, int64_t R1, int64_t R2, int64_t C1, int64_t C2,
                      const int64_t* strides4, const double* in, void* out, int64_t ldo,
                      int64_t kpad, int nblk, int lomask, float* iscale, float target,
-                     void* stream) {
+                     double scale, void* stream) {
   const bool scaled = target != 0.f;
   if ((dtype != 0 && dtype != 1) || !strides4 || !in || !out || (scaled && !iscale))
     return NL_EARG;
@@ -1054,7 +1067,7 @@ int nl_convert_split(int dtype, int64_t R1, int64_t R2, int64_t C1, int64_t C2,
   a.fr2 = nl_fastdiv(R2); a.fc2 = nl_fastdiv(C2);
   for (int d = 0; d < 4; ++d) a.s[d] = strides4[d];
   a.in = in; a.out = (uint16_t*)out; a.iscale = scaled ? iscale : nullptr; a.target = target;
-  a.nblk = nblk; a.lomask = lomask; a.bf16 = dtype == 0;
+  a.nblk = nblk; a.lomask = lomask; a.bf16 = dtype == 0; a.scale = scale;
   hipStream_t st = (hipStream_t)stream;
   const int64_t R = R1 * R2;
   if (a.s[1] == 1 && R2 >= 8 && a.s[3] != 1 && nl_cdiv(kpad, 64) <= 65535) {
@@ -1130,13 +1143,13 @@ int nl_gather_rows16_into(int64_t nb, int64_t kb, int64_t ncol, const int64_t* i
 }
 
 int nl_tau_split(int64_t p, int64_t h, const double* t1, const double* t2, double* tau,
-                 void* out, void* stream) {
-  if (p < 0 || h < 0 || !t1 || !t2 || !out) return NL_EARG;
+                 void* out, int dtype, double scale, void* stream) {
+  if (p < 0 || h < 0 || !t1 || !t2 || !out || (dtype != 0 && dtype != 1)) return NL_EARG;
   if (p * p * h * h == 0) return NL_OK;
   if (p * p > INT32_MAX || h * h > INT32_MAX || p * h > INT32_MAX || h > NT) return NL_EARG;
   hipLaunchKernelGGL(tau_split_kernel, dim3((unsigned)std::min<int64_t>(p * p, 65536)),
                      dim3(NT), 0, (hipStream_t)stream, (int)p, (int)h, nl_fastdiv(p),
-                     nl_fastdiv(h), t1, t2, tau, (uint16_t*)out);
+                     nl_fastdiv(h), t1, t2, tau, (uint16_t*)out, dtype == 0, scale);
   NL_LAUNCH_CHECK();
   return NL_OK;
 }
@@ -1331,7 +1344,7 @@ static int assemble_t2_impl(int64_t p, int64_t h, const double* S0, const double
                             const double* t1n, double mixing, double* t2n, double* H2out,
                             double* part, uint16_t* t2n16, void* stream,
                             const double* dpp = nullptr, const double* dhh = nullptr,
-                            int split16 = 0) {
+                            int split16 = 0, double img_scale = 1.0) {
   if (p < 0 || h < 0 || !Xpp || !Xhh || !t2 || !part) return NL_EARG;
   if (vpphh && !t1n) return NL_EARG;
   int64_t n = p * p * h * h;
@@ -1340,6 +1353,7 @@ static int assemble_t2_impl(int64_t p, int64_t h, const double* S0, const double
   g.p = p; g.h = h; g.S0 = S0; g.Sab = Sab; g.Sij = Sij; g.Spp = Spp; g.Xpp = Xpp; g.Xhh = Xhh;
   g.t2 = t2; g.vpphh = vpphh; g.t1n = t1n; g.mixing = mixing; g.t2n = t2n; g.part = part;
   g.H2out = H2out; g.t2n16 = t2n16; g.dpp = dpp; g.dhh = dhh; g.split16 = split16;
+  g.img_scale = img_scale;
   g.fp = nl_fastdiv(p); g.fh = nl_fastdiv(h);
   if (h <= AH && (Sij || Spp) && p * p < INT32_MAX)
     hipLaunchKernelGGL(assemble_pair_kernel, dim3(grid_for(n, 4096)), dim3(NT), 0,
@@ -1373,10 +1387,10 @@ int nl_assemble_t2_ext(int64_t p, int64_t h, const double* S0, const double* Sab
                        const double* dpp, const double* dhh, const double* t2,
                        const double* vpphh, const double* t1n, double mixing, double* t2n,
                        double* H2out, double* part, void* t2n16, int image_kind,
-                       void* stream) {
-  if (image_kind != 0 && image_kind != 1) return NL_EARG;
+                       double image_scale, void* stream) {
+  if (image_kind < 0 || image_kind > 2) return NL_EARG;
   return assemble_t2_impl(p, h, S0, Sab, Sij, Spp, Xpp, Xhh, t2, vpphh, t1n, mixing, t2n, H2out,
-                          part, (uint16_t*)t2n16, stream, dpp, dhh, image_kind);
+                          part, (uint16_t*)t2n16, stream, dpp, dhh, image_kind, image_scale);
 }
 
 __global__ void take_diag_kernel(int64_t n, double* X, double* d) {
@@ -1465,19 +1479,34 @@ int nl_sub(int64_t n, const double* x, const double* y, double* c, void* stream)
   return NL_OK;
 }
 
-int nl_copy_sub(int64_t n, const double* x, const double* y, double* hcopy, double* c,
-                void* stream) {
+extern "C++" {
+template <typename T>
+static int copy_sub_impl(int64_t n, const double* x, const double* y, T* hcopy, T* c,
+                         void* stream) {
   if (n < 0 || !x || !y || !hcopy || !c) return NL_EARG;
   if (n == 0) return NL_OK;
-  hipLaunchKernelGGL(copy_sub_kernel, dim3(grid_for(n)), dim3(NT), 0, (hipStream_t)stream, n, x,
-                     y, hcopy, c);
+  hipLaunchKernelGGL((copy_sub_kernel<T>), dim3(grid_for(n)), dim3(NT), 0, (hipStream_t)stream,
+                     n, x, y, hcopy, c);
   NL_LAUNCH_CHECK();
   return NL_OK;
 }
+}  // extern "C++"
 
-int nl_gram_part(int64_t n, int k, const double* const* x, double* part, void* stream) {
+int nl_copy_sub(int64_t n, const double* x, const double* y, double* hcopy, double* c,
+                void* stream) {
+  return copy_sub_impl(n, x, y, hcopy, c, stream);
+}
+
+int nl_copy_sub_f32(int64_t n, const double* x, const double* y, float* hcopy, float* c,
+                    void* stream) {
+  return copy_sub_impl(n, x, y, hcopy, c, stream);
+}
+
+extern "C++" {
+template <typename T>
+static int gram_part_impl(int64_t n, int k, const T* const* x, double* part, void* stream) {
   if (n < 0 || k < 1 || k > GRAM_MAX || !x || !part) return NL_EARG;
-  GramArgs g = {};
+  GramArgs<T> g = {};
   for (int a = 0; a < k; ++a) {
     if (n > 0 && !x[a]) return NL_EARG;
     g.x[a] = x[a];
@@ -1486,7 +1515,7 @@ int nl_gram_part(int64_t n, int k, const double* const* x, double* part, void* s
   hipStream_t s = (hipStream_t)stream;
   switch (k) {
 #define NL_GRAM_CASE(K) \
-    case K: hipLaunchKernelGGL((gram_part_kernel<K>), grid, block, 0, s, n, g, part); break;
+    case K: hipLaunchKernelGGL((gram_part_kernel<K, T>), grid, block, 0, s, n, g, part); break;
     NL_GRAM_CASE(1) NL_GRAM_CASE(2) NL_GRAM_CASE(3) NL_GRAM_CASE(4) NL_GRAM_CASE(5)
     NL_GRAM_CASE(6) NL_GRAM_CASE(7) NL_GRAM_CASE(8) NL_GRAM_CASE(9) NL_GRAM_CASE(10)
     NL_GRAM_CASE(11) NL_GRAM_CASE(12)
@@ -1494,6 +1523,15 @@ int nl_gram_part(int64_t n, int k, const double* const* x, double* part, void* s
   }
   NL_LAUNCH_CHECK();
   return NL_OK;
+}
+}  // extern "C++"
+
+int nl_gram_part(int64_t n, int k, const double* const* x, double* part, void* stream) {
+  return gram_part_impl(n, k, x, part, stream);
+}
+
+int nl_gram_part_f32(int64_t n, int k, const float* const* x, double* part, void* stream) {
+  return gram_part_impl(n, k, x, part, stream);
 }
 
 int nl_gram_finalize(int64_t nparts, int npair, const double* part, double* out, void* stream) {
@@ -1504,11 +1542,13 @@ int nl_gram_finalize(int64_t nparts, int npair, const double* part, double* out,
   return NL_OK;
 }
 
-int nl_lincomb(int64_t n, int k, const double* coef, const double* const* x, double* y,
-               void* stream) {
+extern "C++" {
+template <typename T>
+static int lincomb_impl(int64_t n, int k, const double* coef, const T* const* x, double* y,
+                        void* stream) {
   if (n < 0 || k < 1 || k > LINCOMB_MAX || !coef || !x || !y) return NL_EARG;
   if (n == 0) return NL_OK;
-  LincombArgs g = {};
+  LincombArgs<T> g = {};
   for (int a = 0; a < k; ++a) {
     if (!x[a]) return NL_EARG;
     g.x[a] = x[a];
@@ -1518,7 +1558,7 @@ int nl_lincomb(int64_t n, int k, const double* coef, const double* const* x, dou
   hipStream_t s = (hipStream_t)stream;
   switch (k) {
 #define NL_LINCOMB_CASE(K) \
-    case K: hipLaunchKernelGGL((lincomb_kernel<K>), grid, block, 0, s, n, g, y); break;
+    case K: hipLaunchKernelGGL((lincomb_kernel<K, T>), grid, block, 0, s, n, g, y); break;
     NL_LINCOMB_CASE(1) NL_LINCOMB_CASE(2) NL_LINCOMB_CASE(3) NL_LINCOMB_CASE(4)
     NL_LINCOMB_CASE(5) NL_LINCOMB_CASE(6) NL_LINCOMB_CASE(7) NL_LINCOMB_CASE(8)
     NL_LINCOMB_CASE(9) NL_LINCOMB_CASE(10) NL_LINCOMB_CASE(11) NL_LINCOMB_CASE(12)
@@ -1527,6 +1567,17 @@ int nl_lincomb(int64_t n, int k, const double* coef, const double* const* x, dou
   }
   NL_LAUNCH_CHECK();
   return NL_OK;
+}
+}  // extern "C++"
+
+int nl_lincomb(int64_t n, int k, const double* coef, const double* const* x, double* y,
+               void* stream) {
+  return lincomb_impl(n, k, coef, x, y, stream);
+}
+
+int nl_lincomb_f32(int64_t n, int k, const double* coef, const float* const* x, double* y,
+                   void* stream) {
+  return lincomb_impl(n, k, coef, x, y, stream);
 }
 
 }  // extern "C"

@@ -236,15 +236,24 @@ def t1_part(ham, t1, t2, B, prec=DEFAULT_PREC):
 
 SPLIT = (2, 0b10)       # [hi | lo] rows of a split operand image
 
+# Fixed scale of the shared FP16 split images of the amplitudes (t2, tau): 2^12 keeps
+# |t| < 16 inside FP16's range and the elements down to ~1e-6 (far below the ones that set the
+# sums) in normal numbers; a power of two, divided out exactly through alpha.  Larger amplitudes
+# overflow to inf and the solver stops with its NaN report.
+AMP_SCALE = {ops.BF16: 1.0, ops.FP16: 2.0 ** 12}
+
 
 def _fast_split(ham, prec):
-    """Every family on BF16 split operands (the "bf16x3" precision form): t2's and tau's split
+    """Every family on split operands of one type ("bf16x3" or "fp16x3"): t2's and tau's split
     images are made once per iteration (t2's by the previous step's assembly) and shared by the
-    products instead of one conversion per product."""
+    products instead of one conversion per product.  Returns the 16-bit type or None."""
     p, h = ham.p, ham.h
-    return (all(prec.get(k) == "bf16x3" for k in ("ring", "hh", "dress", "ladder"))
-            and not any(k in prec for k in ("ring_W", "ring_S", "hh_W", "hh_S"))
-            and p * p * h * h * 2 < 2 ** 31 - 1)
+    kinds = {prec.get(k) for k in ("ring", "hh", "dress", "ladder")}
+    if (len(kinds) != 1 or kinds.pop() not in ("bf16x3", "fp16x3")
+            or any(k in prec for k in ("ring_W", "ring_S", "hh_W", "hh_S"))
+            or p * p * h * h * 2 >= 2 ** 31 - 1):
+        return None
+    return ops.lp_kind(prec["ring"])[0]
 
 
 def _lp_tau(ham, prec):
@@ -272,16 +281,17 @@ def t2_part(ham, t1, t2, B, prec=DEFAULT_PREC):
     # and, for the Whh product (summed over c,d), as [ij][cd] rows; the FP64 tau is not written
     tau = B.tau
     split = _fast_split(ham, prec)
-    lp_tau = _lp_tau(ham, prec) or split
+    lp_tau = _lp_tau(ham, prec) or split is not None
     tau16 = None
-    if split:
+    if split is not None:
         # tau only as its split image: [ab][hi(ij) | lo(ij)] rows, and [ij][hi(cd) | lo(cd)]
         # rows (the Whh product sums over c, d) by two 16-bit transposes
         n, hh, pp = p * p * h * h, h * h, p * p
         tau16 = ops.WS.get("tau_split", 2 * n, dtype=torch.int16, device=t2.device)
         tauT = ops.WS.get("tau_splitT", 2 * n, dtype=torch.int16, device=t2.device)
         st = ops._stream()
-        _lib.call("nl_tau_split", p, h, ops._p(t1), ops._p(t2), None, ops._p(tau16), st)
+        _lib.call("nl_tau_split", p, h, ops._p(t1), ops._p(t2), None, ops._p(tau16), split,
+                  ctypes.c_double(AMP_SCALE[split]), st)
         for q in range(2):
             _lib.call("nl_transpose16_into", 1, pp, hh, ops._off16(tau16, q * hh), 2 * hh, 0,
                       ops._off16(tauT, q * pp), 2 * pp, 0, pp, st)
@@ -312,9 +322,11 @@ def t2_part(ham, t1, t2, B, prec=DEFAULT_PREC):
     permute_into(B.zz.permute(0, 1, 3, 2), B.Whh, -1.0, 1.0)
     Sab = B.Sab
     with ops.lp_reuse(tau, absent=True) if lp_tau else contextlib.nullcontext():
-        if split:
-            ops.lp_provide(tau, "cdij", ["i", "j"], ["c", "d"], tauT, 2 * p * p, parts=SPLIT)
-            ops.lp_provide(tau, "abkl", ["a", "b"], ["k", "l"], tau16, 2 * h * h, parts=SPLIT)
+        if split is not None:
+            sc = AMP_SCALE[split]
+            ops.lp_provide(tau, "cdij", ["i", "j"], ["c", "d"], tauT, 2 * p * p, split, SPLIT, sc)
+            ops.lp_provide(tau, "abkl", ["a", "b"], ["k", "l"], tau16, 2 * h * h, split, SPLIT,
+                           sc)
         elif lp_tau:
             ops.lp_provide(tau, "cdij", ["i", "j"], ["c", "d"], tauT, ldT)
             ops.lp_provide(tau, "abkl", ["a", "b"], ["k", "l"], tau16, h * h)
@@ -426,8 +438,9 @@ def _block_gemm(pb, src, out, alpha, ncol, tag, precision="fp64", src16=None):
     ev = ops._timed(tag)
     if ev:
         ev[0].record()
-    # split operands with src16 given: src16 rows are [hi(ncol) | lo(ncol)] of src
-    split16 = precision == "bf16x3" and src16 is not None
+    # split operands with src16 given: src16 rows are [hi(ncol) | lo(ncol)] of src * its
+    # fixed scale (AMP_SCALE)
+    split16 = precision in ("bf16x3", "fp16x3") and src16 is not None
     if precision != "bf16" and not split16:
         G = ops.WS.get(tag + "_G", pb.nb * pb.kb * ncol, device=src.device)
         R = ops.WS.get(tag + "_R", pb.nb * pb.mb * ncol, device=src.device)
@@ -453,7 +466,7 @@ def _block_gemm(pb, src, out, alpha, ncol, tag, precision="fp64", src16=None):
             tgt = ctypes.c_float(ops.lp_target(dt, True))
             if split:
                 _lib.call("nl_convert_split", dt, pb.nb, pb.mb, 1, pb.kb, s4, ops._p(pb.V),
-                          ops._p(Vb), ldk, kpad, 2, 0, ops._p(iv), tgt, st)
+                          ops._p(Vb), ldk, kpad, 2, 0, ops._p(iv), tgt, ctypes.c_double(1.0), st)
             else:
                 _lib.call("nl_convert_rows", dt, pb.nb, pb.mb, 1, pb.kb, s4, ops._p(pb.V),
                           ops._p(Vb), ldk, ops._p(iv), tgt, st)
@@ -477,7 +490,8 @@ def _block_gemm(pb, src, out, alpha, ncol, tag, precision="fp64", src16=None):
             else:
                 _lib.call("nl_gather_convert", pb.nb, pb.kb, ncol, ops._p(pb.cols), ops._p(src),
                           ncol, ops._p(Gt), ldk, st)
-            _lib.call("nl_gemm_lp_rows", dt, pb.mb, ncol, ldk, ctypes.c_double(alpha),
+            a_eff = alpha / AMP_SCALE[dt] if split16 else alpha
+            _lib.call("nl_gemm_lp_rows", dt, pb.mb, ncol, ldk, ctypes.c_double(a_eff),
                       ops._p(Vb), ldk, pb.mb * ldk, ops._p(iv), pb.mb, ops._p(Gt), ldk,
                       ncol * ldk, None, 0, ops._p(out), ncol, ops._p(pb.rows), pb.nb, st)
             if ev:
@@ -488,7 +502,8 @@ def _block_gemm(pb, src, out, alpha, ncol, tag, precision="fp64", src16=None):
         s4 = (ctypes.c_int64 * 4)(pb.kb * ncol, 1, 0, ncol)
         if split:
             _lib.call("nl_convert_split", dt, pb.nb, ncol, 1, pb.kb, s4, ops._p(G), ops._p(Gt),
-                      ldk, kpad, 2, 0b10, ops._p(ig), ctypes.c_float(target), st)
+                      ldk, kpad, 2, 0b10, ops._p(ig), ctypes.c_float(target),
+                      ctypes.c_double(1.0), st)
         else:
             _lib.call("nl_convert_rows", dt, pb.nb, ncol, 1, pb.kb, s4, ops._p(G), ops._p(Gt),
                       ldk, ops._p(ig), ctypes.c_float(target), st)
@@ -513,7 +528,8 @@ class T2Image:
         self.bufs = [None, None]
         self.cur = 0
         self.src = None
-        self.kind = 0           # 0: BF16 image, 1: [hi | lo] split image (nl_assemble_t2_ext)
+        self.kind = 0           # 0: BF16 image, 1 / 2: [hi | lo] BF16 / FP16 split image
+                                # (nl_assemble_t2_ext's image kinds)
 
     def buffer(self, k, n, dev):
         b = self.bufs[k]
@@ -534,8 +550,9 @@ def update(ham, t1, t2, B, t1n, t2n, mixing, prec=DEFAULT_PREC, H2out=None, img=
     with ops.lp_reuse(t2):
         B.t2_16 = None
         try:
-            if _fast_split(ham, prec):
-                _t2_split_images(ham, t2, img)
+            dt = _fast_split(ham, prec)
+            if dt is not None:
+                _t2_split_images(ham, t2, dt, img)
             else:
                 _t2_images(ham, t2, B, prec, img)
             t1_part(ham, t1, t2, B, prec)
@@ -596,7 +613,7 @@ def _t2_images(ham, t2, B, prec, img=None):
         ops.lp_provide(t2, "acik", ["a", "i"], ["c", "k"], T3, ph)
 
 
-def _t2_split_images(ham, t2, img=None):
+def _t2_split_images(ham, t2, dt, img=None):
     """Register the [hi | lo] split images of t2 that the bf16x3 products read: the rows
     [a][(d,k,l)] (A19; written by the previous step's assembly, or split here from FP64), and from
     them by 16-bit permutes [(a,i)][(c,k)] (both ring products), [(a,j)][(d,k)] (Y) and
@@ -604,32 +621,33 @@ def _t2_split_images(ham, t2, img=None):
     p, h = ham.p, ham.h
     n, row, ph = p * p * h * h, p * h * h, p * h
     dev, st = t2.device, ops._stream()
-    if img is not None and img.holds(t2, 1):
+    kind, sc = 1 + dt, AMP_SCALE[dt]
+    if img is not None and img.holds(t2, kind):
         N = img.buffer(img.cur, 2 * n, dev)
     else:
         N = ops.WS.get("t2_split", 2 * n, dtype=torch.int16, device=dev)
         s4 = (ctypes.c_int64 * 4)(0, row, 0, 1)
-        _lib.call("nl_convert_split", ops.BF16, 1, p, 1, row, s4, ops._p(t2), ops._p(N), 2 * row,
-                  row, 2, 0b10, None, ctypes.c_float(0.0), st)
-    ops.lp_provide(t2, "adkl", ["a"], ["d", "k", "l"], N, 2 * row, parts=SPLIT)
+        _lib.call("nl_convert_split", dt, 1, p, 1, row, s4, ops._p(t2), ops._p(N), 2 * row,
+                  row, 2, 0b10, None, ctypes.c_float(0.0), ctypes.c_double(sc), st)
+    ops.lp_provide(t2, "adkl", ["a"], ["d", "k", "l"], N, 2 * row, dt, SPLIT, sc)
     # ring operand rows (a,i), per part the columns (c,k): out (a, i, part, c, k)
     R = ops.WS.get("t2_split_ring", 2 * n, dtype=torch.int16, device=dev)
     shp = (ctypes.c_int64 * 5)(p, h, 2, p, h)
     istr = (ctypes.c_int64 * 5)(2 * row, h, row, h * h, 1)
     _lib.call("nl_permute16", 5, shp, istr, ops._p(N), ops._p(R), st)
-    ops.lp_provide(t2, "acik", ["a", "i"], ["c", "k"], R, 2 * ph, parts=SPLIT)
+    ops.lp_provide(t2, "acik", ["a", "i"], ["c", "k"], R, 2 * ph, dt, SPLIT, sc)
     # Y operand rows (a,j), columns (d,k): per a and part, the (d,k) x j block transposed
     Y = ops.WS.get("t2_split_aj", 2 * n, dtype=torch.int16, device=dev)
     for q in range(2):
         _lib.call("nl_transpose16_into", p, ph, h, ops._off16(N, q * row), h, 2 * row,
                   ops._off16(Y, q * ph), 2 * ph, 2 * ph * h, ph, st)
-    ops.lp_provide(t2, "adkj", ["a", "j"], ["d", "k"], Y, 2 * ph, parts=SPLIT)
+    ops.lp_provide(t2, "adkj", ["a", "j"], ["d", "k"], Y, 2 * ph, dt, SPLIT, sc)
     # Sab operand rows (b,i,j), columns d: the d x (b,i,j) matrix transposed, per part
     S = ops.WS.get("t2_split_T", 2 * n, dtype=torch.int16, device=dev)
     for q in range(2):
         _lib.call("nl_transpose16_into", 1, p, row, ops._off16(N, q * row), 2 * row, 0,
                   ops._off16(S, q * p), 2 * p, 0, p, st)
-    ops.lp_provide(t2, "dbij", ["b", "i", "j"], ["d"], S, 2 * p, parts=SPLIT)
+    ops.lp_provide(t2, "dbij", ["b", "i", "j"], ["d"], S, 2 * p, dt, SPLIT, sc)
 
 
 def assemble(ham, t2, B, t1n, t2n, mixing, H2out=None, img=None, prec=None):
@@ -642,11 +660,12 @@ def assemble(ham, t2, B, t1n, t2n, mixing, H2out=None, img=None, prec=None):
             ops._p(B.X2hh), ops._p(getattr(B, "dpp", None)), None, ops._p(t2),
             ops._p(ham.v_pphh), ops._p(t1n), ctypes.c_double(mixing), ops._p(t2n),
             ops._p(H2out), ops._p(B.part))
-    buf, kind = None, 0
-    if img is not None and prec is not None and t2n is not None and _fast_split(ham, prec):
+    buf, kind, sc = None, 0, 1.0
+    dt = _fast_split(ham, prec) if prec is not None else None
+    if img is not None and t2n is not None and dt is not None:
         nxt = 1 - img.cur
-        buf, kind = img.buffer(nxt, 2 * p * p * h * h, t2n.device), 1
-        img.cur, img.src, img.kind = nxt, t2n, 1
+        buf, kind, sc = img.buffer(nxt, 2 * p * p * h * h, t2n.device), 1 + dt, AMP_SCALE[dt]
+        img.cur, img.src, img.kind = nxt, t2n, kind
     elif (img is not None and prec is not None and t2n is not None
             and _images_wanted(ham, prec)):
         nxt = 1 - img.cur
@@ -654,7 +673,12 @@ def assemble(ham, t2, B, t1n, t2n, mixing, H2out=None, img=None, prec=None):
         img.cur, img.src, img.kind = nxt, t2n, 0
     elif img is not None:
         img.src = None
-    _lib.call("nl_assemble_t2_ext", *args, ops._p(buf), kind, st)
+    ev = ops._timed("assemble")
+    if ev:
+        ev[0].record()
+    _lib.call("nl_assemble_t2_ext", *args, ops._p(buf), kind, ctypes.c_double(sc), st)
+    if ev:
+        ev[1].record()
     nparts = int(_lib.load().nl_partials(p * p * h * h))
     _lib.call("nl_finalize", nparts, ops._p(B.part), p * h, ops._p(ham.f_ph), ops._p(t1n),
               ops._p(B.E), st)

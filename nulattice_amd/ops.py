@@ -135,15 +135,17 @@ def _ptrs(ts):
 
 
 def gram(vs):
-    """Host FP64 matrix G[a, b] = vs[a] . vs[b] of contiguous device vectors of one size: one
-    nl_gram_part pass over up to 12 vectors; larger sets run as pairs of groups of 6 (each pair
-    one pass over its 12 vectors).  Fixed summation order: bitwise reproducible."""
+    """Host FP64 matrix G[a, b] = vs[a] . vs[b] of contiguous device vectors of one size and
+    dtype (float64, or float32 for a history kept in FP32; FP64 products and sums either way):
+    one nl_gram_part pass over up to 12 vectors; larger sets run as pairs of groups of 6 (each
+    pair one pass over its 12 vectors).  Fixed summation order: bitwise reproducible."""
     k = len(vs)
     if k == 0:
         return np.zeros((0, 0))
     n = vs[0].numel()
+    f32 = vs[0].dtype == torch.float32
     for v in vs:
-        _check(v, "v")
+        _check(v, "v", torch.float32 if f32 else torch.float64)
         if v.numel() != n or not v.is_contiguous():
             raise ValueError("gram: vectors must be contiguous and of one size")
     G = np.empty((k, k))
@@ -158,7 +160,8 @@ def gram(vs):
         npair = m * (m + 1) // 2
         part = torch.empty(npair * nparts, dtype=torch.float64, device=vs[0].device)
         out = torch.empty(npair, dtype=torch.float64, device=vs[0].device)
-        _lib.call("nl_gram_part", n, m, _ptrs([vs[i] for i in idx]), _p(part), _stream())
+        _lib.call("nl_gram_part_f32" if f32 else "nl_gram_part", n, m,
+                  _ptrs([vs[i] for i in idx]), _p(part), _stream())
         _lib.call("nl_gram_finalize", nparts, npair, _p(part), _p(out), _stream())
         o = out.cpu().numpy()
         q = 0
@@ -170,13 +173,31 @@ def gram(vs):
 
 
 def lincomb(coef, vs, out):
-    """out = sum_k coef[k] * vs[k] (contiguous, same numel): one nl_lincomb pass over up to 16
-    vectors; longer sums continue in chunks that add to out."""
+    """out (float64) = sum_k coef[k] * vs[k] (contiguous, same numel; vs float64, or float32
+    for a history kept in FP32): one nl_lincomb pass over up to 16 vectors; longer sums continue
+    in chunks that add to out."""
     _check(out, "out")
     vs = list(vs)
     if len(coef) != len(vs) or not vs:
         raise ValueError("lincomb: need one coefficient per vector")
     n = out.numel()
+    if vs[0].dtype == torch.float32:
+        for v in vs:
+            _check(v, "v", torch.float32)
+            if v.numel() != n or not v.is_contiguous():
+                raise ValueError("lincomb: vectors must be contiguous and of one size")
+        tmp = None
+        for k0 in range(0, len(vs), LINCOMB_MAX):
+            xs = vs[k0:k0 + LINCOMB_MAX]
+            cs = [float(c) for c in coef[k0:k0 + LINCOMB_MAX]]
+            dst = out if k0 == 0 else tmp
+            _lib.call("nl_lincomb_f32", n, len(xs), (ctypes.c_double * len(cs))(*cs), _ptrs(xs),
+                      _p(dst), _stream())
+            if k0:
+                axpby(1.0, tmp, 1.0, out)
+            elif len(vs) > LINCOMB_MAX:
+                tmp = torch.empty_like(out)
+        return out
     for v in vs + [out]:
         _check(v, "v")
         if v.numel() != n or not v.is_contiguous():
@@ -490,17 +511,18 @@ class lp_reuse:
         return False
 
 
-def lp_provide(t, labels, rows, cols, buf, ldk, dtype=BF16, parts=(1, 0)):
-    """Register ``buf`` ([R][ldk] 16-bit rows, no scales; with parts=(2, 0b10) each row is
-    [hi | lo] of the split) as the conversion of t seen as rows x cols, for products inside an
-    ``lp_reuse(t)`` block: they use it without reading t, so t's FP64 values need not exist."""
+def lp_provide(t, labels, rows, cols, buf, ldk, dtype=BF16, parts=(1, 0), scale=1.0):
+    """Register ``buf`` ([R][ldk] 16-bit rows of t * scale, no row scales; with parts=(2, 0b10)
+    each row is [hi | lo] of the split) as the conversion of t seen as rows x cols, for products
+    inside an ``lp_reuse(t)`` block: they use it without reading t, so t's FP64 values need not
+    exist.  ``scale`` (a power of two) is divided out of the products through alpha."""
     v = _view4(t, labels, rows, cols)
     if v is None or t.data_ptr() not in CONST_PTRS:
         raise _lib.NLError("lp_provide: the view must be two-run and t under lp_reuse")
     key = (t.data_ptr(), v[0], v[1], v[2], v[3], tuple(v[4]), dtype)
     if parts != (1, 0):
         key = key + (parts,)
-    _LP_CACHE[key] = (buf, None, ldk)
+    _LP_CACHE[key] = (buf, None, ldk, 1.0 / scale)
 
 
 def lp_target(dtype, exact):
@@ -549,7 +571,7 @@ def to_lp_rows(t, labels, rows, cols, dtype, tag, parts=(1, 0)):
             key = key + (parts,)
         hit = _LP_CACHE.get(key)
         if hit is not None:
-            return hit
+            return hit          # (buf, row inverse scales or None, ldk, tensor inverse scale)
     if t.data_ptr() in ABSENT_PTRS:
         raise _lib.NLError("to_lp_rows: no 16-bit image registered for this view of a tensor "
                            "whose FP64 values are not formed")
@@ -575,10 +597,11 @@ def to_lp_rows(t, labels, rows, cols, dtype, tag, parts=(1, 0)):
                   _p(buf), ldk, _p(isc), ctypes.c_float(target), _stream())
     else:
         _lib.call("nl_convert_split", dtype, R1, R2, C1, C2, s4, ctypes.c_void_p(t.data_ptr()),
-                  _p(buf), ldk, kpad, nblk, mask, _p(isc), ctypes.c_float(target), _stream())
+                  _p(buf), ldk, kpad, nblk, mask, _p(isc), ctypes.c_float(target),
+                  ctypes.c_double(1.0), _stream())
     if key is not None:
-        _LP_CACHE[key] = (buf, isc, ldk)
-    return buf, isc, ldk
+        _LP_CACHE[key] = (buf, isc, ldk, 1.0)
+    return buf, isc, ldk, 1.0
 
 
 def _sizes(t, labels):
@@ -604,8 +627,9 @@ def _contract_lp(A, la, B, lb, out, lo, Mi, Ni, Ki, M, N, K, alpha, beta, precis
         ex = lambda t, lab, rows: (t.data_ptr() in EXACT_PTRS
                                    and _view4(t, lab, rows, Ki) is not None)
         pa, pb = split_parts(ex(A, la, Mi) or split == "x2b", ex(B, lb, Ni) or split == "x2a")
-    Ab, ia, lda = to_lp_rows(A, la, Mi, Ki, dt, "lpA", pa)
-    Bb, ib, ldb = to_lp_rows(B, lb, Ni, Ki, dt, "lpB", pb)
+    Ab, ia, lda, sa = to_lp_rows(A, la, Mi, Ki, dt, "lpA", pa)
+    Bb, ib, ldb, sb = to_lp_rows(B, lb, Ni, Ki, dt, "lpB", pb)
+    alpha = alpha * sa * sb            # tensor scales of provided images (powers of two)
     # both operands rounded: [hi | lo] rows of each, lo parts passed to the split GEMM
     both = pa == (2, 0b10) and pb == (2, 0b10)
     Kg = lda // 2 if both else lda

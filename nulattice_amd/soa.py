@@ -27,13 +27,18 @@ def ccsd_solver(fock_mats, two_body_int, t1initial=None, eps=1e-8, maxSteps=1000
                 dtype=torch.float64, **options):
     """CCSD with the SoA driver's rules (soa/coupled_cluster.py:440-575).  Returns (E_corr, t1,
     t2), also after a divergence or NaN.  ``device`` must be a HIP device (None: the current
-    one); only ``dtype=torch.float64`` is accepted (``precision="bf16"`` selects the
-    low-precision form, refined in FP64).  Other keyword options are those of
-    ``ccm.ccsd_solver`` (history, info, precision, hamiltonian, ...), except ``diis`` and
-    ``sectors``, which the SoA rules fix."""
-    if dtype != torch.float64:
-        raise ValueError("only dtype=torch.float64 is supported (precision='bf16' or 'fp16' "
-                         "selects the low-precision form)")
+    one).  ``dtype`` is the storage type, as in the reference (soa/coupled_cluster.py:453-460):
+    with torch.float32 the DIIS history (the reference's "major memory hog", :521) is kept in
+    FP32 and t1, t2 are returned as float32, and the contractions default to the split-operand
+    precision form (precision="fp16x3", FP32-class accuracy, no FP64 refinement unless
+    refine=True is passed); energy, denominators, the update and the DIIS solve stay FP64.
+    Other keyword options are those of ``ccm.ccsd_solver`` (history, info, precision,
+    hamiltonian, ...), except ``diis`` and ``sectors``, which the SoA rules fix."""
+    if dtype not in (torch.float64, torch.float32):
+        raise ValueError("dtype must be torch.float64 or torch.float32")
+    if dtype == torch.float32:
+        options.setdefault("precision", "fp16x3")
+        options.setdefault("refine", False)
     if device is not None and torch.device(device).type != "cuda":
         raise _lib.NLError("soa.ccsd_solver: a HIP device is required (no CPU fallback)")
     for k in ("diis", "sectors"):
@@ -41,4 +46,4 @@ def ccsd_solver(fock_mats, two_body_int, t1initial=None, eps=1e-8, maxSteps=1000
             raise TypeError(f"soa.ccsd_solver fixes {k!r}")
     return ccm.ccsd_solver(fock_mats, two_body_int, t1initial, eps, maxSteps, max_diis, delta,
                            mixing, verbose, sparse, ccs, device=device, diis="soa",
-                           _soa_rules=True, **options)
+                           storage=dtype, _soa_rules=True, **options)

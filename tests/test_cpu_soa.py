@@ -8,7 +8,7 @@ from nulattice_amd import _lib, soa
 
 def test_soa_driver_rejects_what_it_cannot_run():
     with pytest.raises(ValueError):
-        soa.ccsd_solver(None, None, dtype=torch.float32)
+        soa.ccsd_solver(None, None, dtype=torch.float16)
     with pytest.raises(_lib.NLError):
         soa.ccsd_solver(None, None, device="cpu")
     with pytest.raises(TypeError):

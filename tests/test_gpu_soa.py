@@ -119,3 +119,28 @@ def test_soa_diis_trajectory_matches_oracle():
     hj = []
     cc.ccsd_solver(fock, v, eps=1e-12, maxSteps=10, max_diis=0, history=hj)
     assert abs(hj[8] - hr[8]) > 1e-8 * abs(hr[8])
+
+
+def test_soa_float32_storage():
+    """dtype=torch.float32 (soa/coupled_cluster.py:453-460): the DIIS history is kept in FP32 and
+    the amplitudes come back as float32.  With the FP64 contractions (precision="fp64") the
+    trajectory is the FP64-storage run's up to the first extrapolation and follows it to the FP32
+    rounding of the history, amplified by the DIIS solve, afterwards (1e-5); below |dE|/|E| =
+    1e-7 the FP32 history stops extrapolating and the run converges to the same FP64 fixed point
+    (1e-9); the default precision (the split form) converges to the FP64 energy within 1e-6."""
+    import torch
+    fock, v = _pw(3, 28)
+    h64, h32 = [], []
+    e64, _, _ = soa.ccsd_solver(fock, v, eps=1e-11, max_diis=6, history=h64)
+    e32, t1, t2 = soa.ccsd_solver(fock, v, eps=1e-11, max_diis=6, history=h32,
+                                  dtype=torch.float32, precision="fp64")
+    assert t1.dtype == torch.float32 and t2.dtype == torch.float32
+    n = min(len(h64), len(h32))
+    rel = np.abs(np.array(h32[:n]) - np.array(h64[:n])) / np.abs(np.array(h64[:n]))
+    assert rel[:7].max() < 1e-13
+    assert rel.max() < 1e-5
+    assert abs(e32 - e64) < 1e-9 * abs(e64)
+    info = {}
+    es, _, t2s = soa.ccsd_solver(fock, v, eps=1e-10, max_diis=6, dtype=torch.float32, info=info)
+    assert t2s.dtype == torch.float32 and info["steps_fp64"] == 0
+    assert abs(es - e64) < 1e-6 * abs(e64)

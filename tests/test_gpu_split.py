@@ -41,8 +41,13 @@ def bf16_value(bits):
     return (bits.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
 
 
-def split_ref(x):
-    """hi = BF16(FP32(x)), lo = BF16(FP32(x - hi)) as the kernels define the split."""
+def split_ref(x, fp16=False):
+    """hi = 16-bit(FP32(x)), lo = 16-bit(FP32(x - hi)) as the kernels define the split (BF16, or
+    FP16 with numpy's round-to-nearest-even FP32 -> FP16)."""
+    if fp16:
+        hi = x.astype(np.float32).astype(np.float16)
+        lo = (x - hi.astype(np.float64)).astype(np.float32).astype(np.float16)
+        return hi.view(np.uint16), lo.view(np.uint16)
     hi = bf16_bits(x.astype(np.float32))
     lo = bf16_bits((x - bf16_value(hi)).astype(np.float32))
     return hi, lo
@@ -50,6 +55,26 @@ def split_ref(x):
 
 def u16(t):
     return t.detach().cpu().numpy().view(np.uint16)
+
+
+def check_split(hi_k, lo_k, y, fp16):
+    """The kernel's split (hi_k, lo_k bit patterns) of y.  BF16: bit for bit split_ref.  FP16: hi
+    is FP16 nearest-even of y rounded via FP32 or directly (the two differ only where the FP32
+    value ties between two FP16 values, about once in 8192 elements; both are nearest
+    roundings), and lo is the FP16 of y - hi for the kernel's own hi, rounded the same two
+    ways."""
+    if not fp16:
+        hi, lo = split_ref(y)
+        np.testing.assert_array_equal(hi_k, hi)
+        np.testing.assert_array_equal(lo_k, lo)
+        return
+    hi2, _ = split_ref(y, fp16=True)
+    hi1 = y.astype(np.float16).view(np.uint16)
+    assert np.all((hi_k == hi1) | (hi_k == hi2))
+    hk = np.ascontiguousarray(hi_k).view(np.float16).astype(np.float64)
+    lo2 = (y - hk).astype(np.float32).astype(np.float16).view(np.uint16)
+    lo1 = (y - hk).astype(np.float16).view(np.uint16)
+    assert np.all((lo_k == lo1) | (lo_k == lo2))
 
 
 def test_bf16_reference_rounding():
@@ -73,7 +98,7 @@ def test_convert_split_blocks(transposed):
     strides = (0, 1, 0, R) if transposed else (0, C, 0, 1)
     out = torch.full((R * 3 * kpad,), -1, dtype=torch.int16, device="cuda")
     rc = L_().nl_convert_split(0, 1, R, 1, C, i64(*strides), P(src), P(out), 3 * kpad, kpad, 3,
-                               0b010, None, ctypes.c_float(0.0), S())
+                               0b010, None, ctypes.c_float(0.0), ctypes.c_double(1.0), S())
     assert rc == 0
     torch.cuda.synchronize()
     o = u16(out).reshape(R, 3, kpad)
@@ -84,6 +109,26 @@ def test_convert_split_blocks(transposed):
     assert not o[:, :, C:].any()
     rec = bf16_value(hi) + bf16_value(lo)
     assert np.max(np.abs(rec - x) / np.abs(x)) < 2.0 ** -16
+
+
+def test_convert_split_fp16_fixed_scale():
+    """FP16 split of x * 2^12 (the amplitude images' fixed scale): bit for bit, and hi + lo
+    reproduces the scaled value to 2^-21 for elements whose lo part is a normal FP16 number."""
+    rng = np.random.default_rng(4)
+    R, C = 64, 96
+    x = rng.standard_normal((R, C)) * 10.0 ** rng.integers(-4, 0, (R, C))
+    src = torch.tensor(x, device="cuda")
+    out = torch.empty(R * 2 * C, dtype=torch.int16, device="cuda")
+    assert L_().nl_convert_split(1, 1, R, 1, C, i64(0, C, 0, 1), P(src), P(out), 2 * C, C, 2,
+                                 0b10, None, ctypes.c_float(0.0), ctypes.c_double(4096.0),
+                                 S()) == 0
+    torch.cuda.synchronize()
+    o = u16(out).reshape(R, 2, C)
+    check_split(o[:, 0], o[:, 1], x * 4096.0, True)
+    hi, lo = o[:, 0].copy(), o[:, 1].copy()
+    rec = hi.view(np.float16).astype(np.float64) + lo.view(np.float16).astype(np.float64)
+    big = np.abs(x) > 1e-2
+    assert np.max(np.abs(rec - 4096.0 * x)[big] / (4096.0 * np.abs(x[big]))) < 2.0 ** -21
 
 
 def test_convert_exact_scales_make_contact_blocks_exact():
@@ -111,10 +156,15 @@ def _bf16(a):
     return torch.tensor(a, dtype=torch.float32, device="cuda").to(torch.bfloat16)
 
 
+def _fp16(a):
+    return torch.tensor(a, dtype=torch.float32, device="cuda").to(torch.float16)
+
+
 @pytest.mark.parametrize("M,N,K", [(300, 513, 96), (4000, 4200, 600), (4100, 4000, 472),
                                    (4096, 4097, 1024)])
 @pytest.mark.parametrize("parts", ["2x2", "1x2", "2x1"])
-def test_split_gemm_exact(M, N, K, parts):
+@pytest.mark.parametrize("dtype", [0, 1])
+def test_split_gemm_exact(M, N, K, parts, dtype):
     """nl_gemm_lp_split = alpha (Ah Bh^T + Ah Bl^T + Al Bh^T) + beta D on integer operands
     (exact in BF16 and in FP32 sums): the 256x256 multi-stage kernel (large shapes, K not a
     multiple of 32 in the two-stage form) and the fallback of three products (small shape)."""
@@ -124,10 +174,11 @@ def test_split_gemm_exact(M, N, K, parts):
     use_al, use_bl = parts[0] == "2", parts[2] == "2"
     ref = Ah @ Bh.T + (Ah @ Bl.T if use_bl else 0) + (Al @ Bh.T if use_al else 0)
     D = rng.integers(-5, 6, (M, N)).astype(np.float64)
-    ta, tal, tb, tbl = _bf16(Ah), _bf16(Al), _bf16(Bh), _bf16(Bl)
+    cv = _bf16 if dtype == 0 else _fp16
+    ta, tal, tb, tbl = cv(Ah), cv(Al), cv(Bh), cv(Bl)
     C = torch.full((M, N), float("nan"), dtype=torch.float64, device="cuda")
     Dd = torch.tensor(D, device="cuda")
-    rc = L_().nl_gemm_lp_split(0, M, N, K, ctypes.c_double(0.5), P(ta), P(tal if use_al else None),
+    rc = L_().nl_gemm_lp_split(dtype, M, N, K, ctypes.c_double(0.5), P(ta), P(tal if use_al else None),
                                K, 0, None, 0, P(tb), P(tbl if use_bl else None), K, 0, None, 0,
                                ctypes.c_double(2.0), P(Dd), P(C), N, 0, 1, 1, None, S())
     assert rc == 0
@@ -154,8 +205,10 @@ def test_split_gemm_strided_output_and_split_k():
     np.testing.assert_array_equal(out.cpu().numpy().transpose(0, 2, 1, 3), ref)
 
 
-def test_tau_split():
-    """nl_tau_split: FP64 tau = t2 + t1 t1 - t1 t1 and its [hi | lo] rows bit for bit."""
+@pytest.mark.parametrize("dtype,scale", [(0, 1.0), (1, 4096.0)])
+def test_tau_split(dtype, scale):
+    """nl_tau_split: FP64 tau = t2 + t1 t1 - t1 t1 and the [hi | lo] rows of tau * scale bit for
+    bit (BF16 unscaled, FP16 at the amplitude images' scale)."""
     rng = np.random.default_rng(7)
     p, h = 24, 6
     t1 = rng.standard_normal((p, h)) * 0.1
@@ -164,24 +217,24 @@ def test_tau_split():
     g1, g2 = torch.tensor(t1, device="cuda"), torch.tensor(t2, device="cuda")
     tau_d = torch.empty_like(g2)
     img = torch.empty(2 * p * p * h * h, dtype=torch.int16, device="cuda")
-    assert L_().nl_tau_split(p, h, P(g1), P(g2), P(tau_d), P(img), S()) == 0
+    assert L_().nl_tau_split(p, h, P(g1), P(g2), P(tau_d), P(img), dtype, ctypes.c_double(scale),
+                             S()) == 0
     torch.cuda.synchronize()
     np.testing.assert_allclose(tau_d.cpu().numpy(), tau, rtol=1e-14, atol=1e-17)
-    hi, lo = split_ref(tau_d.cpu().numpy().reshape(p * p, h * h))
     o = u16(img).reshape(p * p, 2, h * h)
-    np.testing.assert_array_equal(o[:, 0], hi)
-    np.testing.assert_array_equal(o[:, 1], lo)
+    check_split(o[:, 0], o[:, 1], tau_d.cpu().numpy().reshape(p * p, h * h) * scale, dtype == 1)
 
 
-def test_assemble_ext_diagonals_and_split_image():
+@pytest.mark.parametrize("kind,scale", [(1, 1.0), (2, 4096.0)])
+def test_assemble_ext_diagonals_and_split_image(kind, scale):
     """nl_assemble_t2_ext: H2 = S0 + pAB(Sab) + pIJ(Sij) + pIJ pAB(Spp) + (dpp_a + dpp_b + dhh_i +
     dhh_j) t2, the Jacobi step with the dressed denominators, and the [hi | lo] image of t2n."""
     rng = np.random.default_rng(9)
     p, h = 20, 6
-    S0, Sab, Sij, t2, v = (rng.standard_normal((p, p, h, h)) for _ in range(5))
-    Spp = rng.standard_normal((p, h, p, h))
-    Xpp = rng.standard_normal((p, p)) + np.diag(5 + rng.random(p))
-    Xhh = rng.standard_normal((h, h)) - np.diag(5 + rng.random(h))
+    S0, Sab, Sij, t2, v = (rng.standard_normal((p, p, h, h)) * 0.1 for _ in range(5))
+    Spp = rng.standard_normal((p, h, p, h)) * 0.1
+    Xpp = rng.standard_normal((p, p)) + np.diag(8 + rng.random(p))
+    Xhh = rng.standard_normal((h, h)) - np.diag(8 + rng.random(h))
     dpp, dhh = rng.standard_normal(p), rng.standard_normal(h)
     t1n = rng.standard_normal((p, h)) * 0.1
     pAB = lambda x: x - x.transpose(1, 0, 2, 3)
@@ -201,16 +254,14 @@ def test_assemble_ext_diagonals_and_split_image():
                        device="cuda")
     img = torch.empty(2 * p * p * h * h, dtype=torch.int16, device="cuda")
     rc = L_().nl_assemble_t2_ext(p, h, *[P(x) for x in g], ctypes.c_double(m), P(t2n), P(H2),
-                                 P(part), P(img), 1, S())
+                                 P(part), P(img), kind, ctypes.c_double(scale), S())
     assert rc == 0
     torch.cuda.synchronize()
     np.testing.assert_allclose(H2.cpu().numpy(), H, rtol=1e-13, atol=1e-13)
     tnd = t2n.cpu().numpy()
     np.testing.assert_allclose(tnd, tn, rtol=1e-12, atol=1e-14)
-    hi, lo = split_ref(tnd.reshape(p, -1))
     o = u16(img).reshape(p, 2, -1)
-    np.testing.assert_array_equal(o[:, 0], hi)
-    np.testing.assert_array_equal(o[:, 1], lo)
+    check_split(o[:, 0], o[:, 1], tnd.reshape(p, -1) * scale, kind == 2)
 
 
 def _steps(H, prec, n):
@@ -229,27 +280,30 @@ def _steps(H, prec, n):
     return np.array(es), t2.clone()
 
 
-def test_split_images_equal_per_product_conversions():
+@pytest.mark.parametrize("prec,tol", [("bf16x3", 1e-12), ("fp16x3", 1e-7)])
+def test_split_images_equal_per_product_conversions(prec, tol):
     """Plane-wave L=3, A=28 (p=80, h=28): the shared split images of t2 (written by the
     assembly, permuted in 16 bits) and of tau give the same steps as converting every operand
-    per product (FP64 -> split), to summation order (1e-12)."""
+    per product: BF16 to summation order (1e-12: the same roundings); FP16 to the split's
+    precision (1e-7: the shared images use the fixed scale 2^12, the per-product conversions
+    per-row scales, so the roundings differ)."""
     E0, fock, v = planewave_hamiltonian(3, 28, -4.0, 1.0)
     H = engine.Hamiltonian(fock, v, device="cuda")
-    fast = ccm._PREC["bf16x3"]
-    assert engine._fast_split(H, fast)
-    slow = dict(fast, ring_W="bf16x3")          # any sub-family key disables the shared images
-    assert not engine._fast_split(H, slow)
+    fast = ccm._PREC[prec]
+    assert engine._fast_split(H, fast) is not None
+    slow = dict(fast, ring_W=prec)               # any sub-family key disables the shared images
+    assert engine._fast_split(H, slow) is None
     ef, tf = _steps(H, fast, 4)
     es, ts = _steps(H, slow, 4)
-    np.testing.assert_allclose(ef, es, rtol=1e-12)
+    np.testing.assert_allclose(ef, es, rtol=tol)
     scale = float(ts.abs().max())
-    assert float((tf - ts).abs().max()) <= 1e-12 * scale
+    assert float((tf - ts).abs().max()) <= tol * scale
 
 
 def test_precision_form_L5_A28_meets_1e6():
-    """Headline size (L=5, A=28, C0 from seed 0, D0=1): the bf16x3 iteration converges to within
-    1e-6 relative of the FP64 energy (the symmetry-sector FP64 solve, itself equal to the dense
-    FP64 solve to 1e-12), with no FP64 refinement."""
+    """Headline size (L=5, A=28, C0 from seed 0, D0=1): the fp16x3 and bf16x3 iterations converge
+    to within 1e-6 relative of the FP64 energy (the symmetry-sector FP64 solve, itself equal to
+    the dense FP64 solve to 1e-12), with no FP64 refinement."""
     from nulattice_amd.sectors import SectorHamiltonian
     C0 = float(np.random.default_rng(0).uniform(-4.5, -3.5))
     SH = SectorHamiltonian.from_planewave(PlaneWave(5, shell_holes(5, 28), C0, 1.0))
@@ -257,13 +311,15 @@ def test_precision_form_L5_A28_meets_1e6():
     del SH
     E0, fock, v = planewave_hamiltonian(5, 28, C0, 1.0)
     H = engine.Hamiltonian(fock, v, device="cuda")
-    info = {}
-    e, _, _ = ccm.ccsd_solver(fock, v, eps=1e-10, maxSteps=120, max_diis=0, hamiltonian=H,
-                              precision="bf16x3", refine=False, info=info)
-    rel = abs(e - e64) / abs(e64)
-    print(f"L=5 A=28 seed 0: bf16x3 |dE|/|E| = {rel:.3e} after {info['steps_lowprec']} steps")
-    assert info["converged"]
-    assert rel <= 1e-6
+    for prec in ("fp16x3", "bf16x3"):
+        info = {}
+        e, _, _ = ccm.ccsd_solver(fock, v, eps=1e-10, maxSteps=120, max_diis=0, hamiltonian=H,
+                                  precision=prec, refine=False, info=info)
+        rel = abs(e - e64) / abs(e64)
+        print(f"L=5 A=28 seed 0: {prec} |dE|/|E| = {rel:.3e} after {info['steps_lowprec']} "
+              f"steps")
+        assert info["converged"]
+        assert rel <= 1e-6
     # the single-rounding forms for the record (their fixed points miss 1e-6)
     for prec in ("bf16", "fp16"):
         el, _, _ = ccm.ccsd_solver(fock, v, eps=1e-9, maxSteps=60, max_diis=0, hamiltonian=H,
