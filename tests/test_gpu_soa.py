@@ -144,3 +144,26 @@ def test_soa_float32_storage():
     es, _, t2s = soa.ccsd_solver(fock, v, eps=1e-10, max_diis=6, dtype=torch.float32, info=info)
     assert t2s.dtype == torch.float32 and info["steps_fp64"] == 0
     assert abs(es - e64) < 1e-6 * abs(e64)
+
+
+def test_combined_L6_A28_fp32_storage():
+    """The combined configuration at the largest single-GPU size: L=6, A=28 (836 particles x 28
+    holes, T2 5.5e8 amplitudes), C0 and D0 drawn from seed 0 (C0 in [-4.5, -3.5], D0 in
+    [0.5, 1.5]): the SoA driver with dtype=torch.float32 (FP32 DIIS history and amplitudes, the
+    fp16x3 precision form) converges to within 1e-6 of the FP64 symmetry-sector energy."""
+    import torch
+    from nulattice_amd.planewave import PlaneWave, shell_holes
+    from nulattice_amd.sectors import SectorHamiltonian
+    rng = np.random.default_rng(0)
+    C0, D0 = float(rng.uniform(-4.5, -3.5)), float(rng.uniform(0.5, 1.5))
+    SH = SectorHamiltonian.from_planewave(PlaneWave(6, shell_holes(6, 28), C0, D0))
+    e64, _, _ = ccm.ccsd_solver(None, None, eps=1e-12, maxSteps=400, max_diis=0, hamiltonian=SH)
+    del SH
+    _, fock, v = planewave_hamiltonian(6, 28, C0, D0)
+    info = {}
+    e32, t1, t2 = soa.ccsd_solver(fock, v, eps=1e-10, maxSteps=200, dtype=torch.float32,
+                                  info=info)
+    rel = abs(e32 - e64) / abs(e64)
+    print(f"L=6 A=28: FP32-storage fp16x3 |dE|/|E| = {rel:.2e} in {info['steps_lowprec']} steps")
+    assert t2.dtype == torch.float32 and tuple(t2.shape) == (836, 836, 28, 28)
+    assert info["converged"] and rel <= 1e-6

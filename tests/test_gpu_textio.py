@@ -30,14 +30,14 @@ def test_deuteron_trajectory_on_gpu(form):
 
 
 def test_deuteron_bf16_form_refines():
-    """The low-precision form on the same files lands on the FP64 fixed point after refinement
-    (the BF16 iteration without DIIS does not settle on this Hamiltonian; the pulsed DIIS does).  The stopping rule
-    is a relative energy change below eps, which an extrapolation can meet a few 1e-9 short of the
-    fixed point at eps = 1e-11 (the oracle's own solve stops 3e-11 away there); eps = 1e-13 puts
-    both solves within ~1e-12 of it."""
+    """The low-precision form on the same files lands on the FP64 fixed point after refinement.
+    Plain damped Jacobi in both solves: its stopping rule (relative energy change below eps)
+    leaves the solve within ~eps of the fixed point, whereas right after a pulsed-DIIS
+    extrapolation the rule can fire a few 1e-9 short of it."""
     _, fock, dense, _ = deuteron()
-    e64, _, _ = ccm.ccsd_solver(fock, dense, sparse=False, eps=1e-13, diis="pulsed")
+    e64, _, _ = ccm.ccsd_solver(fock, dense, sparse=False, eps=1e-13, max_diis=0)
     info = {}
-    e16, _, _ = ccm.ccsd_solver(fock, dense, sparse=False, eps=1e-13, diis="pulsed",
+    e16, _, _ = ccm.ccsd_solver(fock, dense, sparse=False, eps=1e-13, max_diis=0,
                                 precision="bf16", info=info)
+    assert info["steps_lowprec"] > 0 and info["steps_fp64"] > 0
     assert abs(e16 - e64) < 1e-9 * abs(e64)
