@@ -270,6 +270,16 @@ int nl_energy_part(int64_t p, int64_t h, const double* vpphh, const double* t2, 
 int nl_finalize(int64_t nparts, const double* part, int64_t nx, const double* x, const double* y,
                 double* out, void* stream);
 
+/* ccsd_energy without a workspace: out E[0] = f_ph.t1 + 1/4 v_pphh.t2 + 1/2 v_pphh t1 t1, the
+ * result of nl_energy_part + nl_finalize bit for bit, with the partial sums in a static device
+ * array of the library (4096 doubles per GPU, held until the process ends) instead of a
+ * caller-provided workspace.  No counterpart in the reference: for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  It calls
+ * no allocating routine; the array is resolved once per device on the first call.  Calls that
+ * overlap on one GPU share the array and race. */
+int nl_ccsd_energy_noalloc(int64_t p, int64_t h, const double* f_ph, const double* vpphh,
+                           const double* t2, const double* t1, double* E, void* stream);
+
 /* Partials of x.y (DIIS B-matrix, soa/coupled_cluster.py:536-543). */
 int nl_dot_part(int64_t n, const double* x, const double* y, double* part, void* stream);
 

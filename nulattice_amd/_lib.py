@@ -64,6 +64,7 @@ SIGNATURES = {
     "nl_assemble_t2_adj": (_i32, [_i64, _i64, _vp, _vp, _vp, _vp, _vp]),
     "nl_partials": (_i64, [_i64]),
     "nl_energy_part": (_i32, [_i64, _i64, _vp, _vp, _vp, _vp, _vp]),
+    "nl_ccsd_energy_noalloc": (_i32, [_i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp]),
     "nl_finalize": (_i32, [_i64, _vp, _i64, _vp, _vp, _vp, _vp]),
     "nl_dot_part": (_i32, [_i64, _vp, _vp, _vp, _vp]),
     "nl_axpby": (_i32, [_i64, _dbl, _vp, _dbl, _vp, _vp]),

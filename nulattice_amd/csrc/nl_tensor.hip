@@ -3,6 +3,8 @@
 // and deterministic reductions.  Every kernel is a grid-stride loop over coalesced output
 // elements; reductions use wavefront shuffles and a fixed-order second pass (bitwise
 // reproducible run to run).
+#include <mutex>
+
 #include "nl_common.h"
 
 namespace {
@@ -1449,6 +1451,61 @@ int nl_finalize(int64_t nparts, const double* part, int64_t nx, const double* x,
   if (nparts < 0 || nx < 0 || !out || (nparts && !part) || (nx && (!x || !y))) return NL_EARG;
   hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(NT), 0, (hipStream_t)stream, nparts, part, nx,
                      x, y, out);
+  NL_LAUNCH_CHECK();
+  return NL_OK;
+}
+
+// ---- allocation-free energy: nl_energy_part + nl_finalize with the partials in a file-scope
+// device array instead of a caller workspace.  `scratch` holds NL_SCRATCH_DOUBLES doubles (32 KiB)
+// on every GPU the library runs on, until the process ends.  Host code never passes the array's
+// name to a kernel: its device address is resolved with hipGetSymbolAddress once per device (with
+// that device current, on the first call) and kept in g_scratch.  Every call overwrites each
+// partial it reads, so nothing needs clearing; calls that overlap on one GPU share the array and
+// race.
+}  // extern "C"
+
+constexpr int64_t NL_SCRATCH_DOUBLES = 4096;     // >= nl_partials(n) = grid_for(n, 4096)
+static __device__ __align__(256) double scratch[NL_SCRATCH_DOUBLES];
+constexpr int NL_MAX_DEVICES = 64;
+static double* g_scratch[NL_MAX_DEVICES];
+static std::mutex g_scratch_lock;
+
+static int scratch_for_current_device(double** out) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= NL_MAX_DEVICES) return NL_EHIP;
+  std::lock_guard<std::mutex> lk(g_scratch_lock);
+  if (!g_scratch[dev]) {
+    void* ptr = nullptr;
+    if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) return NL_EHIP;
+    g_scratch[dev] = (double*)ptr;
+  }
+  *out = g_scratch[dev];
+  return NL_OK;
+}
+
+extern "C" {
+
+int nl_ccsd_energy_noalloc(int64_t p, int64_t h, const double* f_ph, const double* vpphh,
+                           const double* t2, const double* t1, double* E, void* stream) {
+  if (p < 0 || h < 0 || !f_ph || !vpphh || !t2 || !t1 || !E) return NL_EARG;
+  const int64_t n = p * p * h * h;
+  const int64_t nparts = n ? grid_for(n, 4096) : 0;
+  if (nparts > NL_SCRATCH_DOUBLES) return NL_EARG;    // beyond the static partials array
+  double* part = nullptr;
+  const int rc = scratch_for_current_device(&part);
+  if (rc != NL_OK) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  if (n > 0) {
+    if (n < INT32_MAX)
+      hipLaunchKernelGGL((energy_part_kernel<true>), dim3(nparts), dim3(NT), 0, st, p, h,
+                         nl_fastdiv(p), nl_fastdiv(h), vpphh, t2, t1, part);
+    else
+      hipLaunchKernelGGL((energy_part_kernel<false>), dim3(nparts), dim3(NT), 0, st, p, h,
+                         nl_fastdiv(1), nl_fastdiv(1), vpphh, t2, t1, part);
+    NL_LAUNCH_CHECK();
+  }
+  hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(NT), 0, st, nparts, part, p * h, f_ph, t1,
+                     E);
   NL_LAUNCH_CHECK();
   return NL_OK;
 }
