@@ -393,6 +393,35 @@ int nl_contact_fill(int64_t nblk, const int64_t* off, const int64_t* ncol, const
                     const int64_t* coff, const unsigned char* rs, const unsigned char* cs,
                     int crossed, double g, int64_t total, double* out, void* stream);
 
+/* Plane-wave contact blocks in the reference's dense layout, written on the GPU (the v_pphh,
+ * v_phph, v_phhh, v_hhhh blocks that get_norm_ordered_ham returns, coupled_cluster.py:752-832
+ * with the sorting of coupled_cluster.py:99-263, for the plane-wave transform of
+ * lattice.contacts).  Orbital codes pack kx | ky << 8 | kz << 16 | sigma << 24 (L <= 255);
+ *   out[i0][i1][i2][i3] = g delta(k0 + k1 = k2 + k3 mod L)
+ *                           [d(s0,s2) d(s1,s3) - d(s0,s3) d(s1,s2)]
+ * for codes c0[i0], c1[i1], c2[i2], c3[i3]; each element is (double)(spin factor) * g. */
+int nl_pw_dense(int64_t n0, int64_t n1, int64_t n2, int64_t n3, const int* c0, const int* c1,
+                const int* c2, const int* c3, int L, double g, double* out, void* stream);
+
+/* Padded pair blocks of the plane-wave contact (the packed form of the v_pppp / v_ppph COO lists
+ * of coupled_cluster.py:219-263 that the pp ladder and the ppph GEMM consume): out[b][r][c] =
+ * <x y||u w> for row pair rows[b*mb+r] = x*rdiv + y (codes r0[x], r1[y]) and column pair
+ * cols[b*kb+c] = u*cdiv + w (codes q0[u], q1[w]), 0 for padding (-1) and for pairs of different
+ * total momentum or spin pair; nonzero values are +-g (0.0 + value, as a scatter-add forms it). */
+int nl_pw_pair_fill(int64_t nb, int64_t mb, int64_t kb, const int64_t* rows, const int64_t* cols,
+                    int64_t rdiv, int64_t cdiv, const int* r0, const int* r1, const int* q0,
+                    const int* q1, int L, double g, double* out, void* stream);
+
+/* The plane-wave contact's COO list (coupled_cluster.py:219-260 form, every antisymmetric
+ * permutation present) in group-major order: group q holds rows rpair[roff[q]..roff[q+1]) x
+ * columns cpair[coff[q]..coff[q+1]), row-major, from entry eoff[q]; entry e is written as
+ * idx[0..3][e] = (x, y, u, w) with (x*rdiv + y, u*cdiv + w) its pairs and val[e] = +g when
+ * sigma_x = sigma_u, else -g.  Groups are (total momentum, unordered spin pair) classes. */
+int nl_pw_coo(int64_t ngrp, const int64_t* eoff, const int64_t* roff, const int64_t* coff,
+              const int64_t* rpair, const int64_t* cpair, int64_t rdiv, int64_t cdiv,
+              const int* r0, const int* r1, const int* q0, const int* q1, int L, double g,
+              int64_t n, int64_t* idx, double* val, void* stream);
+
 /* Fused H2 assembly, Jacobi step, mixing and energy partials on the packed amplitudes
  * (coupled_cluster.py:475-545, 618-620, 623, 265-286): with (a,b,i,j) = orb[0..3][e],
  *   H = V + L + S[sg0] - S[sg1] - S[sg2] + S[sg3] + (X[a] + X[b] + X[p+i] + X[p+j]) t,

@@ -527,6 +527,108 @@ __global__ __launch_bounds__(NT) void contact_fill_kernel(int64_t nblk, const in
   }
 }
 
+// ---------------------------------------------------------------- plane-wave blocks
+// An orbital code packs (kx, ky, kz, sigma) as kx | ky << 8 | kz << 16 | sigma << 24, L <= 255.
+__device__ __forceinline__ int pw_ksum(int a, int b, int L) {     // k_a + k_b mod L, per axis
+  int x = (a & 255) + (b & 255), y = ((a >> 8) & 255) + ((b >> 8) & 255),
+      z = ((a >> 16) & 255) + ((b >> 16) & 255);
+  x -= x >= L ? L : 0; y -= y >= L ? L : 0; z -= z >= L ? L : 0;
+  return x | (y << 8) | (z << 16);
+}
+
+// <r1 r2||c1 c2> = g delta(K) [d(s1,s3) d(s2,s4) - d(s1,s4) d(s2,s3)], written as the product of
+// the small-integer spin factor and g so each element is the same double the host forms.
+__global__ __launch_bounds__(NT) void pw_dense_kernel(int64_t n0, int n1, int n2, int n3,
+                                                      const int* __restrict__ c0,
+                                                      const int* __restrict__ c1,
+                                                      const int* __restrict__ c2,
+                                                      const int* __restrict__ c3, int L, double g,
+                                                      double* __restrict__ out) {
+  const int64_t inner = (int64_t)n2 * n3, total = n0 * n1 * inner;
+  for (int64_t t = blockIdx.x * (int64_t)NT + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * NT) {
+    const int64_t row = t / inner;
+    const int col = (int)(t - row * inner);
+    const int i0 = (int)(row / n1), i1 = (int)(row - (int64_t)i0 * n1);
+    const int i2 = col / n3, i3 = col - i2 * n3;
+    const int a = c0[i0], b = c1[i1], c = c2[i2], d = c3[i3];
+    const int sa = a >> 24, sb = b >> 24, sc = c >> 24, sd = d >> 24;
+    const int spin = (sa == sc && sb == sd) - (sa == sd && sb == sc);
+    out[t] = pw_ksum(a, b, L) == pw_ksum(c, d, L) ? (double)spin * g : 0.0;
+  }
+}
+
+// Value of <x y||u w> for pairs with s_x != s_y, s_u != s_w: g when (s_u, s_w) = (s_x, s_y),
+// -g when swapped, 0 unless total momentum and the unordered spin pair agree.
+__device__ __forceinline__ bool pw_pair_val(int x, int y, int u, int w, int L, double g,
+                                            double* v) {
+  const int sx = x >> 24, sy = y >> 24, su = u >> 24, sw = w >> 24;
+  if (sx == sy || su == sw) return false;
+  if (!((sx == su && sy == sw) || (sx == sw && sy == su))) return false;
+  if (pw_ksum(x, y, L) != pw_ksum(u, w, L)) return false;
+  *v = (sx == su ? 1.0 : -1.0) * g;
+  return true;
+}
+
+struct PwPairArgs {
+  int64_t nb, mb, kb;
+  const int64_t* rows;   // [nb][mb] row pair index (-1 = padding), pair = first * rdiv + second
+  const int64_t* cols;   // [nb][kb] column pair index (-1 = padding)
+  int64_t rdiv, cdiv;
+  const int* r0; const int* r1; const int* q0; const int* q1;   // codes of the pair members
+  int L;
+  double g;
+  double* out;           // [nb][mb][kb]
+};
+
+__global__ __launch_bounds__(NT) void pw_pair_fill_kernel(PwPairArgs a) {
+  const int64_t total = a.nb * a.mb * a.kb;
+  for (int64_t t = blockIdx.x * (int64_t)NT + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * NT) {
+    const int64_t brow = t / a.kb, c = t - brow * a.kb;
+    const int64_t b = brow / a.mb;
+    const int64_t ra = a.rows[brow], ca = a.cols[b * a.kb + c];
+    double v = 0.0, e;
+    if (ra >= 0 && ca >= 0 &&
+        pw_pair_val(a.r0[ra / a.rdiv], a.r1[ra % a.rdiv], a.q0[ca / a.cdiv], a.q1[ca % a.cdiv],
+                    a.L, a.g, &e))
+      v = 0.0 + e;                    // the host scatter adds each value to a zero once
+    a.out[t] = v;
+  }
+}
+
+struct PwCooArgs {
+  int64_t ngrp;
+  const int64_t* eoff;   // [ngrp+1] first entry of each group
+  const int64_t* roff;   // [ngrp+1] rows of group q: rpair[roff[q]..roff[q+1])
+  const int64_t* coff;   // [ngrp+1]
+  const int64_t* rpair; const int64_t* cpair;
+  int64_t rdiv, cdiv;
+  const int* r0; const int* r1; const int* q0; const int* q1;
+  int L;
+  double g;
+  int64_t* idx;          // [4][n]: rpair / rdiv, rpair % rdiv, cpair / cdiv, cpair % cdiv
+  double* val;
+  int64_t n;
+};
+
+__global__ __launch_bounds__(NT) void pw_coo_kernel(PwCooArgs a) {
+  for (int64_t t = blockIdx.x * (int64_t)NT + threadIdx.x; t < a.n;
+       t += (int64_t)gridDim.x * NT) {
+    int64_t lo = 0, hi = a.ngrp - 1;                // group q with eoff[q] <= t < eoff[q+1]
+    while (lo < hi) {
+      const int64_t mid = (lo + hi + 1) >> 1;
+      if (a.eoff[mid] <= t) lo = mid; else hi = mid - 1;
+    }
+    const int64_t nc = a.coff[lo + 1] - a.coff[lo], local = t - a.eoff[lo];
+    const int64_t r = local / nc, c = local - r * nc;
+    const int64_t rp = a.rpair[a.roff[lo] + r], cp = a.cpair[a.coff[lo] + c];
+    const int64_t x = rp / a.rdiv, y = rp % a.rdiv, u = cp / a.cdiv, w = cp % a.cdiv;
+    a.idx[t] = x; a.idx[a.n + t] = y; a.idx[2 * a.n + t] = u; a.idx[3 * a.n + t] = w;
+    a.val[t] = ((a.r0[x] >> 24) == (a.q0[u] >> 24) ? 1.0 : -1.0) * a.g;
+  }
+}
+
 // ---------------------------------------------------------------- assembly
 struct SectorAsm {
   int64_t n, p;
@@ -690,6 +792,51 @@ int nl_contact_fill(int64_t nblk, const int64_t* off, const int64_t* ncol, const
   if (!off || !ncol || !roff || !coff || !rs || !cs || !out) return NL_EARG;
   hipLaunchKernelGGL(contact_fill_kernel, dim3(grid_of(total, 16384)), dim3(NT), 0,
                      (hipStream_t)stream, nblk, off, ncol, roff, coff, rs, cs, crossed, g, out);
+  NL_LAUNCH_CHECK();
+  return NL_OK;
+}
+
+int nl_pw_dense(int64_t n0, int64_t n1, int64_t n2, int64_t n3, const int* c0, const int* c1,
+                const int* c2, const int* c3, int L, double g, double* out, void* stream) {
+  if (n0 < 0 || n1 < 0 || n2 < 0 || n3 < 0 || L < 1 || L > 255) return NL_EARG;
+  if (n1 > INT32_MAX || n2 > INT32_MAX || n3 > INT32_MAX || n2 * n3 > INT32_MAX) return NL_EARG;
+  const int64_t total = n0 * n1 * n2 * n3;
+  if (total == 0) return NL_OK;
+  if (!c0 || !c1 || !c2 || !c3 || !out) return NL_EARG;
+  hipLaunchKernelGGL(pw_dense_kernel, dim3(grid_of(total, 16384)), dim3(NT), 0,
+                     (hipStream_t)stream, n0, (int)n1, (int)n2, (int)n3, c0, c1, c2, c3, L, g,
+                     out);
+  NL_LAUNCH_CHECK();
+  return NL_OK;
+}
+
+int nl_pw_pair_fill(int64_t nb, int64_t mb, int64_t kb, const int64_t* rows, const int64_t* cols,
+                    int64_t rdiv, int64_t cdiv, const int* r0, const int* r1, const int* q0,
+                    const int* q1, int L, double g, double* out, void* stream) {
+  if (nb < 0 || mb < 0 || kb < 0 || rdiv < 1 || cdiv < 1 || L < 1 || L > 255) return NL_EARG;
+  const int64_t total = nb * mb * kb;
+  if (total == 0) return NL_OK;
+  if (!rows || !cols || !r0 || !r1 || !q0 || !q1 || !out) return NL_EARG;
+  PwPairArgs a{nb, mb, kb, rows, cols, rdiv, cdiv, r0, r1, q0, q1, L, g, out};
+  hipLaunchKernelGGL(pw_pair_fill_kernel, dim3(grid_of(total, 16384)), dim3(NT), 0,
+                     (hipStream_t)stream, a);
+  NL_LAUNCH_CHECK();
+  return NL_OK;
+}
+
+int nl_pw_coo(int64_t ngrp, const int64_t* eoff, const int64_t* roff, const int64_t* coff,
+              const int64_t* rpair, const int64_t* cpair, int64_t rdiv, int64_t cdiv,
+              const int* r0, const int* r1, const int* q0, const int* q1, int L, double g,
+              int64_t n, int64_t* idx, double* val, void* stream) {
+  if (ngrp < 0 || n < 0 || rdiv < 1 || cdiv < 1 || L < 1 || L > 255) return NL_EARG;
+  if (n == 0) return NL_OK;
+  if (ngrp == 0 || !eoff || !roff || !coff || !rpair || !cpair || !r0 || !r1 || !q0 || !q1 ||
+      !idx || !val)
+    return NL_EARG;
+  PwCooArgs a{ngrp, eoff, roff, coff, rpair, cpair, rdiv, cdiv, r0, r1, q0, q1, L, g, idx, val,
+              n};
+  hipLaunchKernelGGL(pw_coo_kernel, dim3(grid_of(n, 16384)), dim3(NT), 0, (hipStream_t)stream,
+                     a);
   NL_LAUNCH_CHECK();
   return NL_OK;
 }

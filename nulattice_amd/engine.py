@@ -55,6 +55,9 @@ class Hamiltonian:
     """Normal-ordered Hamiltonian blocks on one GPU (reference layout + derived layouts)."""
 
     def __init__(self, fock_mats, two_body_int, device="cuda", min_block_rows=128):
+        """``two_body_int`` in the reference's order (v_pppp, v_ppph, v_pphh, v_phph, v_phhh,
+        v_hhhh); v_pppp may also be given as built PairBlocks and v_ppph as built PpphPlans
+        (``planewave.PlaneWave.device_hamiltonian`` writes both on the device)."""
         device = torch.device(device)
         if device.type != "cuda":
             raise _lib.NLError("Hamiltonian: a HIP device is required (no CPU fallback)")
@@ -90,11 +93,17 @@ class Hamiltonian:
                   self.v_hhhh, self.vP, self.Wc, self.v_phph_bkic, self.v_pphh_T)
         ops.mark_constant(*consts)
         weakref.finalize(self, ops.forget_ptrs, [t.data_ptr() for t in consts])
-        idx, val = coo_arrays(v_pppp, p, h, "pppp")
-        self.pppp = PairBlocks(idx[:, 0] * p + idx[:, 1], idx[:, 2] * p + idx[:, 3], val, p * p,
-                               p * p, device, min_block_rows)
-        idx3, val3 = coo_arrays(v_ppph, p, h, "ppph")
-        self.ppph = PpphPlans(idx3, val3, p, h, device)
+        if isinstance(v_pppp, PairBlocks):      # plans already built on the device
+            self.pppp = v_pppp
+        else:
+            idx, val = coo_arrays(v_pppp, p, h, "pppp")
+            self.pppp = PairBlocks(idx[:, 0] * p + idx[:, 1], idx[:, 2] * p + idx[:, 3], val,
+                                   p * p, p * p, device, min_block_rows)
+        if isinstance(v_ppph, PpphPlans):
+            self.ppph = v_ppph
+        else:
+            idx3, val3 = coo_arrays(v_ppph, p, h, "ppph")
+            self.ppph = PpphPlans(idx3, val3, p, h, device)
 
     def vpphh_view(self):
         """v_pphh[c,d,k,l] as a view of the [c][k][d][l] copy."""

@@ -172,6 +172,111 @@ class PlaneWave:
             return np.zeros((0, 4), np.int64), np.zeros(0)
         return np.concatenate(out_i).astype(np.int64), np.concatenate(out_v)
 
+    # -------------------------------------------------------------- device build
+    def codes(self, orbs):
+        """Orbital codes kx | ky << 8 | kz << 16 | sigma << 24 (int32) of the kernels that write
+        the blocks on the GPU."""
+        o = np.asarray(orbs)
+        k = self._kv[self._orb_k[o]]
+        return (k[:, 0] | (k[:, 1] << 8) | (k[:, 2] << 16) | (self._orb_s[o] << 24)).astype(
+            np.int32)
+
+    def pair_groups(self, first, second):
+        """Pairs (x, y) of ``first`` x ``second`` with sigma_x != sigma_y, grouped by the key
+        (total momentum, unordered sigma pair) that the contact conserves: (keys ascending,
+        start, end, pairs), pairs = x * len(second) + y ascending inside each group.  These are
+        the row / column classes of ``_coo`` (the two-body matrix couples a row pair only to
+        column pairs of the same key)."""
+        A, B = np.asarray(first), np.asarray(second)
+        s = self._orb_s
+        ia = np.repeat(np.arange(len(A)), len(B))
+        ib = np.tile(np.arange(len(B)), len(A))
+        keep = s[A[ia]] != s[B[ib]]
+        ia, ib = ia[keep], ib[keep]
+        sx, sy = s[A[ia]], s[B[ib]]
+        key = self._K(A[ia], B[ib]) * 16 + np.minimum(sx, sy) * 4 + np.maximum(sx, sy)
+        o = np.argsort(key, kind="stable")
+        key, pair = key[o], (ia * len(B) + ib)[o]
+        u, start = np.unique(key, return_index=True)
+        return u, start, np.append(start[1:], key.size), pair
+
+    def device_hamiltonian(self, device="cuda", min_block_rows=128):
+        """(E_ref, [f_pp, f_ph, f_hh], engine.Hamiltonian) with every two-body block written on
+        the GPU: the four dense blocks by ``nl_pw_dense``, the v_pppp pair blocks by
+        ``nl_pw_pair_fill`` over the analytically known components, the v_ppph COO list by
+        ``nl_pw_coo`` (then its segment plans sorted on the device).  The result equals
+        ``engine.Hamiltonian(*hamiltonian()[1:])`` bit for bit (same blocks, same plans)."""
+        import torch
+        from . import _lib, engine, ops
+        from .plans import PairBlocks, PpphPlans
+        dev = torch.device(device)
+        if dev.type != "cuda":
+            raise _lib.NLError("device_hamiltonian: a HIP device is required (no CPU fallback)")
+        _lib.load()
+        L, g, p, h = self.L, self.g, self.p, self.h
+        if L > 255:
+            raise ValueError("device_hamiltonian supports L <= 255")
+        F64 = torch.float64
+        cP = torch.tensor(self.codes(self.parts), device=dev)
+        cH = torch.tensor(self.codes(self.holes), device=dev)
+        dims = {"p": (p, cP), "h": (h, cH)}
+        dense = []
+        for pattern in ("pphh", "phph", "phhh", "hhhh"):
+            ns, cs = zip(*(dims[x] for x in pattern))
+            out = torch.empty(ns, dtype=F64, device=dev)
+            _lib.call("nl_pw_dense", *ns, *(ops._p(c) for c in cs), L, g, ops._p(out),
+                      ops._stream())
+            dense.append(out)
+
+        def filler(rdiv, r0, r1, cdiv, q0, q1):
+            def fill(rows, cols, nb, mb, kb):
+                V = torch.empty((nb, mb, kb), dtype=F64, device=dev)
+                _lib.call("nl_pw_pair_fill", nb, mb, kb, ops._p(rows), ops._p(cols), rdiv, cdiv,
+                          ops._p(r0), ops._p(r1), ops._p(q0), ops._p(q1), L, g, ops._p(V),
+                          ops._stream())
+                return V
+            return fill
+
+        P, H = self.parts, self.holes
+        ku, ks, ke, kpair = self.pair_groups(P, P)            # row classes of v_pppp and v_ppph
+        pp_groups = {int(u): kpair[a:b] for u, a, b in zip(ku, ks, ke)}
+        # v_pppp: every class couples its pairs with themselves; components ordered by first row
+        comp = sorted(pp_groups.values(), key=lambda r: int(r[0]))
+        pppp = PairBlocks.from_components(comp, comp, filler(p, cP, cP, p, cP, cP), dev,
+                                          min_block_rows)
+        # v_ppph = <cd||ak>: rows (c,d) of a class, columns (a,k) of the same class
+        hu, hs_, he, hpair = self.pair_groups(P, H)
+        ph_groups = {int(u): hpair[a:b] for u, a, b in zip(hu, hs_, he)}
+        common = [u for u in (int(x) for x in ku) if u in ph_groups]
+        if common:
+            R = [pp_groups[u] for u in common]
+            C = [ph_groups[u] for u in common]
+            nr = np.array([len(r) for r in R], np.int64)
+            nc = np.array([len(c) for c in C], np.int64)
+            eoff = np.concatenate([[0], np.cumsum(nr * nc)]).astype(np.int64)
+            roff = np.concatenate([[0], np.cumsum(nr)]).astype(np.int64)
+            coff = np.concatenate([[0], np.cumsum(nc)]).astype(np.int64)
+            n = int(eoff[-1])
+            T = lambda x: torch.tensor(x, dtype=torch.int64, device=dev)  # noqa: E731
+            eoff_d, roff_d, coff_d = T(eoff), T(roff), T(coff)
+            rp_d, cp_d = T(np.concatenate(R)), T(np.concatenate(C))
+            idx = torch.empty((4, n), dtype=torch.int64, device=dev)
+            val = torch.empty(n, dtype=F64, device=dev)
+            _lib.call("nl_pw_coo", len(common), ops._p(eoff_d), ops._p(roff_d), ops._p(coff_d),
+                      ops._p(rp_d), ops._p(cp_d), p, h, ops._p(cP), ops._p(cP), ops._p(cP),
+                      ops._p(cH), L, g, n, ops._p(idx), ops._p(val), ops._stream())
+            # R56 rows (a,k), columns (c,d): components ordered by first (a,k) row
+            order = sorted(range(len(common)), key=lambda q: int(C[q][0]))
+            r56 = PairBlocks.from_components([C[q] for q in order], [R[q] for q in order],
+                                             filler(h, cP, cH, p, cP, cP), dev)
+            ppph = PpphPlans(idx, val, p, h, dev, r56=r56)
+        else:
+            ppph = PpphPlans(np.zeros((0, 4), np.int64), np.zeros(0), p, h, dev)
+        fock = [self.f_pp, self.f_ph, self.f_hh]
+        H = engine.Hamiltonian(fock, [pppp, ppph] + dense, device=dev,
+                               min_block_rows=min_block_rows)
+        return self.ref_energy(), fock, H
+
     def ref_energy(self):
         H = self.holes
         v_hhhh = self.block(H, H, H, H)
@@ -222,3 +327,9 @@ class PlaneWave:
 def planewave_hamiltonian(L, A, C0, D0=0.0, sparse=True):
     """Hamiltonian of the closed-shell plane-wave reference (A = 4 or 28)."""
     return PlaneWave(L, shell_holes(L, A), C0, D0).hamiltonian(sparse)
+
+
+def planewave_device_hamiltonian(L, A, C0, D0=0.0, device="cuda"):
+    """(E_ref, fock_mats, engine.Hamiltonian) of the closed-shell plane-wave reference with the
+    two-body blocks and plans written on the GPU (``PlaneWave.device_hamiltonian``)."""
+    return PlaneWave(L, shell_holes(L, A), C0, D0).device_hamiltonian(device)

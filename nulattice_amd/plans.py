@@ -92,6 +92,28 @@ def _group(items, labels):
     return {int(x): it[s:e] for x, s, e in zip(u, start, ends)}
 
 
+def _pack_bins(r_list, c_list, min_rows):
+    """Greedy block-diagonal packing of components (given as ascending row / column index
+    arrays, in component order) into bins of >= ``min_rows`` rows, largest components first
+    (ties in component order).  Returns (bins, rows (nb, mb), cols (nb, kb)), -1 padded."""
+    order = sorted(range(len(r_list)), key=lambda q: -len(r_list[q]))
+    bins, cur, cur_rows = [], [], 0
+    for q in order:
+        cur.append(q); cur_rows += len(r_list[q])
+        if cur_rows >= min_rows:
+            bins.append(cur); cur, cur_rows = [], 0
+    if cur:
+        bins.append(cur)
+    mb = max(1, max(sum(len(r_list[q]) for q in b) for b in bins))
+    kb = max(1, max(sum(len(c_list[q]) for q in b) for b in bins))
+    rows = -np.ones((len(bins), mb), np.int64)
+    cols = -np.ones((len(bins), kb), np.int64)
+    for bi, b in enumerate(bins):
+        r = np.concatenate([r_list[q] for q in b]); cc = np.concatenate([c_list[q] for q in b])
+        rows[bi, :len(r)] = r; cols[bi, :len(cc)] = cc
+    return bins, rows, cols
+
+
 class PairBlocks:
     """Padded pair-block form of a COO (N,4) matrix over (row pair, col pair)."""
 
@@ -101,36 +123,22 @@ class PairBlocks:
             return
         g = sp.coo_matrix((np.ones(row.size), (row, nrow + col)), shape=(nrow + ncol,) * 2)
         ncomp, lab = csg.connected_components(g, directed=False)
-        rlab, clab = lab[row], lab[nrow + col]
+        rlab = lab[row]
         comps = [int(x) for x in np.unique(rlab)]
         # rows / cols of each component
-        ur, ur_inv = np.unique(row, return_inverse=True)
-        uc, uc_inv = np.unique(col, return_inverse=True)
-        rcomp = lab[ur]
-        ccomp = lab[nrow + uc]
-        r_of = _group(ur, rcomp)
-        c_of = _group(uc, ccomp)
-        # greedy block-diagonal packing of small components into bins of >= min_rows rows
-        order = sorted(comps, key=lambda c: -len(r_of[c]))
-        bins, cur, cur_rows = [], [], 0
-        for c in order:
-            cur.append(c); cur_rows += len(r_of[c])
-            if cur_rows >= min_rows:
-                bins.append(cur); cur, cur_rows = [], 0
-        if cur:
-            bins.append(cur)
-        mb = max(sum(len(r_of[c]) for c in b) for b in bins)
-        kb = max(sum(len(c_of[c]) for c in b) for b in bins)
-        mb = max(1, mb); kb = max(1, kb)
-        nb = len(bins)
-        rows = -np.ones((nb, mb), np.int64)
-        cols = -np.ones((nb, kb), np.int64)
+        ur = np.unique(row)
+        uc = np.unique(col)
+        r_of = _group(ur, lab[ur])
+        c_of = _group(uc, lab[nrow + uc])
+        bins, rows, cols = _pack_bins([r_of[c] for c in comps], [c_of[c] for c in comps],
+                                      min_rows)
+        nb, mb = rows.shape
+        kb = cols.shape[1]
         rpos = np.empty(nrow, np.int64)   # row pair -> (bin, local row)
         cpos = np.empty(ncol, np.int64)
         rbin = np.empty(nrow, np.int64)
-        for bi, b in enumerate(bins):
-            r = np.concatenate([r_of[c] for c in b]); cc = np.concatenate([c_of[c] for c in b])
-            rows[bi, :len(r)] = r; cols[bi, :len(cc)] = cc
+        for bi in range(nb):
+            r = rows[bi][rows[bi] >= 0]; cc = cols[bi][cols[bi] >= 0]
             rpos[r] = np.arange(len(r)); cpos[cc] = np.arange(len(cc)); rbin[r] = bi
         flat = (rbin[row] * mb + rpos[row]) * kb + cpos[col]
         V = np.bincount(flat, weights=val, minlength=nb * mb * kb).reshape(nb, mb, kb)
@@ -139,6 +147,27 @@ class PairBlocks:
         self.V = torch.tensor(V, dtype=torch.float64, device=device)
         self.rows = torch.tensor(rows.reshape(-1), dtype=torch.int64, device=device)
         self.cols = torch.tensor(cols.reshape(-1), dtype=torch.int64, device=device)
+
+    @classmethod
+    def from_components(cls, r_list, c_list, fill, device, min_rows=128):
+        """Pair blocks whose connected components are known: component q couples exactly the
+        rows ``r_list[q]`` and columns ``c_list[q]`` (ascending, components ordered by their
+        smallest row, the order a graph search from row 0 upwards labels them), so the packing
+        equals the one found from the COO list.  ``fill(rows, cols, nb, mb, kb)`` returns V as a
+        (nb, mb, kb) device tensor from the device index tensors."""
+        new = cls.__new__(cls)
+        new.nb = 0
+        if not r_list:
+            return new
+        _, rows, cols = _pack_bins(r_list, c_list, min_rows)
+        nb, mb = rows.shape
+        kb = cols.shape[1]
+        new.nb, new.mb, new.kb = nb, mb, kb
+        new.nnz = int(sum(len(r) * len(c) for r, c in zip(r_list, c_list)))
+        new.rows = torch.tensor(rows.reshape(-1), dtype=torch.int64, device=device)
+        new.cols = torch.tensor(cols.reshape(-1), dtype=torch.int64, device=device)
+        new.V = fill(new.rows, new.cols, nb, mb, kb)
+        return new
 
     def _like(self, V, rows, cols, mb, kb):
         new = PairBlocks.__new__(PairBlocks)
@@ -162,9 +191,22 @@ class PairBlocks:
 
 
 class SegPlan:
-    """Segments of (key -> [(src_off, val)]) for nl_segsum, with per-segment destination offset."""
+    """Segments of (key -> [(src_off, val)]) for nl_segsum, with per-segment destination offset.
+    The inputs are NumPy arrays or device tensors (then the stable sort runs on the device);
+    both give the same plan."""
 
     def __init__(self, key, src_off, val, dst_of_key, device):
+        if isinstance(key, torch.Tensor):
+            key_s, order = torch.sort(key, stable=True)
+            uk, counts = torch.unique_consecutive(key_s, return_counts=True)
+            self.S = int(uk.numel())
+            ptr = torch.zeros(self.S + 1, dtype=torch.int64, device=key.device)
+            torch.cumsum(counts, 0, out=ptr[1:])
+            self.ptr = ptr.to(device)
+            self.src = src_off[order].to(device=device, dtype=torch.int64)
+            self.val = val[order].to(device=device, dtype=torch.float64)
+            self.dst = dst_of_key(uk).to(device=device, dtype=torch.int64)
+            return
         order = np.argsort(key, kind="stable")
         key, src_off, val = key[order], src_off[order], val[order]
         uk, start = np.unique(key, return_index=True)
@@ -195,14 +237,23 @@ class PpphPlans:
       R56[a,k,i,j]   += v tau[c,d,i,j]      pair blocks over rows (a,k), columns (c,d)
     """
 
-    def __init__(self, idx, val, p, h, device):
-        self.n = val.size
+    def __init__(self, idx, val, p, h, device, r56=None):
+        """``idx`` (N, 4) and ``val`` (N,) as NumPy arrays, or ``idx`` a (4, N) int64 device
+        tensor and ``val`` a device tensor (plans then sorted on the device).  ``r56`` may pass
+        the R56 pair blocks when their components are known (``PairBlocks.from_components``)."""
+        on_dev = isinstance(val, torch.Tensor)
+        self.n = int(val.numel()) if on_dev else val.size
         self.p, self.h, self.device = p, h, device
         self._adj = None
-        if val.size == 0:
+        if self.n == 0:
             return
-        c, d, a, k = (idx[:, n].astype(np.int64) for n in range(4))
-        self._coo = (c, d, a, k, np.asarray(val, dtype=np.float64))
+        if on_dev:
+            c, d, a, k = (idx[n] for n in range(4))
+            val = val.to(torch.float64)
+        else:
+            c, d, a, k = (idx[:, n].astype(np.int64) for n in range(4))
+            val = np.asarray(val, dtype=np.float64)
+        self._coo = (c, d, a, k, val)
         hh, ph = h * h, p * h
         # ret0 partials T0[a][k][i] (segments (a,k)); ret0[a,i] = sum_k T0[a,k,i]
         self.ret0 = SegPlan(a * h + k, ((c * p + d) * h + k) * h, val, lambda u: u * h, device)
@@ -219,7 +270,8 @@ class PpphPlans:
                             lambda u: ((u // h) % p * h + u % h) * ph + (u // (p * h)) * h,
                             device)
         # R56[(a,k),(i,j)] = sum_(c,d) v[(c,d),(a,k)] tau[(c,d),(i,j)]: a pair-block GEMM
-        self.r56 = PairBlocks(a * h + k, c * p + d, val, p * h, p * p, device)
+        self.r56 = r56 if r56 is not None else PairBlocks(a * h + k, c * p + d, val, p * h,
+                                                           p * p, device)
 
     def scaled(self, scale, scale_values):
         """The same plans with every value times ``scale`` (adjoint plans are not carried)."""
