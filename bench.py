@@ -120,6 +120,18 @@ def time_sectors(SH, steps, warmup, torch, graph=True):
     return (time.perf_counter() - t0) / steps
 
 
+def profile_value(name, key):
+    """A figure recorded in a committed profile summary (``key = value`` line), or None."""
+    import re
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", name)
+    try:
+        with open(path) as f:
+            m = re.search(rf"^{re.escape(key)}\s*=\s*(\S+)", f.read(), re.M)
+    except OSError:
+        return None
+    return float(m.group(1)) if m else None
+
+
 def combined(a, C0, D0, torch, ccm, engine, ops, adjoint):
     """The combined configuration at the largest single-GPU size (L=6, A=28: T2 836^2 x 28^2 =
     5.5e8 amplitudes): the dense mixed-precision iteration rate (and the FP64 one), the SoA
@@ -550,9 +562,14 @@ def main():
     peak = 2.5e15
     roof = None
     if ring_s:
-        ach = nprod * ring_flop / ring_s
+        ach = ring_flop / ring_s                    # algorithmic: 2 (ph)^3 per launch
+        traffic = (profile_value("ccsd_L5_A28_ring_hbm_traffic.md", "ring_spp_traffic_bytes")
+                   if (a.L, a.A, a.prec) == (5, 28, "fp16x3") else None)
         roof = {"bound": "mfma", "achieved": ach / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
-                "frac": ach / peak, "traffic": None,
+                "frac": ach / peak, "traffic": traffic,
+                "traffic_source": "profiles/ccsd_L5_A28_ring_hbm_traffic.md (calibrated "
+                                  "FETCH_SIZE + WRITE_SIZE of this launch)" if traffic else None,
+                "executed_achieved": nprod * ach / 1e12, "executed_frac": nprod * ach / peak,
                 "kernel": f"ring GEMM Spp ({a.prec}: {nprod} partial products of 2(ph)^3 "
                           f"FLOP)", "kernel_ms": ring_s * 1e3,
                 "algorithmic_flop": ring_flop, "executed_flop": nprod * ring_flop,
