@@ -393,6 +393,15 @@ int nl_contact_fill(int64_t nblk, const int64_t* off, const int64_t* ncol, const
                     const int64_t* coff, const unsigned char* rs, const unsigned char* cs,
                     int crossed, double g, int64_t total, double* out, void* stream);
 
+/* Orbital-pair images of a basis change, out[r][a][b] = C[pr[r]][c1+a] * C[qr[r]][c2+b] for the m
+ * lattice-state pairs (pr[r], qr[r]) on which the two-body interaction is supported (C row-major
+ * with ldc columns).  The quarter transformations of a two-body operator into the Hartree-Fock
+ * basis (the rotation of the V_eff that HF/hartree_fock.py:70-213 contracts, applied to the
+ * coupled_cluster.py:752-832 blocks) become two GEMMs over the support: v'[(ab),(cd)] =
+ * P_ab^T (V_r P_cd). */
+int nl_pair_outer(int64_t m, const int64_t* pr, const int64_t* qr, const double* C, int64_t ldc,
+                  int64_t c1, int64_t n1, int64_t c2, int64_t n2, double* out, void* stream);
+
 /* Plane-wave contact blocks in the reference's dense layout, written on the GPU (the v_pphh,
  * v_phph, v_phhh, v_hhhh blocks that get_norm_ordered_ham returns, coupled_cluster.py:752-832
  * with the sorting of coupled_cluster.py:99-263, for the plane-wave transform of

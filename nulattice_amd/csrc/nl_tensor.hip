@@ -627,6 +627,24 @@ __global__ __launch_bounds__(NT) void gather_rows_kernel(int64_t R, int64_t Cn,
   }
 }
 
+// out[r][a][b] = C[pr[r]*ldc + c1 + a] * C[qr[r]*ldc + c2 + b]  (r < m, a < n1, b < n2): the
+// orbital-pair images of the rows of a basis-change matrix, one row per workgroup pass
+__global__ __launch_bounds__(NT) void pair_outer_kernel(int64_t m, const int64_t* pr,
+                                                        const int64_t* qr, const double* C,
+                                                        int64_t ldc, int64_t c1, int64_t n1,
+                                                        int64_t c2, int64_t n2, double* out) {
+  const int64_t per = n1 * n2;
+  for (int64_t r = blockIdx.x; r < m; r += gridDim.x) {
+    const double* x = C + pr[r] * ldc + c1;
+    const double* y = C + qr[r] * ldc + c2;
+    double* o = out + r * per;
+    for (int64_t a = 0; a < n1; ++a) {
+      const double xa = x[a];
+      for (int64_t b = threadIdx.x; b < n2; b += NT) o[a * n2 + b] = xa * y[b];
+    }
+  }
+}
+
 // out[idx[r]*ldo + c] += alpha * src[r*Cn + c]   (rows with idx < 0 skipped; idx unique)
 __global__ __launch_bounds__(NT) void scatter_add_rows_kernel(int64_t R, int64_t Cn,
                                                               const int64_t* idx,
@@ -1301,6 +1319,18 @@ int nl_segsum(int64_t S, int64_t X, const int64_t* ptr, const int64_t* src_off, 
   int blocks = grid_for(waves * 64);
   hipLaunchKernelGGL(segsum_kernel, dim3(blocks), dim3(NT), 0, (hipStream_t)stream, S, X, G,
                      ptr, src_off, val, dst_off, src_stride, dst_stride, alpha, src, out);
+  NL_LAUNCH_CHECK();
+  return NL_OK;
+}
+
+int nl_pair_outer(int64_t m, const int64_t* pr, const int64_t* qr, const double* C, int64_t ldc,
+                  int64_t c1, int64_t n1, int64_t c2, int64_t n2, double* out, void* stream) {
+  if (m < 0 || n1 < 0 || n2 < 0 || c1 < 0 || c2 < 0 || c1 + n1 > ldc || c2 + n2 > ldc)
+    return NL_EARG;
+  if (m == 0 || n1 == 0 || n2 == 0) return NL_OK;
+  if (!pr || !qr || !C || !out) return NL_EARG;
+  hipLaunchKernelGGL(pair_outer_kernel, dim3((unsigned)std::min<int64_t>(m, 65536)), dim3(NT), 0,
+                     (hipStream_t)stream, m, pr, qr, C, ldc, c1, n1, c2, n2, out);
   NL_LAUNCH_CHECK();
   return NL_OK;
 }

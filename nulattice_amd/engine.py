@@ -48,7 +48,9 @@ F64 = torch.float64
 def _dev(x, device):
     if isinstance(x, torch.Tensor):
         return x.detach().to(device=device, dtype=F64).contiguous()
-    return torch.tensor(np.asarray(x, dtype=np.float64), dtype=F64, device=device)
+    # torch.tensor keeps a NumPy array's strides (an F-ordered block stays column-major), and
+    # the kernels read row-major memory
+    return torch.tensor(np.ascontiguousarray(x, dtype=np.float64), dtype=F64, device=device)
 
 
 class Hamiltonian:

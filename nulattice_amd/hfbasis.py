@@ -168,6 +168,112 @@ def normal_order(op1, op2, op3, rho):
     return e_ref, h + G + 0.5 * O, V
 
 
+def pair_support(op2, op3, rho, n):
+    """The two-body interaction V_eff = V + V3 of ``normal_order`` on its support, without an
+    n^4 array: (pairs (2, m) int64, Vr (m, m)) with V_eff[p,q,r,s] = Vr[i,j] for the ordered
+    lattice-state pairs (p,q) = pairs[:, i], (r,s) = pairs[:, j], and zero for every (p,q,r,s)
+    outside pairs x pairs.  The support is every ordered pair that occurs in a signed image of a
+    two-body entry or as the first two indices of a three-body image (contact interactions: the
+    pairs of states on one site)."""
+    kk, kb, w = [], [], []
+    if op2 is not None and len(op2):
+        idx, val = _split(op2, 4)
+        for pk, sk in _S2:
+            for pb, sb in _S2:
+                kk.append(idx[:, pk[0]] * n + idx[:, pk[1]])
+                kb.append(idx[:, 2 + pb[0]] * n + idx[:, 2 + pb[1]])
+                w.append(sk * sb * val)
+    if op3 is not None and len(op3):
+        for ket, bra, v in _three_body_images(op3):
+            kk.append(ket[:, 0] * n + ket[:, 1])
+            kb.append(bra[:, 0] * n + bra[:, 1])
+            w.append(v * rho[ket[:, 2], bra[:, 2]])
+    if not w:
+        return np.zeros((2, 0), np.int64), np.zeros((0, 0))
+    kk, kb, w = np.concatenate(kk), np.concatenate(kb), np.concatenate(w)
+    keys = np.unique(np.concatenate([kk, kb]))
+    Vr = np.zeros((keys.size, keys.size))
+    np.add.at(Vr, (np.searchsorted(keys, kk), np.searchsorted(keys, kb)), w)
+    return np.stack([keys // n, keys % n]).astype(np.int64), Vr
+
+
+def hf_basis_engine(op1, op2, op3, C, A, device="cuda"):
+    """(E_ref, fock_mats, engine.Hamiltonian) of the NO2B Hamiltonian normal-ordered with respect
+    to the determinant of the first A columns of C, in the basis of C's columns (holes = columns
+    0..A-1), built on the GPU without any n^4 array on the host: the interaction on its pair
+    support (``pair_support``, m pairs) is rotated by two FP64 GEMMs per block,
+      v'[(ab),(cd)] = P_ab^T (Vr P_cd),   P_ab[r, (a,b)] = C[p_r, a] C[q_r, b]  (nl_pair_outer),
+    straight into the engine's layouts: the four dense blocks, v_pppp as one dense pair block
+    and v_ppph as its device COO list (segment plans sorted on the device) with one dense R56
+    block.  The Fock matrix, E_ref and the HF density are host arrays of size n^2."""
+    import torch as _t
+    from . import engine
+    from .plans import PairBlocks, PpphPlans
+    device = _t.device(device)
+    if device.type != "cuda":
+        raise _lib.NLError("hf_basis_engine: a HIP device is required (no CPU fallback)")
+    _lib.load()
+    C = np.asarray(C, dtype=np.float64)
+    n = C.shape[0]
+    if C.shape != (n, n) or not 0 < A < n:
+        raise ValueError("C must be square and 0 < A < its size")
+    rho = C[:, :A] @ C[:, :A].T
+    h1 = one_body_matrix(op1, n)
+    G, O = fock_pieces(op2, op3, rho)
+    e_ref = float(np.sum((h1 + 0.5 * G + O / 6.0) * rho.T))
+    Fr = C.T @ (h1 + G + 0.5 * O) @ C
+    fock = [np.ascontiguousarray(x) for x in (Fr[A:, A:], Fr[A:, :A], Fr[:A, :A])]
+    pairs, Vr = pair_support(op2, op3, rho, n)
+    p, h, m = n - A, A, pairs.shape[1]
+    # nl_pair_outer reads C row-major; torch.tensor would keep an F-ordered array's strides
+    Cd = _t.tensor(np.ascontiguousarray(C), dtype=F64, device=device)
+    Vd = _t.tensor(np.ascontiguousarray(Vr), dtype=F64, device=device)
+    pr = _t.tensor(pairs[0], dtype=_t.int64, device=device)
+    qr = _t.tensor(pairs[1], dtype=_t.int64, device=device)
+    span = {"h": (0, h), "p": (A, p)}
+
+    def images(xy):
+        (c1, n1), (c2, n2) = span[xy[0]], span[xy[1]]
+        P = _t.empty((m, n1 * n2), dtype=F64, device=device)
+        _lib.call("nl_pair_outer", m, ops._p(pr), ops._p(qr), ops._p(Cd), n, c1, n1, c2, n2,
+                  ops._p(P), ops._stream())
+        return P
+
+    P = {xy: images(xy) for xy in ("pp", "ph", "hh")}
+    Q = {}
+    for zw in ("pp", "ph", "hh"):
+        Q[zw] = _t.empty_like(P[zw])
+        ops.contract("rs,sx->rx", Vd, P[zw], Q[zw], beta=0.0)
+    size = {"p": p, "h": h}
+
+    def block(xy, zw):
+        out = _t.empty((size[xy[0]] * size[xy[1]], size[zw[0]] * size[zw[1]]), dtype=F64,
+                       device=device)
+        ops.contract("rx,ry->xy", P[xy], Q[zw], out, beta=0.0)
+        return out.view(size[xy[0]], size[xy[1]], size[zw[0]], size[zw[1]])
+
+    dense = [block("pp", "hh"), block("ph", "ph"), block("ph", "hh"), block("hh", "hh")]
+    v_ppph = block("pp", "ph")
+    v_pppp = block("pp", "pp")
+    del P, Q
+    allp = np.arange(p * p)
+    pppp = PairBlocks.from_components([allp], [allp],
+                                      lambda rows, cols, nb, mb, kb: v_pppp.view(1, mb, kb),
+                                      device)
+    mask = v_ppph != 0
+    idx = mask.nonzero().t().contiguous()
+    val = v_ppph[mask]
+    del mask
+    r56 = PairBlocks.from_components(
+        [np.arange(p * h)], [allp],
+        lambda rows, cols, nb, mb, kb: ops.copy(v_ppph.permute(2, 3, 0, 1)).view(1, mb, kb),
+        device)
+    ppph = PpphPlans(idx, val, p, h, device, r56=r56)
+    del v_ppph
+    H = engine.Hamiltonian(fock, [pppp, ppph] + dense, device=device)
+    return e_ref, fock, H
+
+
 def _blocks(fock, V, A, sparse):
     h, p = slice(0, A), slice(A, None)
     f_pp, f_ph, f_hh = fock[p, p], fock[p, h], fock[h, h]

@@ -89,3 +89,24 @@ def test_two_and_three_body_tensors_are_antisymmetric():
     G, O = hb.fock_pieces(op2, op3, rho)
     np.testing.assert_allclose(G, np.einsum("ptqu,tu->pq", V, rho), atol=1e-13)
     np.testing.assert_allclose(O, np.einsum("ptqu,tu->pq", V3, rho), atol=1e-13)
+
+
+@pytest.mark.parametrize("w3", [0.0, 6.0])
+def test_pair_support_is_the_dense_interaction(w3):
+    """pair_support carries V_eff = V + V3 on its ordered-pair support: scattered back to n^4 it
+    equals two_body_tensor + three_body_no2b at a random density, and nothing outside the support
+    is nonzero (16O lists, L=2)."""
+    g = GOLD["16O_L2"]
+    L = 2
+    op1, op2, op3 = _lists(L, g["vT1"], g["vS1"], w3)
+    n = 4 * L ** 3
+    rng = np.random.default_rng(5)
+    X = rng.standard_normal((n, 16))
+    rho = X @ X.T / n
+    pairs, Vr = hb.pair_support(op2, op3, rho, n)
+    assert pairs.shape[1] == 12 * L ** 3            # ordered pairs of distinct states per site
+    V = hb.two_body_tensor(op2, n) + hb.three_body_no2b(op3, rho)
+    W = np.zeros_like(V)
+    p, q = pairs
+    W[p[:, None], q[:, None], p[None, :], q[None, :]] = Vr
+    np.testing.assert_allclose(W, V, rtol=0, atol=1e-13)

@@ -263,3 +263,14 @@ def test_coo_arrays_rejects_mismatched_layouts():
         coo_arrays((idx, np.zeros(6)), 9, 4, "ppph")
     with pytest.raises(ValueError):
         coo_arrays(CooBlock(idx.T.copy(), np.zeros(5)), 9, 4, "ppph")
+
+
+def test_host_blocks_reach_the_device_row_major():
+    """An F-ordered NumPy block (a column slice, an eigenvector matrix) becomes a row-major
+    tensor: torch.tensor alone keeps the column-major strides, which raw kernels would misread."""
+    import numpy as np
+    from nulattice_amd import engine
+    x = np.asfortranarray(np.arange(12.0).reshape(3, 4))
+    t = engine._dev(x, "cpu")
+    assert t.is_contiguous() and t.stride() == (4, 1)
+    assert np.array_equal(t.numpy(), x)
