@@ -393,6 +393,22 @@ int nl_contact_fill(int64_t nblk, const int64_t* off, const int64_t* ncol, const
                     const int64_t* coff, const unsigned char* rs, const unsigned char* cs,
                     int crossed, double g, int64_t total, double* out, void* stream);
 
+/* Segment plans of the v_ppph intermediates built on the device (the per-output-row grouping of
+ * the COO loop of v_ppph_dgrams, ccDgrams.py:13-45, that nl_segsum consumes).  For a (4, n) list
+ * idx of entries (c, d, a, k) with values val: key = wkey . (c,d,a,k) < 2^kbits, src_out = wsrc .
+ * (c,d,a,k) and val_out = sign * val, both ordered by key with equal keys in list order (stable
+ * radix sort); the number of distinct keys S is written to nseg (device).  nl_segplan_finish then
+ * writes ptr[0..S] (segment starts, ptr[S] = n) and dst[s] = sum_t ((u / div_t) % mod_t) * mul_t
+ * for the s-th key u, dst_spec = {div, mod, mul} x 3 (mod 0: none).  wkey, wsrc and dst_spec
+ * are host arrays; ws is a device workspace of nl_segplan_ws(n) bytes shared by the two calls;
+ * n < 2^32. */
+int64_t nl_segplan_ws(int64_t n);
+int nl_segplan_sort(int64_t n, const int64_t* idx, const int64_t* wkey, const int64_t* wsrc,
+                    const double* val, double sign, int kbits, void* ws, int64_t ws_bytes,
+                    int64_t* src_out, double* val_out, int64_t* nseg, void* stream);
+int nl_segplan_finish(int64_t S, int64_t n, void* ws, int64_t ws_bytes, const int64_t* dst_spec,
+                      int64_t* ptr, int64_t* dst, void* stream);
+
 /* Orbital-pair images of a basis change, out[r][a][b] = C[pr[r]][c1+a] * C[qr[r]][c2+b] for the m
  * lattice-state pairs (pr[r], qr[r]) on which the two-body interaction is supported (C row-major
  * with ldc columns).  The quarter transformations of a two-body operator into the Hartree-Fock

@@ -9,7 +9,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libnlccsd.so")
-SOURCES = ["nl_gemm.hip", "nl_tensor.hip", "nl_sector.hip"]
+SOURCES = ["nl_gemm.hip", "nl_tensor.hip", "nl_sector.hip", "nl_plan.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-shared", "-std=c++17",
          # packed FP32 instructions stay off (the compiler would otherwise form them itself)

@@ -69,6 +69,7 @@ class PlaneWave:
         self.rho = self.nk / L ** 3
         self.g = (self.C0 + 2.0 * self.D0 * self.rho) / L ** 3      # NO2B contact strength
         self._kv, self._orb_k, self._orb_s = kv, orb_k, orb_s
+        self._kd = [np.ascontiguousarray(kv[:, d]) for d in range(3)]
         self.eps = eps_k(L)
         e_h = self.eps[orb_k[self.holes]]
         e_p = self.eps[orb_k[self.parts]]
@@ -92,9 +93,13 @@ class PlaneWave:
     def _K(self, a, b):
         """Encoded total momentum (k_a + k_b mod L) for orbital arrays a, b (broadcasting)."""
         L = self.L
-        ka, kb = self._kv[self._orb_k[a]], self._kv[self._orb_k[b]]
-        t = (ka + kb) % L
-        return (t[..., 0] * L + t[..., 1]) * L + t[..., 2]
+        ka, kb = self._orb_k[a], self._orb_k[b]
+        out = 0
+        for kd in self._kd:          # one axis at a time, on flat integer arrays
+            t = kd[ka] + kd[kb]
+            t -= L * (t >= L)        # = (k_a + k_b) mod L for momenta in [0, L)
+            out = out * L + t
+        return out
 
     def block(self, rows1, rows2, cols1, cols2):
         """Dense <r1 r2 || c1 c2> over orbital lists."""

@@ -10,6 +10,7 @@ v_ppph (COO <cd||ak>) becomes *segment plans* for nl_segsum: for each intermedia
 v_ppph_dgrams (ccDgrams.py:13-45) the entries are grouped by output row so that every output row
 is owned by one wavefront (no atomics).
 """
+import ctypes
 from typing import NamedTuple
 
 import numpy as np
@@ -191,22 +192,9 @@ class PairBlocks:
 
 
 class SegPlan:
-    """Segments of (key -> [(src_off, val)]) for nl_segsum, with per-segment destination offset.
-    The inputs are NumPy arrays or device tensors (then the stable sort runs on the device);
-    both give the same plan."""
+    """Segments of (key -> [(src_off, val)]) for nl_segsum, with per-segment destination offset."""
 
     def __init__(self, key, src_off, val, dst_of_key, device):
-        if isinstance(key, torch.Tensor):
-            key_s, order = torch.sort(key, stable=True)
-            uk, counts = torch.unique_consecutive(key_s, return_counts=True)
-            self.S = int(uk.numel())
-            ptr = torch.zeros(self.S + 1, dtype=torch.int64, device=key.device)
-            torch.cumsum(counts, 0, out=ptr[1:])
-            self.ptr = ptr.to(device)
-            self.src = src_off[order].to(device=device, dtype=torch.int64)
-            self.val = val[order].to(device=device, dtype=torch.float64)
-            self.dst = dst_of_key(uk).to(device=device, dtype=torch.int64)
-            return
         order = np.argsort(key, kind="stable")
         key, src_off, val = key[order], src_off[order], val[order]
         uk, start = np.unique(key, return_index=True)
@@ -218,11 +206,66 @@ class SegPlan:
         self.dst = torch.tensor(np.asarray(dst_of_key(uk), dtype=np.int64), dtype=torch.int64,
                                 device=device)
 
+    @classmethod
+    def on_device(cls, idx, val, wkey, wsrc, sign, kmax, dst_spec):
+        """The same plan for a device COO list, built by nl_segplan_sort / nl_segplan_finish: the
+        entries (c, d, a, k) = idx[0..3] (a (4, N) int64 tensor), key = wkey . (c, d, a, k) <
+        kmax, src_off = wsrc . (c, d, a, k), value sign * val, and destination offset
+        sum_t ((u // div_t) % mod_t) * mul_t of key u for dst_spec = ((div, mod, mul) x 3), mod 0
+        meaning none.  A stable radix sort orders equal keys as the list does, so the plan equals
+        the NumPy one built from the same list."""
+        from . import _lib
+        n = int(val.numel())
+        dev = val.device
+        arr = lambda v: (ctypes.c_int64 * len(v))(*[int(x) for x in v])  # noqa: E731
+        ws_bytes = int(_lib.load().nl_segplan_ws(n))
+        if ws_bytes < 0:
+            raise _lib.NLError(f"nl_segplan_ws: unsupported list length {n}")
+        ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
+        new = cls.__new__(cls)
+        new.src = torch.empty(n, dtype=torch.int64, device=dev)
+        new.val = torch.empty(n, dtype=torch.float64, device=dev)
+        nseg = torch.empty(1, dtype=torch.int64, device=dev)
+        st = ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+        p = lambda t: ctypes.c_void_p(t.data_ptr())  # noqa: E731
+        _lib.call("nl_segplan_sort", n, p(idx), arr(wkey), arr(wsrc), p(val),
+                  ctypes.c_double(sign), max(1, int(kmax - 1).bit_length()), p(ws), ws_bytes,
+                  p(new.src), p(new.val), p(nseg), st)
+        new.S = int(nseg.item())
+        new.ptr = torch.empty(new.S + 1, dtype=torch.int64, device=dev)
+        new.dst = torch.empty(new.S, dtype=torch.int64, device=dev)
+        _lib.call("nl_segplan_finish", new.S, n, p(ws), ws_bytes,
+                  arr([x for t in dst_spec for x in t]), p(new.ptr), p(new.dst), st)
+        return new
+
     def scaled(self, scale, scale_values):
         new = SegPlan.__new__(SegPlan)
         new.S, new.ptr, new.src, new.dst = self.S, self.ptr, self.src, self.dst
         new.val = scale_values(self.val, scale)
         return new
+
+
+def ppph_specs(p, h):
+    """The linear forms of the v_ppph segment plans over an entry (c, d, a, k) of <cd||ak>:
+    name -> (key weights, source-offset weights, value sign, key bound, destination spec), with
+    destination offset sum_t ((u // div_t) % mod_t) * mul_t of key u for spec ((div, mod, mul)
+    x 3, mod 0 = none).  These are the keys, sources and destinations PpphPlans builds from a
+    NumPy list (forward plans, then the transposed ones of ``adjoint``)."""
+    hh, ph = h * h, p * h
+    one = (1, 0, 0)
+    return {
+        "ret0": ((0, 0, h, 1), (p * hh, hh, 0, h), 1.0, ph, ((1, 0, h), one, one)),
+        "ret1": ((0, 1, p, 0), (h, 0, 0, 1), 1.0, p * p, ((1, 0, 1), one, one)),
+        "ret2": ((ph, h, 0, 1), (0, 0, h, 0), 1.0, p * ph,
+                 ((ph, 0, h * ph), (h, p, h), (1, h, 1))),
+        "ret2p": ((ph, h, 0, 1), (0, 0, h, 0), -1.0, p * ph, ((1, 0, h), one, one)),
+        "ret4": ((0, h, ph, 1), (h, 0, 0, 0), 1.0, p * ph,
+                 ((h, p, h * ph), (1, h, ph), (ph, 0, h))),
+        "ret0T": ((ph, h, 0, 1), (0, 0, h, 0), 1.0, p * ph, ((1, 0, h), one, one)),
+        "ret1T": ((h, 0, 0, 1), (0, 1, p, 0), 1.0, ph, ((1, 0, 1), one, one)),
+        "ret2T": ((0, 0, 1, 0), (h * ph, h, 0, 1), 1.0, p, ((1, 0, h), one, one)),
+        "ret4T": ((1, 0, 0, 0), (0, h * ph, h, ph), 1.0, p, ((1, 0, h), one, one)),
+    }
 
 
 class PpphPlans:
@@ -239,22 +282,29 @@ class PpphPlans:
 
     def __init__(self, idx, val, p, h, device, r56=None):
         """``idx`` (N, 4) and ``val`` (N,) as NumPy arrays, or ``idx`` a (4, N) int64 device
-        tensor and ``val`` a device tensor (plans then sorted on the device).  ``r56`` may pass
-        the R56 pair blocks when their components are known (``PairBlocks.from_components``)."""
+        tensor and ``val`` a device tensor (plans then sorted on the device by nl_segplan_sort).
+        ``r56`` may pass the R56 pair blocks when their components are known
+        (``PairBlocks.from_components``)."""
         on_dev = isinstance(val, torch.Tensor)
         self.n = int(val.numel()) if on_dev else val.size
         self.p, self.h, self.device = p, h, device
         self._adj = None
         if self.n == 0:
             return
-        if on_dev:
-            c, d, a, k = (idx[n] for n in range(4))
-            val = val.to(torch.float64)
-        else:
-            c, d, a, k = (idx[:, n].astype(np.int64) for n in range(4))
-            val = np.asarray(val, dtype=np.float64)
-        self._coo = (c, d, a, k, val)
         hh, ph = h * h, p * h
+        if on_dev:
+            self._coo = None
+            self._dcoo = (idx, val.to(torch.float64))
+            specs = ppph_specs(p, h)
+            for name in ("ret0", "ret1", "ret2", "ret2p", "ret4"):
+                setattr(self, name, SegPlan.on_device(*self._dcoo, *specs[name]))
+            if r56 is None:
+                raise ValueError("PpphPlans on a device list needs its R56 pair blocks")
+            self.r56 = r56
+            return
+        c, d, a, k = (idx[:, n].astype(np.int64) for n in range(4))
+        val = np.asarray(val, dtype=np.float64)
+        self._coo = (c, d, a, k, val)
         # ret0 partials T0[a][k][i] (segments (a,k)); ret0[a,i] = sum_k T0[a,k,i]
         self.ret0 = SegPlan(a * h + k, ((c * p + d) * h + k) * h, val, lambda u: u * h, device)
         self.ret1 = SegPlan(a * p + d, c * h + k, val, lambda u: u, device)
@@ -274,11 +324,16 @@ class PpphPlans:
                                                            p * p, device)
 
     def scaled(self, scale, scale_values):
-        """The same plans with every value times ``scale`` (adjoint plans are not carried)."""
+        """The same plans with every value times ``scale`` (adjoint plans are rebuilt on use
+        from the scaled list)."""
         new = PpphPlans.__new__(PpphPlans)
         new.n, new.p, new.h, new.device, new._adj = self.n, self.p, self.h, self.device, None
         if self.n:
-            new._coo = self._coo
+            if self._coo is None:
+                new._coo = None
+                new._dcoo = (self._dcoo[0], scale_values(self._dcoo[1], scale))
+            else:
+                new._coo = self._coo[:4] + (self._coo[4] * scale,)
             for name in ("ret0", "ret1", "ret2", "ret2p", "ret4"):
                 setattr(new, name, getattr(self, name).scaled(scale, scale_values))
             new.r56 = self.r56.scaled(scale, scale_values)
@@ -295,9 +350,15 @@ class PpphPlans:
           r56T:  gtau[(c,d),:] += 1/2 v gN56[(a,k),:]      transposed pair blocks
         """
         if self._adj is None and self.n:
-            c, d, a, k, val = self._coo
             p, h, dev = self.p, self.h, self.device
             ph = p * h
+            if self._coo is None:
+                specs = ppph_specs(p, h)
+                self._adj = {name: SegPlan.on_device(*self._dcoo, *specs[name])
+                             for name in ("ret0T", "ret1T", "ret2T", "ret4T")}
+                self._adj["r56T"] = self.r56.transposed(transpose_blocks)
+                return self._adj
+            c, d, a, k, val = self._coo
             self._adj = {
                 "ret0T": SegPlan((c * p + d) * h + k, a * h, val, lambda u: u * h, dev),
                 "ret1T": SegPlan(c * h + k, a * p + d, val, lambda u: u, dev),

@@ -53,16 +53,26 @@ def test_codes():
     assert np.array_equal(((k[:, 0] * 5 + k[:, 1]) * 5 + k[:, 2]) * 4 + (c >> 24), orbs)
 
 
-def test_segplan_device_sort_matches_numpy():
-    """SegPlan on tensors (the device path, here on CPU tensors) equals the NumPy plan."""
-    from nulattice_amd.plans import SegPlan
-    rng = np.random.default_rng(3)
-    key = rng.integers(0, 50, 2000)
-    src = rng.integers(0, 10 ** 6, 2000)
-    val = rng.standard_normal(2000)
-    a = SegPlan(key, src, val, lambda u: u * 7, "cpu")
-    b = SegPlan(torch.from_numpy(key), torch.from_numpy(src), torch.from_numpy(val),
-                lambda u: u * 7, "cpu")
-    assert a.S == b.S
-    for f in ("ptr", "src", "val", "dst"):
-        assert torch.equal(getattr(a, f), getattr(b, f)), f
+def test_ppph_specs_rebuild_the_host_plans():
+    """The linear key / source / destination forms that nl_segplan_sort and nl_segplan_finish
+    evaluate on the device (ppph_specs), applied here with a stable NumPy sort in their place,
+    give every v_ppph plan (forward and transposed) of the host builder exactly (plane-wave
+    L=3 A=28 list)."""
+    from nulattice_amd.plans import PpphPlans, ppph_specs
+    pw = PlaneWave(3, shell_holes(3, 28), -4.0, 1.0)
+    p, h = pw.p, pw.h
+    idx, val = coo_arrays(pw.coo_ppph(), p, h, "ppph")
+    host = PpphPlans(idx, val, p, h, "cpu")
+    adj = host.adjoint(lambda V: V.transpose(1, 2).contiguous())
+    for name, (wk, ws, sign, kmax, spec) in ppph_specs(p, h).items():
+        ref = adj[name] if name.endswith("T") else getattr(host, name)
+        key, src = idx @ np.array(wk), idx @ np.array(ws)
+        assert key.max() < kmax
+        o = np.argsort(key, kind="stable")
+        uk, start = np.unique(key[o], return_index=True)
+        dst = sum(((uk // dv) % md if md else uk // dv) * mul for dv, md, mul in spec)
+        assert ref.S == uk.size, name
+        assert np.array_equal(ref.ptr.numpy(), np.append(start, key.size)), name
+        assert np.array_equal(ref.src.numpy(), src[o]), name
+        assert np.array_equal(ref.val.numpy(), sign * val[o]), name
+        assert np.array_equal(ref.dst.numpy(), dst), name
