@@ -359,17 +359,20 @@ int nl_sector_dress(int64_t no, const int64_t* cptr, int64_t nchunk, const int64
  * pairs holds 7 int64 per workgroup (offsets of Tr_x and Tr_-x in the crossed layout, of VPH_x and
  * VPH_-x, |R_x|, |R_-x|, self-conjugate flag).  Tr is gathered from the packed amplitudes T
  * through the signed map rmap, W_x = 1/2 VPr_x Tr_-x - VPH_x stays in LDS, and
- * S_x = Tr_x W_-x is written at the offset of Tr_x. */
-int nl_sector_ring(int64_t npair, const int64_t* pairs, const int64_t* rmap, const double* T,
-                   const double* VPr, const double* VPH, double* S, void* stream);
+ * S_x = Tr_x W_-x is written at the offset of Tr_x.  max_rows is the largest |R| in the table:
+ * the kernel holds each sector's images in 32 x 32 LDS tiles, so max_rows > 32 is rejected
+ * (NL_EARG; such layouts take the grouped-GEMM path instead). */
+int nl_sector_ring(int64_t npair, int64_t max_rows, const int64_t* pairs, const int64_t* rmap,
+                   const double* T, const double* VPr, const double* VPH, double* S,
+                   void* stream);
 
 /* Reverse sweep of nl_sector_ring for the Lambda equations of the sector form (no reference
  * counterpart; the transpose of the ring terms of t2Iter): from the adjoint gS of Spp in the
  * crossed layout, gTr_x = gS_x W_-x^T + 1/2 VPr_-x^T (Tr_x^T gS_x) for every crossed sector,
- * written at the offsets of Tr_x (W recomputed in LDS). */
-int nl_sector_ring_adj(int64_t npair, const int64_t* pairs, const int64_t* rmap, const double* T,
-                       const double* VPr, const double* VPH, const double* gS, double* gTr,
-                       void* stream);
+ * written at the offsets of Tr_x (W recomputed in LDS).  Same 32-pair limit (max_rows). */
+int nl_sector_ring_adj(int64_t npair, int64_t max_rows, const int64_t* pairs,
+                       const int64_t* rmap, const double* T, const double* VPr, const double* VPH,
+                       const double* gS, double* gTr, void* stream);
 
 /* out[e] += alpha * (signed in[idx[e]]) (signed map as above): the reverse sweep's gathers. */
 int nl_sector_gather_add(int64_t n, const int64_t* idx, const double* in, double alpha,

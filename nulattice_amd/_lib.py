@@ -80,8 +80,8 @@ SIGNATURES = {
     "nl_sector_gather": (_i32, [_i64, _vp, _vp, _vp, _vp]),
     "nl_sector_scatter": (_i32, [_i64, _vp, _vp, _vp, _vp]),
     "nl_sector_dress": (_i32, [_i64, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _dbl, _vp, _vp, _vp]),
-    "nl_sector_ring": (_i32, [_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
-    "nl_sector_ring_adj": (_i32, [_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "nl_sector_ring": (_i32, [_i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "nl_sector_ring_adj": (_i32, [_i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "nl_sector_gather_add": (_i32, [_i64, _vp, _vp, _dbl, _vp, _vp]),
     "nl_sector_diag_axpy": (_i32, [_i64, _i64, _vp, _vp, _vp, _dbl, _vp, _vp]),
     "nl_contact_fill": (_i32, [_i64, _vp, _vp, _vp, _vp, _vp, _vp, _i32, _dbl, _i64, _vp, _vp]),

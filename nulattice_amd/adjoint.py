@@ -349,8 +349,8 @@ class PlaneWaveCCSDEnergy(torch.autograd.Function):
             if info is not None:
                 info.update(E_ref=E0, E_corr=e)
             return torch.tensor(E0 + e, dtype=F64)
-        E0, fock, v = pw.hamiltonian(sparse=True)
-        ham = engine.Hamiltonian(fock, v, device=dev)
+        E0, fock, ham = pw.device_hamiltonian(dev)       # blocks and plans written on the GPU
+        v = None
         e, t1, t2 = _solve(ccm, ham, fock, v, opts, info, "forward")
         keep = not opts.get("recompute", True)
         ctx.state = dict(pw=pw, ham=ham, fock=fock, v=v, opts=opts, e=e,

@@ -316,6 +316,7 @@ __global__ __launch_bounds__(NT) void sector_ring_kernel(const int64_t* __restri
   __shared__ double Wx[RR][RR + 1], Wy[RR][RR + 1];
   const int64_t* q = pairs + 7 * (int64_t)blockIdx.x;
   const int64_t rx = q[0], ry = q[1], wx = q[2], wy = q[3];
+  // the host checks |R| <= RR (max_rows); the clamp only keeps a bad table inside the LDS arrays
   const int n1 = (int)min(q[4], (int64_t)RR), n2 = (int)min(q[5], (int64_t)RR);
   const bool self = q[6] != 0;
   const int tid = threadIdx.x;
@@ -396,6 +397,7 @@ __global__ __launch_bounds__(NT) void sector_ring_adj_kernel(const int64_t* __re
   __shared__ double Hx[RR][RR + 1], Hy[RR][RR + 1];          // gW_x, gW_y
   const int64_t* q = pairs + 7 * (int64_t)blockIdx.x;
   const int64_t rx = q[0], ry = q[1], wx = q[2], wy = q[3];
+  // the host checks |R| <= RR (max_rows); the clamp only keeps a bad table inside the LDS arrays
   const int n1 = (int)min(q[4], (int64_t)RR), n2 = (int)min(q[5], (int64_t)RR);
   const bool self = q[6] != 0;
   const int tid = threadIdx.x;
@@ -739,9 +741,10 @@ int nl_sector_dress(int64_t no, const int64_t* cptr, int64_t nchunk, const int64
   return NL_OK;
 }
 
-int nl_sector_ring(int64_t npair, const int64_t* pairs, const int64_t* rmap, const double* T,
-                   const double* VPr, const double* VPH, double* S, void* stream) {
-  if (npair < 0 || npair > INT32_MAX) return NL_EARG;
+int nl_sector_ring(int64_t npair, int64_t max_rows, const int64_t* pairs, const int64_t* rmap,
+                   const double* T, const double* VPr, const double* VPH, double* S,
+                   void* stream) {
+  if (npair < 0 || npair > INT32_MAX || max_rows < 0 || max_rows > RR) return NL_EARG;
   if (npair == 0) return NL_OK;
   if (!pairs || !rmap || !T || !VPr || !VPH || !S) return NL_EARG;
   hipLaunchKernelGGL(sector_ring_kernel, dim3((unsigned)npair), dim3(NT), 0, (hipStream_t)stream,
@@ -750,10 +753,10 @@ int nl_sector_ring(int64_t npair, const int64_t* pairs, const int64_t* rmap, con
   return NL_OK;
 }
 
-int nl_sector_ring_adj(int64_t npair, const int64_t* pairs, const int64_t* rmap, const double* T,
-                       const double* VPr, const double* VPH, const double* gS, double* gTr,
-                       void* stream) {
-  if (npair < 0 || npair > INT32_MAX) return NL_EARG;
+int nl_sector_ring_adj(int64_t npair, int64_t max_rows, const int64_t* pairs,
+                       const int64_t* rmap, const double* T, const double* VPr, const double* VPH,
+                       const double* gS, double* gTr, void* stream) {
+  if (npair < 0 || npair > INT32_MAX || max_rows < 0 || max_rows > RR) return NL_EARG;
   if (npair == 0) return NL_OK;
   if (!pairs || !rmap || !T || !VPr || !VPH || !gS || !gTr) return NL_EARG;
   hipLaunchKernelGGL(sector_ring_adj_kernel, dim3((unsigned)npair), dim3(NT), 0,

@@ -274,7 +274,8 @@ class SectorLayout:
         first = xk <= nk
         self.ring_pairs = np.stack([ro_x, ro_n, wo_x, wo_n, Rx, Rn, (xk == nk).astype(np.int64)],
                                    1)[first].astype(np.int64).reshape(-1, 7)
-        self.ring_fused = bool(len(Rx) == 0 or max(Rx.max(), Rn.max()) <= 32)
+        self.ring_rows = int(max(Rx.max(), Rn.max())) if len(Rx) else 0
+        self.ring_fused = self.ring_rows <= 32
         # reverse-sweep maps: the packed element (signed) whose assembly reads each Spp position,
         # and the four signed crossed-image positions of each packed element
         self.sg_inv = np.full(self.nring, -1, np.int64)
@@ -635,7 +636,8 @@ class SectorHamiltonian:
         """Crossed-channel image of t2, W = 1/2 v_pphh t2 - v_phph, Spp = t2 W (one fused
         launch when every crossed sector has at most 32 pairs)."""
         if self.ring_pairs is not None:
-            _lib.call("nl_sector_ring", self.layout.ring_pairs.shape[0], ops._p(self.ring_pairs),
+            _lib.call("nl_sector_ring", self.layout.ring_pairs.shape[0], self.layout.ring_rows,
+                      ops._p(self.ring_pairs),
                       ops._p(self.ring_map), ops._p(t2), ops._p(self.VPr), ops._p(self.VPHPH),
                       ops._p(Bf["S"]), ops._stream())
             return
@@ -819,7 +821,7 @@ def residual_vjp(SH, t2, lam, Bf, out):
     if SH.ring_pairs is None:
         raise _lib.NLError("the sector reverse sweep needs crossed sectors of at most 32 pairs")
     SH._gather(SH.sg_inv, lam, Bf["gS"])
-    _lib.call("nl_sector_ring_adj", lay.ring_pairs.shape[0], ops._p(SH.ring_pairs),
+    _lib.call("nl_sector_ring_adj", lay.ring_pairs.shape[0], lay.ring_rows, ops._p(SH.ring_pairs),
               ops._p(SH.ring_map), ops._p(t2), ops._p(SH.VPr), ops._p(SH.VPHPH), ops._p(Bf["gS"]),
               ops._p(Bf["gTr"]), ops._stream())
     for m in SH.ring_inv:

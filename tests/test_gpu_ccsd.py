@@ -116,6 +116,23 @@ def test_planewave_L3_A4_matches_oracle():
     np.testing.assert_allclose(t2h, -t2h.transpose(1, 0, 2, 3), atol=1e-15)
 
 
+def test_planewave_L4_A28_matches_oracle_fixture():
+    """Plane-wave L=4, A=28 (256 orbitals, p=228, h=28; C0=-4, D0=1): the GPU FP64 iteration on
+    the Hamiltonian written on the GPU follows the oracle's first 20 steps
+    (tests/golden/planewave_L4_A28.json, written by tests/golden/make_planewave_L4.py) to 1e-10
+    relative per step, from the same E_ref."""
+    from nulattice_amd.planewave import planewave_device_hamiltonian
+    g = json.load(open(os.path.join(os.path.dirname(__file__), "golden",
+                                    "planewave_L4_A28.json")))
+    E0, fock, H = planewave_device_hamiltonian(g["L"], g["A"], g["C0"], g["D0"])
+    assert abs(E0 - g["E_ref"]) <= 1e-13 * abs(g["E_ref"])
+    hist = []
+    ccm.ccsd_solver(None, None, eps=1e-14, maxSteps=g["steps"], max_diis=g["max_diis"],
+                    mixing=g["mixing"], hamiltonian=H, history=hist)
+    assert len(hist) == len(g["energies"]) == g["steps"] + 1
+    np.testing.assert_allclose(hist, g["energies"], rtol=1e-10)
+
+
 @pytest.mark.parametrize("prec", ["bf16", "fp16"])
 def test_precision_form_refines_to_fp64(prec):
     """Low-precision contractions converge near the FP64 fixed point; the FP64 refinement lands
