@@ -122,10 +122,8 @@ def test_planewave_L4_A28_matches_oracle_fixture():
     (tests/golden/planewave_L4_A28.json, written by tests/golden/make_planewave_L4.py) to 1e-10
     relative per step, from the same E_ref."""
     from nulattice_amd.planewave import planewave_device_hamiltonian
-    path = os.path.join(os.path.dirname(__file__), "golden", "planewave_L4_A28.json")
-    if not os.path.exists(path):
-        pytest.skip("fixture not generated yet (tests/golden/make_planewave_L4.py)")
-    g = json.load(open(path))
+    g = json.load(open(os.path.join(os.path.dirname(__file__), "golden",
+                                    "planewave_L4_A28.json")))
     E0, fock, H = planewave_device_hamiltonian(g["L"], g["A"], g["C0"], g["D0"])
     assert abs(E0 - g["E_ref"]) <= 1e-13 * abs(g["E_ref"])
     hist = []
