@@ -1026,7 +1026,7 @@ __device__ inline void mfma_block(v4f (&acc)[8][4], const uint4 (&a)[8], const u
     }
 }
 
-template <bool IS_BF16, int PA, int PB, int NS>
+template <bool IS_BF16, int PA, int PB, int NS, bool ZFILL = false>
 __global__ __launch_bounds__(512) void gemm_lp_split_kernel(GemmLPArgs g) {
   constexpr int NP = PA + PB;
   constexpr int LPS = 2 * NP;          // DMA instructions per wave per stage
@@ -1070,12 +1070,13 @@ __global__ __launch_bounds__(512) void gemm_lp_split_kernel(GemmLPArgs g) {
         const int r0 = w * 32 + hh * 16, r = r0 + drow;
         const int c = dpos ^ ((r >> 2) & 3);
         const int row = min(o0 + r, lim);
-        if (NS > 2 || k0 + c * 8 < kend) {
+        if (!ZFILL || k0 + c * 8 < kend) {
           lds_dma16(base + (int64_t)row * ld + k0 + c * 8, &T[q][r0][0]);
         } else {
-          // two stages only: the last step of a K range that is not a multiple of 32 is
-          // zero-filled beyond it (every wait there is a full one, so the count of DMA
-          // instructions does not matter)
+          // (ZFILL: two stages only, K not a multiple of 32) the last step of the K range is
+          // zero-filled beyond it; every wait there is a full one, so the count of DMA
+          // instructions does not matter.  Without ZFILL no lane branches here, which keeps
+          // the per-step issue free of exec-mask changes.
           *(uint4*)&T[q][r][dpos * 8] = make_uint4(0, 0, 0, 0);
         }
       }
@@ -1684,18 +1685,22 @@ static int gemm_lp_impl(int dtype, int64_t M, int64_t N, int64_t K, double alpha
     g.tiles_m = (int)nl_cdiv(M, BT); g.tiles_n = (int)nl_cdiv(N, BT);
     grid = dim3((unsigned)(g.tiles_m * g.tiles_n), (unsigned)splits, (unsigned)batch);
     const int pa = Alo ? 2 : 1, pb = Blo ? 2 : 1;
-#define NL_SPLIT(BF, PA_, PB_, NS_)                                                          \
-  hipLaunchKernelGGL((gemm_lp_split_kernel<BF, PA_, PB_, NS_>), grid, dim3(512), 0, st, g)
+#define NL_SPLIT(BF, PA_, PB_, NS_, ZF_)                                                     \
+  hipLaunchKernelGGL((gemm_lp_split_kernel<BF, PA_, PB_, NS_, ZF_>), grid, dim3(512), 0, st, g)
+    // only the two-stage form takes a K that is not a multiple of 32 (zero-filled last step)
+    const bool zf = K % SBK != 0;
     if (dtype == 0) {
-      if (pa == 2 && pb == 2) NL_SPLIT(true, 2, 2, 2);
-      else if (pa == 2) NL_SPLIT(true, 2, 1, 3);
-      else if (pb == 2) NL_SPLIT(true, 1, 2, 3);
-      else NL_SPLIT(true, 1, 1, 4);
+      if (pa == 2 && pb == 2) {
+        if (zf) NL_SPLIT(true, 2, 2, 2, true); else NL_SPLIT(true, 2, 2, 2, false);
+      } else if (pa == 2) NL_SPLIT(true, 2, 1, 3, false);
+      else if (pb == 2) NL_SPLIT(true, 1, 2, 3, false);
+      else NL_SPLIT(true, 1, 1, 4, false);
     } else {
-      if (pa == 2 && pb == 2) NL_SPLIT(false, 2, 2, 2);
-      else if (pa == 2) NL_SPLIT(false, 2, 1, 3);
-      else if (pb == 2) NL_SPLIT(false, 1, 2, 3);
-      else NL_SPLIT(false, 1, 1, 4);
+      if (pa == 2 && pb == 2) {
+        if (zf) NL_SPLIT(false, 2, 2, 2, true); else NL_SPLIT(false, 2, 2, 2, false);
+      } else if (pa == 2) NL_SPLIT(false, 2, 1, 3, false);
+      else if (pb == 2) NL_SPLIT(false, 1, 2, 3, false);
+      else NL_SPLIT(false, 1, 1, 4, false);
     }
 #undef NL_SPLIT
   } else if (split_ops) {
