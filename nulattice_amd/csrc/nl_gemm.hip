@@ -878,7 +878,7 @@ __device__ inline void big_step(const BigTile& T, v4f (&acc)[8][4], int wm, int 
   }
 }
 
-template <bool IS_BF16>
+template <bool IS_BF16, bool KAL = false>
 __global__ __launch_bounds__(512) void gemm_lp_big_kernel(GemmLPArgs g) {
   // two distinct LDS objects (not one array indexed by a step parity), so the compiler can tell
   // the buffer being read from the buffer the in-flight loads are filling
@@ -905,6 +905,29 @@ __global__ __launch_bounds__(512) void gemm_lp_big_kernel(GemmLPArgs g) {
   const int drow = lane >> 3, dpos = lane & 7;
   // wave w fills rows [32 w, 32 w + 32) of both operand tiles, 4 instructions of 8 rows each
   auto issue = [&](BigTile& T, int k0) {
+    if constexpr (KAL) {
+      // K range a multiple of 64: a step is wholly inside or wholly beyond it (a uniform
+      // branch), and rows beyond M or N read a clamped row, whose products reach only outputs
+      // that are not stored, so no lane branches around a DMA
+      if (k0 < kend) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int r0 = w * 32 + q * 8, r = r0 + drow;
+          const int k = k0 + (dpos ^ ((r >> 1) & 7)) * 8;
+          const int m = min(m0 + r, g.M - 1), n = min(n0 + r, g.N - 1);
+          lds_dma16(A + (int64_t)m * g.lda + k, &T[0][r0][0]);
+          lds_dma16(B + (int64_t)n * g.ldb + k, &T[1][r0][0]);
+        }
+      } else {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int r = w * 32 + q * 8 + drow;
+          *(uint4*)&T[0][r][dpos * 8] = make_uint4(0, 0, 0, 0);
+          *(uint4*)&T[1][r][dpos * 8] = make_uint4(0, 0, 0, 0);
+        }
+      }
+      return;
+    }
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const int r0 = w * 32 + q * 8, r = r0 + drow;         // tile row
@@ -1721,10 +1744,15 @@ static int gemm_lp_impl(int dtype, int64_t M, int64_t N, int64_t K, double alpha
   } else if (big) {
     g.tiles_m = (int)nl_cdiv(M, BT); g.tiles_n = (int)nl_cdiv(N, BT);
     grid = dim3((unsigned)(g.tiles_m * g.tiles_n), (unsigned)splits, (unsigned)batch);
-    if (dtype == 0)
-      hipLaunchKernelGGL((gemm_lp_big_kernel<true>), grid, dim3(512), 0, st, g);
-    else
-      hipLaunchKernelGGL((gemm_lp_big_kernel<false>), grid, dim3(512), 0, st, g);
+    // every split's K range a multiple of 64 (kchunk is): steps never straddle its end
+    const bool kal = K % BKD == 0;
+    if (dtype == 0) {
+      if (kal) hipLaunchKernelGGL((gemm_lp_big_kernel<true, true>), grid, dim3(512), 0, st, g);
+      else hipLaunchKernelGGL((gemm_lp_big_kernel<true, false>), grid, dim3(512), 0, st, g);
+    } else {
+      if (kal) hipLaunchKernelGGL((gemm_lp_big_kernel<false, true>), grid, dim3(512), 0, st, g);
+      else hipLaunchKernelGGL((gemm_lp_big_kernel<false, false>), grid, dim3(512), 0, st, g);
+    }
   } else if (dtype == 0) {
     hipLaunchKernelGGL((gemm_lp_kernel<true>), grid, dim3(256), 0, st, g);
   } else {

@@ -129,7 +129,8 @@ def test_gemm_lp_exact_on_small_integers(dt):
 
 @pytest.mark.parametrize("dt", [0, 1])
 @pytest.mark.parametrize("M,N,K,splits", [(4000, 4100, 4136, 1), (4096, 4352, 1024, 1),
-                                          (1024, 1000, 4104, 16)])
+                                          (1024, 1000, 4104, 16), (4000, 4100, 4160, 1),
+                                          (1000, 1030, 2048, 16)])
 def test_gemm_lp_large_tile_path_exact(dt, M, N, K, splits):
     """Shapes that take the 256x256-tile kernel (LDS DMA, swizzled LDS rows): edge tiles in M
     and N, a partial last K step and an odd step count (K = 4136 = 64.6 x 64), split-K, per-row
