@@ -1701,9 +1701,12 @@ static int gemm_lp_impl(int dtype, int64_t M, int64_t N, int64_t K, double alpha
   hipStream_t st = (hipStream_t)stream;
   // 256x256 tiles when they alone give every CU a workgroup, else 128x128 tiles
   const int64_t big_tiles = nl_cdiv(M, BT) * nl_cdiv(N, BT);
-  const bool big = M >= BT && N >= BT && big_tiles * splits * batch >= 256;
+  const bool fills = big_tiles * splits * batch >= 256;
+  const bool big = M >= BT && N >= BT && fills;
   const bool split_ops = Alo || Blo;
-  if (big && (split_ops || staged) && (K % SBK == 0 || (Alo && Blo))) {
+  // the split kernel clamps its operand rows, so M or N below one tile is fine there (the many
+  // small pair blocks of the pp ladder, batched) as long as the tiles fill the GPU
+  if (fills && (split_ops || staged) && (K % SBK == 0 || (Alo && Blo))) {
     // hi/lo operands (or the staged pipeline on request): the multi-stage split kernel
     g.tiles_m = (int)nl_cdiv(M, BT); g.tiles_n = (int)nl_cdiv(N, BT);
     grid = dim3((unsigned)(g.tiles_m * g.tiles_n), (unsigned)splits, (unsigned)batch);

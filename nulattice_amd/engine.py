@@ -465,7 +465,8 @@ def _block_gemm(pb, src, out, alpha, ncol, tag, precision="fp64", src16=None):
     else:
         dt, split = ops.lp_kind(precision)
         nblk = 2 if split else 1
-        kpad = (pb.kb + 7) // 8 * 8
+        # split images pad K to 32, the split GEMM's step (zero columns contribute nothing)
+        kpad = (pb.kb + 31) // 32 * 32 if split else (pb.kb + 7) // 8 * 8
         ldk = nblk * kpad
         lp = getattr(pb, "lp", None)
         if lp is None or lp[0] != (dt, split):
@@ -502,9 +503,17 @@ def _block_gemm(pb, src, out, alpha, ncol, tag, precision="fp64", src16=None):
                 _lib.call("nl_gather_convert", pb.nb, pb.kb, ncol, ops._p(pb.cols), ops._p(src),
                           ncol, ops._p(Gt), ldk, st)
             a_eff = alpha / AMP_SCALE[dt] if split16 else alpha
-            _lib.call("nl_gemm_lp_rows", dt, pb.mb, ncol, ldk, ctypes.c_double(a_eff),
-                      ops._p(Vb), ldk, pb.mb * ldk, ops._p(iv), pb.mb, ops._p(Gt), ldk,
-                      ncol * ldk, None, 0, ops._p(out), ncol, ops._p(pb.rows), pb.nb, st)
+            if split16:
+                # V (exact, its first K block) x [hi | lo] of the gathered rows: one split GEMM
+                # with the row-scatter epilogue (V.hi + V.lo per K step)
+                _lib.call("nl_gemm_lp_rows_split", dt, pb.mb, ncol, kpad,
+                          ctypes.c_double(a_eff), ops._p(Vb), None, ldk, pb.mb * ldk,
+                          ops._p(iv), pb.mb, ops._p(Gt), ops._off16(Gt, kpad), ldk, ncol * ldk,
+                          None, 0, ops._p(out), ncol, ops._p(pb.rows), pb.nb, st)
+            else:
+                _lib.call("nl_gemm_lp_rows", dt, pb.mb, ncol, ldk, ctypes.c_double(a_eff),
+                          ops._p(Vb), ldk, pb.mb * ldk, ops._p(iv), pb.mb, ops._p(Gt), ldk,
+                          ncol * ldk, None, 0, ops._p(out), ncol, ops._p(pb.rows), pb.nb, st)
             if ev:
                 ev[1].record()
             return
@@ -518,9 +527,18 @@ def _block_gemm(pb, src, out, alpha, ncol, tag, precision="fp64", src16=None):
         else:
             _lib.call("nl_convert_rows", dt, pb.nb, ncol, 1, pb.kb, s4, ops._p(G), ops._p(Gt),
                       ldk, ops._p(ig), ctypes.c_float(target), st)
-        _lib.call("nl_gemm_lp", dt, pb.mb, ncol, ldk, ctypes.c_double(1.0), ops._p(Vb), ldk,
-                  pb.mb * ldk, ops._p(iv), pb.mb, ops._p(Gt), ldk, ncol * ldk, ops._p(ig), ncol,
-                  ctypes.c_double(0.0), None, ops._p(R), ncol, pb.mb * ncol, pb.nb, 1, None, st)
+        if split:
+            # V (exact, its first K block) x [hi | lo]: the split GEMM, as in the shared-image path
+            _lib.call("nl_gemm_lp_split", dt, pb.mb, ncol, kpad, ctypes.c_double(1.0),
+                      ops._p(Vb), None, ldk, pb.mb * ldk, ops._p(iv), pb.mb, ops._p(Gt),
+                      ops._off16(Gt, kpad), ldk, ncol * ldk, ops._p(ig), ncol,
+                      ctypes.c_double(0.0), None, ops._p(R), ncol, pb.mb * ncol, pb.nb, 1, None,
+                      st)
+        else:
+            _lib.call("nl_gemm_lp", dt, pb.mb, ncol, ldk, ctypes.c_double(1.0), ops._p(Vb), ldk,
+                      pb.mb * ldk, ops._p(iv), pb.mb, ops._p(Gt), ldk, ncol * ldk, ops._p(ig),
+                      ncol, ctypes.c_double(0.0), None, ops._p(R), ncol, pb.mb * ncol, pb.nb, 1,
+                      None, st)
     _lib.call("nl_scatter_add_rows", pb.nb * pb.mb, ncol, ops._p(pb.rows), ops._p(R),
               ctypes.c_double(alpha), ops._p(out), ncol, st)
     if ev:
