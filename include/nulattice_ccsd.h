@@ -134,6 +134,15 @@ int nl_gather_rows16_into(int64_t nb, int64_t kb, int64_t ncol, const int64_t* i
                           const void* src, int64_t lds, void* out, int64_t ldk, int64_t ldo,
                           void* stream);
 
+/* Both parts of split rows in one pass: the source rows are [hi(ncol) | lo(ncol)] (row pitch
+ * lds) and part q of gathered row (b, k) lands transposed at out[(b*ncol + n)*ldo + q*kpad + k]
+ * (k < kb; zeros for k in [kb, kpad) and for idx < 0): the [hi(kpad) | lo(kpad)] rows of the
+ * pair-block products' right operand (ccDgrams.py:215-237 and :13-45 regrouped by pair
+ * blocks), as two nl_gather_rows16_into calls would write them. */
+int nl_gather_rows16_split(int64_t nb, int64_t kb, int64_t ncol, const int64_t* idx,
+                           const void* src, int64_t lds, void* out, int64_t kpad, int64_t ldo,
+                           void* stream);
+
 /* nl_convert_rows writing a hi/lo split: with y = x * scale * (per-row scale as in
  * nl_convert_rows; target 0: none), hi = 16-bit(y), lo = 16-bit(y - hi), so hi + lo carries ~17 (BF16) or ~23
  * (FP16) significant bits.  Row r gets nblk blocks of kpad columns, block q at

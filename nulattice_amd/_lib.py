@@ -87,6 +87,7 @@ SIGNATURES = {
     "nl_contact_fill": (_i32, [_i64, _vp, _vp, _vp, _vp, _vp, _vp, _i32, _dbl, _i64, _vp, _vp]),
     "nl_sector_assemble": (_i32, [_i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _dbl, _dbl, _vp,
                                   _vp, _vp, _vp]),
+    "nl_gather_rows16_split": (_i32, [_i64, _i64, _i64, _vp, _vp, _i64, _vp, _i64, _i64, _vp]),
     "nl_segplan_ws": (_i64, [_i64]),
     "nl_segplan_sort": (_i32, [_i64, _vp, _vp, _vp, _vp, _dbl, _i32, _vp, _i64, _vp, _vp, _vp,
                                _vp]),

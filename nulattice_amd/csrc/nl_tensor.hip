@@ -424,18 +424,23 @@ __global__ __launch_bounds__(NT) void transpose16_kernel(int64_t R, int64_t C, c
 // thread reads 4 consecutive n (8 bytes) of one source row, and writes 4 consecutive k of one
 // output row (8 bytes) from the LDS tile.  Needs ncol % 4 == 0, lds % 4 == 0 and 8-byte aligned
 // rows (checked on the host).
+// NPART = 2: the source rows are [hi(ncol) | lo(ncol)] and part q of column n lands at
+// out[(b*ncol + n)*ldo + q*kpart + k] (the [hi(kpart) | lo(kpart)] rows of a split operand).
+template <int NPART>
 __global__ __launch_bounds__(NT) void gather16_kernel(int64_t ncol, int64_t kb, int64_t ldk,
                                                       const int64_t* idx, const uint16_t* src,
-                                                      int64_t lds, uint16_t* out, int64_t ldo) {
+                                                      int64_t lds, uint16_t* out, int64_t ldo,
+                                                      int64_t kpart) {
   __shared__ uint16_t tile[64][256 + 4];
   const int b = blockIdx.z, tid = threadIdx.x;
   const int64_t n0 = (int64_t)blockIdx.x * 256, k0 = (int64_t)blockIdx.y * 64;
+  const int64_t nall = NPART * ncol;
   const int nl = (tid & 63) * 4;                 // 4 n per thread, 64 threads per k row
 #pragma unroll 4
   for (int kk = tid >> 6; kk < 64; kk += 4) {
     const int64_t k = k0 + kk, n = n0 + nl;
     uint2 v = make_uint2(0u, 0u);
-    if (k < kb && n < ncol) {
+    if (k < kb && n < nall) {
       const int64_t row = idx[(int64_t)b * kb + k];
       if (row >= 0) v = *(const uint2*)(src + row * lds + n);
     }
@@ -445,12 +450,15 @@ __global__ __launch_bounds__(NT) void gather16_kernel(int64_t ncol, int64_t kb, 
   const int kq = (tid & 15) * 4;                 // 4 k per thread, 16 threads per n row
 #pragma unroll 4
   for (int nn = tid >> 4; nn < 256; nn += 16) {
-    const int64_t n = n0 + nn, k = k0 + kq;
-    if (n >= ncol || k >= ldk) continue;
+    int64_t n = n0 + nn;
+    const int64_t k = k0 + kq;
+    if (n >= nall || k >= ldk) continue;
+    int64_t o = k;
+    if (NPART == 2 && n >= ncol) { n -= ncol; o += kpart; }
     uint16_t w[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) w[u] = tile[kq + u][nn];
-    *(uint2*)(out + ((int64_t)b * ncol + n) * ldo + k) =
+    *(uint2*)(out + ((int64_t)b * ncol + n) * ldo + o) =
         make_uint2(w[0] | ((unsigned)w[1] << 16), w[2] | ((unsigned)w[3] << 16));
   }
 }
@@ -1138,8 +1146,8 @@ static int gather_rows16_impl(int64_t nb, int64_t kb, int64_t ncol, const int64_
   if (ncol % 4 == 0 && lds % 4 == 0 && ldo % 4 == 0 && (((uintptr_t)src) & 7) == 0 &&
       (((uintptr_t)out) & 7) == 0 && nb <= 65535) {
     dim3 g((unsigned)nl_cdiv(ncol, 256), (unsigned)nl_cdiv(ldk, 64), (unsigned)nb);
-    hipLaunchKernelGGL(gather16_kernel, g, dim3(NT), 0, (hipStream_t)stream, ncol, kb, ldk, idx,
-                       (const uint16_t*)src, lds, (uint16_t*)out, ldo);
+    hipLaunchKernelGGL(gather16_kernel<1>, g, dim3(NT), 0, (hipStream_t)stream, ncol, kb, ldk,
+                       idx, (const uint16_t*)src, lds, (uint16_t*)out, ldo, (int64_t)0);
     NL_LAUNCH_CHECK();
     return NL_OK;
   }
@@ -1160,6 +1168,25 @@ int nl_gather_rows16_into(int64_t nb, int64_t kb, int64_t ncol, const int64_t* i
                           const void* src, int64_t lds, void* out, int64_t ldk, int64_t ldo,
                           void* stream) {
   return gather_rows16_impl(nb, kb, ncol, idx, src, lds, out, ldk, ldo, stream);
+}
+
+int nl_gather_rows16_split(int64_t nb, int64_t kb, int64_t ncol, const int64_t* idx,
+                           const void* src, int64_t lds, void* out, int64_t kpad, int64_t ldo,
+                           void* stream) {
+  if (nb < 0 || kb < 0 || ncol < 0 || kpad < kb || (kpad % 8) || lds < 2 * ncol ||
+      ldo < 2 * kpad)
+    return NL_EARG;
+  if (nb * ncol == 0 || kpad == 0) return NL_OK;
+  if (!idx || !src || !out || nb > 65535 || nb * ncol >= INT32_MAX ||
+      nl_cdiv(kpad, 64) > 65535)
+    return NL_EARG;
+  if (ncol % 4 || lds % 4 || ldo % 4 || (((uintptr_t)src) & 7) || (((uintptr_t)out) & 7))
+    return NL_EARG;
+  dim3 g((unsigned)nl_cdiv(2 * ncol, 256), (unsigned)nl_cdiv(kpad, 64), (unsigned)nb);
+  hipLaunchKernelGGL(gather16_kernel<2>, g, dim3(NT), 0, (hipStream_t)stream, ncol, kb, kpad,
+                     idx, (const uint16_t*)src, lds, (uint16_t*)out, ldo, kpad);
+  NL_LAUNCH_CHECK();
+  return NL_OK;
 }
 
 int nl_tau_split(int64_t p, int64_t h, const double* t1, const double* t2, double* tau,

@@ -492,10 +492,8 @@ def _block_gemm(pb, src, out, alpha, ncol, tag, precision="fp64", src16=None):
             # BF16: gather + transposing cast in one pass, block rows added straight into out
             # (split: the hi and lo halves of the gathered rows, [V | V] x [hi | lo])
             if split16:
-                for q in range(2):
-                    _lib.call("nl_gather_rows16_into", pb.nb, pb.kb, ncol, ops._p(pb.cols),
-                              ops._off16(src16, q * ncol), 2 * ncol, ops._off16(Gt, q * kpad),
-                              kpad, ldk, st)
+                _lib.call("nl_gather_rows16_split", pb.nb, pb.kb, ncol, ops._p(pb.cols),
+                          ops._p(src16), 2 * ncol, ops._p(Gt), kpad, ldk, st)
             elif src16 is not None:
                 _lib.call("nl_gather_rows16", pb.nb, pb.kb, ncol, ops._p(pb.cols), ops._p(src16),
                           ncol, ops._p(Gt), ldk, st)

@@ -325,3 +325,32 @@ def test_precision_form_L5_A28_meets_1e6():
         el, _, _ = ccm.ccsd_solver(fock, v, eps=1e-9, maxSteps=60, max_diis=0, hamiltonian=H,
                                    precision=prec, refine=False)
         print(f"  {prec}: |dE|/|E| = {abs(el - e64) / abs(e64):.3e}")
+
+
+@pytest.mark.parametrize("nb,kb,ncol", [(3, 37, 40), (5, 236, 784), (1, 1, 4)])
+def test_gather_rows16_split_equals_two_gathers(nb, kb, ncol):
+    """nl_gather_rows16_split (both parts of [hi | lo] rows in one pass) writes exactly what two
+    nl_gather_rows16_into calls write: the transposed gathered rows at part offsets 0 and kpad,
+    zeros for padding rows (idx < 0) and for k in [kb, kpad)."""
+    g = torch.Generator(device="cuda").manual_seed(nb * 1000 + kb)
+    nsrc = 3 * kb + 5
+    src = torch.randint(-30000, 30000, (nsrc, 2 * ncol), device="cuda", generator=g,
+                        dtype=torch.int16)
+    idx = torch.randint(-1, nsrc, (nb * kb,), device="cuda", generator=g)
+    kpad = (kb + 31) // 32 * 32
+    ldk = 2 * kpad
+    a = torch.full((nb * ncol * ldk,), 7, dtype=torch.int16, device="cuda")
+    b = a.clone()
+    st = ops._stream()
+    _lib.call("nl_gather_rows16_split", nb, kb, ncol, ops._p(idx), ops._p(src), 2 * ncol,
+              ops._p(a), kpad, ldk, st)
+    for q in range(2):
+        _lib.call("nl_gather_rows16_into", nb, kb, ncol, ops._p(idx), ops._off16(src, q * ncol),
+                  2 * ncol, ops._off16(b, q * kpad), kpad, ldk, st)
+    torch.cuda.synchronize()
+    assert torch.equal(a, b)
+    # and the content: part q of column n of block bb, row k = src[idx[bb*kb + k], q*ncol + n]
+    bb, n, q, k = nb - 1, ncol - 1, 1, kb - 1
+    r = int(idx[bb * kb + k])
+    want = 0 if r < 0 else int(src[r, q * ncol + n])
+    assert int(a[(bb * ncol + n) * ldk + q * kpad + k]) == want
