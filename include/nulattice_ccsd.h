@@ -20,6 +20,18 @@
 extern "C" {
 #endif
 
+/* Element types of a product's output C and addend D ("io" flags): double unless the flag is
+ * set, then float.  The precision form keeps its T2-sized accumulators in FP32, which halves the
+ * bytes of every read-modify-write pass over them (no reference counterpart: the reference
+ * stores everything in FP64).  nl_contract_f64, nl_segsum take them as an argument; the 16-bit
+ * GEMMs (nl_gemm_lp*) as dtype | io << 8. */
+#define NL_C_F32 1
+#define NL_D_F32 2
+/* Flags of nl_assemble_t2_ext: the four accumulators are FP32; v_pphh is added to H2 (S0 does
+ * not hold it). */
+#define NL_ACC_F32 1
+#define NL_ADD_V 2
+
 /* C[b] = alpha * op(A[b]) * op(B[b]) + beta * C[b]  (FP64, MFMA 16x16x4).
  * Replaces every dense opt_einsum pairwise contraction of the diagram library
  * (ccDgrams.py:48-578, coupled_cluster.py:282-284,405-408,528-536) after a layout change.
@@ -39,21 +51,24 @@ int nl_gemm_f64(int a_kcontig, int b_ncontig, int64_t M, int64_t N, int64_t K, d
  * This is how every dense einsum of the diagram library (ccDgrams.py:48-578) runs without an
  * operand transpose.  splits>1 forces the MFMA path with that K split (ws: batch*splits*M*N
  * doubles); splits<=0 chooses automatically between the 128x128 MFMA tiles and the skinny
- * (N<=64 or M<=64) bandwidth kernel and their split, with ws of nl_contract_ws(dims) doubles. */
+ * (N<=64 or M<=64) bandwidth kernel and their split, with ws of nl_contract_ws(dims) doubles.
+ * io: NL_C_F32 / NL_D_F32 for a float C / D (products accumulate in FP64 either way). */
 int64_t nl_contract_ws(const int64_t* dims);
 int nl_contract_f64(const int64_t* dims, double alpha, const double* A, const int64_t* sa,
-                    const double* B, const int64_t* sb, double beta, const double* D, double* C,
-                    const int64_t* sc, int64_t splits, double* ws, void* stream);
+                    const double* B, const int64_t* sb, double beta, const void* D, void* C,
+                    const int64_t* sc, int64_t splits, double* ws, int io, void* stream);
 
-/* C[b] (FP64) = alpha * diag(ia) (A[b] B[b]^T) diag(ib) + beta * D[b] with BF16 (dtype 0) or
+/* C[b] = alpha * diag(ia) (A[b] B[b]^T) diag(ib) + beta * D[b] with BF16 (dtype 0) or
  * FP16 (dtype 1) operands, FP32 accumulation (MFMA 16x16x32).  The precision form of the ring
  * and ladder contractions (ccDgrams.py:240-303); no reference counterpart for the precision.
  * K, lda, ldb multiples of 8; A, B 16-byte aligned.  D (addend, C's layout) may be C or null.
+ * C and D are double unless dtype carries NL_C_F32 << 8 / NL_D_F32 << 8 (float), in every
+ * nl_gemm_lp* entry point.
  * splits>1 splits K (FP64 partials in ws, batch*splits*M*N doubles, fixed-order reduction). */
 int nl_gemm_lp(int dtype, int64_t M, int64_t N, int64_t K, double alpha, const void* A,
                int64_t lda, int64_t strideA, const float* ia, int64_t strideIa, const void* B,
                int64_t ldb, int64_t strideB, const float* ib, int64_t strideIb, double beta,
-               const double* D, double* C, int64_t ldc, int64_t strideC, int64_t batch,
+               const void* D, void* C, int64_t ldc, int64_t strideC, int64_t batch,
                int64_t splits, double* ws, void* stream);
 
 /* Row-scatter form of nl_gemm_lp: for each batch b and row m with crow[b*M + m] >= 0,
@@ -64,7 +79,7 @@ int nl_gemm_lp(int dtype, int64_t M, int64_t N, int64_t K, double alpha, const v
  * batches); ia / ib may be null. */
 int nl_gemm_lp_rows(int dtype, int64_t M, int64_t N, int64_t K, double alpha, const void* A,
                     int64_t lda, int64_t strideA, const float* ia, int64_t strideIa, const void* B,
-                    int64_t ldb, int64_t strideB, const float* ib, int64_t strideIb, double* C,
+                    int64_t ldb, int64_t strideB, const float* ib, int64_t strideIb, void* C,
                     int64_t ldc, const int64_t* crow, int64_t batch, void* stream);
 
 /* nl_gemm_lp (batch 1) with C and D addressed like nl_contract_f64's output: cdims = {M1, M2,
@@ -74,7 +89,7 @@ int nl_gemm_lp_rows(int dtype, int64_t M, int64_t N, int64_t K, double alpha, co
  * coupled_cluster.py:476-536) without a row-major staging copy. */
 int nl_gemm_lp_strided(int dtype, int64_t K, double alpha, const void* A, int64_t lda,
                        const float* ia, const void* B, int64_t ldb, const float* ib, double beta,
-                       const double* D, double* C, const int64_t* cdims, const int64_t* sc,
+                       const void* D, void* C, const int64_t* cdims, const int64_t* sc,
                        int64_t splits, double* ws, void* stream);
 
 /* out (contiguous, out_shape) = alpha * in[sum_d idx_d * in_strides[d]] + beta * out.
@@ -92,14 +107,14 @@ int nl_gemm_lp_split(int dtype, int64_t M, int64_t N, int64_t K, double alpha, c
                      const void* Alo, int64_t lda, int64_t strideA, const float* ia,
                      int64_t strideIa, const void* B, const void* Blo, int64_t ldb,
                      int64_t strideB, const float* ib, int64_t strideIb, double beta,
-                     const double* D, double* C, int64_t ldc, int64_t strideC, int64_t batch,
+                     const void* D, void* C, int64_t ldc, int64_t strideC, int64_t batch,
                      int64_t splits, double* ws, void* stream);
 
 /* nl_gemm_lp_strided with split operands (see nl_gemm_lp_split). */
 int nl_gemm_lp_strided_split(int dtype, int64_t K, double alpha, const void* A, const void* Alo,
                              int64_t lda, const float* ia, const void* B, const void* Blo,
-                             int64_t ldb, const float* ib, double beta, const double* D,
-                             double* C, const int64_t* cdims, const int64_t* sc, int64_t splits,
+                             int64_t ldb, const float* ib, double beta, const void* D,
+                             void* C, const int64_t* cdims, const int64_t* sc, int64_t splits,
                              double* ws, void* stream);
 
 /* nl_gemm_lp_rows with split operands (see nl_gemm_lp_split). */
@@ -107,7 +122,7 @@ int nl_gemm_lp_rows_split(int dtype, int64_t M, int64_t N, int64_t K, double alp
                           const void* A, const void* Alo, int64_t lda, int64_t strideA,
                           const float* ia, int64_t strideIa, const void* B, const void* Blo,
                           int64_t ldb, int64_t strideB, const float* ib, int64_t strideIb,
-                          double* C, int64_t ldc, const int64_t* crow, int64_t batch,
+                          void* C, int64_t ldc, const int64_t* crow, int64_t batch,
                           void* stream);
 
 /* Row-wise FP64 -> BF16 (dtype 0) / FP16 (dtype 1) conversion of a 4-index view
@@ -148,9 +163,10 @@ int nl_gather_rows16_split(int64_t nb, int64_t kb, int64_t ncol, const int64_t* 
  * (FP16) significant bits.  Row r gets nblk blocks of kpad columns, block q at
  * out[r*ldo + q*kpad + c] holding lo when bit q of lomask is set and hi otherwise (columns
  * C1*C2 <= c < kpad zero).  GEMM operands [hi|lo|hi] x [hi|hi|lo] over the K-concatenated blocks
- * sum hi.hi + lo.hi + hi.lo.  Precision form with split operands (no reference counterpart). */
+ * sum hi.hi + lo.hi + hi.lo.  Precision form with split operands (no reference counterpart).
+ * The source is double, or float with dtype | NL_D_F32 << 8 (an FP32 accumulator). */
 int nl_convert_split(int dtype, int64_t R1, int64_t R2, int64_t C1, int64_t C2,
-                     const int64_t* strides4, const double* in, void* out, int64_t ldo,
+                     const int64_t* strides4, const void* in, void* out, int64_t ldo,
                      int64_t kpad, int nblk, int lomask, float* iscale, float target,
                      double scale, void* stream);
 
@@ -205,10 +221,11 @@ int nl_tau(int64_t p, int64_t h, const double* t1, const double* t2, double* tau
 /* Segmented COO accumulation: for each segment s,
  *   out[dst_off[s] + x*dst_stride] += alpha * sum_{e=ptr[s]}^{ptr[s+1]-1} val[e] * src[src_off[e] + x*src_stride]
  * for x < X.  Replaces the scalar loops of v_ppph_dgrams (ccDgrams.py:13-45) for the
- * intermediates ret0..ret4 with precomputed segment plans (conflict-free, no atomics). */
+ * intermediates ret0..ret4 with precomputed segment plans (conflict-free, no atomics).
+ * io: NL_C_F32 for a float out (sums in FP64, one rounding per output element). */
 int nl_segsum(int64_t S, int64_t X, const int64_t* ptr, const int64_t* src_off, const double* val,
               const int64_t* dst_off, int64_t src_stride, int64_t dst_stride, double alpha,
-              const double* src, double* out, void* stream);
+              const double* src, void* out, int io, void* stream);
 
 /* out[r][:] = src[idx[r]][:] (0 where idx<0) and out[idx[r]][:] += alpha*src[r][:]:
  * pair-row gather/scatter around the block-dense pp ladder and v_ppph contractions
@@ -249,13 +266,16 @@ int nl_assemble_t2_16(int64_t p, int64_t h, const double* S0, const double* Sab,
  * update.  The precision form runs the X_pp t2 dressing (coupled_cluster.py:536) in low precision
  * without X_pp's diagonal, whose product with t2 is the largest term of the residual and is
  * added here exactly.  image_kind 0: t2n16 is the BF16 image of nl_assemble_t2_16; 1 (BF16) or 2
- * (FP16): the hi/lo split of t2n * image_scale, row a as [hi(p*h*h) | lo(p*h*h)]. */
-int nl_assemble_t2_ext(int64_t p, int64_t h, const double* S0, const double* Sab,
-                       const double* Sij, const double* Spp, const double* Xpp, const double* Xhh,
+ * (FP16): the hi/lo split of t2n * image_scale, row a as [hi(p*h*h) | lo(p*h*h)].
+ * io: NL_ACC_F32 when S0, Sab, Sij, Spp are float (the precision form's FP32 accumulators);
+ * NL_ADD_V adds v_pphh (the vpphh argument, required then) to H2 in FP64, for an S0 that does
+ * not hold the bare interaction term of t2Iter (coupled_cluster.py:477). */
+int nl_assemble_t2_ext(int64_t p, int64_t h, const void* S0, const void* Sab,
+                       const void* Sij, const void* Spp, const double* Xpp, const double* Xhh,
                        const double* dpp, const double* dhh, const double* t2,
                        const double* vpphh, const double* t1n, double mixing, double* t2n,
                        double* H2out, double* part, void* t2n16, int image_kind,
-                       double image_scale, void* stream);
+                       double image_scale, int io, void* stream);
 
 /* d[i] = X[i,i], then X[i,i] = 0, for a row-major n x n FP64 matrix (the diagonal split of a
  * dressing matrix before its low-precision product; coupled_cluster.py:536). */

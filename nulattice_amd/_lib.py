@@ -18,7 +18,7 @@ SIGNATURES = {
                            _dbl, _vp, _i64, _i64, _i64, _i64, _vp, _vp]),
     "nl_contract_ws": (_i64, [_vp]),
     "nl_contract_f64": (_i32, [_vp, _dbl, _vp, _vp, _vp, _vp, _dbl, _vp, _vp, _vp, _i64, _vp,
-                               _vp]),
+                               _i32, _vp]),
     "nl_gemm_lp": (_i32, [_i32, _i64, _i64, _i64, _dbl, _vp, _i64, _i64, _vp, _i64, _vp, _i64,
                           _i64, _vp, _i64, _dbl, _vp, _vp, _i64, _i64, _i64, _i64, _vp, _vp]),
     "nl_gemm_lp_split": (_i32, [_i32, _i64, _i64, _i64, _dbl, _vp, _vp, _i64, _i64, _vp, _i64,
@@ -48,7 +48,8 @@ SIGNATURES = {
     "nl_convert_rows": (_i32, [_i32, _i64, _i64, _i64, _i64, _vp, _vp, _vp, _i64, _vp, _flt,
                                _vp]),
     "nl_tau": (_i32, [_i64, _i64, _vp, _vp, _vp, _vp]),
-    "nl_segsum": (_i32, [_i64, _i64, _vp, _vp, _vp, _vp, _i64, _i64, _dbl, _vp, _vp, _vp]),
+    "nl_segsum": (_i32, [_i64, _i64, _vp, _vp, _vp, _vp, _i64, _i64, _dbl, _vp, _vp, _i32,
+                         _vp]),
     "nl_gather_rows": (_i32, [_i64, _i64, _vp, _vp, _i64, _vp, _vp]),
     "nl_scatter_add_rows": (_i32, [_i64, _i64, _vp, _vp, _dbl, _vp, _i64, _vp]),
     "nl_update_t1": (_i32, [_i64, _i64, _vp, _vp, _vp, _vp, _dbl, _vp, _vp]),
@@ -57,7 +58,7 @@ SIGNATURES = {
     "nl_assemble_t2_16": (_i32, [_i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _dbl,
                                  _vp, _vp, _vp, _vp, _vp]),
     "nl_assemble_t2_ext": (_i32, [_i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
-                                  _vp, _dbl, _vp, _vp, _vp, _vp, _i32, _dbl, _vp]),
+                                  _vp, _dbl, _vp, _vp, _vp, _vp, _i32, _dbl, _i32, _vp]),
     "nl_tau_split": (_i32, [_i64, _i64, _vp, _vp, _vp, _vp, _i32, _dbl, _vp]),
     "nl_gather_rows16_into": (_i32, [_i64, _i64, _i64, _vp, _vp, _i64, _vp, _i64, _i64, _vp]),
     "nl_take_diag": (_i32, [_i64, _vp, _vp, _vp]),

@@ -21,6 +21,44 @@ typedef _Float16 v8h __attribute__((ext_vector_type(8)));
 
 static inline int64_t nl_cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
+// Element types of a product's output C and addend D (the "io" flags of the entry points):
+// FP64 by default; FP32 for the accumulators of the precision form, whose read-modify-write
+// passes then move half the bytes.  A set flag selects float for that array.
+#define NL_C_F32 1
+#define NL_D_F32 2
+// Flags of the T2 assembly: its four accumulators are FP32; it adds v_pphh to the residual.
+#define NL_ACC_F32 1
+#define NL_ADD_V 2
+
+// An array of double or float elements addressed by element offset (a wave-uniform type).
+struct AnyOut {
+  void* p; bool f32;
+  __device__ inline void put(int64_t o, double v) const {
+    if (f32) static_cast<float*>(p)[o] = (float)v; else static_cast<double*>(p)[o] = v;
+  }
+  __device__ inline double get(int64_t o) const {
+    return f32 ? (double)static_cast<const float*>(p)[o] : static_cast<const double*>(p)[o];
+  }
+  __device__ inline AnyOut at(int64_t o) const {
+    AnyOut r; r.f32 = f32;
+    r.p = f32 ? (void*)(static_cast<float*>(p) + o) : (void*)(static_cast<double*>(p) + o);
+    return r;
+  }
+};
+struct AnyIn {
+  const void* p; bool f32;
+  __device__ inline double get(int64_t o) const {
+    return f32 ? (double)static_cast<const float*>(p)[o] : static_cast<const double*>(p)[o];
+  }
+  __device__ inline AnyIn at(int64_t o) const {
+    AnyIn r; r.f32 = f32;
+    r.p = f32 ? (const void*)(static_cast<const float*>(p) + o)
+              : (const void*)(static_cast<const double*>(p) + o);
+    return r;
+  }
+  __device__ inline explicit operator bool() const { return p != nullptr; }
+};
+
 // Division by a fixed divisor d >= 1 for dividends 0 <= i < 2^31 without the GPU's multi-
 // instruction integer divide: i / d = (i * mg) >> sh with sh = 31 + ceil(log2 d) and
 // mg = ceil(2^sh / d) (Granlund and Montgomery, "Division by invariant integers using

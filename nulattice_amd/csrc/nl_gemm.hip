@@ -14,6 +14,8 @@
 // columns stay in that XCD's L2), then grouped by 8 tile-rows.
 #include "nl_common.h"
 
+#include <type_traits>
+
 namespace {
 
 constexpr int TM = 128, TN = 128;
@@ -74,8 +76,8 @@ struct GemmF64Args {
   double alpha, beta;
   const double* A; Idx2 am, ak, ab;
   const double* B; Idx2 bk, bn, bb;
-  double* C; Idx2 cm, cn, cb;
-  const double* D;                          // addend (same layout as C); may equal C or be null
+  AnyOut C; Idx2 cm, cn, cb;                // FP64 or FP32 output
+  AnyIn D;                                  // addend (same layout as C); may equal C or be null
   int tiles_m, tiles_n, splits, kchunk;
   double* ws;  // split-K partials [batch][splits][M][N] (only when splits > 1)
 };
@@ -237,14 +239,14 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(GemmF64Args g) {
       ro[r] = row < g.M ? cb + off2(row, g.cm) : -1;
 #pragma unroll
       for (int j = 0; j < 4; ++j)
-        dv[r][j] = (addend && ro[r] >= 0 && coff[j] >= 0) ? g.D[ro[r] + coff[j]] : 0.0;
+        dv[r][j] = (addend && ro[r] >= 0 && coff[j] >= 0) ? g.D.get(ro[r] + coff[j]) : 0.0;
     }
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       if (ro[r] < 0) continue;
 #pragma unroll
       for (int j = 0; j < 4; ++j)
-        if (coff[j] >= 0) g.C[ro[r] + coff[j]] = g.alpha * acc[i][j][r] + g.beta * dv[r][j];
+        if (coff[j] >= 0) g.C.put(ro[r] + coff[j], g.alpha * acc[i][j][r] + g.beta * dv[r][j]);
     }
   }
 }
@@ -357,8 +359,8 @@ __global__ __launch_bounds__(256) void skinny_f64_kernel(GemmF64Args g) {
         W[(int64_t)m * g.N + n] = v;
       } else {
         const int64_t o = cbo + off2(m, g.cm) + off2(n, g.cn);
-        if (g.D && g.beta != 0.0) v += g.beta * g.D[o];
-        g.C[o] = v;
+        if (g.D && g.beta != 0.0) v += g.beta * g.D.get(o);
+        g.C.put(o, v);
       }
     }
   }
@@ -429,8 +431,8 @@ __global__ __launch_bounds__(256) void smallmn_f64_kernel(GemmF64Args g) {
     } else {
       const int64_t off = off2(batch, g.cb) + off2(m, g.cm) + off2(n, g.cn);
       double v = g.alpha * acc[j];
-      if (g.D && g.beta != 0.0) v += g.beta * g.D[off];
-      g.C[off] = v;
+      if (g.D && g.beta != 0.0) v += g.beta * g.D.get(off);
+      g.C.put(off, v);
     }
   }
 }
@@ -532,7 +534,7 @@ __global__ __launch_bounds__(256) void smallmn32_f64_kernel(GemmF64Args g) {
         g.ws[((int64_t)batch * g.splits + split) * (int64_t)MN + (int64_t)m * g.N + n] = v;
       } else {
         const int64_t off = off2(batch, g.cb) + off2(m, g.cm) + off2(n, g.cn);
-        g.C[off] = (g.D && g.beta != 0.0) ? v + g.beta * g.D[off] : v;
+        g.C.put(off, (g.D && g.beta != 0.0) ? v + g.beta * g.D.get(off) : v);
       }
     }
 }
@@ -548,8 +550,8 @@ __device__ inline double wave_sum64(double v) {
 // a thread-per-output loop would walk thousands of dependent loads).
 __global__ __launch_bounds__(256) void splitk_reduce_wave_kernel(int M, int N, int splits,
                                                                  int nbatch, const double* ws,
-                                                                 double beta, const double* D,
-                                                                 double* C, Idx2 cm, Idx2 cn,
+                                                                 double beta, AnyIn D,
+                                                                 AnyOut C, Idx2 cm, Idx2 cn,
                                                                  Idx2 cb) {
   const int lane = threadIdx.x & 63;
   const int64_t MN = (int64_t)M * N, total = (int64_t)nbatch * MN;
@@ -563,7 +565,7 @@ __global__ __launch_bounds__(256) void splitk_reduce_wave_kernel(int M, int N, i
     if (lane == 0) {
       const int64_t m = mn / N, n = mn - m * N;
       const int64_t o = off2((int)b, cb) + off2((int)m, cm) + off2((int)n, cn);
-      C[o] = (D && beta != 0.0) ? s + beta * D[o] : s;
+      C.put(o, (D && beta != 0.0) ? s + beta * D.get(o) : s);
     }
   }
 }
@@ -575,8 +577,10 @@ __global__ __launch_bounds__(256) void splitk_reduce_wave_kernel(int M, int N, i
 // of C moves in 128-byte runs.
 constexpr int LKT = 64, LKK = 32;
 
-template <bool LN>
+template <bool LN, typename TC, typename TD>   // TC, TD: element types of C and D
 __global__ __launch_bounds__(256) void lowk_f64_kernel(GemmF64Args g) {
+  TC* const Cp = static_cast<TC*>(g.C.p);
+  const TD* const Dp = static_cast<const TD*>(g.D.p);
   __shared__ double As[LKK][LKT + 1];
   __shared__ double Bs[LKK][LKT + 1];
   const int tid = threadIdx.x;
@@ -610,7 +614,8 @@ __global__ __launch_bounds__(256) void lowk_f64_kernel(GemmF64Args g) {
   for (int x = 0; x < 4; ++x)
 #pragma unroll
     for (int y = 0; y < 4; ++y)
-      dv[x][y] = (addend && roff[x] >= 0 && n0 + tn + 16 * y < g.N) ? g.D[roff[x] + coff[y]] : 0.0;
+      dv[x][y] = (addend && roff[x] >= 0 && n0 + tn + 16 * y < g.N) ? (double)Dp[roff[x] + coff[y]]
+                                                                     : 0.0;
   // all staging loads are issued before any LDS store, so their latencies overlap; lanes run
   // along each operand's unit-stride index (k: 32 lanes x 8 rows, else 64 lanes x 4 k)
   constexpr int LQ = LKK * LKT / 256;
@@ -652,15 +657,15 @@ __global__ __launch_bounds__(256) void lowk_f64_kernel(GemmF64Args g) {
       if (n0 + tn + 16 * y >= g.N) continue;
       double v = g.alpha * acc[x][y];
       if (addend) v += g.beta * dv[x][y];
-      g.C[roff[x] + coff[y]] = v;
+      Cp[roff[x] + coff[y]] = (TC)v;
     }
   }
 }
 
 // C[b](m,n) = sum_s ws[b][s][m][n] + beta * D[b](m,n)
 __global__ void splitk_reduce_strided_kernel(int M, int N, int splits, int nbatch,
-                                             const double* ws, double beta, const double* D,
-                                             double* C, Idx2 cm, Idx2 cn, Idx2 cb) {
+                                             const double* ws, double beta, AnyIn D,
+                                             AnyOut C, Idx2 cm, Idx2 cn, Idx2 cb) {
   int64_t total = (int64_t)nbatch * M * N;
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
        t += (int64_t)gridDim.x * blockDim.x) {
@@ -671,8 +676,8 @@ __global__ void splitk_reduce_strided_kernel(int M, int N, int splits, int nbatc
     double s = 0.0;
     for (int q = 0; q < splits; ++q) s += W[(int64_t)q * M * N];
     int64_t o = off2(b, cb) + off2(m, cm) + off2(n, cn);
-    if (D && beta != 0.0) s += beta * D[o];
-    C[o] = s;
+    if (D && beta != 0.0) s += beta * D.get(o);
+    C.put(o, s);
   }
 }
 
@@ -690,13 +695,84 @@ struct GemmLPArgs {
   const uint16_t* Blo;                  // null); the product is hi.hi + hi.lo + lo.hi
   const float* ia; int64_t sia;         // per-row inverse scale of A (or null)
   const float* ib; int64_t sib;         // per-row inverse scale of B (or null)
-  double* C; Idx2 cm, cn; int64_t sC;   // C(m,n) = C[off2(m,cm) + off2(n,cn)] (+ batch * sC)
-  const double* D;                      // addend with C's layout (may be C or null)
+  AnyOut C; Idx2 cm, cn; int64_t sC;   // C(m,n) = C[off2(m,cm) + off2(n,cn)] (+ batch * sC),
+                                        // FP64 or FP32
+  AnyIn D;                              // addend with C's layout (may be C or null)
   int tiles_m, tiles_n, splits, kchunk;
   double* ws;                           // split-K partials [batch][splits][M][N]
   const int64_t* crow;                  // row scatter: row m of batch b goes to C row
                                         // crow[b*M + m] (< 0: dropped), no batch offset
 };
+
+// Epilogue of the 16-bit GEMMs: C = alpha ia[m] acc ib[n] + beta D for this wave's NI x 4 MFMA
+// tiles (rows m0 + wm*16*NI + i*16 + (lane/16)*4 + r, columns n0 + wn*64 + j*16 + lane%16), into
+// the split-K partials, scattered rows (crow) or C's two-run layout.  The element types of C and
+// D are dispatched once, outside the loops, so the addend loads of a row group issue back to
+// back (a type test per access would split them into dependent blocks).
+template <int NI>
+__device__ __attribute__((always_inline)) inline void lp_epilogue(const GemmLPArgs& g, const v4f (&acc)[NI][4], int m0, int n0,
+                                   int wm, int wn, int lane, int batch, int split) {
+  const bool partial = g.splits > 1;
+  const int64_t cbo = g.crow ? 0 : (int64_t)batch * g.sC;
+  const double beta = partial ? 0.0 : g.beta;
+  const bool addend = !partial && g.D && beta != 0.0;
+  const float* ia = g.ia ? g.ia + (int64_t)batch * g.sia : nullptr;
+  const float* ib = g.ib ? g.ib + (int64_t)batch * g.sib : nullptr;
+  double sc[4];
+  int64_t coff[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    int col = n0 + wn * 64 + j * 16 + (lane & 15);
+    sc[j] = (ib && col < g.N) ? (double)ib[col] : 1.0;
+    coff[j] = col < g.N ? (partial ? col : off2(col, g.cn)) : -1;
+  }
+  // (always inlined: a call would pass the accumulators through scratch memory)
+  auto body = [&](auto* C, const auto* D) __attribute__((always_inline)) {
+    using TC = std::remove_pointer_t<decltype(C)>;
+    // per 16-row group: all addend loads first, then the stores (D may be C; each element is
+    // read and written by this lane only), so the loads overlap instead of waiting on each store
+#pragma unroll
+    for (int i = 0; i < NI; ++i) {
+      int64_t ro[4];
+      double dv[4][4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = m0 + wm * 16 * NI + i * 16 + (lane >> 4) * 4 + r;
+        ro[r] = row >= g.M ? -1
+                : partial ? (int64_t)row * g.N
+                : g.crow ? (g.crow[(int64_t)batch * g.M + row] >= 0
+                                ? g.crow[(int64_t)batch * g.M + row] * g.cm.s1 : -1)
+                : off2(row, g.cm);
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          dv[r][j] = (addend && ro[r] >= 0 && coff[j] >= 0) ? (double)D[ro[r] + coff[j]] : 0.0;
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        if (ro[r] < 0) continue;
+        const int row = m0 + wm * 16 * NI + i * 16 + (lane >> 4) * 4 + r;
+        const double sr = g.alpha * (ia ? (double)ia[row] : 1.0);
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          if (coff[j] >= 0)
+            C[ro[r] + coff[j]] = (TC)(sr * (double)acc[i][j][r] * sc[j] + beta * dv[r][j]);
+      }
+    }
+  };
+  if (partial) {
+    body(g.ws + ((int64_t)batch * g.splits + split) * (int64_t)g.M * g.N, (const double*)nullptr);
+  } else {
+    const AnyOut C = g.C.at(cbo);
+    const AnyIn D = g.D ? g.D.at(cbo) : AnyIn{nullptr, false};
+    if (C.f32) {
+      if (D.f32) body(static_cast<float*>(C.p), static_cast<const float*>(D.p));
+      else body(static_cast<float*>(C.p), static_cast<const double*>(D.p));
+    } else {
+      if (D.f32) body(static_cast<double*>(C.p), static_cast<const float*>(D.p));
+      else body(static_cast<double*>(C.p), static_cast<const double*>(D.p));
+    }
+  }
+}
 
 template <bool IS_BF16>
 __global__ __launch_bounds__(256) void gemm_lp_kernel(GemmLPArgs g) {
@@ -778,50 +854,7 @@ __global__ __launch_bounds__(256) void gemm_lp_kernel(GemmLPArgs g) {
   }
   }
 
-  const bool partial = g.splits > 1;
-  const int64_t cbo = g.crow ? 0 : (int64_t)batch * g.sC;
-  double* C = partial ? g.ws + ((int64_t)batch * g.splits + split) * (int64_t)g.M * g.N
-                      : g.C + cbo;
-  const double* Dp = (partial || !g.D) ? nullptr : g.D + cbo;
-  const double beta = partial ? 0.0 : g.beta;
-  const float* ia = g.ia ? g.ia + (int64_t)batch * g.sia : nullptr;
-  const float* ib = g.ib ? g.ib + (int64_t)batch * g.sib : nullptr;
-  double sc[4];
-  int64_t coff[4];
-#pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    int col = n0 + wn * 64 + j * 16 + (lane & 15);
-    sc[j] = (ib && col < g.N) ? (double)ib[col] : 1.0;
-    coff[j] = col < g.N ? (partial ? col : off2(col, g.cn)) : -1;
-  }
-  // per 16-row group: all addend loads first, then the stores (D may be C, and each element is
-  // read and written by this lane only), so the loads overlap instead of waiting on each store
-#pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    int64_t ro[4];
-    double dv[4][4];
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int row = m0 + wm * 64 + i * 16 + (lane >> 4) * 4 + r;
-      ro[r] = row >= g.M ? -1
-              : partial ? (int64_t)row * g.N
-              : g.crow ? (g.crow[(int64_t)batch * g.M + row] >= 0
-                              ? g.crow[(int64_t)batch * g.M + row] * g.cm.s1 : -1)
-              : off2(row, g.cm);
-#pragma unroll
-      for (int j = 0; j < 4; ++j)
-        dv[r][j] = (Dp && beta != 0.0 && ro[r] >= 0 && coff[j] >= 0) ? Dp[ro[r] + coff[j]] : 0.0;
-    }
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      if (ro[r] < 0) continue;
-      const int row = m0 + wm * 64 + i * 16 + (lane >> 4) * 4 + r;
-      const double sr = g.alpha * (ia ? (double)ia[row] : 1.0);
-#pragma unroll
-      for (int j = 0; j < 4; ++j)
-        if (coff[j] >= 0) C[ro[r] + coff[j]] = sr * (double)acc[i][j][r] * sc[j] + beta * dv[r][j];
-    }
-  }
+  lp_epilogue<4>(g, acc, m0, n0, wm, wn, lane, batch, split);
 }
 
 // Large-tile BF16/FP16 GEMM for the big square products (the two ring GEMMs): 256x256 block
@@ -962,50 +995,7 @@ __global__ __launch_bounds__(512) void gemm_lp_big_kernel(GemmLPArgs g) {
     big_step<IS_BF16>(S1, acc, wm, wn, lane);
   }
 
-  const bool partial = g.splits > 1;
-  const int64_t cbo = g.crow ? 0 : (int64_t)batch * g.sC;
-  double* C = partial ? g.ws + ((int64_t)batch * g.splits + split) * (int64_t)g.M * g.N
-                      : g.C + cbo;
-  const double* Dp = (partial || !g.D) ? nullptr : g.D + cbo;
-  const double beta = partial ? 0.0 : g.beta;
-  const float* ia = g.ia ? g.ia + (int64_t)batch * g.sia : nullptr;
-  const float* ib = g.ib ? g.ib + (int64_t)batch * g.sib : nullptr;
-  double sc[4];
-  int64_t coff[4];
-#pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    int col = n0 + wn * 64 + j * 16 + (lane & 15);
-    sc[j] = (ib && col < g.N) ? (double)ib[col] : 1.0;
-    coff[j] = col < g.N ? (partial ? col : off2(col, g.cn)) : -1;
-  }
-  // per 16-row group: all addend loads first, then the stores (D may be C; each element is read
-  // and written by this lane only), so the loads overlap instead of waiting on each store
-#pragma unroll
-  for (int i = 0; i < 8; ++i) {
-    int64_t ro[4];
-    double dv[4][4];
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int row = m0 + wm * 128 + i * 16 + (lane >> 4) * 4 + r;
-      ro[r] = row >= g.M ? -1
-              : partial ? (int64_t)row * g.N
-              : g.crow ? (g.crow[(int64_t)batch * g.M + row] >= 0
-                              ? g.crow[(int64_t)batch * g.M + row] * g.cm.s1 : -1)
-              : off2(row, g.cm);
-#pragma unroll
-      for (int j = 0; j < 4; ++j)
-        dv[r][j] = (Dp && beta != 0.0 && ro[r] >= 0 && coff[j] >= 0) ? Dp[ro[r] + coff[j]] : 0.0;
-    }
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      if (ro[r] < 0) continue;
-      const int row = m0 + wm * 128 + i * 16 + (lane >> 4) * 4 + r;
-      const double sr = g.alpha * (ia ? (double)ia[row] : 1.0);
-#pragma unroll
-      for (int j = 0; j < 4; ++j)
-        if (coff[j] >= 0) C[ro[r] + coff[j]] = sr * (double)acc[i][j][r] * sc[j] + beta * dv[r][j];
-    }
-  }
+  lp_epilogue<8>(g, acc, m0, n0, wm, wn, lane, batch, split);
 }
 
 // ---------------------------------------------------------------------------------- split GEMM
@@ -1178,48 +1168,7 @@ __global__ __launch_bounds__(512) void gemm_lp_split_kernel(GemmLPArgs g) {
     }
   }
 
-  const bool partial = g.splits > 1;
-  const int64_t cbo = g.crow ? 0 : (int64_t)batch * g.sC;
-  double* C = partial ? g.ws + ((int64_t)batch * g.splits + split) * (int64_t)g.M * g.N
-                      : g.C + cbo;
-  const double* Dp = (partial || !g.D) ? nullptr : g.D + cbo;
-  const double beta = partial ? 0.0 : g.beta;
-  const float* ia = g.ia ? g.ia + (int64_t)batch * g.sia : nullptr;
-  const float* ib = g.ib ? g.ib + (int64_t)batch * g.sib : nullptr;
-  double sc[4];
-  int64_t coff[4];
-#pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    int col = n0 + wn * 64 + j * 16 + (lane & 15);
-    sc[j] = (ib && col < g.N) ? (double)ib[col] : 1.0;
-    coff[j] = col < g.N ? (partial ? col : off2(col, g.cn)) : -1;
-  }
-#pragma unroll
-  for (int i = 0; i < 8; ++i) {
-    int64_t ro[4];
-    double dv[4][4];
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int row = m0 + wm * 128 + i * 16 + (lane >> 4) * 4 + r;
-      ro[r] = row >= g.M ? -1
-              : partial ? (int64_t)row * g.N
-              : g.crow ? (g.crow[(int64_t)batch * g.M + row] >= 0
-                              ? g.crow[(int64_t)batch * g.M + row] * g.cm.s1 : -1)
-              : off2(row, g.cm);
-#pragma unroll
-      for (int j = 0; j < 4; ++j)
-        dv[r][j] = (Dp && beta != 0.0 && ro[r] >= 0 && coff[j] >= 0) ? Dp[ro[r] + coff[j]] : 0.0;
-    }
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      if (ro[r] < 0) continue;
-      const int row = m0 + wm * 128 + i * 16 + (lane >> 4) * 4 + r;
-      const double sr = g.alpha * (ia ? (double)ia[row] : 1.0);
-#pragma unroll
-      for (int j = 0; j < 4; ++j)
-        if (coff[j] >= 0) C[ro[r] + coff[j]] = sr * (double)acc[i][j][r] * sc[j] + beta * dv[r][j];
-    }
-  }
+  lp_epilogue<8>(g, acc, m0, n0, wm, wn, lane, batch, split);
 }
 
 // ---------------------------------------------------------------------------------- FP64, large
@@ -1436,14 +1385,14 @@ __global__ __launch_bounds__(512) void gemm_f64_big_kernel(GemmF64Args g) {
       ro[r] = row < g.M ? cb + off2(row, g.cm) : -1;
 #pragma unroll
       for (int j = 0; j < 4; ++j)
-        dv[r][j] = (addend && ro[r] >= 0 && coff[j] >= 0) ? g.D[ro[r] + coff[j]] : 0.0;
+        dv[r][j] = (addend && ro[r] >= 0 && coff[j] >= 0) ? g.D.get(ro[r] + coff[j]) : 0.0;
     }
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       if (ro[r] < 0) continue;
 #pragma unroll
       for (int j = 0; j < 4; ++j)
-        if (coff[j] >= 0) g.C[ro[r] + coff[j]] = g.alpha * acc[i][j][r] + g.beta * dv[r][j];
+        if (coff[j] >= 0) g.C.put(ro[r] + coff[j], g.alpha * acc[i][j][r] + g.beta * dv[r][j]);
     }
   }
 }
@@ -1497,7 +1446,7 @@ int64_t nl_contract_ws(const int64_t* dims) {
 // sc[6] = C (and D) {m1, m2, n1, n2, b1, b2}.  C = alpha*A*B + beta*D (D may be C or null).
 // Split-K reduction of the [batch][splits][M][N] partials into C (layout cm, cn, cb).
 static void reduce_splits(hipStream_t st, int M, int N, int splits, int nb, const double* ws,
-                          double beta, const double* D, double* C, Idx2 cm, Idx2 cn, Idx2 cb) {
+                          double beta, AnyIn D, AnyOut C, Idx2 cm, Idx2 cn, Idx2 cb) {
   const int64_t total = (int64_t)nb * M * N;
   if (splits >= 64) {
     int blocks = (int)std::min<int64_t>(nl_cdiv(total * 64, 256), 16384);
@@ -1520,9 +1469,9 @@ static bool chunk_group(int64_t n1, int64_t n2, int64_t s1, int64_t s2) {
 static bool even_stride(int64_t n, int64_t s) { return n <= 1 || s % 2 == 0; }
 
 int nl_contract_f64(const int64_t* dims, double alpha, const double* A, const int64_t* sa,
-                    const double* B, const int64_t* sb, double beta, const double* D, double* C,
-                    const int64_t* sc, int64_t splits, double* ws, void* stream) {
-  if (!dims || !sa || !sb || !sc) return NL_EARG;
+                    const double* B, const int64_t* sb, double beta, const void* D, void* C,
+                    const int64_t* sc, int64_t splits, double* ws, int io, void* stream) {
+  if (!dims || !sa || !sb || !sc || (io & ~(NL_C_F32 | NL_D_F32))) return NL_EARG;
   int kind = 0;
   const bool auto_plan = splits <= 0;
   if (auto_plan) {   // automatic kernel choice and split (ws sized by nl_contract_ws)
@@ -1541,12 +1490,12 @@ int nl_contract_f64(const int64_t* dims, double alpha, const double* A, const in
   hipStream_t st = (hipStream_t)stream;
   GemmF64Args g;
   g.M = (int)M; g.N = (int)N; g.K = (int)K; g.nbatch = (int)nb; g.alpha = alpha;
-  g.beta = beta; g.D = D;
+  g.beta = beta; g.D = AnyIn{D, (io & NL_D_F32) != 0};
   g.A = A; g.am = mk(dims[1], sa[0], sa[1]); g.ak = mk(dims[5], sa[2], sa[3]);
   g.ab = mk(dims[7], sa[4], sa[5]);
   g.B = B; g.bk = mk(dims[5], sb[0], sb[1]); g.bn = mk(dims[3], sb[2], sb[3]);
   g.bb = mk(dims[7], sb[4], sb[5]);
-  g.C = C; g.cm = mk(dims[1], sc[0], sc[1]); g.cn = mk(dims[3], sc[2], sc[3]);
+  g.C = AnyOut{C, (io & NL_C_F32) != 0}; g.cm = mk(dims[1], sc[0], sc[1]); g.cn = mk(dims[3], sc[2], sc[3]);
   g.cb = mk(dims[7], sc[4], sc[5]);
   g.tiles_m = (int)nl_cdiv(M, TM); g.tiles_n = (int)nl_cdiv(N, TN);
   int64_t kchunk = nl_cdiv(nl_cdiv(K > 0 ? K : 1, splits), BK64) * BK64;
@@ -1565,7 +1514,7 @@ int nl_contract_f64(const int64_t* dims, double alpha, const double* A, const in
 #undef NL_SM
     NL_LAUNCH_CHECK();
     if (g.splits > 1) {
-      reduce_splits(st, (int)M, (int)N, g.splits, (int)nb, ws, beta, D, C, g.cm, g.cn, g.cb);
+      reduce_splits(st, (int)M, (int)N, g.splits, (int)nb, ws, beta, g.D, g.C, g.cm, g.cn, g.cb);
       NL_LAUNCH_CHECK();
     }
     return NL_OK;
@@ -1576,8 +1525,12 @@ int nl_contract_f64(const int64_t* dims, double alpha, const double* A, const in
     const bool ln = (g.cn.n2 > 1 ? g.cn.s2 == 1 : g.cn.s1 == 1) ||
                     !(g.cm.n2 > 1 ? g.cm.s2 == 1 : g.cm.s1 == 1);
     dim3 lg((unsigned)(g.tiles_m * g.tiles_n), 1u, (unsigned)nb);
-    if (ln) hipLaunchKernelGGL((lowk_f64_kernel<true>), lg, dim3(256), 0, st, g);
-    else    hipLaunchKernelGGL((lowk_f64_kernel<false>), lg, dim3(256), 0, st, g);
+#define NL_LK(TC, TD)                                                                        \
+    if (ln) hipLaunchKernelGGL((lowk_f64_kernel<true, TC, TD>), lg, dim3(256), 0, st, g);    \
+    else    hipLaunchKernelGGL((lowk_f64_kernel<false, TC, TD>), lg, dim3(256), 0, st, g);
+    if (g.C.f32) { if (g.D.f32) { NL_LK(float, float) } else { NL_LK(float, double) } }
+    else { if (g.D.f32) { NL_LK(double, float) } else { NL_LK(double, double) } }
+#undef NL_LK
     NL_LAUNCH_CHECK();
     return NL_OK;
   }
@@ -1601,7 +1554,7 @@ int nl_contract_f64(const int64_t* dims, double alpha, const double* A, const in
 #undef NL_SK
     NL_LAUNCH_CHECK();
     if (g.splits > 1) {
-      reduce_splits(st, g.M, g.N, g.splits, (int)nb, ws, beta, D, C, g.cm, g.cn, g.cb);
+      reduce_splits(st, g.M, g.N, g.splits, (int)nb, ws, beta, g.D, g.C, g.cm, g.cn, g.cb);
       NL_LAUNCH_CHECK();
     }
     return NL_OK;
@@ -1643,7 +1596,7 @@ int nl_contract_f64(const int64_t* dims, double alpha, const double* A, const in
   else            hipLaunchKernelGGL((gemm_f64_kernel<false, false>), grid, dim3(256), 0, st, g);
   NL_LAUNCH_CHECK();
   if (g.splits > 1) {
-    reduce_splits(st, (int)M, (int)N, g.splits, (int)nb, ws, beta, D, C, g.cm, g.cn, g.cb);
+    reduce_splits(st, (int)M, (int)N, g.splits, (int)nb, ws, beta, g.D, g.C, g.cm, g.cn, g.cb);
     NL_LAUNCH_CHECK();
   }
   return NL_OK;
@@ -1662,7 +1615,7 @@ int nl_gemm_f64(int a_kcontig, int b_ncontig, int64_t M, int64_t N, int64_t K, d
   const int64_t sa[6] = {0, a_kcontig ? lda : 1, 0, a_kcontig ? 1 : lda, 0, strideA};
   const int64_t sb[6] = {0, b_ncontig ? ldb : 1, 0, b_ncontig ? 1 : ldb, 0, strideB};
   const int64_t sc[6] = {0, ldc, 0, 1, 0, strideC};
-  return nl_contract_f64(dims, alpha, A, sa, B, sb, beta, C, C, sc, splits, ws, stream);
+  return nl_contract_f64(dims, alpha, A, sa, B, sb, beta, C, C, sc, splits, ws, 0, stream);
 }
 
 // C[b] (FP64, row-major) = alpha * diag(ia) * (A[b] * B[b]^T) * diag(ib) + beta * C[b]
@@ -1672,10 +1625,14 @@ int nl_gemm_f64(int a_kcontig, int b_ncontig, int64_t M, int64_t N, int64_t K, d
 static int gemm_lp_impl(int dtype, int64_t M, int64_t N, int64_t K, double alpha, const void* A,
                         int64_t lda, int64_t strideA, const float* ia, int64_t strideIa,
                         const void* B, int64_t ldb, int64_t strideB, const float* ib,
-                        int64_t strideIb, double beta, const double* D, double* C, Idx2 cm,
+                        int64_t strideIb, double beta, const void* D, void* C, Idx2 cm,
                         Idx2 cn, int64_t strideC, int64_t batch, int64_t splits, double* ws,
                         void* stream, const int64_t* crow = nullptr, const void* Alo = nullptr,
                         const void* Blo = nullptr, bool staged = false) {
+  // dtype: operand type (bits 0-7) | io flags << 8 (NL_C_F32, NL_D_F32: FP32 output / addend)
+  const int io = dtype >> 8;
+  dtype &= 0xff;
+  if (io & ~(NL_C_F32 | NL_D_F32)) return NL_EARG;
   if (M < 0 || N < 0 || K < 0 || batch < 0 || (dtype != 0 && dtype != 1)) return NL_EARG;
   if (M == 0 || N == 0 || batch == 0) return NL_OK;
   if (!A || !B || !C) return NL_EARG;
@@ -1692,7 +1649,8 @@ static int gemm_lp_impl(int dtype, int64_t M, int64_t N, int64_t K, double alpha
   g.B = (const uint16_t*)B; g.ldb = ldb; g.sB = strideB;
   g.Alo = (const uint16_t*)Alo; g.Blo = (const uint16_t*)Blo;
   g.ia = ia; g.sia = strideIa; g.ib = ib; g.sib = strideIb;
-  g.C = C; g.cm = cm; g.cn = cn; g.sC = strideC; g.D = D; g.crow = crow;
+  g.C = AnyOut{C, (io & NL_C_F32) != 0}; g.cm = cm; g.cn = cn; g.sC = strideC;
+  g.D = AnyIn{D, (io & NL_D_F32) != 0}; g.crow = crow;
   if (crow && splits != 1) return NL_EARG;
   g.tiles_m = (int)nl_cdiv(M, TM); g.tiles_n = (int)nl_cdiv(N, TN);
   g.splits = (int)splits; g.ws = ws;
@@ -1732,16 +1690,19 @@ static int gemm_lp_impl(int dtype, int64_t M, int64_t N, int64_t K, double alpha
   } else if (split_ops) {
     // shapes the split kernel does not take: the three products in turn, each accumulating
     // into C (the first applies beta and D)
-    int rc = gemm_lp_impl(dtype, M, N, K, alpha, A, lda, strideA, ia, strideIa, B, ldb, strideB,
+    // (the later products add to C itself, so their addend has C's element type)
+    const int t0 = dtype | (io << 8);
+    const int tc = dtype | ((io & NL_C_F32) ? (NL_C_F32 | NL_D_F32) << 8 : 0);
+    int rc = gemm_lp_impl(t0, M, N, K, alpha, A, lda, strideA, ia, strideIa, B, ldb, strideB,
                           ib, strideIb, beta, D, C, cm, cn, strideC, batch, splits, ws, stream,
                           crow);
     if (rc == NL_OK && Blo)
-      rc = gemm_lp_impl(dtype, M, N, K, alpha, A, lda, strideA, ia, strideIa, Blo, ldb, strideB,
-                        ib, strideIb, 1.0, crow ? D : C, C, cm, cn, strideC, batch, splits, ws,
+      rc = gemm_lp_impl(tc, M, N, K, alpha, A, lda, strideA, ia, strideIa, Blo, ldb, strideB,
+                        ib, strideIb, 1.0, C, C, cm, cn, strideC, batch, splits, ws,
                         stream, crow);
     if (rc == NL_OK && Alo)
-      rc = gemm_lp_impl(dtype, M, N, K, alpha, Alo, lda, strideA, ia, strideIa, B, ldb, strideB,
-                        ib, strideIb, 1.0, crow ? D : C, C, cm, cn, strideC, batch, splits, ws,
+      rc = gemm_lp_impl(tc, M, N, K, alpha, Alo, lda, strideA, ia, strideIa, B, ldb, strideB,
+                        ib, strideIb, 1.0, C, C, cm, cn, strideC, batch, splits, ws,
                         stream, crow);
     return rc;
   } else if (big) {
@@ -1763,7 +1724,7 @@ static int gemm_lp_impl(int dtype, int64_t M, int64_t N, int64_t K, double alpha
   }
   NL_LAUNCH_CHECK();
   if (splits > 1) {
-    reduce_splits(st, (int)M, (int)N, (int)splits, (int)batch, ws, beta, D, C, cm, cn,
+    reduce_splits(st, (int)M, (int)N, (int)splits, (int)batch, ws, beta, g.D, g.C, cm, cn,
                   mk(1, strideC, 0));
     NL_LAUNCH_CHECK();
   }
@@ -1773,7 +1734,7 @@ static int gemm_lp_impl(int dtype, int64_t M, int64_t N, int64_t K, double alpha
 int nl_gemm_lp(int dtype, int64_t M, int64_t N, int64_t K, double alpha, const void* A,
                int64_t lda, int64_t strideA, const float* ia, int64_t strideIa, const void* B,
                int64_t ldb, int64_t strideB, const float* ib, int64_t strideIb, double beta,
-               const double* D, double* C, int64_t ldc, int64_t strideC, int64_t batch,
+               const void* D, void* C, int64_t ldc, int64_t strideC, int64_t batch,
                int64_t splits, double* ws, void* stream) {
   return gemm_lp_impl(dtype, M, N, K, alpha, A, lda, strideA, ia, strideIa, B, ldb, strideB, ib,
                       strideIb, beta, D, C, mk(1, ldc, 0), mk(1, 1, 0), strideC, batch, splits,
@@ -1782,9 +1743,10 @@ int nl_gemm_lp(int dtype, int64_t M, int64_t N, int64_t K, double alpha, const v
 
 int nl_gemm_lp_rows(int dtype, int64_t M, int64_t N, int64_t K, double alpha, const void* A,
                     int64_t lda, int64_t strideA, const float* ia, int64_t strideIa, const void* B,
-                    int64_t ldb, int64_t strideB, const float* ib, int64_t strideIb, double* C,
+                    int64_t ldb, int64_t strideB, const float* ib, int64_t strideIb, void* C,
                     int64_t ldc, const int64_t* crow, int64_t batch, void* stream) {
   if (!crow || ldc < N) return NL_EARG;
+  if (dtype & (NL_C_F32 << 8)) dtype |= NL_D_F32 << 8;       // the rows add to C itself
   return gemm_lp_impl(dtype, M, N, K, alpha, A, lda, strideA, ia, strideIa, B, ldb, strideB, ib,
                       strideIb, 1.0, C, C, mk(1, ldc, 0), mk(1, 1, 0), 0, batch, 1, nullptr,
                       stream, crow);
@@ -1792,7 +1754,7 @@ int nl_gemm_lp_rows(int dtype, int64_t M, int64_t N, int64_t K, double alpha, co
 
 int nl_gemm_lp_strided(int dtype, int64_t K, double alpha, const void* A, int64_t lda,
                        const float* ia, const void* B, int64_t ldb, const float* ib, double beta,
-                       const double* D, double* C, const int64_t* cdims, const int64_t* sc,
+                       const void* D, void* C, const int64_t* cdims, const int64_t* sc,
                        int64_t splits, double* ws, void* stream) {
   if (!cdims || !sc) return NL_EARG;
   for (int d = 0; d < 4; ++d)
@@ -1807,7 +1769,7 @@ int nl_gemm_lp_split(int dtype, int64_t M, int64_t N, int64_t K, double alpha, c
                      const void* Alo, int64_t lda, int64_t strideA, const float* ia,
                      int64_t strideIa, const void* B, const void* Blo, int64_t ldb,
                      int64_t strideB, const float* ib, int64_t strideIb, double beta,
-                     const double* D, double* C, int64_t ldc, int64_t strideC, int64_t batch,
+                     const void* D, void* C, int64_t ldc, int64_t strideC, int64_t batch,
                      int64_t splits, double* ws, void* stream) {
   return gemm_lp_impl(dtype, M, N, K, alpha, A, lda, strideA, ia, strideIa, B, ldb, strideB, ib,
                       strideIb, beta, D, C, mk(1, ldc, 0), mk(1, 1, 0), strideC, batch, splits,
@@ -1816,8 +1778,8 @@ int nl_gemm_lp_split(int dtype, int64_t M, int64_t N, int64_t K, double alpha, c
 
 int nl_gemm_lp_strided_split(int dtype, int64_t K, double alpha, const void* A, const void* Alo,
                              int64_t lda, const float* ia, const void* B, const void* Blo,
-                             int64_t ldb, const float* ib, double beta, const double* D,
-                             double* C, const int64_t* cdims, const int64_t* sc, int64_t splits,
+                             int64_t ldb, const float* ib, double beta, const void* D,
+                             void* C, const int64_t* cdims, const int64_t* sc, int64_t splits,
                              double* ws, void* stream) {
   if (!cdims || !sc) return NL_EARG;
   for (int d = 0; d < 4; ++d)
@@ -1833,9 +1795,10 @@ int nl_gemm_lp_rows_split(int dtype, int64_t M, int64_t N, int64_t K, double alp
                           const void* A, const void* Alo, int64_t lda, int64_t strideA,
                           const float* ia, int64_t strideIa, const void* B, const void* Blo,
                           int64_t ldb, int64_t strideB, const float* ib, int64_t strideIb,
-                          double* C, int64_t ldc, const int64_t* crow, int64_t batch,
+                          void* C, int64_t ldc, const int64_t* crow, int64_t batch,
                           void* stream) {
   if (!crow || ldc < N) return NL_EARG;
+  if (dtype & (NL_C_F32 << 8)) dtype |= NL_D_F32 << 8;       // the rows add to C itself
   return gemm_lp_impl(dtype, M, N, K, alpha, A, lda, strideA, ia, strideIa, B, ldb, strideB, ib,
                       strideIb, 1.0, C, C, mk(1, ldc, 0), mk(1, 1, 0), 0, batch, 1, nullptr,
                       stream, crow, Alo, Blo, true);

@@ -185,7 +185,7 @@ struct SplitArgs {
   int64_t R2, C, C2, ldo, kpad;   // rows = R1*R2, cols C = C1*C2, row pitch ldo, block pitch kpad
   FastDiv fr2, fc2;
   int64_t s[4];
-  const double* in;
+  AnyIn in;                       // FP64 or FP32 source
   uint16_t* out;
   float* iscale;                  // per-row inverse scale (null: unscaled)
   float target;
@@ -215,13 +215,13 @@ __global__ __launch_bounds__(NT) void split_rows_kernel(SplitArgs a) {
   __shared__ float red[NT / 64];
   const int64_t r = blockIdx.x;
   const int64_t r1 = fdiv((int)r, a.fr2), r2 = r - r1 * a.R2;
-  const double* base = a.in + r1 * a.s[0] + r2 * a.s[1];
+  const AnyIn base = a.in.at(r1 * a.s[0] + r2 * a.s[1]);
   float sc = 1.f;
   if (a.iscale) {
     float m = 0.f;
     for (int64_t c = threadIdx.x; c < a.C; c += NT) {
       const int64_t c1 = fdiv((int)c, a.fc2), c2 = c - c1 * a.C2;
-      m = fmaxf(m, (float)fabs(base[c1 * a.s[2] + c2 * a.s[3]]));
+      m = fmaxf(m, (float)fabs(base.get(c1 * a.s[2] + c2 * a.s[3])));
     }
     m = wave_max(m);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
@@ -235,7 +235,7 @@ __global__ __launch_bounds__(NT) void split_rows_kernel(SplitArgs a) {
     uint16_t hi = 0, lo = 0;
     if (c < a.C) {
       const int64_t c1 = fdiv((int)c, a.fc2), c2 = c - c1 * a.C2;
-      split16(base[c1 * a.s[2] + c2 * a.s[3]] * ((double)sc * a.scale), a.bf16, hi, lo);
+      split16(base.get(c1 * a.s[2] + c2 * a.s[3]) * ((double)sc * a.scale), a.bf16, hi, lo);
     }
     for (int q = 0; q < a.nblk; ++q) o[q * a.kpad + c] = ((a.lomask >> q) & 1) ? lo : hi;
   }
@@ -250,12 +250,12 @@ __global__ __launch_bounds__(NT) void amax_split_kernel(SplitArgs a, int64_t R, 
   float m = 0.f;
   if (r < R) {
     const int64_t r1 = fdiv((int)r, a.fr2), r2 = r - r1 * a.R2;
-    const double* base = a.in + r1 * a.s[0] + r2 * a.s[1];
+    const AnyIn base = a.in.at(r1 * a.s[0] + r2 * a.s[1]);
     for (int cc = tg; cc < 64; cc += 4) {
       const int64_t c = c0 + cc;
       if (c < a.C) {
         const int64_t c1 = fdiv((int)c, a.fc2), c2 = c - c1 * a.C2;
-        m = fmaxf(m, (float)fabs(base[c1 * a.s[2] + c2 * a.s[3]]));
+        m = fmaxf(m, (float)fabs(base.get(c1 * a.s[2] + c2 * a.s[3])));
       }
     }
   }
@@ -275,13 +275,13 @@ __global__ __launch_bounds__(NT) void split_tiles_kernel(SplitArgs a, int64_t R,
   if (r < R) {
     const float sc = bits ? row_scale(__uint_as_float(bits[r]), a.target) : 1.f;
     const int64_t r1 = fdiv((int)r, a.fr2), r2 = r - r1 * a.R2;
-    const double* base = a.in + r1 * a.s[0] + r2 * a.s[1];
+    const AnyIn base = a.in.at(r1 * a.s[0] + r2 * a.s[1]);
     for (int cc = tg; cc < 64; cc += 4) {
       const int64_t c = c0 + cc;
       uint16_t hi = 0, lo = 0;
       if (c < a.C) {
         const int64_t c1 = fdiv((int)c, a.fc2), c2 = c - c1 * a.C2;
-        split16(base[c1 * a.s[2] + c2 * a.s[3]] * ((double)sc * a.scale), a.bf16, hi, lo);
+        split16(base.get(c1 * a.s[2] + c2 * a.s[3]) * ((double)sc * a.scale), a.bf16, hi, lo);
       }
       thi[tr][cc] = hi;
       tlo[tr][cc] = lo;
@@ -599,11 +599,12 @@ __global__ __launch_bounds__(NT) void tau_split_kernel(int p, int h, FastDiv fp,
 // G lanes per segment (G = the power of two >= X, at most 64), so a wavefront serves 64/G
 // segments; each lane keeps two partial sums (entries alternate between them, added in a fixed
 // order), which keeps two independent loads in flight along a segment.
+template <typename T>   // output elements: double, or float for an FP32 accumulator
 __global__ __launch_bounds__(NT) void segsum_kernel(int64_t S, int64_t X, int G,
                                                   const int64_t* ptr, const int64_t* so,
                                                   const double* val, const int64_t* dst,
                                                   int64_t ss, int64_t ds, double alpha,
-                                                  const double* src, double* out) {
+                                                  const double* src, T* out) {
   const int lane = threadIdx.x & 63, sub = lane / G, xl = lane - sub * G, per = 64 / G;
   const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
   for (int64_t wv = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6; wv * per < S;
@@ -619,7 +620,7 @@ __global__ __launch_bounds__(NT) void segsum_kernel(int64_t S, int64_t X, int G,
         acc1 += val[e + 1] * src[so[e + 1] + x * ss];
       }
       if (e < e1) acc0 += val[e] * src[so[e] + x * ss];
-      out[d + x * ds] += alpha * (acc0 + acc1);
+      out[d + x * ds] = (T)((double)out[d + x * ds] + alpha * (acc0 + acc1));
     }
   }
 }
@@ -684,7 +685,9 @@ __global__ void update_t1_kernel(int64_t p, int64_t h, const double* t1, const d
 struct AssembleArgs {
   int64_t p, h;
   FastDiv fp, fh;                       // divisors p, h
-  const double *S0, *Sab, *Sij, *Spp;   // S0, Sab, Sij [a][b][i][j]; Spp [(a,i)][(b,j)]; any null
+  const void *S0, *Sab, *Sij, *Spp;     // S0, Sab, Sij [a][b][i][j]; Spp [(a,i)][(b,j)]; any null
+                                        // (all four double, or all four float: the kernels'
+                                        // template type T)
   const double *Xpp, *Xhh;              // dressed diagonals (full p x p, h x h matrices)
   const double *dpp, *dhh;              // optional: (dpp[a] + dpp[b] + dhh[i] + dhh[j]) t2[a,b,i,j]
                                         // is added to H2 in FP64 (the diagonal of a dressing
@@ -699,6 +702,7 @@ struct AssembleArgs {
                                         // [hi(p*h*h) | lo(p*h*h)] (row pitch 2*p*h*h);
                                         // 1: BF16, 2: FP16 of t2n * img_scale
   double img_scale;
+  int addv;                             // H2 += v_pphh (when S0 does not hold it)
 };
 
 __device__ inline void write_image(const AssembleArgs& g, int64_t t, int64_t a, double tn) {
@@ -713,7 +717,16 @@ __device__ inline void write_image(const AssembleArgs& g, int64_t t, int64_t a, 
   g.t2n16[o + row] = lo;
 }
 
+template <typename T>   // accumulator elements: double, or float (the split form's)
 __global__ __launch_bounds__(NT) void assemble_t2_kernel(AssembleArgs g) {
+  const T* const S0t = static_cast<const T*>(g.S0);
+  const T* const Sabt = static_cast<const T*>(g.Sab);
+  const T* const Sijt = static_cast<const T*>(g.Sij);
+  const T* const Sppt = static_cast<const T*>(g.Spp);
+  auto S0p = [&](int64_t o) { return (double)S0t[o]; };
+  auto Sabp = [&](int64_t o) { return (double)Sabt[o]; };
+  auto Sijp = [&](int64_t o) { return (double)Sijt[o]; };
+  auto Sppp = [&](int64_t o) { return (double)Sppt[o]; };
   const int64_t p = g.p, h = g.h, hh = h * h, total = p * p * hh;
   double e_acc = 0.0, r_acc = 0.0;
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
@@ -721,16 +734,17 @@ __global__ __launch_bounds__(NT) void assemble_t2_kernel(AssembleArgs g) {
     int64_t j = t % h, r = t / h, i = r % h;
     r /= h;
     int64_t b = r % p, a = r / p;
-    double H = g.S0 ? g.S0[t] : 0.0;
-    if (g.Sab) H += g.Sab[t] - g.Sab[((b * p + a) * h + i) * h + j];
+    double H = g.S0 ? S0p(t) : 0.0;
+    if (g.Sab) H += Sabp(t) - Sabp(((b * p + a) * h + i) * h + j);
     const int64_t ph = p * h;
-    if (g.Sij) H += g.Sij[t] - g.Sij[((a * p + b) * h + j) * h + i];
+    if (g.Sij) H += Sijp(t) - Sijp(((a * p + b) * h + j) * h + i);
     if (g.Spp) {
-      H += g.Spp[(a * h + i) * ph + b * h + j] - g.Spp[(b * h + i) * ph + a * h + j]
-         - g.Spp[(a * h + j) * ph + b * h + i] + g.Spp[(b * h + j) * ph + a * h + i];
+      H += Sppp((a * h + i) * ph + b * h + j) - Sppp((b * h + i) * ph + a * h + j)
+         - Sppp((a * h + j) * ph + b * h + i) + Sppp((b * h + j) * ph + a * h + i);
     }
     double denom = -(g.Xpp[a * p + a] + g.Xpp[b * p + b] + g.Xhh[i * h + i] + g.Xhh[j * h + j]);
     double told = g.t2[t];
+    if (g.addv) H += g.vpphh[t];
     if (g.dpp) H += (g.dpp[a] + g.dpp[b]) * told;
     if (g.dhh) H += (g.dhh[i] + g.dhh[j]) * told;
     double nt = told + H / denom;
@@ -761,7 +775,16 @@ __global__ __launch_bounds__(NT) void assemble_t2_kernel(AssembleArgs g) {
 // H(b,a) uses the same staged blocks: its Spp part is minus that of H(a,b).
 constexpr int AH = 32;
 
+template <typename T>   // accumulator elements: double, or float (the split form's)
 __global__ __launch_bounds__(NT) void assemble_pair_kernel(AssembleArgs g) {
+  const T* const S0t = static_cast<const T*>(g.S0);
+  const T* const Sabt = static_cast<const T*>(g.Sab);
+  const T* const Sijt = static_cast<const T*>(g.Sij);
+  const T* const Sppt = static_cast<const T*>(g.Spp);
+  auto S0p = [&](int64_t o) { return (double)S0t[o]; };
+  auto Sabp = [&](int64_t o) { return (double)Sabt[o]; };
+  auto Sijp = [&](int64_t o) { return (double)Sijt[o]; };
+  auto Sppp = [&](int64_t o) { return (double)Sppt[o]; };
   __shared__ double SiA[AH][AH + 1], SiB[AH][AH + 1], SpA[AH][AH + 1], SpB[AH][AH + 1];
   const int64_t p = g.p, ph = p * g.h;
   const int h = (int)g.h, hh = h * h;
@@ -769,6 +792,7 @@ __global__ __launch_bounds__(NT) void assemble_pair_kernel(AssembleArgs g) {
   auto finish = [&](int64_t a, int64_t b, int i, int j, int64_t t, double H) {
     double denom = -(g.Xpp[a * p + a] + g.Xpp[b * p + b] + g.Xhh[i * h + i] + g.Xhh[j * h + j]);
     double told = g.t2[t];
+    if (g.addv) H += g.vpphh[t];
     if (g.dpp) H += (g.dpp[a] + g.dpp[b]) * told;
     if (g.dhh) H += (g.dhh[i] + g.dhh[j]) * told;
     double tn = g.mixing * told + (1.0 - g.mixing) * (told + H / denom);
@@ -787,19 +811,19 @@ __global__ __launch_bounds__(NT) void assemble_pair_kernel(AssembleArgs g) {
     for (int e = threadIdx.x; e < hh; e += NT) {
       const int i = fdiv(e, g.fh), j = e - i * h;
       const int64_t oab = (a * g.h + i) * ph + b * g.h + j, oba = (b * g.h + i) * ph + a * g.h + j;
-      SiA[i][j] = g.Sij ? g.Sij[(a * p + b) * hh + e] : 0.0;
-      SiB[i][j] = g.Sij ? g.Sij[(b * p + a) * hh + e] : 0.0;
-      SpA[i][j] = g.Spp ? g.Spp[oab] : 0.0;
-      SpB[i][j] = g.Spp ? g.Spp[oba] : 0.0;
+      SiA[i][j] = g.Sij ? Sijp((a * p + b) * hh + e) : 0.0;
+      SiB[i][j] = g.Sij ? Sijp((b * p + a) * hh + e) : 0.0;
+      SpA[i][j] = g.Spp ? Sppp(oab) : 0.0;
+      SpB[i][j] = g.Spp ? Sppp(oba) : 0.0;
     }
     __syncthreads();
     for (int e = threadIdx.x; e < hh; e += NT) {
       const int i = fdiv(e, g.fh), j = e - i * h;
       const int64_t tab = (a * p + b) * hh + e, tba = (b * p + a) * hh + e;
-      double Hab = g.S0 ? g.S0[tab] : 0.0;
-      double Hba = g.S0 ? g.S0[tba] : 0.0;
+      double Hab = g.S0 ? S0p(tab) : 0.0;
+      double Hba = g.S0 ? S0p(tba) : 0.0;
       if (g.Sab) {
-        const double x = g.Sab[tab], y = g.Sab[tba];
+        const double x = Sabp(tab), y = Sabp(tba);
         Hab += x - y;
         Hba += y - x;
       }
@@ -1080,9 +1104,13 @@ int nl_convert_rows(int dtype, int64_t R1, int64_t R2, int64_t C1, int64_t C2,
 }
 
 int nl_convert_split(int dtype, int64_t R1, int64_t R2, int64_t C1, int64_t C2,
-                     const int64_t* strides4, const double* in, void* out, int64_t ldo,
+                     const int64_t* strides4, const void* in, void* out, int64_t ldo,
                      int64_t kpad, int nblk, int lomask, float* iscale, float target,
                      double scale, void* stream) {
+  // dtype: 16-bit type (bits 0-7) | NL_D_F32 << 8 for an FP32 source (else FP64)
+  const bool in32 = (dtype >> 8) == NL_D_F32;
+  if (dtype >> 8 && !in32) return NL_EARG;
+  dtype &= 0xff;
   const bool scaled = target != 0.f;
   if ((dtype != 0 && dtype != 1) || !strides4 || !in || !out || (scaled && !iscale))
     return NL_EARG;
@@ -1094,7 +1122,8 @@ int nl_convert_split(int dtype, int64_t R1, int64_t R2, int64_t C1, int64_t C2,
   a.R2 = R2; a.C = C1 * C2; a.C2 = C2; a.ldo = ldo; a.kpad = kpad;
   a.fr2 = nl_fastdiv(R2); a.fc2 = nl_fastdiv(C2);
   for (int d = 0; d < 4; ++d) a.s[d] = strides4[d];
-  a.in = in; a.out = (uint16_t*)out; a.iscale = scaled ? iscale : nullptr; a.target = target;
+  a.in = AnyIn{in, in32}; a.out = (uint16_t*)out; a.iscale = scaled ? iscale : nullptr;
+  a.target = target;
   a.nblk = nblk; a.lomask = lomask; a.bf16 = dtype == 0; a.scale = scale;
   hipStream_t st = (hipStream_t)stream;
   const int64_t R = R1 * R2;
@@ -1336,16 +1365,22 @@ int nl_tau(int64_t p, int64_t h, const double* t1, const double* t2, double* tau
 
 int nl_segsum(int64_t S, int64_t X, const int64_t* ptr, const int64_t* src_off, const double* val,
               const int64_t* dst_off, int64_t src_stride, int64_t dst_stride, double alpha,
-              const double* src, double* out, void* stream) {
-  if (S < 0 || X < 0) return NL_EARG;
+              const double* src, void* out, int io, void* stream) {
+  if (S < 0 || X < 0 || (io & ~NL_C_F32)) return NL_EARG;
   if (S == 0 || X == 0) return NL_OK;
   if (!ptr || !src_off || !val || !dst_off || !src || !out) return NL_EARG;
   int G = 1;
   while (G < X && G < 64) G *= 2;
   const int64_t waves = (S + 64 / G - 1) / (64 / G);
   int blocks = grid_for(waves * 64);
-  hipLaunchKernelGGL(segsum_kernel, dim3(blocks), dim3(NT), 0, (hipStream_t)stream, S, X, G,
-                     ptr, src_off, val, dst_off, src_stride, dst_stride, alpha, src, out);
+  if (io & NL_C_F32)
+    hipLaunchKernelGGL(segsum_kernel<float>, dim3(blocks), dim3(NT), 0, (hipStream_t)stream, S,
+                       X, G, ptr, src_off, val, dst_off, src_stride, dst_stride, alpha, src,
+                       (float*)out);
+  else
+    hipLaunchKernelGGL(segsum_kernel<double>, dim3(blocks), dim3(NT), 0, (hipStream_t)stream, S,
+                       X, G, ptr, src_off, val, dst_off, src_stride, dst_stride, alpha, src,
+                       (double*)out);
   NL_LAUNCH_CHECK();
   return NL_OK;
 }
@@ -1397,29 +1432,35 @@ int nl_update_t1(int64_t p, int64_t h, const double* t1, const double* H1, const
 // Number of partial-sum slots (per quantity) nl_assemble_t2 / nl_energy / nl_dot use.
 int64_t nl_partials(int64_t n) { return grid_for(n, 4096); }
 
-static int assemble_t2_impl(int64_t p, int64_t h, const double* S0, const double* Sab,
-                            const double* Sij, const double* Spp, const double* Xpp,
+static int assemble_t2_impl(int64_t p, int64_t h, const void* S0, const void* Sab,
+                            const void* Sij, const void* Spp, const double* Xpp,
                             const double* Xhh, const double* t2, const double* vpphh,
                             const double* t1n, double mixing, double* t2n, double* H2out,
                             double* part, uint16_t* t2n16, void* stream,
                             const double* dpp = nullptr, const double* dhh = nullptr,
-                            int split16 = 0, double img_scale = 1.0) {
+                            int split16 = 0, double img_scale = 1.0, int io = 0) {
   if (p < 0 || h < 0 || !Xpp || !Xhh || !t2 || !part) return NL_EARG;
   if (vpphh && !t1n) return NL_EARG;
+  if ((io & ~(NL_ACC_F32 | NL_ADD_V)) || ((io & NL_ADD_V) && !vpphh)) return NL_EARG;
   int64_t n = p * p * h * h;
   if (n == 0) return NL_OK;
   AssembleArgs g;
-  g.p = p; g.h = h; g.S0 = S0; g.Sab = Sab; g.Sij = Sij; g.Spp = Spp; g.Xpp = Xpp; g.Xhh = Xhh;
+  const bool f32 = (io & NL_ACC_F32) != 0;
+  g.p = p; g.h = h; g.S0 = S0; g.Sab = Sab; g.Sij = Sij; g.Spp = Spp;
+  g.Xpp = Xpp; g.Xhh = Xhh; g.addv = (io & NL_ADD_V) != 0;
   g.t2 = t2; g.vpphh = vpphh; g.t1n = t1n; g.mixing = mixing; g.t2n = t2n; g.part = part;
   g.H2out = H2out; g.t2n16 = t2n16; g.dpp = dpp; g.dhh = dhh; g.split16 = split16;
   g.img_scale = img_scale;
   g.fp = nl_fastdiv(p); g.fh = nl_fastdiv(h);
-  if (h <= AH && (Sij || Spp) && p * p < INT32_MAX)
-    hipLaunchKernelGGL(assemble_pair_kernel, dim3(grid_for(n, 4096)), dim3(NT), 0,
-                       (hipStream_t)stream, g);
-  else
-    hipLaunchKernelGGL(assemble_t2_kernel, dim3(grid_for(n, 4096)), dim3(NT), 0,
-                       (hipStream_t)stream, g);
+  const dim3 grid(grid_for(n, 4096));
+  hipStream_t st = (hipStream_t)stream;
+  if (h <= AH && (Sij || Spp) && p * p < INT32_MAX) {
+    if (f32) hipLaunchKernelGGL(assemble_pair_kernel<float>, grid, dim3(NT), 0, st, g);
+    else hipLaunchKernelGGL(assemble_pair_kernel<double>, grid, dim3(NT), 0, st, g);
+  } else {
+    if (f32) hipLaunchKernelGGL(assemble_t2_kernel<float>, grid, dim3(NT), 0, st, g);
+    else hipLaunchKernelGGL(assemble_t2_kernel<double>, grid, dim3(NT), 0, st, g);
+  }
   NL_LAUNCH_CHECK();
   return NL_OK;
 }
@@ -1441,15 +1482,15 @@ int nl_assemble_t2_16(int64_t p, int64_t h, const double* S0, const double* Sab,
                           part, (uint16_t*)t2n16, stream);
 }
 
-int nl_assemble_t2_ext(int64_t p, int64_t h, const double* S0, const double* Sab,
-                       const double* Sij, const double* Spp, const double* Xpp, const double* Xhh,
+int nl_assemble_t2_ext(int64_t p, int64_t h, const void* S0, const void* Sab,
+                       const void* Sij, const void* Spp, const double* Xpp, const double* Xhh,
                        const double* dpp, const double* dhh, const double* t2,
                        const double* vpphh, const double* t1n, double mixing, double* t2n,
                        double* H2out, double* part, void* t2n16, int image_kind,
-                       double image_scale, void* stream) {
+                       double image_scale, int io, void* stream) {
   if (image_kind < 0 || image_kind > 2) return NL_EARG;
   return assemble_t2_impl(p, h, S0, Sab, Sij, Spp, Xpp, Xhh, t2, vpphh, t1n, mixing, t2n, H2out,
-                          part, (uint16_t*)t2n16, stream, dpp, dhh, image_kind, image_scale);
+                          part, (uint16_t*)t2n16, stream, dpp, dhh, image_kind, image_scale, io);
 }
 
 __global__ void take_diag_kernel(int64_t n, double* X, double* d) {

@@ -44,6 +44,9 @@ from .plans import PairBlocks, PpphPlans, coo_arrays
 
 F64 = torch.float64
 
+# nl_assemble_t2_ext flags: FP32 accumulators; v_pphh added to H2 by the assembly
+ACC_F32, ADD_V = 1, 2
+
 
 def _dev(x, device):
     if isinstance(x, torch.Tensor):
@@ -172,14 +175,38 @@ class Buffers:
         self.Ubkij = z(p, h, h, h)
         self.part = z(2 * int(_lib.load().nl_partials(p * p * h * h)) + 8)
         self.E = z(1)
+        self.p, self.h, self.device = p, h, device
+        self._acc32 = None
+
+    def acc32(self):
+        """The FP32 accumulators of the split precision form (allocated on first use): S0, Sab,
+        Sij, Spp and the ring intermediate W, with the FP64 buffers' layouts.  Their products
+        accumulate in FP32 (16-bit MFMA) or FP64 (the rest) and are rounded once per pass, so
+        each read-modify-write pass over them moves half the bytes of an FP64 one."""
+        if self._acc32 is None:
+            p, h = self.p, self.h
+            z = lambda *s: torch.zeros(s, dtype=torch.float32, device=self.device)
+            # W is the ring product's operand: FP32 only where that product runs on the 16-bit
+            # matrix cores (which read it through a split conversion); thinner rings are FP64
+            W = z(p, h, p, h) if p * h >= ops.LP_MIN_DIM else self.W
+            self._acc32 = Acc(z(p, p, h, h), z(p, p, h, h), z(p, p, h, h), z(p, h, p, h), W)
+        return self._acc32
+
+
+class Acc:
+    """A set of T2-sized accumulators (S0, Sab, Sij as [a][b][i][j]; Spp, W as [a][i][b][j])."""
+
+    def __init__(self, S0, Sab, Sij, Spp, W):
+        self.S0, self.Sab, self.Sij, self.Spp, self.W = S0, Sab, Sij, Spp, W
 
 
 def _segsum(plan, X, src, out, alpha, src_stride=1, dst_stride=1):
     if plan is None:
         return
+    io = ops.C_F32 if out.dtype == torch.float32 else 0
     _lib.call("nl_segsum", plan.S, X, ops._p(plan.ptr), ops._p(plan.src), ops._p(plan.val),
               ops._p(plan.dst), src_stride, dst_stride, ctypes.c_double(alpha), ops._p(src),
-              ops._p(out), ops._stream())
+              ops._p(out), io, ops._stream())
 
 
 DEFAULT_PREC = {"ring": "fp64", "hh": "fp64", "dress": "fp64", "ladder": "fp64"}
@@ -190,6 +217,10 @@ DIAG_SPLIT = True
 
 # Form tau directly in BF16 when every product that reads it runs in BF16 (False: always FP64).
 TAU16 = True
+
+# The split form (shared split images) keeps S0, Sab, Sij, Spp and W in FP32 (False: FP64, for
+# checks that isolate the images from the accumulators' rounding).
+ACC32 = True
 
 
 def t1_part(ham, t1, t2, B, prec=DEFAULT_PREC):
@@ -325,13 +356,16 @@ def t2_part(ham, t1, t2, B, prec=DEFAULT_PREC):
     ops.axpby(1.0, B.Dhh, 1.0, Xhh)                                          # A42 (= A18)
     contract("ck,cj->kj", ham.f_ph, t1, Xhh, alpha=-1.0)                     # A44
     contract("dk,dj->kj", B.u_ph, t1, Xhh, alpha=-1.0)                      # A45
-    # ---- S0: v_pphh + 1/2 tau.Whh + 1/2 pp ladder(tau)
-    S0 = B.S0
+    # ---- S0: v_pphh + 1/2 tau.Whh + 1/2 pp ladder(tau).  The split form accumulates S0, Sab,
+    # Sij, Spp and W in FP32 and leaves v_pphh out of S0: the assembly adds it in FP64
+    acc32 = split is not None and ACC32
+    acc = B.acc32() if acc32 else B
+    S0 = acc.S0
     permute_into(ham.v_hhhh, B.Whh)
     contract("ci,cjkl->klij", t1, ham.v_phhh, B.zz, beta=0.0)
     permute_into(B.zz, B.Whh, 1.0, 1.0)
     permute_into(B.zz.permute(0, 1, 3, 2), B.Whh, -1.0, 1.0)
-    Sab = B.Sab
+    Sab = acc.Sab
     with ops.lp_reuse(tau, absent=True) if lp_tau else contextlib.nullcontext():
         if split is not None:
             sc = AMP_SCALE[split]
@@ -343,8 +377,12 @@ def t2_part(ham, t1, t2, B, prec=DEFAULT_PREC):
             ops.lp_provide(tau, "abkl", ["a", "b"], ["k", "l"], tau16, h * h)
         contract("cdkl,cdij->klij", ham.v_pphh, tau, B.Whh, alpha=0.5,
                  precision=prec.get("hh_W", prec["hh"]))
-        contract("abkl,klij->abij", tau, B.Whh, S0, alpha=0.5,
-                 precision=prec.get("hh_S", prec["hh"]), addend=ham.v_pphh)
+        if acc32:
+            contract("abkl,klij->abij", tau, B.Whh, S0, alpha=0.5, beta=0.0,
+                     precision=prec.get("hh_S", prec["hh"]))
+        else:
+            contract("abkl,klij->abij", tau, B.Whh, S0, alpha=0.5,
+                     precision=prec.get("hh_S", prec["hh"]), addend=ham.v_pphh)
         _pp_ladder(ham, tau, S0, prec, tau16)
         # ---- Sab: P(ab) group
         permute_into(ham.v_phhh, B.N56)
@@ -367,14 +405,14 @@ def t2_part(ham, t1, t2, B, prec=DEFAULT_PREC):
         contract("bkij,ak->abij", B.N56, t1, Sab, beta=0.0)                  # A23 + A39
         contract("ad,dbij->abij", B.Mpp, t2, Sab, precision=prec["dress"])
     # ---- Sij: P(ij) group, [a][b][i][j]
-    Sij = B.Sij
+    Sij = acc.Sij
     contract("cikl,ck->li", ham.v_phhh, t1, B.q_hh, beta=0.0)                # A28
     _lib.call("nl_sub", h * h, ops._p(Xhh), ops._p(B.q_hh), ops._p(B.Mhh), ops._stream())
     contract("kj,abik->abij", B.Mhh, t2, Sij, beta=0.0)                      # X_hh t2 - A28 t2
     if pl.n:
-        _segsum(pl.ret2p, h, t1, B.Sij, 1.0)                                 # A36
+        _segsum(pl.ret2p, h, t1, Sij, 1.0)                                   # A36
     # ---- Spp: P(ij)P(ab) group in [(a,i)][(b,j)] layout
-    W = B.W
+    W = acc.W
     # t2[d,b,l,j] = t2[b,d,j,l]: ring_W reads t2 in the ring's own [(b,j)][(d,l)] rows, so the
     # two ring GEMMs share one 16-bit conversion of it
     # the FP64 ring GEMMs read t2 as [(a,i)][(c,k)] rows of a permuted copy (long contiguous K
@@ -391,14 +429,14 @@ def t2_part(ham, t1, t2, B, prec=DEFAULT_PREC):
         if pl.n:
             _segsum(pl.ret4, h, t1, W, 1.0)                                  # A37
         contract("cjkl,bl->ckbj", ham.v_phhh, t1, W, alpha=-1.0)             # A29
-        contract("acik,ckbj->aibj", t2ring, W, B.Spp, beta=0.0,
+        contract("acik,ckbj->aibj", t2ring, W, acc.Spp, beta=0.0,
                  precision=prec.get("ring_S", prec["ring"]), tag="ring")
     contract("idkl,adkj->ilaj", B.Xidkl, t2, B.Yilaj, beta=0.0, precision=prec["hh"])  # A34
     # Spp enters only through P(ij)P(ab), which is invariant under (a,i) <-> (b,j): the A27 term
     # -t1[a,k] U[b,k,i,j] may stand as -U[a,l,j,i] t1[b,l], so both tails are one product
     permute_into(B.Yilaj, B.Zilaj)
     permute_into(B.Ubkij.permute(3, 1, 0, 2), B.Zilaj, alpha=-1.0, beta=1.0)
-    contract("ilaj,bl->aibj", B.Zilaj, t1, B.Spp)
+    contract("ilaj,bl->aibj", B.Zilaj, t1, acc.Spp)
 
 
 def _sab_bf16(ham, t1, B, Sab):
@@ -452,6 +490,9 @@ def _block_gemm(pb, src, out, alpha, ncol, tag, precision="fp64", src16=None):
     # split operands with src16 given: src16 rows are [hi(ncol) | lo(ncol)] of src * its
     # fixed scale (AMP_SCALE)
     split16 = precision in ("bf16x3", "fp16x3") and src16 is not None
+    out32 = (ops.C_F32 << 8) if out.dtype == torch.float32 else 0
+    if out32 and precision != "bf16" and not split16:
+        raise _lib.NLError("_block_gemm: an FP32 output takes only the row-scatter products")
     if precision != "bf16" and not split16:
         G = ops.WS.get(tag + "_G", pb.nb * pb.kb * ncol, device=src.device)
         R = ops.WS.get(tag + "_R", pb.nb * pb.mb * ncol, device=src.device)
@@ -504,12 +545,12 @@ def _block_gemm(pb, src, out, alpha, ncol, tag, precision="fp64", src16=None):
             if split16:
                 # V (exact, its first K block) x [hi | lo] of the gathered rows: one split GEMM
                 # with the row-scatter epilogue (V.hi + V.lo per K step)
-                _lib.call("nl_gemm_lp_rows_split", dt, pb.mb, ncol, kpad,
+                _lib.call("nl_gemm_lp_rows_split", dt | out32, pb.mb, ncol, kpad,
                           ctypes.c_double(a_eff), ops._p(Vb), None, ldk, pb.mb * ldk,
                           ops._p(iv), pb.mb, ops._p(Gt), ops._off16(Gt, kpad), ldk, ncol * ldk,
                           None, 0, ops._p(out), ncol, ops._p(pb.rows), pb.nb, st)
             else:
-                _lib.call("nl_gemm_lp_rows", dt, pb.mb, ncol, ldk, ctypes.c_double(a_eff),
+                _lib.call("nl_gemm_lp_rows", dt | out32, pb.mb, ncol, ldk, ctypes.c_double(a_eff),
                           ops._p(Vb), ldk, pb.mb * ldk, ops._p(iv), pb.mb, ops._p(Gt), ldk,
                           ncol * ldk, None, 0, ops._p(out), ncol, ops._p(pb.rows), pb.nb, st)
             if ev:
@@ -683,12 +724,15 @@ def assemble(ham, t2, B, t1n, t2n, mixing, H2out=None, img=None, prec=None):
     in BF16."""
     p, h = ham.p, ham.h
     st = ops._stream()
-    args = (p, h, ops._p(B.S0), ops._p(B.Sab), ops._p(B.Sij), ops._p(B.Spp), ops._p(B.X2pp),
+    dt = _fast_split(ham, prec) if prec is not None else None
+    # the split form's FP32 accumulators, with v_pphh added here (S0 does not hold it)
+    acc, io = (B.acc32(), ACC_F32 | ADD_V) if dt is not None and ACC32 else (B, 0)
+    args = (p, h, ops._p(acc.S0), ops._p(acc.Sab), ops._p(acc.Sij), ops._p(acc.Spp),
+            ops._p(B.X2pp),
             ops._p(B.X2hh), ops._p(getattr(B, "dpp", None)), None, ops._p(t2),
             ops._p(ham.v_pphh), ops._p(t1n), ctypes.c_double(mixing), ops._p(t2n),
             ops._p(H2out), ops._p(B.part))
     buf, kind, sc = None, 0, 1.0
-    dt = _fast_split(ham, prec) if prec is not None else None
     if img is not None and t2n is not None and dt is not None:
         nxt = 1 - img.cur
         buf, kind, sc = img.buffer(nxt, 2 * p * p * h * h, t2n.device), 1 + dt, AMP_SCALE[dt]
@@ -703,7 +747,7 @@ def assemble(ham, t2, B, t1n, t2n, mixing, H2out=None, img=None, prec=None):
     ev = ops._timed("assemble")
     if ev:
         ev[0].record()
-    _lib.call("nl_assemble_t2_ext", *args, ops._p(buf), kind, ctypes.c_double(sc), st)
+    _lib.call("nl_assemble_t2_ext", *args, ops._p(buf), kind, ctypes.c_double(sc), io, st)
     if ev:
         ev[1].record()
     nparts = int(_lib.load().nl_partials(p * p * h * h))

@@ -35,8 +35,23 @@ def _stream():
 def _check(t, name, dtype=torch.float64):
     if not isinstance(t, torch.Tensor) or not t.is_cuda:
         raise _lib.NLError(f"{name}: expected a HIP device tensor (no CPU fallback)")
-    if t.dtype != dtype:
+    if t.dtype not in (dtype if isinstance(dtype, tuple) else (dtype,)):
         raise _lib.NLError(f"{name}: expected {dtype}, got {t.dtype}")
+
+
+# Element-type flags of a product's output C and addend D (include/nulattice_ccsd.h): float
+# instead of double.  The precision form keeps its T2-sized accumulators in FP32.
+C_F32, D_F32 = 1, 2
+F64_OR_F32 = (torch.float64, torch.float32)
+
+
+def _io(out, addend, beta):
+    """io flags of out = alpha A.B + beta (addend or out)."""
+    io = C_F32 if out.dtype == torch.float32 else 0
+    d = addend if addend is not None else out
+    if beta != 0.0 and d.dtype == torch.float32:
+        io |= D_F32
+    return io
 
 
 class Workspace:
@@ -356,9 +371,9 @@ def contract(spec, A, B, out, alpha=1.0, beta=1.0, precision="fp64", tag=None, a
     """
     la, lb, lo = _parse(spec)
     _timed.spec = spec
-    _check(A, "A"); _check(B, "B"); _check(out, "out")
+    _check(A, "A", F64_OR_F32); _check(B, "B", F64_OR_F32); _check(out, "out", F64_OR_F32)
     if addend is not None:
-        _check(addend, "addend")
+        _check(addend, "addend", F64_OR_F32)
         if tuple(addend.shape) != tuple(out.shape) or addend.stride() != out.stride():
             raise _lib.NLError(f"contract {spec}: addend must match out's shape and strides")
     if len(la) != A.dim() or len(lb) != B.dim() or len(lo) != out.dim():
@@ -379,6 +394,8 @@ def contract(spec, A, B, out, alpha=1.0, beta=1.0, precision="fp64", tag=None, a
         M = math.prod(size[c] for c in Mi); N = math.prod(size[c] for c in Ni)
         if min(M, N) < LP_MIN_DIM or any(c in la and c in lb for c in lo):
             precision = "fp64"      # thin or batched products stay exact
+    if precision == "fp64" and (A.dtype != torch.float64 or B.dtype != torch.float64):
+        raise _lib.NLError(f"contract {spec}: FP32 operands are read by the 16-bit products only")
     if ABSENT_PTRS and precision == "fp64" and (A.data_ptr() in ABSENT_PTRS or
                                                 B.data_ptr() in ABSENT_PTRS):
         raise _lib.NLError(f"contract {spec}: FP64 read of a tensor whose FP64 values are absent")
@@ -387,6 +404,7 @@ def contract(spec, A, B, out, alpha=1.0, beta=1.0, precision="fp64", tag=None, a
         K = math.prod(size[c] for c in Ki)
         return _contract_lp(A, la, B, lb, out, lo, Mi, Ni, Ki, M, N, K, alpha, beta, precision,
                             tag, addend)
+    io = _io(out, addend, beta)
     # choose the operand order with the best coalescing among the copy-free plans
     cands = []
     for (X, lx, Y, ly) in ((A, la, B, lb), (B, lb, A, la)):
@@ -415,6 +433,8 @@ def contract(spec, A, B, out, alpha=1.0, beta=1.0, precision="fp64", tag=None, a
             tv = tmp.permute(*[order.index(c) for c in lo])
             pl = _plan(X, lx, Y, ly, tv, lo, size)
             direct = False
+            if io:
+                raise _lib.NLError(f"contract {spec}: an FP32 output must be addressable directly")
     dims, sa, sb, sc = pl["dims"], pl["sa"], pl["sb"], pl["sc"]
     arr = lambda v: (ctypes.c_int64 * len(v))(*[int(x) for x in v])
     nws = int(_lib.load().nl_contract_ws(arr(dims)))
@@ -428,7 +448,7 @@ def contract(spec, A, B, out, alpha=1.0, beta=1.0, precision="fp64", tag=None, a
     count_flops("fp64", 2.0 * math.prod(dims))
     _lib.call("nl_contract_f64", arr(dims), ctypes.c_double(alpha), _p(X), arr(sa), _p(Y),
               arr(sb), ctypes.c_double(beta if direct else 0.0), _p(D), _p(C), arr(sc), splits,
-              _p(ws), _stream())
+              _p(ws), io, _stream())
     if ev:
         ev[1].record()
     if not direct:
@@ -576,7 +596,7 @@ def to_lp_rows(t, labels, rows, cols, dtype, tag, parts=(1, 0)):
         raise _lib.NLError("to_lp_rows: no 16-bit image registered for this view of a tensor "
                            "whose FP64 values are not formed")
     if v is None:
-        t = _as_layout(t, labels, rows + cols); labels = "".join(rows + cols)
+        t, labels = _copy_layout(t, labels, rows + cols)
         v = _view4(t, labels, rows, cols)
     R1, R2, C1, C2, st = v
     R, Ck = R1 * R2, C1 * C2
@@ -593,10 +613,13 @@ def to_lp_rows(t, labels, rows, cols, dtype, tag, parts=(1, 0)):
         isc = WS.get(tag + "_s", R, dtype=torch.float32, device=t.device) if scaled else None
     s4 = (ctypes.c_int64 * 4)(*st)
     if parts == (1, 0):
+        if t.dtype != torch.float64:
+            raise _lib.NLError("to_lp_rows: the single-rounding conversion reads FP64")
         _lib.call("nl_convert_rows", dtype, R1, R2, C1, C2, s4, ctypes.c_void_p(t.data_ptr()),
                   _p(buf), ldk, _p(isc), ctypes.c_float(target), _stream())
     else:
-        _lib.call("nl_convert_split", dtype, R1, R2, C1, C2, s4, ctypes.c_void_p(t.data_ptr()),
+        src_t = dtype | ((D_F32 << 8) if t.dtype == torch.float32 else 0)   # FP32 source
+        _lib.call("nl_convert_split", src_t, R1, R2, C1, C2, s4, ctypes.c_void_p(t.data_ptr()),
                   _p(buf), ldk, kpad, nblk, mask, _p(isc), ctypes.c_float(target),
                   ctypes.c_double(1.0), _stream())
     if key is not None:
@@ -644,8 +667,12 @@ def _contract_lp(A, la, B, lb, out, lo, Mi, Ni, Ki, M, N, K, alpha, beta, precis
         if rm is not None and rn is not None:
             runs = (rm, rn)
     direct = ldc is not None or runs is not None
+    io = _io(out, addend, beta)
+    if io and not direct:
+        raise _lib.NLError("contract: an FP32 output must be addressable directly")
     C = out if direct else WS.get("contract_out", M * N, device=out.device).view(M, N)
     D = (addend if addend is not None else out) if (direct and beta != 0.0) else None
+    dt = dt | (io << 8)             # C / D element types ride in the dtype argument
     splits = _lp_splits(M, N, Kg, 1)
     ws = WS.get("splitk", splits * M * N, device=out.device) if splits > 1 else None
     count_flops("lp", 2.0 * M * N * (3 * Kg if both else lda))

@@ -191,7 +191,7 @@ def test_permute_tau_segsum():
     L = lambda a: torch.tensor(a, dtype=torch.int64, device="cuda")
     keep = [L(ptr), L(so), dev(val), L(dst), dev(src)]   # alive until after the launch
     _lib.call("nl_segsum", 3, 4, ops._p(keep[0]), ops._p(keep[1]), ops._p(keep[2]),
-              ops._p(keep[3]), 1, 2, ctypes.c_double(0.5), ops._p(keep[4]), ops._p(od),
+              ops._p(keep[3]), 1, 2, ctypes.c_double(0.5), ops._p(keep[4]), ops._p(od), 0,
               ops._stream())
     torch.cuda.synchronize()
     np.testing.assert_allclose(host(od), oref, rtol=1e-14, atol=1e-14)
@@ -220,7 +220,7 @@ def test_segsum_lane_packing(X):
     L = lambda a: torch.tensor(a, dtype=torch.int64, device="cuda")
     keep = [L(ptr), L(so), dev(val), L(dst), dev(src), dev(out0)]
     _lib.call("nl_segsum", S, X, ops._p(keep[0]), ops._p(keep[1]), ops._p(keep[2]),
-              ops._p(keep[3]), ss, ds, ctypes.c_double(0.7), ops._p(keep[4]), ops._p(keep[5]),
+              ops._p(keep[3]), ss, ds, ctypes.c_double(0.7), ops._p(keep[4]), ops._p(keep[5]), 0,
               ops._stream())
     np.testing.assert_allclose(host(keep[5]), ref, rtol=1e-13, atol=1e-13)
 
@@ -340,7 +340,7 @@ def test_error_codes():
     assert lib.nl_gemm_lp(0, 8, 8, 7, 1.0, None, 8, 0, None, 0, None, 8, 0, None, 0, 0.0, None,
                           None, 8, 0, 1, 1, None, None) == -1
     assert lib.nl_contract_f64(None, 1.0, None, None, None, None, 0.0, None, None, None, 1, None,
-                               None) == -1
+                               0, None) == -1
 
 
 @pytest.mark.parametrize("k,n", [(1, 1000), (5, 1), (10, 300_001), (12, 4096 * 256 + 3),

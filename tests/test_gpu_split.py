@@ -254,7 +254,7 @@ def test_assemble_ext_diagonals_and_split_image(kind, scale):
                        device="cuda")
     img = torch.empty(2 * p * p * h * h, dtype=torch.int16, device="cuda")
     rc = L_().nl_assemble_t2_ext(p, h, *[P(x) for x in g], ctypes.c_double(m), P(t2n), P(H2),
-                                 P(part), P(img), kind, ctypes.c_double(scale), S())
+                                 P(part), P(img), kind, ctypes.c_double(scale), 0, S())
     assert rc == 0
     torch.cuda.synchronize()
     np.testing.assert_allclose(H2.cpu().numpy(), H, rtol=1e-13, atol=1e-13)
@@ -286,18 +286,24 @@ def test_split_images_equal_per_product_conversions(prec, tol):
     assembly, permuted in 16 bits) and of tau give the same steps as converting every operand
     per product: BF16 to summation order (1e-12: the same roundings); FP16 to the split's
     precision (1e-7: the shared images use the fixed scale 2^12, the per-product conversions
-    per-row scales, so the roundings differ)."""
+    per-row scales, so the roundings differ), with the shared-image path's accumulators in FP64
+    as the per-product path's; with its FP32 accumulators (the default) to 1e-6."""
     E0, fock, v = planewave_hamiltonian(3, 28, -4.0, 1.0)
     H = engine.Hamiltonian(fock, v, device="cuda")
     fast = ccm._PREC[prec]
     assert engine._fast_split(H, fast) is not None
     slow = dict(fast, ring_W=prec)               # any sub-family key disables the shared images
     assert engine._fast_split(H, slow) is None
-    ef, tf = _steps(H, fast, 4)
     es, ts = _steps(H, slow, 4)
-    np.testing.assert_allclose(ef, es, rtol=tol)
     scale = float(ts.abs().max())
-    assert float((tf - ts).abs().max()) <= tol * scale
+    for acc32, tl in ((False, tol), (True, 1e-6)):
+        engine.ACC32 = acc32
+        try:
+            ef, tf = _steps(H, fast, 4)
+        finally:
+            engine.ACC32 = True
+        np.testing.assert_allclose(ef, es, rtol=tl)
+        assert float((tf - ts).abs().max()) <= tl * scale
 
 
 def test_precision_form_L5_A28_meets_1e6():
