@@ -235,6 +235,12 @@ int nl_gather_rows(int64_t R, int64_t Cn, const int64_t* idx, const double* src,
 int nl_scatter_add_rows(int64_t R, int64_t Cn, const int64_t* idx, const double* src, double alpha,
                         double* out, int64_t ldo, void* stream);
 
+/* out[r][c] = A[r][c] for c < n1, B[r][c - n1] for n1 <= c < n1 + n2 (FP64, row pitches lda,
+ * ldb, ldo): the rows [t1 | X_pp] of the K-concatenated operand that runs t2Iter's two P(ab)
+ * products (coupled_cluster.py:519-536, the A23/A39 and X_pp dressings) as one GEMM. */
+int nl_concat_rows(int64_t R, int64_t n1, const double* A, int64_t lda, int64_t n2,
+                   const double* B, int64_t ldb, double* out, int64_t ldo, void* stream);
+
 /* t1n = m*t1 + (1-m)*(t1 + H1 / (-(Xpp[a,a] + Xhh[i,i])))
  * (coupled_cluster.py:410-414 and the mixing of :614). */
 int nl_update_t1(int64_t p, int64_t h, const double* t1, const double* H1, const double* Xpp,

@@ -572,17 +572,39 @@ __global__ __launch_bounds__(256) void splitk_reduce_wave_kernel(int M, int N, i
 
 // Low-K products (K <= 32 with M, N > 64: outer-product-like accumulations into T2-sized
 // outputs).  Bandwidth lies in reading and writing C, so a workgroup stages the whole K of a 64x64
-// output tile's A and B slices in LDS once, each thread owns a 4x4 block of outputs spaced 16
-// apart, and the lanes run along C's unit-stride index (LN: n, else m), so the read-modify-write
-// of C moves in 128-byte runs.
+// output tile's A and B slices in LDS once, and each thread owns 4 x 4 outputs: 4 consecutive
+// along C's unit-stride index (LN: n, else m), so its addend loads and stores are one 16-byte
+// (FP32) or two 16-byte (FP64) vector accesses where the four are contiguous and aligned, and
+// 4 spaced 16 apart along the other index; 16 lanes cover 64 consecutive elements.
 constexpr int LKT = 64, LKK = 32;
+
+template <typename T>   // 4 consecutive elements at p (16-byte aligned when T is float)
+__device__ inline void ld4(const T* p, double (&v)[4]) {
+  if constexpr (sizeof(T) == 4) {
+    const float4 q = *reinterpret_cast<const float4*>(p);
+    v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
+  } else {
+    const double2 q0 = reinterpret_cast<const double2*>(p)[0];
+    const double2 q1 = reinterpret_cast<const double2*>(p)[1];
+    v[0] = q0.x; v[1] = q0.y; v[2] = q1.x; v[3] = q1.y;
+  }
+}
+template <typename T>
+__device__ inline void st4(T* p, const double (&v)[4]) {
+  if constexpr (sizeof(T) == 4) {
+    *reinterpret_cast<float4*>(p) = make_float4((float)v[0], (float)v[1], (float)v[2], (float)v[3]);
+  } else {
+    reinterpret_cast<double2*>(p)[0] = make_double2(v[0], v[1]);
+    reinterpret_cast<double2*>(p)[1] = make_double2(v[2], v[3]);
+  }
+}
 
 template <bool LN, typename TC, typename TD>   // TC, TD: element types of C and D
 __global__ __launch_bounds__(256) void lowk_f64_kernel(GemmF64Args g) {
   TC* const Cp = static_cast<TC*>(g.C.p);
   const TD* const Dp = static_cast<const TD*>(g.D.p);
-  __shared__ double As[LKK][LKT + 1];
-  __shared__ double Bs[LKK][LKT + 1];
+  __shared__ double As[LKK][LKT + 2];
+  __shared__ double Bs[LKK][LKT + 2];
   const int tid = threadIdx.x;
   const int batch = blockIdx.z;
   TileCoord tc = tile_of(blockIdx.x, gridDim.x, g.tiles_m, g.tiles_n);
@@ -593,29 +615,51 @@ __global__ __launch_bounds__(256) void lowk_f64_kernel(GemmF64Args g) {
   // operand tiles: along k when that is the operand's unit stride, else along m / n
   const bool ak1 = g.ak.n2 > 1 ? g.ak.s2 == 1 : g.ak.s1 == 1;
   const bool bk1 = g.bk.n2 > 1 ? g.bk.s2 == 1 : g.bk.s1 == 1;
-  const int tm = LN ? (tid >> 4) : (tid & 15), tn = LN ? (tid & 15) : (tid >> 4);
-  // output offsets and the addend are read before the operand staging, so their latency overlaps
-  // it (each output is read and written by this thread only, so D may be C)
-  const int64_t cb = off2(batch, g.cb);
-  int64_t coff[4], roff[4];
-#pragma unroll
-  for (int y = 0; y < 4; ++y) {
-    const int col = n0 + tn + 16 * y;
-    coff[y] = col < g.N ? off2(col, g.cn) : 0;
-  }
+  // tile rows / columns of this thread's outputs
+  const int tu = tid & 15, tv = tid >> 4;
+  int rm[4], rn[4];
 #pragma unroll
   for (int x = 0; x < 4; ++x) {
-    const int row = m0 + tm + 16 * x;
-    roff[x] = row < g.M ? cb + off2(row, g.cm) : -1;
+    rm[x] = LN ? tv + 16 * x : 4 * tu + x;
+    rn[x] = LN ? 4 * tu + x : tv + 16 * x;
   }
+  // output offsets and the addend are read before the operand staging, so their latency overlaps
+  // it (each output is read and written by this thread only, so D may be C).  Group q holds the
+  // four outputs consecutive along C's unit-stride index: (row x = q, col y = t) for LN, else
+  // (row x = t, col y = q).
+  const int64_t cb = off2(batch, g.cb);
+  int64_t roff[4], coff[4];
+  bool rok[4], cok[4];
+#pragma unroll
+  for (int x = 0; x < 4; ++x) {
+    rok[x] = m0 + rm[x] < g.M;
+    cok[x] = n0 + rn[x] < g.N;
+    roff[x] = rok[x] ? cb + off2(m0 + rm[x], g.cm) : 0;
+    coff[x] = cok[x] ? off2(n0 + rn[x], g.cn) : 0;
+  }
+  auto off = [&](int q, int t) { return LN ? roff[q] + coff[t] : roff[t] + coff[q]; };
+  auto ok = [&](int q, int t) { return LN ? rok[q] && cok[t] : rok[t] && cok[q]; };
   const bool addend = g.D && g.beta != 0.0;
+  // a group is one vector access when its four offsets are consecutive, all four in range and
+  // the addresses in C (and D) 16-byte aligned
+  bool vec[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    vec[q] = ok(q, 3) && (reinterpret_cast<uintptr_t>(Cp + off(q, 0)) & 15) == 0 &&
+             (!addend || (reinterpret_cast<uintptr_t>(Dp + off(q, 0)) & 15) == 0);
+#pragma unroll
+    for (int t = 1; t < 4; ++t) vec[q] = vec[q] && ok(q, t) && off(q, t) == off(q, 0) + t;
+  }
   double dv[4][4];
 #pragma unroll
-  for (int x = 0; x < 4; ++x)
+  for (int q = 0; q < 4; ++q) {
+    if (addend && vec[q]) {
+      ld4(Dp + off(q, 0), dv[q]);
+    } else {
 #pragma unroll
-    for (int y = 0; y < 4; ++y)
-      dv[x][y] = (addend && roff[x] >= 0 && n0 + tn + 16 * y < g.N) ? (double)Dp[roff[x] + coff[y]]
-                                                                     : 0.0;
+      for (int t = 0; t < 4; ++t) dv[q][t] = (addend && ok(q, t)) ? (double)Dp[off(q, t)] : 0.0;
+    }
+  }
   // all staging loads are issued before any LDS store, so their latencies overlap; lanes run
   // along each operand's unit-stride index (k: 32 lanes x 8 rows, else 64 lanes x 4 k)
   constexpr int LQ = LKK * LKT / 256;
@@ -643,21 +687,26 @@ __global__ __launch_bounds__(256) void lowk_f64_kernel(GemmF64Args g) {
   for (int k = 0; k < K; ++k) {
     double a[4], b[4];
 #pragma unroll
-    for (int x = 0; x < 4; ++x) { a[x] = As[k][tm + 16 * x]; b[x] = Bs[k][tn + 16 * x]; }
+    for (int x = 0; x < 4; ++x) { a[x] = As[k][rm[x]]; b[x] = Bs[k][rn[x]]; }
 #pragma unroll
     for (int x = 0; x < 4; ++x)
 #pragma unroll
       for (int y = 0; y < 4; ++y) acc[x][y] = fma(a[x], b[y], acc[x][y]);
   }
 #pragma unroll
-  for (int x = 0; x < 4; ++x) {
-    if (roff[x] < 0) continue;
+  for (int q = 0; q < 4; ++q) {
+    double v[4];
 #pragma unroll
-    for (int y = 0; y < 4; ++y) {
-      if (n0 + tn + 16 * y >= g.N) continue;
-      double v = g.alpha * acc[x][y];
-      if (addend) v += g.beta * dv[x][y];
-      Cp[roff[x] + coff[y]] = (TC)v;
+    for (int t = 0; t < 4; ++t) {
+      v[t] = g.alpha * (LN ? acc[q][t] : acc[t][q]);
+      if (addend) v[t] += g.beta * dv[q][t];
+    }
+    if (vec[q]) {
+      st4(Cp + off(q, 0), v);
+    } else {
+#pragma unroll
+      for (int t = 0; t < 4; ++t)
+        if (ok(q, t)) Cp[off(q, t)] = (TC)v[t];
     }
   }
 }

@@ -3,6 +3,7 @@
 // and deterministic reductions.  Every kernel is a grid-stride loop over coalesced output
 // elements; reductions use wavefront shuffles and a fixed-order second pass (bitwise
 // reproducible run to run).
+#include <algorithm>
 #include <mutex>
 
 #include "nl_common.h"
@@ -668,6 +669,16 @@ __global__ __launch_bounds__(NT) void scatter_add_rows_kernel(int64_t R, int64_t
   }
 }
 
+// out[r*ldo + c] = c < n1 ? A[r*lda + c] : B[r*ldb + c - n1]   (r < R, c < n1 + n2)
+__global__ __launch_bounds__(NT) void concat_rows_kernel(int64_t R, int64_t n1, const double* A,
+                                                         int64_t lda, int64_t n2, const double* B,
+                                                         int64_t ldb, double* out, int64_t ldo) {
+  const int64_t n = n1 + n2;
+  for (int64_t r = blockIdx.x; r < R; r += gridDim.x)
+    for (int64_t c = threadIdx.x; c < n; c += NT)
+      out[r * ldo + c] = c < n1 ? A[r * lda + c] : B[r * ldb + c - n1];
+}
+
 // t1n = m*t1 + (1-m)*(t1 + H1/(-(Xpp[a,a] + Xhh[i,i])))
 __global__ void update_t1_kernel(int64_t p, int64_t h, const double* t1, const double* H1,
                                  const double* Xpp, const double* Xhh, double mixing,
@@ -768,75 +779,116 @@ __global__ __launch_bounds__(NT) void assemble_t2_kernel(AssembleArgs g) {
 
 // The same assembly with one workgroup per pair {a, b} (a <= b), for h <= AH.  The h x h blocks
 // Sij[a,b,:,:], Sij[b,a,:,:] (contiguous) and Spp[(a,i),(b,j)], Spp[(b,i),(a,j)] (h rows of h
-// contiguous doubles) are staged in LDS, where the transposed (i <-> j) reads of the
+// contiguous elements) are staged in LDS, where the transposed (i <-> j) reads of the
 // antisymmetrisers are served; S0, Sab, t2, v_pphh, t2n are contiguous h x h blocks at (a,b) and
 // (b,a).  Every tensor is therefore read once with row-contiguous accesses (the flat kernel
-// above reads two of the four Spp terms at a stride of p*h doubles).
+// above reads two of the four Spp terms at a stride of p*h elements).
 // H(b,a) uses the same staged blocks: its Spp part is minus that of H(a,b).
+// One thread per element (i, j) of the block (blockDim = h*h rounded up to wavefronts): all
+// twelve loads of a pair are issued together, and the next pair's loads are in flight while
+// this pair is finished.  The staged blocks alternate between two LDS buffers, so one barrier
+// per pair separates a buffer's stores from its transposed reads: before a buffer is written
+// again every thread has passed the barrier of the pair in between, hence finished its reads.
 constexpr int AH = 32;
+constexpr int ALD = AH + 1;              // LDS row pitch of a staged block
+
+template <int NW>   // block-wide sum over NW wavefronts; result valid in thread 0
+__device__ inline double block_sum_w(double v) {
+  __shared__ double part[NW];
+  v = wave_sum(v);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) part[w] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < (int)(blockDim.x >> 6); ++i) s += part[i];
+  __syncthreads();
+  return s;
+}
 
 template <typename T>   // accumulator elements: double, or float (the split form's)
-__global__ __launch_bounds__(NT) void assemble_pair_kernel(AssembleArgs g) {
+__global__ __launch_bounds__(AH * AH) void assemble_pair_kernel(AssembleArgs g) {
   const T* const S0t = static_cast<const T*>(g.S0);
   const T* const Sabt = static_cast<const T*>(g.Sab);
   const T* const Sijt = static_cast<const T*>(g.Sij);
   const T* const Sppt = static_cast<const T*>(g.Spp);
-  auto S0p = [&](int64_t o) { return (double)S0t[o]; };
-  auto Sabp = [&](int64_t o) { return (double)Sabt[o]; };
-  auto Sijp = [&](int64_t o) { return (double)Sijt[o]; };
-  auto Sppp = [&](int64_t o) { return (double)Sppt[o]; };
-  __shared__ double SiA[AH][AH + 1], SiB[AH][AH + 1], SpA[AH][AH + 1], SpB[AH][AH + 1];
-  const int64_t p = g.p, ph = p * g.h;
+  __shared__ T Si[2][2][AH * ALD], Sp[2][2][AH * ALD];     // [buffer][(a,b) / (b,a)][i][j]
+  const int64_t p = g.p, ph = p * g.h, pp = p * p;
   const int h = (int)g.h, hh = h * h;
+  const int e = threadIdx.x;
+  const bool act = e < hh;
+  const int i = act ? fdiv(e, g.fh) : 0, j = act ? e - i * h : 0;
   double e_acc = 0.0, r_acc = 0.0;
-  auto finish = [&](int64_t a, int64_t b, int i, int j, int64_t t, double H) {
+  auto finish = [&](int64_t a, int64_t b, int64_t t, double H, double told, double v) {
     double denom = -(g.Xpp[a * p + a] + g.Xpp[b * p + b] + g.Xhh[i * h + i] + g.Xhh[j * h + j]);
-    double told = g.t2[t];
-    if (g.addv) H += g.vpphh[t];
+    if (g.addv) H += v;
     if (g.dpp) H += (g.dpp[a] + g.dpp[b]) * told;
     if (g.dhh) H += (g.dhh[i] + g.dhh[j]) * told;
     double tn = g.mixing * told + (1.0 - g.mixing) * (told + H / denom);
     if (g.t2n) g.t2n[t] = tn;
     if (g.t2n16) write_image(g, t, a, tn);
     if (g.H2out) g.H2out[t] = H;
-    if (g.vpphh) {
-      double v = g.vpphh[t];
-      e_acc += 0.25 * v * tn + 0.5 * v * g.t1n[a * h + i] * g.t1n[b * h + j];
-    }
+    if (g.vpphh) e_acc += 0.25 * v * tn + 0.5 * v * g.t1n[a * h + i] * g.t1n[b * h + j];
     r_acc += H * H;
   };
-  for (int64_t q = blockIdx.x; q < p * p; q += gridDim.x) {
+  // the pairs of this workgroup: q = blockIdx.x + k * gridDim.x with a <= b
+  auto next = [&](int64_t q) {
+    for (; q < pp; q += gridDim.x) {
+      const int64_t a = fdiv((int)q, g.fp);
+      if (a <= q - a * p) break;
+    }
+    return q;
+  };
+  struct Pair { T sa, sb, pa, pb, s0a, s0b, xa, xb; double ta, tb, va, vb; };
+  auto load = [&](int64_t q, Pair& r) {
     const int64_t a = fdiv((int)q, g.fp), b = q - a * p;
-    if (a > b) continue;                                    // uniform in the block
-    for (int e = threadIdx.x; e < hh; e += NT) {
-      const int i = fdiv(e, g.fh), j = e - i * h;
-      const int64_t oab = (a * g.h + i) * ph + b * g.h + j, oba = (b * g.h + i) * ph + a * g.h + j;
-      SiA[i][j] = g.Sij ? Sijp((a * p + b) * hh + e) : 0.0;
-      SiB[i][j] = g.Sij ? Sijp((b * p + a) * hh + e) : 0.0;
-      SpA[i][j] = g.Spp ? Sppp(oab) : 0.0;
-      SpB[i][j] = g.Spp ? Sppp(oba) : 0.0;
+    const int64_t tab = (a * p + b) * hh + e, tba = (b * p + a) * hh + e;
+    const int64_t oab = (a * g.h + i) * ph + b * g.h + j, oba = (b * g.h + i) * ph + a * g.h + j;
+    r.sa = g.Sij ? Sijt[tab] : T(0); r.sb = g.Sij ? Sijt[tba] : T(0);
+    r.pa = g.Spp ? Sppt[oab] : T(0); r.pb = g.Spp ? Sppt[oba] : T(0);
+    r.s0a = g.S0 ? S0t[tab] : T(0); r.s0b = g.S0 ? S0t[tba] : T(0);
+    r.xa = g.Sab ? Sabt[tab] : T(0); r.xb = g.Sab ? Sabt[tba] : T(0);
+    r.ta = g.t2[tab]; r.tb = g.t2[tba];
+    r.va = g.vpphh ? g.vpphh[tab] : 0.0; r.vb = g.vpphh ? g.vpphh[tba] : 0.0;
+  };
+  int64_t q = next(blockIdx.x);
+  Pair cur;
+  if (act && q < pp) load(q, cur);
+  int buf = 0;
+  while (q < pp) {
+    const int64_t a = fdiv((int)q, g.fp), b = q - a * p;
+    T* const SiA = Si[buf][0]; T* const SiB = Si[buf][1];
+    T* const SpA = Sp[buf][0]; T* const SpB = Sp[buf][1];
+    if (act) {
+      SiA[i * ALD + j] = cur.sa; SiB[i * ALD + j] = cur.sb;
+      SpA[i * ALD + j] = cur.pa; SpB[i * ALD + j] = cur.pb;
     }
-    __syncthreads();
-    for (int e = threadIdx.x; e < hh; e += NT) {
-      const int i = fdiv(e, g.fh), j = e - i * h;
+    const int64_t qn = next(q + gridDim.x);
+    Pair nxt;
+    if (act && qn < pp) load(qn, nxt);
+    // this buffer's stores complete, then the barrier (a compiler barrier too: the LDS reads
+    // below stay after it; the next pair's global loads stay in flight)
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+    if (act) {
       const int64_t tab = (a * p + b) * hh + e, tba = (b * p + a) * hh + e;
-      double Hab = g.S0 ? S0p(tab) : 0.0;
-      double Hba = g.S0 ? S0p(tba) : 0.0;
-      if (g.Sab) {
-        const double x = Sabp(tab), y = Sabp(tba);
-        Hab += x - y;
-        Hba += y - x;
-      }
-      Hab += SiA[i][j] - SiA[j][i];
-      Hba += SiB[i][j] - SiB[j][i];
-      const double sp = SpA[i][j] - SpB[i][j] - SpA[j][i] + SpB[j][i];
-      finish(a, b, i, j, tab, Hab + sp);
-      if (a != b) finish(b, a, i, j, tba, Hba - sp);
+      double Hab = (double)cur.s0a;
+      double Hba = (double)cur.s0b;
+      const double x = cur.xa, y = cur.xb;
+      Hab += x - y;
+      Hba += y - x;
+      Hab += (double)SiA[i * ALD + j] - (double)SiA[j * ALD + i];
+      Hba += (double)SiB[i * ALD + j] - (double)SiB[j * ALD + i];
+      const double sp = (double)SpA[i * ALD + j] - (double)SpB[i * ALD + j] -
+                        (double)SpA[j * ALD + i] + (double)SpB[j * ALD + i];
+      finish(a, b, tab, Hab + sp, cur.ta, cur.va);
+      if (a != b) finish(b, a, tba, Hba - sp, cur.tb, cur.vb);
     }
-    __syncthreads();
+    cur = nxt;
+    q = qn;
+    buf ^= 1;
   }
-  double es = block_sum(e_acc);
-  double rs = block_sum(r_acc);
+  double es = block_sum_w<AH * AH / 64>(e_acc);
+  double rs = block_sum_w<AH * AH / 64>(r_acc);
   if (threadIdx.x == 0) {
     g.part[blockIdx.x] = es;
     g.part[gridDim.x + blockIdx.x] = rs;
@@ -1408,6 +1460,17 @@ int nl_gather_rows(int64_t R, int64_t Cn, const int64_t* idx, const double* src,
   return NL_OK;
 }
 
+int nl_concat_rows(int64_t R, int64_t n1, const double* A, int64_t lda, int64_t n2,
+                   const double* B, int64_t ldb, double* out, int64_t ldo, void* stream) {
+  if (R < 0 || n1 < 0 || n2 < 0 || lda < n1 || ldb < n2 || ldo < n1 + n2) return NL_EARG;
+  if (R == 0 || n1 + n2 == 0) return NL_OK;
+  if ((n1 && !A) || (n2 && !B) || !out) return NL_EARG;
+  hipLaunchKernelGGL(concat_rows_kernel, dim3((unsigned)std::min<int64_t>(R, 65535)), dim3(NT),
+                     0, (hipStream_t)stream, R, n1, A, lda, n2, B, ldb, out, ldo);
+  NL_LAUNCH_CHECK();
+  return NL_OK;
+}
+
 int nl_scatter_add_rows(int64_t R, int64_t Cn, const int64_t* idx, const double* src, double alpha,
                         double* out, int64_t ldo, void* stream) {
   if (R < 0 || Cn < 0) return NL_EARG;
@@ -1455,8 +1518,9 @@ static int assemble_t2_impl(int64_t p, int64_t h, const void* S0, const void* Sa
   const dim3 grid(grid_for(n, 4096));
   hipStream_t st = (hipStream_t)stream;
   if (h <= AH && (Sij || Spp) && p * p < INT32_MAX) {
-    if (f32) hipLaunchKernelGGL(assemble_pair_kernel<float>, grid, dim3(NT), 0, st, g);
-    else hipLaunchKernelGGL(assemble_pair_kernel<double>, grid, dim3(NT), 0, st, g);
+    const dim3 block((unsigned)nl_cdiv(h * h, 64) * 64);      // one thread per block element
+    if (f32) hipLaunchKernelGGL(assemble_pair_kernel<float>, grid, block, 0, st, g);
+    else hipLaunchKernelGGL(assemble_pair_kernel<double>, grid, block, 0, st, g);
   } else {
     if (f32) hipLaunchKernelGGL(assemble_t2_kernel<float>, grid, dim3(NT), 0, st, g);
     else hipLaunchKernelGGL(assemble_t2_kernel<double>, grid, dim3(NT), 0, st, g);

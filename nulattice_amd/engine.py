@@ -177,6 +177,16 @@ class Buffers:
         self.E = z(1)
         self.p, self.h, self.device = p, h, device
         self._acc32 = None
+        self._sab_img = None
+        self.t2_16 = self.t2_split = None
+
+    def sab_image(self, kp):
+        """The K-concatenated [N56 | t2^T] operand of the split Sab product: p*h*h rows of
+        [hi(kp) | lo(kp)] 16-bit elements, zero-initialised once (its pad columns stay zero)."""
+        n = self.p * self.h * self.h * 2 * kp
+        if self._sab_img is None or self._sab_img.numel() != n:
+            self._sab_img = torch.zeros(n, dtype=torch.int16, device=self.device)
+        return self._sab_img
 
     def acc32(self):
         """The FP32 accumulators of the split precision form (allocated on first use): S0, Sab,
@@ -399,7 +409,9 @@ def t2_part(ham, t1, t2, B, prec=DEFAULT_PREC):
     if prec["dress"] != "fp64" and DIAG_SPLIT:
         _lib.call("nl_take_diag", p, ops._p(B.Mpp), ops._p(B.dMpp), ops._stream())
         B.dpp = B.dMpp
-    if getattr(B, "t2_16", None) is not None and p >= ops.LP_MIN_DIM:
+    if split is not None and p >= ops.LP_MIN_DIM:
+        _sab_split(ham, t1, B, Sab, split)
+    elif getattr(B, "t2_16", None) is not None and p >= ops.LP_MIN_DIM:
         _sab_bf16(ham, t1, B, Sab)
     else:
         contract("bkij,ak->abij", B.N56, t1, Sab, beta=0.0)                  # A23 + A39
@@ -465,6 +477,49 @@ def _sab_bf16(ham, t1, B, Sab):
     _lib.call("nl_gemm_lp", ops.BF16, p, p * h * h, ldk, ctypes.c_double(1.0), ops._p(A16), ldk,
               0, None, 0, ops._p(B16), ldk, 0, None, 0, ctypes.c_double(0.0), None, ops._p(Sab),
               p * h * h, 0, 1, splits, ops._p(ws), st)
+
+
+def _sab_split(ham, t1, B, Sab, dt):
+    """Sab[a,(b,i,j)] = sum_k t1[a,k] N56[b,k,i,j] + sum_d Mpp[a,d] t2[d,b,i,j] as ONE split GEMM
+    over K = (k | d) (kp = h + p padded to 32): A = [t1 | Mpp] rows a as [hi(kp) | lo(kp)] with
+    per-row scales, B = [N56 | t2^T] rows (b,i,j) at the amplitude images' fixed scale; the
+    t2^T columns are 16-bit transposes of this step's split image of t2 (B.t2_split).  Sab is
+    written once (beta = 0) instead of by a low-K FP64 pass and a product that reads it back."""
+    p, h = ham.p, ham.h
+    kp = (h + p + 31) // 32 * 32
+    ld, row = 2 * kp, p * h * h
+    dev, st = t1.device, ops._stream()
+    sc = AMP_SCALE[dt]
+    img = B.sab_image(kp)           # zero-initialised: the columns [h + p, kp) stay zero
+    i4 = lambda *v: (ctypes.c_int64 * 4)(*v)
+    # N56[b,k,i,j] as rows (b,i,j), columns k: hi into [0, h), lo into [kp, kp + h)
+    for off, mask in ((0, 0), (kp, 1)):
+        _lib.call("nl_convert_split", dt, p, h * h, 1, h, i4(h ** 3, 1, 0, h * h), ops._p(B.N56),
+                  ops._off16(img, off), ld, h, 1, mask, None, ctypes.c_float(0.0),
+                  ctypes.c_double(sc), st)
+    N = B.t2_split                  # [d][hi(b,i,j) | lo(b,i,j)]
+    for q in range(2):
+        _lib.call("nl_transpose16_into", 1, p, row, ops._off16(N, q * row), 2 * row, 0,
+                  ops._off16(img, q * kp + h), ld, 0, p, st)
+    A64 = ops.WS.get("sab_A64", p * (h + p), device=dev)
+    _lib.call("nl_concat_rows", p, h, ops._p(t1), h, p, ops._p(B.Mpp), p, ops._p(A64), h + p, st)
+    A16 = ops.WS.get("sab_A16", p * ld, dtype=torch.int16, device=dev)
+    target = ops.lp_target(dt, False)
+    ia = ops.WS.get("sab_ia", p, dtype=torch.float32, device=dev) if target != 0.0 else None
+    _lib.call("nl_convert_split", dt, 1, p, 1, h + p, i4(0, h + p, 0, 1), ops._p(A64),
+              ops._p(A16), ld, kp, 2, 0b10, ops._p(ia), ctypes.c_float(target),
+              ctypes.c_double(1.0), st)
+    io = ops.C_F32 if Sab.dtype == torch.float32 else 0
+    ops.count_flops("lp", 3 * 2.0 * p * row * kp)
+    ev = ops._timed("sab")
+    if ev:
+        ev[0].record()
+    _lib.call("nl_gemm_lp_split", dt | (io << 8), p, row, kp, ctypes.c_double(1.0 / sc),
+              ops._p(A16), ops._off16(A16, kp), ld, 0, ops._p(ia), 0, ops._p(img),
+              ops._off16(img, kp), ld, 0, None, 0, ctypes.c_double(0.0), None, ops._p(Sab), row,
+              0, 1, 1, None, st)
+    if ev:
+        ev[1].record()
 
 
 def _pp_ladder(ham, tau, S0, prec, tau16=None):
@@ -620,7 +675,7 @@ def update(ham, t1, t2, B, t1n, t2n, mixing, prec=DEFAULT_PREC, H2out=None, img=
         try:
             dt = _fast_split(ham, prec)
             if dt is not None:
-                _t2_split_images(ham, t2, dt, img)
+                _t2_split_images(ham, t2, dt, B, img)
             else:
                 _t2_images(ham, t2, B, prec, img)
             t1_part(ham, t1, t2, B, prec)
@@ -629,6 +684,7 @@ def update(ham, t1, t2, B, t1n, t2n, mixing, prec=DEFAULT_PREC, H2out=None, img=
             t2_part(ham, t1, t2, B, prec)
         finally:
             B.t2_16 = None
+            B.t2_split = None
     assemble(ham, t2, B, t1n, t2n, mixing, H2out, img=img, prec=prec)
 
 
@@ -681,11 +737,11 @@ def _t2_images(ham, t2, B, prec, img=None):
         ops.lp_provide(t2, "acik", ["a", "i"], ["c", "k"], T3, ph)
 
 
-def _t2_split_images(ham, t2, dt, img=None):
+def _t2_split_images(ham, t2, dt, B, img=None):
     """Register the [hi | lo] split images of t2 that the bf16x3 products read: the rows
-    [a][(d,k,l)] (A19; written by the previous step's assembly, or split here from FP64), and from
-    them by 16-bit permutes [(a,i)][(c,k)] (both ring products), [(a,j)][(d,k)] (Y) and
-    [(b,i,j)][d] (Sab).  Needs t2 under ops.lp_reuse."""
+    [a][(d,k,l)] (A19; written by the previous step's assembly, or split here from FP64; kept
+    as B.t2_split for the Sab product), and from them by 16-bit permutes [(a,i)][(c,k)] (both
+    ring products) and [(a,j)][(d,k)] (Y).  Needs t2 under ops.lp_reuse."""
     p, h = ham.p, ham.h
     n, row, ph = p * p * h * h, p * h * h, p * h
     dev, st = t2.device, ops._stream()
@@ -710,12 +766,8 @@ def _t2_split_images(ham, t2, dt, img=None):
         _lib.call("nl_transpose16_into", p, ph, h, ops._off16(N, q * row), h, 2 * row,
                   ops._off16(Y, q * ph), 2 * ph, 2 * ph * h, ph, st)
     ops.lp_provide(t2, "adkj", ["a", "j"], ["d", "k"], Y, 2 * ph, dt, SPLIT, sc)
-    # Sab operand rows (b,i,j), columns d: the d x (b,i,j) matrix transposed, per part
-    S = ops.WS.get("t2_split_T", 2 * n, dtype=torch.int16, device=dev)
-    for q in range(2):
-        _lib.call("nl_transpose16_into", 1, p, row, ops._off16(N, q * row), 2 * row, 0,
-                  ops._off16(S, q * p), 2 * p, 0, p, st)
-    ops.lp_provide(t2, "dbij", ["b", "i", "j"], ["d"], S, 2 * p, dt, SPLIT, sc)
+    # the Sab product's t2 columns are transposed from N into its own operand (_sab_split)
+    B.t2_split = N
 
 
 def assemble(ham, t2, B, t1n, t2n, mixing, H2out=None, img=None, prec=None):

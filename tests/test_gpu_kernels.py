@@ -403,3 +403,16 @@ def test_copy_sub_is_a_copy_and_a_difference():
         _lib.call("nl_copy_sub", n, ops._p(x), ops._p(y), ops._p(hc), ops._p(c), ops._stream())
         assert torch.equal(hc, x) and torch.equal(c, x - y)
     assert _lib.load().nl_copy_sub(-1, None, None, None, None, None) == -1
+
+
+def test_concat_rows():
+    """nl_concat_rows: [A | B] rows at pitch ldo, exactly; the columns beyond n1 + n2 untouched."""
+    A = torch.randn(37, 9, dtype=torch.float64, device="cuda")
+    Bm = torch.randn(37, 50, dtype=torch.float64, device="cuda")
+    out = torch.full((37, 64), 7.0, dtype=torch.float64, device="cuda")
+    _lib.call("nl_concat_rows", 37, 5, ops._p(A), 9, 50, ops._p(Bm), 50, ops._p(out), 64,
+              ops._stream())
+    ref = torch.cat([A[:, :5], Bm, torch.full((37, 9), 7.0, dtype=torch.float64,
+                                               device="cuda")], 1)
+    assert torch.equal(out, ref)
+    assert _lib.load().nl_concat_rows(3, 5, None, 5, 2, None, 2, None, 7, None) == -1
