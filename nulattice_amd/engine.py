@@ -443,7 +443,13 @@ def t2_part(ham, t1, t2, B, prec=DEFAULT_PREC):
         contract("cjkl,bl->ckbj", ham.v_phhh, t1, W, alpha=-1.0)             # A29
         contract("acik,ckbj->aibj", t2ring, W, acc.Spp, beta=0.0,
                  precision=prec.get("ring_S", prec["ring"]), tag="ring")
-    contract("idkl,adkj->ilaj", B.Xidkl, t2, B.Yilaj, beta=0.0, precision=prec["hh"])  # A34
+    if split is not None:
+        # t2[a,d,k,j] = -t2[a,d,j,k]: the Y product reads the ring image (rows (a,j), columns
+        # (d,k)) with the sign in alpha, so it needs no image of t2 of its own
+        contract("idkl,adjk->ilaj", B.Xidkl, t2, B.Yilaj, alpha=-1.0, beta=0.0,
+                 precision=prec["hh"])                                                 # A34
+    else:
+        contract("idkl,adkj->ilaj", B.Xidkl, t2, B.Yilaj, beta=0.0, precision=prec["hh"])
     # Spp enters only through P(ij)P(ab), which is invariant under (a,i) <-> (b,j): the A27 term
     # -t1[a,k] U[b,k,i,j] may stand as -U[a,l,j,i] t1[b,l], so both tails are one product
     permute_into(B.Yilaj, B.Zilaj)
@@ -740,8 +746,8 @@ def _t2_images(ham, t2, B, prec, img=None):
 def _t2_split_images(ham, t2, dt, B, img=None):
     """Register the [hi | lo] split images of t2 that the bf16x3 products read: the rows
     [a][(d,k,l)] (A19; written by the previous step's assembly, or split here from FP64; kept
-    as B.t2_split for the Sab product), and from them by 16-bit permutes [(a,i)][(c,k)] (both
-    ring products) and [(a,j)][(d,k)] (Y).  Needs t2 under ops.lp_reuse."""
+    as B.t2_split for the Sab product), and from them by a 16-bit permute [(a,i)][(c,k)] (both
+    ring products and, with a sign, Y).  Needs t2 under ops.lp_reuse."""
     p, h = ham.p, ham.h
     n, row, ph = p * p * h * h, p * h * h, p * h
     dev, st = t2.device, ops._stream()
@@ -760,12 +766,6 @@ def _t2_split_images(ham, t2, dt, B, img=None):
     istr = (ctypes.c_int64 * 5)(2 * row, h, row, h * h, 1)
     _lib.call("nl_permute16", 5, shp, istr, ops._p(N), ops._p(R), st)
     ops.lp_provide(t2, "acik", ["a", "i"], ["c", "k"], R, 2 * ph, dt, SPLIT, sc)
-    # Y operand rows (a,j), columns (d,k): per a and part, the (d,k) x j block transposed
-    Y = ops.WS.get("t2_split_aj", 2 * n, dtype=torch.int16, device=dev)
-    for q in range(2):
-        _lib.call("nl_transpose16_into", p, ph, h, ops._off16(N, q * row), h, 2 * row,
-                  ops._off16(Y, q * ph), 2 * ph, 2 * ph * h, ph, st)
-    ops.lp_provide(t2, "adkj", ["a", "j"], ["d", "k"], Y, 2 * ph, dt, SPLIT, sc)
     # the Sab product's t2 columns are transposed from N into its own operand (_sab_split)
     B.t2_split = N
 
