@@ -31,6 +31,10 @@ extern "C" {
  * not hold it). */
 #define NL_ACC_F32 1
 #define NL_ADD_V 2
+/* ... and S0 holds the hole pairs i < j only (see nl_hole_pairs): row a*p + b of P32 slots
+ * (P = h(h-1)/2 rounded up to a multiple of 32; slots from P on unused), S0[a,b,i,j] =
+ * S0[(a,b)][q(i,j)], -S0[(a,b)][q(j,i)] or 0. */
+#define NL_S0_PAIRS 4
 
 /* C[b] = alpha * op(A[b]) * op(B[b]) + beta * C[b]  (FP64, MFMA 16x16x4).
  * Replaces every dense opt_einsum pairwise contraction of the diagram library
@@ -142,6 +146,23 @@ int nl_convert_rows(int dtype, int64_t R1, int64_t R2, int64_t C1, int64_t C2,
  * row (a,b) of out is [hi(h*h) | lo(h*h)]; tau (FP64, may be null) receives the value too. */
 int nl_tau_split(int64_t p, int64_t h, const double* t1, const double* t2, double* tau,
                  void* out, int dtype, double scale, void* stream);
+
+/* nl_tau_split over the hole pairs i < j only: row (a,b) of out is [hi(Pp) | lo(Pp)] holding
+ * tau[a,b,i,j] * scale at column q(i,j) = i*h - i(i+1)/2 + j - i - 1 (P = h(h-1)/2 pairs;
+ * columns P <= q < Pp zero).  tau is antisymmetric in (i, j) (ccDgrams.py:215-237 applies it
+ * through pIJ), so the hh-ladder, pp-ladder and R56 products (coupled_cluster.py:476-536) run on
+ * these P columns and, summed over (k, l), on P terms with a factor 2. */
+int nl_tau_split_pairs(int64_t p, int64_t h, const double* t1, const double* t2, void* out,
+                       int64_t Pp, int dtype, double scale, void* stream);
+
+/* Hole-pair form of an array antisymmetric in an index pair (i, j) < h: expand = 0:
+ * out[r][q][c] = in[r][i][j][c] for the P = h(h-1)/2 pairs q = (i < j), numbered row by row;
+ * expand = 1: out[r][i][j][c] = in[r][q(i,j)][c] (i < j), -in[r][q(j,i)][c] (i > j), 0 (i = j).
+ * r < R, c < w (FP64); pair-form rows have ldp >= P slots (those from P on untouched).  The
+ * hh-ladder intermediate Whh and the R56 result of the precision form (coupled_cluster.py:
+ * 476-536) pass between the two forms with it. */
+int nl_hole_pairs(int64_t R, int64_t h, int64_t ldp, int64_t w, const double* in, double* out,
+                  int expand, void* stream);
 
 /* nl_gather_rows16 writing k < ldk of each output row at row pitch ldo >= ldk (the hi and lo
  * halves of a split operand row are two gathers into one [hi | lo] row). */

@@ -29,6 +29,7 @@ static inline int64_t nl_cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 // Flags of the T2 assembly: its four accumulators are FP32; it adds v_pphh to the residual.
 #define NL_ACC_F32 1
 #define NL_ADD_V 2
+#define NL_S0_PAIRS 4   // S0 holds the hole pairs i < j of each (a,b) row only
 
 // An array of double or float elements addressed by element offset (a wave-uniform type).
 struct AnyOut {

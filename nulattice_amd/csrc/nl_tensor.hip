@@ -596,6 +596,67 @@ __global__ __launch_bounds__(NT) void tau_split_kernel(int p, int h, FastDiv fp,
   }
 }
 
+// Hole pairs i < j of an h x h block, numbered row by row: q(i,j) = i*h - i*(i+1)/2 + j - i - 1.
+// pair_of(q) = (i, j) by walking the rows (at most h steps).
+__device__ inline void pair_of(int q, int h, int& i, int& j) {
+  int r = 0, start = 0;
+  while (start + (h - 1 - r) <= q) { start += h - 1 - r; ++r; }
+  i = r;
+  j = r + 1 + (q - start);
+}
+
+// The tau of tau_split_kernel over the hole pairs i < j only: row (a, b) of out is
+// [hi(P) 0... | lo(P) 0...] at pitch 2*Pp (P = h(h-1)/2 pairs, columns P <= q < Pp zero).
+// tau is antisymmetric in (i, j), so these rows carry all of it.
+__global__ __launch_bounds__(NT) void tau_split_pairs_kernel(int64_t pp, int p, int h, int P,
+                                                             int Pp, FastDiv fp, FastDiv fPp,
+                                                             const double* t1, const double* t2,
+                                                             uint16_t* out, int bf16,
+                                                             double scale) {
+  const int64_t total = pp * Pp;                         // < 2^31 (checked by the host)
+  for (int64_t t = blockIdx.x * (int64_t)NT + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * NT) {
+    const int64_t row = fdiv((int)t, fPp);
+    const int q = (int)(t - row * Pp);
+    uint16_t hi = 0, lo = 0;
+    if (q < P) {
+      int i, j;
+      pair_of(q, h, i, j);
+      const int a = fdiv((int)row, fp), b = (int)row - a * p;
+      const double v = tau_value(t2[row * h * h + i * h + j], t1[a * h + i], t1[b * h + j],
+                                 t1[a * h + j], t1[b * h + i]);
+      split16(v * scale, bf16, hi, lo);
+    }
+    uint16_t* y = out + row * 2 * Pp + q;
+    y[0] = hi;
+    y[Pp] = lo;
+  }
+}
+
+// Reduce: out[r][q][w] = in[r][i][j][w] for the pairs q = (i < j); expand: the antisymmetric
+// block, out[r][i][j][w] = in[r][q(i,j)][w] (i < j), -in[r][q(j,i)][w] (i > j), 0 (i = j).
+// Pair-form rows hold ldp >= P pair slots (the slots beyond P are neither read nor written).
+__global__ __launch_bounds__(NT) void hole_pairs_kernel(int64_t R, int h, int P, int64_t ldp,
+                                                        int64_t w, const double* in, double* out,
+                                                        int expand) {
+  const int64_t per = (expand ? (int64_t)h * h : P) * w, total = R * per;
+  for (int64_t t = blockIdx.x * (int64_t)NT + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * NT) {
+    const int64_t r = t / per, rem = t - r * per, x = rem / w, c = rem - x * w;
+    if (expand) {
+      const int i = (int)(x / h), j = (int)(x - (int64_t)i * h);
+      double v = 0.0;
+      if (i < j) v = in[(r * ldp + (i * h - i * (i + 1) / 2 + j - i - 1)) * w + c];
+      else if (i > j) v = -in[(r * ldp + (j * h - j * (j + 1) / 2 + i - j - 1)) * w + c];
+      out[t] = v;
+    } else {
+      int i, j;
+      pair_of((int)x, h, i, j);
+      out[(r * ldp + x) * w + c] = in[((r * h + i) * h + j) * w + c];
+    }
+  }
+}
+
 // out[dst[s] + x*ds] += alpha * sum_{e in [ptr[s], ptr[s+1])} val[e] * src[so[e] + x*ss]
 // G lanes per segment (G = the power of two >= X, at most 64), so a wavefront serves 64/G
 // segments; each lane keeps two partial sums (entries alternate between them, added in a fixed
@@ -714,7 +775,18 @@ struct AssembleArgs {
                                         // 1: BF16, 2: FP16 of t2n * img_scale
   double img_scale;
   int addv;                             // H2 += v_pphh (when S0 does not hold it)
+  int s0pairs;                          // S0 holds the hole pairs i < j only: row (a,b) of
+                                        // s0ld >= h(h-1)/2 slots, S0[a,b,i,j] = +-S0[(a,b)][q]
+  int64_t s0ld;
 };
+
+// S0[a,b,i,j] from S0's hole-pair rows (row = a*p + b): the antisymmetric expansion.
+template <typename T>
+__device__ inline double s0_pair(const T* S0, int64_t row, int64_t ld, int h, int i, int j) {
+  if (i < j) return (double)S0[row * ld + (i * h - i * (i + 1) / 2 + j - i - 1)];
+  if (i > j) return -(double)S0[row * ld + (j * h - j * (j + 1) / 2 + i - j - 1)];
+  return 0.0;
+}
 
 __device__ inline void write_image(const AssembleArgs& g, int64_t t, int64_t a, double tn) {
   if (!g.split16) {
@@ -745,7 +817,8 @@ __global__ __launch_bounds__(NT) void assemble_t2_kernel(AssembleArgs g) {
     int64_t j = t % h, r = t / h, i = r % h;
     r /= h;
     int64_t b = r % p, a = r / p;
-    double H = g.S0 ? S0p(t) : 0.0;
+    double H = !g.S0 ? 0.0 : g.s0pairs ? s0_pair(S0t, a * p + b, g.s0ld, (int)h, (int)i, (int)j)
+                                        : S0p(t);
     if (g.Sab) H += Sabp(t) - Sabp(((b * p + a) * h + i) * h + j);
     const int64_t ph = p * h;
     if (g.Sij) H += Sijp(t) - Sijp(((a * p + b) * h + j) * h + i);
@@ -818,6 +891,10 @@ __global__ __launch_bounds__(AH * AH) void assemble_pair_kernel(AssembleArgs g) 
   const int e = threadIdx.x;
   const bool act = e < hh;
   const int i = act ? fdiv(e, g.fh) : 0, j = act ? e - i * h : 0;
+  // this thread's slot and sign in S0's hole-pair rows (fixed over all pairs a, b)
+  const int lo = i < j ? i : j, hi = i < j ? j : i;
+  const int64_t s0q = lo * h - lo * (lo + 1) / 2 + hi - lo - 1;
+  const T s0s = i < j ? T(1) : i > j ? T(-1) : T(0);
   double e_acc = 0.0, r_acc = 0.0;
   auto finish = [&](int64_t a, int64_t b, int64_t t, double H, double told, double v) {
     double denom = -(g.Xpp[a * p + a] + g.Xpp[b * p + b] + g.Xhh[i * h + i] + g.Xhh[j * h + j]);
@@ -846,7 +923,12 @@ __global__ __launch_bounds__(AH * AH) void assemble_pair_kernel(AssembleArgs g) 
     const int64_t oab = (a * g.h + i) * ph + b * g.h + j, oba = (b * g.h + i) * ph + a * g.h + j;
     r.sa = g.Sij ? Sijt[tab] : T(0); r.sb = g.Sij ? Sijt[tba] : T(0);
     r.pa = g.Spp ? Sppt[oab] : T(0); r.pb = g.Spp ? Sppt[oba] : T(0);
-    r.s0a = g.S0 ? S0t[tab] : T(0); r.s0b = g.S0 ? S0t[tba] : T(0);
+    if (g.s0pairs) {
+      r.s0a = s0s * S0t[(a * p + b) * g.s0ld + (i != j ? s0q : 0)];
+      r.s0b = s0s * S0t[(b * p + a) * g.s0ld + (i != j ? s0q : 0)];
+    } else {
+      r.s0a = g.S0 ? S0t[tab] : T(0); r.s0b = g.S0 ? S0t[tba] : T(0);
+    }
     r.xa = g.Sab ? Sabt[tab] : T(0); r.xb = g.Sab ? Sabt[tba] : T(0);
     r.ta = g.t2[tab]; r.tb = g.t2[tba];
     r.va = g.vpphh ? g.vpphh[tab] : 0.0; r.vb = g.vpphh ? g.vpphh[tba] : 0.0;
@@ -1282,6 +1364,33 @@ int nl_tau_split(int64_t p, int64_t h, const double* t1, const double* t2, doubl
   return NL_OK;
 }
 
+int nl_tau_split_pairs(int64_t p, int64_t h, const double* t1, const double* t2, void* out,
+                       int64_t Pp, int dtype, double scale, void* stream) {
+  const int64_t P = h * (h - 1) / 2;
+  if (p < 0 || h < 0 || !t1 || !t2 || !out || (dtype != 0 && dtype != 1) || Pp < P) return NL_EARG;
+  if (p * p * Pp == 0) return NL_OK;
+  if (p * p * Pp >= INT32_MAX || p * h > INT32_MAX) return NL_EARG;
+  hipLaunchKernelGGL(tau_split_pairs_kernel, dim3(grid_for(p * p * Pp)), dim3(NT), 0,
+                     (hipStream_t)stream, p * p, (int)p, (int)h, (int)P, (int)Pp, nl_fastdiv(p),
+                     nl_fastdiv(Pp), t1, t2, (uint16_t*)out, dtype == 0, scale);
+  NL_LAUNCH_CHECK();
+  return NL_OK;
+}
+
+int nl_hole_pairs(int64_t R, int64_t h, int64_t ldp, int64_t w, const double* in, double* out,
+                  int expand, void* stream) {
+  const int64_t P = h * (h - 1) / 2;
+  if (R < 0 || h < 0 || w < 1 || ldp < P || !in || !out || (expand != 0 && expand != 1))
+    return NL_EARG;
+  if (R * P * w == 0 && !expand) return NL_OK;
+  if (R * h * h * w == 0) return NL_OK;
+  if (h > 65535) return NL_EARG;
+  hipLaunchKernelGGL(hole_pairs_kernel, dim3(grid_for(R * h * h * w)), dim3(NT), 0,
+                     (hipStream_t)stream, R, (int)h, (int)P, ldp, w, in, out, expand);
+  NL_LAUNCH_CHECK();
+  return NL_OK;
+}
+
 int nl_tau16(int64_t p, int64_t h, const double* t1, const double* t2, void* tau16,
              void* stream) {
   if (p < 0 || h < 0 || !t1 || !t2 || !tau16) return NL_EARG;
@@ -1504,13 +1613,17 @@ static int assemble_t2_impl(int64_t p, int64_t h, const void* S0, const void* Sa
                             int split16 = 0, double img_scale = 1.0, int io = 0) {
   if (p < 0 || h < 0 || !Xpp || !Xhh || !t2 || !part) return NL_EARG;
   if (vpphh && !t1n) return NL_EARG;
-  if ((io & ~(NL_ACC_F32 | NL_ADD_V)) || ((io & NL_ADD_V) && !vpphh)) return NL_EARG;
+  if ((io & ~(NL_ACC_F32 | NL_ADD_V | NL_S0_PAIRS)) || ((io & NL_ADD_V) && !vpphh) ||
+      ((io & NL_S0_PAIRS) && !S0))
+    return NL_EARG;
   int64_t n = p * p * h * h;
   if (n == 0) return NL_OK;
   AssembleArgs g;
   const bool f32 = (io & NL_ACC_F32) != 0;
   g.p = p; g.h = h; g.S0 = S0; g.Sab = Sab; g.Sij = Sij; g.Spp = Spp;
   g.Xpp = Xpp; g.Xhh = Xhh; g.addv = (io & NL_ADD_V) != 0;
+  g.s0pairs = (io & NL_S0_PAIRS) != 0;
+  g.s0ld = nl_cdiv(h * (h - 1) / 2, 32) * 32;
   g.t2 = t2; g.vpphh = vpphh; g.t1n = t1n; g.mixing = mixing; g.t2n = t2n; g.part = part;
   g.H2out = H2out; g.t2n16 = t2n16; g.dpp = dpp; g.dhh = dhh; g.split16 = split16;
   g.img_scale = img_scale;

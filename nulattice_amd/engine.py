@@ -44,8 +44,9 @@ from .plans import PairBlocks, PpphPlans, coo_arrays
 
 F64 = torch.float64
 
-# nl_assemble_t2_ext flags: FP32 accumulators; v_pphh added to H2 by the assembly
-ACC_F32, ADD_V = 1, 2
+# nl_assemble_t2_ext flags: FP32 accumulators; v_pphh added to H2 by the assembly; S0 over the
+# hole pairs i < j
+ACC_F32, ADD_V, S0_PAIRS = 1, 2, 4
 
 
 def _dev(x, device):
@@ -180,6 +181,23 @@ class Buffers:
         self._sab_img = None
         self.t2_16 = self.t2_split = None
 
+    def s0_pairs(self, Pp):
+        """S0 of the split form over the hole pairs: [p*p][Pp] FP32 (slots from P on unused)."""
+        n = self.p * self.p * Pp
+        b = getattr(self, "_s0p", None)
+        if b is None or b.numel() != n:
+            b = self._s0p = torch.zeros(n, dtype=torch.float32, device=self.device)
+        return b
+
+    def pair_operand(self, Pp):
+        """[Pp][hi(Pp) | lo(Pp)] 16-bit rows of Whh^T over the hole pairs (zero-initialised: the
+        rows and columns from P on stay zero) and their FP32 row scales (ones initially)."""
+        b = getattr(self, "_whhT", None)
+        if b is None or b[0].numel() != Pp * 2 * Pp:
+            b = self._whhT = (torch.zeros(Pp * 2 * Pp, dtype=torch.int16, device=self.device),
+                              torch.ones(Pp, dtype=torch.float32, device=self.device))
+        return b
+
     def sab_image(self, kp):
         """The K-concatenated [N56 | t2^T] operand of the split Sab product: p*h*h rows of
         [hi(kp) | lo(kp)] 16-bit elements, zero-initialised once (its pad columns stay zero)."""
@@ -231,6 +249,15 @@ TAU16 = True
 # The split form (shared split images) keeps S0, Sab, Sij, Spp and W in FP32 (False: FP64, for
 # checks that isolate the images from the accumulators' rounding).
 ACC32 = True
+
+# The split form runs the hh ladder, the pp ladder and R56 on the hole pairs i < j (tau, Whh,
+# S0 and R56 are antisymmetric in their hole pairs; False: all (i, j), for comparison checks).
+HOLE_PAIRS = True
+
+
+def _pairs(ham, split):
+    """Whether this split-form step uses the hole-pair products."""
+    return split is not None and ACC32 and HOLE_PAIRS and ham.h >= 2
 
 
 def t1_part(ham, t1, t2, B, prec=DEFAULT_PREC):
@@ -333,9 +360,12 @@ def t2_part(ham, t1, t2, B, prec=DEFAULT_PREC):
     # and, for the Whh product (summed over c,d), as [ij][cd] rows; the FP64 tau is not written
     tau = B.tau
     split = _fast_split(ham, prec)
-    lp_tau = _lp_tau(ham, prec) or split is not None
+    pairs = _pairs(ham, split)
+    lp_tau = (_lp_tau(ham, prec) or split is not None) and not pairs
     tau16 = None
-    if split is not None:
+    if pairs:
+        pass                        # tau is formed over the hole pairs by _hh_pairs
+    elif split is not None:
         # tau only as its split image: [ab][hi(ij) | lo(ij)] rows, and [ij][hi(cd) | lo(cd)]
         # rows (the Whh product sums over c, d) by two 16-bit transposes
         n, hh, pp = p * p * h * h, h * h, p * p
@@ -376,8 +406,13 @@ def t2_part(ham, t1, t2, B, prec=DEFAULT_PREC):
     permute_into(B.zz, B.Whh, 1.0, 1.0)
     permute_into(B.zz.permute(0, 1, 3, 2), B.Whh, -1.0, 1.0)
     Sab = acc.Sab
-    with ops.lp_reuse(tau, absent=True) if lp_tau else contextlib.nullcontext():
-        if split is not None:
+    if pairs:
+        _hh_pairs(ham, t1, t2, B, split, prec)
+    with (ops.lp_reuse(tau, absent=True) if lp_tau else contextlib.nullcontext()) \
+            if not pairs else contextlib.nullcontext():
+        if pairs:
+            pass
+        elif split is not None:
             sc = AMP_SCALE[split]
             ops.lp_provide(tau, "cdij", ["i", "j"], ["c", "d"], tauT, 2 * p * p, split, SPLIT, sc)
             ops.lp_provide(tau, "abkl", ["a", "b"], ["k", "l"], tau16, 2 * h * h, split, SPLIT,
@@ -385,20 +420,21 @@ def t2_part(ham, t1, t2, B, prec=DEFAULT_PREC):
         elif lp_tau:
             ops.lp_provide(tau, "cdij", ["i", "j"], ["c", "d"], tauT, ldT)
             ops.lp_provide(tau, "abkl", ["a", "b"], ["k", "l"], tau16, h * h)
-        contract("cdkl,cdij->klij", ham.v_pphh, tau, B.Whh, alpha=0.5,
-                 precision=prec.get("hh_W", prec["hh"]))
-        if acc32:
-            contract("abkl,klij->abij", tau, B.Whh, S0, alpha=0.5, beta=0.0,
-                     precision=prec.get("hh_S", prec["hh"]))
-        else:
-            contract("abkl,klij->abij", tau, B.Whh, S0, alpha=0.5,
-                     precision=prec.get("hh_S", prec["hh"]), addend=ham.v_pphh)
-        _pp_ladder(ham, tau, S0, prec, tau16)
-        # ---- Sab: P(ab) group
-        permute_into(ham.v_phhh, B.N56)
-        if pl.n:
-            _block_gemm(pl.r56, tau, B.N56, 0.5, h * h, "r56", prec.get("ladder", "fp64"),
-                        tau16)
+        if not pairs:
+            contract("cdkl,cdij->klij", ham.v_pphh, tau, B.Whh, alpha=0.5,
+                     precision=prec.get("hh_W", prec["hh"]))
+            if acc32:
+                contract("abkl,klij->abij", tau, B.Whh, S0, alpha=0.5, beta=0.0,
+                         precision=prec.get("hh_S", prec["hh"]))
+            else:
+                contract("abkl,klij->abij", tau, B.Whh, S0, alpha=0.5,
+                         precision=prec.get("hh_S", prec["hh"]), addend=ham.v_pphh)
+            _pp_ladder(ham, tau, S0, prec, tau16)
+            # ---- Sab: P(ab) group
+            permute_into(ham.v_phhh, B.N56)
+            if pl.n:
+                _block_gemm(pl.r56, tau, B.N56, 0.5, h * h, "r56", prec.get("ladder", "fp64"),
+                            tau16)
     permute_into(Xpp, B.Mpp)                                                 # X_pp t2 (P(ab) image)
     if pl.n:
         ops.axpby(1.0, B.ret1, 1.0, B.Mpp)                                   # + A38
@@ -483,6 +519,102 @@ def _sab_bf16(ham, t1, B, Sab):
     _lib.call("nl_gemm_lp", ops.BF16, p, p * h * h, ldk, ctypes.c_double(1.0), ops._p(A16), ldk,
               0, None, 0, ops._p(B16), ldk, 0, None, 0, ctypes.c_double(0.0), None, ops._p(Sab),
               p * h * h, 0, 1, splits, ops._p(ws), st)
+
+
+def _pair_dims(ham):
+    """P hole pairs i < j, their slots per row Pp (P rounded up to 32), and the (c, d) pair
+    count rounded up to 32 (the K extent of the Whh product)."""
+    p, h = ham.p, ham.h
+    P = h * (h - 1) // 2
+    return P, (P + 31) // 32 * 32, (p * p + 31) // 32 * 32
+
+
+def _v_pairs_image(ham, dt):
+    """The exact 16-bit image of v_pphh[c,d,k,l] over the hole pairs k < l: rows q(k,l), columns
+    (c, d) (zero beyond p^2), per-row scales that make a contact interaction's elements exact;
+    built once per Hamiltonian and type."""
+    hit = getattr(ham, "_v_pairs", None)
+    if hit is not None and hit[0] == dt:
+        return hit[1], hit[2]
+    p, h = ham.p, ham.h
+    P, Pp, Kc = _pair_dims(ham)
+    dev, st = ham.v_pphh.device, ops._stream()
+    vr = torch.empty(p * p * Pp, dtype=F64, device=dev)            # [(c,d)][q(k,l)]
+    _lib.call("nl_hole_pairs", p * p, h, Pp, 1, ops._p(ham.v_pphh), ops._p(vr), 0, st)
+    img = torch.empty(P * Kc, dtype=torch.int16, device=dev)
+    iv = torch.empty(P, dtype=torch.float32, device=dev)
+    _lib.call("nl_convert_rows", dt, 1, P, 1, p * p, (ctypes.c_int64 * 4)(0, 1, 0, Pp),
+              ops._p(vr), ops._p(img), Kc, ops._p(iv), ctypes.c_float(ops.lp_target(dt, True)),
+              st)
+    del vr                          # (stream-ordered: its memory is reused after these kernels)
+    ham._v_pairs = (dt, img, iv)
+    return img, iv
+
+
+def _hh_pairs(ham, t1, t2, B, dt, prec):
+    """The hh ladder, the pp ladder and R56 of the split form on the hole pairs i < j.  tau, Whh,
+    S0 and R56 are antisymmetric in their hole pairs (ij) and (kl), so the products need only
+    the P = h(h-1)/2 pairs i < j as output columns, and the hh ladder's sum over (k, l) is twice
+    its sum over k < l:
+        Whh[kl,ij] (k<l, i<j) = its v_hhhh and t1 terms + 1/2 sum_cd v[c,d,k,l] tau[c,d,i,j]
+        S0[ab,ij]  (i<j)      = sum_{k<l} tau[a,b,k,l] Whh[kl,ij] + 1/2 sum_cd V[ab,cd] tau[cd,ij]
+        N56[bk,ij] (i<j)      = v_phhh + 1/2 sum_cd v_ppph[c,d,b,k] tau[c,d,i,j]
+    S0 stays in the pair form (the assembly expands it); N56 is expanded for the Sab product."""
+    p, h = ham.p, ham.h
+    pp = p * p
+    P, Pp, Kc = _pair_dims(ham)
+    dev, st = t2.device, ops._stream()
+    sc = AMP_SCALE[dt]
+    i16 = torch.int16
+    off = ops._off16
+    # tau over the pairs: [ab][hi(Pp) | lo(Pp)], and [q(i,j)][hi(Kc) | lo(Kc)] (summed over cd)
+    tauR = ops.WS.get("tau_pairs", pp * 2 * Pp, dtype=i16, device=dev)
+    _lib.call("nl_tau_split_pairs", p, h, ops._p(t1), ops._p(t2), ops._p(tauR), Pp, dt,
+              ctypes.c_double(sc), st)
+    tauRT = ops.WS.get("tau_pairsT", P * 2 * Kc, dtype=i16, device=dev)
+    for q in range(2):
+        _lib.call("nl_transpose16_into", 1, pp, P, off(tauR, q * Pp), 2 * Pp, 0,
+                  off(tauRT, q * Kc), 2 * Kc, 0, Kc, st)
+    # Whh over the pairs, [q(k,l)][q(i,j)] at pitch Pp: B.Whh's v_hhhh and t1 terms reduced in
+    # both pairs, then + 1/2 v tau (exact v rows x split tau rows, split-K)
+    Wt = ops.WS.get("whh_pairs_t", h * h * Pp, device=dev)
+    _lib.call("nl_hole_pairs", h * h, h, Pp, 1, ops._p(B.Whh), ops._p(Wt), 0, st)
+    Wr = ops.WS.get("whh_pairs", P * Pp, device=dev)
+    _lib.call("nl_hole_pairs", 1, h, P, Pp, ops._p(Wt), ops._p(Wr), 0, st)
+    vimg, iv = _v_pairs_image(ham, dt)
+    splits = ops._lp_splits(P, P, Kc, 1)
+    ws = ops.WS.get("splitk", splits * P * P, device=dev) if splits > 1 else None
+    ops.count_flops("lp", 2 * 2.0 * P * P * Kc)
+    _lib.call("nl_gemm_lp_split", dt, P, P, Kc, ctypes.c_double(0.5 / sc), ops._p(vimg), None,
+              Kc, 0, ops._p(iv), 0, ops._p(tauRT), off(tauRT, Kc), 2 * Kc, 0, None, 0,
+              ctypes.c_double(1.0), ops._p(Wr), ops._p(Wr), Pp, 0, 1, splits, ops._p(ws), st)
+    # S0 over the pairs = sum_{k<l} tau[ab,kl] Whh[kl,ij]: Whh^T rows q(i,j) split with per-row
+    # scales (rows from P on stay zero), FP32 output
+    WT, iw = B.pair_operand(Pp)
+    target = ops.lp_target(dt, False)
+    _lib.call("nl_convert_split", dt, 1, P, 1, P, (ctypes.c_int64 * 4)(0, 1, 0, Pp), ops._p(Wr),
+              ops._p(WT), 2 * Pp, Pp, 2, 0b10, ops._p(iw) if target != 0.0 else None,
+              ctypes.c_float(target), ctypes.c_double(1.0), st)
+    S0 = B.s0_pairs(Pp)
+    ops.count_flops("lp", 3 * 2.0 * pp * P * Pp)
+    ev = ops._timed("hh_S")
+    if ev:
+        ev[0].record()
+    _lib.call("nl_gemm_lp_split", dt | (ops.C_F32 << 8), pp, P, Pp, ctypes.c_double(1.0 / sc),
+              ops._p(tauR), off(tauR, Pp), 2 * Pp, 0, None, 0, ops._p(WT), off(WT, Pp), 2 * Pp,
+              0, ops._p(iw) if target != 0.0 else None, 0, ctypes.c_double(0.0), None,
+              ops._p(S0), Pp, 0, 1, 1, None, st)
+    if ev:
+        ev[1].record()
+    # pp ladder into S0's pair rows (columns from P on receive zeros)
+    lad = prec.get("ladder", "fp64")
+    _block_gemm(ham.pppp, t2, S0, 0.5, Pp, "ladder", lad, tauR)
+    # R56 over the pairs, then N56 = v_phhh + 1/2 R56 expanded to [b][k][i][j]
+    N56r = ops.WS.get("n56_pairs", p * h * Pp, device=dev)
+    _lib.call("nl_hole_pairs", p * h, h, Pp, 1, ops._p(ham.v_phhh), ops._p(N56r), 0, st)
+    if ham.ppph.n:
+        _block_gemm(ham.ppph.r56, t2, N56r, 0.5, Pp, "r56", lad, tauR)
+    _lib.call("nl_hole_pairs", p * h, h, Pp, 1, ops._p(N56r), ops._p(B.N56), 1, st)
 
 
 def _sab_split(ham, t1, B, Sab, dt):
@@ -779,7 +911,11 @@ def assemble(ham, t2, B, t1n, t2n, mixing, H2out=None, img=None, prec=None):
     dt = _fast_split(ham, prec) if prec is not None else None
     # the split form's FP32 accumulators, with v_pphh added here (S0 does not hold it)
     acc, io = (B.acc32(), ACC_F32 | ADD_V) if dt is not None and ACC32 else (B, 0)
-    args = (p, h, ops._p(acc.S0), ops._p(acc.Sab), ops._p(acc.Sij), ops._p(acc.Spp),
+    S0 = acc.S0
+    if _pairs(ham, dt):
+        S0 = B.s0_pairs(_pair_dims(ham)[1])      # S0 over the hole pairs (expanded here)
+        io |= S0_PAIRS
+    args = (p, h, ops._p(S0), ops._p(acc.Sab), ops._p(acc.Sij), ops._p(acc.Spp),
             ops._p(B.X2pp),
             ops._p(B.X2hh), ops._p(getattr(B, "dpp", None)), None, ops._p(t2),
             ops._p(ham.v_pphh), ops._p(t1n), ctypes.c_double(mixing), ops._p(t2n),

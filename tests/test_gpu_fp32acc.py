@@ -234,3 +234,83 @@ def test_assemble_fp32_accumulators_add_v():
     assert L_().nl_assemble_t2_ext(p, h, *[P(a) for a in acc32], P(Xpp), P(Xhh), None, None,
                                    P(t2), None, None, ctypes.c_double(0.3), P(t2), None, P(part),
                                    None, 0, ctypes.c_double(1.0), 2, S()) == -1   # ADD_V, no v
+
+
+def _pairs(h):
+    return [(i, j) for i in range(h) for j in range(i + 1, h)]
+
+
+@pytest.mark.parametrize("h,w", [(6, 1), (28, 1), (5, 7)])
+def test_hole_pairs_reduce_expand(h, w):
+    """nl_hole_pairs: reduce keeps the pairs i < j in row order (slots beyond P untouched);
+    expand rebuilds the antisymmetric block (zero diagonal)."""
+    R, npr = 3, h * (h - 1) // 2
+    ldp = npr + 5
+    x = np.random.default_rng(h).standard_normal((R, h, h, w))
+    x = x - x.transpose(0, 2, 1, 3)
+    xin = torch.tensor(x, device="cuda")
+    red = torch.full((R, ldp, w), 9.0, dtype=F64, device="cuda")
+    assert L_().nl_hole_pairs(R, h, ldp, w, P(xin), P(red), 0, S()) == 0
+    back = torch.empty_like(xin)
+    assert L_().nl_hole_pairs(R, h, ldp, w, P(red), P(back), 1, S()) == 0
+    torch.cuda.synchronize()
+    ref = np.stack([x[:, i, j] for i, j in _pairs(h)], 1)
+    r = red.cpu().numpy()
+    np.testing.assert_array_equal(r[:, :npr], ref)
+    assert np.all(r[:, npr:] == 9.0)
+    np.testing.assert_array_equal(back.cpu().numpy(), x)
+
+
+@pytest.mark.parametrize("dtype,scale", [(0, 1.0), (1, 4096.0)])
+def test_tau_split_pairs_equals_full_tau_columns(dtype, scale):
+    """nl_tau_split_pairs: the [hi | lo] columns of the pairs i < j are those of nl_tau_split at
+    (i, j), bit for bit, and the slots from P on are zero."""
+    p, h = 12, 6
+    npr, Pp = h * (h - 1) // 2, 32
+    t1 = rnd((p, h), 50, 0.1)
+    t2 = rnd((p, p, h, h), 51, 0.01)
+    t2 = t2 - t2.transpose(2, 3)
+    full = torch.empty(2 * p * p * h * h, dtype=torch.int16, device="cuda")
+    red = torch.full((2 * p * p * Pp,), 7, dtype=torch.int16, device="cuda")
+    assert L_().nl_tau_split(p, h, P(t1), P(t2), None, P(full), dtype, ctypes.c_double(scale),
+                             S()) == 0
+    assert L_().nl_tau_split_pairs(p, h, P(t1), P(t2), P(red), Pp, dtype,
+                                   ctypes.c_double(scale), S()) == 0
+    torch.cuda.synchronize()
+    f = full.cpu().numpy().reshape(p * p, 2, h, h)
+    r = red.cpu().numpy().reshape(p * p, 2, Pp)
+    ref = np.stack([f[:, :, i, j] for i, j in _pairs(h)], 2)
+    np.testing.assert_array_equal(r[:, :, :npr], ref)
+    assert np.all(r[:, :, npr:] == 0)
+
+
+def test_assemble_s0_pairs_equals_full_s0():
+    """nl_assemble_t2_ext with NL_S0_PAIRS: S0 given over the hole pairs (rows of P32 slots)
+    gives the assembly of the antisymmetric full S0, bit for bit."""
+    p, h = 14, 6
+    npr, Pp = h * (h - 1) // 2, 32
+    n = p * p * h * h
+    S0 = rnd((p, p, h, h), 60, 0.1).to(F32)
+    S0 = S0 - S0.transpose(2, 3)
+    red = torch.zeros((p * p, Pp), dtype=F32, device="cuda")
+    for q, (i, j) in enumerate(_pairs(h)):
+        red[:, q] = S0[:, :, i, j].reshape(-1)
+    acc = [rnd((p, p, h, h), 61 + k, 0.1).to(F32) for k in range(2)] + \
+          [rnd((p, h, p, h), 63, 0.1).to(F32)]
+    v = rnd((p, p, h, h), 64, 0.05)
+    t2, t1n = rnd((p, p, h, h), 65, 0.01), rnd((p, h), 66, 0.01)
+    Xpp = rnd((p, p), 67) + 10.0 * torch.eye(p, dtype=F64, device="cuda")
+    Xhh = rnd((h, h), 68) - 10.0 * torch.eye(h, dtype=F64, device="cuda")
+    npart = 2 * int(L_().nl_partials(n)) + 8
+    out = []
+    for s0, io in ((red, 1 | 2 | 4), (S0, 1 | 2)):
+        t2n, H2 = torch.empty_like(t2), torch.empty_like(t2)
+        part = torch.empty(npart, dtype=F64, device="cuda")
+        assert L_().nl_assemble_t2_ext(p, h, P(s0), *[P(a) for a in acc], P(Xpp), P(Xhh), None,
+                                       None, P(t2), P(v), P(t1n), ctypes.c_double(0.3), P(t2n),
+                                       P(H2), P(part), None, 0, ctypes.c_double(1.0), io,
+                                       S()) == 0
+        out.append((t2n, H2))
+    torch.cuda.synchronize()
+    for k in range(2):
+        assert torch.equal(out[0][k], out[1][k])

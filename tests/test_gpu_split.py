@@ -287,7 +287,8 @@ def test_split_images_equal_per_product_conversions(prec, tol):
     per product: BF16 to summation order (1e-12: the same roundings); FP16 to the split's
     precision (1e-7: the shared images use the fixed scale 2^12, the per-product conversions
     per-row scales, so the roundings differ), with the shared-image path's accumulators in FP64
-    as the per-product path's; with its FP32 accumulators (the default) to 1e-6."""
+    as the per-product path's; with its FP32 accumulators, and with those and the hole-pair
+    products (the default), to 1e-6."""
     E0, fock, v = planewave_hamiltonian(3, 28, -4.0, 1.0)
     H = engine.Hamiltonian(fock, v, device="cuda")
     fast = ccm._PREC[prec]
@@ -296,12 +297,12 @@ def test_split_images_equal_per_product_conversions(prec, tol):
     assert engine._fast_split(H, slow) is None
     es, ts = _steps(H, slow, 4)
     scale = float(ts.abs().max())
-    for acc32, tl in ((False, tol), (True, 1e-6)):
-        engine.ACC32 = acc32
+    for acc32, pairs, tl in ((False, False, tol), (True, False, 1e-6), (True, True, 1e-6)):
+        engine.ACC32, engine.HOLE_PAIRS = acc32, pairs
         try:
             ef, tf = _steps(H, fast, 4)
         finally:
-            engine.ACC32 = True
+            engine.ACC32, engine.HOLE_PAIRS = True, True
         np.testing.assert_allclose(ef, es, rtol=tl)
         assert float((tf - ts).abs().max()) <= tl * scale
 
