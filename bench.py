@@ -48,12 +48,14 @@ def parse():
                     help="sample: one oracle iteration at the bench size with every dense "
                          "contraction in full and the two COO loops over every 16th entry "
                          "(their time scaled by 16); full: the whole iteration")
-    ap.add_argument("--form", default="all", choices=["all", "sectors", "diis", "combined"],
+    ap.add_argument("--form", default="all",
+                    choices=["all", "sectors", "diis", "combined", "iter"],
                     help="sectors: time only the symmetry-sector form and print its line; "
                          "diis: time only one DIIS extrapolation over 10 T2-sized vectors; "
                          "combined (the default for L >= 6): the largest-size line, FP32 "
                          "storage through the SoA driver, mixed-precision residual, adjoint "
-                         "gradient, seeded C0 and D0")
+                         "gradient, seeded C0 and D0; iter: only the FP64 and the precision "
+                         "form's iterations (the launches a kernel profile covers)")
     ap.add_argument("--grad", default="on", choices=["on", "off"],
                     help="adjoint dE/dC0, dE/dD0 at C0=-4, D0=1 checked against central FD")
     return ap.parse_args()
@@ -495,6 +497,12 @@ def main():
     print(f"# fp64: {s64 * 1e3:.1f} ms/iter", file=sys.stderr, flush=True)
     slp, klp, _ = time_iterations(ccm, engine, ops, H, a.prec, a.steps, a.warmup, torch)
     print(f"# {a.prec}: {slp * 1e3:.1f} ms/iter", file=sys.stderr, flush=True)
+    if a.form == "iter":
+        print(json.dumps({"form": "iter", "L": a.L, "A": a.A, "prec": a.prec,
+                          "ms_per_step": slp * 1e3, "fp64_ms_per_step": s64 * 1e3,
+                          "speedup_vs_fp64": s64 / slp,
+                          "ring_ms": (klp.get("ring") or 0) * 1e3}))
+        return
     # the single-rounding 16-bit forms, for reference (their fixed points miss 1e-6)
     singles = {}
     for other in ("bf16", "fp16"):
