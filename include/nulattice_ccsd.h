@@ -35,6 +35,11 @@ extern "C" {
  * (P = h(h-1)/2 rounded up to a multiple of 32; slots from P on unused), S0[a,b,i,j] =
  * S0[(a,b)][q(i,j)], -S0[(a,b)][q(j,i)] or 0. */
 #define NL_S0_PAIRS 4
+/* ... the split image of the new t2 (image_kind 1 or 2) is written over the hole pairs only:
+ * row a as [hi((b, q)) | lo((b, q))] with P32 slots per b (slots from P on untouched); and Sab
+ * comes over the hole pairs like S0 (NL_SAB_PAIRS). */
+#define NL_IMG_PAIRS 8
+#define NL_SAB_PAIRS 16
 
 /* C[b] = alpha * op(A[b]) * op(B[b]) + beta * C[b]  (FP64, MFMA 16x16x4).
  * Replaces every dense opt_einsum pairwise contraction of the diagram library
@@ -147,13 +152,23 @@ int nl_convert_rows(int dtype, int64_t R1, int64_t R2, int64_t C1, int64_t C2,
 int nl_tau_split(int64_t p, int64_t h, const double* t1, const double* t2, double* tau,
                  void* out, int dtype, double scale, void* stream);
 
-/* nl_tau_split over the hole pairs i < j only: row (a,b) of out is [hi(Pp) | lo(Pp)] holding
- * tau[a,b,i,j] * scale at column q(i,j) = i*h - i(i+1)/2 + j - i - 1 (P = h(h-1)/2 pairs;
- * columns P <= q < Pp zero).  tau is antisymmetric in (i, j) (ccDgrams.py:215-237 applies it
- * through pIJ), so the hh-ladder, pp-ladder and R56 products (coupled_cluster.py:476-536) run on
- * these P columns and, summed over (k, l), on P terms with a factor 2. */
+/* nl_tau_split over the hole pairs i < j only: tau[a,b,i,j] * scale at slot q(i,j) =
+ * i*h - i(i+1)/2 + j - i - 1 (P = h(h-1)/2 pairs; slots P <= q < Pp zero).  grp = 1: row (a,b)
+ * of out is [hi(Pp) | lo(Pp)]; grp = p: row a is [hi((b, q)) | lo((b, q))] (the layout of
+ * NL_IMG_PAIRS).  t1 may be null (then the image is that of t2).  tau is antisymmetric in
+ * (i, j) (ccDgrams.py:215-237 applies it through pIJ), so the hh-ladder, pp-ladder and R56
+ * products (coupled_cluster.py:476-536) run on these P columns and, summed over (k, l), on P
+ * terms with a factor 2. */
 int nl_tau_split_pairs(int64_t p, int64_t h, const double* t1, const double* t2, void* out,
-                       int64_t Pp, int dtype, double scale, void* stream);
+                       int64_t Pp, int64_t grp, int dtype, double scale, void* stream);
+
+/* The ring image of t2 from its hole-pair split image (NL_IMG_PAIRS layout, Pp slots per b):
+ * row (a,i) of out is [hi((c,k)) | lo((c,k))] of t2[a,c,i,k] (the ring operand of t2Iter's
+ * ccDgrams.py:240-303 products, coupled_cluster.py:476-536), each element the pair slot of
+ * (min(i,k), max(i,k)) with the sign of (i,k) (0 on i = k).  Pp a multiple of 32 (at most
+ * 16384: h <= 181), in 16-byte aligned. */
+int nl_ring_from_pairs(int64_t p, int64_t h, int64_t Pp, const void* in, void* out,
+                       void* stream);
 
 /* Hole-pair form of an array antisymmetric in an index pair (i, j) < h: expand = 0:
  * out[r][q][c] = in[r][i][j][c] for the P = h(h-1)/2 pairs q = (i < j), numbered row by row;

@@ -30,6 +30,8 @@ static inline int64_t nl_cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 #define NL_ACC_F32 1
 #define NL_ADD_V 2
 #define NL_S0_PAIRS 4   // S0 holds the hole pairs i < j of each (a,b) row only
+#define NL_IMG_PAIRS 8  // the split image of the new t2 over the hole pairs only
+#define NL_SAB_PAIRS 16 // Sab holds the hole pairs i < j of each (a,b) row only
 
 // An array of double or float elements addressed by element offset (a wave-uniform type).
 struct AnyOut {

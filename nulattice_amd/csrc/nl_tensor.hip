@@ -609,7 +609,8 @@ __device__ inline void pair_of(int q, int h, int& i, int& j) {
 // [hi(P) 0... | lo(P) 0...] at pitch 2*Pp (P = h(h-1)/2 pairs, columns P <= q < Pp zero).
 // tau is antisymmetric in (i, j), so these rows carry all of it.
 __global__ __launch_bounds__(NT) void tau_split_pairs_kernel(int64_t pp, int p, int h, int P,
-                                                             int Pp, FastDiv fp, FastDiv fPp,
+                                                             int Pp, int grp, FastDiv fp,
+                                                             FastDiv fPp, FastDiv fgrp,
                                                              const double* t1, const double* t2,
                                                              uint16_t* out, int bf16,
                                                              double scale) {
@@ -623,13 +624,70 @@ __global__ __launch_bounds__(NT) void tau_split_pairs_kernel(int64_t pp, int p, 
       int i, j;
       pair_of(q, h, i, j);
       const int a = fdiv((int)row, fp), b = (int)row - a * p;
-      const double v = tau_value(t2[row * h * h + i * h + j], t1[a * h + i], t1[b * h + j],
-                                 t1[a * h + j], t1[b * h + i]);
+      const double x = t2[row * h * h + i * h + j];
+      const double v = t1 ? tau_value(x, t1[a * h + i], t1[b * h + j], t1[a * h + j],
+                                      t1[b * h + i]) : x;
       split16(v * scale, bf16, hi, lo);
     }
-    uint16_t* y = out + row * 2 * Pp + q;
+    // rows grouped by grp: group g = row / grp holds [hi(grp * Pp) | lo(grp * Pp)]
+    const int64_t gq = fdiv((int)row, fgrp), rin = row - gq * grp;
+    uint16_t* y = out + gq * 2 * grp * Pp + rin * Pp + q;
     y[0] = hi;
-    y[Pp] = lo;
+    y[(int64_t)grp * Pp] = lo;
+  }
+}
+
+// The ring image of t2 from its hole-pair image: out row (a,i) = [hi((c,k)) | lo((c,k))] of
+// t2[a,c,i,k] = +-in[a][part][(c, q(min(i,k), max(i,k)))] (0 for i = k): in row a holds
+// [hi((c,q)) | lo((c,q))], Pp slots per c.  A 16-bit value is negated by its sign bit.
+// One workgroup per (a, block of ct columns c): both parts' slots of those c are read once into
+// LDS with 16-byte loads, then every output row (a,i) receives its two contiguous segments of
+// ct*h elements, 8 per thread and store (16 bytes when VEC: segment starts 8-aligned).
+template <bool VEC>
+__global__ __launch_bounds__(NT) void ring_from_pairs_kernel(int p, int h, int Pp, int ct,
+                                                             int nct, const uint16_t* in,
+                                                             uint16_t* out) {
+  extern __shared__ uint4 ring_lds_raw[];
+  uint16_t* lds = reinterpret_cast<uint16_t*>(ring_lds_raw);     // [part][cl][Pp]
+  const int a = blockIdx.x / nct, c0 = (blockIdx.x - a * nct) * ct;
+  const int cta = min(ct, p - c0);
+  const int64_t half = (int64_t)p * Pp, ph = (int64_t)p * h;
+  const int seg = cta * Pp;                                       // slots per part, mult of 32
+  for (int v = threadIdx.x; v < 2 * seg / 8; v += NT) {
+    const int part = v >= seg / 8, o = (v - part * (seg / 8)) * 8;
+    const uint4 x = *reinterpret_cast<const uint4*>(in + a * 2 * half + part * half +
+                                                    (int64_t)c0 * Pp + o);
+    *reinterpret_cast<uint4*>(lds + part * ct * Pp + o) = x;
+  }
+  __syncthreads();
+  const int len = cta * h, nch = (len + 7) / 8;                   // a segment's 8-chunks
+  for (int w = threadIdx.x; w < h * 2 * nch; w += NT) {
+    const int i = w / (2 * nch), rem = w - i * 2 * nch;
+    const int part = rem >= nch, ch = rem - part * nch;
+    const int s0 = ch * 8;
+    int cl = s0 / h, k = s0 - cl * h;
+    const uint16_t* src = lds + part * ct * Pp;
+    uint16_t val[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      uint16_t v = 0;
+      if (s0 + u < len && i != k) {
+        const int lo = i < k ? i : k, hi = i < k ? k : i;
+        v = src[cl * Pp + lo * h - lo * (lo + 1) / 2 + hi - lo - 1];
+        if (i > k) v ^= 0x8000;
+      }
+      val[u] = v;
+      if (++k == h) { k = 0; ++cl; }
+    }
+    uint16_t* dst = out + ((int64_t)a * h + i) * 2 * ph + part * ph + (int64_t)c0 * h + s0;
+    if (VEC && s0 + 8 <= len) {
+      uint4 x;
+      x.x = val[0] | ((uint32_t)val[1] << 16); x.y = val[2] | ((uint32_t)val[3] << 16);
+      x.z = val[4] | ((uint32_t)val[5] << 16); x.w = val[6] | ((uint32_t)val[7] << 16);
+      *reinterpret_cast<uint4*>(dst) = x;
+    } else {
+      for (int u = 0; u < 8 && s0 + u < len; ++u) dst[u] = val[u];
+    }
   }
 }
 
@@ -778,6 +836,9 @@ struct AssembleArgs {
   int s0pairs;                          // S0 holds the hole pairs i < j only: row (a,b) of
                                         // s0ld >= h(h-1)/2 slots, S0[a,b,i,j] = +-S0[(a,b)][q]
   int64_t s0ld;
+  int sabpairs;                         // Sab over the hole pairs too (rows (a,b) of s0ld slots)
+  int imgpairs;                         // the split image of t2n over the hole pairs only: row a
+                                        // as [hi((b,q)) | lo((b,q))], b of s0ld slots each
 };
 
 // S0[a,b,i,j] from S0's hole-pair rows (row = a*p + b): the antisymmetric expansion.
@@ -786,6 +847,17 @@ __device__ inline double s0_pair(const T* S0, int64_t row, int64_t ld, int h, in
   if (i < j) return (double)S0[row * ld + (i * h - i * (i + 1) / 2 + j - i - 1)];
   if (i > j) return -(double)S0[row * ld + (j * h - j * (j + 1) / 2 + i - j - 1)];
   return 0.0;
+}
+
+// The split image over the hole pairs: t2n[a,b,i,j] (i < j, pair slot q) at row a of
+// [hi(p * s0ld) | lo(p * s0ld)].
+__device__ inline void write_image_pair(const AssembleArgs& g, int64_t a, int64_t b, int64_t q,
+                                        double tn) {
+  const int64_t half = g.p * g.s0ld, o = a * 2 * half + b * g.s0ld + q;
+  uint16_t hi, lo;
+  split16(tn * g.img_scale, g.split16 == 1, hi, lo);
+  g.t2n16[o] = hi;
+  g.t2n16[o + half] = lo;
 }
 
 __device__ inline void write_image(const AssembleArgs& g, int64_t t, int64_t a, double tn) {
@@ -819,7 +891,10 @@ __global__ __launch_bounds__(NT) void assemble_t2_kernel(AssembleArgs g) {
     int64_t b = r % p, a = r / p;
     double H = !g.S0 ? 0.0 : g.s0pairs ? s0_pair(S0t, a * p + b, g.s0ld, (int)h, (int)i, (int)j)
                                         : S0p(t);
-    if (g.Sab) H += Sabp(t) - Sabp(((b * p + a) * h + i) * h + j);
+    if (g.Sab && g.sabpairs)
+      H += s0_pair(Sabt, a * p + b, g.s0ld, (int)h, (int)i, (int)j) -
+           s0_pair(Sabt, b * p + a, g.s0ld, (int)h, (int)i, (int)j);
+    else if (g.Sab) H += Sabp(t) - Sabp(((b * p + a) * h + i) * h + j);
     const int64_t ph = p * h;
     if (g.Sij) H += Sijp(t) - Sijp(((a * p + b) * h + j) * h + i);
     if (g.Spp) {
@@ -834,7 +909,11 @@ __global__ __launch_bounds__(NT) void assemble_t2_kernel(AssembleArgs g) {
     double nt = told + H / denom;
     double tn = g.mixing * told + (1.0 - g.mixing) * nt;
     if (g.t2n) g.t2n[t] = tn;
-    if (g.t2n16) write_image(g, t, a, tn);
+    if (g.t2n16 && g.imgpairs) {
+      if (i < j) write_image_pair(g, a, b, i * h - i * (i + 1) / 2 + j - i - 1, tn);
+    } else if (g.t2n16) {
+      write_image(g, t, a, tn);
+    }
     if (g.H2out) g.H2out[t] = H;
     if (g.vpphh) {
       double v = g.vpphh[t];
@@ -886,15 +965,21 @@ __global__ __launch_bounds__(AH * AH) void assemble_pair_kernel(AssembleArgs g) 
   const T* const Sijt = static_cast<const T*>(g.Sij);
   const T* const Sppt = static_cast<const T*>(g.Spp);
   __shared__ T Si[2][2][AH * ALD], Sp[2][2][AH * ALD];     // [buffer][(a,b) / (b,a)][i][j]
+  // hole-pair rows of S0 and Sab ((a,b) and (b,a) each), read coalesced (slot e by thread e)
+  // and staged here; each thread then takes its slot with its sign (FP32 accumulators only)
+  constexpr int PZ = sizeof(T) == 4 ? AH * (AH - 1) / 2 : 1;
+  __shared__ T Pz[2][4][PZ];                                // [buffer][S0 ab, ba, Sab ab, ba]
   const int64_t p = g.p, ph = p * g.h, pp = p * p;
-  const int h = (int)g.h, hh = h * h;
+  const int h = (int)g.h, hh = h * h, npair = h * (h - 1) / 2;
   const int e = threadIdx.x;
   const bool act = e < hh;
   const int i = act ? fdiv(e, g.fh) : 0, j = act ? e - i * h : 0;
-  // this thread's slot and sign in S0's hole-pair rows (fixed over all pairs a, b)
+  // this thread's slot and sign in the hole-pair rows (fixed over all pairs a, b)
   const int lo = i < j ? i : j, hi = i < j ? j : i;
-  const int64_t s0q = lo * h - lo * (lo + 1) / 2 + hi - lo - 1;
+  const int s0q = i != j ? lo * h - lo * (lo + 1) / 2 + hi - lo - 1 : 0;
   const T s0s = i < j ? T(1) : i > j ? T(-1) : T(0);
+  const bool pz = sizeof(T) == 4 && (g.s0pairs || g.sabpairs);
+  const bool slot = e < npair;                              // (slots < npair imply act)
   double e_acc = 0.0, r_acc = 0.0;
   auto finish = [&](int64_t a, int64_t b, int64_t t, double H, double told, double v) {
     double denom = -(g.Xpp[a * p + a] + g.Xpp[b * p + b] + g.Xhh[i * h + i] + g.Xhh[j * h + j]);
@@ -903,7 +988,11 @@ __global__ __launch_bounds__(AH * AH) void assemble_pair_kernel(AssembleArgs g) 
     if (g.dhh) H += (g.dhh[i] + g.dhh[j]) * told;
     double tn = g.mixing * told + (1.0 - g.mixing) * (told + H / denom);
     if (g.t2n) g.t2n[t] = tn;
-    if (g.t2n16) write_image(g, t, a, tn);
+    if (g.t2n16 && g.imgpairs) {
+      if (i < j) write_image_pair(g, a, b, s0q, tn);
+    } else if (g.t2n16) {
+      write_image(g, t, a, tn);
+    }
     if (g.H2out) g.H2out[t] = H;
     if (g.vpphh) e_acc += 0.25 * v * tn + 0.5 * v * g.t1n[a * h + i] * g.t1n[b * h + j];
     r_acc += H * H;
@@ -923,13 +1012,21 @@ __global__ __launch_bounds__(AH * AH) void assemble_pair_kernel(AssembleArgs g) 
     const int64_t oab = (a * g.h + i) * ph + b * g.h + j, oba = (b * g.h + i) * ph + a * g.h + j;
     r.sa = g.Sij ? Sijt[tab] : T(0); r.sb = g.Sij ? Sijt[tba] : T(0);
     r.pa = g.Spp ? Sppt[oab] : T(0); r.pb = g.Spp ? Sppt[oba] : T(0);
+    // pair rows: slot e (staged through LDS; pz) or this thread's slot with its sign
+    const int64_t qab = (a * p + b) * g.s0ld + (pz ? e : s0q);
+    const int64_t qba = (b * p + a) * g.s0ld + (pz ? e : s0q);
+    const T sg = pz ? T(1) : s0s;
+    const bool take = pz ? slot : true;
     if (g.s0pairs) {
-      r.s0a = s0s * S0t[(a * p + b) * g.s0ld + (i != j ? s0q : 0)];
-      r.s0b = s0s * S0t[(b * p + a) * g.s0ld + (i != j ? s0q : 0)];
+      r.s0a = take ? sg * S0t[qab] : T(0); r.s0b = take ? sg * S0t[qba] : T(0);
     } else {
       r.s0a = g.S0 ? S0t[tab] : T(0); r.s0b = g.S0 ? S0t[tba] : T(0);
     }
-    r.xa = g.Sab ? Sabt[tab] : T(0); r.xb = g.Sab ? Sabt[tba] : T(0);
+    if (g.sabpairs) {
+      r.xa = take ? sg * Sabt[qab] : T(0); r.xb = take ? sg * Sabt[qba] : T(0);
+    } else {
+      r.xa = g.Sab ? Sabt[tab] : T(0); r.xb = g.Sab ? Sabt[tba] : T(0);
+    }
     r.ta = g.t2[tab]; r.tb = g.t2[tba];
     r.va = g.vpphh ? g.vpphh[tab] : 0.0; r.vb = g.vpphh ? g.vpphh[tba] : 0.0;
   };
@@ -945,6 +1042,10 @@ __global__ __launch_bounds__(AH * AH) void assemble_pair_kernel(AssembleArgs g) 
       SiA[i * ALD + j] = cur.sa; SiB[i * ALD + j] = cur.sb;
       SpA[i * ALD + j] = cur.pa; SpB[i * ALD + j] = cur.pb;
     }
+    if (pz && slot) {
+      Pz[buf][0][e] = cur.s0a; Pz[buf][1][e] = cur.s0b;
+      Pz[buf][2][e] = cur.xa; Pz[buf][3][e] = cur.xb;
+    }
     const int64_t qn = next(q + gridDim.x);
     Pair nxt;
     if (act && qn < pp) load(qn, nxt);
@@ -953,9 +1054,12 @@ __global__ __launch_bounds__(AH * AH) void assemble_pair_kernel(AssembleArgs g) 
     asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
     if (act) {
       const int64_t tab = (a * p + b) * hh + e, tba = (b * p + a) * hh + e;
-      double Hab = (double)cur.s0a;
-      double Hba = (double)cur.s0b;
-      const double x = cur.xa, y = cur.xb;
+      // a staged row's slot (0 on the diagonal i = j, which has none)
+      auto staged = [&](int k) { return i != j ? s0s * Pz[buf][k][s0q] : T(0); };
+      double Hab = (double)(pz && g.s0pairs ? staged(0) : cur.s0a);
+      double Hba = (double)(pz && g.s0pairs ? staged(1) : cur.s0b);
+      const double x = pz && g.sabpairs ? staged(2) : cur.xa;
+      const double y = pz && g.sabpairs ? staged(3) : cur.xb;
       Hab += x - y;
       Hba += y - x;
       Hab += (double)SiA[i * ALD + j] - (double)SiA[j * ALD + i];
@@ -1365,14 +1469,43 @@ int nl_tau_split(int64_t p, int64_t h, const double* t1, const double* t2, doubl
 }
 
 int nl_tau_split_pairs(int64_t p, int64_t h, const double* t1, const double* t2, void* out,
-                       int64_t Pp, int dtype, double scale, void* stream) {
+                       int64_t Pp, int64_t grp, int dtype, double scale, void* stream) {
   const int64_t P = h * (h - 1) / 2;
-  if (p < 0 || h < 0 || !t1 || !t2 || !out || (dtype != 0 && dtype != 1) || Pp < P) return NL_EARG;
+  if (p < 0 || h < 0 || !t2 || !out || (dtype != 0 && dtype != 1) || Pp < P) return NL_EARG;
+  if (grp != 1 && grp != p) return NL_EARG;
   if (p * p * Pp == 0) return NL_OK;
   if (p * p * Pp >= INT32_MAX || p * h > INT32_MAX) return NL_EARG;
   hipLaunchKernelGGL(tau_split_pairs_kernel, dim3(grid_for(p * p * Pp)), dim3(NT), 0,
-                     (hipStream_t)stream, p * p, (int)p, (int)h, (int)P, (int)Pp, nl_fastdiv(p),
-                     nl_fastdiv(Pp), t1, t2, (uint16_t*)out, dtype == 0, scale);
+                     (hipStream_t)stream, p * p, (int)p, (int)h, (int)P, (int)Pp, (int)grp,
+                     nl_fastdiv(p), nl_fastdiv(Pp), nl_fastdiv(grp), t1, t2, (uint16_t*)out,
+                     dtype == 0, scale);
+  NL_LAUNCH_CHECK();
+  return NL_OK;
+}
+
+int nl_ring_from_pairs(int64_t p, int64_t h, int64_t Pp, const void* in, void* out,
+                       void* stream) {
+  const int64_t P = h * (h - 1) / 2;
+  if (p < 0 || h < 0 || Pp < P || (Pp % 32) || !in || !out) return NL_EARG;
+  if (p * h == 0) return NL_OK;
+  if ((((uintptr_t)in) & 15) || p > INT32_MAX || h > 65535) return NL_EARG;
+  // columns c per workgroup: 16, fewer when both parts' slots would pass 64 KB of LDS
+  const int64_t ct = std::max<int64_t>(1, std::min<int64_t>(16, 65536 / (4 * std::max<int64_t>(Pp, 32))));
+  if (4 * Pp * ct > 65536) return NL_EARG;                        // h above 181
+  const int64_t nct = nl_cdiv(p, ct), blocks = p * nct;
+  if (blocks > INT32_MAX) return NL_EARG;
+  // 16-byte stores when every segment start (a,i,part,c0) is 8-aligned
+  const bool vec = ((p * h) % 8 == 0) && ((ct * h) % 8 == 0) && !(((uintptr_t)out) & 15);
+  const size_t lds = (size_t)(2 * ct * Pp * 2);
+  hipStream_t st = (hipStream_t)stream;
+  if (vec)
+    hipLaunchKernelGGL(ring_from_pairs_kernel<true>, dim3((unsigned)blocks), dim3(NT), lds, st,
+                       (int)p, (int)h, (int)Pp, (int)ct, (int)nct, (const uint16_t*)in,
+                       (uint16_t*)out);
+  else
+    hipLaunchKernelGGL(ring_from_pairs_kernel<false>, dim3((unsigned)blocks), dim3(NT), lds, st,
+                       (int)p, (int)h, (int)Pp, (int)ct, (int)nct, (const uint16_t*)in,
+                       (uint16_t*)out);
   NL_LAUNCH_CHECK();
   return NL_OK;
 }
@@ -1613,8 +1746,9 @@ static int assemble_t2_impl(int64_t p, int64_t h, const void* S0, const void* Sa
                             int split16 = 0, double img_scale = 1.0, int io = 0) {
   if (p < 0 || h < 0 || !Xpp || !Xhh || !t2 || !part) return NL_EARG;
   if (vpphh && !t1n) return NL_EARG;
-  if ((io & ~(NL_ACC_F32 | NL_ADD_V | NL_S0_PAIRS)) || ((io & NL_ADD_V) && !vpphh) ||
-      ((io & NL_S0_PAIRS) && !S0))
+  if ((io & ~(NL_ACC_F32 | NL_ADD_V | NL_S0_PAIRS | NL_IMG_PAIRS | NL_SAB_PAIRS)) ||
+      ((io & NL_ADD_V) && !vpphh) || ((io & NL_S0_PAIRS) && !S0) ||
+      ((io & NL_SAB_PAIRS) && !Sab) || ((io & NL_IMG_PAIRS) && (!t2n16 || split16 == 0)))
     return NL_EARG;
   int64_t n = p * p * h * h;
   if (n == 0) return NL_OK;
@@ -1623,6 +1757,8 @@ static int assemble_t2_impl(int64_t p, int64_t h, const void* S0, const void* Sa
   g.p = p; g.h = h; g.S0 = S0; g.Sab = Sab; g.Sij = Sij; g.Spp = Spp;
   g.Xpp = Xpp; g.Xhh = Xhh; g.addv = (io & NL_ADD_V) != 0;
   g.s0pairs = (io & NL_S0_PAIRS) != 0;
+  g.sabpairs = (io & NL_SAB_PAIRS) != 0;
+  g.imgpairs = (io & NL_IMG_PAIRS) != 0;
   g.s0ld = nl_cdiv(h * (h - 1) / 2, 32) * 32;
   g.t2 = t2; g.vpphh = vpphh; g.t1n = t1n; g.mixing = mixing; g.t2n = t2n; g.part = part;
   g.H2out = H2out; g.t2n16 = t2n16; g.dpp = dpp; g.dhh = dhh; g.split16 = split16;

@@ -45,8 +45,8 @@ from .plans import PairBlocks, PpphPlans, coo_arrays
 F64 = torch.float64
 
 # nl_assemble_t2_ext flags: FP32 accumulators; v_pphh added to H2 by the assembly; S0 over the
-# hole pairs i < j
-ACC_F32, ADD_V, S0_PAIRS = 1, 2, 4
+# hole pairs i < j; the split image of t2n written over the hole pairs; Sab over the hole pairs
+ACC_F32, ADD_V, S0_PAIRS, IMG_PAIRS_IO, SAB_PAIRS = 1, 2, 4, 8, 16
 
 
 def _dev(x, device):
@@ -179,7 +179,8 @@ class Buffers:
         self.p, self.h, self.device = p, h, device
         self._acc32 = None
         self._sab_img = None
-        self.t2_16 = self.t2_split = None
+        self.t2_16 = self.t2_split = self.t2_pairs = None
+        self.sab_pairs = False       # Sab of this step over the hole pairs (NL_SAB_PAIRS)
 
     def s0_pairs(self, Pp):
         """S0 of the split form over the hole pairs: [p*p][Pp] FP32 (slots from P on unused)."""
@@ -198,10 +199,20 @@ class Buffers:
                               torch.ones(Pp, dtype=torch.float32, device=self.device))
         return b
 
-    def sab_image(self, kp):
-        """The K-concatenated [N56 | t2^T] operand of the split Sab product: p*h*h rows of
-        [hi(kp) | lo(kp)] 16-bit elements, zero-initialised once (its pad columns stay zero)."""
-        n = self.p * self.h * self.h * 2 * kp
+    def n56_pairs(self, Pp):
+        """N56 over the hole pairs: [(b,k)][Pp] FP64, zero-initialised (slots from P on stay
+        zero: the split Sab product reads them)."""
+        n = self.p * self.h * Pp
+        b = getattr(self, "_n56p", None)
+        if b is None or b.numel() != n:
+            b = self._n56p = torch.zeros(n, dtype=torch.float64, device=self.device)
+        return b
+
+    def sab_image(self, kp, rows=None):
+        """The K-concatenated [N56 | t2^T] operand of the split Sab product: ``rows`` (p*h*h, or
+        p*Pp over the hole pairs) rows of [hi(kp) | lo(kp)] 16-bit elements, zero-initialised
+        once (its pad columns stay zero)."""
+        n = (self.p * self.h * self.h if rows is None else rows) * 2 * kp
         if self._sab_img is None or self._sab_img.numel() != n:
             self._sab_img = torch.zeros(n, dtype=torch.int16, device=self.device)
         return self._sab_img
@@ -260,6 +271,22 @@ def _pairs(ham, split):
     return split is not None and ACC32 and HOLE_PAIRS and ham.h >= 2
 
 
+# With the hole-pair products, t2's split image is carried over the hole pairs i < j too (half
+# the full image): the ring operand is expanded from it with the antisymmetry's signs, and the
+# A19 and Sab products read it directly (False: the full image, for comparison checks).
+IMG_PAIRS = True
+
+
+def _img_pairs(ham, split):
+    """Whether this split-form step carries t2's split image over the hole pairs."""
+    return _pairs(ham, split) and IMG_PAIRS
+
+
+def _sab_pairs(ham, split):
+    """Whether this step's split Sab product runs over the hole pairs (output columns (b, q))."""
+    return _img_pairs(ham, split) and ham.p >= ops.LP_MIN_DIM
+
+
 def t1_part(ham, t1, t2, B, prec=DEFAULT_PREC):
     """H1, X1pp, X1hh of t1Iter (coupled_cluster.py:372-408) into B.H1/B.X1pp/B.X1hh."""
     vpphh = ham.vpphh_view()
@@ -303,8 +330,11 @@ def t1_part(ham, t1, t2, B, prec=DEFAULT_PREC):
     permute_into(ham.f_pp, Xpp)
     contract("ck,ak->ac", ham.f_ph, t1, Xpp, alpha=-0.5)                     # A16
     # A19 into Dpp: equal to A41 of the T2 part (antisymmetry of v_pphh in c,d and of t2 in a,b)
-    contract("dckl,adkl->ac", ham.v_pphh_T.permute(1, 0, 2, 3), t2, B.Dpp, alpha=0.5,
-             beta=0.0, precision=prec["dress"])         # t2[d,a,k,l] = -t2[a,d,k,l]
+    if B.t2_pairs is not None:
+        _dpp_pairs(ham, B)
+    else:
+        contract("dckl,adkl->ac", ham.v_pphh_T.permute(1, 0, 2, 3), t2, B.Dpp, alpha=0.5,
+                 beta=0.0, precision=prec["dress"])     # t2[d,a,k,l] = -t2[a,d,k,l]
     ops.axpby(1.0, B.Dpp, 1.0, Xpp)
     contract("cl,al->ac", B.w_ph, t1, Xpp, alpha=0.5)                       # A20 (pp)
     if pl.n:
@@ -314,6 +344,48 @@ def t1_part(ham, t1, t2, B, prec=DEFAULT_PREC):
 
 
 SPLIT = (2, 0b10)       # [hi | lo] rows of a split operand image
+
+
+def _v_dpp_image(ham, dt):
+    """The exact 16-bit image of v_pphh[d,c,k,l] over the hole pairs k < l: rows c, columns
+    (d, q(k,l)) (Pp slots per d, zero from P on), per-row scales that make a contact
+    interaction's elements exact; built once per Hamiltonian and type."""
+    hit = getattr(ham, "_v_dpp", None)
+    if hit is not None and hit[0] == dt:
+        return hit[1], hit[2]
+    p, h = ham.p, ham.h
+    Pp = _pair_dims(ham)[1]
+    dev, st = ham.v_pphh.device, ops._stream()
+    K = p * Pp
+    vr = torch.zeros(p * K, dtype=F64, device=dev)                 # [(c,d)][q(k,l)]
+    _lib.call("nl_hole_pairs", p * p, h, Pp, 1, ops._p(ham.v_pphh_T), ops._p(vr), 0, st)
+    img = torch.empty(p * K, dtype=torch.int16, device=dev)
+    iv = torch.empty(p, dtype=torch.float32, device=dev)
+    _lib.call("nl_convert_rows", dt, 1, p, 1, K, (ctypes.c_int64 * 4)(0, K, 0, 1), ops._p(vr),
+              ops._p(img), K, ops._p(iv), ctypes.c_float(ops.lp_target(dt, True)), st)
+    del vr                          # (stream-ordered: its memory is reused after these kernels)
+    ham._v_dpp = (dt, img, iv)
+    return img, iv
+
+
+def _dpp_pairs(ham, B):
+    """A19 from the hole-pair image of t2: Dpp[a,c] = 1/2 sum_{d,k,l} v[d,c,k,l] t2[a,d,k,l]
+    = sum_{d, k<l} v[d,c,k,l] t2[a,d,k,l] (both antisymmetric in (k, l)), as the split product
+    t2_hi v + t2_lo v over K = (d, q): rows a of the pair image [hi | lo], rows c of v's exact
+    pair image."""
+    p = ham.p
+    dt = B.t2_pairs_dt
+    Pp = _pair_dims(ham)[1]
+    K = p * Pp
+    NR = B.t2_pairs
+    vimg, iv = _v_dpp_image(ham, dt)
+    splits = ops._lp_splits(p, p, K, 1)
+    ws = ops.WS.get("splitk", splits * p * p, device=NR.device) if splits > 1 else None
+    ops.count_flops("lp", 2 * 2.0 * p * p * K)
+    _lib.call("nl_gemm_lp_split", dt, p, p, K, ctypes.c_double(1.0 / AMP_SCALE[dt]),
+              ops._p(NR), ops._off16(NR, K), 2 * K, 0, None, 0, ops._p(vimg), None, K, 0,
+              ops._p(iv), 0, ctypes.c_double(0.0), None, ops._p(B.Dpp), p, 0, 1, splits,
+              ops._p(ws), ops._stream())
 
 # Fixed scale of the shared FP16 split images of the amplitudes (t2, tau): 2^12 keeps
 # |t| < 16 inside FP16's range and the elements down to ~1e-6 (far below the ones that set the
@@ -445,8 +517,10 @@ def t2_part(ham, t1, t2, B, prec=DEFAULT_PREC):
     if prec["dress"] != "fp64" and DIAG_SPLIT:
         _lib.call("nl_take_diag", p, ops._p(B.Mpp), ops._p(B.dMpp), ops._stream())
         B.dpp = B.dMpp
+    B.sab_pairs = False
     if split is not None and p >= ops.LP_MIN_DIM:
         _sab_split(ham, t1, B, Sab, split)
+        B.sab_pairs = B.t2_pairs is not None
     elif getattr(B, "t2_16", None) is not None and p >= ops.LP_MIN_DIM:
         _sab_bf16(ham, t1, B, Sab)
     else:
@@ -559,7 +633,8 @@ def _hh_pairs(ham, t1, t2, B, dt, prec):
         Whh[kl,ij] (k<l, i<j) = its v_hhhh and t1 terms + 1/2 sum_cd v[c,d,k,l] tau[c,d,i,j]
         S0[ab,ij]  (i<j)      = sum_{k<l} tau[a,b,k,l] Whh[kl,ij] + 1/2 sum_cd V[ab,cd] tau[cd,ij]
         N56[bk,ij] (i<j)      = v_phhh + 1/2 sum_cd v_ppph[c,d,b,k] tau[c,d,i,j]
-    S0 stays in the pair form (the assembly expands it); N56 is expanded for the Sab product."""
+    S0 stays in the pair form (the assembly expands it); N56 too when the Sab product runs over
+    the pairs (B.N56r), else it is expanded for that product."""
     p, h = ham.p, ham.h
     pp = p * p
     P, Pp, Kc = _pair_dims(ham)
@@ -569,7 +644,7 @@ def _hh_pairs(ham, t1, t2, B, dt, prec):
     off = ops._off16
     # tau over the pairs: [ab][hi(Pp) | lo(Pp)], and [q(i,j)][hi(Kc) | lo(Kc)] (summed over cd)
     tauR = ops.WS.get("tau_pairs", pp * 2 * Pp, dtype=i16, device=dev)
-    _lib.call("nl_tau_split_pairs", p, h, ops._p(t1), ops._p(t2), ops._p(tauR), Pp, dt,
+    _lib.call("nl_tau_split_pairs", p, h, ops._p(t1), ops._p(t2), ops._p(tauR), Pp, 1, dt,
               ctypes.c_double(sc), st)
     tauRT = ops.WS.get("tau_pairsT", P * 2 * Kc, dtype=i16, device=dev)
     for q in range(2):
@@ -609,12 +684,16 @@ def _hh_pairs(ham, t1, t2, B, dt, prec):
     # pp ladder into S0's pair rows (columns from P on receive zeros)
     lad = prec.get("ladder", "fp64")
     _block_gemm(ham.pppp, t2, S0, 0.5, Pp, "ladder", lad, tauR)
-    # R56 over the pairs, then N56 = v_phhh + 1/2 R56 expanded to [b][k][i][j]
-    N56r = ops.WS.get("n56_pairs", p * h * Pp, device=dev)
+    # R56 over the pairs: N56 = v_phhh + 1/2 R56 as [(b,k)][q(i,j)], expanded to [b][k][i][j]
+    # unless the Sab product reads the pairs
+    N56r = B.n56_pairs(Pp)
     _lib.call("nl_hole_pairs", p * h, h, Pp, 1, ops._p(ham.v_phhh), ops._p(N56r), 0, st)
     if ham.ppph.n:
         _block_gemm(ham.ppph.r56, t2, N56r, 0.5, Pp, "r56", lad, tauR)
-    _lib.call("nl_hole_pairs", p * h, h, Pp, 1, ops._p(N56r), ops._p(B.N56), 1, st)
+    if _sab_pairs(ham, dt):
+        B.N56r = N56r
+    else:
+        _lib.call("nl_hole_pairs", p * h, h, Pp, 1, ops._p(N56r), ops._p(B.N56), 1, st)
 
 
 def _sab_split(ham, t1, B, Sab, dt):
@@ -622,20 +701,34 @@ def _sab_split(ham, t1, B, Sab, dt):
     over K = (k | d) (kp = h + p padded to 32): A = [t1 | Mpp] rows a as [hi(kp) | lo(kp)] with
     per-row scales, B = [N56 | t2^T] rows (b,i,j) at the amplitude images' fixed scale; the
     t2^T columns are 16-bit transposes of this step's split image of t2 (B.t2_split).  Sab is
-    written once (beta = 0) instead of by a low-K FP64 pass and a product that reads it back."""
+    written once (beta = 0) instead of by a low-K FP64 pass and a product that reads it back.
+    Over the hole pairs (B.t2_pairs set): Sab is antisymmetric in (i, j) as N56 and t2 are, so
+    the product runs on the rows (b, q(i<j)) of N56's and t2's pair forms and writes Sab as
+    [a][(b, q)] (Pp slots per b, the assembly's NL_SAB_PAIRS layout)."""
     p, h = ham.p, ham.h
     kp = (h + p + 31) // 32 * 32
-    ld, row = 2 * kp, p * h * h
+    ld = 2 * kp
     dev, st = t1.device, ops._stream()
     sc = AMP_SCALE[dt]
-    img = B.sab_image(kp)           # zero-initialised: the columns [h + p, kp) stay zero
     i4 = lambda *v: (ctypes.c_int64 * 4)(*v)
-    # N56[b,k,i,j] as rows (b,i,j), columns k: hi into [0, h), lo into [kp, kp + h)
+    if B.t2_pairs is not None:
+        Pp = _pair_dims(ham)[1]
+        row = p * Pp
+        img = B.sab_image(kp, row)
+        # N56 over the pairs [(b,k)][q] as rows (b,q), columns k
+        n56, n56s = B.N56r, i4(h * Pp, 1, 0, Pp)
+        R2, N = Pp, B.t2_pairs      # [d][hi(b,q) | lo(b,q)]
+    else:
+        row = p * h * h
+        img = B.sab_image(kp)
+        # N56[b,k,i,j] as rows (b,i,j), columns k
+        n56, n56s = B.N56, i4(h ** 3, 1, 0, h * h)
+        R2, N = h * h, B.t2_split   # [d][hi(b,i,j) | lo(b,i,j)]
+    # (img is zero-initialised: the columns [h + p, kp) stay zero)
+    # N56: hi into columns [0, h), lo into [kp, kp + h)
     for off, mask in ((0, 0), (kp, 1)):
-        _lib.call("nl_convert_split", dt, p, h * h, 1, h, i4(h ** 3, 1, 0, h * h), ops._p(B.N56),
-                  ops._off16(img, off), ld, h, 1, mask, None, ctypes.c_float(0.0),
-                  ctypes.c_double(sc), st)
-    N = B.t2_split                  # [d][hi(b,i,j) | lo(b,i,j)]
+        _lib.call("nl_convert_split", dt, p, R2, 1, h, n56s, ops._p(n56), ops._off16(img, off),
+                  ld, h, 1, mask, None, ctypes.c_float(0.0), ctypes.c_double(sc), st)
     for q in range(2):
         _lib.call("nl_transpose16_into", 1, p, row, ops._off16(N, q * row), 2 * row, 0,
                   ops._off16(img, q * kp + h), ld, 0, p, st)
@@ -790,13 +883,18 @@ class T2Image:
         self.cur = 0
         self.src = None
         self.kind = 0           # 0: BF16 image, 1 / 2: [hi | lo] BF16 / FP16 split image
-                                # (nl_assemble_t2_ext's image kinds)
+                                # (nl_assemble_t2_ext's image kinds), 3 / 4: the same split
+                                # over the hole pairs i < j (NL_IMG_PAIRS)
+        self.zeroed = [False, False]
 
-    def buffer(self, k, n, dev):
+    def buffer(self, k, n, dev, zero=False):
+        """Buffer k of n 16-bit elements; ``zero``: zero-initialised (an image that leaves slots
+        unwritten, whose zeros the products read) and not since written by a full image."""
         b = self.bufs[k]
-        if b is None or b.numel() < n or b.device != dev:
-            b = torch.empty(n, dtype=torch.int16, device=dev)
+        if b is None or b.numel() < n or b.device != dev or (zero and not self.zeroed[k]):
+            b = (torch.zeros if zero else torch.empty)(n, dtype=torch.int16, device=dev)
             self.bufs[k] = b
+        self.zeroed[k] = zero
         return b[:n]
 
     def holds(self, t2, kind=0):
@@ -822,7 +920,7 @@ def update(ham, t1, t2, B, t1n, t2n, mixing, prec=DEFAULT_PREC, H2out=None, img=
             t2_part(ham, t1, t2, B, prec)
         finally:
             B.t2_16 = None
-            B.t2_split = None
+            B.t2_split = B.t2_pairs = B.N56r = None
     assemble(ham, t2, B, t1n, t2n, mixing, H2out, img=img, prec=prec)
 
 
@@ -879,11 +977,28 @@ def _t2_split_images(ham, t2, dt, B, img=None):
     """Register the [hi | lo] split images of t2 that the bf16x3 products read: the rows
     [a][(d,k,l)] (A19; written by the previous step's assembly, or split here from FP64; kept
     as B.t2_split for the Sab product), and from them by a 16-bit permute [(a,i)][(c,k)] (both
-    ring products and, with a sign, Y).  Needs t2 under ops.lp_reuse."""
+    ring products and, with a sign, Y).  Needs t2 under ops.lp_reuse.
+    Over the hole pairs (_img_pairs): the image is [a][hi((d,q)) | lo((d,q))] (Pp slots per d,
+    zero from P on), kept as B.t2_pairs for the A19 and Sab products, and the ring rows are
+    expanded from it with the antisymmetry's signs (nl_ring_from_pairs)."""
     p, h = ham.p, ham.h
     n, row, ph = p * p * h * h, p * h * h, p * h
     dev, st = t2.device, ops._stream()
-    kind, sc = 1 + dt, AMP_SCALE[dt]
+    sc = AMP_SCALE[dt]
+    R = ops.WS.get("t2_split_ring", 2 * n, dtype=torch.int16, device=dev)
+    if _img_pairs(ham, dt):
+        Pp = _pair_dims(ham)[1]
+        if img is not None and img.holds(t2, 3 + dt):
+            NR = img.buffer(img.cur, 2 * p * p * Pp, dev, zero=True)
+        else:
+            NR = ops.WS.get("t2_split_pairs", 2 * p * p * Pp, dtype=torch.int16, device=dev)
+            _lib.call("nl_tau_split_pairs", p, h, None, ops._p(t2), ops._p(NR), Pp, p, dt,
+                      ctypes.c_double(sc), st)
+        _lib.call("nl_ring_from_pairs", p, h, Pp, ops._p(NR), ops._p(R), st)
+        ops.lp_provide(t2, "acik", ["a", "i"], ["c", "k"], R, 2 * ph, dt, SPLIT, sc)
+        B.t2_pairs, B.t2_pairs_dt = NR, dt
+        return
+    kind = 1 + dt
     if img is not None and img.holds(t2, kind):
         N = img.buffer(img.cur, 2 * n, dev)
     else:
@@ -893,7 +1008,6 @@ def _t2_split_images(ham, t2, dt, B, img=None):
                   row, 2, 0b10, None, ctypes.c_float(0.0), ctypes.c_double(sc), st)
     ops.lp_provide(t2, "adkl", ["a"], ["d", "k", "l"], N, 2 * row, dt, SPLIT, sc)
     # ring operand rows (a,i), per part the columns (c,k): out (a, i, part, c, k)
-    R = ops.WS.get("t2_split_ring", 2 * n, dtype=torch.int16, device=dev)
     shp = (ctypes.c_int64 * 5)(p, h, 2, p, h)
     istr = (ctypes.c_int64 * 5)(2 * row, h, row, h * h, 1)
     _lib.call("nl_permute16", 5, shp, istr, ops._p(N), ops._p(R), st)
@@ -915,13 +1029,23 @@ def assemble(ham, t2, B, t1n, t2n, mixing, H2out=None, img=None, prec=None):
     if _pairs(ham, dt):
         S0 = B.s0_pairs(_pair_dims(ham)[1])      # S0 over the hole pairs (expanded here)
         io |= S0_PAIRS
+    if B.sab_pairs:
+        io |= SAB_PAIRS                          # Sab over the hole pairs (from _sab_split)
+        B.sab_pairs = False
     args = (p, h, ops._p(S0), ops._p(acc.Sab), ops._p(acc.Sij), ops._p(acc.Spp),
             ops._p(B.X2pp),
             ops._p(B.X2hh), ops._p(getattr(B, "dpp", None)), None, ops._p(t2),
             ops._p(ham.v_pphh), ops._p(t1n), ctypes.c_double(mixing), ops._p(t2n),
             ops._p(H2out), ops._p(B.part))
     buf, kind, sc = None, 0, 1.0
-    if img is not None and t2n is not None and dt is not None:
+    if img is not None and t2n is not None and dt is not None and _img_pairs(ham, dt):
+        # the split image over the hole pairs (zero-initialised: slots from P on stay zero)
+        nxt = 1 - img.cur
+        buf = img.buffer(nxt, 2 * p * p * _pair_dims(ham)[1], t2n.device, zero=True)
+        kind, sc = 1 + dt, AMP_SCALE[dt]
+        img.cur, img.src, img.kind = nxt, t2n, 3 + dt
+        io |= IMG_PAIRS_IO
+    elif img is not None and t2n is not None and dt is not None:
         nxt = 1 - img.cur
         buf, kind, sc = img.buffer(nxt, 2 * p * p * h * h, t2n.device), 1 + dt, AMP_SCALE[dt]
         img.cur, img.src, img.kind = nxt, t2n, kind

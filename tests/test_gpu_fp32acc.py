@@ -274,7 +274,7 @@ def test_tau_split_pairs_equals_full_tau_columns(dtype, scale):
     red = torch.full((2 * p * p * Pp,), 7, dtype=torch.int16, device="cuda")
     assert L_().nl_tau_split(p, h, P(t1), P(t2), None, P(full), dtype, ctypes.c_double(scale),
                              S()) == 0
-    assert L_().nl_tau_split_pairs(p, h, P(t1), P(t2), P(red), Pp, dtype,
+    assert L_().nl_tau_split_pairs(p, h, P(t1), P(t2), P(red), Pp, 1, dtype,
                                    ctypes.c_double(scale), S()) == 0
     torch.cuda.synchronize()
     f = full.cpu().numpy().reshape(p * p, 2, h, h)
@@ -314,3 +314,109 @@ def test_assemble_s0_pairs_equals_full_s0():
     torch.cuda.synchronize()
     for k in range(2):
         assert torch.equal(out[0][k], out[1][k])
+
+
+@pytest.mark.parametrize("dtype,scale", [(0, 1.0), (1, 4096.0)])
+def test_tau_split_pairs_grouped_image_of_t2(dtype, scale):
+    """nl_tau_split_pairs with grp = p and no t1: row a holds [hi((b,q)) | lo((b,q))], the same
+    bits as the grp = 1 rows (a,b) of tau with t1 = 0 (tau = t2), slots from P on zero."""
+    p, h = 12, 6
+    npr, Pp = h * (h - 1) // 2, 32
+    t2 = rnd((p, p, h, h), 52, 0.01)
+    t2 = t2 - t2.transpose(2, 3)
+    z1 = torch.zeros(p, h, dtype=F64, device="cuda")
+    rows = torch.full((2 * p * p * Pp,), 7, dtype=torch.int16, device="cuda")
+    grp = torch.full((2 * p * p * Pp,), 7, dtype=torch.int16, device="cuda")
+    assert L_().nl_tau_split_pairs(p, h, P(z1), P(t2), P(rows), Pp, 1, dtype,
+                                   ctypes.c_double(scale), S()) == 0
+    assert L_().nl_tau_split_pairs(p, h, None, P(t2), P(grp), Pp, p, dtype,
+                                   ctypes.c_double(scale), S()) == 0
+    assert L_().nl_tau_split_pairs(p, h, None, P(t2), P(grp), Pp, 2, dtype,
+                                   ctypes.c_double(scale), S()) == -1    # grp neither 1 nor p
+    torch.cuda.synchronize()
+    r = rows.cpu().numpy().reshape(p, p, 2, Pp).transpose(0, 2, 1, 3)
+    g = grp.cpu().numpy().reshape(p, 2, p, Pp)
+    np.testing.assert_array_equal(g, r)
+    assert np.all(g[:, :, :, npr:] == 0) and np.any(g[:, 0, :, :npr] != 0)
+
+
+@pytest.mark.parametrize("p,h", [(6, 5), (10, 28), (18, 28), (20, 6), (3, 2)])
+def test_ring_from_pairs(p, h):
+    """nl_ring_from_pairs: row (a,i), part, column (c,k) of the ring image is the pair slot of
+    (min(i,k), max(i,k)) of row a, part, block c, with the sign bit flipped for i > k and zero
+    for i = k (integer reference over every element)."""
+    npr = h * (h - 1) // 2
+    Pp = (npr + 31) // 32 * 32
+    rng = np.random.default_rng(h)
+    src = rng.integers(-32768, 32768, size=(p, 2, p, Pp), dtype=np.int64).astype(np.int16)
+    out = torch.full((p * h * 2 * p * h,), 5, dtype=torch.int16, device="cuda")
+    assert L_().nl_ring_from_pairs(p, h, Pp, P(torch.tensor(src, device="cuda")), P(out),
+                                   S()) == 0
+    assert L_().nl_ring_from_pairs(p, h, npr - 1, P(out), P(out), S()) == -1   # Pp < P
+    torch.cuda.synchronize()
+    q = np.zeros((h, h), dtype=np.int64)
+    sgn = np.zeros((h, h), dtype=np.int64)
+    for n_, (i, j) in enumerate(_pairs(h)):
+        q[i, j] = q[j, i] = n_
+        sgn[i, j], sgn[j, i] = 1, -1
+    u = src.view(np.uint16)
+    ref = np.zeros((p, h, 2, p, h), dtype=np.uint16)
+    for i in range(h):
+        for k in range(h):
+            if i == k:
+                continue
+            v = u[:, :, :, q[i, k]]                       # [a][part][c]
+            ref[:, i, :, :, k] = v if sgn[i, k] > 0 else v ^ 0x8000
+    got = out.cpu().numpy().view(np.uint16).reshape(p, h, 2, p, h)
+    np.testing.assert_array_equal(got, ref)
+
+
+@pytest.mark.parametrize("kernel", ["pair", "flat"])
+def test_assemble_image_and_sab_pairs(kernel):
+    """nl_assemble_t2_ext with NL_SAB_PAIRS | NL_IMG_PAIRS: Sab given over the hole pairs gives
+    the assembly of the antisymmetric full Sab bit for bit, and the split image written over the
+    pairs holds the full image's bits at every pair (i < j) and leaves the slots from P on
+    untouched.  Both assembly kernels: the pair kernel (Sij and Spp given) and the flat one."""
+    p, h = 14, 6
+    npr, Pp = h * (h - 1) // 2, 32
+    n = p * p * h * h
+    S0 = rnd((p, p, h, h), 70, 0.1).to(F32)
+    Sab = rnd((p, p, h, h), 71, 0.1).to(F32)
+    Sab = Sab - Sab.transpose(2, 3)
+    red = torch.zeros((p * p, Pp), dtype=F32, device="cuda")
+    for q, (i, j) in enumerate(_pairs(h)):
+        red[:, q] = Sab[:, :, i, j].reshape(-1)
+    if kernel == "pair":
+        Sij, Spp = rnd((p, p, h, h), 72, 0.1).to(F32), rnd((p, h, p, h), 73, 0.1).to(F32)
+    else:
+        Sij = Spp = None
+    v = rnd((p, p, h, h), 74, 0.05)
+    v = v - v.transpose(2, 3)
+    t2, t1n = rnd((p, p, h, h), 75, 0.01), rnd((p, h), 76, 0.01)
+    t2 = t2 - t2.transpose(2, 3)
+    Xpp = rnd((p, p), 77) + 10.0 * torch.eye(p, dtype=F64, device="cuda")
+    Xhh = rnd((h, h), 78) - 10.0 * torch.eye(h, dtype=F64, device="cuda")
+    npart = 2 * int(L_().nl_partials(n)) + 8
+    out = []
+    for sab, nimg, io in ((Sab, 2 * n, 1 | 2), (red, 2 * p * p * Pp, 1 | 2 | 8 | 16)):
+        t2n, H2 = torch.empty_like(t2), torch.empty_like(t2)
+        part = torch.empty(npart, dtype=F64, device="cuda")
+        img = torch.full((nimg,), 3, dtype=torch.int16, device="cuda")
+        assert L_().nl_assemble_t2_ext(p, h, P(S0), P(sab), P(Sij), P(Spp), P(Xpp), P(Xhh),
+                                       None, None, P(t2), P(v), P(t1n), ctypes.c_double(0.3),
+                                       P(t2n), P(H2), P(part), P(img), 2,
+                                       ctypes.c_double(4096.0), io, S()) == 0
+        out.append((t2n, H2, img))
+    # the pair image needs a split image kind
+    assert L_().nl_assemble_t2_ext(p, h, P(S0), P(red), P(Sij), P(Spp), P(Xpp), P(Xhh), None,
+                                   None, P(t2), P(v), P(t1n), ctypes.c_double(0.3), P(t2n),
+                                   None, P(part), P(img), 0, ctypes.c_double(1.0), 1 | 2 | 8,
+                                   S()) == -1
+    torch.cuda.synchronize()
+    for k in range(2):
+        assert torch.equal(out[0][k], out[1][k])
+    full = out[0][2].cpu().numpy().reshape(p, 2, p, h, h)
+    pim = out[1][2].cpu().numpy().reshape(p, 2, p, Pp)
+    ref = np.stack([full[:, :, :, i, j] for i, j in _pairs(h)], 3)
+    np.testing.assert_array_equal(pim[:, :, :, :npr], ref)
+    assert np.all(pim[:, :, :, npr:] == 3)

@@ -287,8 +287,8 @@ def test_split_images_equal_per_product_conversions(prec, tol):
     per product: BF16 to summation order (1e-12: the same roundings); FP16 to the split's
     precision (1e-7: the shared images use the fixed scale 2^12, the per-product conversions
     per-row scales, so the roundings differ), with the shared-image path's accumulators in FP64
-    as the per-product path's; with its FP32 accumulators, and with those and the hole-pair
-    products (the default), to 1e-6."""
+    as the per-product path's; with its FP32 accumulators, with those and the hole-pair
+    products, and with those and t2's image over the hole pairs (the default), to 1e-6."""
     E0, fock, v = planewave_hamiltonian(3, 28, -4.0, 1.0)
     H = engine.Hamiltonian(fock, v, device="cuda")
     fast = ccm._PREC[prec]
@@ -297,14 +297,35 @@ def test_split_images_equal_per_product_conversions(prec, tol):
     assert engine._fast_split(H, slow) is None
     es, ts = _steps(H, slow, 4)
     scale = float(ts.abs().max())
-    for acc32, pairs, tl in ((False, False, tol), (True, False, 1e-6), (True, True, 1e-6)):
-        engine.ACC32, engine.HOLE_PAIRS = acc32, pairs
+    for acc32, pairs, imgp, tl in ((False, False, False, tol), (True, False, False, 1e-6),
+                                   (True, True, False, 1e-6), (True, True, True, 1e-6)):
+        engine.ACC32, engine.HOLE_PAIRS, engine.IMG_PAIRS = acc32, pairs, imgp
         try:
             ef, tf = _steps(H, fast, 4)
         finally:
-            engine.ACC32, engine.HOLE_PAIRS = True, True
+            engine.ACC32, engine.HOLE_PAIRS, engine.IMG_PAIRS = True, True, True
         np.testing.assert_allclose(ef, es, rtol=tl)
         assert float((tf - ts).abs().max()) <= tl * scale
+
+
+def test_pair_image_equals_full_image_with_split_sab(monkeypatch):
+    """Plane-wave L=3, A=28 with the 16-bit products' size floor lowered to 64 (so that the
+    split Sab product runs at p = 80): t2's image over the hole pairs (ring rows expanded from
+    it, A19 and Sab over the pairs) gives the steps of the full image to 1e-7 (A19's FP32
+    accumulation order and the rounding of t2's antisymmetric partner elements differ; a wrong
+    sign or slot moves them by far more)."""
+    monkeypatch.setattr(ops, "LP_MIN_DIM", 64)
+    E0, fock, v = planewave_hamiltonian(3, 28, -4.0, 1.0)
+    H = engine.Hamiltonian(fock, v, device="cuda")
+    fast = ccm._PREC["fp16x3"]
+    assert engine._sab_pairs(H, engine._fast_split(H, fast))
+    out = []
+    for imgp in (False, True):
+        monkeypatch.setattr(engine, "IMG_PAIRS", imgp)
+        out.append(_steps(H, fast, 4))
+    (ef, tf), (ep, tp) = out
+    np.testing.assert_allclose(ep, ef, rtol=1e-7)
+    assert float((tp - tf).abs().max()) <= 1e-7 * float(tf.abs().max())
 
 
 def test_precision_form_L5_A28_meets_1e6():
