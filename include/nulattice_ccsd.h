@@ -162,6 +162,16 @@ int nl_tau_split(int64_t p, int64_t h, const double* t1, const double* t2, doubl
 int nl_tau_split_pairs(int64_t p, int64_t h, const double* t1, const double* t2, void* out,
                        int64_t Pp, int64_t grp, int dtype, double scale, void* stream);
 
+/* FP16 delta of a split image against an anchor image (both FP16 [hi(K) | lo(K)] rows, pitch
+ * ldx): D[r*ldd + k] = FP16((hi + lo) - (hi_a + lo_a)) for k < K (each pair summed in FP32), 0
+ * for K <= k < ldd.  The anchored form of the ring intermediate: W = W(t_anchor) + the
+ * single-rounded product with t2 - t_anchor, whose rounding shrinks with the difference (no
+ * reference counterpart: the precision form has none).  K, ldx, ldd multiples of 8; X, Xa, D
+ * 16-byte aligned. */
+int nl_split_delta16(int64_t rows, int64_t K, int64_t ldx, const void* X, const void* Xa,
+                     void* D, int64_t ldd, void* stream);
+
+
 /* The ring image of t2 from its hole-pair split image (NL_IMG_PAIRS layout, Pp slots per b):
  * row (a,i) of out is [hi((c,k)) | lo((c,k))] of t2[a,c,i,k] (the ring operand of t2Iter's
  * ccDgrams.py:240-303 products, coupled_cluster.py:476-536), each element the pair slot of

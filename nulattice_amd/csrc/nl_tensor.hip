@@ -1257,6 +1257,38 @@ __global__ void sub_kernel(int64_t n, const double* x, const double* y, double* 
     c[t] = x[t] - y[t];
 }
 
+// Delta image of a split operand against an anchor: for rows r < rows and columns k < K of the
+// split images X, Xa (row r at r*ldx: [hi(K) | lo(K)], hi at column k, lo at K + k)
+//   D[r*ldd + k] = FP16((hi + lo) - (hi_a + lo_a))   (each pair summed in FP32),
+// and D[r*ldd + k] = 0 for K <= k < ldd (a zero pad to the product's K step).  Eight columns per
+// thread (16-byte loads and store; K, ldx, ldd multiples of 8, the host checks).
+__global__ __launch_bounds__(NT) void split_delta16_kernel(int64_t rows, int64_t K, int64_t ldx,
+                                                           const uint16_t* X,
+                                                           const uint16_t* Xa, uint16_t* D,
+                                                           int64_t ldd) {
+  const int64_t per_row = ldd / 8, total = rows * per_row;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = t / per_row, k = (t - r * per_row) * 8;
+    uint4 out = make_uint4(0, 0, 0, 0);
+    if (k < K) {
+      const int64_t o = r * ldx + k;
+      const uint4 hi = *(const uint4*)(X + o), lo = *(const uint4*)(X + o + K);
+      const uint4 ha = *(const uint4*)(Xa + o), la = *(const uint4*)(Xa + o + K);
+      const v8h h0 = __builtin_bit_cast(v8h, hi), l0 = __builtin_bit_cast(v8h, lo);
+      const v8h h1 = __builtin_bit_cast(v8h, ha), l1 = __builtin_bit_cast(v8h, la);
+      v8h d;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const float a = (float)h0[e] + (float)l0[e], b = (float)h1[e] + (float)l1[e];
+        d[e] = (_Float16)(a - b);
+      }
+      out = __builtin_bit_cast(uint4, d);
+    }
+    *(uint4*)(D + r * ldd + k) = out;
+  }
+}
+
 inline int grid_for(int64_t n, int cap = 16384) {
   int64_t g = nl_cdiv(n, NT);
   if (g < 1) g = 1;
@@ -1479,6 +1511,21 @@ int nl_tau_split_pairs(int64_t p, int64_t h, const double* t1, const double* t2,
                      (hipStream_t)stream, p * p, (int)p, (int)h, (int)P, (int)Pp, (int)grp,
                      nl_fastdiv(p), nl_fastdiv(Pp), nl_fastdiv(grp), t1, t2, (uint16_t*)out,
                      dtype == 0, scale);
+  NL_LAUNCH_CHECK();
+  return NL_OK;
+}
+
+int nl_split_delta16(int64_t rows, int64_t K, int64_t ldx, const void* X, const void* Xa,
+                     void* D, int64_t ldd, void* stream) {
+  if (rows < 0 || K < 0 || ldx < 2 * K || ldd < K || (K % 8) || (ldx % 8) || (ldd % 8))
+    return NL_EARG;
+  if (rows == 0 || ldd == 0) return NL_OK;
+  if (!X || !Xa || !D || (((uintptr_t)X) & 15) || (((uintptr_t)Xa) & 15) ||
+      (((uintptr_t)D) & 15))
+    return NL_EARG;
+  hipLaunchKernelGGL(split_delta16_kernel, dim3(grid_for(rows * (ldd / 8))), dim3(NT), 0,
+                     (hipStream_t)stream, rows, K, ldx, (const uint16_t*)X, (const uint16_t*)Xa,
+                     (uint16_t*)D, ldd);
   NL_LAUNCH_CHECK();
   return NL_OK;
 }

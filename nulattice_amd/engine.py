@@ -545,9 +545,11 @@ def t2_part(ham, t1, t2, B, prec=DEFAULT_PREC):
         permute_into(t2.permute(0, 2, 1, 3), Tr)
         t2ring = Tr.permute(0, 2, 1, 3)
     with ops.lp_reuse(t2ring):
-        contract("cdkl,bdjl->ckbj", vpphh, t2ring, W, alpha=0.5,
-                 precision=prec.get("ring_W", prec["ring"]), tag="ring_W",
-                 addend=ham.Wc)                                              # A24 + A22
+        if not _ring_w_anchored(ham, B, W):
+            contract("cdkl,bdjl->ckbj", vpphh, t2ring, W, alpha=0.5,
+                     precision=prec.get("ring_W", prec["ring"]), tag="ring_W",
+                     addend=ham.Wc)                                          # A24 + A22
+            _ring_w_anchor(ham, B, W)
         if pl.n:
             _segsum(pl.ret4, h, t1, W, 1.0)                                  # A37
         contract("cjkl,bl->ckbj", ham.v_phhh, t1, W, alpha=-1.0)             # A29
@@ -565,6 +567,82 @@ def t2_part(ham, t1, t2, B, prec=DEFAULT_PREC):
     permute_into(B.Yilaj, B.Zilaj)
     permute_into(B.Ubkij.permute(3, 1, 0, 2), B.Zilaj, alpha=-1.0, beta=1.0)
     contract("ilaj,bl->aibj", B.Zilaj, t1, acc.Spp)
+
+
+# Anchored ring intermediate (fp16x3 form): every RING_ANCHOR-th step runs the split product
+# W_A = Wc + 1/2 v t2_A (two partial products) and keeps W_A and t2_A's ring image; the steps in
+# between add to W_A the single-rounded product with the difference, W = W_A + 1/2 v (t2 - t2_A),
+# one partial product on FP16(t2 - t2_A).  Its rounding is 2^-11 of the difference, which shrinks
+# to nothing as the amplitudes converge, so the fixed point is the split product's.  0: off.
+RING_ANCHOR = 5
+_KSTEP = 64                     # the 256x256 16-bit kernel's K step (a zero-padded K)
+
+
+def _ring_anchor_on(ham, B, W):
+    ring = getattr(B, "t2_ring", None)
+    return (RING_ANCHOR > 0 and ring is not None and ring[1] == ops.FP16
+            and W.dtype == torch.float32 and W.is_contiguous()
+            and ham.p * ham.h >= 256)
+
+
+def _v_ring_image(ham, dt, kp):
+    """v_pphh as [(c,k)][(d,l)] rows (ham.vP), its exact 16-bit image with rows of kp slots
+    (zero from p*h on) and per-row inverse scales; built once per Hamiltonian and type."""
+    hit = getattr(ham, "_v_ring", None)
+    if hit is not None and hit[0] == (dt, kp):
+        return hit[1], hit[2]
+    ph = ham.p * ham.h
+    dev, st = ham.vP.device, ops._stream()
+    img = torch.zeros(ph * kp, dtype=torch.int16, device=dev)
+    iv = torch.empty(ph, dtype=torch.float32, device=dev)
+    _lib.call("nl_convert_rows", dt, 1, ph, 1, ph, (ctypes.c_int64 * 4)(0, ph, 0, 1),
+              ops._p(ham.vP), ops._p(img), kp, ops._p(iv),
+              ctypes.c_float(ops.lp_target(dt, True)), st)
+    ham._v_ring = ((dt, kp), img, iv)
+    return img, iv
+
+
+def _ring_w_anchor(ham, B, W):
+    """After a full W product: keep W (Wc + 1/2 v t2, FP32) and t2's ring image as the anchor."""
+    if not _ring_anchor_on(ham, B, W):
+        B.ring_step = 0
+        return
+    R, _ = B.t2_ring
+    ph = ham.p * ham.h
+    WA = ops.WS.get("ring_WA", ph * ph, dtype=torch.float32, device=W.device)
+    RA = ops.WS.get("ring_RA", 2 * ph * ph, dtype=torch.int16, device=W.device)
+    WA.copy_(W.view(-1))
+    RA.copy_(R[:2 * ph * ph])
+    B.ring_step = 1
+
+
+def _ring_w_anchored(ham, B, W):
+    """W = W_A + 1/2 v (t2 - t2_A) on the anchor's steps in between; False when this step takes
+    the full product (the first step, every RING_ANCHOR-th step, other forms)."""
+    step = getattr(B, "ring_step", 0)
+    if not _ring_anchor_on(ham, B, W) or step == 0 or step >= RING_ANCHOR:
+        return False
+    R, dt = B.t2_ring
+    ph = ham.p * ham.h
+    kp = (ph + _KSTEP - 1) // _KSTEP * _KSTEP
+    dev, st = W.device, ops._stream()
+    WA = ops.WS.get("ring_WA", ph * ph, dtype=torch.float32, device=dev)
+    RA = ops.WS.get("ring_RA", 2 * ph * ph, dtype=torch.int16, device=dev)
+    D = ops.WS.get("ring_dT", ph * kp, dtype=torch.int16, device=dev)
+    _lib.call("nl_split_delta16", ph, ph, 2 * ph, ops._p(R), ops._p(RA), ops._p(D), kp, st)
+    vimg, iv = _v_ring_image(ham, dt, kp)
+    ops.count_flops("lp", 2.0 * ph * ph * kp)
+    ev = ops._timed("ring_W")
+    if ev:
+        ev[0].record()
+    io = (ops.C_F32 | ops.D_F32) << 8
+    _lib.call("nl_gemm_lp", dt | io, ph, ph, kp, ctypes.c_double(0.5 / AMP_SCALE[dt]),
+              ops._p(vimg), kp, 0, ops._p(iv), 0, ops._p(D), kp, 0, None, 0,
+              ctypes.c_double(1.0), ops._p(WA), ops._p(W), ph, 0, 1, 1, None, st)
+    if ev:
+        ev[1].record()
+    B.ring_step = step + 1
+    return True
 
 
 def _sab_bf16(ham, t1, B, Sab):
@@ -920,7 +998,7 @@ def update(ham, t1, t2, B, t1n, t2n, mixing, prec=DEFAULT_PREC, H2out=None, img=
             t2_part(ham, t1, t2, B, prec)
         finally:
             B.t2_16 = None
-            B.t2_split = B.t2_pairs = B.N56r = None
+            B.t2_split = B.t2_pairs = B.N56r = B.t2_ring = None
     assemble(ham, t2, B, t1n, t2n, mixing, H2out, img=img, prec=prec)
 
 
@@ -997,6 +1075,7 @@ def _t2_split_images(ham, t2, dt, B, img=None):
         _lib.call("nl_ring_from_pairs", p, h, Pp, ops._p(NR), ops._p(R), st)
         ops.lp_provide(t2, "acik", ["a", "i"], ["c", "k"], R, 2 * ph, dt, SPLIT, sc)
         B.t2_pairs, B.t2_pairs_dt = NR, dt
+        B.t2_ring = (R, dt)
         return
     kind = 1 + dt
     if img is not None and img.holds(t2, kind):
@@ -1014,6 +1093,7 @@ def _t2_split_images(ham, t2, dt, B, img=None):
     ops.lp_provide(t2, "acik", ["a", "i"], ["c", "k"], R, 2 * ph, dt, SPLIT, sc)
     # the Sab product's t2 columns are transposed from N into its own operand (_sab_split)
     B.t2_split = N
+    B.t2_ring = (R, dt)
 
 
 def assemble(ham, t2, B, t1n, t2n, mixing, H2out=None, img=None, prec=None):

@@ -288,9 +288,12 @@ def test_split_images_equal_per_product_conversions(prec, tol):
     precision (1e-7: the shared images use the fixed scale 2^12, the per-product conversions
     per-row scales, so the roundings differ), with the shared-image path's accumulators in FP64
     as the per-product path's; with its FP32 accumulators, with those and the hole-pair
-    products, and with those and t2's image over the hole pairs (the default), to 1e-6."""
+    products, and with those and t2's image over the hole pairs (the default), to 1e-6.  The
+    ring intermediate runs unanchored here (the anchored form's steps between anchors differ
+    by the rounding of the amplitudes' difference: tests/test_gpu_anchor.py)."""
     E0, fock, v = planewave_hamiltonian(3, 28, -4.0, 1.0)
     H = engine.Hamiltonian(fock, v, device="cuda")
+    anchor, engine.RING_ANCHOR = engine.RING_ANCHOR, 0
     fast = ccm._PREC[prec]
     assert engine._fast_split(H, fast) is not None
     slow = dict(fast, ring_W=prec)               # any sub-family key disables the shared images
@@ -306,6 +309,7 @@ def test_split_images_equal_per_product_conversions(prec, tol):
             engine.ACC32, engine.HOLE_PAIRS, engine.IMG_PAIRS = True, True, True
         np.testing.assert_allclose(ef, es, rtol=tl)
         assert float((tf - ts).abs().max()) <= tl * scale
+    engine.RING_ANCHOR = anchor
 
 
 def test_pair_image_equals_full_image_with_split_sab(monkeypatch):
