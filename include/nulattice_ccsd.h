@@ -163,13 +163,14 @@ int nl_tau_split_pairs(int64_t p, int64_t h, const double* t1, const double* t2,
                        int64_t Pp, int64_t grp, int dtype, double scale, void* stream);
 
 /* FP16 delta of a split image against an anchor image (both FP16 [hi(K) | lo(K)] rows, pitch
- * ldx): D[r*ldd + k] = FP16((hi + lo) - (hi_a + lo_a)) for k < K (each pair summed in FP32), 0
- * for K <= k < ldd.  The anchored form of the ring intermediate: W = W(t_anchor) + the
- * single-rounded product with t2 - t_anchor, whose rounding shrinks with the difference (no
- * reference counterpart: the precision form has none).  K, ldx, ldd multiples of 8; X, Xa, D
- * 16-byte aligned. */
+ * ldx): D[r*ldd + off + k] = FP16((hi + lo) - (hi_a + lo_a)) for k < K (each pair summed in FP32),
+ * 0 for K <= k < kp; with with_hi = 1 also D[r*ldd + k] = hi (0 from K to kp) and off = kp, else
+ * off = 0.  The anchored ring products of the precision form: W = W(t_A) + the single-rounded
+ * product with t2 - t_A, Spp = Spp(t_A) + [t2_hi | t2 - t_A] x [W - W_A | W_A], whose roundings
+ * shrink with the difference (no reference counterpart: the precision form has none).  K, kp,
+ * ldx, ldd multiples of 8; X, Xa, D 16-byte aligned. */
 int nl_split_delta16(int64_t rows, int64_t K, int64_t ldx, const void* X, const void* Xa,
-                     void* D, int64_t ldd, void* stream);
+                     void* D, int64_t ldd, int64_t kp, int with_hi, void* stream);
 
 
 /* The ring image of t2 from its hole-pair split image (NL_IMG_PAIRS layout, Pp slots per b):
@@ -215,6 +216,16 @@ int nl_convert_split(int dtype, int64_t R1, int64_t R2, int64_t C1, int64_t C2,
                      const int64_t* strides4, const void* in, void* out, int64_t ldo,
                      int64_t kpad, int nblk, int lomask, float* iscale, float target,
                      double scale, void* stream);
+
+/* nl_convert_split of in - sub (sub: the source's element type and layout; each difference
+ * exact in FP64 for FP32 operands), with the per-row inverse scales either computed (iscale_in
+ * null: written to iscale as nl_convert_split does) or given (iscale_in: powers of two, no amax
+ * pass; iscale may be null): the difference of the ring intermediate from its anchor, scaled
+ * like the anchor's own image, in the anchored precision form (no reference counterpart). */
+int nl_convert_split_delta(int dtype, int64_t R1, int64_t R2, int64_t C1, int64_t C2,
+                           const int64_t* strides4, const void* in, const void* sub, void* out,
+                           int64_t ldo, int64_t kpad, int nblk, int lomask, float* iscale,
+                           const float* iscale_in, float target, double scale, void* stream);
 
 /* out[b][n][k] (BF16) = src[idx[b*kb + k]*lds + n] for n < ncol, k < kb (0 where idx < 0), and 0
  * for kb <= k < ldk (ldk a multiple of 8): the row gather of the packed pair-block products

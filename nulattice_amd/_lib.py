@@ -33,6 +33,8 @@ SIGNATURES = {
                                _i64, _vp, _i64, _vp, _i64, _vp, _i64, _vp]),
     "nl_convert_split": (_i32, [_i32, _i64, _i64, _i64, _i64, _vp, _vp, _vp, _i64, _i64, _i32,
                                 _i32, _vp, _flt, _dbl, _vp]),
+    "nl_convert_split_delta": (_i32, [_i32, _i64, _i64, _i64, _i64, _vp, _vp, _vp, _vp, _i64,
+                                      _i64, _i32, _i32, _vp, _vp, _flt, _dbl, _vp]),
     "nl_gather_convert": (_i32, [_i64, _i64, _i64, _vp, _vp, _i64, _vp, _i64, _vp]),
     "nl_gather_rows16": (_i32, [_i64, _i64, _i64, _vp, _vp, _i64, _vp, _i64, _vp]),
     "nl_tau16": (_i32, [_i64, _i64, _vp, _vp, _vp, _vp]),
@@ -54,7 +56,7 @@ SIGNATURES = {
     "nl_scatter_add_rows": (_i32, [_i64, _i64, _vp, _vp, _dbl, _vp, _i64, _vp]),
     "nl_tau_split_pairs": (_i32, [_i64, _i64, _vp, _vp, _vp, _i64, _i64, _i32, _dbl, _vp]),
     "nl_ring_from_pairs": (_i32, [_i64, _i64, _i64, _vp, _vp, _vp]),
-    "nl_split_delta16": (_i32, [_i64, _i64, _i64, _vp, _vp, _vp, _i64, _vp]),
+    "nl_split_delta16": (_i32, [_i64, _i64, _i64, _vp, _vp, _vp, _i64, _i64, _i32, _vp]),
     "nl_hole_pairs": (_i32, [_i64, _i64, _i64, _i64, _vp, _vp, _i32, _vp]),
     "nl_concat_rows": (_i32, [_i64, _i64, _vp, _i64, _i64, _vp, _i64, _vp, _i64, _vp]),
     "nl_update_t1": (_i32, [_i64, _i64, _vp, _vp, _vp, _vp, _dbl, _vp, _vp]),

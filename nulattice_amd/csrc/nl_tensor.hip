@@ -187,8 +187,10 @@ struct SplitArgs {
   FastDiv fr2, fc2;
   int64_t s[4];
   AnyIn in;                       // FP64 or FP32 source
+  AnyIn sub;                      // (SUB kernels) subtrahend with the source's type and layout
   uint16_t* out;
   float* iscale;                  // per-row inverse scale (null: unscaled)
+  const float* given;             // per-row inverse scales to use (no amax pass), or null
   float target;
   int nblk, lomask, bf16;         // block q holds the lo part when lomask bit q is set, else hi
   double scale;                   // fixed factor applied before the split (with the row scale)
@@ -211,18 +213,29 @@ __device__ inline void split16(double xs, int bf16, uint16_t& hi, uint16_t& lo) 
   lo = bf16 ? to_bf16_bits(r) : to_f16_bits(r);
 }
 
+// Element o of the value a split converts: the source, less the subtrahend (SUB; exact in FP64
+// for an FP32 pair).
+template <bool SUB>
+__device__ inline double split_value(const AnyIn& base, const AnyIn& sbase, int64_t o) {
+  return SUB ? base.get(o) - sbase.get(o) : base.get(o);
+}
+
 // One workgroup per output row: amax pass (when scaled), then the split written to every block.
+template <bool SUB>
 __global__ __launch_bounds__(NT) void split_rows_kernel(SplitArgs a) {
   __shared__ float red[NT / 64];
   const int64_t r = blockIdx.x;
   const int64_t r1 = fdiv((int)r, a.fr2), r2 = r - r1 * a.R2;
   const AnyIn base = a.in.at(r1 * a.s[0] + r2 * a.s[1]);
+  const AnyIn sbase = SUB ? a.sub.at(r1 * a.s[0] + r2 * a.s[1]) : base;
   float sc = 1.f;
-  if (a.iscale) {
+  if (a.given) {
+    sc = 1.f / a.given[r];
+  } else if (a.iscale) {
     float m = 0.f;
     for (int64_t c = threadIdx.x; c < a.C; c += NT) {
       const int64_t c1 = fdiv((int)c, a.fc2), c2 = c - c1 * a.C2;
-      m = fmaxf(m, (float)fabs(base.get(c1 * a.s[2] + c2 * a.s[3])));
+      m = fmaxf(m, (float)fabs(split_value<SUB>(base, sbase, c1 * a.s[2] + c2 * a.s[3])));
     }
     m = wave_max(m);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
@@ -236,7 +249,8 @@ __global__ __launch_bounds__(NT) void split_rows_kernel(SplitArgs a) {
     uint16_t hi = 0, lo = 0;
     if (c < a.C) {
       const int64_t c1 = fdiv((int)c, a.fc2), c2 = c - c1 * a.C2;
-      split16(base.get(c1 * a.s[2] + c2 * a.s[3]) * ((double)sc * a.scale), a.bf16, hi, lo);
+      split16(split_value<SUB>(base, sbase, c1 * a.s[2] + c2 * a.s[3]) * ((double)sc * a.scale),
+              a.bf16, hi, lo);
     }
     for (int q = 0; q < a.nblk; ++q) o[q * a.kpad + c] = ((a.lomask >> q) & 1) ? lo : hi;
   }
@@ -244,6 +258,7 @@ __global__ __launch_bounds__(NT) void split_rows_kernel(SplitArgs a) {
 
 // Transposing split (output rows are the unit-stride direction of the source): 64x64 tiles
 // through LDS, per-row scales from amax bits gathered by amax_split_kernel.
+template <bool SUB>
 __global__ __launch_bounds__(NT) void amax_split_kernel(SplitArgs a, int64_t R, unsigned* bits) {
   __shared__ float red[4][64];
   const int tr = threadIdx.x & 63, tg = threadIdx.x >> 6;
@@ -252,11 +267,12 @@ __global__ __launch_bounds__(NT) void amax_split_kernel(SplitArgs a, int64_t R, 
   if (r < R) {
     const int64_t r1 = fdiv((int)r, a.fr2), r2 = r - r1 * a.R2;
     const AnyIn base = a.in.at(r1 * a.s[0] + r2 * a.s[1]);
+    const AnyIn sbase = SUB ? a.sub.at(r1 * a.s[0] + r2 * a.s[1]) : base;
     for (int cc = tg; cc < 64; cc += 4) {
       const int64_t c = c0 + cc;
       if (c < a.C) {
         const int64_t c1 = fdiv((int)c, a.fc2), c2 = c - c1 * a.C2;
-        m = fmaxf(m, (float)fabs(base.get(c1 * a.s[2] + c2 * a.s[3])));
+        m = fmaxf(m, (float)fabs(split_value<SUB>(base, sbase, c1 * a.s[2] + c2 * a.s[3])));
       }
     }
   }
@@ -268,21 +284,25 @@ __global__ __launch_bounds__(NT) void amax_split_kernel(SplitArgs a, int64_t R, 
   }
 }
 
+template <bool SUB>
 __global__ __launch_bounds__(NT) void split_tiles_kernel(SplitArgs a, int64_t R,
                                                          const unsigned* bits) {
   __shared__ uint16_t thi[64][66], tlo[64][66];
   const int tr = threadIdx.x & 63, tg = threadIdx.x >> 6;
   const int64_t r0 = blockIdx.x * 64, c0 = blockIdx.y * 64, r = r0 + tr;
   if (r < R) {
-    const float sc = bits ? row_scale(__uint_as_float(bits[r]), a.target) : 1.f;
+    const float sc = a.given ? 1.f / a.given[r]
+                     : bits ? row_scale(__uint_as_float(bits[r]), a.target) : 1.f;
     const int64_t r1 = fdiv((int)r, a.fr2), r2 = r - r1 * a.R2;
     const AnyIn base = a.in.at(r1 * a.s[0] + r2 * a.s[1]);
+    const AnyIn sbase = SUB ? a.sub.at(r1 * a.s[0] + r2 * a.s[1]) : base;
     for (int cc = tg; cc < 64; cc += 4) {
       const int64_t c = c0 + cc;
       uint16_t hi = 0, lo = 0;
       if (c < a.C) {
         const int64_t c1 = fdiv((int)c, a.fc2), c2 = c - c1 * a.C2;
-        split16(base.get(c1 * a.s[2] + c2 * a.s[3]) * ((double)sc * a.scale), a.bf16, hi, lo);
+        split16(split_value<SUB>(base, sbase, c1 * a.s[2] + c2 * a.s[3]) * ((double)sc * a.scale),
+                a.bf16, hi, lo);
       }
       thi[tr][cc] = hi;
       tlo[tr][cc] = lo;
@@ -1259,18 +1279,21 @@ __global__ void sub_kernel(int64_t n, const double* x, const double* y, double* 
 
 // Delta image of a split operand against an anchor: for rows r < rows and columns k < K of the
 // split images X, Xa (row r at r*ldx: [hi(K) | lo(K)], hi at column k, lo at K + k)
-//   D[r*ldd + k] = FP16((hi + lo) - (hi_a + lo_a))   (each pair summed in FP32),
-// and D[r*ldd + k] = 0 for K <= k < ldd (a zero pad to the product's K step).  Eight columns per
-// thread (16-byte loads and store; K, ldx, ldd multiples of 8, the host checks).
+//   D[r*ldd + off + k] = FP16((hi + lo) - (hi_a + lo_a))   (each pair summed in FP32),
+// zero for K <= k < kp (a pad to the product's K step); off = kp when D also takes X's hi part
+// (D[r*ldd + k] = hi, zero from K to kp: the [t2_hi | t2 - t2_A] operand of a K-concatenated
+// product), else 0.  Eight columns per thread (16-byte loads and stores; K, kp, ldx, ldd
+// multiples of 8, the host checks).
+template <bool HI>
 __global__ __launch_bounds__(NT) void split_delta16_kernel(int64_t rows, int64_t K, int64_t ldx,
                                                            const uint16_t* X,
                                                            const uint16_t* Xa, uint16_t* D,
-                                                           int64_t ldd) {
-  const int64_t per_row = ldd / 8, total = rows * per_row;
+                                                           int64_t ldd, int64_t kp) {
+  const int64_t per_row = kp / 8, total = rows * per_row;
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
        t += (int64_t)gridDim.x * blockDim.x) {
     const int64_t r = t / per_row, k = (t - r * per_row) * 8;
-    uint4 out = make_uint4(0, 0, 0, 0);
+    uint4 out = make_uint4(0, 0, 0, 0), hic = make_uint4(0, 0, 0, 0);
     if (k < K) {
       const int64_t o = r * ldx + k;
       const uint4 hi = *(const uint4*)(X + o), lo = *(const uint4*)(X + o + K);
@@ -1284,8 +1307,10 @@ __global__ __launch_bounds__(NT) void split_delta16_kernel(int64_t rows, int64_t
         d[e] = (_Float16)(a - b);
       }
       out = __builtin_bit_cast(uint4, d);
+      hic = hi;
     }
-    *(uint4*)(D + r * ldd + k) = out;
+    if (HI) *(uint4*)(D + r * ldd + k) = hic;
+    *(uint4*)(D + r * ldd + (HI ? kp : 0) + k) = out;
   }
 }
 
@@ -1373,15 +1398,16 @@ int nl_convert_rows(int dtype, int64_t R1, int64_t R2, int64_t C1, int64_t C2,
   return NL_OK;
 }
 
-int nl_convert_split(int dtype, int64_t R1, int64_t R2, int64_t C1, int64_t C2,
-                     const int64_t* strides4, const void* in, void* out, int64_t ldo,
-                     int64_t kpad, int nblk, int lomask, float* iscale, float target,
-                     double scale, void* stream) {
+static int convert_split_impl(int dtype, int64_t R1, int64_t R2, int64_t C1, int64_t C2,
+                              const int64_t* strides4, const void* in, const void* sub,
+                              void* out, int64_t ldo, int64_t kpad, int nblk, int lomask,
+                              float* iscale, float target, double scale, void* stream,
+                              const float* given = nullptr) {
   // dtype: 16-bit type (bits 0-7) | NL_D_F32 << 8 for an FP32 source (else FP64)
   const bool in32 = (dtype >> 8) == NL_D_F32;
   if (dtype >> 8 && !in32) return NL_EARG;
   dtype &= 0xff;
-  const bool scaled = target != 0.f;
+  const bool scaled = target != 0.f && !given;
   if ((dtype != 0 && dtype != 1) || !strides4 || !in || !out || (scaled && !iscale))
     return NL_EARG;
   if (nblk < 1 || nblk > 4 || lomask < 0 || lomask >= (1 << nblk)) return NL_EARG;
@@ -1392,7 +1418,8 @@ int nl_convert_split(int dtype, int64_t R1, int64_t R2, int64_t C1, int64_t C2,
   a.R2 = R2; a.C = C1 * C2; a.C2 = C2; a.ldo = ldo; a.kpad = kpad;
   a.fr2 = nl_fastdiv(R2); a.fc2 = nl_fastdiv(C2);
   for (int d = 0; d < 4; ++d) a.s[d] = strides4[d];
-  a.in = AnyIn{in, in32}; a.out = (uint16_t*)out; a.iscale = scaled ? iscale : nullptr;
+  a.in = AnyIn{in, in32}; a.sub = AnyIn{sub, in32};
+  a.out = (uint16_t*)out; a.iscale = scaled ? iscale : nullptr; a.given = given;
   a.target = target;
   a.nblk = nblk; a.lomask = lomask; a.bf16 = dtype == 0; a.scale = scale;
   hipStream_t st = (hipStream_t)stream;
@@ -1403,11 +1430,17 @@ int nl_convert_split(int dtype, int64_t R1, int64_t R2, int64_t C1, int64_t C2,
       hipLaunchKernelGGL(zero_u32_kernel, dim3(grid_for(R)), dim3(NT), 0, st, R, bits);
       NL_LAUNCH_CHECK();
       dim3 g1((unsigned)nl_cdiv(R, 64), (unsigned)nl_cdiv(a.C, 64));
-      hipLaunchKernelGGL(amax_split_kernel, g1, dim3(NT), 0, st, a, R, bits);
+      if (sub) hipLaunchKernelGGL(amax_split_kernel<true>, g1, dim3(NT), 0, st, a, R, bits);
+      else hipLaunchKernelGGL(amax_split_kernel<false>, g1, dim3(NT), 0, st, a, R, bits);
       NL_LAUNCH_CHECK();
     }
     dim3 g2((unsigned)nl_cdiv(R, 64), (unsigned)nl_cdiv(kpad, 64));
-    hipLaunchKernelGGL(split_tiles_kernel, g2, dim3(NT), 0, st, a, R, (const unsigned*)bits);
+    if (sub)
+      hipLaunchKernelGGL(split_tiles_kernel<true>, g2, dim3(NT), 0, st, a, R,
+                         (const unsigned*)bits);
+    else
+      hipLaunchKernelGGL(split_tiles_kernel<false>, g2, dim3(NT), 0, st, a, R,
+                         (const unsigned*)bits);
     NL_LAUNCH_CHECK();
     if (scaled) {
       hipLaunchKernelGGL(bits_to_iscale_kernel, dim3(grid_for(R)), dim3(NT), 0, st, R, target,
@@ -1416,9 +1449,27 @@ int nl_convert_split(int dtype, int64_t R1, int64_t R2, int64_t C1, int64_t C2,
     }
     return NL_OK;
   }
-  hipLaunchKernelGGL(split_rows_kernel, dim3((unsigned)R), dim3(NT), 0, st, a);
+  if (sub) hipLaunchKernelGGL(split_rows_kernel<true>, dim3((unsigned)R), dim3(NT), 0, st, a);
+  else hipLaunchKernelGGL(split_rows_kernel<false>, dim3((unsigned)R), dim3(NT), 0, st, a);
   NL_LAUNCH_CHECK();
   return NL_OK;
+}
+
+int nl_convert_split(int dtype, int64_t R1, int64_t R2, int64_t C1, int64_t C2,
+                     const int64_t* strides4, const void* in, void* out, int64_t ldo,
+                     int64_t kpad, int nblk, int lomask, float* iscale, float target,
+                     double scale, void* stream) {
+  return convert_split_impl(dtype, R1, R2, C1, C2, strides4, in, nullptr, out, ldo, kpad, nblk,
+                            lomask, iscale, target, scale, stream);
+}
+
+int nl_convert_split_delta(int dtype, int64_t R1, int64_t R2, int64_t C1, int64_t C2,
+                           const int64_t* strides4, const void* in, const void* sub, void* out,
+                           int64_t ldo, int64_t kpad, int nblk, int lomask, float* iscale,
+                           const float* iscale_in, float target, double scale, void* stream) {
+  if (!sub) return NL_EARG;
+  return convert_split_impl(dtype, R1, R2, C1, C2, strides4, in, sub, out, ldo, kpad, nblk,
+                            lomask, iscale, target, scale, stream, iscale_in);
 }
 
 int nl_gather_convert(int64_t nb, int64_t kb, int64_t ncol, const int64_t* idx, const double* src,
@@ -1516,16 +1567,22 @@ int nl_tau_split_pairs(int64_t p, int64_t h, const double* t1, const double* t2,
 }
 
 int nl_split_delta16(int64_t rows, int64_t K, int64_t ldx, const void* X, const void* Xa,
-                     void* D, int64_t ldd, void* stream) {
-  if (rows < 0 || K < 0 || ldx < 2 * K || ldd < K || (K % 8) || (ldx % 8) || (ldd % 8))
+                     void* D, int64_t ldd, int64_t kp, int with_hi, void* stream) {
+  if (rows < 0 || K < 0 || kp < K || ldx < 2 * K || ldd < (with_hi ? 2 : 1) * kp || (K % 8) ||
+      (kp % 8) || (ldx % 8) || (ldd % 8) || (with_hi != 0 && with_hi != 1))
     return NL_EARG;
-  if (rows == 0 || ldd == 0) return NL_OK;
+  if (rows == 0 || kp == 0) return NL_OK;
   if (!X || !Xa || !D || (((uintptr_t)X) & 15) || (((uintptr_t)Xa) & 15) ||
       (((uintptr_t)D) & 15))
     return NL_EARG;
-  hipLaunchKernelGGL(split_delta16_kernel, dim3(grid_for(rows * (ldd / 8))), dim3(NT), 0,
-                     (hipStream_t)stream, rows, K, ldx, (const uint16_t*)X, (const uint16_t*)Xa,
-                     (uint16_t*)D, ldd);
+  const dim3 grid(grid_for(rows * (kp / 8)));
+  hipStream_t st = (hipStream_t)stream;
+  if (with_hi)
+    hipLaunchKernelGGL(split_delta16_kernel<true>, grid, dim3(NT), 0, st, rows, K, ldx,
+                       (const uint16_t*)X, (const uint16_t*)Xa, (uint16_t*)D, ldd, kp);
+  else
+    hipLaunchKernelGGL(split_delta16_kernel<false>, grid, dim3(NT), 0, st, rows, K, ldx,
+                       (const uint16_t*)X, (const uint16_t*)Xa, (uint16_t*)D, ldd, kp);
   NL_LAUNCH_CHECK();
   return NL_OK;
 }

@@ -553,8 +553,10 @@ def t2_part(ham, t1, t2, B, prec=DEFAULT_PREC):
         if pl.n:
             _segsum(pl.ret4, h, t1, W, 1.0)                                  # A37
         contract("cjkl,bl->ckbj", ham.v_phhh, t1, W, alpha=-1.0)             # A29
-        contract("acik,ckbj->aibj", t2ring, W, acc.Spp, beta=0.0,
-                 precision=prec.get("ring_S", prec["ring"]), tag="ring")
+        if not _ring_s_anchored(ham, B, W, acc.Spp):
+            contract("acik,ckbj->aibj", t2ring, W, acc.Spp, beta=0.0,
+                     precision=prec.get("ring_S", prec["ring"]), tag="ring")
+            _ring_s_anchor(ham, B, W, acc.Spp)
     if split is not None:
         # t2[a,d,k,j] = -t2[a,d,j,k]: the Y product reads the ring image (rows (a,j), columns
         # (d,k)) with the sign in alpha, so it needs no image of t2 of its own
@@ -569,11 +571,14 @@ def t2_part(ham, t1, t2, B, prec=DEFAULT_PREC):
     contract("ilaj,bl->aibj", B.Zilaj, t1, acc.Spp)
 
 
-# Anchored ring intermediate (fp16x3 form): every RING_ANCHOR-th step runs the split product
-# W_A = Wc + 1/2 v t2_A (two partial products) and keeps W_A and t2_A's ring image; the steps in
-# between add to W_A the single-rounded product with the difference, W = W_A + 1/2 v (t2 - t2_A),
-# one partial product on FP16(t2 - t2_A).  Its rounding is 2^-11 of the difference, which shrinks
-# to nothing as the amplitudes converge, so the fixed point is the split product's.  0: off.
+# Anchored ring products (fp16x3 form): every RING_ANCHOR-th step runs the split products
+# W_A = Wc + 1/2 v t2_A (two partial products) and Spp_A = t2_A W_A (three) and keeps them with
+# t2_A's ring image, W_A and W_A's FP16 image; the steps in between form W = W_A + 1/2 v (t2 -
+# t2_A) (one partial product on FP16(t2 - t2_A)) and Spp = Spp_A + t2 (W - W_A) + (t2 - t2_A) W_A
+# (one single-rounded product over the concatenated K: [t2_hi | FP16(t2 - t2_A)] against
+# [FP16(W - W_A) | FP16(W_A)], both W halves on W_A's per-row scales).  Each rounding is 2^-11 of
+# a difference from the anchor, which shrinks to nothing as the amplitudes converge, so the fixed
+# point is the split products'.  0: off.
 RING_ANCHOR = 5
 _KSTEP = 64                     # the 256x256 16-bit kernel's K step (a zero-padded K)
 
@@ -583,6 +588,11 @@ def _ring_anchor_on(ham, B, W):
     return (RING_ANCHOR > 0 and ring is not None and ring[1] == ops.FP16
             and W.dtype == torch.float32 and W.is_contiguous()
             and ham.p * ham.h >= 256)
+
+
+def _ring_kp(ham):
+    ph = ham.p * ham.h
+    return ph, (ph + _KSTEP - 1) // _KSTEP * _KSTEP
 
 
 def _v_ring_image(ham, dt, kp):
@@ -606,6 +616,7 @@ def _ring_w_anchor(ham, B, W):
     """After a full W product: keep W (Wc + 1/2 v t2, FP32) and t2's ring image as the anchor."""
     if not _ring_anchor_on(ham, B, W):
         B.ring_step = 0
+        B.ring_mode = None
         return
     R, _ = B.t2_ring
     ph = ham.p * ham.h
@@ -614,22 +625,26 @@ def _ring_w_anchor(ham, B, W):
     WA.copy_(W.view(-1))
     RA.copy_(R[:2 * ph * ph])
     B.ring_step = 1
+    B.ring_mode = "anchor"
 
 
 def _ring_w_anchored(ham, B, W):
     """W = W_A + 1/2 v (t2 - t2_A) on the anchor's steps in between; False when this step takes
-    the full product (the first step, every RING_ANCHOR-th step, other forms)."""
+    the full product (the first step, every RING_ANCHOR-th step, other forms).  Also writes the
+    Spp delta product's left operand [t2_hi | FP16(t2 - t2_A)] (rows (a,i), 2 kp columns), whose
+    second half is this product's right operand."""
     step = getattr(B, "ring_step", 0)
+    B.ring_mode = None
     if not _ring_anchor_on(ham, B, W) or step == 0 or step >= RING_ANCHOR:
         return False
     R, dt = B.t2_ring
-    ph = ham.p * ham.h
-    kp = (ph + _KSTEP - 1) // _KSTEP * _KSTEP
+    ph, kp = _ring_kp(ham)
     dev, st = W.device, ops._stream()
     WA = ops.WS.get("ring_WA", ph * ph, dtype=torch.float32, device=dev)
     RA = ops.WS.get("ring_RA", 2 * ph * ph, dtype=torch.int16, device=dev)
-    D = ops.WS.get("ring_dT", ph * kp, dtype=torch.int16, device=dev)
-    _lib.call("nl_split_delta16", ph, ph, 2 * ph, ops._p(R), ops._p(RA), ops._p(D), kp, st)
+    TD = ops.WS.get("ring_TD", ph * 2 * kp, dtype=torch.int16, device=dev)
+    _lib.call("nl_split_delta16", ph, ph, 2 * ph, ops._p(R), ops._p(RA), ops._p(TD), 2 * kp, kp,
+              1, st)
     vimg, iv = _v_ring_image(ham, dt, kp)
     ops.count_flops("lp", 2.0 * ph * ph * kp)
     ev = ops._timed("ring_W")
@@ -637,11 +652,76 @@ def _ring_w_anchored(ham, B, W):
         ev[0].record()
     io = (ops.C_F32 | ops.D_F32) << 8
     _lib.call("nl_gemm_lp", dt | io, ph, ph, kp, ctypes.c_double(0.5 / AMP_SCALE[dt]),
-              ops._p(vimg), kp, 0, ops._p(iv), 0, ops._p(D), kp, 0, None, 0,
+              ops._p(vimg), kp, 0, ops._p(iv), 0, ops._off16(TD, kp), 2 * kp, 0, None, 0,
               ctypes.c_double(1.0), ops._p(WA), ops._p(W), ph, 0, 1, 1, None, st)
     if ev:
         ev[1].record()
     B.ring_step = step + 1
+    B.ring_mode = "delta"
+    return True
+
+
+# W as rows (b,j) of columns (c,k): the [c][k][b][j] FP32 buffer seen transposed
+def _w_rows(ham):
+    p, h = ham.p, ham.h
+    return p, h, p, h, (ctypes.c_int64 * 4)(h, 1, p * h * h, p * h)
+
+
+def _ring_s_anchor(ham, B, W, Spp):
+    """After a full Spp product on an anchor step: keep Spp (before its T1 tails), W and W's
+    single-rounded FP16 image (rows (b,j), K padded to the product's step, per-row scales) as the
+    second half of the delta product's right operand."""
+    if getattr(B, "ring_mode", None) != "anchor" or Spp.dtype != torch.float32:
+        B.ring_s_ok = False
+        return
+    ph, kp = _ring_kp(ham)
+    dev, st = W.device, ops._stream()
+    SA = ops.WS.get("ring_SA", ph * ph, dtype=torch.float32, device=dev)
+    WF = ops.WS.get("ring_WF", ph * ph, dtype=torch.float32, device=dev)
+    WB = ops.WS.get("ring_WB", ph * 2 * kp, dtype=torch.int16, device=dev)
+    iw = ops.WS.get("ring_WB_s", ph, dtype=torch.float32, device=dev)
+    SA.copy_(Spp.view(-1))
+    WF.copy_(W.view(-1))
+    R1, R2, C1, C2, s4 = _w_rows(ham)
+    _lib.call("nl_convert_split", ops.FP16 | (ops.D_F32 << 8), R1, R2, C1, C2, s4, ops._p(W),
+              ops._off16(WB, kp), 2 * kp, kp, 1, 0, ops._p(iw),
+              ctypes.c_float(ops.lp_target(ops.FP16, False)), ctypes.c_double(1.0), st)
+    B.ring_s_ok = True
+
+
+def _ring_s_anchored(ham, B, W, Spp):
+    """Spp = Spp_A + t2 (W - W_A) + (t2 - t2_A) W_A on the steps between anchors, as one
+    single-rounded product over the concatenated K, [t2_hi | FP16(t2 - t2_A)] against
+    [FP16(W - W_A) | FP16(W_A)] (the W halves on W_A's per-row scales: W - W_A is small against
+    W_A, so the shared scale cannot overflow and only narrows a rounding already far below
+    W_A's): the roundings scale with the differences from the anchor.  False when this step
+    takes the full product."""
+    if getattr(B, "ring_mode", None) != "delta" or not getattr(B, "ring_s_ok", False):
+        return False
+    if Spp.dtype != torch.float32 or not Spp.is_contiguous():
+        return False
+    _, dt = B.t2_ring
+    ph, kp = _ring_kp(ham)
+    dev, st = W.device, ops._stream()
+    SA = ops.WS.get("ring_SA", ph * ph, dtype=torch.float32, device=dev)
+    WF = ops.WS.get("ring_WF", ph * ph, dtype=torch.float32, device=dev)
+    WB = ops.WS.get("ring_WB", ph * 2 * kp, dtype=torch.int16, device=dev)
+    iw = ops.WS.get("ring_WB_s", ph, dtype=torch.float32, device=dev)
+    TD = ops.WS.get("ring_TD", ph * 2 * kp, dtype=torch.int16, device=dev)
+    R1, R2, C1, C2, s4 = _w_rows(ham)
+    _lib.call("nl_convert_split_delta", dt | (ops.D_F32 << 8), R1, R2, C1, C2, s4, ops._p(W),
+              ops._p(WF), ops._p(WB), 2 * kp, kp, 1, 0, None, ops._p(iw),
+              ctypes.c_float(ops.lp_target(dt, False)), ctypes.c_double(1.0), st)
+    io = (ops.C_F32 | ops.D_F32) << 8
+    ops.count_flops("lp", 2.0 * ph * ph * 2 * kp)
+    ev = ops._timed("ring_delta")
+    if ev:
+        ev[0].record()
+    _lib.call("nl_gemm_lp", dt | io, ph, ph, 2 * kp, ctypes.c_double(1.0 / AMP_SCALE[dt]),
+              ops._p(TD), 2 * kp, 0, None, 0, ops._p(WB), 2 * kp, 0, ops._p(iw), 0,
+              ctypes.c_double(1.0), ops._p(SA), ops._p(Spp), ph, 0, 1, 1, None, st)
+    if ev:
+        ev[1].record()
     return True
 
 
