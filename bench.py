@@ -501,7 +501,8 @@ def main():
         print(json.dumps({"form": "iter", "L": a.L, "A": a.A, "prec": a.prec,
                           "ms_per_step": slp * 1e3, "fp64_ms_per_step": s64 * 1e3,
                           "speedup_vs_fp64": s64 / slp,
-                          "ring_ms": (klp.get("ring") or 0) * 1e3}))
+                          "ring_ms": (klp.get("ring") or 0) * 1e3,
+                          "ring_delta_ms": (klp.get("ring_delta") or 0) * 1e3}))
         return
     # the single-rounding 16-bit forms, for reference (their fixed points miss 1e-6)
     singles = {}
@@ -562,28 +563,57 @@ def main():
     print(f"# sectors: {sector}", file=sys.stderr, flush=True)
 
     # roofline of the dominant kernel: the T2 ring GEMM Spp = T[(a,i),(c,k)] W[(c,k),(b,j)],
-    # timed by events on its launch stream inside the timed loop.  Its matrix-core work is
-    # 2 (ph)^3 FLOP per partial product: 3 for split operands (hi.hi + hi.lo + lo.hi), else 1.
+    # timed by events on its launch stream inside the timed loop; its algorithmic work is
+    # 2 (ph)^3 FLOP per launch.  The fp16x3 form computes it as split products (3 partial
+    # products) every engine.RING_ANCHOR-th step and as one single-rounded product over the
+    # concatenated K = 2 kp, [t2_hi | t2 - t2_A] x [W - W_A | W_A], on the steps in between: that
+    # launch (tag "ring_delta") runs on RING_ANCHOR - 1 of every RING_ANCHOR steps and is the
+    # dominant kernel.
+    ph = p * h
+    kp = (ph + 63) // 64 * 64
+    ring_flop = 2.0 * ph ** 3
+    full_s, delta_s = klp.get("ring"), klp.get("ring_delta")
     nprod = 3 if a.prec.endswith("x3") else 1
-    ring_flop = 2.0 * (p * h) ** 3
-    ring_s = klp.get("ring")
     peak = 2.5e15
     roof = None
-    if ring_s:
-        ach = ring_flop / ring_s                    # algorithmic: 2 (ph)^3 per launch
-        traffic = (profile_value("ccsd_L5_A28_ring_hbm_traffic.md", "ring_spp_traffic_bytes")
-                   if (a.L, a.A, a.prec) == (5, 28, "fp16x3") else None)
+    fp65 = (a.L, a.A, a.prec) == (5, 28, "fp16x3")
+    if delta_s:
+        ach = ring_flop / delta_s
+        exe = 2.0 * ph * ph * 2 * kp
+        traffic = (profile_value("ccsd_L5_A28_ring_hbm_traffic.md", "ring_delta_traffic_bytes")
+                   if fp65 else None)
         roof = {"bound": "mfma", "achieved": ach / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
                 "frac": ach / peak, "traffic": traffic,
                 "traffic_source": "profiles/ccsd_L5_A28_ring_hbm_traffic.md (calibrated "
                                   "FETCH_SIZE + WRITE_SIZE of this launch)" if traffic else None,
+                "executed_achieved": exe / delta_s / 1e12, "executed_frac": exe / delta_s / peak,
+                "kernel": "ring GEMM Spp between anchors (one FP16 product over K = 2 kp = "
+                          f"{2 * kp}: [t2_hi | t2 - t2_A] x [W - W_A | W_A])",
+                "kernel_ms": delta_s * 1e3, "algorithmic_flop": ring_flop, "executed_flop": exe,
+                "anchor_period": engine.RING_ANCHOR}
+    if full_s:
+        ach = ring_flop / full_s
+        full = {"kernel": f"ring GEMM Spp ({a.prec}: {nprod} partial products of 2(ph)^3 FLOP)",
+                "kernel_ms": full_s * 1e3, "achieved": ach / 1e12, "frac": ach / peak,
                 "executed_achieved": nprod * ach / 1e12, "executed_frac": nprod * ach / peak,
-                "kernel": f"ring GEMM Spp ({a.prec}: {nprod} partial products of 2(ph)^3 "
-                          f"FLOP)", "kernel_ms": ring_s * 1e3,
-                "algorithmic_flop": ring_flop, "executed_flop": nprod * ring_flop,
-                "fp64_ring_ms": (k64.get("ring") or 0) * 1e3,
-                "fp64_ring_frac_of_78.6TF": (ring_flop / k64["ring"] / 78.6e12
-                                             if k64.get("ring") else None)}
+                "traffic": (profile_value("ccsd_L5_A28_ring_hbm_traffic.md",
+                                          "ring_spp_traffic_bytes") if fp65 else None)}
+        if roof is None:
+            roof = {"bound": "mfma", "achieved": full["achieved"], "peak": peak / 1e12,
+                    "unit": "TFLOP/s", "frac": full["frac"], "traffic": full["traffic"],
+                    "traffic_source": "profiles/ccsd_L5_A28_ring_hbm_traffic.md (calibrated "
+                                      "FETCH_SIZE + WRITE_SIZE of this launch)"
+                                      if full["traffic"] else None,
+                    "executed_achieved": full["executed_achieved"],
+                    "executed_frac": full["executed_frac"], "kernel": full["kernel"],
+                    "kernel_ms": full_s * 1e3, "algorithmic_flop": ring_flop,
+                    "executed_flop": nprod * ring_flop}
+        else:
+            roof["anchor_step_ring"] = full
+    if roof is not None:
+        roof["fp64_ring_ms"] = (k64.get("ring") or 0) * 1e3
+        roof["fp64_ring_frac_of_78.6TF"] = (ring_flop / k64["ring"] / 78.6e12
+                                            if k64.get("ring") else None)
     grad = None
     if a.grad == "on":
         del H
