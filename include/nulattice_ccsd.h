@@ -70,7 +70,9 @@ int nl_contract_f64(const int64_t* dims, double alpha, const double* A, const in
 /* C[b] = alpha * diag(ia) (A[b] B[b]^T) diag(ib) + beta * D[b] with BF16 (dtype 0) or
  * FP16 (dtype 1) operands, FP32 accumulation (MFMA 16x16x32).  The precision form of the ring
  * and ladder contractions (ccDgrams.py:240-303); no reference counterpart for the precision.
- * K, lda, ldb multiples of 8; A, B 16-byte aligned.  D (addend, C's layout) may be C or null.
+ * K, lda, ldb multiples of 8; A, B 16-byte aligned.  dtype NL_FP8 (2): OCP E4M3 operands, one
+ * byte each (MFMA 16x16x128), 256x256 tiles only: K a multiple of 128, lda, ldb and the batch
+ * strides of 16, M, N >= 256 with at least 256 tiles x splits x batch.  D (addend, C's layout) may be C or null.
  * C and D are double unless dtype carries NL_C_F32 << 8 / NL_D_F32 << 8 (float), in every
  * nl_gemm_lp* entry point.
  * splits>1 splits K (FP64 partials in ws, batch*splits*M*N doubles, fixed-order reduction). */
@@ -171,6 +173,18 @@ int nl_tau_split_pairs(int64_t p, int64_t h, const double* t1, const double* t2,
  * ldx, ldd multiples of 8; X, Xa, D 16-byte aligned. */
 int nl_split_delta16(int64_t rows, int64_t K, int64_t ldx, const void* X, const void* Xa,
                      void* D, int64_t ldd, int64_t kp, int with_hi, void* stream);
+/* FP8 (OCP E4M3) operands of the anchored W product (no reference counterpart): per-row
+ * power-of-two scale s (the row maximum times s in (128, 256]), iscale[r] = 1/s, rounding to
+ * nearest even, zero from K to kp (D: bytes, pitch ldd).  nl_split_delta8 writes
+ * (hi + lo) - (hi_a + lo_a) of two split images (as nl_split_delta16), nl_convert_rows8 an FP64
+ * matrix (pitch ldx).  K <= NL_FP8_MAXK; K, kp, ldd multiples of 8.  The FP8 product is
+ * nl_gemm_lp with dtype NL_FP8. */
+#define NL_FP8 2
+#define NL_FP8_MAXK 16384
+int nl_split_delta8(int64_t rows, int64_t K, int64_t ldx, const void* X, const void* Xa,
+                    void* D, int64_t ldd, int64_t kp, float* iscale, void* stream);
+int nl_convert_rows8(int64_t rows, int64_t K, int64_t ldx, const double* X, void* D,
+                     int64_t ldd, int64_t kp, float* iscale, void* stream);
 
 
 /* The ring image of t2 from its hole-pair split image (NL_IMG_PAIRS layout, Pp slots per b):

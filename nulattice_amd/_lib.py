@@ -57,6 +57,8 @@ SIGNATURES = {
     "nl_tau_split_pairs": (_i32, [_i64, _i64, _vp, _vp, _vp, _i64, _i64, _i32, _dbl, _vp]),
     "nl_ring_from_pairs": (_i32, [_i64, _i64, _i64, _vp, _vp, _vp]),
     "nl_split_delta16": (_i32, [_i64, _i64, _i64, _vp, _vp, _vp, _i64, _i64, _i32, _vp]),
+    "nl_split_delta8": (_i32, [_i64, _i64, _i64, _vp, _vp, _vp, _i64, _i64, _vp, _vp]),
+    "nl_convert_rows8": (_i32, [_i64, _i64, _i64, _vp, _vp, _i64, _i64, _vp, _vp]),
     "nl_hole_pairs": (_i32, [_i64, _i64, _i64, _i64, _vp, _vp, _i32, _vp]),
     "nl_concat_rows": (_i32, [_i64, _i64, _vp, _i64, _i64, _vp, _i64, _vp, _i64, _vp]),
     "nl_update_t1": (_i32, [_i64, _i64, _vp, _vp, _vp, _vp, _dbl, _vp, _vp]),

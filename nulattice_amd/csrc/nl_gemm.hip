@@ -960,7 +960,37 @@ __device__ inline void big_step(const BigTile& T, v4f (&acc)[8][4], int wm, int 
   }
 }
 
-template <bool IS_BF16, bool KAL = false>
+// The same K step for FP8 (OCP E4M3) operands: a 128-byte tile row holds 128 elements, one
+// v_mfma_scale_f32_16x16x128_f8f6f4 per 16x16 output tile (unit scales).  Lane group q = lane >> 4
+// feeds the MFMA the 32 bytes of logical chunks q and 4 + q of its row, for A and B alike: the
+// instruction pairs A's and B's operand bytes position by position, so whichever k each position
+// stands for, every k of the row meets its own k once and the sum over the step is the full dot
+// product (the FP16 step's two 32-wide slices, fused).
+typedef int v8i __attribute__((ext_vector_type(8)));
+__device__ inline v8i frag8(const uint16_t (&T)[BT][BKD], int r, int q) {
+  const uint4 x = *(const uint4*)&T[r][(q ^ ((r >> 1) & 7)) * 8];
+  const uint4 y = *(const uint4*)&T[r][((q + 4) ^ ((r >> 1) & 7)) * 8];
+  return (v8i){(int)x.x, (int)x.y, (int)x.z, (int)x.w, (int)y.x, (int)y.y, (int)y.z, (int)y.w};
+}
+
+__device__ inline void big_step8(const BigTile& T, v4f (&acc)[8][4], int wm, int wn, int lane) {
+  const int q = lane >> 4;
+  const int ra = wm * 128 + (lane & 15), rb = wn * 64 + (lane & 15);
+  v8i b[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) b[j] = frag8(T[1], rb + j * 16, q);
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    const v8i a = frag8(T[0], ra + i * 16, q);
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      acc[i][j] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(a, b[j], acc[i][j], 0, 0, 0,
+                                                                   127, 0, 127);
+  }
+  __builtin_amdgcn_sched_barrier(0);
+}
+
+template <bool IS_BF16, bool KAL = false, bool F8 = false>
 __global__ __launch_bounds__(512) void gemm_lp_big_kernel(GemmLPArgs g) {
   // two distinct LDS objects (not one array indexed by a step parity), so the compiler can tell
   // the buffer being read from the buffer the in-flight loads are filling
@@ -1037,11 +1067,13 @@ __global__ __launch_bounds__(512) void gemm_lp_big_kernel(GemmLPArgs g) {
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     issue(S1, k0 + BKD);
-    big_step<IS_BF16>(S0, acc, wm, wn, lane);
+    if constexpr (F8) big_step8(S0, acc, wm, wn, lane);
+    else big_step<IS_BF16>(S0, acc, wm, wn, lane);
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     if (p + 1 < npairs) issue(S0, k0 + 2 * BKD);
-    big_step<IS_BF16>(S1, acc, wm, wn, lane);
+    if constexpr (F8) big_step8(S1, acc, wm, wn, lane);
+    else big_step<IS_BF16>(S1, acc, wm, wn, lane);
   }
 
   lp_epilogue<8>(g, acc, m0, n0, wm, wn, lane, batch, split);
@@ -1682,8 +1714,42 @@ static int gemm_lp_impl(int dtype, int64_t M, int64_t N, int64_t K, double alpha
   const int io = dtype >> 8;
   dtype &= 0xff;
   if (io & ~(NL_C_F32 | NL_D_F32)) return NL_EARG;
-  if (M < 0 || N < 0 || K < 0 || batch < 0 || (dtype != 0 && dtype != 1)) return NL_EARG;
+  if (M < 0 || N < 0 || K < 0 || batch < 0 || dtype < 0 || dtype > 2) return NL_EARG;
   if (M == 0 || N == 0 || batch == 0) return NL_OK;
+  if (dtype == 2) {
+    // FP8 (E4M3) operands: one byte per element; the 256x256 kernel only, read as 16-bit units
+    // (a 16-byte DMA moves 16 elements, a 64-unit K step holds 128).  K, the leading dimensions
+    // and the batch strides in elements: K a multiple of 128, the rest of 16.
+    if (Alo || Blo || crow || staged) return NL_EARG;
+    if ((K % 128) || (lda % 16) || (ldb % 16) || lda < K || ldb < K || (strideA % 16) ||
+        (strideB % 16) || !A || !B || !C || (((uintptr_t)A) & 15) || (((uintptr_t)B) & 15))
+      return NL_EARG;
+    if (M < BT || N < BT || nl_cdiv(M, BT) * nl_cdiv(N, BT) * splits * batch < 256)
+      return NL_EARG;
+    if (M > INT32_MAX || N > INT32_MAX || K > INT32_MAX || batch > 65535) return NL_EARG;
+    if (splits < 1 || splits > 65535 || (splits > 1 && !ws)) return NL_EARG;
+    GemmLPArgs g;
+    g.M = (int)M; g.N = (int)N; g.K = (int)(K / 2); g.alpha = alpha; g.beta = beta;
+    g.A = (const uint16_t*)A; g.lda = lda / 2; g.sA = strideA / 2;
+    g.B = (const uint16_t*)B; g.ldb = ldb / 2; g.sB = strideB / 2;
+    g.Alo = nullptr; g.Blo = nullptr;
+    g.ia = ia; g.sia = strideIa; g.ib = ib; g.sib = strideIb;
+    g.C = AnyOut{C, (io & NL_C_F32) != 0}; g.cm = cm; g.cn = cn; g.sC = strideC;
+    g.D = AnyIn{D, (io & NL_D_F32) != 0}; g.crow = nullptr;
+    g.tiles_m = (int)nl_cdiv(M, BT); g.tiles_n = (int)nl_cdiv(N, BT);
+    g.splits = (int)splits; g.ws = ws;
+    g.kchunk = (int)(nl_cdiv(nl_cdiv(g.K, splits), BKD) * BKD);
+    const dim3 grid((unsigned)(g.tiles_m * g.tiles_n), (unsigned)splits, (unsigned)batch);
+    hipStream_t st = (hipStream_t)stream;
+    hipLaunchKernelGGL((gemm_lp_big_kernel<false, true, true>), grid, dim3(512), 0, st, g);
+    NL_LAUNCH_CHECK();
+    if (splits > 1) {
+      reduce_splits(st, (int)M, (int)N, (int)splits, (int)batch, ws, beta, g.D, g.C, cm, cn,
+                    mk(1, strideC, 0));
+      NL_LAUNCH_CHECK();
+    }
+    return NL_OK;
+  }
   if (!A || !B || !C) return NL_EARG;
   if ((K % 8) || (lda % 8) || (ldb % 8) || lda < K || ldb < K) return NL_EARG;
   if ((((uintptr_t)A) & 15) || (((uintptr_t)B) & 15) || (strideA % 8) || (strideB % 8))

@@ -1320,6 +1320,66 @@ inline int grid_for(int64_t n, int cap = 16384) {
   return (int)(g < cap ? g : cap);
 }
 
+// FP8 (OCP E4M3) rows with a per-row power-of-two scale: x s with |x s| in (128, 256] for the
+// row maximum (below E4M3's 448), rounded to nearest even by the hardware conversion; zero from
+// K to kp.  One workgroup per row; the row's values wait in LDS (K floats) between the amax
+// pass and the conversion.  F: (X, Xa) split 16-bit images [hi(K) | lo(K)] at pitch ldx, value
+// (hi + lo) - (hi_a + lo_a) in FP32; else X is FP64 at pitch ldx.
+__device__ inline unsigned pack_fp8x4(float a, float b, float c, float d) {
+  int w = __builtin_amdgcn_cvt_pk_fp8_f32(a, b, 0, false);
+  w = __builtin_amdgcn_cvt_pk_fp8_f32(c, d, w, true);
+  return (unsigned)w;
+}
+
+template <bool F>
+__global__ __launch_bounds__(NT) void rows_fp8_kernel(int64_t K, int64_t ldx, const void* X,
+                                                      const void* Xa, uint8_t* D, int64_t ldd,
+                                                      int64_t kp, float* iscale) {
+  extern __shared__ float vals[];
+  __shared__ float red[NT / 64];
+  const int64_t r = blockIdx.x;
+  float m = 0.f;
+  for (int64_t k = threadIdx.x * 8; k < K; k += NT * 8) {
+    float v[8];
+    if constexpr (F) {
+      const uint16_t* x = (const uint16_t*)X + r * ldx + k;
+      const uint16_t* xa = (const uint16_t*)Xa + r * ldx + k;
+      const v8h h0 = __builtin_bit_cast(v8h, *(const uint4*)x);
+      const v8h l0 = __builtin_bit_cast(v8h, *(const uint4*)(x + K));
+      const v8h h1 = __builtin_bit_cast(v8h, *(const uint4*)xa);
+      const v8h l1 = __builtin_bit_cast(v8h, *(const uint4*)(xa + K));
+#pragma unroll
+      for (int e = 0; e < 8; ++e)
+        v[e] = ((float)h0[e] + (float)l0[e]) - ((float)h1[e] + (float)l1[e]);
+    } else {
+      const double* x = (const double*)X + r * ldx + k;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) v[e] = (float)x[e];
+    }
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      vals[k + e] = v[e];
+      m = fmaxf(m, fabsf(v[e]));
+    }
+  }
+  m = wave_max(m);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  const float amax = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+  const float sc = row_scale(amax, 256.f);
+  if (threadIdx.x == 0) iscale[r] = row_iscale(amax, 256.f);
+  for (int64_t k = threadIdx.x * 8; k < kp; k += NT * 8) {
+    uint2 o = make_uint2(0, 0);
+    if (k < K) {
+      const float* v = vals + k;
+      o.x = pack_fp8x4(v[0] * sc, v[1] * sc, v[2] * sc, v[3] * sc);
+      o.y = pack_fp8x4(v[4] * sc, v[5] * sc, v[6] * sc, v[7] * sc);
+    }
+    *(uint2*)(D + r * ldd + k) = o;
+  }
+}
+
+
 }  // namespace
 
 extern "C" {
@@ -1585,6 +1645,41 @@ int nl_split_delta16(int64_t rows, int64_t K, int64_t ldx, const void* X, const 
                        (const uint16_t*)X, (const uint16_t*)Xa, (uint16_t*)D, ldd, kp);
   NL_LAUNCH_CHECK();
   return NL_OK;
+}
+
+constexpr int64_t kFp8MaxK = 16384;      // NL_FP8_MAXK: the row's FP32 values in 64 KB of LDS
+
+static int rows_fp8(bool split, int64_t rows, int64_t K, int64_t ldx, const void* X,
+                    const void* Xa, void* D, int64_t ldd, int64_t kp, float* iscale,
+                    void* stream) {
+  if (rows < 0 || K < 0 || kp < K || (K % 8) || (kp % 8) || (ldd % 8) || ldd < kp ||
+      ldx < (split ? 2 : 1) * K || (split && (ldx % 8)) || K > kFp8MaxK)
+    return NL_EARG;
+  if (rows == 0 || kp == 0) return NL_OK;
+  if (!X || !D || !iscale || (split && !Xa) || (((uintptr_t)D) & 7) ||
+      (split && ((((uintptr_t)X) & 15) || (((uintptr_t)Xa) & 15))) ||
+      (!split && (((uintptr_t)X) & 7)) || rows > INT32_MAX)
+    return NL_EARG;
+  const size_t lds = (size_t)nl_cdiv(K, NT * 8) * NT * 8 * sizeof(float);
+  hipStream_t st = (hipStream_t)stream;
+  if (split)
+    hipLaunchKernelGGL(rows_fp8_kernel<true>, dim3((unsigned)rows), dim3(NT), lds, st, K, ldx, X,
+                       Xa, (uint8_t*)D, ldd, kp, iscale);
+  else
+    hipLaunchKernelGGL(rows_fp8_kernel<false>, dim3((unsigned)rows), dim3(NT), lds, st, K, ldx,
+                       X, nullptr, (uint8_t*)D, ldd, kp, iscale);
+  NL_LAUNCH_CHECK();
+  return NL_OK;
+}
+
+int nl_split_delta8(int64_t rows, int64_t K, int64_t ldx, const void* X, const void* Xa,
+                    void* D, int64_t ldd, int64_t kp, float* iscale, void* stream) {
+  return rows_fp8(true, rows, K, ldx, X, Xa, D, ldd, kp, iscale, stream);
+}
+
+int nl_convert_rows8(int64_t rows, int64_t K, int64_t ldx, const double* X, void* D,
+                     int64_t ldd, int64_t kp, float* iscale, void* stream) {
+  return rows_fp8(false, rows, K, ldx, X, nullptr, D, ldd, kp, iscale, stream);
 }
 
 int nl_ring_from_pairs(int64_t p, int64_t h, int64_t Pp, const void* in, void* out,
