@@ -579,7 +579,7 @@ def t2_part(ham, t1, t2, B, prec=DEFAULT_PREC):
 # [FP16(W - W_A) | FP16(W_A)], both W halves on W_A's per-row scales).  Each rounding is 2^-11 of
 # a difference from the anchor, which shrinks to nothing as the amplitudes converge, so the fixed
 # point is the split products'.  0: off.
-RING_ANCHOR = 5
+RING_ANCHOR = 8
 _KSTEP = 64                     # the 256x256 16-bit kernel's K step (a zero-padded K)
 # The W product between anchors on FP8 (E4M3) operands: v's and (t2 - t2_A)'s rows on per-row
 # power-of-two scales.  Its roundings (2^-4) multiply t2 - t2_A, so they too vanish at
